@@ -1,0 +1,151 @@
+/*
+ * liggghts_dem.h -- C-ABI of the MI355X DEM timestep engine.
+ *
+ * The engine owns one simulation per context: particle state in HBM, the
+ * neighbour list with its pair-keyed contact history, primitive planes and
+ * material tables, and one HIP stream of default priority.  Calls on one
+ * context are not thread safe; they are serialised by the caller (the
+ * LIGGGHTS main thread in the style plug-in).  Every entry point returns 0
+ * on success or a negative DEM_ERR_* code; dem_last_error() gives the text.
+ *
+ * Each entry point names the reference interface whose work it takes over
+ * (paths relative to the LIGGGHTS-INL src/ directory).
+ */
+#ifndef LIGGGHTS_DEM_H
+#define LIGGGHTS_DEM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    DEM_OK = 0,
+    DEM_ERR_ARG = -1,      /* bad argument or call order                    */
+    DEM_ERR_HIP = -2,      /* HIP runtime failure                           */
+    DEM_ERR_OVERFLOW = -3, /* "Neighbor list overflow, boost neigh_modify one"
+                              (neigh_gran.cpp:553,633)                       */
+    DEM_ERR_STATE = -4,    /* "ilegal situation, particle positions
+                              identical" (pair_gran_base.h:361-362), NaN      */
+    DEM_ERR_NOMEM = -5
+};
+
+enum { DEM_FP64 = 0, DEM_MIXED = 1 };
+enum { DEM_ROLL_OFF = 0, DEM_ROLL_EPSD = 1, DEM_ROLL_EPSD2 = 2, DEM_ROLL_EPSD3 = 3 };
+
+#define DEM_MAXTYPES 8
+#define DEM_MAXPLANES 8
+
+typedef struct {
+    double dt;                    /* timestep (update.cpp timestep command)  */
+    double boxlo[3], boxhi[3];    /* orthogonal box (domain.cpp)             */
+    int periodic[3];              /* boundary p / non-periodic per dim       */
+    double skin;                  /* neighbor <skin> bin                     */
+    double cdf;                   /* contactDistanceFactor (neighbor.h:141)  */
+    int neigh_every, neigh_delay, neigh_check;  /* neigh_modify             */
+    int neigh_one;                /* neigh_modify one (default 2000)         */
+    int rolling_model;            /* DEM_ROLL_*  (rolling_friction keyword)  */
+    int tangential_damping;       /* hertz setting, default 1                */
+    int limit_force;              /* hertz setting, default 0                */
+    int torsion_torque;           /* rolling setting, default 0              */
+    int precision;                /* DEM_FP64 or DEM_MIXED                   */
+    int device;                   /* HIP device ordinal                      */
+} dem_params;
+
+typedef struct dem_ctx dem_ctx;
+
+/* Creates a context.  Replaces the device-side state that Atom, Neighbor,
+ * PairGran and FixWallGran hold for one run (atom.cpp, neighbor.cpp,
+ * pair_gran.cpp:312-560).  Returns NULL on failure (dem_last_error(NULL)). */
+dem_ctx *dem_ctx_create(const dem_params *p);
+void dem_ctx_destroy(dem_ctx *ctx);
+const char *dem_last_error(const dem_ctx *ctx);
+
+/* Host helper: effective material tables as global_properties.cpp:428-562
+ * builds them (createYeff, createGeff, createBetaEff).  Y, nu per type
+ * (length ntypes+1, index 0 unused); e per type pair, (ntypes+1)^2
+ * row-major.  Outputs (ntypes+1)^2 each. */
+int dem_material_tables(int ntypes, const double *Y, const double *nu, const double *e,
+                        double *Yeff, double *Geff, double *betaeff);
+
+/* Material tables of the contact models, (ntypes+1)^2 row-major each:
+ * Yeff, Geff, betaeff (normal_model_hertz.h:150-160), coeffFrict
+ * (tangential_model_history.h:126-131), coeffRollFrict, coeffRollVisc
+ * (rolling_model_epsd.h:86-90; NULL when off) and the scalar
+ * coefficientRollingStiffness (rolling_model_epsd3.h:86-91). */
+int dem_set_materials(dem_ctx *ctx, int ntypes, const double *Yeff, const double *Geff,
+                      const double *betaeff, const double *mu, const double *mur,
+                      const double *etar, double krs);
+
+/* Particle state upload (AtomVecSphere arrays, atom_vec_sphere.cpp:125-160):
+ * x, v, omega are n*3 row-major; radius, rmass per atom; type 1..ntypes;
+ * tag > 0 unique; freeze != 0 marks members of fix freeze (may be NULL).
+ * Replaces any previous state; contact history is cleared. */
+int dem_upload_state(dem_ctx *ctx, int n, const double *x, const double *v,
+                     const double *omega, const double *radius, const double *rmass,
+                     const int *type, const int *tag, const int *freeze);
+
+/* Download in the order of the last upload (NULL skips an array).
+ * Replaces the host copies of x, v, omega, f, torque. */
+int dem_download(dem_ctx *ctx, double *x, double *v, double *omega, double *f, double *torque);
+
+/* fix wall/gran ... primitive type <wtype> {x,y,z}plane <pos>
+ * (fix_wall_gran.cpp:93-360, primitive_wall_definitions.h:130-148). */
+int dem_add_plane(dem_ctx *ctx, int dim, double pos, int wtype);
+/* fix gravity acceleration (fix_gravity.cpp:301-352, :395-397). */
+int dem_set_gravity(dem_ctx *ctx, double ax, double ay, double az);
+/* fix freeze present (fix_freeze.cpp:121-144; pair_gran.cpp:532-537). */
+int dem_set_freeze(dem_ctx *ctx, int on);
+
+/* Verlet::setup (verlet.cpp:134-199): pbc, neighbour build (with history
+ * re-seed), one force pass with shearupdate = 0. */
+int dem_setup(dem_ctx *ctx);
+/* Verlet::run(n) (verlet.cpp:264-390) with fix nve/sphere
+ * (fix_nve_sphere.cpp:134-243), the granular pair style, primitive walls
+ * and gravity, all on the device. */
+int dem_step(dem_ctx *ctx, int nsteps);
+/* One force evaluation on the current list: force_clear + pair + walls +
+ * gravity (+ freeze).  shearupdate as PairGran::compute sets it
+ * (pair_gran.cpp:632-635).  For set-up passes and parity probes. */
+int dem_compute_forces(dem_ctx *ctx, int shearupdate);
+/* Neighbour rebuild now: FixContactHistory::pre_exchange
+ * (fix_contact_history.cpp:305-426) + Domain::pbc + granular_bin_no_newton
+ * (neigh_gran.cpp:485-640) + primitive-wall lists. */
+int dem_rebuild(dem_ctx *ctx);
+
+/* Directed pair set of the current (full) list and its contact history:
+ * entry (tag_i, tag_j, shift, hist[dnum], flag), shift packs periodic
+ * image offsets as (sx+1)+3(sy+1)+9(sz+1), 13 = none.  This is the data
+ * `compute pair/gran/local id history` reports (compute_pair_gran_local.cpp
+ * :100-116).  Pass NULL arrays to get only the count in *count. */
+int dem_get_pairs(dem_ctx *ctx, int64_t *count, int *tag_i, int *tag_j, int *shift,
+                  double *hist, int *flag);
+/* History doubles per pair entry (3 shear + 3 rolling when on). */
+int dem_history_size(const dem_ctx *ctx);
+/* Contact-history injection (FixContactHistory::unpack_restart,
+ * fix_contact_history.cpp:588-656): per-atom partner data used by the next
+ * rebuild's re-seed, keyed by (tag_i, tag_j). */
+int dem_set_history(dem_ctx *ctx, int64_t count, const int *tag_i, const int *tag_j,
+                    const double *hist);
+/* Primitive-wall history per atom (property/atom history_<id>,
+ * fix_wall_gran.cpp:476-505), n * nplanes * dnum, upload order. */
+int dem_get_wall_history(dem_ctx *ctx, double *hist);
+/* Restores it (the property/atom restart path, fix_property_atom.cpp
+ * unpack_restart), same layout. */
+int dem_set_wall_history(dem_ctx *ctx, const double *hist);
+
+/* Kernel timing with HIP events on the context's stream. */
+int dem_timing_enable(dem_ctx *ctx, int on);
+/* name: "contact" | "integrate" | "rebuild".  Total ms and launch count
+ * since enable. */
+int dem_timing_get(dem_ctx *ctx, const char *name, double *ms, int64_t *launches);
+/* Mean full-list neighbours and contacts per atom of the last force pass. */
+int dem_list_stats(dem_ctx *ctx, double *mean_neigh, double *mean_contacts, int64_t *entries);
+int dem_sync(dem_ctx *ctx);
+int64_t dem_timestep(const dem_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,348 @@
+// contact_law.h -- the sphere contact-model chain of pair_style gran
+// (surface default -> normal hertz -> tangential history -> rolling epsd*)
+// for one contact, as seen from atom i.
+//
+// FP64 form: same operations in the same order as the reference, so that
+// compiled without contraction (-ffp-contract=off) every per-contact value
+// rounds identically:
+//   surface_model_default.h:146-208, normal_model_hertz.h:205-405,
+//   tangential_model_history.h:135-425, rolling_model_epsd.h:96-335,
+//   rolling_model_epsd2.h:161-230, rolling_model_epsd3.h:178-262,
+//   fix_wall_gran_base.h:159-378 (wall variant).
+//
+// Mixed form (contact_law_mixed): overlap, separation and the history
+// integration stay FP64; the contact-law arithmetic runs in FP32.
+#pragma once
+#include "dem_device.h"
+
+namespace dem {
+
+struct ContactIn {
+    int itype, jtype;
+    double radi, radj;
+    double mi, mj, meff;
+    double delta[3];     // x_i - x_j (pair) or particle - wall point (wall)
+    double r, rinv;
+    double radsum;
+    double deltan_in;    // wall only: overlap handed over by the wall code
+    double vi[3], vj[3], wi[3], wj[3];
+    bool is_wall;
+};
+
+__device__ inline double tbl(const double *mat, int slot, int it, int jt)
+{
+    return mat[slot * kT1 * kT1 + it * kT1 + jt];
+}
+
+// Returns the i-side force and torque increments (ForceData i_forces).
+// h: dnum history doubles (shear[3], rolling torque[3]), updated in place.
+__device__ inline void contact_fp64(const Model &M, const double *mat, const ContactIn &c,
+                                    double *h, bool update, double F[3], double T[3])
+{
+    const double SQRT_FIVE_OVER_SIX = 0.91287092917527685576161630466800355658790782499663875;
+    const double enx = c.delta[0] * c.rinv;
+    const double eny = c.delta[1] * c.rinv;
+    const double enz = c.delta[2] * c.rinv;
+
+    // ---- surface default
+    const double vr1 = c.vi[0] - c.vj[0];
+    const double vr2 = c.vi[1] - c.vj[1];
+    const double vr3 = c.vi[2] - c.vj[2];
+    const double vn = vr1 * enx + vr2 * eny + vr3 * enz;
+    const double vn1 = vn * enx, vn2 = vn * eny, vn3 = vn * enz;
+    const double vt1 = vr1 - vn1, vt2 = vr2 - vn2, vt3 = vr3 - vn3;
+    const double deltan = c.radsum - c.r;
+    const double dx = c.delta[0], dy = c.delta[1], dz = c.delta[2];
+    double wr1, wr2, wr3, cri, crj = 0.0;
+    if (c.is_wall) {
+        const double cr = c.radi - 0.5 * c.deltan_in;
+        wr1 = cr * c.wi[0] * c.rinv;
+        wr2 = cr * c.wi[1] * c.rinv;
+        wr3 = cr * c.wi[2] * c.rinv;
+        cri = cr;
+    } else {
+        cri = c.radi - 0.5 * deltan;
+        crj = c.radj - 0.5 * deltan;
+        wr1 = (cri * c.wi[0] + crj * c.wj[0]) * c.rinv;
+        wr2 = (cri * c.wi[1] + crj * c.wj[1]) * c.rinv;
+        wr3 = (cri * c.wi[2] + crj * c.wj[2]) * c.rinv;
+    }
+    const double vtr1 = vt1 - (dz * wr2 - dy * wr3);
+    const double vtr2 = vt2 - (dx * wr3 - dz * wr1);
+    const double vtr3 = vt3 - (dy * wr1 - dx * wr2);
+
+    // ---- normal hertz
+    const int it = c.itype, jt = c.jtype;
+    const double reff = c.is_wall ? c.radi : (c.radi * c.radj / (c.radi + c.radj));
+    const double sqrtval = sqrt(reff * deltan);
+    const double Yeff = tbl(mat, M_YEFF, it, jt);
+    const double Geff = tbl(mat, M_GEFF, it, jt);
+    const double beta = tbl(mat, M_BETA, it, jt);
+    const double Sn = 2. * Yeff * sqrtval;
+    const double St = 8. * Geff * sqrtval;
+    const double kn = 4. / 3. * Yeff * sqrtval;
+    const double kt = St;
+    const double gamman = -2. * SQRT_FIVE_OVER_SIX * beta * sqrt(Sn * c.meff);
+    const double gammat = M.tdamp ? -2. * SQRT_FIVE_OVER_SIX * beta * sqrt(St * c.meff) : 0.0;
+    const double Fn_damping = -gamman * vn;
+    const double Fn_contact = kn * deltan;
+    double Fn = Fn_damping + Fn_contact;
+    if (M.limitf && (Fn < 0.0)) Fn = 0.0;
+    F[0] = Fn * enx;
+    F[1] = Fn * eny;
+    F[2] = Fn * enz;
+
+    // ---- tangential history
+    double *shear = h;
+    if (update) {
+        const double dt = M.dt;
+        shear[0] += vtr1 * dt;
+        shear[1] += vtr2 * dt;
+        shear[2] += vtr3 * dt;
+        const double rsht = shear[0] * enx + shear[1] * eny + shear[2] * enz;
+        shear[0] -= rsht * enx;
+        shear[1] -= rsht * eny;
+        shear[2] -= rsht * enz;
+    }
+    const double shrmag = sqrt(shear[0] * shear[0] + shear[1] * shear[1] + shear[2] * shear[2]);
+    const double xmu = tbl(mat, M_MU, it, jt);
+    double Ft1 = -(kt * shear[0]);
+    double Ft2 = -(kt * shear[1]);
+    double Ft3 = -(kt * shear[2]);
+    const double Ft_shear = kt * shrmag;
+    const double Ft_friction = xmu * fabs(Fn);
+    if (Ft_shear > Ft_friction) {
+        if (shrmag != 0.0) {
+            const double ratio = Ft_friction / Ft_shear;
+            Ft1 *= ratio;
+            Ft2 *= ratio;
+            Ft3 *= ratio;
+            if (update) {
+                shear[0] = -Ft1 / kt;
+                shear[1] = -Ft2 / kt;
+                shear[2] = -Ft3 / kt;
+            }
+        } else {
+            Ft1 = Ft2 = Ft3 = 0.0;
+        }
+    } else {
+        Ft1 -= (gammat * vtr1);
+        Ft2 -= (gammat * vtr2);
+        Ft3 -= (gammat * vtr3);
+    }
+    const double tor1 = eny * Ft3 - enz * Ft2;
+    const double tor2 = enz * Ft1 - enx * Ft3;
+    const double tor3 = enx * Ft2 - eny * Ft1;
+    F[0] += Ft1;
+    F[1] += Ft2;
+    F[2] += Ft3;
+    T[0] = -cri * tor1;
+    T[1] = -cri * tor2;
+    T[2] = -cri * tor3;
+
+    // ---- rolling friction
+    if (M.rolling) {
+        double w1, w2, w3, r_inertia;
+        if (c.is_wall) {
+            w1 = wr1; w2 = wr2; w3 = wr3;
+            r_inertia = 1.4 * c.mi * reff * reff;
+        } else {
+            w1 = c.wi[0] - c.wj[0];
+            w2 = c.wi[1] - c.wj[1];
+            w3 = c.wi[2] - c.wj[2];
+            const double ri = c.mi * c.radi * c.radi;
+            const double rj = c.mj * c.radj * c.radj;
+            r_inertia = 1.4 * ri * rj / (ri + rj);
+        }
+        double wt0, wt1, wt2;
+        if (M.torsion) {
+            wt0 = w1; wt1 = w2; wt2 = w3;
+        } else {
+            const double wdd = w1 * enx + w2 * eny + w3 * enz;
+            const double wn0 = enx * wdd, wn1 = eny * wdd, wn2 = enz * wdd;
+            wt0 = w1 - wn0; wt1 = w2 - wn1; wt2 = w3 - wn2;
+        }
+        const double rmu = tbl(mat, M_MUR, it, jt);
+        double kr;
+        if (M.rolling == 1) kr = 2.25 * kn * rmu * rmu * reff * reff;
+        else if (M.rolling == 2) kr = kt * reff * reff;
+        else kr = M.krs * kn * rmu * rmu * reff * reff;
+        const double dtkr = M.dt * kr;
+        double *ch = h + 3;
+        double rt0 = ch[0] + dtkr * wt0;
+        double rt1 = ch[1] + dtkr * wt1;
+        double rt2 = ch[2] + dtkr * wt2;
+        const double mag = sqrt(rt0 * rt0 + rt1 * rt1 + rt2 * rt2);
+        const double rmax = fabs(Fn) * reff * rmu;
+        if (mag > rmax) {
+            const double factor = rmax / mag;
+            rt0 *= factor; rt1 *= factor; rt2 *= factor;
+            if (update) { ch[0] = rt0; ch[1] = rt1; ch[2] = rt2; }
+        } else {
+            if (update) { ch[0] = rt0; ch[1] = rt1; ch[2] = rt2; }
+            if (M.rolling != 2) {
+                const double r_coef = tbl(mat, M_ETAR, it, jt) * 2 * sqrt(r_inertia * kr);
+                rt0 += r_coef * wt0;
+                rt1 += r_coef * wt1;
+                rt2 += r_coef * wt2;
+            }
+        }
+        T[0] -= rt0;
+        T[1] -= rt1;
+        T[2] -= rt2;
+    }
+}
+
+// Mixed precision: the overlap deltan (a small difference of large numbers)
+// and the history integration are FP64; everything else of the law is FP32.
+__device__ inline void contact_mixed(const Model &M, const double *mat, const ContactIn &c,
+                                     double *h, bool update, double F[3], double T[3])
+{
+#pragma clang fp contract(fast)
+    const float SQRT_FIVE_OVER_SIX = 0.912870929175276856f;
+    const float rinv = (float)c.rinv;
+    const float enx = (float)c.delta[0] * rinv;
+    const float eny = (float)c.delta[1] * rinv;
+    const float enz = (float)c.delta[2] * rinv;
+
+    const float vr1 = (float)(c.vi[0] - c.vj[0]);
+    const float vr2 = (float)(c.vi[1] - c.vj[1]);
+    const float vr3 = (float)(c.vi[2] - c.vj[2]);
+    const float vn = vr1 * enx + vr2 * eny + vr3 * enz;
+    const float vt1 = vr1 - vn * enx, vt2 = vr2 - vn * eny, vt3 = vr3 - vn * enz;
+    const double deltan_d = c.radsum - c.r;
+    const float deltan = (float)deltan_d;
+    const float dx = (float)c.delta[0], dy = (float)c.delta[1], dz = (float)c.delta[2];
+    float wr1, wr2, wr3, cri;
+    if (c.is_wall) {
+        const float cr = (float)(c.radi - 0.5 * c.deltan_in);
+        wr1 = cr * (float)c.wi[0] * rinv;
+        wr2 = cr * (float)c.wi[1] * rinv;
+        wr3 = cr * (float)c.wi[2] * rinv;
+        cri = cr;
+    } else {
+        cri = (float)(c.radi - 0.5 * deltan_d);
+        const float crj = (float)(c.radj - 0.5 * deltan_d);
+        wr1 = (cri * (float)c.wi[0] + crj * (float)c.wj[0]) * rinv;
+        wr2 = (cri * (float)c.wi[1] + crj * (float)c.wj[1]) * rinv;
+        wr3 = (cri * (float)c.wi[2] + crj * (float)c.wj[2]) * rinv;
+    }
+    const float vtr1 = vt1 - (dz * wr2 - dy * wr3);
+    const float vtr2 = vt2 - (dx * wr3 - dz * wr1);
+    const float vtr3 = vt3 - (dy * wr1 - dx * wr2);
+
+    const int it = c.itype, jt = c.jtype;
+    const float radi = (float)c.radi, radj = (float)c.radj;
+    const float reff = c.is_wall ? radi : (radi * radj / (radi + radj));
+    const float sqrtval = sqrtf(reff * deltan);
+    const float Yeff = (float)tbl(mat, M_YEFF, it, jt);
+    const float Geff = (float)tbl(mat, M_GEFF, it, jt);
+    const float beta = (float)tbl(mat, M_BETA, it, jt);
+    const float meff = (float)c.meff;
+    const float Sn = 2.f * Yeff * sqrtval;
+    const float St = 8.f * Geff * sqrtval;
+    const float kn = (4.f / 3.f) * Yeff * sqrtval;
+    const float kt = St;
+    const float gamman = -2.f * SQRT_FIVE_OVER_SIX * beta * sqrtf(Sn * meff);
+    const float gammat = M.tdamp ? -2.f * SQRT_FIVE_OVER_SIX * beta * sqrtf(St * meff) : 0.f;
+    float Fn = -gamman * vn + kn * deltan;
+    if (M.limitf && (Fn < 0.f)) Fn = 0.f;
+
+    double *shear = h;
+    if (update) {
+        const double dt = M.dt;
+        shear[0] += (double)vtr1 * dt;
+        shear[1] += (double)vtr2 * dt;
+        shear[2] += (double)vtr3 * dt;
+        const double rsht = shear[0] * enx + shear[1] * eny + shear[2] * enz;
+        shear[0] -= rsht * enx;
+        shear[1] -= rsht * eny;
+        shear[2] -= rsht * enz;
+    }
+    const float s0 = (float)shear[0], s1 = (float)shear[1], s2 = (float)shear[2];
+    const float shrmag = sqrtf(s0 * s0 + s1 * s1 + s2 * s2);
+    const float xmu = (float)tbl(mat, M_MU, it, jt);
+    float Ft1 = -(kt * s0), Ft2 = -(kt * s1), Ft3 = -(kt * s2);
+    const float Ft_shear = kt * shrmag;
+    const float Ft_friction = xmu * fabsf(Fn);
+    if (Ft_shear > Ft_friction) {
+        if (shrmag != 0.f) {
+            const float ratio = Ft_friction / Ft_shear;
+            Ft1 *= ratio; Ft2 *= ratio; Ft3 *= ratio;
+            if (update) {
+                const double dratio = (double)ratio;
+                shear[0] *= dratio;
+                shear[1] *= dratio;
+                shear[2] *= dratio;
+            }
+        } else {
+            Ft1 = Ft2 = Ft3 = 0.f;
+        }
+    } else {
+        Ft1 -= gammat * vtr1;
+        Ft2 -= gammat * vtr2;
+        Ft3 -= gammat * vtr3;
+    }
+    const float tor1 = eny * Ft3 - enz * Ft2;
+    const float tor2 = enz * Ft1 - enx * Ft3;
+    const float tor3 = enx * Ft2 - eny * Ft1;
+    F[0] = (double)(Fn * enx + Ft1);
+    F[1] = (double)(Fn * eny + Ft2);
+    F[2] = (double)(Fn * enz + Ft3);
+    float T0 = -cri * tor1, T1 = -cri * tor2, T2 = -cri * tor3;
+
+    if (M.rolling) {
+        float w1, w2, w3, r_inertia;
+        if (c.is_wall) {
+            w1 = wr1; w2 = wr2; w3 = wr3;
+            r_inertia = 1.4f * (float)c.mi * reff * reff;
+        } else {
+            w1 = (float)(c.wi[0] - c.wj[0]);
+            w2 = (float)(c.wi[1] - c.wj[1]);
+            w3 = (float)(c.wi[2] - c.wj[2]);
+            const float ri = (float)c.mi * radi * radi;
+            const float rj = (float)c.mj * radj * radj;
+            r_inertia = 1.4f * ri * rj / (ri + rj);
+        }
+        float wt0, wt1, wt2;
+        if (M.torsion) {
+            wt0 = w1; wt1 = w2; wt2 = w3;
+        } else {
+            const float wdd = w1 * enx + w2 * eny + w3 * enz;
+            wt0 = w1 - enx * wdd; wt1 = w2 - eny * wdd; wt2 = w3 - enz * wdd;
+        }
+        const float rmu = (float)tbl(mat, M_MUR, it, jt);
+        float kr;
+        if (M.rolling == 1) kr = 2.25f * kn * rmu * rmu * reff * reff;
+        else if (M.rolling == 2) kr = kt * reff * reff;
+        else kr = (float)M.krs * kn * rmu * rmu * reff * reff;
+        const double dtkr = M.dt * (double)kr;
+        double *ch = h + 3;
+        double rt0 = ch[0] + dtkr * wt0;
+        double rt1 = ch[1] + dtkr * wt1;
+        double rt2 = ch[2] + dtkr * wt2;
+        const float mag = sqrtf((float)(rt0 * rt0 + rt1 * rt1 + rt2 * rt2));
+        const float rmax = fabsf(Fn) * reff * rmu;
+        if (mag > rmax) {
+            const double factor = (double)(rmax / mag);
+            rt0 *= factor; rt1 *= factor; rt2 *= factor;
+            if (update) { ch[0] = rt0; ch[1] = rt1; ch[2] = rt2; }
+        } else {
+            if (update) { ch[0] = rt0; ch[1] = rt1; ch[2] = rt2; }
+            if (M.rolling != 2) {
+                const float r_coef = (float)tbl(mat, M_ETAR, it, jt) * 2.f * sqrtf(r_inertia * kr);
+                rt0 += (double)(r_coef * wt0);
+                rt1 += (double)(r_coef * wt1);
+                rt2 += (double)(r_coef * wt2);
+            }
+        }
+        T0 -= (float)rt0;
+        T1 -= (float)rt1;
+        T2 -= (float)rt2;
+    }
+    T[0] = (double)T0;
+    T[1] = (double)T1;
+    T[2] = (double)T2;
+}
+
+}  // namespace dem

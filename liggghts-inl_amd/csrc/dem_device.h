@@ -1,0 +1,85 @@
+// dem_device.h -- device-side data layout shared by the engine's kernels.
+//
+// HBM layout (all arrays in Morton cell order after each rebuild):
+//   xr  double4  (x, y, z, radius)           32 B / particle
+//   vm  double4  (vx, vy, vz, rmass)         32 B
+//   wt  double4  (wx, wy, wz, meta)          32 B   meta = type + 16*freeze
+//   f4  double4  (fx, fy, fz, -)             32 B
+//   t4  double4  (tx, ty, tz, -)             32 B
+//   tag int, orig int (upload position), img int4 (periodic image counts)
+// Neighbour list: full (both directions), CSR rows per particle:
+//   nstart[n+1]; nj[e] = j | shift << 26 | flag << 31; hist[e*dnum + k]
+// Primitive walls: wmask[i] bit k = atom in wall k's list; whist[(i*np+k)*dnum]
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace dem {
+
+constexpr int kMaxTypes = 8;
+constexpr int kT1 = kMaxTypes + 1;
+constexpr int kMaxPlanes = 8;
+constexpr int kMaxDnum = 6;
+constexpr uint32_t kJMask = (1u << 26) - 1u;
+constexpr uint32_t kFlagBit = 1u << 31;
+constexpr int kNoShift = 13;
+constexpr int kBlock = 256;
+
+// material table slots, each kT1*kT1 doubles, index [itype*kT1 + jtype]
+enum { M_YEFF = 0, M_GEFF, M_BETA, M_MU, M_MUR, M_ETAR, M_COUNT };
+
+struct Planes {
+    int n;
+    int dim[kMaxPlanes];
+    int wtype[kMaxPlanes];
+    double pos[kMaxPlanes];
+};
+
+struct Model {
+    double dt;
+    double cdf;          // contactDistanceFactor
+    double skin;
+    double cutneighmax;
+    double prd[3];
+    double gacc[3];
+    double krs;          // coefficientRollingStiffness (epsd3)
+    int gravity_on;
+    int freeze_on;
+    int rolling;         // 0 off, 1 epsd, 2 epsd2, 3 epsd3
+    int tdamp;           // tangential_damping
+    int limitf;          // limitForce
+    int torsion;         // rolling torsionTorque
+    int dnum;            // history doubles per pair
+};
+
+struct Grid {
+    double lo[3];
+    double inv[3];       // cells per unit length
+    int nc[3];
+    int periodic[3];
+    int nkeys;           // size of the Morton key space
+    const uint32_t *spread;  // spread[d*maxc + c] = interleaved bits of cell c in dim d
+    int maxc;
+};
+
+__host__ __device__ inline void shift_decode(int code, int &sx, int &sy, int &sz)
+{
+    sx = code % 3 - 1;
+    sy = (code / 3) % 3 - 1;
+    sz = code / 9 - 1;
+}
+
+// x[j] + dx with dx = pbc * prd, the expression of comm->borders()
+// (atom_vec_sphere.cpp:625-637)
+__device__ inline double3 image_of(const double4 &p, int code, const double prd[3])
+{
+    int sx, sy, sz;
+    shift_decode(code, sx, sy, sz);
+    double3 r;
+    r.x = p.x + sx * prd[0];
+    r.y = p.y + sy * prd[1];
+    r.z = p.z + sz * prd[2];
+    return r;
+}
+
+}  // namespace dem

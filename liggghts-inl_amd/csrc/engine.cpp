@@ -1,0 +1,889 @@
+// engine.cpp -- host side of the MI355X DEM engine: context, device memory,
+// the Verlet step sequence and the C-ABI declared in include/liggghts_dem.h.
+//
+// Step sequence, Verlet::run (verlet.cpp:264-390):
+//   initial_integrate -> [decide -> rebuild] -> force pass -> final_integrate
+// final_integrate(n) and initial_integrate(n+1) are one kernel (mode 3);
+// no force is computed between them, so both read the same f.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/liggghts_dem.h"
+#include "kernels.h"
+
+using namespace dem;
+
+namespace {
+
+std::string g_last_error;
+
+struct EventPair {
+    hipEvent_t a, b;
+};
+
+struct Timer {
+    std::vector<EventPair> ev;
+    double ms = 0.0;
+    int64_t launches = 0;
+};
+
+}  // namespace
+
+struct dem_ctx {
+    dem_params p;
+    hipStream_t stream = nullptr;
+    std::string err;
+
+    int ntypes = 0;
+    double *d_mat = nullptr;
+    bool mat_set = false;
+
+    int n = 0, cap = 0;
+    int cur = 0;
+    double4 *xr[2] = {}, *vm[2] = {}, *wt[2] = {};
+    double4 *f4 = nullptr, *t4 = nullptr, *xhold = nullptr;
+    int *tag[2] = {}, *orig[2] = {};
+    int4 *img[2] = {};
+    double *whist[2] = {};
+    uint8_t *wmask = nullptr;
+    int *ncontact = nullptr;
+    double maxrad = 0.0;
+
+    // counting sort
+    uint32_t *key = nullptr;
+    int *sorted = nullptr;
+    int *cstart = nullptr, *cursor = nullptr;
+    int nkeys_cap = 0;
+    int *scan_tmp = nullptr;
+    int scan_tmp_cap = 0;
+    uint32_t *spread = nullptr;
+    int spread_cap = 0;
+
+    // neighbour list, double buffered for the history remap
+    int lcur = 0;
+    int *nstart[2] = {};
+    uint32_t *nj[2] = {};
+    double *hist[2] = {};
+    int64_t ecap[2] = {0, 0};
+    bool have_list = false;
+    bool list_valid = false;
+    int *d_small = nullptr;   // [0] err flag, [1] maxcount, [2] move flag
+
+    Grid G{};
+    Model M{};
+    Planes P{};
+    long long ntimestep = 0;
+    int ago = 0;
+
+    bool timing = false;
+    Timer t_contact, t_integrate, t_rebuild;
+};
+
+#define HIPCHK(call)                                                              \
+    do {                                                                          \
+        hipError_t e_ = (call);                                                   \
+        if (e_ != hipSuccess) {                                                   \
+            ctx->err = std::string("HIP error ") + hipGetErrorString(e_) + " at " \
+                       + #call;                                                   \
+            return DEM_ERR_HIP;                                                   \
+        }                                                                         \
+    } while (0)
+
+#define KCHK()                                                                    \
+    do {                                                                          \
+        hipError_t e_ = hipGetLastError();                                        \
+        if (e_ != hipSuccess) {                                                   \
+            ctx->err = std::string("kernel launch: ") + hipGetErrorString(e_);    \
+            return DEM_ERR_HIP;                                                   \
+        }                                                                         \
+    } while (0)
+
+static int fail(dem_ctx *ctx, int code, const std::string &msg)
+{
+    if (ctx) ctx->err = msg;
+    else g_last_error = msg;
+    return code;
+}
+
+template <typename T>
+static int dev_alloc(dem_ctx *ctx, T **ptr, size_t count)
+{
+    if (*ptr) {
+        HIPCHK(hipFree(*ptr));
+        *ptr = nullptr;
+    }
+    if (count == 0) count = 1;
+    HIPCHK(hipMalloc((void **)ptr, count * sizeof(T)));
+    return DEM_OK;
+}
+
+static void timer_begin(dem_ctx *ctx, Timer &t)
+{
+    if (!ctx->timing) return;
+    EventPair e;
+    hipEventCreate(&e.a);
+    hipEventCreate(&e.b);
+    hipEventRecord(e.a, ctx->stream);
+    t.ev.push_back(e);
+}
+
+static void timer_end(dem_ctx *ctx, Timer &t)
+{
+    if (!ctx->timing || t.ev.empty()) return;
+    hipEventRecord(t.ev.back().b, ctx->stream);
+    t.launches++;
+}
+
+static void timer_collect(Timer &t)
+{
+    for (auto &e : t.ev) {
+        float ms = 0.f;
+        hipEventSynchronize(e.b);
+        hipEventElapsedTime(&ms, e.a, e.b);
+        t.ms += ms;
+        hipEventDestroy(e.a);
+        hipEventDestroy(e.b);
+    }
+    t.ev.clear();
+}
+
+// ---------------------------------------------------------------------------
+
+extern "C" const char *dem_last_error(const dem_ctx *ctx)
+{
+    return ctx ? ctx->err.c_str() : g_last_error.c_str();
+}
+
+extern "C" dem_ctx *dem_ctx_create(const dem_params *p)
+{
+    if (!p) {
+        g_last_error = "dem_ctx_create: null params";
+        return nullptr;
+    }
+    if (!(p->dt > 0.0) || !(p->skin >= 0.0) || !(p->cdf >= 1.0) || p->neigh_every < 1
+        || p->rolling_model < 0 || p->rolling_model > 3 || p->precision < 0 || p->precision > 1) {
+        g_last_error = "dem_ctx_create: invalid parameters";
+        return nullptr;
+    }
+    for (int d = 0; d < 3; d++)
+        if (!(p->boxhi[d] > p->boxlo[d])) {
+            g_last_error = "dem_ctx_create: empty box";
+            return nullptr;
+        }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= p->device || p->device < 0) {
+        g_last_error = "dem_ctx_create: no HIP device";
+        return nullptr;
+    }
+    if (hipSetDevice(p->device) != hipSuccess) {
+        g_last_error = "dem_ctx_create: hipSetDevice failed";
+        return nullptr;
+    }
+    dem_ctx *ctx = new dem_ctx();
+    ctx->p = *p;
+    if (ctx->p.neigh_one <= 0) ctx->p.neigh_one = 2000;
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        g_last_error = "dem_ctx_create: stream creation failed";
+        delete ctx;
+        return nullptr;
+    }
+    if (hipMalloc((void **)&ctx->d_mat, sizeof(double) * M_COUNT * kT1 * kT1) != hipSuccess
+        || hipMalloc((void **)&ctx->d_small, sizeof(int) * 8) != hipSuccess) {
+        g_last_error = "dem_ctx_create: out of device memory";
+        hipStreamDestroy(ctx->stream);
+        delete ctx;
+        return nullptr;
+    }
+    hipMemset(ctx->d_small, 0, sizeof(int) * 8);
+    Model &M = ctx->M;
+    M.dt = p->dt;
+    M.cdf = p->cdf;
+    M.skin = p->skin;
+    for (int d = 0; d < 3; d++) M.prd[d] = p->boxhi[d] - p->boxlo[d];
+    M.rolling = p->rolling_model;
+    M.tdamp = p->tangential_damping;
+    M.limitf = p->limit_force;
+    M.torsion = p->torsion_torque;
+    M.dnum = 3 + (p->rolling_model ? 3 : 0);
+    ctx->P.n = 0;
+    return ctx;
+}
+
+static void free_all(dem_ctx *ctx)
+{
+    auto F = [](void *q) {
+        if (q) hipFree(q);
+    };
+    for (int b = 0; b < 2; b++) {
+        F(ctx->xr[b]); F(ctx->vm[b]); F(ctx->wt[b]); F(ctx->tag[b]); F(ctx->orig[b]);
+        F(ctx->img[b]); F(ctx->whist[b]); F(ctx->nstart[b]); F(ctx->nj[b]); F(ctx->hist[b]);
+    }
+    F(ctx->f4); F(ctx->t4); F(ctx->xhold); F(ctx->wmask); F(ctx->ncontact); F(ctx->key);
+    F(ctx->sorted); F(ctx->cstart); F(ctx->cursor); F(ctx->scan_tmp); F(ctx->spread);
+    F(ctx->d_mat); F(ctx->d_small);
+}
+
+extern "C" void dem_ctx_destroy(dem_ctx *ctx)
+{
+    if (!ctx) return;
+    hipSetDevice(ctx->p.device);
+    hipStreamSynchronize(ctx->stream);
+    timer_collect(ctx->t_contact);
+    timer_collect(ctx->t_integrate);
+    timer_collect(ctx->t_rebuild);
+    free_all(ctx);
+    hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+// global_properties.cpp:428-562
+extern "C" int dem_material_tables(int ntypes, const double *Y, const double *nu, const double *e,
+                                   double *Yeff, double *Geff, double *betaeff)
+{
+    if (ntypes < 1 || ntypes > DEM_MAXTYPES || !Y || !nu || !e || !Yeff || !Geff || !betaeff)
+        return fail(nullptr, DEM_ERR_ARG, "dem_material_tables: bad arguments");
+    const int w = ntypes + 1;
+    for (int k = 0; k < w * w; k++) Yeff[k] = Geff[k] = betaeff[k] = 0.0;
+    for (int i = 1; i <= ntypes; i++)
+        for (int j = 1; j <= ntypes; j++) {
+            const double Yi = Y[i], Yj = Y[j], vi = nu[i], vj = nu[j];
+            Yeff[i * w + j] = 1. / ((1. - std::pow(vi, 2.)) / Yi + (1. - std::pow(vj, 2.)) / Yj);
+            Geff[i * w + j] = 1. / (2. * (2. - vi) * (1. + vi) / Yi + 2. * (2. - vj) * (1. + vj) / Yj);
+            const double lg = std::log(e[i * w + j]);
+            betaeff[i * w + j] = lg / std::sqrt(std::pow(lg, 2.) + std::pow(M_PI, 2.));
+        }
+    return DEM_OK;
+}
+
+extern "C" int dem_set_materials(dem_ctx *ctx, int ntypes, const double *Yeff, const double *Geff,
+                                 const double *betaeff, const double *mu, const double *mur,
+                                 const double *etar, double krs)
+{
+    if (!ctx) return fail(nullptr, DEM_ERR_ARG, "null context");
+    if (ntypes < 1 || ntypes > DEM_MAXTYPES || !Yeff || !Geff || !betaeff || !mu)
+        return fail(ctx, DEM_ERR_ARG, "dem_set_materials: bad arguments");
+    if (ctx->p.rolling_model && !mur) return fail(ctx, DEM_ERR_ARG, "rolling model needs coeffRollFrict");
+    if ((ctx->p.rolling_model == 1 || ctx->p.rolling_model == 3) && !etar)
+        return fail(ctx, DEM_ERR_ARG, "rolling model needs coeffRollVisc");
+    hipSetDevice(ctx->p.device);
+    std::vector<double> h(M_COUNT * kT1 * kT1, 0.0);
+    const int w = ntypes + 1;
+    const double *src[M_COUNT] = {Yeff, Geff, betaeff, mu, mur, etar};
+    for (int s = 0; s < M_COUNT; s++) {
+        if (!src[s]) continue;
+        for (int i = 0; i < w; i++)
+            for (int j = 0; j < w; j++) h[s * kT1 * kT1 + i * kT1 + j] = src[s][i * w + j];
+    }
+    HIPCHK(hipMemcpy(ctx->d_mat, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+    ctx->ntypes = ntypes;
+    ctx->M.krs = krs;
+    ctx->mat_set = true;
+    return DEM_OK;
+}
+
+extern "C" int dem_add_plane(dem_ctx *ctx, int dim, double pos, int wtype)
+{
+    if (!ctx) return fail(nullptr, DEM_ERR_ARG, "null context");
+    if (ctx->P.n >= kMaxPlanes || dim < 0 || dim > 2 || wtype < 1 || wtype > DEM_MAXTYPES)
+        return fail(ctx, DEM_ERR_ARG, "dem_add_plane: bad arguments");
+    if (ctx->n > 0) return fail(ctx, DEM_ERR_ARG, "dem_add_plane: add walls before uploading state");
+    const int k = ctx->P.n++;
+    ctx->P.dim[k] = dim;
+    ctx->P.pos[k] = pos;
+    ctx->P.wtype[k] = wtype;
+    return DEM_OK;
+}
+
+extern "C" int dem_set_gravity(dem_ctx *ctx, double ax, double ay, double az)
+{
+    if (!ctx) return fail(nullptr, DEM_ERR_ARG, "null context");
+    ctx->M.gacc[0] = ax;
+    ctx->M.gacc[1] = ay;
+    ctx->M.gacc[2] = az;
+    ctx->M.gravity_on = 1;
+    return DEM_OK;
+}
+
+extern "C" int dem_set_freeze(dem_ctx *ctx, int on)
+{
+    if (!ctx) return fail(nullptr, DEM_ERR_ARG, "null context");
+    ctx->M.freeze_on = on ? 1 : 0;
+    return DEM_OK;
+}
+
+// Cell grid + Morton spread tables for the current cut-off.
+static int setup_grid(dem_ctx *ctx)
+{
+    const dem_params &p = ctx->p;
+    const double cut = (ctx->maxrad + ctx->maxrad) * p.cdf + p.skin;
+    ctx->M.cutneighmax = cut;
+    Grid &G = ctx->G;
+    int bits[3];
+    int maxc = 1;
+    for (int d = 0; d < 3; d++) {
+        const double L = p.boxhi[d] - p.boxlo[d];
+        int nc = (int)std::floor(L / (cut * 1.0001));
+        if (nc < 1) nc = 1;
+        if (p.periodic[d] && nc < 3)
+            return fail(ctx, DEM_ERR_ARG,
+                        "periodic box dimension must span at least 3 neighbour cells "
+                        "(3 x cutneighmax)");
+        if (nc > 4096) nc = 4096;
+        G.nc[d] = nc;
+        G.lo[d] = p.boxlo[d];
+        G.inv[d] = nc / L;
+        G.periodic[d] = p.periodic[d];
+        int b = 0;
+        while ((1 << b) < nc) b++;
+        bits[d] = b;
+        maxc = std::max(maxc, nc);
+    }
+    int total = bits[0] + bits[1] + bits[2];
+    while (total > 26) {  // coarsen the largest non-periodic dimension
+        int dm = -1;
+        for (int d = 0; d < 3; d++)
+            if (!p.periodic[d] && (dm < 0 || G.nc[d] > G.nc[dm])) dm = d;
+        if (dm < 0) return fail(ctx, DEM_ERR_ARG, "cell grid too large");
+        G.nc[dm] = (G.nc[dm] + 1) / 2;
+        G.inv[dm] = G.nc[dm] / (p.boxhi[dm] - p.boxlo[dm]);
+        int b = 0;
+        while ((1 << b) < G.nc[dm]) b++;
+        bits[dm] = b;
+        total = bits[0] + bits[1] + bits[2];
+    }
+    maxc = std::max(G.nc[0], std::max(G.nc[1], G.nc[2]));
+    // bit positions of the interleave, dimensions drop out as they run out
+    int posmap[3][32];
+    int pos = 0;
+    for (int l = 0; l < 27; l++)
+        for (int d = 0; d < 3; d++)
+            if (l < bits[d]) posmap[d][l] = pos++;
+    std::vector<uint32_t> h(3 * maxc, 0u);
+    for (int d = 0; d < 3; d++)
+        for (int c = 0; c < G.nc[d]; c++) {
+            uint32_t v = 0;
+            for (int l = 0; l < bits[d]; l++)
+                if (c & (1 << l)) v |= 1u << posmap[d][l];
+            h[d * maxc + c] = v;
+        }
+    if (3 * maxc > ctx->spread_cap) {
+        if (dev_alloc(ctx, &ctx->spread, 3 * maxc)) return DEM_ERR_HIP;
+        ctx->spread_cap = 3 * maxc;
+    }
+    HIPCHK(hipMemcpy(ctx->spread, h.data(), h.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    G.spread = ctx->spread;
+    G.maxc = maxc;
+    G.nkeys = 1 << total;
+    if (G.nkeys + 1 > ctx->nkeys_cap) {
+        if (dev_alloc(ctx, &ctx->cstart, G.nkeys + 1) || dev_alloc(ctx, &ctx->cursor, G.nkeys + 1))
+            return DEM_ERR_HIP;
+        ctx->nkeys_cap = G.nkeys + 1;
+    }
+    const int need = std::max(scan_tmp_size(G.nkeys + 1), scan_tmp_size(ctx->n + 1));
+    if (need > ctx->scan_tmp_cap) {
+        if (dev_alloc(ctx, &ctx->scan_tmp, need)) return DEM_ERR_HIP;
+        ctx->scan_tmp_cap = need;
+    }
+    return DEM_OK;
+}
+
+extern "C" int dem_upload_state(dem_ctx *ctx, int n, const double *x, const double *v,
+                                const double *omega, const double *radius, const double *rmass,
+                                const int *type, const int *tag, const int *freeze)
+{
+    if (!ctx) return fail(nullptr, DEM_ERR_ARG, "null context");
+    if (n < 0 || n > (int)kJMask || (n > 0 && (!x || !v || !omega || !radius || !rmass || !type || !tag)))
+        return fail(ctx, DEM_ERR_ARG, "dem_upload_state: bad arguments");
+    if (!ctx->mat_set) return fail(ctx, DEM_ERR_ARG, "dem_upload_state: set materials first");
+    hipSetDevice(ctx->p.device);
+    const int dnum = ctx->M.dnum;
+    const int wdn = ctx->P.n * dnum;
+    std::vector<double4> hx(n), hv(n), hw(n);
+    std::vector<int> ho(n);
+    std::vector<int4> hi(n, make_int4(0, 0, 0, 0));
+    double maxrad = 0.0;
+    for (int i = 0; i < n; i++) {
+        if (type[i] < 1 || type[i] > ctx->ntypes)
+            return fail(ctx, DEM_ERR_ARG, "dem_upload_state: atom type out of range");
+        if (!(radius[i] > 0.0) || !(rmass[i] > 0.0))
+            return fail(ctx, DEM_ERR_ARG, "Fix nve/sphere requires extended particles");
+        hx[i] = make_double4(x[3 * i], x[3 * i + 1], x[3 * i + 2], radius[i]);
+        hv[i] = make_double4(v[3 * i], v[3 * i + 1], v[3 * i + 2], rmass[i]);
+        const int meta = type[i] + ((freeze && freeze[i]) ? 16 : 0);
+        hw[i] = make_double4(omega[3 * i], omega[3 * i + 1], omega[3 * i + 2], (double)meta);
+        ho[i] = i;
+        maxrad = std::max(maxrad, radius[i]);
+    }
+    if (n > ctx->cap) {
+        const size_t c = (size_t)n;
+        for (int b = 0; b < 2; b++) {
+            if (dev_alloc(ctx, &ctx->xr[b], c) || dev_alloc(ctx, &ctx->vm[b], c)
+                || dev_alloc(ctx, &ctx->wt[b], c) || dev_alloc(ctx, &ctx->tag[b], c)
+                || dev_alloc(ctx, &ctx->orig[b], c) || dev_alloc(ctx, &ctx->img[b], c)
+                || dev_alloc(ctx, &ctx->whist[b], c * std::max(wdn, 1))
+                || dev_alloc(ctx, &ctx->nstart[b], c + 1))
+                return DEM_ERR_HIP;
+        }
+        if (dev_alloc(ctx, &ctx->f4, c) || dev_alloc(ctx, &ctx->t4, c) || dev_alloc(ctx, &ctx->xhold, c)
+            || dev_alloc(ctx, &ctx->wmask, c) || dev_alloc(ctx, &ctx->ncontact, c)
+            || dev_alloc(ctx, &ctx->key, c) || dev_alloc(ctx, &ctx->sorted, c))
+            return DEM_ERR_HIP;
+        ctx->cap = n;
+    }
+    ctx->n = n;
+    ctx->cur = 0;
+    ctx->maxrad = maxrad;
+    if (n > 0) {
+        HIPCHK(hipMemcpy(ctx->xr[0], hx.data(), n * sizeof(double4), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(ctx->vm[0], hv.data(), n * sizeof(double4), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(ctx->wt[0], hw.data(), n * sizeof(double4), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(ctx->tag[0], tag, n * sizeof(int), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(ctx->orig[0], ho.data(), n * sizeof(int), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(ctx->img[0], hi.data(), n * sizeof(int4), hipMemcpyHostToDevice));
+        if (wdn) HIPCHK(hipMemset(ctx->whist[0], 0, sizeof(double) * n * wdn));
+        HIPCHK(hipMemset(ctx->f4, 0, n * sizeof(double4)));
+        HIPCHK(hipMemset(ctx->t4, 0, n * sizeof(double4)));
+        HIPCHK(hipMemset(ctx->wmask, 0, n));
+    }
+    ctx->have_list = false;
+    ctx->list_valid = false;
+    int rc = setup_grid(ctx);
+    if (rc) return rc;
+    return DEM_OK;
+}
+
+static int ensure_list_cap(dem_ctx *ctx, int b, int64_t entries)
+{
+    if (entries <= ctx->ecap[b]) return DEM_OK;
+    const int64_t c = entries + entries / 4 + 1024;
+    // keep the old contents of the other buffer; this one is about to be overwritten
+    if (dev_alloc(ctx, &ctx->nj[b], (size_t)c) || dev_alloc(ctx, &ctx->hist[b], (size_t)c * ctx->M.dnum))
+        return DEM_ERR_NOMEM;
+    ctx->ecap[b] = c;
+    return DEM_OK;
+}
+
+// Neighbour rebuild: pre_exchange + pbc + sort + list + wall lists.
+static int rebuild(dem_ctx *ctx)
+{
+    const int n = ctx->n;
+    hipStream_t s = ctx->stream;
+    timer_begin(ctx, ctx->t_rebuild);
+    const Grid &G = ctx->G;
+    const int c0 = ctx->cur, c1 = 1 - ctx->cur;
+    HIPCHK(hipMemsetAsync(ctx->cstart, 0, sizeof(int) * (G.nkeys + 1), s));
+    BinArgs ba{};
+    ba.xr = ctx->xr[c0];
+    ba.img = ctx->img[c0];
+    ba.key = ctx->key;
+    ba.count = ctx->cstart;
+    ba.n = n;
+    ba.G = G;
+    for (int d = 0; d < 3; d++) {
+        ba.prd[d] = ctx->M.prd[d];
+        ba.hi[d] = ctx->p.boxhi[d];
+    }
+    launch_bin(ba, s);
+    KCHK();
+    launch_scan(ctx->cstart, G.nkeys + 1, ctx->scan_tmp, s);
+    KCHK();
+    HIPCHK(hipMemcpyAsync(ctx->cursor, ctx->cstart, sizeof(int) * (G.nkeys + 1), hipMemcpyDeviceToDevice, s));
+    launch_scatter(ctx->key, ctx->cursor, ctx->sorted, n, s);
+    launch_cellsort(ctx->cstart, ctx->sorted, G.nkeys, s);
+    KCHK();
+    PermuteArgs pa{};
+    pa.sorted = ctx->sorted;
+    pa.xr0 = ctx->xr[c0]; pa.vm0 = ctx->vm[c0]; pa.wt0 = ctx->wt[c0];
+    pa.tag0 = ctx->tag[c0]; pa.orig0 = ctx->orig[c0]; pa.img0 = ctx->img[c0];
+    pa.whist0 = ctx->whist[c0];
+    pa.xr1 = ctx->xr[c1]; pa.vm1 = ctx->vm[c1]; pa.wt1 = ctx->wt[c1];
+    pa.tag1 = ctx->tag[c1]; pa.orig1 = ctx->orig[c1]; pa.img1 = ctx->img[c1];
+    pa.whist1 = ctx->whist[c1];
+    pa.xhold = ctx->xhold;
+    pa.wmask = ctx->wmask;
+    pa.n = n;
+    pa.wdn = ctx->P.n * ctx->M.dnum;
+    pa.P = ctx->P;
+    pa.skin = ctx->p.skin;
+    launch_permute(pa, s);
+    KCHK();
+    ctx->cur = c1;
+
+    const int l0 = ctx->lcur, l1 = 1 - ctx->lcur;
+    NeighArgs na{};
+    na.xr = ctx->xr[c1];
+    na.tag = ctx->tag[c1];
+    na.cstart = ctx->cstart;
+    na.count = ctx->nstart[l1];
+    na.maxcount = ctx->d_small + 1;
+    na.n = n;
+    na.dnum = ctx->M.dnum;
+    na.cdf = ctx->p.cdf;
+    na.skin = ctx->p.skin;
+    na.G = G;
+    for (int d = 0; d < 3; d++) na.prd[d] = ctx->M.prd[d];
+    HIPCHK(hipMemsetAsync(ctx->d_small + 1, 0, sizeof(int), s));
+    HIPCHK(hipMemsetAsync(ctx->nstart[l1] + n, 0, sizeof(int), s));
+    launch_neigh_count(na, s);
+    KCHK();
+    launch_scan(ctx->nstart[l1], n + 1, ctx->scan_tmp, s);
+    KCHK();
+    int hsmall[2] = {0, 0};
+    HIPCHK(hipMemcpyAsync(&hsmall[0], ctx->nstart[l1] + n, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(&hsmall[1], ctx->d_small + 1, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (hsmall[1] > ctx->p.neigh_one)
+        return fail(ctx, DEM_ERR_OVERFLOW, "Neighbor list overflow, boost neigh_modify one");
+    if (ensure_list_cap(ctx, l1, hsmall[0])) return DEM_ERR_NOMEM;
+    na.nstart = ctx->nstart[l1];
+    na.nj = ctx->nj[l1];
+    na.hist = ctx->hist[l1];
+    na.perm = ctx->sorted;
+    na.have_old = ctx->have_list ? 1 : 0;
+    na.onstart = ctx->nstart[l0];
+    na.onj = ctx->nj[l0];
+    na.ohist = ctx->hist[l0];
+    na.otag = ctx->tag[c0];
+    launch_neigh_fill(na, s);
+    KCHK();
+    ctx->lcur = l1;
+    ctx->have_list = true;
+    ctx->list_valid = true;
+    ctx->ago = 0;
+    timer_end(ctx, ctx->t_rebuild);
+    return DEM_OK;
+}
+
+static int force_pass(dem_ctx *ctx, int shearupdate)
+{
+    ForceArgs fa{};
+    const int c = ctx->cur, l = ctx->lcur;
+    fa.xr = ctx->xr[c]; fa.vm = ctx->vm[c]; fa.wt = ctx->wt[c];
+    fa.nstart = ctx->nstart[l];
+    fa.nj = ctx->nj[l];
+    fa.hist = ctx->hist[l];
+    fa.f4 = ctx->f4; fa.t4 = ctx->t4;
+    fa.mat = ctx->d_mat;
+    fa.wmask = ctx->wmask;
+    fa.whist = ctx->whist[c];
+    fa.ncontact = ctx->timing ? ctx->ncontact : nullptr;
+    fa.err = ctx->d_small;
+    fa.n = ctx->n;
+    fa.shearupdate = shearupdate;
+    fa.M = ctx->M;
+    fa.P = ctx->P;
+    timer_begin(ctx, ctx->t_contact);
+    if (ctx->p.precision == DEM_MIXED) launch_force_mixed(fa, ctx->stream);
+    else launch_force_fp64(fa, ctx->stream);
+    timer_end(ctx, ctx->t_contact);
+    KCHK();
+    return DEM_OK;
+}
+
+static int integrate(dem_ctx *ctx, int mode)
+{
+    IntegArgs ia{};
+    const int c = ctx->cur;
+    ia.xr = ctx->xr[c]; ia.vm = ctx->vm[c]; ia.wt = ctx->wt[c];
+    ia.f4 = ctx->f4; ia.t4 = ctx->t4;
+    ia.n = ctx->n;
+    ia.dtf = 0.5 * ctx->p.dt;   // dtf = 0.5*dt*ftm2v (fix_nve.cpp:85), ftm2v = 1 (si)
+    ia.dtv = ctx->p.dt;
+    ia.dtfrotate = ia.dtf / 0.4;
+    timer_begin(ctx, ctx->t_integrate);
+    launch_integrate(ia, mode, ctx->stream);
+    timer_end(ctx, ctx->t_integrate);
+    KCHK();
+    return DEM_OK;
+}
+
+static int check_err_flag(dem_ctx *ctx)
+{
+    int h = 0;
+    HIPCHK(hipMemcpyAsync(&h, ctx->d_small, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (h & 1)
+        return fail(ctx, DEM_ERR_STATE,
+                    "ilegal situation, particle positions identical. Most probable reason: Simulation diverged");
+    return DEM_OK;
+}
+
+extern "C" int dem_rebuild(dem_ctx *ctx)
+{
+    if (!ctx) return fail(nullptr, DEM_ERR_ARG, "null context");
+    hipSetDevice(ctx->p.device);
+    return rebuild(ctx);
+}
+
+extern "C" int dem_compute_forces(dem_ctx *ctx, int shearupdate)
+{
+    if (!ctx) return fail(nullptr, DEM_ERR_ARG, "null context");
+    if (!ctx->list_valid) return fail(ctx, DEM_ERR_ARG, "dem_compute_forces: no neighbour list (call dem_setup)");
+    hipSetDevice(ctx->p.device);
+    int rc = force_pass(ctx, shearupdate);
+    if (rc) return rc;
+    return check_err_flag(ctx);
+}
+
+extern "C" int dem_setup(dem_ctx *ctx)
+{
+    if (!ctx) return fail(nullptr, DEM_ERR_ARG, "null context");
+    if (!ctx->mat_set) return fail(ctx, DEM_ERR_ARG, "dem_setup: set materials first");
+    hipSetDevice(ctx->p.device);
+    int rc = rebuild(ctx);
+    if (rc) return rc;
+    rc = force_pass(ctx, 0);
+    if (rc) return rc;
+    return check_err_flag(ctx);
+}
+
+// Neighbor::decide + check_distance (neighbor.cpp:1362-1465)
+static int decide(dem_ctx *ctx, int *out)
+{
+    ctx->ago++;
+    *out = 0;
+    if (!(ctx->ago >= ctx->p.neigh_delay && ctx->ago % ctx->p.neigh_every == 0)) return DEM_OK;
+    if (!ctx->p.neigh_check) {
+        *out = 1;
+        return DEM_OK;
+    }
+    const double trig = 0.5 * ctx->p.skin;
+    HIPCHK(hipMemsetAsync(ctx->d_small + 2, 0, sizeof(int), ctx->stream));
+    launch_check_distance(ctx->xr[ctx->cur], ctx->xhold, ctx->n, trig * trig, ctx->d_small + 2, ctx->stream);
+    KCHK();
+    int h = 0;
+    HIPCHK(hipMemcpyAsync(&h, ctx->d_small + 2, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    *out = h;
+    return DEM_OK;
+}
+
+extern "C" int dem_step(dem_ctx *ctx, int nsteps)
+{
+    if (!ctx) return fail(nullptr, DEM_ERR_ARG, "null context");
+    if (!ctx->list_valid) return fail(ctx, DEM_ERR_ARG, "dem_step: call dem_setup first");
+    if (nsteps <= 0) return DEM_OK;
+    hipSetDevice(ctx->p.device);
+    int rc;
+    for (int st = 0; st < nsteps; st++) {
+        ctx->ntimestep++;
+        if (st == 0 && (rc = integrate(ctx, 1))) return rc;
+        int nflag = 0;
+        if ((rc = decide(ctx, &nflag))) return rc;
+        if (nflag && (rc = rebuild(ctx))) return rc;
+        if ((rc = force_pass(ctx, 1))) return rc;
+        if ((rc = integrate(ctx, st == nsteps - 1 ? 2 : 3))) return rc;
+    }
+    return check_err_flag(ctx);
+}
+
+extern "C" int dem_download(dem_ctx *ctx, double *x, double *v, double *omega, double *f, double *torque)
+{
+    if (!ctx) return fail(nullptr, DEM_ERR_ARG, "null context");
+    hipSetDevice(ctx->p.device);
+    const int n = ctx->n, c = ctx->cur;
+    if (n == 0) return DEM_OK;
+    std::vector<double4> hx(n), hv(n), hw(n), hf(n), ht(n);
+    std::vector<int> ho(n);
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipMemcpy(ho.data(), ctx->orig[c], n * sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(hx.data(), ctx->xr[c], n * sizeof(double4), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(hv.data(), ctx->vm[c], n * sizeof(double4), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(hw.data(), ctx->wt[c], n * sizeof(double4), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(hf.data(), ctx->f4, n * sizeof(double4), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(ht.data(), ctx->t4, n * sizeof(double4), hipMemcpyDeviceToHost));
+    auto put = [&](double *dst, const std::vector<double4> &src) {
+        if (!dst) return;
+        for (int i = 0; i < n; i++) {
+            const int o = ho[i];
+            dst[3 * o] = src[i].x;
+            dst[3 * o + 1] = src[i].y;
+            dst[3 * o + 2] = src[i].z;
+        }
+    };
+    put(x, hx); put(v, hv); put(omega, hw); put(f, hf); put(torque, ht);
+    return DEM_OK;
+}
+
+extern "C" int dem_history_size(const dem_ctx *ctx)
+{
+    return ctx ? ctx->M.dnum : 0;
+}
+
+extern "C" int dem_get_pairs(dem_ctx *ctx, int64_t *count, int *tag_i, int *tag_j, int *shift,
+                             double *hist, int *flag)
+{
+    if (!ctx || !count) return fail(ctx, DEM_ERR_ARG, "dem_get_pairs: bad arguments");
+    hipSetDevice(ctx->p.device);
+    const int n = ctx->n, l = ctx->lcur, c = ctx->cur, dnum = ctx->M.dnum;
+    if (!ctx->have_list || n == 0) {
+        *count = 0;
+        return DEM_OK;
+    }
+    std::vector<int> ns(n + 1), tg(n);
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipMemcpy(ns.data(), ctx->nstart[l], (n + 1) * sizeof(int), hipMemcpyDeviceToHost));
+    const int64_t E = ns[n];
+    if (!tag_i) {
+        *count = E;
+        return DEM_OK;
+    }
+    if (*count < E) return fail(ctx, DEM_ERR_ARG, "dem_get_pairs: output arrays too small");
+    std::vector<uint32_t> nj(E);
+    std::vector<double> hh((size_t)E * dnum);
+    HIPCHK(hipMemcpy(tg.data(), ctx->tag[c], n * sizeof(int), hipMemcpyDeviceToHost));
+    if (E) {
+        HIPCHK(hipMemcpy(nj.data(), ctx->nj[l], E * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(hh.data(), ctx->hist[l], E * dnum * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    for (int i = 0; i < n; i++)
+        for (int e = ns[i]; e < ns[i + 1]; e++) {
+            tag_i[e] = tg[i];
+            tag_j[e] = tg[nj[e] & kJMask];
+            if (shift) shift[e] = (int)((nj[e] >> 26) & 31u);
+            if (flag) flag[e] = (nj[e] & kFlagBit) ? 1 : 0;
+            if (hist)
+                for (int k = 0; k < dnum; k++) hist[(size_t)e * dnum + k] = hh[(size_t)e * dnum + k];
+        }
+    *count = E;
+    return DEM_OK;
+}
+
+extern "C" int dem_set_history(dem_ctx *ctx, int64_t count, const int *tag_i, const int *tag_j,
+                               const double *hist)
+{
+    if (!ctx || count < 0 || (count > 0 && (!tag_i || !tag_j || !hist)))
+        return fail(ctx, DEM_ERR_ARG, "dem_set_history: bad arguments");
+    hipSetDevice(ctx->p.device);
+    const int n = ctx->n, c = ctx->cur, dnum = ctx->M.dnum;
+    std::vector<int> tg(n);
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (n) HIPCHK(hipMemcpy(tg.data(), ctx->tag[c], n * sizeof(int), hipMemcpyDeviceToHost));
+    std::unordered_map<int, int> idx;
+    idx.reserve(n * 2 + 1);
+    for (int i = 0; i < n; i++) idx[tg[i]] = i;
+    std::vector<int> cnt(n + 1, 0);
+    for (int64_t k = 0; k < count; k++) {
+        auto it = idx.find(tag_i[k]);
+        if (it == idx.end() || idx.find(tag_j[k]) == idx.end())
+            return fail(ctx, DEM_ERR_ARG, "dem_set_history: unknown tag");
+        cnt[it->second + 1]++;
+    }
+    for (int i = 0; i < n; i++) cnt[i + 1] += cnt[i];
+    std::vector<int> fillp(cnt.begin(), cnt.end() - 1);
+    std::vector<uint32_t> nj(std::max<int64_t>(count, 1));
+    std::vector<double> hh(std::max<int64_t>(count, 1) * dnum);
+    for (int64_t k = 0; k < count; k++) {
+        const int i = idx[tag_i[k]];
+        const int e = fillp[i]++;
+        nj[e] = (uint32_t)idx[tag_j[k]] | ((uint32_t)kNoShift << 26) | kFlagBit;
+        for (int d = 0; d < dnum; d++) hh[(size_t)e * dnum + d] = hist[(size_t)k * dnum + d];
+    }
+    const int l = ctx->lcur;
+    if (ensure_list_cap(ctx, l, count)) return DEM_ERR_NOMEM;
+    HIPCHK(hipMemcpy(ctx->nstart[l], cnt.data(), (n + 1) * sizeof(int), hipMemcpyHostToDevice));
+    if (count) {
+        HIPCHK(hipMemcpy(ctx->nj[l], nj.data(), count * sizeof(uint32_t), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(ctx->hist[l], hh.data(), count * dnum * sizeof(double), hipMemcpyHostToDevice));
+    }
+    ctx->have_list = true;
+    ctx->list_valid = false;
+    return DEM_OK;
+}
+
+extern "C" int dem_get_wall_history(dem_ctx *ctx, double *hist)
+{
+    if (!ctx || !hist) return fail(ctx, DEM_ERR_ARG, "dem_get_wall_history: bad arguments");
+    hipSetDevice(ctx->p.device);
+    const int n = ctx->n, c = ctx->cur, wdn = ctx->P.n * ctx->M.dnum;
+    if (n == 0 || wdn == 0) return DEM_OK;
+    std::vector<double> h((size_t)n * wdn);
+    std::vector<int> ho(n);
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipMemcpy(h.data(), ctx->whist[c], h.size() * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(ho.data(), ctx->orig[c], n * sizeof(int), hipMemcpyDeviceToHost));
+    for (int i = 0; i < n; i++)
+        for (int k = 0; k < wdn; k++) hist[(size_t)ho[i] * wdn + k] = h[(size_t)i * wdn + k];
+    return DEM_OK;
+}
+
+extern "C" int dem_set_wall_history(dem_ctx *ctx, const double *hist)
+{
+    if (!ctx || !hist) return fail(ctx, DEM_ERR_ARG, "dem_set_wall_history: bad arguments");
+    hipSetDevice(ctx->p.device);
+    const int n = ctx->n, c = ctx->cur, wdn = ctx->P.n * ctx->M.dnum;
+    if (n == 0 || wdn == 0) return DEM_OK;
+    std::vector<double> h((size_t)n * wdn);
+    std::vector<int> ho(n);
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipMemcpy(ho.data(), ctx->orig[c], n * sizeof(int), hipMemcpyDeviceToHost));
+    for (int i = 0; i < n; i++)
+        for (int k = 0; k < wdn; k++) h[(size_t)i * wdn + k] = hist[(size_t)ho[i] * wdn + k];
+    HIPCHK(hipMemcpy(ctx->whist[c], h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+    return DEM_OK;
+}
+
+extern "C" int dem_timing_enable(dem_ctx *ctx, int on)
+{
+    if (!ctx) return fail(nullptr, DEM_ERR_ARG, "null context");
+    hipStreamSynchronize(ctx->stream);
+    for (Timer *t : {&ctx->t_contact, &ctx->t_integrate, &ctx->t_rebuild}) {
+        timer_collect(*t);
+        t->ms = 0.0;
+        t->launches = 0;
+    }
+    ctx->timing = on != 0;
+    return DEM_OK;
+}
+
+extern "C" int dem_timing_get(dem_ctx *ctx, const char *name, double *ms, int64_t *launches)
+{
+    if (!ctx || !name || !ms || !launches) return fail(ctx, DEM_ERR_ARG, "dem_timing_get: bad arguments");
+    Timer *t = nullptr;
+    if (!strcmp(name, "contact")) t = &ctx->t_contact;
+    else if (!strcmp(name, "integrate")) t = &ctx->t_integrate;
+    else if (!strcmp(name, "rebuild")) t = &ctx->t_rebuild;
+    else return fail(ctx, DEM_ERR_ARG, "dem_timing_get: unknown timer");
+    timer_collect(*t);
+    *ms = t->ms;
+    *launches = t->launches;
+    return DEM_OK;
+}
+
+extern "C" int dem_list_stats(dem_ctx *ctx, double *mean_neigh, double *mean_contacts, int64_t *entries)
+{
+    if (!ctx) return fail(nullptr, DEM_ERR_ARG, "null context");
+    hipSetDevice(ctx->p.device);
+    const int n = ctx->n;
+    if (!ctx->have_list || n == 0) return fail(ctx, DEM_ERR_ARG, "dem_list_stats: no list");
+    int E = 0;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipMemcpy(&E, ctx->nstart[ctx->lcur] + n, sizeof(int), hipMemcpyDeviceToHost));
+    std::vector<int> nc(n);
+    HIPCHK(hipMemcpy(nc.data(), ctx->ncontact, n * sizeof(int), hipMemcpyDeviceToHost));
+    double s = 0.0;
+    for (int i = 0; i < n; i++) s += nc[i];
+    if (mean_neigh) *mean_neigh = (double)E / n;
+    if (mean_contacts) *mean_contacts = s / n;
+    if (entries) *entries = E;
+    return DEM_OK;
+}
+
+extern "C" int dem_sync(dem_ctx *ctx)
+{
+    if (!ctx) return fail(nullptr, DEM_ERR_ARG, "null context");
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return DEM_OK;
+}
+
+extern "C" int64_t dem_timestep(const dem_ctx *ctx)
+{
+    return ctx ? ctx->ntimestep : 0;
+}

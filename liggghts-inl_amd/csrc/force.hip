@@ -1,0 +1,260 @@
+// force.hip -- contact force pass and velocity-Verlet integrator.
+//
+// k_force: one thread per particle gathers over its full neighbour row, so
+// every force and torque is summed in registers by its owner: no atomics,
+// no scatter, results independent of scheduling.  Replaces, per step,
+// Verlet::force_clear (verlet.cpp:406-430), PairGran::compute
+// (pair_gran_base.h:187-503), FixWallGran::post_force_primitive
+// (fix_wall_gran.cpp:988-1120), FixGravity::post_force
+// (fix_gravity.cpp:301-352) and FixFreeze::post_force (fix_freeze.cpp:121-144).
+// The list is full (both directions), so a pair is evaluated once from each
+// side; in the reference the j-side gets -F of the i-side evaluation, which
+// for these laws is the same value (the per-side arithmetic is sign
+// symmetric), and periodic-image pairs are evaluated from both sides there
+// too (neigh_gran.cpp:570-640).
+//
+// k_integrate: FixNVESphere::initial_integrate / final_integrate
+// (fix_nve_sphere.cpp:134-243) with dtf = 0.5*dt, INERTIA = 0.4; mode 3 runs
+// final(n) and initial(n+1) in one pass over the state.
+#include "contact_law.h"
+#include "kernels.h"
+
+namespace dem {
+
+// Blocks are dealt to the 8 XCDs round robin; remap so each XCD walks a
+// contiguous range of Morton-ordered particles and their neighbours stay in
+// that XCD's L2.
+__device__ inline int xcd_block(int b, int nb_padded)
+{
+    const int per = nb_padded >> 3;
+    return (b & 7) * per + (b >> 3);
+}
+
+template <bool MIXED>
+__global__ void __launch_bounds__(kBlock) k_force(ForceArgs a)
+{
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    const int i = blk * kBlock + threadIdx.x;
+    if (i >= a.n) return;
+    const Model &M = a.M;
+    const double4 pi = a.xr[i];
+    const double4 vmi = a.vm[i];
+    const double4 wi = a.wt[i];
+    const int metai = (int)wi.w;
+    const int itype = metai & 15;
+    const bool fzi = (metai >> 4) != 0;
+    double F[3] = {0.0, 0.0, 0.0}, T[3] = {0.0, 0.0, 0.0};
+    const int dnum = M.dnum;
+    const double cdm = M.cdf * M.cdf;
+    int nc = 0;
+
+    ContactIn c;
+    c.is_wall = false;
+    c.itype = itype;
+    c.radi = pi.w;
+    c.mi = vmi.w;
+    c.vi[0] = vmi.x; c.vi[1] = vmi.y; c.vi[2] = vmi.z;
+    c.wi[0] = wi.x; c.wi[1] = wi.y; c.wi[2] = wi.z;
+    c.deltan_in = 0.0;
+
+    const int e0 = a.nstart[i], e1 = a.nstart[i + 1];
+    for (int e = e0; e < e1; e++) {
+        const uint32_t w = a.nj[e];
+        const int j = (int)(w & kJMask);
+        const int code = (int)((w >> 26) & 31u);
+        const double4 pj = a.xr[j];
+        double xj0 = pj.x, xj1 = pj.y, xj2 = pj.z;
+        if (code != kNoShift) {
+            const double3 im = image_of(pj, code, M.prd);
+            xj0 = im.x; xj1 = im.y; xj2 = im.z;
+        }
+        const double delx = pi.x - xj0;
+        const double dely = pi.y - xj1;
+        const double delz = pi.z - xj2;
+        const double rsq = delx * delx + dely * dely + delz * delz;
+        const double radj = pj.w;
+        const double radsum = pi.w + radj;
+        if (rsq < radsum * radsum) {
+            const double r = sqrt(rsq);
+            if (r < 1e-20) atomicOr(a.err, 1);
+            const double rinv = 1.0 / r;
+            const double4 vmj = a.vm[j];
+            const double4 wj = a.wt[j];
+            const int metaj = (int)wj.w;
+            c.jtype = metaj & 15;
+            c.radj = radj;
+            c.mj = vmj.w;
+            double meff = c.mi * c.mj / (c.mi + c.mj);
+            if (M.freeze_on && fzi) meff = c.mj;
+            if (M.freeze_on && (metaj >> 4)) meff = c.mi;
+            c.meff = meff;
+            c.delta[0] = delx; c.delta[1] = dely; c.delta[2] = delz;
+            c.r = r; c.rinv = rinv; c.radsum = radsum;
+            c.vj[0] = vmj.x; c.vj[1] = vmj.y; c.vj[2] = vmj.z;
+            c.wj[0] = wj.x; c.wj[1] = wj.y; c.wj[2] = wj.z;
+            double h[kMaxDnum];
+            double *hp = a.hist + (size_t)e * dnum;
+            for (int k = 0; k < dnum; k++) h[k] = hp[k];
+            double dF[3], dT[3];
+            if (MIXED) contact_mixed(M, a.mat, c, h, a.shearupdate, dF, dT);
+            else contact_fp64(M, a.mat, c, h, a.shearupdate, dF, dT);
+            if (a.shearupdate)
+                for (int k = 0; k < dnum; k++) hp[k] = h[k];
+            if (!(w & kFlagBit)) a.nj[e] = w | kFlagBit;
+            F[0] += dF[0]; F[1] += dF[1]; F[2] += dF[2];
+            T[0] += dT[0]; T[1] += dT[1]; T[2] += dT[2];
+            nc++;
+        } else if (rsq < cdm * radsum * radsum) {
+            // surfacesClose: tangential and rolling history reset
+            double *hp = a.hist + (size_t)e * dnum;
+            for (int k = 0; k < dnum; k++) hp[k] = 0.0;
+        }
+    }
+
+    // primitive planes, in the order the fixes were defined
+    if (a.P.n) {
+        const uint32_t wm = a.wmask[i];
+        const double *xc = &pi.x;
+        const double cdmw = M.cdf - 1.0;
+        for (int k = 0; k < a.P.n; k++) {
+            if (!(wm & (1u << k))) continue;
+            const int d = a.P.dim[k];
+            const double p = a.P.pos[k];
+            const double radi = pi.w;
+            const double pos = xc[d];
+            double delta[3] = {0.0, 0.0, 0.0};
+            double deltan;
+            if (pos > p) {
+                delta[d] = p - pos;
+                deltan = pos - p - radi;
+            } else {
+                delta[d] = p - pos;
+                deltan = p - pos - radi;
+            }
+            double *hp = a.whist + ((size_t)i * a.P.n + k) * dnum;
+            if (deltan > M.cutneighmax) continue;
+            if (deltan <= 0 || deltan < cdmw * radi) {
+                if (deltan <= 0) {
+                    ContactIn cw;
+                    cw.is_wall = true;
+                    cw.itype = itype;
+                    cw.jtype = a.P.wtype[k];
+                    cw.radi = radi;
+                    cw.radj = 0.0;
+                    cw.deltan_in = -deltan;
+                    cw.delta[0] = -delta[0]; cw.delta[1] = -delta[1]; cw.delta[2] = -delta[2];
+                    cw.r = cw.radi - cw.deltan_in;
+                    cw.rinv = 1.0 / cw.r;
+                    cw.radsum = cw.radi;
+                    cw.mi = vmi.w; cw.mj = 0.0; cw.meff = vmi.w;
+                    cw.vi[0] = vmi.x; cw.vi[1] = vmi.y; cw.vi[2] = vmi.z;
+                    cw.vj[0] = 0.0; cw.vj[1] = 0.0; cw.vj[2] = 0.0;
+                    cw.wi[0] = wi.x; cw.wi[1] = wi.y; cw.wi[2] = wi.z;
+                    cw.wj[0] = 0.0; cw.wj[1] = 0.0; cw.wj[2] = 0.0;
+                    double h[kMaxDnum];
+                    for (int q = 0; q < dnum; q++) h[q] = hp[q];
+                    double dF[3], dT[3];
+                    if (MIXED) contact_mixed(M, a.mat, cw, h, a.shearupdate, dF, dT);
+                    else contact_fp64(M, a.mat, cw, h, a.shearupdate, dF, dT);
+                    if (a.shearupdate)
+                        for (int q = 0; q < dnum; q++) hp[q] = h[q];
+                    F[0] += dF[0]; F[1] += dF[1]; F[2] += dF[2];
+                    T[0] += dT[0]; T[1] += dT[1]; T[2] += dT[2];
+                } else {
+                    for (int q = 0; q < dnum; q++) hp[q] = 0.0;
+                }
+            } else {
+                for (int q = 0; q < dnum; q++) hp[q] = 0.0;
+            }
+        }
+    }
+    if (M.gravity_on) {
+        const double massone = vmi.w;
+        F[0] += massone * M.gacc[0];
+        F[1] += massone * M.gacc[1];
+        F[2] += massone * M.gacc[2];
+    }
+    if (M.freeze_on && fzi) {
+        F[0] = F[1] = F[2] = 0.0;
+        T[0] = T[1] = T[2] = 0.0;
+    }
+    a.f4[i] = make_double4(F[0], F[1], F[2], 0.0);
+    a.t4[i] = make_double4(T[0], T[1], T[2], 0.0);
+    if (a.ncontact) a.ncontact[i] = nc;
+}
+
+static int padded_blocks(int n)
+{
+    int nb = (n + kBlock - 1) / kBlock;
+    return (nb + 7) & ~7;
+}
+
+void launch_force_fp64(const ForceArgs &a, hipStream_t s)
+{
+    if (a.n == 0) return;
+    k_force<false><<<padded_blocks(a.n), kBlock, 0, s>>>(a);
+}
+
+void launch_force_mixed(const ForceArgs &a, hipStream_t s)
+{
+    if (a.n == 0) return;
+    k_force<true><<<padded_blocks(a.n), kBlock, 0, s>>>(a);
+}
+
+// ---------------------------------------------------------------------------
+
+template <int MODE>
+__global__ void __launch_bounds__(kBlock) k_integrate(IntegArgs a)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    double4 x = a.xr[i];
+    double4 v = a.vm[i];
+    double4 w = a.wt[i];
+    const double4 f = a.f4[i];
+    const double4 t = a.t4[i];
+    // dtfm = dtf / (rmass*(1.+CAddRhoFluid/density)), CAddRhoFluid = 0
+    const double dtfm = a.dtf / (v.w * (1. + 0.0 / 1.0));
+    const double dtirotate = a.dtfrotate / (x.w * x.w * v.w);
+    if (MODE & 2) {  // final_integrate(n)
+        v.x += dtfm * f.x; v.y += dtfm * f.y; v.z += dtfm * f.z;
+        w.x += dtirotate * t.x; w.y += dtirotate * t.y; w.z += dtirotate * t.z;
+    }
+    if (MODE & 1) {  // initial_integrate(n+1)
+        v.x += dtfm * f.x; v.y += dtfm * f.y; v.z += dtfm * f.z;
+        x.x += a.dtv * v.x; x.y += a.dtv * v.y; x.z += a.dtv * v.z;
+        w.x += dtirotate * t.x; w.y += dtirotate * t.y; w.z += dtirotate * t.z;
+        a.xr[i] = x;
+    }
+    a.vm[i] = v;
+    a.wt[i] = w;
+}
+
+void launch_integrate(const IntegArgs &a, int mode, hipStream_t s)
+{
+    if (a.n == 0) return;
+    const int nb = (a.n + kBlock - 1) / kBlock;
+    if (mode == 1) k_integrate<1><<<nb, kBlock, 0, s>>>(a);
+    else if (mode == 2) k_integrate<2><<<nb, kBlock, 0, s>>>(a);
+    else k_integrate<3><<<nb, kBlock, 0, s>>>(a);
+}
+
+__global__ void __launch_bounds__(kBlock) k_check_distance(const double4 *xr, const double4 *xhold, int n,
+                                                          double deltasq, int *flag)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double4 a = xr[i], b = xhold[i];
+    const double dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
+    const double rsq = dx * dx + dy * dy + dz * dz;
+    if (rsq > deltasq) atomicOr(flag, 1);
+}
+
+void launch_check_distance(const double4 *xr, const double4 *xhold, int n, double deltasq, int *flag,
+                           hipStream_t s)
+{
+    if (n == 0) return;
+    k_check_distance<<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(xr, xhold, n, deltasq, flag);
+}
+
+}  // namespace dem

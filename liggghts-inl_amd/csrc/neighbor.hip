@@ -1,0 +1,317 @@
+// neighbor.hip -- neighbour rebuild on the device.
+//
+// Replaces Domain::pbc (domain.cpp:550-620), Neighbor::bin_atoms
+// (neighbor.cpp:2034-2070), granular_bin_no_newton (neigh_gran.cpp:485-640)
+// and the history carry-over of FixContactHistory::pre_exchange
+// (fix_contact_history.cpp:305-426).
+//
+// Pipeline (one rebuild):
+//   k_bin       wrap periodic coordinates, Morton key of the cell, histogram
+//   scan        exclusive prefix sum over the key space (cell starts)
+//   k_scatter   counting-sort scatter (atomic slot within the cell)
+//   k_cellsort  order each cell by previous index, so the result is unique
+//   k_permute   gather every per-atom array into Morton order; wall lists
+//   k_ncount    full-list candidates per atom over the 27 cells, scan
+//   k_nfill     write the list; carry contact history of pairs that touch,
+//               looked up by partner tag in the atom's previous row
+//
+// The membership predicate is the reference's, in FP64 without contraction:
+//   rsq <= ((r_i + r_j) * cdf + skin)^2
+// and history is carried iff rsq < ((r_i + r_j) * cdf)^2 and the previous
+// row holds a flagged entry with the same partner tag (neigh_gran.cpp:592-612).
+#include "kernels.h"
+
+namespace dem {
+
+__device__ inline int cell_coord(double x, int d, const Grid &G)
+{
+    int c = (int)floor((x - G.lo[d]) * G.inv[d]);
+    c = c < 0 ? 0 : c;
+    c = c >= G.nc[d] ? G.nc[d] - 1 : c;
+    return c;
+}
+
+__device__ inline uint32_t cell_key(int cx, int cy, int cz, const Grid &G)
+{
+    return G.spread[cx] | G.spread[G.maxc + cy] | G.spread[2 * G.maxc + cz];
+}
+
+__global__ void __launch_bounds__(kBlock) k_bin(BinArgs a)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    double4 p = a.xr[i];
+    int4 im = a.img[i];
+    double *pc = &p.x;
+    int *ic = &im.x;
+    // Domain::pbc, orthogonal box
+    for (int d = 0; d < 3; d++) {
+        if (!a.G.periodic[d]) continue;
+        const double lo = a.G.lo[d], hi = a.hi[d];
+        if (pc[d] < lo) {
+            pc[d] += a.prd[d];
+            ic[d]--;
+        }
+        if (pc[d] >= hi) {
+            pc[d] -= a.prd[d];
+            pc[d] = pc[d] > lo ? pc[d] : lo;
+            ic[d]++;
+        }
+    }
+    a.xr[i] = p;
+    a.img[i] = im;
+    const uint32_t k = cell_key(cell_coord(p.x, 0, a.G), cell_coord(p.y, 1, a.G),
+                                cell_coord(p.z, 2, a.G), a.G);
+    a.key[i] = k;
+    atomicAdd(&a.count[k], 1);
+}
+
+// ---------------------------------------------------------------------------
+// Exclusive prefix sum, 4096 elements per 1024-thread block, wave64 shuffles.
+
+constexpr int kScanThreads = 1024;
+constexpr int kScanPer = 4;
+constexpr int kScanTile = kScanThreads * kScanPer;
+
+__global__ void __launch_bounds__(kScanThreads) k_scan_local(int *data, int n, int *blocksum)
+{
+    __shared__ int wsum[kScanThreads / 64];
+    const int base = blockIdx.x * kScanTile + threadIdx.x * kScanPer;
+    int v[kScanPer];
+    int tsum = 0;
+    for (int k = 0; k < kScanPer; k++) {
+        v[k] = (base + k < n) ? data[base + k] : 0;
+        tsum += v[k];
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    int incl = tsum;
+    for (int off = 1; off < 64; off <<= 1) {
+        const int t = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += t;
+    }
+    if (lane == 63) wsum[wid] = incl;
+    __syncthreads();
+    if (wid == 0) {
+        int w = lane < kScanThreads / 64 ? wsum[lane] : 0;
+        int wi = w;
+        for (int off = 1; off < 64; off <<= 1) {
+            const int t = __shfl_up(wi, off, 64);
+            if (lane >= off) wi += t;
+        }
+        if (lane < kScanThreads / 64) wsum[lane] = wi - w;  // exclusive wave offsets
+        if (lane == kScanThreads / 64 - 1 && blocksum) blocksum[blockIdx.x] = wi;
+    }
+    __syncthreads();
+    int run = wsum[wid] + incl - tsum;
+    for (int k = 0; k < kScanPer; k++) {
+        if (base + k < n) data[base + k] = run;
+        run += v[k];
+    }
+}
+
+__global__ void __launch_bounds__(kScanThreads) k_scan_add(int *data, int n, const int *blockoff)
+{
+    const int base = blockIdx.x * kScanTile + threadIdx.x * kScanPer;
+    const int off = blockoff[blockIdx.x];
+    for (int k = 0; k < kScanPer; k++)
+        if (base + k < n) data[base + k] += off;
+}
+
+int scan_tmp_size(int n)
+{
+    int total = 0;
+    int m = n;
+    while (m > kScanTile) {
+        m = (m + kScanTile - 1) / kScanTile;
+        total += m + 1;
+    }
+    return total + 1;
+}
+
+void launch_scan(int *data, int n, int *tmp, hipStream_t s)
+{
+    const int nb = (n + kScanTile - 1) / kScanTile;
+    if (nb <= 1) {
+        k_scan_local<<<1, kScanThreads, 0, s>>>(data, n, nullptr);
+        return;
+    }
+    k_scan_local<<<nb, kScanThreads, 0, s>>>(data, n, tmp);
+    launch_scan(tmp, nb, tmp + nb + 1, s);
+    k_scan_add<<<nb, kScanThreads, 0, s>>>(data, n, tmp);
+}
+
+// ---------------------------------------------------------------------------
+
+__global__ void __launch_bounds__(kBlock) k_scatter(const uint32_t *key, int *cursor, int *sorted, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int pos = atomicAdd(&cursor[key[i]], 1);
+    sorted[pos] = i;
+}
+
+void launch_scatter(const uint32_t *key, int *cursor, int *sorted, int n, hipStream_t s)
+{
+    if (n == 0) return;
+    k_scatter<<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(key, cursor, sorted, n);
+}
+
+__global__ void __launch_bounds__(kBlock) k_cellsort(const int *cstart, int *sorted, int nkeys)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nkeys) return;
+    const int b = cstart[k], e = cstart[k + 1];
+    for (int p = b + 1; p < e; p++) {
+        const int v = sorted[p];
+        int q = p - 1;
+        while (q >= b && sorted[q] > v) {
+            sorted[q + 1] = sorted[q];
+            q--;
+        }
+        sorted[q + 1] = v;
+    }
+}
+
+void launch_cellsort(const int *cstart, int *sorted, int nkeys, hipStream_t s)
+{
+    k_cellsort<<<(nkeys + kBlock - 1) / kBlock, kBlock, 0, s>>>(cstart, sorted, nkeys);
+}
+
+void launch_bin(const BinArgs &a, hipStream_t s)
+{
+    if (a.n == 0) return;
+    k_bin<<<(a.n + kBlock - 1) / kBlock, kBlock, 0, s>>>(a);
+}
+
+// ---------------------------------------------------------------------------
+
+__global__ void __launch_bounds__(kBlock) k_permute(PermuteArgs a)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.n) return;
+    const int o = a.sorted[p];
+    const double4 x = a.xr0[o];
+    a.xr1[p] = x;
+    a.xhold[p] = x;
+    a.vm1[p] = a.vm0[o];
+    a.wt1[p] = a.wt0[o];
+    a.tag1[p] = a.tag0[o];
+    a.orig1[p] = a.orig0[o];
+    a.img1[p] = a.img0[o];
+    for (int k = 0; k < a.wdn; k++) a.whist1[(size_t)p * a.wdn + k] = a.whist0[(size_t)o * a.wdn + k];
+    // PrimitiveWall::buildNeighList with treshold = skin
+    // (fix_wall_gran.cpp:700-703, primitive_wall_definitions.h:142-148)
+    uint32_t m = 0;
+    const double *xc = &x.x;
+    for (int k = 0; k < a.P.n; k++) {
+        const double dMax = x.w + a.skin;
+        const double dist = xc[a.P.dim[k]] - a.P.pos[k];
+        const double absdist = (dist > 0.0) ? dist : -dist;
+        if (absdist <= dMax) m |= 1u << k;
+    }
+    a.wmask[p] = (uint8_t)m;
+}
+
+void launch_permute(const PermuteArgs &a, hipStream_t s)
+{
+    if (a.n == 0) return;
+    k_permute<<<(a.n + kBlock - 1) / kBlock, kBlock, 0, s>>>(a);
+}
+
+// ---------------------------------------------------------------------------
+
+template <bool FILL>
+__global__ void __launch_bounds__(kBlock) k_neigh(NeighArgs a)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.n) return;
+    const double4 xp = a.xr[p];
+    const int cx = cell_coord(xp.x, 0, a.G), cy = cell_coord(xp.y, 1, a.G), cz = cell_coord(xp.z, 2, a.G);
+    const int cc[3] = {cx, cy, cz};
+    int cnt = 0;
+    int e = FILL ? a.nstart[p] : 0;
+    int ob = 0, oe = 0;
+    if (FILL && a.have_old) {
+        const int o = a.perm[p];
+        ob = a.onstart[o];
+        oe = a.onstart[o + 1];
+    }
+    for (int dz = -1; dz <= 1; dz++)
+        for (int dy = -1; dy <= 1; dy++)
+            for (int dx = -1; dx <= 1; dx++) {
+                const int off[3] = {dx, dy, dz};
+                int c[3], sh[3];
+                bool ok = true;
+                for (int d = 0; d < 3; d++) {
+                    c[d] = cc[d] + off[d];
+                    sh[d] = 0;
+                    if (c[d] < 0) {
+                        if (a.G.periodic[d]) { c[d] += a.G.nc[d]; sh[d] = -1; }
+                        else ok = false;
+                    } else if (c[d] >= a.G.nc[d]) {
+                        if (a.G.periodic[d]) { c[d] -= a.G.nc[d]; sh[d] = 1; }
+                        else ok = false;
+                    }
+                }
+                if (!ok) continue;
+                const int code = (sh[0] + 1) + 3 * (sh[1] + 1) + 9 * (sh[2] + 1);
+                const uint32_t key = cell_key(c[0], c[1], c[2], a.G);
+                const int qb = a.cstart[key], qe = a.cstart[key + 1];
+                for (int q = qb; q < qe; q++) {
+                    if (q == p && code == kNoShift) continue;
+                    const double4 xq = a.xr[q];
+                    const double xj0 = xq.x + sh[0] * a.prd[0];
+                    const double xj1 = xq.y + sh[1] * a.prd[1];
+                    const double xj2 = xq.z + sh[2] * a.prd[2];
+                    const double delx = xp.x - xj0;
+                    const double dely = xp.y - xj1;
+                    const double delz = xp.z - xj2;
+                    const double rsq = delx * delx + dely * dely + delz * delz;
+                    const double radsum = (xp.w + xq.w) * a.cdf;
+                    const double cutsq = (radsum + a.skin) * (radsum + a.skin);
+                    if (!(rsq <= cutsq)) continue;
+                    if (!FILL) {
+                        cnt++;
+                        continue;
+                    }
+                    uint32_t w = (uint32_t)q | ((uint32_t)code << 26);
+                    double *h = a.hist + (size_t)e * a.dnum;
+                    bool carried = false;
+                    if (a.have_old && rsq < radsum * radsum) {
+                        const int tq = a.tag[q];
+                        for (int oe2 = ob; oe2 < oe; oe2++) {
+                            const uint32_t ow = a.onj[oe2];
+                            if (!(ow & kFlagBit)) continue;
+                            if (a.otag[ow & kJMask] != tq) continue;
+                            const double *oh = a.ohist + (size_t)oe2 * a.dnum;
+                            for (int k = 0; k < a.dnum; k++) h[k] = oh[k];
+                            carried = true;
+                            break;
+                        }
+                    }
+                    if (carried) w |= kFlagBit;
+                    else
+                        for (int k = 0; k < a.dnum; k++) h[k] = 0.0;
+                    a.nj[e] = w;
+                    e++;
+                }
+            }
+    if (!FILL) {
+        a.count[p] = cnt;
+        atomicMax(a.maxcount, cnt);
+    }
+}
+
+void launch_neigh_count(const NeighArgs &a, hipStream_t s)
+{
+    if (a.n == 0) return;
+    k_neigh<false><<<(a.n + kBlock - 1) / kBlock, kBlock, 0, s>>>(a);
+}
+
+void launch_neigh_fill(const NeighArgs &a, hipStream_t s)
+{
+    if (a.n == 0) return;
+    k_neigh<true><<<(a.n + kBlock - 1) / kBlock, kBlock, 0, s>>>(a);
+}
+
+}  // namespace dem

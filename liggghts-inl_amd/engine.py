@@ -1,0 +1,275 @@
+"""ctypes binding of the engine's C-ABI (include/liggghts_dem.h).
+
+This mirrors what the LIGGGHTS style plug-in does from C++: set up a
+context from the script's pair/fix/neighbor settings, upload the Atom
+arrays, run Verlet steps on the device and read results back.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIBNAME = "libliggghts_dem.so"
+_LIB = None
+
+DEM_FP64 = 0
+DEM_MIXED = 1
+ROLL = {"off": 0, "epsd": 1, "epsd2": 2, "epsd3": 3}
+
+
+class DemError(RuntimeError):
+    pass
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("dt", C.c_double),
+        ("boxlo", C.c_double * 3),
+        ("boxhi", C.c_double * 3),
+        ("periodic", C.c_int * 3),
+        ("skin", C.c_double),
+        ("cdf", C.c_double),
+        ("neigh_every", C.c_int),
+        ("neigh_delay", C.c_int),
+        ("neigh_check", C.c_int),
+        ("neigh_one", C.c_int),
+        ("rolling_model", C.c_int),
+        ("tangential_damping", C.c_int),
+        ("limit_force", C.c_int),
+        ("torsion_torque", C.c_int),
+        ("precision", C.c_int),
+        ("device", C.c_int),
+    ]
+
+
+def library_path():
+    return os.path.join(_HERE, _LIBNAME)
+
+
+def build():
+    """Compile csrc/ for gfx950 with its Makefile (hipcc cross-compiles)."""
+    subprocess.run(["make", "-s", "-C", os.path.join(_HERE, "csrc")], check=True)
+
+
+def library():
+    """Load the HIP library; raise if it is not built."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    path = library_path()
+    if not os.path.exists(path):
+        raise DemError("%s is not built; run __graft_entry__.build()" % path)
+    L = C.CDLL(path)
+    dp = C.POINTER(C.c_double)
+    ip = C.POINTER(C.c_int)
+    i64p = C.POINTER(C.c_int64)
+    vp = C.c_void_p
+    sig = {
+        "dem_ctx_create": (vp, [C.POINTER(Params)]),
+        "dem_ctx_destroy": (None, [vp]),
+        "dem_last_error": (C.c_char_p, [vp]),
+        "dem_material_tables": (C.c_int, [C.c_int, dp, dp, dp, dp, dp, dp]),
+        "dem_set_materials": (C.c_int, [vp, C.c_int, dp, dp, dp, dp, dp, dp, C.c_double]),
+        "dem_upload_state": (C.c_int, [vp, C.c_int, dp, dp, dp, dp, dp, ip, ip, ip]),
+        "dem_download": (C.c_int, [vp, dp, dp, dp, dp, dp]),
+        "dem_add_plane": (C.c_int, [vp, C.c_int, C.c_double, C.c_int]),
+        "dem_set_gravity": (C.c_int, [vp, C.c_double, C.c_double, C.c_double]),
+        "dem_set_freeze": (C.c_int, [vp, C.c_int]),
+        "dem_setup": (C.c_int, [vp]),
+        "dem_step": (C.c_int, [vp, C.c_int]),
+        "dem_compute_forces": (C.c_int, [vp, C.c_int]),
+        "dem_rebuild": (C.c_int, [vp]),
+        "dem_get_pairs": (C.c_int, [vp, i64p, ip, ip, ip, dp, ip]),
+        "dem_history_size": (C.c_int, [vp]),
+        "dem_set_history": (C.c_int, [vp, C.c_int64, ip, ip, dp]),
+        "dem_get_wall_history": (C.c_int, [vp, dp]),
+        "dem_set_wall_history": (C.c_int, [vp, dp]),
+        "dem_timing_enable": (C.c_int, [vp, C.c_int]),
+        "dem_timing_get": (C.c_int, [vp, C.c_char_p, dp, i64p]),
+        "dem_list_stats": (C.c_int, [vp, dp, dp, i64p]),
+        "dem_sync": (C.c_int, [vp]),
+        "dem_timestep": (C.c_int64, [vp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _LIB = L
+    return L
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def material_tables(ntypes, Y, nu, e):
+    """Yeff, Geff, betaeff as global_properties.cpp:428-562 builds them."""
+    L = library()
+    w = ntypes + 1
+    Y = np.ascontiguousarray(Y, dtype=np.float64).reshape(w)
+    nu = np.ascontiguousarray(nu, dtype=np.float64).reshape(w)
+    e = np.ascontiguousarray(e, dtype=np.float64).reshape(w, w)
+    out = [np.zeros((w, w)) for _ in range(3)]
+    rc = L.dem_material_tables(ntypes, _dp(Y), _dp(nu), _dp(e), _dp(out[0]), _dp(out[1]), _dp(out[2]))
+    if rc:
+        raise DemError(L.dem_last_error(None).decode())
+    return out
+
+
+class Simulation:
+    """One device simulation.
+
+    ``spec``: dict with box_lo, box_hi, periodic, dt, skin, cdf,
+    neigh_every, neigh_delay, neigh_check, rolling, tangential_damping,
+    limit_force, materials {ntypes, Y, nu, e, mu, mur, etar, krs},
+    planes [(dim, pos, wtype)], gravity (ax, ay, az) or None, freeze (bool).
+    ``atoms``: dict of numpy arrays x, v, omega (n,3), radius, rmass, type,
+    tag, optional freeze.
+    """
+
+    def __init__(self, spec, atoms, precision="fp64", device=0):
+        self.L = library()
+        p = Params()
+        p.dt = spec["dt"]
+        for d in range(3):
+            p.boxlo[d] = spec["box_lo"][d]
+            p.boxhi[d] = spec["box_hi"][d]
+            p.periodic[d] = int(spec["periodic"][d])
+        p.skin = spec["skin"]
+        p.cdf = spec.get("cdf", 1.0)
+        p.neigh_every = spec.get("neigh_every", 1)
+        p.neigh_delay = spec.get("neigh_delay", 0)
+        p.neigh_check = int(spec.get("neigh_check", False))
+        p.neigh_one = spec.get("neigh_one", 2000)
+        p.rolling_model = ROLL[spec.get("rolling", "off")]
+        p.tangential_damping = int(spec.get("tangential_damping", True))
+        p.limit_force = int(spec.get("limit_force", False))
+        p.torsion_torque = int(spec.get("torsion_torque", False))
+        p.precision = DEM_MIXED if precision == "mixed" else DEM_FP64
+        p.device = device
+        self.h = self.L.dem_ctx_create(C.byref(p))
+        if not self.h:
+            raise DemError(self.L.dem_last_error(None).decode())
+        for dim, pos, wt in spec.get("planes", []):
+            self._chk(self.L.dem_add_plane(self.h, int(dim), float(pos), int(wt)))
+        self.nplanes = len(spec.get("planes", []))
+        g = spec.get("gravity")
+        if g is not None:
+            self._chk(self.L.dem_set_gravity(self.h, float(g[0]), float(g[1]), float(g[2])))
+        if spec.get("freeze", False):
+            self._chk(self.L.dem_set_freeze(self.h, 1))
+        m = spec["materials"]
+        nt = m["ntypes"]
+        w = nt + 1
+        Yeff, Geff, beta = material_tables(nt, m["Y"], m["nu"], m["e"])
+        mu = np.ascontiguousarray(m["mu"], dtype=np.float64).reshape(w, w)
+        mur = np.ascontiguousarray(m.get("mur", np.zeros((w, w))), dtype=np.float64).reshape(w, w)
+        etar = np.ascontiguousarray(m.get("etar", np.zeros((w, w))), dtype=np.float64).reshape(w, w)
+        self._chk(self.L.dem_set_materials(self.h, nt, _dp(Yeff), _dp(Geff), _dp(beta), _dp(mu),
+                                           _dp(mur), _dp(etar), float(m.get("krs", 0.0))))
+        self.upload(atoms)
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise DemError("dem error %d: %s" % (rc, self.L.dem_last_error(self.h).decode()))
+
+    def upload(self, atoms):
+        n = len(atoms["radius"])
+        self.n = n
+        a = {k: np.ascontiguousarray(atoms[k], dtype=np.float64) for k in ("x", "v", "omega", "radius", "rmass")}
+        ty = np.ascontiguousarray(atoms["type"], dtype=np.int32)
+        tag = np.ascontiguousarray(atoms["tag"], dtype=np.int32)
+        fr = np.ascontiguousarray(atoms.get("freeze", np.zeros(n)), dtype=np.int32)
+        self.tag = tag.copy()
+        self._chk(self.L.dem_upload_state(self.h, n, _dp(a["x"]), _dp(a["v"]), _dp(a["omega"]),
+                                          _dp(a["radius"]), _dp(a["rmass"]), _ip(ty), _ip(tag), _ip(fr)))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.dem_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def setup(self):
+        self._chk(self.L.dem_setup(self.h))
+
+    def step(self, n):
+        self._chk(self.L.dem_step(self.h, int(n)))
+
+    def compute_forces(self, shearupdate=1):
+        self._chk(self.L.dem_compute_forces(self.h, int(shearupdate)))
+
+    def rebuild(self):
+        self._chk(self.L.dem_rebuild(self.h))
+
+    def sync(self):
+        self._chk(self.L.dem_sync(self.h))
+
+    def state(self):
+        n = self.n
+        out = {k: np.zeros((n, 3)) for k in ("x", "v", "omega", "f", "torque")}
+        self._chk(self.L.dem_download(self.h, _dp(out["x"]), _dp(out["v"]), _dp(out["omega"]),
+                                      _dp(out["f"]), _dp(out["torque"])))
+        out["tag"] = self.tag.copy()
+        return out
+
+    @property
+    def dnum(self):
+        return self.L.dem_history_size(self.h)
+
+    def pairs(self):
+        """Directed full list: (tag_i, tag_j, shift, hist[dnum], flag)."""
+        cnt = C.c_int64(0)
+        self._chk(self.L.dem_get_pairs(self.h, C.byref(cnt), None, None, None, None, None))
+        c = cnt.value
+        ti = np.zeros(max(c, 1), np.int32)
+        tj = np.zeros(max(c, 1), np.int32)
+        sh = np.zeros(max(c, 1), np.int32)
+        fl = np.zeros(max(c, 1), np.int32)
+        hi = np.zeros((max(c, 1), self.dnum))
+        self._chk(self.L.dem_get_pairs(self.h, C.byref(cnt), _ip(ti), _ip(tj), _ip(sh), _dp(hi), _ip(fl)))
+        return ti[:c], tj[:c], sh[:c], hi[:c], fl[:c]
+
+    def set_history(self, tag_i, tag_j, hist):
+        ti = np.ascontiguousarray(tag_i, dtype=np.int32)
+        tj = np.ascontiguousarray(tag_j, dtype=np.int32)
+        h = np.ascontiguousarray(hist, dtype=np.float64).reshape(len(ti), self.dnum)
+        self._chk(self.L.dem_set_history(self.h, len(ti), _ip(ti), _ip(tj), _dp(h)))
+
+    def wall_history(self):
+        h = np.zeros((self.n, max(self.nplanes, 1), self.dnum))
+        if self.nplanes:
+            self._chk(self.L.dem_get_wall_history(self.h, _dp(h)))
+        return h
+
+    def set_wall_history(self, hist):
+        if self.nplanes:
+            h = np.ascontiguousarray(hist, dtype=np.float64).reshape(self.n, self.nplanes, self.dnum)
+            self._chk(self.L.dem_set_wall_history(self.h, _dp(h)))
+
+    def timing(self, on=True):
+        self._chk(self.L.dem_timing_enable(self.h, int(on)))
+
+    def timer(self, name):
+        ms = C.c_double(0)
+        cnt = C.c_int64(0)
+        self._chk(self.L.dem_timing_get(self.h, name.encode(), C.byref(ms), C.byref(cnt)))
+        return ms.value, cnt.value
+
+    def list_stats(self):
+        a = C.c_double(0)
+        b = C.c_double(0)
+        e = C.c_int64(0)
+        self._chk(self.L.dem_list_stats(self.h, C.byref(a), C.byref(b), C.byref(e)))
+        return a.value, b.value, e.value
+
+    def timestep(self):
+        return self.L.dem_timestep(self.h)

@@ -1,0 +1,1017 @@
+/*
+ * dem_oracle.c -- CPU restatement of the LIGGGHTS-INL sphere DEM timestep.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker, never the product).  See
+ * dem_oracle.h.  Each function cites the reference source it restates;
+ * arithmetic is written in the same operation order so that per-contact
+ * results round identically.
+ *
+ * Deliberate differences from the reference, none of which changes a
+ * per-contact value:
+ *   - atoms are never re-sorted (atom.cpp:1326); only summation order moves;
+ *   - the neighbour list visits 27 cells of edge >= cutneighmax instead of
+ *     the reference's half-size bins and stencil; membership is decided by
+ *     the same predicate, only entry order differs;
+ *   - periodic images are not materialised as ghost atoms; a list entry
+ *     carries the image offset and the image position is x_j + s*prd, the
+ *     same expression comm->borders() uses (atom_vec_sphere.cpp:625-637).
+ */
+#include "dem_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define MAXDNUM 8
+#define T1 (OR_MAXTYPES + 1)
+
+enum { OP_WALL = 1, OP_GRAVITY = 2, OP_FREEZE = 3 };
+
+typedef struct {
+    int j;       /* owned atom index */
+    int shift;   /* (sx+1)+3(sy+1)+9(sz+1), 13 = no image */
+    int flag;    /* contact flag (nonzero once surfacesIntersect ran) */
+} entry_t;
+
+struct or_sim {
+    or_params p;
+    int ntypes;
+    double Yeff[T1 * T1], Geff[T1 * T1], betaeff[T1 * T1];
+    double mu[T1 * T1], mur[T1 * T1], etar[T1 * T1];
+    double krs;
+
+    int n;
+    double *x, *v, *omega, *f, *torque, *radius, *rmass, *xhold;
+    int *type, *tag, *freeze, *image;
+    double *fscale, *tscale;
+
+    int dnum;          /* pair history doubles: 3 shear (+3 rolling) */
+    int dnum_w;        /* wall history doubles */
+
+    /* half neighbour list (granular_bin_no_newton) */
+    int *nstart, *ncount;
+    entry_t *ent;
+    double *hist;
+    int nent, cap;
+    int have_list;
+
+    /* per-atom partner storage filled by pre_exchange */
+    int *npartner, *pstart, *ptag;
+    double *phist;
+
+    /* primitive planes */
+    int nplanes;
+    int pdim[OR_MAXPLANES], ptype[OR_MAXPLANES];
+    double ppos[OR_MAXPLANES];
+    unsigned char *pin;     /* n * nplanes: atom in the wall's neighbour list */
+    double *phist_w;        /* n * nplanes * dnum_w */
+
+    int nops, ops[16], opsarg[16];
+    int has_freeze;
+    double gacc[3];
+
+    long long ntimestep;
+    int ago;
+    int setupflag;
+    int wall_rebuild;
+    double cutneighmax;
+};
+
+static const double SQRT_FIVE_OVER_SIX = 0.91287092917527685576161630466800355658790782499663875;
+
+/* ------------------------------------------------------------------------ */
+
+or_sim *or_create(const or_params *p)
+{
+    or_sim *s = (or_sim *)calloc(1, sizeof(or_sim));
+    s->p = *p;
+    s->dnum = 3 + (p->rolling_model != OR_ROLL_OFF ? 3 : 0);
+    s->dnum_w = s->dnum;
+    return s;
+}
+
+static void free_atoms(or_sim *s)
+{
+    free(s->x); free(s->v); free(s->omega); free(s->f); free(s->torque);
+    free(s->radius); free(s->rmass); free(s->xhold); free(s->type); free(s->tag);
+    free(s->freeze); free(s->image); free(s->fscale); free(s->tscale);
+    free(s->nstart); free(s->ncount); free(s->npartner); free(s->pstart);
+    free(s->ptag); free(s->phist); free(s->pin); free(s->phist_w);
+}
+
+void or_destroy(or_sim *s)
+{
+    if (!s) return;
+    free_atoms(s);
+    free(s->ent); free(s->hist);
+    free(s);
+}
+
+/* global_properties.cpp:428-562 (createYeff, createGeff, createCoeffRestLog,
+ * createBetaEff) and :565-584 (per-type-pair pass-through). */
+int or_set_materials(or_sim *s, int ntypes, const double *Y, const double *nu,
+                     const double *e, const double *mu, const double *mur,
+                     const double *etar, double krs)
+{
+    if (ntypes < 1 || ntypes > OR_MAXTYPES) return -1;
+    s->ntypes = ntypes;
+    const int w = ntypes + 1;
+    for (int i = 1; i <= ntypes; i++)
+        for (int j = 1; j <= ntypes; j++) {
+            const double Yi = Y[i], Yj = Y[j], vi = nu[i], vj = nu[j];
+            s->Yeff[i * T1 + j] = 1. / ((1. - pow(vi, 2.)) / Yi + (1. - pow(vj, 2.)) / Yj);
+            s->Geff[i * T1 + j] = 1. / (2. * (2. - vi) * (1. + vi) / Yi + 2. * (2. - vj) * (1. + vj) / Yj);
+            const double lg = log(e[i * w + j]);
+            s->betaeff[i * T1 + j] = lg / sqrt(pow(lg, 2.) + pow(M_PI, 2.));
+            s->mu[i * T1 + j] = mu[i * w + j];
+            s->mur[i * T1 + j] = mur ? mur[i * w + j] : 0.0;
+            s->etar[i * T1 + j] = etar ? etar[i * w + j] : 0.0;
+        }
+    s->krs = krs;
+    return 0;
+}
+
+void or_get_material_tables(const or_sim *s, double *Yeff, double *Geff, double *betaeff)
+{
+    const int w = s->ntypes + 1;
+    for (int i = 0; i < w; i++)
+        for (int j = 0; j < w; j++) {
+            Yeff[i * w + j] = s->Yeff[i * T1 + j];
+            Geff[i * w + j] = s->Geff[i * T1 + j];
+            betaeff[i * w + j] = s->betaeff[i * T1 + j];
+        }
+}
+
+int or_set_atoms(or_sim *s, int n, const double *x, const double *v, const double *omega,
+                 const double *radius, const double *rmass, const int *type,
+                 const int *tag, const int *freeze)
+{
+    free_atoms(s);
+    s->n = n;
+    size_t n3 = (size_t)n * 3 * sizeof(double), n1 = (size_t)n * sizeof(double);
+    size_t ni = (size_t)n * sizeof(int);
+    s->x = malloc(n3); s->v = malloc(n3); s->omega = malloc(n3);
+    s->f = calloc(1, n3); s->torque = calloc(1, n3); s->xhold = malloc(n3);
+    s->radius = malloc(n1); s->rmass = malloc(n1);
+    s->fscale = calloc(1, n1); s->tscale = calloc(1, n1);
+    s->type = malloc(ni); s->tag = malloc(ni); s->freeze = calloc(1, ni);
+    s->image = calloc(3, ni);
+    s->nstart = calloc(1, ni); s->ncount = calloc(1, ni);
+    s->npartner = calloc(1, ni); s->pstart = calloc(1, ni);
+    s->ptag = NULL; s->phist = NULL;
+    memcpy(s->x, x, n3); memcpy(s->v, v, n3); memcpy(s->omega, omega, n3);
+    memcpy(s->radius, radius, n1); memcpy(s->rmass, rmass, n1);
+    memcpy(s->type, type, ni); memcpy(s->tag, tag, ni);
+    if (freeze) memcpy(s->freeze, freeze, ni);
+    s->pin = calloc((size_t)n * OR_MAXPLANES, 1);
+    s->phist_w = calloc((size_t)n * OR_MAXPLANES * s->dnum_w, sizeof(double));
+    s->nent = 0;
+    s->have_list = 0;
+    return 0;
+}
+
+int or_add_plane(or_sim *s, int dim, double pos, int wtype)
+{
+    if (s->nplanes >= OR_MAXPLANES || dim < 0 || dim > 2) return -1;
+    const int k = s->nplanes++;
+    s->pdim[k] = dim; s->ppos[k] = pos; s->ptype[k] = wtype;
+    s->ops[s->nops] = OP_WALL; s->opsarg[s->nops++] = k;
+    return 0;
+}
+
+void or_add_gravity(or_sim *s, double ax, double ay, double az)
+{
+    s->gacc[0] = ax; s->gacc[1] = ay; s->gacc[2] = az;
+    s->ops[s->nops] = OP_GRAVITY; s->opsarg[s->nops++] = 0;
+}
+
+void or_add_freeze(or_sim *s)
+{
+    s->ops[s->nops] = OP_FREEZE; s->opsarg[s->nops++] = 0;
+    s->has_freeze = 1;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Geometry helpers                                                          */
+
+static inline void shift_decode(int code, int *sx, int *sy, int *sz)
+{
+    *sx = code % 3 - 1;
+    *sy = (code / 3) % 3 - 1;
+    *sz = code / 9 - 1;
+}
+
+/* position of atom j's image: x[j] + dx with dx = pbc*prd
+ * (atom_vec_sphere.cpp:625-637) */
+static inline void image_pos(const or_sim *s, int j, int code, double xj[3])
+{
+    int sh[3];
+    shift_decode(code, &sh[0], &sh[1], &sh[2]);
+    for (int d = 0; d < 3; d++) {
+        const double prd = s->p.boxhi[d] - s->p.boxlo[d];
+        const double dx = sh[d] * prd;
+        xj[d] = s->x[3 * j + d] + dx;
+    }
+}
+
+/* Domain::pbc (domain.cpp:550-620), orthogonal box */
+static void domain_pbc(or_sim *s)
+{
+    for (int i = 0; i < s->n; i++)
+        for (int d = 0; d < 3; d++) {
+            if (!s->p.periodic[d]) continue;
+            const double lo = s->p.boxlo[d], hi = s->p.boxhi[d];
+            const double prd = hi - lo;
+            double *xi = &s->x[3 * i + d];
+            if (*xi < lo) {
+                *xi += prd;
+                s->image[3 * i + d]--;
+            }
+            if (*xi >= hi) {
+                *xi -= prd;
+                *xi = (*xi > lo) ? *xi : lo;
+                s->image[3 * i + d]++;
+            }
+        }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Contact history: FixContactHistory::pre_exchange                          */
+/* (fix_contact_history.cpp:305-426).  All history slots of the Hertz/       */
+/* history/EPSD* chain have newtonflag 1 (tangential_model_history.h:79-81,  */
+/* rolling_model_epsd.h:71-73), so the partner's copy is negated.            */
+
+static void pre_exchange(or_sim *s)
+{
+    const int n = s->n, dnum = s->dnum;
+    for (int i = 0; i < n; i++) s->npartner[i] = 0;
+    if (!s->have_list) return;
+    for (int i = 0; i < n; i++)
+        for (int k = 0; k < s->ncount[i]; k++) {
+            const entry_t *e = &s->ent[s->nstart[i] + k];
+            if (e->flag) {
+                s->npartner[i]++;
+                if (e->shift == 13) s->npartner[e->j]++;
+            }
+        }
+    int tot = 0;
+    for (int i = 0; i < n; i++) { s->pstart[i] = tot; tot += s->npartner[i]; }
+    free(s->ptag); free(s->phist);
+    s->ptag = malloc((size_t)(tot + 1) * sizeof(int));
+    s->phist = malloc((size_t)(tot + 1) * dnum * sizeof(double));
+    for (int i = 0; i < n; i++) s->npartner[i] = 0;
+    for (int i = 0; i < n; i++)
+        for (int k = 0; k < s->ncount[i]; k++) {
+            const int idx = s->nstart[i] + k;
+            const entry_t *e = &s->ent[idx];
+            if (!e->flag) continue;
+            const double *h = &s->hist[(size_t)idx * dnum];
+            int m = s->pstart[i] + s->npartner[i];
+            s->ptag[m] = s->tag[e->j];
+            for (int d = 0; d < dnum; d++) s->phist[(size_t)m * dnum + d] = h[d];
+            s->npartner[i]++;
+            if (e->shift == 13) {
+                const int j = e->j;
+                m = s->pstart[j] + s->npartner[j];
+                s->ptag[m] = s->tag[i];
+                for (int d = 0; d < dnum; d++) s->phist[(size_t)m * dnum + d] = -h[d];
+                s->npartner[j]++;
+            }
+        }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Neighbour list: Neighbor::granular_bin_no_newton (neigh_gran.cpp:485-640) */
+
+static void push_entry(or_sim *s, int j, int code, int flag, const double *h)
+{
+    if (s->nent == s->cap) {
+        s->cap = s->cap ? 2 * s->cap : 1024;
+        s->ent = realloc(s->ent, (size_t)s->cap * sizeof(entry_t));
+        s->hist = realloc(s->hist, (size_t)s->cap * s->dnum * sizeof(double));
+    }
+    entry_t *e = &s->ent[s->nent];
+    e->j = j; e->shift = code; e->flag = flag;
+    double *dst = &s->hist[(size_t)s->nent * s->dnum];
+    for (int d = 0; d < s->dnum; d++) dst[d] = h ? h[d] : 0.0;
+    s->nent++;
+}
+
+/* candidate (i, j-image) test and history re-seed, neigh_gran.cpp:570-612 */
+static void try_pair(or_sim *s, int i, int j, int code)
+{
+    double xj[3];
+    image_pos(s, j, code, xj);
+    const double delx = s->x[3 * i] - xj[0];
+    const double dely = s->x[3 * i + 1] - xj[1];
+    const double delz = s->x[3 * i + 2] - xj[2];
+    const double rsq = delx * delx + dely * dely + delz * delz;
+    const double radsum = (s->radius[i] + s->radius[j]) * s->p.cdf;
+    const double cutsq = (radsum + s->p.skin) * (radsum + s->p.skin);
+    if (!(rsq <= cutsq)) return;
+    if (rsq < radsum * radsum) {
+        int m;
+        const int np = s->npartner[i], ps = s->pstart[i];
+        for (m = 0; m < np; m++)
+            if (s->ptag[ps + m] == s->tag[j]) break;
+        if (m < np) {
+            push_entry(s, j, code, 1, &s->phist[(size_t)(ps + m) * s->dnum]);
+            return;
+        }
+    }
+    push_entry(s, j, code, 0, NULL);
+}
+
+static int cmp_pair(const void *a, const void *b)
+{
+    const int *x = (const int *)a, *y = (const int *)b;
+    if (x[0] != y[0]) return (x[0] > y[0]) - (x[0] < y[0]);
+    return (x[1] > y[1]) - (x[1] < y[1]);
+}
+
+static void build_list(or_sim *s)
+{
+    const int n = s->n;
+    double maxrad = 0.0;
+    for (int i = 0; i < n; i++) if (s->radius[i] > maxrad) maxrad = s->radius[i];
+    /* PairGran::init_one (pair_gran.cpp:591-603) + skin */
+    s->cutneighmax = (maxrad + maxrad) * s->p.cdf + s->p.skin;
+    s->nent = 0;
+
+    /* cell grid: periodic dims span the box, others the atoms' extent */
+    double lo[3], hi[3];
+    int nc[3];
+    for (int d = 0; d < 3; d++) {
+        if (s->p.periodic[d]) { lo[d] = s->p.boxlo[d]; hi[d] = s->p.boxhi[d]; }
+        else {
+            lo[d] = 1e300; hi[d] = -1e300;
+            for (int i = 0; i < n; i++) {
+                const double xi = s->x[3 * i + d];
+                if (xi < lo[d]) lo[d] = xi;
+                if (xi > hi[d]) hi[d] = xi;
+            }
+            if (n == 0) { lo[d] = 0; hi[d] = 1; }
+        }
+        nc[d] = (int)((hi[d] - lo[d]) / s->cutneighmax);
+        if (nc[d] < 1) nc[d] = 1;
+        if (!s->p.periodic[d] && nc[d] > 4096) nc[d] = 4096;
+    }
+    int brute = 0;
+    for (int d = 0; d < 3; d++)
+        if (s->p.periodic[d] && nc[d] < 3) brute = 1;
+
+    if (brute || n < 64) {
+        /* every image offset the periodic dims allow */
+        for (int i = 0; i < n; i++) {
+            s->nstart[i] = s->nent;
+            for (int j = 0; j < n; j++)
+                for (int code = 0; code < 27; code++) {
+                    int sh[3];
+                    shift_decode(code, &sh[0], &sh[1], &sh[2]);
+                    int ok = 1;
+                    for (int d = 0; d < 3; d++)
+                        if (sh[d] != 0 && !s->p.periodic[d]) ok = 0;
+                    if (!ok) continue;
+                    if (code == 13 && j <= i) continue;
+                    try_pair(s, i, j, code);
+                }
+            s->ncount[i] = s->nent - s->nstart[i];
+        }
+        s->have_list = 1;
+        return;
+    }
+
+    const long long ncell = (long long)nc[0] * nc[1] * nc[2];
+    int *cstart = calloc((size_t)ncell + 1, sizeof(int));
+    int *cell = malloc((size_t)n * sizeof(int));
+    int *sorted = malloc((size_t)n * sizeof(int));
+    for (int i = 0; i < n; i++) {
+        int c[3];
+        for (int d = 0; d < 3; d++) {
+            int ci = (int)floor((s->x[3 * i + d] - lo[d]) / (hi[d] - lo[d]) * nc[d]);
+            if (ci < 0) ci = 0;
+            if (ci >= nc[d]) ci = nc[d] - 1;
+            c[d] = ci;
+        }
+        cell[i] = (c[2] * nc[1] + c[1]) * nc[0] + c[0];
+        cstart[cell[i] + 1]++;
+    }
+    for (long long c = 0; c < ncell; c++) cstart[c + 1] += cstart[c];
+    {
+        int *fill = malloc((size_t)ncell * sizeof(int));
+        for (long long c = 0; c < ncell; c++) fill[c] = cstart[c];
+        for (int i = 0; i < n; i++) sorted[fill[cell[i]]++] = i; /* ascending index per cell */
+        free(fill);
+    }
+    int *cand = malloc(sizeof(int) * 2 * 4096);
+    int ccap = 4096;
+    for (int i = 0; i < n; i++) {
+        s->nstart[i] = s->nent;
+        const int ci = cell[i];
+        const int cx = ci % nc[0], cy = (ci / nc[0]) % nc[1], cz = ci / (nc[0] * nc[1]);
+        int ncand = 0;
+        for (int dz = -1; dz <= 1; dz++)
+            for (int dy = -1; dy <= 1; dy++)
+                for (int dx = -1; dx <= 1; dx++) {
+                    int c[3] = {cx + dx, cy + dy, cz + dz}, sh[3] = {0, 0, 0}, ok = 1;
+                    for (int d = 0; d < 3; d++) {
+                        if (c[d] < 0) {
+                            if (s->p.periodic[d]) { c[d] += nc[d]; sh[d] = -1; } else ok = 0;
+                        } else if (c[d] >= nc[d]) {
+                            if (s->p.periodic[d]) { c[d] -= nc[d]; sh[d] = 1; } else ok = 0;
+                        }
+                    }
+                    if (!ok) continue;
+                    const int code = (sh[0] + 1) + 3 * (sh[1] + 1) + 9 * (sh[2] + 1);
+                    const int cc = (c[2] * nc[1] + c[1]) * nc[0] + c[0];
+                    for (int q = cstart[cc]; q < cstart[cc + 1]; q++) {
+                        const int j = sorted[q];
+                        if (code == 13 && j <= i) continue;
+                        if (ncand == ccap) { ccap *= 2; cand = realloc(cand, sizeof(int) * 2 * ccap); }
+                        cand[2 * ncand] = j; cand[2 * ncand + 1] = code;
+                        ncand++;
+                    }
+                }
+        /* canonical order: by (j, shift) */
+        qsort(cand, ncand, 2 * sizeof(int), cmp_pair);
+        for (int q = 0; q < ncand; q++) try_pair(s, i, cand[2 * q], cand[2 * q + 1]);
+        s->ncount[i] = s->nent - s->nstart[i];
+    }
+    free(cand); free(cstart); free(cell); free(sorted);
+    s->have_list = 1;
+}
+
+/* PrimitiveWall::buildNeighList with treshold = skin
+ * (fix_wall_gran.cpp:688-706, primitive_wall.h:139-148,
+ *  primitive_wall_definitions.h:142-148) */
+static void build_wall_lists(or_sim *s)
+{
+    for (int k = 0; k < s->nplanes; k++) {
+        const int d = s->pdim[k];
+        for (int i = 0; i < s->n; i++) {
+            const double dMax = s->radius[i] + s->p.skin;
+            const double dist = s->x[3 * i + d] - s->ppos[k];
+            const double absdist = (dist > 0.0) ? dist : -dist;
+            s->pin[(size_t)i * OR_MAXPLANES + k] = (absdist <= dMax);
+        }
+    }
+}
+
+void or_rebuild(or_sim *s)
+{
+    pre_exchange(s);
+    domain_pbc(s);
+    build_list(s);
+    memcpy(s->xhold, s->x, (size_t)s->n * 3 * sizeof(double));
+    s->ago = 0;
+    s->wall_rebuild = 1;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Contact model chain: ContactModel::surfacesIntersect (contact_models.h:   */
+/* 228-238) = surface default -> normal hertz -> (cohesion off) ->           */
+/* tangential history -> rolling.                                            */
+
+typedef struct {
+    int is_wall;
+    int itype, jtype;
+    double radi, radj, radsum, r, rinv, rsq;
+    double delta[3], en[3];
+    double meff, mi, mj;
+    const double *v_i, *v_j, *omega_i, *omega_j;
+    double deltan;      /* wall: set by caller before the chain */
+    double vn, wr[3], vtr[3], cri, crj;
+    double Fn, kn, kt, gamman, gammat;
+    int update_history;
+    double *hist;       /* dnum doubles */
+} sidata_t;
+
+/* surface_model_default.h:146-208 */
+static void surface_default(sidata_t *sd)
+{
+    const double enx = sd->en[0], eny = sd->en[1], enz = sd->en[2];
+    const double vr1 = sd->v_i[0] - sd->v_j[0];
+    const double vr2 = sd->v_i[1] - sd->v_j[1];
+    const double vr3 = sd->v_i[2] - sd->v_j[2];
+    const double vn = vr1 * enx + vr2 * eny + vr3 * enz;
+    const double vn1 = vn * enx, vn2 = vn * eny, vn3 = vn * enz;
+    const double vt1 = vr1 - vn1, vt2 = vr2 - vn2, vt3 = vr3 - vn3;
+    const double deltan = sd->radsum - sd->r;
+    const double dx = sd->delta[0], dy = sd->delta[1], dz = sd->delta[2];
+    const double rinv = sd->rinv;
+    double wr1, wr2, wr3;
+    if (sd->is_wall) {
+        const double cr = sd->radi - 0.5 * sd->deltan;   /* caller's deltan */
+        wr1 = cr * sd->omega_i[0] * rinv;
+        wr2 = cr * sd->omega_i[1] * rinv;
+        wr3 = cr * sd->omega_i[2] * rinv;
+        sd->cri = cr;
+    } else {
+        const double cri = sd->radi - 0.5 * deltan;
+        const double crj = sd->radj - 0.5 * deltan;
+        wr1 = (cri * sd->omega_i[0] + crj * sd->omega_j[0]) * rinv;
+        wr2 = (cri * sd->omega_i[1] + crj * sd->omega_j[1]) * rinv;
+        wr3 = (cri * sd->omega_i[2] + crj * sd->omega_j[2]) * rinv;
+        sd->cri = cri;
+        sd->crj = crj;
+    }
+    sd->vtr[0] = vt1 - (dz * wr2 - dy * wr3);
+    sd->vtr[1] = vt2 - (dx * wr3 - dz * wr1);
+    sd->vtr[2] = vt3 - (dy * wr1 - dx * wr2);
+    sd->vn = vn;
+    sd->deltan = deltan;
+    sd->wr[0] = wr1; sd->wr[1] = wr2; sd->wr[2] = wr3;
+}
+
+/* normal_model_hertz.h:205-405 (no heating, energy or bond options) */
+static void normal_hertz(const or_sim *s, sidata_t *sd, double Fi[3], double Fj[3])
+{
+    const int t = sd->itype * T1 + sd->jtype;
+    const double reff = sd->is_wall ? sd->radi : (sd->radi * sd->radj / (sd->radi + sd->radj));
+    const double meff = sd->meff;
+    const double sqrtval = sqrt(reff * sd->deltan);
+    const double deltan = sd->deltan;
+    const double Sn = 2. * s->Yeff[t] * sqrtval;
+    const double St = 8. * s->Geff[t] * sqrtval;
+    double kn = 4. / 3. * s->Yeff[t] * sqrtval;
+    double kt = St;
+    const double gamman = -2. * SQRT_FIVE_OVER_SIX * s->betaeff[t] * sqrt(Sn * meff);
+    const double gammat = s->p.tangential_damping ? -2. * SQRT_FIVE_OVER_SIX * s->betaeff[t] * sqrt(St * meff) : 0.0;
+    /* kn /= nktv2p, kt /= nktv2p with nktv2p = 1 for units si */
+    const double Fn_damping = -gamman * sd->vn;
+    const double Fn_contact = kn * deltan;
+    double Fn = Fn_damping + Fn_contact;
+    if (s->p.limit_force && (Fn < 0.0)) Fn = 0.0;
+    sd->Fn = Fn; sd->kn = kn; sd->kt = kt; sd->gamman = gamman; sd->gammat = gammat;
+    if (sd->is_wall) {
+        const double Fn_ = Fn * 1.0;  /* area_ratio = 1 */
+        Fi[0] += Fn_ * sd->en[0];
+        Fi[1] += Fn_ * sd->en[1];
+        Fi[2] += Fn_ * sd->en[2];
+    } else {
+        Fi[0] += Fn * sd->en[0];
+        Fi[1] += Fn * sd->en[1];
+        Fi[2] += Fn * sd->en[2];
+        Fj[0] += -Fi[0];
+        Fj[1] += -Fi[1];
+        Fj[2] += -Fi[2];
+    }
+}
+
+/* tangential_model_history.h:135-425 (torsion off, no energy terms) */
+static void tangential_history(const or_sim *s, sidata_t *sd, double Fi[3], double Ti[3],
+                               double Fj[3], double Tj[3])
+{
+    const double enx = sd->en[0], eny = sd->en[1], enz = sd->en[2];
+    double *shear = sd->hist;
+    if (sd->update_history) {
+        const double dt = s->p.dt;
+        shear[0] += sd->vtr[0] * dt;
+        shear[1] += sd->vtr[1] * dt;
+        shear[2] += sd->vtr[2] * dt;
+        const double rsht = shear[0] * enx + shear[1] * eny + shear[2] * enz;
+        shear[0] -= rsht * enx;
+        shear[1] -= rsht * eny;
+        shear[2] -= rsht * enz;
+    }
+    const double shrmag = sqrt(shear[0] * shear[0] + shear[1] * shear[1] + shear[2] * shear[2]);
+    const double kt = sd->kt;
+    const double xmu = s->mu[sd->itype * T1 + sd->jtype];
+    double Ft1 = -(kt * shear[0]);
+    double Ft2 = -(kt * shear[1]);
+    double Ft3 = -(kt * shear[2]);
+    const double Ft_shear = kt * shrmag;
+    const double Ft_friction = xmu * fabs(sd->Fn);
+    if (Ft_shear > Ft_friction) {
+        if (shrmag != 0.0) {
+            const double ratio = Ft_friction / Ft_shear;
+            Ft1 *= ratio; Ft2 *= ratio; Ft3 *= ratio;
+            if (sd->update_history) {
+                shear[0] = -Ft1 / kt;
+                shear[1] = -Ft2 / kt;
+                shear[2] = -Ft3 / kt;
+            }
+        } else
+            Ft1 = Ft2 = Ft3 = 0.0;
+    } else {
+        const double gammat = sd->gammat;
+        Ft1 -= (gammat * sd->vtr[0]);
+        Ft2 -= (gammat * sd->vtr[1]);
+        Ft3 -= (gammat * sd->vtr[2]);
+    }
+    const double tor1 = eny * Ft3 - enz * Ft2;
+    const double tor2 = enz * Ft1 - enx * Ft3;
+    const double tor3 = enx * Ft2 - eny * Ft1;
+    /* Tn_shear = 0 without torsion: x + 0.0*en == x */
+    const double ti0 = -sd->cri * tor1, ti1 = -sd->cri * tor2, ti2 = -sd->cri * tor3;
+    if (sd->is_wall) {
+        Fi[0] += Ft1 * 1.0; Fi[1] += Ft2 * 1.0; Fi[2] += Ft3 * 1.0;
+        Ti[0] += ti0 * 1.0; Ti[1] += ti1 * 1.0; Ti[2] += ti2 * 1.0;
+    } else {
+        const double tj0 = -sd->crj * tor1, tj1 = -sd->crj * tor2, tj2 = -sd->crj * tor3;
+        Fi[0] += Ft1; Fi[1] += Ft2; Fi[2] += Ft3;
+        Fj[0] += -Ft1; Fj[1] += -Ft2; Fj[2] += -Ft3;
+        Ti[0] += ti0; Ti[1] += ti1; Ti[2] += ti2;
+        Tj[0] += tj0; Tj[1] += tj1; Tj[2] += tj2;
+    }
+}
+
+/* rolling_model_epsd.h:230-330, rolling_model_epsd2.h:161-230,
+ * rolling_model_epsd3.h:178-262 (calcRollTorque) */
+static void roll_torque(const or_sim *s, const sidata_t *sd, double reff,
+                        double wr1, double wr2, double wr3, double r_inertia, double rt[3])
+{
+    const int t = sd->itype * T1 + sd->jtype;
+    const double enx = sd->en[0], eny = sd->en[1], enz = sd->en[2];
+    const double dt = s->p.dt;
+    double *c_history = sd->hist + 3;
+    const double rmu = s->mur[t];
+    double wr_tot[3];
+    if (s->p.torsion_torque) {
+        wr_tot[0] = wr1; wr_tot[1] = wr2; wr_tot[2] = wr3;
+    } else {
+        const double wr_dot_delta = wr1 * enx + wr2 * eny + wr3 * enz;
+        const double wn0 = enx * wr_dot_delta, wn1 = eny * wr_dot_delta, wn2 = enz * wr_dot_delta;
+        wr_tot[0] = wr1 - wn0; wr_tot[1] = wr2 - wn1; wr_tot[2] = wr3 - wn2;
+    }
+    double kr;
+    if (s->p.rolling_model == OR_ROLL_EPSD) kr = 2.25 * sd->kn * rmu * rmu * reff * reff;
+    else if (s->p.rolling_model == OR_ROLL_EPSD2) kr = sd->kt * reff * reff;
+    else kr = s->krs * sd->kn * rmu * rmu * reff * reff;
+    const double dtkr = dt * kr;
+    const double dr0 = dtkr * wr_tot[0], dr1 = dtkr * wr_tot[1], dr2 = dtkr * wr_tot[2];
+    rt[0] = c_history[0] + dr0;
+    rt[1] = c_history[1] + dr1;
+    rt[2] = c_history[2] + dr2;
+    const double mag = sqrt(rt[0] * rt[0] + rt[1] * rt[1] + rt[2] * rt[2]);
+    const double rmax = fabs(sd->Fn) * reff * rmu;
+    if (s->p.rolling_model == OR_ROLL_EPSD2) {
+        if (mag > rmax) {
+            const double factor = rmax / mag;
+            rt[0] *= factor; rt[1] *= factor; rt[2] *= factor;
+        }
+        if (sd->update_history) {
+            c_history[0] = rt[0]; c_history[1] = rt[1]; c_history[2] = rt[2];
+        }
+        return;
+    }
+    if (mag > rmax) {
+        const double factor = rmax / mag;
+        rt[0] *= factor; rt[1] *= factor; rt[2] *= factor;
+        if (sd->update_history) {
+            c_history[0] = rt[0]; c_history[1] = rt[1]; c_history[2] = rt[2];
+        }
+    } else {
+        if (sd->update_history) {
+            c_history[0] = rt[0]; c_history[1] = rt[1]; c_history[2] = rt[2];
+        }
+        const double r_coef = s->etar[t] * 2 * sqrt(r_inertia * kr);
+        rt[0] += r_coef * wr_tot[0];
+        rt[1] += r_coef * wr_tot[1];
+        rt[2] += r_coef * wr_tot[2];
+    }
+}
+
+/* RollingModel<EPSD*>::surfacesIntersect (rolling_model_epsd.h:96-220 etc.) */
+static void rolling(const or_sim *s, sidata_t *sd, double Ti[3], double Tj[3])
+{
+    double rt[3] = {0, 0, 0};
+    const double radi = sd->radi, radj = sd->radj;
+    const double reff = sd->is_wall ? radi : (radi * radj / (radi + radj));
+    if (sd->is_wall) {
+        /* epsd: 1.4*mi*reff*reff; epsd3: 1.4*mi*radius*radius, radius = radi */
+        const double r_inertia = 1.4 * sd->mi * reff * reff;
+        roll_torque(s, sd, reff, sd->wr[0], sd->wr[1], sd->wr[2], r_inertia, rt);
+    } else {
+        const double w0 = sd->omega_i[0] - sd->omega_j[0];
+        const double w1 = sd->omega_i[1] - sd->omega_j[1];
+        const double w2 = sd->omega_i[2] - sd->omega_j[2];
+        const double ri = sd->mi * radi * radi;
+        const double rj = sd->mj * radj * radj;
+        const double r_inertia = 1.4 * ri * rj / (ri + rj);
+        roll_torque(s, sd, reff, w0, w1, w2, r_inertia, rt);
+    }
+    Ti[0] -= rt[0]; Ti[1] -= rt[1]; Ti[2] -= rt[2];
+    Tj[0] += rt[0]; Tj[1] += rt[1]; Tj[2] += rt[2];
+}
+
+static void chain_intersect(const or_sim *s, sidata_t *sd, double Fi[3], double Ti[3],
+                            double Fj[3], double Tj[3])
+{
+    surface_default(sd);
+    normal_hertz(s, sd, Fi, Fj);
+    tangential_history(s, sd, Fi, Ti, Fj, Tj);
+    if (s->p.rolling_model != OR_ROLL_OFF) rolling(s, sd, Ti, Tj);
+}
+
+/* surfacesClose chain: tangential zeroes shear (tangential_model_history.h:
+ * 428-440), rolling zeroes its torque history (rolling_model_epsd.h:222-229) */
+static void chain_close(const or_sim *s, double *hist, int *flag)
+{
+    (void)flag;
+    hist[0] = hist[1] = hist[2] = 0.0;
+    if (s->p.rolling_model != OR_ROLL_OFF) hist[3] = hist[4] = hist[5] = 0.0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Granular<>::compute_force (pair_gran_base.h:187-503)                      */
+
+static void pair_compute(or_sim *s, int shearupdate)
+{
+    const double cdm = s->p.cdf * s->p.cdf;
+    for (int i = 0; i < s->n; i++) {
+        const double xtmp = s->x[3 * i], ytmp = s->x[3 * i + 1], ztmp = s->x[3 * i + 2];
+        const double radi = s->radius[i];
+        for (int k = 0; k < s->ncount[i]; k++) {
+            const int idx = s->nstart[i] + k;
+            entry_t *e = &s->ent[idx];
+            const int j = e->j;
+            double xj[3];
+            image_pos(s, j, e->shift, xj);
+            const double delx = xtmp - xj[0];
+            const double dely = ytmp - xj[1];
+            const double delz = ztmp - xj[2];
+            const double rsq = delx * delx + dely * dely + delz * delz;
+            const double radj = s->radius[j];
+            const double radsum = radi + radj;
+            double *h = &s->hist[(size_t)idx * s->dnum];
+            double Fi[3] = {0, 0, 0}, Ti[3] = {0, 0, 0}, Fj[3] = {0, 0, 0}, Tj[3] = {0, 0, 0};
+            if (rsq < radsum * radsum) {
+                const double r = sqrt(rsq);
+                const double rinv = 1.0 / r;
+                sidata_t sd;
+                memset(&sd, 0, sizeof(sd));
+                sd.is_wall = 0;
+                sd.itype = s->type[i]; sd.jtype = s->type[j];
+                sd.radi = radi; sd.radj = radj; sd.radsum = radsum;
+                sd.r = r; sd.rinv = rinv; sd.rsq = rsq;
+                sd.delta[0] = delx; sd.delta[1] = dely; sd.delta[2] = delz;
+                sd.en[0] = delx * rinv; sd.en[1] = dely * rinv; sd.en[2] = delz * rinv;
+                const double mi = s->rmass[i], mj = s->rmass[j];
+                double meff = mi * mj / (mi + mj);
+                /* freeze_group_bit is nonzero only with a fix freeze
+                 * (pair_gran.cpp:532-537) */
+                if (s->has_freeze && s->freeze[i]) meff = mj;
+                if (s->has_freeze && s->freeze[j]) meff = mi;
+                sd.meff = meff; sd.mi = mi; sd.mj = mj;
+                sd.v_i = &s->v[3 * i]; sd.v_j = &s->v[3 * j];
+                sd.omega_i = &s->omega[3 * i]; sd.omega_j = &s->omega[3 * j];
+                sd.update_history = shearupdate;
+                sd.hist = h;
+                e->flag = 1;
+                chain_intersect(s, &sd, Fi, Ti, Fj, Tj);
+                for (int d = 0; d < 3; d++) {
+                    s->f[3 * i + d] += Fi[d];
+                    s->torque[3 * i + d] += Ti[d];
+                }
+                s->fscale[i] += sqrt(Fi[0] * Fi[0] + Fi[1] * Fi[1] + Fi[2] * Fi[2]);
+                s->tscale[i] += sqrt(Ti[0] * Ti[0] + Ti[1] * Ti[1] + Ti[2] * Ti[2]);
+                if (e->shift == 13) {
+                    for (int d = 0; d < 3; d++) {
+                        s->f[3 * j + d] += Fj[d];
+                        s->torque[3 * j + d] += Tj[d];
+                    }
+                    s->fscale[j] += sqrt(Fj[0] * Fj[0] + Fj[1] * Fj[1] + Fj[2] * Fj[2]);
+                    s->tscale[j] += sqrt(Tj[0] * Tj[0] + Tj[1] * Tj[1] + Tj[2] * Tj[2]);
+                }
+            } else if (rsq < cdm * radsum * radsum) {
+                chain_close(s, h, &e->flag);
+            }
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* FixWallGran::post_force_primitive (fix_wall_gran.cpp:988-1120) with       */
+/* Walls::Granular<>::compute_force (fix_wall_gran_base.h:159-378) and       */
+/* Plane::resolveContact (primitive_wall_definitions.h:130-141).             */
+
+static void wall_plane(or_sim *s, int k, int shearupdate)
+{
+    const int d = s->pdim[k];
+    const double p = s->ppos[k];
+    const double cdm = s->p.cdf - 1.0;
+    static const double vwall[3] = {0.0, 0.0, 0.0};
+    for (int i = 0; i < s->n; i++) {
+        if (!s->pin[(size_t)i * OR_MAXPLANES + k]) continue;
+        double *h = &s->phist_w[((size_t)i * OR_MAXPLANES + k) * s->dnum_w];
+        const double radi = s->radius[i];
+        const double pos = s->x[3 * i + d];
+        double delta[3] = {0, 0, 0}, deltan;
+        if (pos > p) {
+            delta[d] = p - pos;
+            deltan = pos - p - radi;
+        } else {
+            delta[d] = p - pos;
+            deltan = p - pos - radi;
+        }
+        if (deltan > s->cutneighmax) continue;
+        if (deltan <= 0 || deltan < cdm * radi) {
+            const int intersect = (deltan <= 0);
+            sidata_t sd;
+            memset(&sd, 0, sizeof(sd));
+            sd.is_wall = 1;
+            sd.radi = radi;
+            sd.deltan = -deltan;
+            sd.delta[0] = -delta[0]; sd.delta[1] = -delta[1]; sd.delta[2] = -delta[2];
+            sd.v_i = &s->v[3 * i]; sd.v_j = vwall;
+            sd.omega_i = &s->omega[3 * i]; sd.omega_j = vwall;
+            sd.r = sd.radi - sd.deltan;
+            sd.rsq = sd.r * sd.r;
+            const double rinv = 1.0 / sd.r;
+            sd.rinv = rinv;
+            sd.itype = s->type[i];
+            sd.jtype = s->ptype[k];
+            sd.meff = s->rmass[i];
+            sd.mi = s->rmass[i];
+            sd.en[0] = sd.delta[0] * rinv;
+            sd.en[1] = sd.delta[1] * rinv;
+            sd.en[2] = sd.delta[2] * rinv;
+            sd.radsum = sd.radi;
+            sd.update_history = shearupdate;
+            sd.hist = h;
+            double Fi[3] = {0, 0, 0}, Ti[3] = {0, 0, 0}, Fj[3] = {0, 0, 0}, Tj[3] = {0, 0, 0};
+            if (intersect) {
+                chain_intersect(s, &sd, Fi, Ti, Fj, Tj);
+                for (int q = 0; q < 3; q++) {
+                    s->f[3 * i + q] += Fi[q];
+                    s->torque[3 * i + q] += Ti[q];
+                }
+                s->fscale[i] += sqrt(Fi[0] * Fi[0] + Fi[1] * Fi[1] + Fi[2] * Fi[2]);
+                s->tscale[i] += sqrt(Ti[0] * Ti[0] + Ti[1] * Ti[1] + Ti[2] * Ti[2]);
+            } else {
+                chain_close(s, h, NULL);
+            }
+        } else {
+            for (int q = 0; q < s->dnum_w; q++) h[q] = 0.0;
+        }
+    }
+}
+
+/* FixGravity::post_force (fix_gravity.cpp:301-352), rmass branch */
+static void gravity(or_sim *s)
+{
+    for (int i = 0; i < s->n; i++) {
+        const double massone = s->rmass[i];
+        s->f[3 * i] += massone * s->gacc[0];
+        s->f[3 * i + 1] += massone * s->gacc[1];
+        s->f[3 * i + 2] += massone * s->gacc[2];
+        s->fscale[i] += fabs(massone * s->gacc[0]) + fabs(massone * s->gacc[1]) + fabs(massone * s->gacc[2]);
+    }
+}
+
+static void freeze(or_sim *s)
+{
+    for (int i = 0; i < s->n; i++)
+        if (s->freeze[i])
+            for (int d = 0; d < 3; d++) s->f[3 * i + d] = s->torque[3 * i + d] = 0.0;
+}
+
+void or_compute_forces(or_sim *s, int shearupdate)
+{
+    /* Verlet::force_clear (verlet.cpp:406-430) */
+    memset(s->f, 0, (size_t)s->n * 3 * sizeof(double));
+    memset(s->torque, 0, (size_t)s->n * 3 * sizeof(double));
+    memset(s->fscale, 0, (size_t)s->n * sizeof(double));
+    memset(s->tscale, 0, (size_t)s->n * sizeof(double));
+    /* FixWallGran::pre_force (fix_wall_gran.cpp:688-706) */
+    if (s->wall_rebuild) build_wall_lists(s);
+    s->wall_rebuild = 0;
+    pair_compute(s, shearupdate);
+    for (int q = 0; q < s->nops; q++) {
+        if (s->ops[q] == OP_WALL) wall_plane(s, s->opsarg[q], shearupdate);
+        else if (s->ops[q] == OP_GRAVITY) gravity(s);
+        else if (s->ops[q] == OP_FREEZE) freeze(s);
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* FixNVESphere (fix_nve_sphere.cpp:134-243), CAddRhoFluid = 0, 3d           */
+
+static void initial_integrate(or_sim *s)
+{
+    const double dtf = 0.5 * s->p.dt;   /* dtf = 0.5*dt*ftm2v, ftm2v = 1 */
+    const double dtv = s->p.dt;
+    const double dtfrotate = dtf / 0.4;
+    for (int i = 0; i < s->n; i++) {
+        const double dtfm = dtf / (s->rmass[i] * (1. + 0.0 / 1.0));
+        double *v = &s->v[3 * i], *x = &s->x[3 * i], *f = &s->f[3 * i];
+        v[0] += dtfm * f[0]; v[1] += dtfm * f[1]; v[2] += dtfm * f[2];
+        x[0] += dtv * v[0]; x[1] += dtv * v[1]; x[2] += dtv * v[2];
+        const double dtirotate = dtfrotate / (s->radius[i] * s->radius[i] * s->rmass[i]);
+        double *w = &s->omega[3 * i], *t = &s->torque[3 * i];
+        w[0] += dtirotate * t[0]; w[1] += dtirotate * t[1]; w[2] += dtirotate * t[2];
+    }
+}
+
+static void final_integrate(or_sim *s)
+{
+    const double dtf = 0.5 * s->p.dt;
+    const double dtfrotate = dtf / 0.4;
+    for (int i = 0; i < s->n; i++) {
+        const double dtfm = dtf / (s->rmass[i] * (1. + 0.0 / 1.0));
+        double *v = &s->v[3 * i], *f = &s->f[3 * i];
+        v[0] += dtfm * f[0]; v[1] += dtfm * f[1]; v[2] += dtfm * f[2];
+        const double dtirotate = dtfrotate / (s->radius[i] * s->radius[i] * s->rmass[i]);
+        double *w = &s->omega[3 * i], *t = &s->torque[3 * i];
+        w[0] += dtirotate * t[0]; w[1] += dtirotate * t[1]; w[2] += dtirotate * t[2];
+    }
+}
+
+/* Neighbor::decide / check_distance (neighbor.cpp:1362-1465), no box change */
+static int decide(or_sim *s)
+{
+    s->ago++;
+    if (s->ago >= s->p.neigh_delay && s->ago % s->p.neigh_every == 0) {
+        if (!s->p.neigh_check) return 1;
+        const double trig = 0.5 * s->p.skin;
+        const double deltasq = trig * trig;
+        int flag = 0;
+        for (int i = 0; i < s->n; i++) {
+            const double dx = s->x[3 * i] - s->xhold[3 * i];
+            const double dy = s->x[3 * i + 1] - s->xhold[3 * i + 1];
+            const double dz = s->x[3 * i + 2] - s->xhold[3 * i + 2];
+            const double rsq = dx * dx + dy * dy + dz * dz;
+            if (rsq > deltasq) flag = 1;
+        }
+        return flag;
+    }
+    return 0;
+}
+
+void or_setup(or_sim *s)
+{
+    s->setupflag = 1;
+    or_rebuild(s);
+    or_compute_forces(s, 0);
+    s->setupflag = 0;
+}
+
+void or_run(or_sim *s, int nsteps)
+{
+    for (int st = 0; st < nsteps; st++) {
+        s->ntimestep++;
+        initial_integrate(s);
+        if (decide(s)) or_rebuild(s);
+        or_compute_forces(s, 1);
+        final_integrate(s);
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+
+void or_get_state(const or_sim *s, double *x, double *v, double *omega, double *f, double *torque)
+{
+    const size_t n3 = (size_t)s->n * 3 * sizeof(double);
+    if (x) memcpy(x, s->x, n3);
+    if (v) memcpy(v, s->v, n3);
+    if (omega) memcpy(omega, s->omega, n3);
+    if (f) memcpy(f, s->f, n3);
+    if (torque) memcpy(torque, s->torque, n3);
+}
+
+long long or_timestep(const or_sim *s) { return s->ntimestep; }
+int or_dnum(const or_sim *s) { return s->dnum; }
+int or_dnum_wall(const or_sim *s) { return s->dnum_w; }
+
+int or_get_pairs(const or_sim *s, int *tag_i, int *tag_j, int *shift, double *hist, int *flag)
+{
+    int c = 0;
+    const int dnum = s->dnum;
+    for (int i = 0; i < s->n; i++)
+        for (int k = 0; k < s->ncount[i]; k++) {
+            const int idx = s->nstart[i] + k;
+            const entry_t *e = &s->ent[idx];
+            const double *h = &s->hist[(size_t)idx * dnum];
+            if (tag_i) {
+                tag_i[c] = s->tag[i]; tag_j[c] = s->tag[e->j]; shift[c] = e->shift;
+                if (hist) for (int d = 0; d < dnum; d++) hist[(size_t)c * dnum + d] = h[d];
+                if (flag) flag[c] = e->flag;
+            }
+            c++;
+            if (e->shift == 13) {
+                if (tag_i) {
+                    tag_i[c] = s->tag[e->j]; tag_j[c] = s->tag[i]; shift[c] = 13;
+                    if (hist) for (int d = 0; d < dnum; d++) hist[(size_t)c * dnum + d] = -h[d];
+                    if (flag) flag[c] = e->flag;
+                }
+                c++;
+            }
+        }
+    return c;
+}
+
+void or_get_wall_history(const or_sim *s, double *hist)
+{
+    for (int i = 0; i < s->n; i++)
+        for (int k = 0; k < s->nplanes; k++)
+            for (int d = 0; d < s->dnum_w; d++)
+                hist[((size_t)i * s->nplanes + k) * s->dnum_w + d] =
+                    s->phist_w[((size_t)i * OR_MAXPLANES + k) * s->dnum_w + d];
+}
+
+void or_get_force_scale(const or_sim *s, double *fscale, double *tscale)
+{
+    memcpy(fscale, s->fscale, (size_t)s->n * sizeof(double));
+    memcpy(tscale, s->tscale, (size_t)s->n * sizeof(double));
+}

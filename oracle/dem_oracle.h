@@ -1,0 +1,109 @@
+/*
+ * dem_oracle.h -- CPU restatement of the LIGGGHTS-INL sphere DEM timestep.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP engine:
+ * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+ * load it.  The product path never links or calls it.
+ *
+ * Everything is IEEE FP64 and single threaded, in the operation order of the
+ * reference C++ sources cited per function in dem_oracle.c.  Compiled with
+ * -O2 -ffp-contract=off so no fused multiply-add changes a rounding.
+ *
+ * Indexing: atoms are kept in the order they were given (the reference sorts
+ * atoms spatially every 1000 steps, atom.cpp:98; that only changes summation
+ * order, so results are reported per tag).  Material tables are
+ * (ntypes+1)x(ntypes+1) row-major with row/column 0 unused, like the
+ * reference's double** matrices indexed [itype][jtype].
+ */
+#ifndef DEM_ORACLE_H
+#define DEM_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OR_MAXTYPES 8
+#define OR_MAXPLANES 8
+
+/* rolling_friction keyword of pair_style gran */
+enum { OR_ROLL_OFF = 0, OR_ROLL_EPSD = 1, OR_ROLL_EPSD2 = 2, OR_ROLL_EPSD3 = 3 };
+
+typedef struct {
+    double dt;
+    double boxlo[3], boxhi[3];
+    int periodic[3];
+    double skin;                 /* neighbor <skin> bin                      */
+    double cdf;                  /* neighbor->contactDistanceFactor (1.0)    */
+    int neigh_every, neigh_delay, neigh_check;   /* neigh_modify            */
+    int rolling_model;           /* OR_ROLL_*                               */
+    int tangential_damping;      /* hertz setting, reference default on     */
+    int limit_force;             /* hertz setting, reference default off    */
+    int torsion_torque;          /* rolling setting, default off            */
+} or_params;
+
+typedef struct or_sim or_sim;
+
+or_sim *or_create(const or_params *p);
+void or_destroy(or_sim *s);
+
+/* Effective material tables exactly as global_properties.cpp builds them.
+ * Y, nu: per type (index 1..ntypes, length ntypes+1).  e, mu, mur, etar:
+ * per type pair, (ntypes+1)^2 row-major.  krs: coefficientRollingStiffness. */
+int or_set_materials(or_sim *s, int ntypes, const double *Y, const double *nu,
+                     const double *e, const double *mu, const double *mur,
+                     const double *etar, double krs);
+/* Read back the derived tables (Yeff, Geff, betaeff), (ntypes+1)^2 each. */
+void or_get_material_tables(const or_sim *s, double *Yeff, double *Geff, double *betaeff);
+
+/* atoms: x,v,omega are n*3 row-major; freeze marks members of fix freeze. */
+int or_set_atoms(or_sim *s, int n, const double *x, const double *v, const double *omega,
+                 const double *radius, const double *rmass, const int *type,
+                 const int *tag, const int *freeze);
+
+/* post_force fixes run in the order they are added, like Modify's fix list
+ * (modify.cpp:1673-1750).
+ * fix wall/gran ... primitive type <wtype> {x,y,z}plane <pos> */
+int or_add_plane(or_sim *s, int dim, double pos, int wtype);
+/* fix gravity: acceleration vector xacc,yacc,zacc (fix_gravity.cpp:395-397) */
+void or_add_gravity(or_sim *s, double ax, double ay, double az);
+/* fix freeze on the atoms flagged in or_set_atoms (fix_freeze.cpp:121-144) */
+void or_add_freeze(or_sim *s);
+
+/* Verlet::setup (verlet.cpp:134-199): pbc, build list, force pass with
+ * shearupdate = 0. */
+void or_setup(or_sim *s);
+/* Verlet::run (verlet.cpp:264-390) for nsteps. */
+void or_run(or_sim *s, int nsteps);
+/* One force evaluation on the current list (force_clear + pair + walls +
+ * gravity), shearupdate as given.  No list rebuild. */
+void or_compute_forces(or_sim *s, int shearupdate);
+/* Force a neighbour rebuild now (pre_exchange + pbc + build + wall lists). */
+void or_rebuild(or_sim *s);
+
+void or_get_state(const or_sim *s, double *x, double *v, double *omega,
+                  double *f, double *torque);
+long long or_timestep(const or_sim *s);
+int or_dnum(const or_sim *s);
+int or_dnum_wall(const or_sim *s);
+
+/* Directed pair set of the current list: every half-list entry (i,j) with j
+ * an owned atom yields (tag_i,tag_j,13) and (tag_j,tag_i,13); an entry with a
+ * periodic image of j yields (tag_i,tag_j,shift).  shift packs the image
+ * offsets (sx,sy,sz) in {-1,0,1} as (sx+1) + 3(sy+1) + 9(sz+1) (13 = none).
+ * hist gets dnum doubles per directed entry (the reversed entry of an owned
+ * pair carries the negated newton slots), flag the contact flag.
+ * Returns the count; pass NULL arrays to query the count. */
+int or_get_pairs(const or_sim *s, int *tag_i, int *tag_j, int *shift,
+                 double *hist, int *flag);
+
+/* Per-atom primitive-wall history, n * nplanes * dnum_wall. */
+void or_get_wall_history(const or_sim *s, double *hist);
+
+/* Per-atom per-contact contributions of the last force pass, for scaling
+ * tolerances: sum over contributions of |dF| and |dT| per atom (n each). */
+void or_get_force_scale(const or_sim *s, double *fscale, double *tscale);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,172 @@
+"""HIP engine vs the CPU oracle on the same seeded inputs (MI355X only).
+
+Bars:
+* pair sets (directed full list with periodic image codes): bit-exact,
+  also against an O(N^2) brute force;
+* FP64 forces and torques from an identical state: |gpu - oracle| <=
+  1e-12 x the atom's force scale (sum of |contact contributions| and
+  |gravity|), since only the order of summation differs;
+* contact history per directed pair after a pass: <= 1e-12 relative;
+* short horizon (20-40 steps, two rebuilds): positions within 1e-12 of the
+  box size, forces within 1e-9 of the force scale (chaos amplifies the
+  summation-order rounding step by step);
+* mixed precision vs FP64 oracle: forces within 1e-5 of the force scale;
+* two runs of the same input give bit-identical results (no atomics).
+"""
+import numpy as np
+import pytest
+
+from conftest import load_engine
+from oracle import Oracle
+from test_oracle_pins import brute_pairs
+
+pytestmark = pytest.mark.gpu
+
+
+def W():
+    return load_engine().workloads
+
+
+def pour_state(steps=3000):
+    """config-1 pour advanced on the oracle until contacts exist."""
+    spec, atoms = W().pour_box()
+    o = Oracle(spec, atoms)
+    o.setup()
+    o.run(steps)
+    return spec, atoms, o
+
+
+def column_state(n=20000, steps=40):
+    spec, atoms = W().packed_column(n=n, slab_n=5000)
+    o = Oracle(spec, atoms)
+    o.setup()
+    o.run(steps)
+    return spec, atoms, o
+
+
+def snapshot_atoms(atoms, o):
+    st = o.state()
+    a = dict(atoms)
+    a["x"], a["v"], a["omega"] = st["x"], st["v"], st["omega"]
+    return a
+
+
+def sim_from_oracle(spec, atoms, o, precision="fp64"):
+    """GPU simulation holding exactly the oracle's state and history; both
+    then rebuild and take a force pass with shearupdate = 0."""
+    eng = load_engine()
+    a = snapshot_atoms(atoms, o)
+    s = eng.Simulation(spec, a, precision=precision)
+    ti, tj, sh, hi, fl = o.pairs()
+    k = fl != 0
+    s.set_history(ti[k], tj[k], hi[k])
+    s.set_wall_history(o.wall_history())
+    s.setup()
+    o.rebuild()
+    o.compute_forces(0)
+    return s
+
+
+def pair_dict(ti, tj, sh, hi):
+    return {(a, b, c): h for a, b, c, h in zip(ti.tolist(), tj.tolist(), sh.tolist(), hi)}
+
+
+def check_forces(s, o, tol):
+    g = s.state()
+    r = o.state()
+    fs, ts = o.force_scale()
+    # a contact torque is bounded by its force times the lever arm (< radius)
+    ts = ts + fs * r["radius"] if "radius" in r else ts
+    fs = np.maximum(fs, 1e-300)
+    ts = np.maximum(ts, 1e-300)
+    ef = np.abs(g["f"] - r["f"]).max(1) / fs
+    et = np.abs(g["torque"] - r["torque"]).max(1) / ts
+    assert ef.max() <= tol, ef.max()
+    assert et.max() <= tol, et.max()
+    return ef.max(), et.max()
+
+
+def check_pairs(s, o, hist_tol=1e-12):
+    gi, gj, gs, gh, gf = s.pairs()
+    oi, oj, osh, oh, of = o.pairs()
+    G = pair_dict(gi, gj, gs, gh)
+    O = pair_dict(oi, oj, osh, oh)
+    assert len(G) == len(gi)
+    assert set(G) == set(O)
+    scale = max(np.abs(oh).max(), 1e-300)
+    err = max(np.abs(G[k] - O[k]).max() for k in O) if O else 0.0
+    assert err <= hist_tol * scale, err / scale
+
+
+@pytest.mark.parametrize("case", ["pour", "column"])
+def test_forces_identical_state(case):
+    spec, atoms, o = pour_state() if case == "pour" else column_state()
+    s = sim_from_oracle(spec, atoms, o)
+    check_pairs(s, o)
+    check_forces(s, o, 1e-12)
+    fs, _ = o.force_scale()
+    assert (fs > 0).sum() > 100
+
+
+def test_pair_set_brute_force_periodic():
+    eng = load_engine()
+    rng = np.random.default_rng(11)
+    n = 600
+    lo, hi = [0, 0, 0], [0.02, 0.02, 0.02]
+    x = rng.uniform(0, 0.02, size=(n, 3))
+    r = rng.uniform(0.4e-3, 0.9e-3, size=n)
+    spec = dict(box_lo=lo, box_hi=hi, periodic=[1, 1, 1], dt=1e-7, skin=3e-4,
+                materials=W()._materials())
+    atoms = dict(x=x, v=np.zeros((n, 3)), omega=np.zeros((n, 3)), radius=r,
+                 rmass=4 / 3 * np.pi * r ** 3 * 2500, type=np.ones(n, int), tag=np.arange(1, n + 1))
+    s = eng.Simulation(spec, atoms)
+    s.setup()
+    gi, gj, gs, _, _ = s.pairs()
+    got = set(zip(gi.tolist(), gj.tolist(), gs.tolist()))
+    assert len(got) == len(gi)
+    assert got == brute_pairs(x, r, lo, hi, [1, 1, 1], 3e-4)
+
+
+@pytest.mark.parametrize("case", ["pour", "column"])
+def test_short_horizon_trajectory(case):
+    spec, atoms, o = pour_state() if case == "pour" else column_state()
+    s = sim_from_oracle(spec, atoms, o)
+    nsteps = 25
+    s.step(nsteps)
+    o.run(nsteps)
+    g, r = s.state(), o.state()
+    L = np.max(np.asarray(spec["box_hi"]) - np.asarray(spec["box_lo"]))
+    assert np.abs(g["x"] - r["x"]).max() <= 1e-12 * L
+    vs = np.abs(r["v"]).max()
+    assert np.abs(g["v"] - r["v"]).max() <= 1e-9 * vs
+    check_forces(s, o, 1e-9)
+    check_pairs(s, o, hist_tol=1e-9)
+
+
+def test_mixed_precision_force_error():
+    spec, atoms, o = column_state()
+    s = sim_from_oracle(spec, atoms, o, precision="mixed")
+    ef, et = check_forces(s, o, 1e-5)
+
+
+def test_deterministic_bitwise():
+    spec, atoms = W().packed_column(n=20000, slab_n=5000)
+    eng = load_engine()
+    out = []
+    for _ in range(2):
+        s = eng.Simulation(spec, atoms)
+        s.setup()
+        s.step(30)
+        out.append(s.state())
+        s.close()
+    for k in ("x", "v", "omega", "f", "torque"):
+        assert np.array_equal(out[0][k], out[1][k])
+
+
+def test_neighbor_overflow_error():
+    eng = load_engine()
+    spec, atoms = W().packed_column(n=5000, slab_n=5000)
+    spec = dict(spec, neigh_one=3)
+    s = eng.Simulation(spec, atoms)
+    with pytest.raises(eng.DemError, match="Neighbor list overflow"):
+        s.setup()
