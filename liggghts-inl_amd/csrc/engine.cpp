@@ -30,6 +30,7 @@ struct EventPair {
 
 struct Timer {
     std::vector<EventPair> ev;
+    std::vector<EventPair> pool;
     double ms = 0.0;
     int64_t launches = 0;
 };
@@ -128,8 +129,13 @@ static void timer_begin(dem_ctx *ctx, Timer &t)
 {
     if (!ctx->timing) return;
     EventPair e;
-    hipEventCreate(&e.a);
-    hipEventCreate(&e.b);
+    if (!t.pool.empty()) {
+        e = t.pool.back();
+        t.pool.pop_back();
+    } else {
+        hipEventCreate(&e.a);
+        hipEventCreate(&e.b);
+    }
     hipEventRecord(e.a, ctx->stream);
     t.ev.push_back(e);
 }
@@ -141,17 +147,23 @@ static void timer_end(dem_ctx *ctx, Timer &t)
     t.launches++;
 }
 
-static void timer_collect(Timer &t)
+static void timer_collect(Timer &t, bool release = false)
 {
     for (auto &e : t.ev) {
         float ms = 0.f;
         hipEventSynchronize(e.b);
         hipEventElapsedTime(&ms, e.a, e.b);
         t.ms += ms;
-        hipEventDestroy(e.a);
-        hipEventDestroy(e.b);
+        t.pool.push_back(e);
     }
     t.ev.clear();
+    if (release) {
+        for (auto &e : t.pool) {
+            hipEventDestroy(e.a);
+            hipEventDestroy(e.b);
+        }
+        t.pool.clear();
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -235,9 +247,9 @@ extern "C" void dem_ctx_destroy(dem_ctx *ctx)
     if (!ctx) return;
     hipSetDevice(ctx->p.device);
     hipStreamSynchronize(ctx->stream);
-    timer_collect(ctx->t_contact);
-    timer_collect(ctx->t_integrate);
-    timer_collect(ctx->t_rebuild);
+    timer_collect(ctx->t_contact, true);
+    timer_collect(ctx->t_integrate, true);
+    timer_collect(ctx->t_rebuild, true);
     free_all(ctx);
     hipStreamDestroy(ctx->stream);
     delete ctx;

@@ -25,20 +25,51 @@
 #define MAXDNUM 8
 #define T1 (OR_MAXTYPES + 1)
 
-enum { OP_WALL = 1, OP_GRAVITY = 2, OP_FREEZE = 3 };
+enum { OP_WALL = 1, OP_GRAVITY = 2, OP_FREEZE = 3, OP_SETFORCE = 4, OP_AVEFORCE = 5 };
+
+/* contact_model_constants.h:64-67 */
+enum { CONTACT_NORMAL_MODEL = 1, CONTACT_COHESION_MODEL = 2, CONTACT_TANGENTIAL_MODEL = 4,
+       CONTACT_ROLLING_MODEL = 8 };
 
 typedef struct {
     int j;       /* owned atom index */
     int shift;   /* (sx+1)+3(sy+1)+9(sz+1), 13 = no image */
-    int flag;    /* contact flag (nonzero once surfacesIntersect ran) */
+    int flag;    /* contact flags (CONTACT_* bits) */
 } entry_t;
+
+/* history layout of one model chain; slots are registered in constructor
+ * order surface, normal, cohesion, tangential, rolling (contact_models.h:
+ * 150-157) */
+typedef struct {
+    int normal;      /* OR_NORMAL_* */
+    int limitf;
+    int off_norm;    /* hysteretic: 12 slots */
+    int off_shear;   /* 3 slots */
+    int off_roll;    /* 3 slots or -1 */
+    int dnum;
+    int newton[32];  /* newtonflag per slot (fix_contact_history.cpp:404-408) */
+} model_t;
+
+typedef struct {
+    int id, kind, arg, bit;
+    int style[3];
+    double val[3];
+    double c0, t0, c1;
+    double foriginal[3];
+} op_t;
 
 struct or_sim {
     or_params p;
     int ntypes;
     double Yeff[T1 * T1], Geff[T1 * T1], betaeff[T1 * T1];
     double mu[T1 * T1], mur[T1 * T1], etar[T1 * T1];
+    double restlog[T1 * T1];
+    double hyst[OR_NHYST][T1 * T1];
     double krs;
+    model_t mp, mw;   /* pair and wall model chains */
+    int *gmask;       /* group bits per atom */
+    double atime;
+    long long atimestep;
 
     int n;
     double *x, *v, *omega, *f, *torque, *radius, *rmass, *xhold;
@@ -66,7 +97,9 @@ struct or_sim {
     unsigned char *pin;     /* n * nplanes: atom in the wall's neighbour list */
     double *phist_w;        /* n * nplanes * dnum_w */
 
-    int nops, ops[16], opsarg[16];
+    int nops;
+    op_t ops[32];
+    int next_op_id;
     int has_freeze;
     double gacc[3];
 

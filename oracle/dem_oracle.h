@@ -37,9 +37,22 @@ typedef struct {
     int neigh_every, neigh_delay, neigh_check;   /* neigh_modify            */
     int rolling_model;           /* OR_ROLL_*                               */
     int tangential_damping;      /* hertz setting, reference default on     */
-    int limit_force;             /* hertz setting, reference default off    */
+    int limit_force;             /* model setting: hertz default off,
+                                    hysteretic default on                   */
     int torsion_torque;          /* rolling setting, default off            */
+    int normal_model;            /* OR_NORMAL_* of pair_style gran          */
+    int wall_normal_model;       /* OR_NORMAL_* of the fix wall/gran model  */
+    int wall_limit_force;        /* limitForce of the wall model            */
 } or_params;
+
+/* model keyword of pair_style gran / fix wall/gran */
+enum { OR_NORMAL_HERTZ = 0, OR_NORMAL_HYST1 = 1, OR_NORMAL_HYST2 = 2 };
+
+/* hysteretic/nonlinear* tables, in this order, each (ntypes+1)^2:
+ * LoadingStiffness, UnloadingStiffness, coefficientAdhesionStiffness,
+ * coefficientPlasticityDepth, pullOffForce, alphaCustom, cinCustom,
+ * aoneCustom, atwoCustom, athreeCustom, kcinCustom */
+#define OR_NHYST 11
 
 typedef struct or_sim or_sim;
 
@@ -68,6 +81,23 @@ int or_add_plane(or_sim *s, int dim, double pos, int wtype);
 void or_add_gravity(or_sim *s, double ax, double ay, double az);
 /* fix freeze on the atoms flagged in or_set_atoms (fix_freeze.cpp:121-144) */
 void or_add_freeze(or_sim *s);
+/* hysteretic/nonlinear* parameter tables, see OR_NHYST */
+int or_set_hysteretic(or_sim *s, const double *tables);
+/* group membership: bit 'bit' of atom i's mask (group command) */
+void or_set_group(or_sim *s, int bit, const int *members);
+/* fix setforce <group> with constant values; style 0 = NULL (untouched)
+ * (fix_setforce.cpp:246-277).  Returns the fix id. */
+int or_add_setforce(or_sim *s, int bit, const int style[3], const double value[3]);
+/* fix aveforce <group> x y z (fix_aveforce.cpp:222-281); x, y constant, z an
+ * equal-style variable of the form c0 - (time - t0)*c1.  Returns the id. */
+int or_add_aveforce(or_sim *s, int bit, const int style[3], double xv, double yv,
+                    double c0, double t0, double c1);
+/* unfix <id> (modify.cpp delete_fix): removes it from the post_force order */
+void or_unfix(or_sim *s, int id);
+/* fix aveforce compute_vector: group force before it was replaced */
+void or_aveforce_vector(const or_sim *s, int id, double out[3]);
+/* simulation time (the `time` thermo keyword, update.cpp) */
+double or_time(const or_sim *s);
 
 /* Verlet::setup (verlet.cpp:134-199): pbc, build list, force pass with
  * shearupdate = 0. */
