@@ -760,9 +760,11 @@ extern "C" int dem_get_pairs(dem_ctx *ctx, int64_t *count, int *tag_i, int *tag_
             tag_i[e] = tg[i];
             tag_j[e] = tg[nj[e] & kJMask];
             if (shift) shift[e] = (int)((nj[e] >> 26) & 31u);
-            if (flag) flag[e] = (nj[e] & kFlagBit) ? 1 : 0;
+            const bool known = (nj[e] & kFlagBit) != 0;
+            if (flag) flag[e] = known ? 1 : 0;
+            // an entry that never touched holds no history (zero by definition)
             if (hist)
-                for (int k = 0; k < dnum; k++) hist[(size_t)e * dnum + k] = hh[(size_t)e * dnum + k];
+                for (int k = 0; k < dnum; k++) hist[(size_t)e * dnum + k] = known ? hh[(size_t)e * dnum + k] : 0.0;
         }
     *count = E;
     return DEM_OK;

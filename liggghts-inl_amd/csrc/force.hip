@@ -1,8 +1,10 @@
 // force.hip -- contact force pass and velocity-Verlet integrator.
 //
-// k_force: one thread per particle gathers over its full neighbour row, so
-// every force and torque is summed in registers by its owner: no atomics,
-// no scatter, results independent of scheduling.  Replaces, per step,
+// k_force: a group of G lanes per particle gathers over the particle's full
+// neighbour row (lane g takes entries g, g+G, ...), then the group sums its
+// partial forces and torques with a fixed xor-butterfly of shuffles, so every
+// force and torque is summed by its owner in a fixed order: no atomics, no
+// scatter, results independent of scheduling.  Replaces, per step,
 // Verlet::force_clear (verlet.cpp:406-430), PairGran::compute
 // (pair_gran_base.h:187-503), FixWallGran::post_force_primitive
 // (fix_wall_gran.cpp:988-1120), FixGravity::post_force
@@ -13,11 +15,16 @@
 // symmetric), and periodic-image pairs are evaluated from both sides there
 // too (neigh_gran.cpp:570-640).
 //
+// An entry whose contact flag is still clear has never touched, so its
+// history is zero; it is neither read nor, unless it touches now, written.
+//
 // k_integrate: FixNVESphere::initial_integrate / final_integrate
 // (fix_nve_sphere.cpp:134-243) with dtf = 0.5*dt, INERTIA = 0.4; mode 3 runs
 // final(n) and initial(n+1) in one pass over the state.
 #include "contact_law.h"
 #include "kernels.h"
+
+#include <stdlib.h>
 
 namespace dem {
 
@@ -30,16 +37,33 @@ __device__ inline int xcd_block(int b, int nb_padded)
     return (b & 7) * per + (b >> 3);
 }
 
-template <bool MIXED>
+template <int G>
+__device__ inline double group_sum(double v)
+{
+    for (int o = G >> 1; o > 0; o >>= 1) v += __shfl_xor(v, o, G);
+    return v;
+}
+
+template <int G>
+__device__ inline int group_sum_int(int v)
+{
+    for (int o = G >> 1; o > 0; o >>= 1) v += __shfl_xor(v, o, G);
+    return v;
+}
+
+template <bool MIXED, int G>
 __global__ void __launch_bounds__(kBlock) k_force(ForceArgs a)
 {
     const int blk = xcd_block(blockIdx.x, gridDim.x);
-    const int i = blk * kBlock + threadIdx.x;
-    if (i >= a.n) return;
+    const int tid = blk * kBlock + threadIdx.x;
+    const int g = threadIdx.x & (G - 1);
+    const int i = tid / G;
+    const bool live = i < a.n;      // dead groups still join the shuffles
+    const int ii = live ? i : a.n - 1;
     const Model &M = a.M;
-    const double4 pi = a.xr[i];
-    const double4 vmi = a.vm[i];
-    const double4 wi = a.wt[i];
+    const double4 pi = a.xr[ii];
+    const double4 vmi = a.vm[ii];
+    const double4 wi = a.wt[ii];
     const int metai = (int)wi.w;
     const int itype = metai & 15;
     const bool fzi = (metai >> 4) != 0;
@@ -57,8 +81,9 @@ __global__ void __launch_bounds__(kBlock) k_force(ForceArgs a)
     c.wi[0] = wi.x; c.wi[1] = wi.y; c.wi[2] = wi.z;
     c.deltan_in = 0.0;
 
-    const int e0 = a.nstart[i], e1 = a.nstart[i + 1];
-    for (int e = e0; e < e1; e++) {
+    const int e0 = live ? a.nstart[i] : 0;
+    const int e1 = live ? a.nstart[i + 1] : 0;
+    for (int e = e0 + g; e < e1; e += G) {
         const uint32_t w = a.nj[e];
         const int j = (int)(w & kJMask);
         const int code = (int)((w >> 26) & 31u);
@@ -94,13 +119,14 @@ __global__ void __launch_bounds__(kBlock) k_force(ForceArgs a)
             c.wj[0] = wj.x; c.wj[1] = wj.y; c.wj[2] = wj.z;
             double h[kMaxDnum];
             double *hp = a.hist + (size_t)e * dnum;
-            for (int k = 0; k < dnum; k++) h[k] = hp[k];
+            const bool known = (w & kFlagBit) != 0;
+            for (int k = 0; k < dnum; k++) h[k] = known ? hp[k] : 0.0;
             double dF[3], dT[3];
             if (MIXED) contact_mixed(M, a.mat, c, h, a.shearupdate, dF, dT);
             else contact_fp64(M, a.mat, c, h, a.shearupdate, dF, dT);
-            if (a.shearupdate)
+            if (a.shearupdate || !known)
                 for (int k = 0; k < dnum; k++) hp[k] = h[k];
-            if (!(w & kFlagBit)) a.nj[e] = w | kFlagBit;
+            if (!known) a.nj[e] = w | kFlagBit;
             F[0] += dF[0]; F[1] += dF[1]; F[2] += dF[2];
             T[0] += dT[0]; T[1] += dT[1]; T[2] += dT[2];
             nc++;
@@ -110,6 +136,14 @@ __global__ void __launch_bounds__(kBlock) k_force(ForceArgs a)
             for (int k = 0; k < dnum; k++) hp[k] = 0.0;
         }
     }
+    if (G > 1) {
+        for (int k = 0; k < 3; k++) {
+            F[k] = group_sum<G>(F[k]);
+            T[k] = group_sum<G>(T[k]);
+        }
+        nc = group_sum_int<G>(nc);
+    }
+    if (!live || g != 0) return;
 
     // primitive planes, in the order the fixes were defined
     if (a.P.n) {
@@ -183,23 +217,40 @@ __global__ void __launch_bounds__(kBlock) k_force(ForceArgs a)
     if (a.ncontact) a.ncontact[i] = nc;
 }
 
-static int padded_blocks(int n)
+static int padded_blocks(long long threads)
 {
-    int nb = (n + kBlock - 1) / kBlock;
-    return (nb + 7) & ~7;
+    long long nb = (threads + kBlock - 1) / kBlock;
+    return (int)((nb + 7) & ~7LL);
 }
 
-void launch_force_fp64(const ForceArgs &a, hipStream_t s)
+// lanes per particle: DEM_CONTACT_LANES (1, 4, 8 or 16), default 8
+static int lanes_per_particle()
 {
-    if (a.n == 0) return;
-    k_force<false><<<padded_blocks(a.n), kBlock, 0, s>>>(a);
+    static int g = -1;
+    if (g < 0) {
+        const char *s = getenv("DEM_CONTACT_LANES");
+        g = s ? atoi(s) : 8;
+        if (g != 1 && g != 4 && g != 8 && g != 16) g = 8;
+    }
+    return g;
 }
 
-void launch_force_mixed(const ForceArgs &a, hipStream_t s)
+template <bool MIXED>
+static void launch_force(const ForceArgs &a, hipStream_t s)
 {
     if (a.n == 0) return;
-    k_force<true><<<padded_blocks(a.n), kBlock, 0, s>>>(a);
+    const int G = lanes_per_particle();
+    const int nb = padded_blocks((long long)a.n * G);
+    switch (G) {
+    case 1: k_force<MIXED, 1><<<nb, kBlock, 0, s>>>(a); break;
+    case 4: k_force<MIXED, 4><<<nb, kBlock, 0, s>>>(a); break;
+    case 16: k_force<MIXED, 16><<<nb, kBlock, 0, s>>>(a); break;
+    default: k_force<MIXED, 8><<<nb, kBlock, 0, s>>>(a); break;
+    }
 }
+
+void launch_force_fp64(const ForceArgs &a, hipStream_t s) { launch_force<false>(a, s); }
+void launch_force_mixed(const ForceArgs &a, hipStream_t s) { launch_force<true>(a, s); }
 
 // ---------------------------------------------------------------------------
 

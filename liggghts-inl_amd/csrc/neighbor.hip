@@ -220,98 +220,146 @@ void launch_permute(const PermuteArgs &a, hipStream_t s)
 
 // ---------------------------------------------------------------------------
 
+// One wavefront per particle.  Lanes 0..26 each own one cell of the 27-cell
+// stencil (dz, dy, dx ascending, as a sequential scan visits them); the
+// cells' candidates are enumerated 64 at a time, tested in parallel, and the
+// passing ones compacted with a ballot + popcount prefix.  The row therefore
+// comes out in exactly the order a sequential scan would write it, with
+// contiguous stores.  A candidate that touches (rsq < (radsum*cdf)^2) looks
+// up its history in the particle's previous row by partner tag; entries that
+// carry nothing are written with flag 0 and no history (a flag-0 entry has
+// never been in contact, so its history is zero by definition and the
+// contact pass does not read it).
+constexpr int kNeighWaves = kBlock / 64;
+
 template <bool FILL>
 __global__ void __launch_bounds__(kBlock) k_neigh(NeighArgs a)
 {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= a.n) return;
+    __shared__ int s_ex[kNeighWaves][28], s_qb[kNeighWaves][27], s_code[kNeighWaves][27];
+    const int lane = threadIdx.x & 63;
+    const int wv = threadIdx.x >> 6;
+    const int p = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (p >= a.n) return;  // p is uniform across the wavefront
     const double4 xp = a.xr[p];
-    const int cx = cell_coord(xp.x, 0, a.G), cy = cell_coord(xp.y, 1, a.G), cz = cell_coord(xp.z, 2, a.G);
-    const int cc[3] = {cx, cy, cz};
-    int cnt = 0;
-    int e = FILL ? a.nstart[p] : 0;
+    const int cc[3] = {cell_coord(xp.x, 0, a.G), cell_coord(xp.y, 1, a.G), cell_coord(xp.z, 2, a.G)};
+    int qb = 0, cnt = 0, code = kNoShift;
+    if (lane < 27) {
+        const int off[3] = {lane % 3 - 1, (lane / 3) % 3 - 1, lane / 9 - 1};
+        int c[3], sh[3];
+        bool ok = true;
+        for (int d = 0; d < 3; d++) {
+            c[d] = cc[d] + off[d];
+            sh[d] = 0;
+            if (c[d] < 0) {
+                if (a.G.periodic[d]) { c[d] += a.G.nc[d]; sh[d] = -1; }
+                else ok = false;
+            } else if (c[d] >= a.G.nc[d]) {
+                if (a.G.periodic[d]) { c[d] -= a.G.nc[d]; sh[d] = 1; }
+                else ok = false;
+            }
+        }
+        if (ok) {
+            const uint32_t key = cell_key(c[0], c[1], c[2], a.G);
+            qb = a.cstart[key];
+            cnt = a.cstart[key + 1] - qb;
+            code = (sh[0] + 1) + 3 * (sh[1] + 1) + 9 * (sh[2] + 1);
+        }
+    }
+    int incl = cnt;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += t;
+    }
+    const int total = __shfl(incl, 63, 64);
+    if (lane < 27) {
+        s_ex[wv][lane] = incl - cnt;
+        s_qb[wv][lane] = qb;
+        s_code[wv][lane] = code;
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+
     int ob = 0, oe = 0;
     if (FILL && a.have_old) {
         const int o = a.perm[p];
         ob = a.onstart[o];
         oe = a.onstart[o + 1];
     }
-    for (int dz = -1; dz <= 1; dz++)
-        for (int dy = -1; dy <= 1; dy++)
-            for (int dx = -1; dx <= 1; dx++) {
-                const int off[3] = {dx, dy, dz};
-                int c[3], sh[3];
-                bool ok = true;
-                for (int d = 0; d < 3; d++) {
-                    c[d] = cc[d] + off[d];
-                    sh[d] = 0;
-                    if (c[d] < 0) {
-                        if (a.G.periodic[d]) { c[d] += a.G.nc[d]; sh[d] = -1; }
-                        else ok = false;
-                    } else if (c[d] >= a.G.nc[d]) {
-                        if (a.G.periodic[d]) { c[d] -= a.G.nc[d]; sh[d] = 1; }
-                        else ok = false;
-                    }
-                }
-                if (!ok) continue;
-                const int code = (sh[0] + 1) + 3 * (sh[1] + 1) + 9 * (sh[2] + 1);
-                const uint32_t key = cell_key(c[0], c[1], c[2], a.G);
-                const int qb = a.cstart[key], qe = a.cstart[key + 1];
-                for (int q = qb; q < qe; q++) {
-                    if (q == p && code == kNoShift) continue;
-                    const double4 xq = a.xr[q];
-                    const double xj0 = xq.x + sh[0] * a.prd[0];
-                    const double xj1 = xq.y + sh[1] * a.prd[1];
-                    const double xj2 = xq.z + sh[2] * a.prd[2];
-                    const double delx = xp.x - xj0;
-                    const double dely = xp.y - xj1;
-                    const double delz = xp.z - xj2;
-                    const double rsq = delx * delx + dely * dely + delz * delz;
-                    const double radsum = (xp.w + xq.w) * a.cdf;
-                    const double cutsq = (radsum + a.skin) * (radsum + a.skin);
-                    if (!(rsq <= cutsq)) continue;
-                    if (!FILL) {
-                        cnt++;
-                        continue;
-                    }
-                    uint32_t w = (uint32_t)q | ((uint32_t)code << 26);
+    const int ebase = FILL ? a.nstart[p] : 0;
+    int written = 0;
+    for (int c0 = 0; c0 < total; c0 += 64) {
+        const int c = c0 + lane;
+        bool pass = false, touch = false;
+        int q = 0, qcode = kNoShift;
+        if (c < total) {
+            int lo = 0, hi = 26;  // largest k with ex[k] <= c
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (s_ex[wv][mid] <= c) lo = mid;
+                else hi = mid - 1;
+            }
+            q = s_qb[wv][lo] + (c - s_ex[wv][lo]);
+            qcode = s_code[wv][lo];
+            if (!(q == p && qcode == kNoShift)) {
+                int sx, sy, sz;
+                shift_decode(qcode, sx, sy, sz);
+                const double4 xq = a.xr[q];
+                const double xj0 = xq.x + sx * a.prd[0];
+                const double xj1 = xq.y + sy * a.prd[1];
+                const double xj2 = xq.z + sz * a.prd[2];
+                const double delx = xp.x - xj0;
+                const double dely = xp.y - xj1;
+                const double delz = xp.z - xj2;
+                const double rsq = delx * delx + dely * dely + delz * delz;
+                const double radsum = (xp.w + xq.w) * a.cdf;
+                const double cutsq = (radsum + a.skin) * (radsum + a.skin);
+                pass = rsq <= cutsq;
+                touch = rsq < radsum * radsum;
+            }
+        }
+        const uint64_t m = __ballot(pass);
+        if (FILL && pass) {
+            const int e = ebase + written + __popcll(m & ((1ull << lane) - 1ull));
+            uint32_t w = (uint32_t)q | ((uint32_t)qcode << 26);
+            if (a.have_old && touch) {
+                const int tq = a.tag[q];
+                for (int oe2 = ob; oe2 < oe; oe2++) {
+                    const uint32_t ow = a.onj[oe2];
+                    if (!(ow & kFlagBit)) continue;
+                    if (a.otag[ow & kJMask] != tq) continue;
+                    const double *oh = a.ohist + (size_t)oe2 * a.dnum;
                     double *h = a.hist + (size_t)e * a.dnum;
-                    bool carried = false;
-                    if (a.have_old && rsq < radsum * radsum) {
-                        const int tq = a.tag[q];
-                        for (int oe2 = ob; oe2 < oe; oe2++) {
-                            const uint32_t ow = a.onj[oe2];
-                            if (!(ow & kFlagBit)) continue;
-                            if (a.otag[ow & kJMask] != tq) continue;
-                            const double *oh = a.ohist + (size_t)oe2 * a.dnum;
-                            for (int k = 0; k < a.dnum; k++) h[k] = oh[k];
-                            carried = true;
-                            break;
-                        }
-                    }
-                    if (carried) w |= kFlagBit;
-                    else
-                        for (int k = 0; k < a.dnum; k++) h[k] = 0.0;
-                    a.nj[e] = w;
-                    e++;
+                    for (int k = 0; k < a.dnum; k++) h[k] = oh[k];
+                    w |= kFlagBit;
+                    break;
                 }
             }
-    if (!FILL) {
-        a.count[p] = cnt;
-        atomicMax(a.maxcount, cnt);
+            a.nj[e] = w;
+        }
+        written += __popcll(m);
     }
+    if (!FILL && lane == 0) {
+        a.count[p] = written;
+        atomicMax(a.maxcount, written);
+    }
+}
+
+static int wave_blocks(int n)
+{
+    return (int)(((long long)n + kNeighWaves - 1) / kNeighWaves);
 }
 
 void launch_neigh_count(const NeighArgs &a, hipStream_t s)
 {
     if (a.n == 0) return;
-    k_neigh<false><<<(a.n + kBlock - 1) / kBlock, kBlock, 0, s>>>(a);
+    k_neigh<false><<<wave_blocks(a.n), kBlock, 0, s>>>(a);
 }
 
 void launch_neigh_fill(const NeighArgs &a, hipStream_t s)
 {
     if (a.n == 0) return;
-    k_neigh<true><<<(a.n + kBlock - 1) / kBlock, kBlock, 0, s>>>(a);
+    k_neigh<true><<<wave_blocks(a.n), kBlock, 0, s>>>(a);
 }
 
 }  // namespace dem

@@ -114,12 +114,43 @@ static const double SQRT_FIVE_OVER_SIX = 0.9128709291752768557616163046680035565
 
 /* ------------------------------------------------------------------------ */
 
+/* history slots in registration order: hysteretic normal (12: deltaMax,
+ * deltaZero, k1, deltaZero_old, k1_old, delta_old, deltaMin, betan, f0,
+ * f0_old with newtonflag 0; kc, fo with newtonflag 1 --
+ * normal_model_hysteretic_nonlinear1.h:84-97), shear x/y/z
+ * (tangential_model_history.h:79-81), rolling torque x/y/z
+ * (rolling_model_epsd.h:71-73), all with newtonflag 1. */
+static void model_layout(model_t *m, int normal, int limitf, int rolling)
+{
+    int k = 0;
+    memset(m, 0, sizeof(*m));
+    m->normal = normal;
+    m->limitf = limitf;
+    m->off_norm = -1;
+    if (normal != OR_NORMAL_HERTZ) {
+        m->off_norm = k;
+        for (int q = 0; q < 10; q++) m->newton[k++] = 0;
+        m->newton[k++] = 1;
+        m->newton[k++] = 1;
+    }
+    m->off_shear = k;
+    for (int q = 0; q < 3; q++) m->newton[k++] = 1;
+    m->off_roll = -1;
+    if (rolling != OR_ROLL_OFF) {
+        m->off_roll = k;
+        for (int q = 0; q < 3; q++) m->newton[k++] = 1;
+    }
+    m->dnum = k;
+}
+
 or_sim *or_create(const or_params *p)
 {
     or_sim *s = (or_sim *)calloc(1, sizeof(or_sim));
     s->p = *p;
-    s->dnum = 3 + (p->rolling_model != OR_ROLL_OFF ? 3 : 0);
-    s->dnum_w = s->dnum;
+    model_layout(&s->mp, p->normal_model, p->limit_force, p->rolling_model);
+    model_layout(&s->mw, p->wall_normal_model, p->wall_limit_force, p->rolling_model);
+    s->dnum = s->mp.dnum;
+    s->dnum_w = s->mw.dnum;
     return s;
 }
 
@@ -129,7 +160,7 @@ static void free_atoms(or_sim *s)
     free(s->radius); free(s->rmass); free(s->xhold); free(s->type); free(s->tag);
     free(s->freeze); free(s->image); free(s->fscale); free(s->tscale);
     free(s->nstart); free(s->ncount); free(s->npartner); free(s->pstart);
-    free(s->ptag); free(s->phist); free(s->pin); free(s->phist_w);
+    free(s->ptag); free(s->phist); free(s->pin); free(s->phist_w); free(s->gmask);
 }
 
 void or_destroy(or_sim *s)
@@ -156,11 +187,23 @@ int or_set_materials(or_sim *s, int ntypes, const double *Y, const double *nu,
             s->Geff[i * T1 + j] = 1. / (2. * (2. - vi) * (1. + vi) / Yi + 2. * (2. - vj) * (1. + vj) / Yj);
             const double lg = log(e[i * w + j]);
             s->betaeff[i * T1 + j] = lg / sqrt(pow(lg, 2.) + pow(M_PI, 2.));
+            s->restlog[i * T1 + j] = lg;
             s->mu[i * T1 + j] = mu[i * w + j];
             s->mur[i * T1 + j] = mur ? mur[i * w + j] : 0.0;
             s->etar[i * T1 + j] = etar ? etar[i * w + j] : 0.0;
         }
     s->krs = krs;
+    return 0;
+}
+
+int or_set_hysteretic(or_sim *s, const double *tables)
+{
+    const int w = s->ntypes + 1;
+    if (s->ntypes < 1) return -1;
+    for (int t = 0; t < OR_NHYST; t++)
+        for (int i = 1; i <= s->ntypes; i++)
+            for (int j = 1; j <= s->ntypes; j++)
+                s->hyst[t][i * T1 + j] = tables[(size_t)t * w * w + i * w + j];
     return 0;
 }
 
@@ -189,6 +232,7 @@ int or_set_atoms(or_sim *s, int n, const double *x, const double *v, const doubl
     s->fscale = calloc(1, n1); s->tscale = calloc(1, n1);
     s->type = malloc(ni); s->tag = malloc(ni); s->freeze = calloc(1, ni);
     s->image = calloc(3, ni);
+    s->gmask = calloc(1, ni);
     s->nstart = calloc(1, ni); s->ncount = calloc(1, ni);
     s->npartner = calloc(1, ni); s->pstart = calloc(1, ni);
     s->ptag = NULL; s->phist = NULL;
@@ -203,25 +247,83 @@ int or_set_atoms(or_sim *s, int n, const double *x, const double *v, const doubl
     return 0;
 }
 
+static op_t *new_op(or_sim *s, int kind)
+{
+    op_t *o = &s->ops[s->nops++];
+    memset(o, 0, sizeof(*o));
+    o->id = s->next_op_id++;
+    o->kind = kind;
+    return o;
+}
+
 int or_add_plane(or_sim *s, int dim, double pos, int wtype)
 {
     if (s->nplanes >= OR_MAXPLANES || dim < 0 || dim > 2) return -1;
     const int k = s->nplanes++;
     s->pdim[k] = dim; s->ppos[k] = pos; s->ptype[k] = wtype;
-    s->ops[s->nops] = OP_WALL; s->opsarg[s->nops++] = k;
-    return 0;
+    op_t *o = new_op(s, OP_WALL);
+    o->arg = k;
+    return o->id;
 }
 
 void or_add_gravity(or_sim *s, double ax, double ay, double az)
 {
     s->gacc[0] = ax; s->gacc[1] = ay; s->gacc[2] = az;
-    s->ops[s->nops] = OP_GRAVITY; s->opsarg[s->nops++] = 0;
+    new_op(s, OP_GRAVITY);
 }
 
 void or_add_freeze(or_sim *s)
 {
-    s->ops[s->nops] = OP_FREEZE; s->opsarg[s->nops++] = 0;
+    new_op(s, OP_FREEZE);
     s->has_freeze = 1;
+}
+
+void or_set_group(or_sim *s, int bit, const int *members)
+{
+    for (int i = 0; i < s->n; i++) {
+        if (members[i]) s->gmask[i] |= 1 << bit;
+        else s->gmask[i] &= ~(1 << bit);
+    }
+}
+
+int or_add_setforce(or_sim *s, int bit, const int style[3], const double value[3])
+{
+    op_t *o = new_op(s, OP_SETFORCE);
+    o->bit = bit;
+    for (int d = 0; d < 3; d++) { o->style[d] = style[d]; o->val[d] = value[d]; }
+    return o->id;
+}
+
+int or_add_aveforce(or_sim *s, int bit, const int style[3], double xv, double yv,
+                    double c0, double t0, double c1)
+{
+    op_t *o = new_op(s, OP_AVEFORCE);
+    o->bit = bit;
+    for (int d = 0; d < 3; d++) o->style[d] = style[d];
+    o->val[0] = xv; o->val[1] = yv;
+    o->c0 = c0; o->t0 = t0; o->c1 = c1;
+    return o->id;
+}
+
+void or_unfix(or_sim *s, int id)
+{
+    int k = 0;
+    for (int q = 0; q < s->nops; q++)
+        if (s->ops[q].id != id) s->ops[k++] = s->ops[q];
+    s->nops = k;
+}
+
+void or_aveforce_vector(const or_sim *s, int id, double out[3])
+{
+    for (int q = 0; q < s->nops; q++)
+        if (s->ops[q].id == id)
+            for (int d = 0; d < 3; d++) out[d] = s->ops[q].foriginal[d];
+}
+
+/* update.cpp: atime + (ntimestep - atimestep)*dt */
+double or_time(const or_sim *s)
+{
+    return s->atime + (s->ntimestep - s->atimestep) * s->p.dt;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -270,9 +372,8 @@ static void domain_pbc(or_sim *s)
 
 /* ------------------------------------------------------------------------ */
 /* Contact history: FixContactHistory::pre_exchange                          */
-/* (fix_contact_history.cpp:305-426).  All history slots of the Hertz/       */
-/* history/EPSD* chain have newtonflag 1 (tangential_model_history.h:79-81,  */
-/* rolling_model_epsd.h:71-73), so the partner's copy is negated.            */
+/* (fix_contact_history.cpp:305-426).  The partner's copy negates the slots */
+/* registered with newtonflag 1 (model_layout).                              */
 
 static void pre_exchange(or_sim *s)
 {
@@ -307,7 +408,8 @@ static void pre_exchange(or_sim *s)
                 const int j = e->j;
                 m = s->pstart[j] + s->npartner[j];
                 s->ptag[m] = s->tag[i];
-                for (int d = 0; d < dnum; d++) s->phist[(size_t)m * dnum + d] = -h[d];
+                for (int d = 0; d < dnum; d++)
+                    s->phist[(size_t)m * dnum + d] = s->mp.newton[d] ? -h[d] : h[d];
                 s->npartner[j]++;
             }
         }
@@ -517,6 +619,8 @@ typedef struct {
     double Fn, kn, kt, gamman, gammat;
     int update_history;
     double *hist;       /* dnum doubles */
+    const model_t *model;
+    int *flags;         /* contact flags, NULL for walls */
 } sidata_t;
 
 /* surface_model_default.h:146-208 */
@@ -574,7 +678,7 @@ static void normal_hertz(const or_sim *s, sidata_t *sd, double Fi[3], double Fj[
     const double Fn_damping = -gamman * sd->vn;
     const double Fn_contact = kn * deltan;
     double Fn = Fn_damping + Fn_contact;
-    if (s->p.limit_force && (Fn < 0.0)) Fn = 0.0;
+    if (sd->model->limitf && (Fn < 0.0)) Fn = 0.0;
     sd->Fn = Fn; sd->kn = kn; sd->kt = kt; sd->gamman = gamman; sd->gammat = gammat;
     if (sd->is_wall) {
         const double Fn_ = Fn * 1.0;  /* area_ratio = 1 */
@@ -596,7 +700,7 @@ static void tangential_history(const or_sim *s, sidata_t *sd, double Fi[3], doub
                                double Fj[3], double Tj[3])
 {
     const double enx = sd->en[0], eny = sd->en[1], enz = sd->en[2];
-    double *shear = sd->hist;
+    double *shear = sd->hist + sd->model->off_shear;
     if (sd->update_history) {
         const double dt = s->p.dt;
         shear[0] += sd->vtr[0] * dt;
@@ -657,7 +761,7 @@ static void roll_torque(const or_sim *s, const sidata_t *sd, double reff,
     const int t = sd->itype * T1 + sd->jtype;
     const double enx = sd->en[0], eny = sd->en[1], enz = sd->en[2];
     const double dt = s->p.dt;
-    double *c_history = sd->hist + 3;
+    double *c_history = sd->hist + sd->model->off_roll;
     const double rmu = s->mur[t];
     double wr_tot[3];
     if (s->p.torsion_torque) {
@@ -728,22 +832,204 @@ static void rolling(const or_sim *s, sidata_t *sd, double Ti[3], double Tj[3])
     Tj[0] += rt[0]; Tj[1] += rt[1]; Tj[2] += rt[2];
 }
 
+/* NormalModel<HYSTERETIC_NONLINEAR1/2>::surfacesIntersect
+ * (normal_model_hysteretic_nonlinear1.h:149-420, ..._nonlinear2.h:149-420).
+ * The history is written whatever shearupdate says, and sqrt(5/4) is the
+ * integer expression of the reference, i.e. sqrt(1) = 1. */
+static void normal_hysteretic(const or_sim *s, sidata_t *sd, double Fi[3], double Fj[3])
+{
+    const int t = sd->itype * T1 + sd->jtype;
+    const int nl2 = sd->model->normal == OR_NORMAL_HYST2;
+    const double deltan = sd->deltan;
+    const double ri = sd->radi, rj = sd->radj;
+    const double reff = sd->is_wall ? sd->radi : (ri * rj / (ri + rj));
+    const double meff = sd->meff;
+    double kn = s->hyst[0][t];
+    const double Alpha_in = s->hyst[5][t];
+    const double Cin_in = s->hyst[6][t];
+    const double A1_in = s->hyst[7][t];
+    const double A2_in = s->hyst[8][t];
+    const double A3_in = s->hyst[9][t];
+    const double kcin_in = s->hyst[10][t];
+    const double k_c = kcin_in * A2_in;
+    const double k2Max = kn * s->hyst[1][t];
+    const double kc = s->hyst[2][t] * kn;
+    const double f_0 = s->hyst[4][t];
+    const double crl = s->restlog[t];
+    const double sqrt54 = sqrt((double)(5 / 4));
+    const double fcc = 1.0;
+    double gamman, gammat;
+    if (sd->flags) *sd->flags |= CONTACT_NORMAL_MODEL;
+    double *const history = sd->hist + sd->model->off_norm;
+    double *const kc_history = history + 10;
+    double *const fo_history = history + 11;
+    double deltaMax;
+    if (deltan > history[0]) {
+        history[0] = deltan;
+        deltaMax = deltan;
+    } else {
+        deltaMax = history[0];
+    }
+    double deltaZero = history[1];
+    double k1 = history[2];
+    double deltaZero_old = history[3];
+    double k1_old = history[4];
+    const double delta_old = history[5];
+    double deltaMin = history[6];
+    double betan = history[7];
+    double f0 = history[8];
+    double f0_old = history[9];
+    (void)k2Max; (void)reff;  /* deltaMaxLim is computed but unused there */
+    double k2 = A3_in * k1;
+    double beta;
+    int tag_status;
+    if (deltan >= delta_old) {
+        if (delta_old == 0) tag_status = 1;
+        else tag_status = (sd->vn > 0) ? 2 : 1;
+    } else {
+        tag_status = 2;
+    }
+    double fHys;
+    const double dampfac = 1. + (M_PI / crl) * (M_PI / crl);
+    if (tag_status == 1) {
+        if (deltaZero == 0) k1 = A2_in;
+        else k1 = history[2];
+        deltaMin = history[6];
+        f0_old = f0;
+        if (deltan <= deltaZero) {
+            if (nl2)
+                fHys = Cin_in * k2 * (exp(betan * (deltan - deltaZero)) - 1);
+            else if (deltan <= deltaMin)
+                fHys = -k_c * deltan;
+            else
+                fHys = betan * (deltan - deltaZero);
+        } else {
+            fHys = Alpha_in * k1 * pow(deltan - deltaZero, 2) + f0;
+        }
+        const double pw = nl2 ? 0.5 : 0.25;
+        gamman = sqrt54 * sqrt(4. * meff * Alpha_in * k1 / dampfac) * (pow(deltan, pw) + pow(deltaZero, pw));
+        gammat = sqrt54 * sqrt(4. * meff * Alpha_in * k1 / dampfac) * (pow(deltan, pw) + pow(deltaZero, pw));
+        history[1] = deltaZero;
+        history[2] = k1;
+        history[3] = deltaZero;
+        history[4] = k1;
+        history[5] = deltan;
+        history[6] = deltaMin;
+        history[7] = betan;
+        history[8] = f0;
+        history[9] = f0_old;
+    } else {
+        if (!nl2) {
+            k2 = A3_in * k1_old;
+            deltaZero = (1 - k1_old / k2) * deltaMax;
+            beta = Alpha_in * k1_old * pow(deltaMax - deltaZero_old, 2) / k2 / (deltaMax - deltaZero);
+            deltaMin = beta * (k2 - k1_old) / (beta * k2 + k_c) * deltaMax;
+            k1 = deltaMax * A1_in + A2_in;
+            if (deltan >= deltaMin) {
+                if (deltan >= deltaZero) {
+                    betan = beta * k2;
+                    fHys = beta * k2 * (deltan - deltaZero) + (deltan - deltaZero) * f0_old / (deltaMax - deltaZero);
+                    f0 = fHys - Alpha_in * k1 * pow(deltan - deltaZero, 2);
+                } else {
+                    betan = beta * k2;
+                    fHys = beta * k2 * (deltan - deltaZero);
+                    f0 = 0;
+                }
+            } else {
+                fHys = -k_c * deltan;
+                f0 = 0;
+            }
+            gamman = sqrt54 * sqrt(4. * meff * Alpha_in * k1_old / dampfac) * (pow(deltan, 0.25));
+            gammat = sqrt54 * sqrt(4. * meff * Alpha_in * k1_old / dampfac) * (pow(deltan, 0.25));
+        } else {
+            k2 = A3_in * (A1_in * deltaMax + A2_in);
+            deltaZero = fcc * (1 - k1_old / k2) * deltaMax;
+            deltaZero_old = history[3];
+            betan = log(Alpha_in * k1_old / Cin_in / k2 * pow(deltaMax - deltaZero_old, 2) + 1)
+                    / (deltaMax - fcc * (1 - k1_old / k2) * deltaMax);
+            deltaMin = betan * (k2 - k1_old) / (betan * k2 + k_c) * deltaMax;
+            k1 = deltaMax * A1_in + A2_in;
+            if (deltan >= deltaZero) {
+                fHys = Cin_in * k2 * (exp(betan * (deltan - deltaZero)) - 1)
+                       + (deltan - deltaZero) * f0_old / (deltaMax - deltaZero);
+                f0 = fHys - Alpha_in * k1 * pow(deltan - deltaZero, 2);
+            } else {
+                fHys = Cin_in * k2 * (exp(betan * (deltan - deltaZero)) - 1);
+                f0 = 0;
+            }
+            gamman = 1 * 0.001 * sqrt54 * sqrt(4. * meff * Alpha_in * k1_old / dampfac) * (pow(deltan, -0.25));
+            gammat = 1 * 0.001 * sqrt54 * sqrt(4. * meff * Alpha_in * k1_old / dampfac) * (pow(deltan, -0.25));
+        }
+        history[1] = deltaZero;
+        history[2] = k1;
+        history[3] = deltaZero_old;
+        history[4] = k1_old;
+        history[5] = deltan;
+        history[6] = deltaMin;
+        history[7] = betan;
+        history[8] = f0;
+    }
+    kn = k1;
+    const double kt = kn;
+    const double Fn_damping = -gamman * sd->vn;
+    double Fn = fHys + Fn_damping + f_0;
+    if (sd->model->limitf && (Fn < 0.0) && kc == 0 && f_0 == 0.0) Fn = 0.0;
+    sd->Fn = Fn;
+    sd->kn = kn;
+    sd->kt = kt;
+    kc_history[0] = kc;
+    fo_history[0] = f_0;
+    sd->gamman = gamman;
+    sd->gammat = gammat;
+    if (sd->is_wall) {
+        const double Fn_ = Fn * 1.0;
+        Fi[0] = Fn_ * sd->en[0];
+        Fi[1] = Fn_ * sd->en[1];
+        Fi[2] = Fn_ * sd->en[2];
+    } else {
+        Fi[0] = sd->Fn * sd->en[0];
+        Fi[1] = sd->Fn * sd->en[1];
+        Fi[2] = sd->Fn * sd->en[2];
+        Fj[0] = -Fi[0];
+        Fj[1] = -Fi[1];
+        Fj[2] = -Fi[2];
+    }
+}
+
 static void chain_intersect(const or_sim *s, sidata_t *sd, double Fi[3], double Ti[3],
                             double Fj[3], double Tj[3])
 {
     surface_default(sd);
-    normal_hertz(s, sd, Fi, Fj);
+    if (sd->model->normal == OR_NORMAL_HERTZ) {
+        if (sd->flags) *sd->flags |= CONTACT_NORMAL_MODEL;
+        normal_hertz(s, sd, Fi, Fj);
+    } else {
+        normal_hysteretic(s, sd, Fi, Fj);
+    }
+    if (sd->flags) *sd->flags |= CONTACT_TANGENTIAL_MODEL;
     tangential_history(s, sd, Fi, Ti, Fj, Tj);
-    if (s->p.rolling_model != OR_ROLL_OFF) rolling(s, sd, Ti, Tj);
+    if (sd->model->off_roll >= 0) {
+        if (sd->flags) *sd->flags |= CONTACT_ROLLING_MODEL;
+        rolling(s, sd, Ti, Tj);
+    }
 }
 
-/* surfacesClose chain: tangential zeroes shear (tangential_model_history.h:
- * 428-440), rolling zeroes its torque history (rolling_model_epsd.h:222-229) */
-static void chain_close(const or_sim *s, double *hist, int *flag)
+/* surfacesClose chain: hysteretic zeroes its ten state slots and clears its
+ * flag (normal_model_hysteretic_nonlinear1.h:444-458; hertz keeps its flag),
+ * tangential zeroes shear (tangential_model_history.h:428-440), rolling
+ * zeroes its torque history (rolling_model_epsd.h:222-229). */
+static void chain_close(const model_t *m, double *hist, int *flag)
 {
-    (void)flag;
-    hist[0] = hist[1] = hist[2] = 0.0;
-    if (s->p.rolling_model != OR_ROLL_OFF) hist[3] = hist[4] = hist[5] = 0.0;
+    if (m->normal != OR_NORMAL_HERTZ) {
+        if (flag) *flag &= ~CONTACT_NORMAL_MODEL;
+        for (int q = 0; q < 10; q++) hist[m->off_norm + q] = 0.0;
+    }
+    if (flag) *flag &= ~CONTACT_TANGENTIAL_MODEL;
+    for (int q = 0; q < 3; q++) hist[m->off_shear + q] = 0.0;
+    if (m->off_roll >= 0) {
+        if (flag) *flag &= ~CONTACT_ROLLING_MODEL;
+        for (int q = 0; q < 3; q++) hist[m->off_roll + q] = 0.0;
+    }
 }
 
 /* ------------------------------------------------------------------------ */
@@ -791,7 +1077,8 @@ static void pair_compute(or_sim *s, int shearupdate)
                 sd.omega_i = &s->omega[3 * i]; sd.omega_j = &s->omega[3 * j];
                 sd.update_history = shearupdate;
                 sd.hist = h;
-                e->flag = 1;
+                sd.model = &s->mp;
+                sd.flags = &e->flag;
                 chain_intersect(s, &sd, Fi, Ti, Fj, Tj);
                 for (int d = 0; d < 3; d++) {
                     s->f[3 * i + d] += Fi[d];
@@ -808,7 +1095,7 @@ static void pair_compute(or_sim *s, int shearupdate)
                     s->tscale[j] += sqrt(Tj[0] * Tj[0] + Tj[1] * Tj[1] + Tj[2] * Tj[2]);
                 }
             } else if (rsq < cdm * radsum * radsum) {
-                chain_close(s, h, &e->flag);
+                chain_close(&s->mp, h, &e->flag);
             }
         }
     }
@@ -863,6 +1150,8 @@ static void wall_plane(or_sim *s, int k, int shearupdate)
             sd.radsum = sd.radi;
             sd.update_history = shearupdate;
             sd.hist = h;
+            sd.model = &s->mw;
+            sd.flags = NULL;
             double Fi[3] = {0, 0, 0}, Ti[3] = {0, 0, 0}, Fj[3] = {0, 0, 0}, Tj[3] = {0, 0, 0};
             if (intersect) {
                 chain_intersect(s, &sd, Fi, Ti, Fj, Tj);
@@ -873,7 +1162,7 @@ static void wall_plane(or_sim *s, int k, int shearupdate)
                 s->fscale[i] += sqrt(Fi[0] * Fi[0] + Fi[1] * Fi[1] + Fi[2] * Fi[2]);
                 s->tscale[i] += sqrt(Ti[0] * Ti[0] + Ti[1] * Ti[1] + Ti[2] * Ti[2]);
             } else {
-                chain_close(s, h, NULL);
+                chain_close(&s->mw, h, NULL);
             }
         } else {
             for (int q = 0; q < s->dnum_w; q++) h[q] = 0.0;
@@ -900,6 +1189,44 @@ static void freeze(or_sim *s)
             for (int d = 0; d < 3; d++) s->f[3 * i + d] = s->torque[3 * i + d] = 0.0;
 }
 
+/* FixSetForce::post_force, constant values (fix_setforce.cpp:246-277) */
+static void setforce(or_sim *s, op_t *o)
+{
+    o->foriginal[0] = o->foriginal[1] = o->foriginal[2] = 0.0;
+    for (int i = 0; i < s->n; i++)
+        if (s->gmask[i] & (1 << o->bit)) {
+            for (int d = 0; d < 3; d++) o->foriginal[d] += s->f[3 * i + d];
+            for (int d = 0; d < 3; d++)
+                if (o->style[d]) s->f[3 * i + d] = o->val[d];
+        }
+}
+
+/* FixAveForce::post_force (fix_aveforce.cpp:222-281); the z value is the
+ * equal-style variable c0 - (time - t0)*c1 evaluated at this step */
+static void aveforce(or_sim *s, op_t *o)
+{
+    double fo[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int i = 0; i < s->n; i++)
+        if (s->gmask[i] & (1 << o->bit)) {
+            fo[0] += s->f[3 * i];
+            fo[1] += s->f[3 * i + 1];
+            fo[2] += s->f[3 * i + 2];
+            fo[3] += 1.0;
+        }
+    for (int d = 0; d < 3; d++) o->foriginal[d] = fo[d];
+    const int ncount = (int)fo[3];
+    if (ncount == 0) return;
+    const double zvalue = o->c0 - (or_time(s) - o->t0) * o->c1;
+    double fave[3];
+    fave[0] = fo[0] / ncount + o->val[0];
+    fave[1] = fo[1] / ncount + o->val[1];
+    fave[2] = fo[2] / ncount + zvalue;
+    for (int i = 0; i < s->n; i++)
+        if (s->gmask[i] & (1 << o->bit))
+            for (int d = 0; d < 3; d++)
+                if (o->style[d]) s->f[3 * i + d] = fave[d];
+}
+
 void or_compute_forces(or_sim *s, int shearupdate)
 {
     /* Verlet::force_clear (verlet.cpp:406-430) */
@@ -912,9 +1239,12 @@ void or_compute_forces(or_sim *s, int shearupdate)
     s->wall_rebuild = 0;
     pair_compute(s, shearupdate);
     for (int q = 0; q < s->nops; q++) {
-        if (s->ops[q] == OP_WALL) wall_plane(s, s->opsarg[q], shearupdate);
-        else if (s->ops[q] == OP_GRAVITY) gravity(s);
-        else if (s->ops[q] == OP_FREEZE) freeze(s);
+        op_t *o = &s->ops[q];
+        if (o->kind == OP_WALL) wall_plane(s, o->arg, shearupdate);
+        else if (o->kind == OP_GRAVITY) gravity(s);
+        else if (o->kind == OP_FREEZE) freeze(s);
+        else if (o->kind == OP_SETFORCE) setforce(s, o);
+        else if (o->kind == OP_AVEFORCE) aveforce(s, o);
     }
 }
 
@@ -989,6 +1319,13 @@ void or_run(or_sim *s, int nsteps)
         or_compute_forces(s, 1);
         final_integrate(s);
     }
+}
+
+/* end of a `run` command: Verlet::cleanup -> Update::update_time */
+void or_cleanup(or_sim *s)
+{
+    s->atime += (s->ntimestep - s->atimestep) * s->p.dt;
+    s->atimestep = s->ntimestep;
 }
 
 /* ------------------------------------------------------------------------ */

@@ -104,6 +104,8 @@ double or_time(const or_sim *s);
 void or_setup(or_sim *s);
 /* Verlet::run (verlet.cpp:264-390) for nsteps. */
 void or_run(or_sim *s, int nsteps);
+/* end of a run command: Update::update_time (the `time` bookkeeping) */
+void or_cleanup(or_sim *s);
 /* One force evaluation on the current list (force_clear + pair + walls +
  * gravity), shearupdate as given.  No list rebuild. */
 void or_compute_forces(or_sim *s, int shearupdate);

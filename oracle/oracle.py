@@ -31,7 +31,16 @@ class _Params(C.Structure):
         ("tangential_damping", C.c_int),
         ("limit_force", C.c_int),
         ("torsion_torque", C.c_int),
+        ("normal_model", C.c_int),
+        ("wall_normal_model", C.c_int),
+        ("wall_limit_force", C.c_int),
     ]
+
+
+NORMAL = {"hertz": 0, "hysteretic/nonlinear1": 1, "hysteretic/nonlinear2": 2}
+HYST_KEYS = ("LoadingStiffness", "UnloadingStiffness", "coefficientAdhesionStiffness",
+             "coefficientPlasticityDepth", "pullOffForce", "alphaCustom", "cinCustom",
+             "aoneCustom", "atwoCustom", "athreeCustom", "kcinCustom")
 
 
 def build():
@@ -71,6 +80,16 @@ def lib():
         L.or_get_pairs.argtypes = [C.c_void_p, ip, ip, ip, dp, ip]
         L.or_get_wall_history.argtypes = [C.c_void_p, dp]
         L.or_get_force_scale.argtypes = [C.c_void_p, dp, dp]
+        L.or_set_hysteretic.argtypes = [C.c_void_p, dp]
+        L.or_set_group.argtypes = [C.c_void_p, C.c_int, ip]
+        L.or_add_setforce.argtypes = [C.c_void_p, C.c_int, ip, dp]
+        L.or_add_aveforce.argtypes = [C.c_void_p, C.c_int, ip, C.c_double, C.c_double, C.c_double,
+                                      C.c_double, C.c_double]
+        L.or_unfix.argtypes = [C.c_void_p, C.c_int]
+        L.or_aveforce_vector.argtypes = [C.c_void_p, C.c_int, dp]
+        L.or_cleanup.argtypes = [C.c_void_p]
+        L.or_time.restype = C.c_double
+        L.or_time.argtypes = [C.c_void_p]
         _LIB = L
     return _LIB
 
@@ -108,7 +127,11 @@ class Oracle:
         p.neigh_check = int(spec.get("neigh_check", False))
         p.rolling_model = ROLL[spec.get("rolling", "off")]
         p.tangential_damping = int(spec.get("tangential_damping", True))
-        p.limit_force = int(spec.get("limit_force", False))
+        p.normal_model = NORMAL[spec.get("normal", "hertz")]
+        p.wall_normal_model = NORMAL[spec.get("wall_normal", "hertz")]
+        # limitForce defaults: off for hertz, on for hysteretic (model settings)
+        p.limit_force = int(spec.get("limit_force", p.normal_model != 0))
+        p.wall_limit_force = int(spec.get("wall_limit_force", p.wall_normal_model != 0))
         p.torsion_torque = int(spec.get("torsion_torque", False))
         self.h = self.L.or_create(C.byref(p))
         m = spec["materials"]
@@ -131,6 +154,11 @@ class Oracle:
                                      _dp(etar), float(m.get("krs", 0.0)))
         if rc != 0:
             raise ValueError("oracle: bad materials")
+        if "hysteretic" in m:
+            tabs = np.ascontiguousarray(np.stack([np.asarray(m["hysteretic"][k], dtype=np.float64).reshape(w, w)
+                                                  for k in HYST_KEYS]))
+            self._keep.append(tabs)
+            self.L.or_set_hysteretic(self.h, _dp(tabs))
         self.n = len(atoms["radius"])
         self.ntypes = nt
         a = {k: np.ascontiguousarray(atoms[k], dtype=np.float64)
@@ -215,3 +243,42 @@ class Oracle:
 
     def timestep(self):
         return self.L.or_timestep(self.h)
+
+    def time(self):
+        return self.L.or_time(self.h)
+
+    def cleanup(self):
+        """End of a `run` command (time bookkeeping)."""
+        self.L.or_cleanup(self.h)
+
+    def set_group(self, bit, members):
+        m = np.ascontiguousarray(members, dtype=np.int32)
+        self.L.or_set_group(self.h, int(bit), _ip(m))
+
+    def add_setforce(self, bit, values):
+        """values: 3 entries, None for NULL (untouched)."""
+        st = np.array([v is not None for v in values], dtype=np.int32)
+        va = np.array([0.0 if v is None else v for v in values])
+        return self.L.or_add_setforce(self.h, int(bit), _ip(st), _dp(va))
+
+    def add_aveforce(self, bit, xv, yv, c0, t0=0.0, c1=0.0):
+        """fix aveforce with z = c0 - (time - t0)*c1 (None x/y = NULL)."""
+        st = np.array([xv is not None, yv is not None, 1], dtype=np.int32)
+        return self.L.or_add_aveforce(self.h, int(bit), _ip(st), 0.0 if xv is None else xv,
+                                      0.0 if yv is None else yv, c0, t0, c1)
+
+    def add_plane(self, dim, pos, wtype):
+        fid = self.L.or_add_plane(self.h, int(dim), float(pos), int(wtype))
+        self.nplanes += 1
+        return fid
+
+    def add_freeze(self):
+        self.L.or_add_freeze(self.h)
+
+    def unfix(self, fid):
+        self.L.or_unfix(self.h, int(fid))
+
+    def aveforce_vector(self, fid):
+        out = np.zeros(3)
+        self.L.or_aveforce_vector(self.h, int(fid), _dp(out))
+        return out

@@ -263,7 +263,7 @@ def test_history_carry_rules():
     o.setup()
     o.run(50)
     h, fl = pair_hist(o, 1, 2)
-    assert fl == 1 and abs(h[1]) > 0          # shear accumulated, carried across rebuilds
+    assert fl != 0 and abs(h[1]) > 0          # shear accumulated, carried across rebuilds
     h21, _ = pair_hist(o, 2, 1)
     assert np.array_equal(h21, -h)            # partner copy is negated (newtonflag 1)
 
@@ -277,7 +277,7 @@ def test_history_dropped_when_apart_at_rebuild():
     st = o.state()
     assert st["x"][1, 0] - st["x"][0, 0] > 2 * r
     h, fl = pair_hist(o, 1, 2)
-    assert fl == 1 and abs(h[1]) > 0          # stale shear kept (cdf = 1: no surfacesClose)
+    assert fl != 0 and abs(h[1]) > 0          # stale shear kept (cdf = 1: no surfacesClose)
     o.rebuild()
     h, fl = pair_hist(o, 1, 2)
     assert fl == 0 and not h.any()            # not touching at rebuild: dropped
