@@ -26,6 +26,11 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per particle of one mixed contact pass on this workload, from the
+# FETCH_SIZE and WRITE_SIZE counter passes of rocprofv3 on k_force<true,4,0,0>
+# (FETCH_SIZE scaled x2.0, WRITE_SIZE x1.0 against a 256 MiB device copy):
+# profiles/pmc_traffic_k_force_column1M.md.  Re-measured when the kernel changes.
+MEASURED_TRAFFIC_BPP = 847.7
 
 
 def package():
@@ -117,7 +122,21 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-check", action="store_true", help="skip the mixed-vs-oracle force check")
     ap.add_argument("--only", choices=["mixed", "fp64"], default=None)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MB",
+                    help="only run a device-to-device copy of MB MiB, 10 times (for calibrating "
+                         "the FETCH_SIZE/WRITE_SIZE counters against known traffic)")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+
+        n = args.copy_probe * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(10):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_each_way": n * 4, "launches": 10}))
+        return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py: this build runs on one GPU; multi-GPU sharding is a design note, "
@@ -174,7 +193,9 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": MEASURED_TRAFFIC_BPP * n if main_prec == "mixed" else None,
+            "traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE passes, copy-calibrated "
+                              "(profiles/pmc_traffic_k_force_column1M.md)",
             "kernel": "k_force (contact pass)",
             "bytes_per_particle": bpp,
             "avg_launch_ms": avg_ms,

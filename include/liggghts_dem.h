@@ -49,9 +49,17 @@ typedef struct {
     int tangential_damping;       /* hertz setting, default 1                */
     int limit_force;              /* hertz setting, default 0                */
     int torsion_torque;           /* rolling setting, default 0              */
-    int precision;                /* DEM_FP64 or DEM_MIXED                   */
+    int precision;                /* DEM_FP64 or DEM_MIXED (Hertz chains)    */
     int device;                   /* HIP device ordinal                      */
+    int normal_model;             /* DEM_NORMAL_* of pair_style gran model   */
+    int wall_normal_model;        /* DEM_NORMAL_* of the fix wall/gran model */
+    int wall_limit_force;         /* limitForce of the wall model            */
 } dem_params;
+
+/* `model` keyword: hertz (normal_model_hertz.h), hysteretic/nonlinear1
+ * (normal_model_hysteretic_nonlinear1.h), hysteretic/nonlinear2 */
+enum { DEM_NORMAL_HERTZ = 0, DEM_NORMAL_HYST1 = 1, DEM_NORMAL_HYST2 = 2 };
+#define DEM_NHYST 11
 
 typedef struct dem_ctx dem_ctx;
 
@@ -77,6 +85,15 @@ int dem_material_tables(int ntypes, const double *Y, const double *nu, const dou
 int dem_set_materials(dem_ctx *ctx, int ntypes, const double *Yeff, const double *Geff,
                       const double *betaeff, const double *mu, const double *mur,
                       const double *etar, double krs);
+
+/* hysteretic/nonlinear1/2 parameters (normal_model_hysteretic_nonlinear1.h
+ * :105-133): CoeffRestLog = log(coefficientRestitution)
+ * (global_properties.cpp createCoeffRestLog) and DEM_NHYST tables, each
+ * (ntypes+1)^2, in the order LoadingStiffness, UnloadingStiffness,
+ * coefficientAdhesionStiffness, coefficientPlasticityDepth, pullOffForce,
+ * alphaCustom, cinCustom, aoneCustom, atwoCustom, athreeCustom, kcinCustom.
+ * Call after dem_set_materials. */
+int dem_set_hysteretic(dem_ctx *ctx, const double *coeff_rest_log, const double *tables);
 
 /* Particle state upload (AtomVecSphere arrays, atom_vec_sphere.cpp:125-160):
  * x, v, omega are n*3 row-major; radius, rmass per atom; type 1..ntypes;
@@ -121,8 +138,11 @@ int dem_rebuild(dem_ctx *ctx);
  * :100-116).  Pass NULL arrays to get only the count in *count. */
 int dem_get_pairs(dem_ctx *ctx, int64_t *count, int *tag_i, int *tag_j, int *shift,
                   double *hist, int *flag);
-/* History doubles per pair entry (3 shear + 3 rolling when on). */
+/* History doubles per pair entry: 12 hysteretic state (when that normal
+ * model is on) + 3 shear + 3 rolling torque (when rolling is on). */
 int dem_history_size(const dem_ctx *ctx);
+/* History doubles per (atom, plane) of the wall model. */
+int dem_wall_history_size(const dem_ctx *ctx);
 /* Contact-history injection (FixContactHistory::unpack_restart,
  * fix_contact_history.cpp:588-656): per-atom partner data used by the next
  * rebuild's re-seed, keyed by (tag_i, tag_j). */

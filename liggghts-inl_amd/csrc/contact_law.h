@@ -1,17 +1,19 @@
 // contact_law.h -- the sphere contact-model chain of pair_style gran
-// (surface default -> normal hertz -> tangential history -> rolling epsd*)
-// for one contact, as seen from atom i.
+// (surface default -> normal hertz | hysteretic/nonlinear1/2 -> tangential
+// history -> rolling epsd*) for one contact, as seen from atom i.
 //
 // FP64 form: same operations in the same order as the reference, so that
 // compiled without contraction (-ffp-contract=off) every per-contact value
-// rounds identically:
+// rounds identically (transcendentals aside: pow/exp/log come from the
+// device math library):
 //   surface_model_default.h:146-208, normal_model_hertz.h:205-405,
+//   normal_model_hysteretic_nonlinear{1,2}.h:149-420,
 //   tangential_model_history.h:135-425, rolling_model_epsd.h:96-335,
 //   rolling_model_epsd2.h:161-230, rolling_model_epsd3.h:178-262,
 //   fix_wall_gran_base.h:159-378 (wall variant).
 //
-// Mixed form (contact_law_mixed): overlap, separation and the history
-// integration stay FP64; the contact-law arithmetic runs in FP32.
+// Mixed form (contact_law_mixed, Hertz chains): overlap, separation and the
+// history integration stay FP64; the contact-law arithmetic runs in FP32.
 #pragma once
 #include "dem_device.h"
 
@@ -34,9 +36,154 @@ __device__ inline double tbl(const double *mat, int slot, int it, int jt)
     return mat[slot * kT1 * kT1 + it * kT1 + jt];
 }
 
+struct NormalOut {
+    double Fn, kn, kt, gammat;
+};
+
+// NormalModel<HYSTERETIC_NONLINEAR1/2>::surfacesIntersect; writes its
+// twelve history slots whatever shearupdate says, and sqrt(5/4) is the
+// reference's integer expression, sqrt(1) = 1.
+template <int NM>
+__device__ inline NormalOut normal_hysteretic(const double *mat, const ContactIn &c, double deltan, double vn,
+                                              int limitf, double *history)
+{
+    const int it = c.itype, jt = c.jtype;
+    const double meff = c.meff;
+    const double kn0 = tbl(mat, M_HK, it, jt);
+    const double Alpha_in = tbl(mat, M_HALPHA, it, jt);
+    const double Cin_in = tbl(mat, M_HCIN, it, jt);
+    const double A1_in = tbl(mat, M_HA1, it, jt);
+    const double A2_in = tbl(mat, M_HA2, it, jt);
+    const double A3_in = tbl(mat, M_HA3, it, jt);
+    const double kcin_in = tbl(mat, M_HKCIN, it, jt);
+    const double crl = tbl(mat, M_RESTLOG, it, jt);
+    const double k_c = kcin_in * A2_in;
+    const double kc = tbl(mat, M_HKN2KC, it, jt) * kn0;
+    const double f_0 = tbl(mat, M_HFADH, it, jt);
+    const double sqrt54 = 1.0;
+    const double fcc = 1.0;
+    double deltaMax;
+    if (deltan > history[0]) {
+        history[0] = deltan;
+        deltaMax = deltan;
+    } else {
+        deltaMax = history[0];
+    }
+    double deltaZero = history[1];
+    double k1 = history[2];
+    double deltaZero_old = history[3];
+    const double k1_old = history[4];
+    const double delta_old = history[5];
+    double deltaMin = history[6];
+    double betan = history[7];
+    double f0 = history[8];
+    double f0_old = history[9];
+    double k2 = A3_in * k1;
+    int tag_status;
+    if (deltan >= delta_old) {
+        if (delta_old == 0) tag_status = 1;
+        else tag_status = (vn > 0) ? 2 : 1;
+    } else {
+        tag_status = 2;
+    }
+    const double dampfac = 1. + (M_PI / crl) * (M_PI / crl);
+    double fHys, gamman, gammat;
+    if (tag_status == 1) {
+        if (deltaZero == 0) k1 = A2_in;
+        else k1 = history[2];
+        deltaMin = history[6];
+        f0_old = f0;
+        if (deltan <= deltaZero) {
+            if (NM == NM_HYST2) fHys = Cin_in * k2 * (exp(betan * (deltan - deltaZero)) - 1);
+            else if (deltan <= deltaMin) fHys = -k_c * deltan;
+            else fHys = betan * (deltan - deltaZero);
+        } else {
+            fHys = Alpha_in * k1 * pow(deltan - deltaZero, 2) + f0;
+        }
+        const double pw = (NM == NM_HYST2) ? 0.5 : 0.25;
+        gamman = sqrt54 * sqrt(4. * meff * Alpha_in * k1 / dampfac) * (pow(deltan, pw) + pow(deltaZero, pw));
+        gammat = gamman;
+        history[1] = deltaZero;
+        history[2] = k1;
+        history[3] = deltaZero;
+        history[4] = k1;
+        history[5] = deltan;
+        history[6] = deltaMin;
+        history[7] = betan;
+        history[8] = f0;
+        history[9] = f0_old;
+    } else {
+        if (NM == NM_HYST1) {
+            k2 = A3_in * k1_old;
+            deltaZero = (1 - k1_old / k2) * deltaMax;
+            const double beta = Alpha_in * k1_old * pow(deltaMax - deltaZero_old, 2) / k2 / (deltaMax - deltaZero);
+            deltaMin = beta * (k2 - k1_old) / (beta * k2 + k_c) * deltaMax;
+            k1 = deltaMax * A1_in + A2_in;
+            if (deltan >= deltaMin) {
+                betan = beta * k2;
+                if (deltan >= deltaZero) {
+                    fHys = beta * k2 * (deltan - deltaZero) + (deltan - deltaZero) * f0_old / (deltaMax - deltaZero);
+                    f0 = fHys - Alpha_in * k1 * pow(deltan - deltaZero, 2);
+                } else {
+                    fHys = beta * k2 * (deltan - deltaZero);
+                    f0 = 0;
+                }
+            } else {
+                fHys = -k_c * deltan;
+                f0 = 0;
+            }
+            gamman = sqrt54 * sqrt(4. * meff * Alpha_in * k1_old / dampfac) * (pow(deltan, 0.25));
+        } else {
+            k2 = A3_in * (A1_in * deltaMax + A2_in);
+            deltaZero = fcc * (1 - k1_old / k2) * deltaMax;
+            deltaZero_old = history[3];
+            betan = log(Alpha_in * k1_old / Cin_in / k2 * pow(deltaMax - deltaZero_old, 2) + 1)
+                    / (deltaMax - fcc * (1 - k1_old / k2) * deltaMax);
+            deltaMin = betan * (k2 - k1_old) / (betan * k2 + k_c) * deltaMax;
+            k1 = deltaMax * A1_in + A2_in;
+            if (deltan >= deltaZero) {
+                fHys = Cin_in * k2 * (exp(betan * (deltan - deltaZero)) - 1)
+                       + (deltan - deltaZero) * f0_old / (deltaMax - deltaZero);
+                f0 = fHys - Alpha_in * k1 * pow(deltan - deltaZero, 2);
+            } else {
+                fHys = Cin_in * k2 * (exp(betan * (deltan - deltaZero)) - 1);
+                f0 = 0;
+            }
+            gamman = 1 * 0.001 * sqrt54 * sqrt(4. * meff * Alpha_in * k1_old / dampfac) * (pow(deltan, -0.25));
+        }
+        gammat = gamman;
+        history[1] = deltaZero;
+        history[2] = k1;
+        history[3] = deltaZero_old;
+        history[4] = k1_old;
+        history[5] = deltan;
+        history[6] = deltaMin;
+        history[7] = betan;
+        history[8] = f0;
+    }
+    const double Fn_damping = -gamman * vn;
+    double Fn = fHys + Fn_damping + f_0;
+    if (limitf && (Fn < 0.0) && kc == 0 && f_0 == 0.0) Fn = 0.0;
+    history[10] = kc;
+    history[11] = f_0;
+    NormalOut o;
+    o.Fn = Fn;
+    o.kn = k1;
+    o.kt = k1;
+    o.gammat = gammat;
+    return o;
+}
+
+// History layout of a chain with normal model NM: hysteretic slots 0-11,
+// then tangential (3), then rolling (3).  Offsets are compile-time so the
+// per-contact history lives in registers.
+__host__ __device__ constexpr int chain_off_shear(int nm) { return nm == NM_HERTZ ? 0 : 12; }
+__host__ __device__ constexpr int chain_max_dnum(int nm) { return chain_off_shear(nm) + 6; }
+
 // Returns the i-side force and torque increments (ForceData i_forces).
-// h: dnum history doubles (shear[3], rolling torque[3]), updated in place.
-__device__ inline void contact_fp64(const Model &M, const double *mat, const ContactIn &c,
+// h: L.dnum history doubles in the chain's layout, updated in place.
+template <int NM>
+__device__ inline void contact_fp64(const Model &M, const Layout &L, const double *mat, const ContactIn &c,
                                     double *h, bool update, double F[3], double T[3])
 {
     const double SQRT_FIVE_OVER_SIX = 0.91287092917527685576161630466800355658790782499663875;
@@ -71,29 +218,38 @@ __device__ inline void contact_fp64(const Model &M, const double *mat, const Con
     const double vtr2 = vt2 - (dx * wr3 - dz * wr1);
     const double vtr3 = vt3 - (dy * wr1 - dx * wr2);
 
-    // ---- normal hertz
+    // ---- normal model
     const int it = c.itype, jt = c.jtype;
     const double reff = c.is_wall ? c.radi : (c.radi * c.radj / (c.radi + c.radj));
-    const double sqrtval = sqrt(reff * deltan);
-    const double Yeff = tbl(mat, M_YEFF, it, jt);
-    const double Geff = tbl(mat, M_GEFF, it, jt);
-    const double beta = tbl(mat, M_BETA, it, jt);
-    const double Sn = 2. * Yeff * sqrtval;
-    const double St = 8. * Geff * sqrtval;
-    const double kn = 4. / 3. * Yeff * sqrtval;
-    const double kt = St;
-    const double gamman = -2. * SQRT_FIVE_OVER_SIX * beta * sqrt(Sn * c.meff);
-    const double gammat = M.tdamp ? -2. * SQRT_FIVE_OVER_SIX * beta * sqrt(St * c.meff) : 0.0;
-    const double Fn_damping = -gamman * vn;
-    const double Fn_contact = kn * deltan;
-    double Fn = Fn_damping + Fn_contact;
-    if (M.limitf && (Fn < 0.0)) Fn = 0.0;
+    double Fn, kn, kt, gammat;
+    if (NM == NM_HERTZ) {
+        const double sqrtval = sqrt(reff * deltan);
+        const double Yeff = tbl(mat, M_YEFF, it, jt);
+        const double Geff = tbl(mat, M_GEFF, it, jt);
+        const double beta = tbl(mat, M_BETA, it, jt);
+        const double Sn = 2. * Yeff * sqrtval;
+        const double St = 8. * Geff * sqrtval;
+        kn = 4. / 3. * Yeff * sqrtval;
+        kt = St;
+        const double gamman = -2. * SQRT_FIVE_OVER_SIX * beta * sqrt(Sn * c.meff);
+        gammat = M.tdamp ? -2. * SQRT_FIVE_OVER_SIX * beta * sqrt(St * c.meff) : 0.0;
+        const double Fn_damping = -gamman * vn;
+        const double Fn_contact = kn * deltan;
+        Fn = Fn_damping + Fn_contact;
+        if (L.limitf && (Fn < 0.0)) Fn = 0.0;
+    } else {
+        const NormalOut o = normal_hysteretic<NM>(mat, c, deltan, vn, L.limitf, h);
+        Fn = o.Fn;
+        kn = o.kn;
+        kt = o.kt;
+        gammat = o.gammat;
+    }
     F[0] = Fn * enx;
     F[1] = Fn * eny;
     F[2] = Fn * enz;
 
     // ---- tangential history
-    double *shear = h;
+    double *shear = h + chain_off_shear(NM);
     if (update) {
         const double dt = M.dt;
         shear[0] += vtr1 * dt;
@@ -141,7 +297,7 @@ __device__ inline void contact_fp64(const Model &M, const double *mat, const Con
     T[2] = -cri * tor3;
 
     // ---- rolling friction
-    if (M.rolling) {
+    if (M.rolling && L.off_roll >= 0) {
         double w1, w2, w3, r_inertia;
         if (c.is_wall) {
             w1 = wr1; w2 = wr2; w3 = wr3;
@@ -168,7 +324,7 @@ __device__ inline void contact_fp64(const Model &M, const double *mat, const Con
         else if (M.rolling == 2) kr = kt * reff * reff;
         else kr = M.krs * kn * rmu * rmu * reff * reff;
         const double dtkr = M.dt * kr;
-        double *ch = h + 3;
+        double *ch = h + chain_off_shear(NM) + 3;
         double rt0 = ch[0] + dtkr * wt0;
         double rt1 = ch[1] + dtkr * wt1;
         double rt2 = ch[2] + dtkr * wt2;
@@ -193,8 +349,24 @@ __device__ inline void contact_fp64(const Model &M, const double *mat, const Con
     }
 }
 
+// surfacesClose of a chain: hysteretic zeroes its ten state slots and clears
+// its flag, tangential and rolling zero their history
+// (normal_model_hysteretic_nonlinear1.h:444-458,
+// tangential_model_history.h:428-440, rolling_model_epsd.h:222-229).
+// Returns true if the chain's contact flag stays set (hertz keeps it).
+__device__ inline bool contact_close(const Layout &L, double *hp)
+{
+    if (L.off_norm >= 0)
+        for (int q = 0; q < 10; q++) hp[L.off_norm + q] = 0.0;
+    for (int q = 0; q < 3; q++) hp[L.off_shear + q] = 0.0;
+    if (L.off_roll >= 0)
+        for (int q = 0; q < 3; q++) hp[L.off_roll + q] = 0.0;
+    return L.normal == NM_HERTZ;
+}
+
 // Mixed precision: the overlap deltan (a small difference of large numbers)
 // and the history integration are FP64; everything else of the law is FP32.
+// Hertz chains only (shear at slot 0, rolling torque at slot 3).
 __device__ inline void contact_mixed(const Model &M, const double *mat, const ContactIn &c,
                                      double *h, bool update, double F[3], double T[3])
 {
@@ -246,7 +418,7 @@ __device__ inline void contact_mixed(const Model &M, const double *mat, const Co
     const float gamman = -2.f * SQRT_FIVE_OVER_SIX * beta * sqrtf(Sn * meff);
     const float gammat = M.tdamp ? -2.f * SQRT_FIVE_OVER_SIX * beta * sqrtf(St * meff) : 0.f;
     float Fn = -gamman * vn + kn * deltan;
-    if (M.limitf && (Fn < 0.f)) Fn = 0.f;
+    if ((c.is_wall ? M.lw.limitf : M.lp.limitf) && (Fn < 0.f)) Fn = 0.f;
 
     double *shear = h;
     if (update) {

@@ -19,14 +19,35 @@ namespace dem {
 constexpr int kMaxTypes = 8;
 constexpr int kT1 = kMaxTypes + 1;
 constexpr int kMaxPlanes = 8;
-constexpr int kMaxDnum = 6;
+constexpr int kMaxDnum = 18;   // hysteretic 12 + shear 3 + rolling 3
 constexpr uint32_t kJMask = (1u << 26) - 1u;
 constexpr uint32_t kFlagBit = 1u << 31;
 constexpr int kNoShift = 13;
 constexpr int kBlock = 256;
 
-// material table slots, each kT1*kT1 doubles, index [itype*kT1 + jtype]
-enum { M_YEFF = 0, M_GEFF, M_BETA, M_MU, M_MUR, M_ETAR, M_COUNT };
+// material table slots, each kT1*kT1 doubles, index [itype*kT1 + jtype];
+// the hysteretic/nonlinear* parameters follow in the order of
+// normal_model_hysteretic_nonlinear1.h:105-133
+enum {
+    M_YEFF = 0, M_GEFF, M_BETA, M_MU, M_MUR, M_ETAR, M_RESTLOG,
+    M_HK, M_HKN2K1, M_HKN2KC, M_HPHIF, M_HFADH, M_HALPHA, M_HCIN, M_HA1, M_HA2, M_HA3, M_HKCIN,
+    M_COUNT
+};
+constexpr int kNHyst = 11;
+
+// normal model of a contact chain
+enum { NM_HERTZ = 0, NM_HYST1 = 1, NM_HYST2 = 2 };
+
+// history layout of a contact chain (slots in registration order:
+// normal model, tangential history, rolling)
+struct Layout {
+    int normal;      // NM_*
+    int limitf;      // limitForce
+    int off_norm;    // hysteretic: 12 slots, else -1
+    int off_shear;
+    int off_roll;    // -1 without rolling friction
+    int dnum;
+};
 
 struct Planes {
     int n;
@@ -47,9 +68,10 @@ struct Model {
     int freeze_on;
     int rolling;         // 0 off, 1 epsd, 2 epsd2, 3 epsd3
     int tdamp;           // tangential_damping
-    int limitf;          // limitForce
     int torsion;         // rolling torsionTorque
     int dnum;            // history doubles per pair
+    int wdnum;           // history doubles per (atom, plane)
+    Layout lp, lw;       // pair and wall chains
 };
 
 struct Grid {

@@ -173,6 +173,32 @@ extern "C" const char *dem_last_error(const dem_ctx *ctx)
     return ctx ? ctx->err.c_str() : g_last_error.c_str();
 }
 
+// History slots in the reference's registration order (contact_models.h:
+// 150-157): hysteretic normal state (12, normal_model_hysteretic_nonlinear1.h
+// :84-97), shear (3, tangential_model_history.h:79-81), rolling torque (3,
+// rolling_model_epsd.h:71-73).
+static Layout chain_layout(int normal, int limitf, int rolling)
+{
+    Layout L;
+    int k = 0;
+    L.normal = normal;
+    L.limitf = limitf;
+    L.off_norm = -1;
+    if (normal != NM_HERTZ) {
+        L.off_norm = k;
+        k += 12;
+    }
+    L.off_shear = k;
+    k += 3;
+    L.off_roll = -1;
+    if (rolling) {
+        L.off_roll = k;
+        k += 3;
+    }
+    L.dnum = k;
+    return L;
+}
+
 extern "C" dem_ctx *dem_ctx_create(const dem_params *p)
 {
     if (!p) {
@@ -180,7 +206,8 @@ extern "C" dem_ctx *dem_ctx_create(const dem_params *p)
         return nullptr;
     }
     if (!(p->dt > 0.0) || !(p->skin >= 0.0) || !(p->cdf >= 1.0) || p->neigh_every < 1
-        || p->rolling_model < 0 || p->rolling_model > 3 || p->precision < 0 || p->precision > 1) {
+        || p->rolling_model < 0 || p->rolling_model > 3 || p->precision < 0 || p->precision > 1
+        || p->normal_model < 0 || p->normal_model > 2 || p->wall_normal_model < 0 || p->wall_normal_model > 2) {
         g_last_error = "dem_ctx_create: invalid parameters";
         return nullptr;
     }
@@ -221,9 +248,11 @@ extern "C" dem_ctx *dem_ctx_create(const dem_params *p)
     for (int d = 0; d < 3; d++) M.prd[d] = p->boxhi[d] - p->boxlo[d];
     M.rolling = p->rolling_model;
     M.tdamp = p->tangential_damping;
-    M.limitf = p->limit_force;
     M.torsion = p->torsion_torque;
-    M.dnum = 3 + (p->rolling_model ? 3 : 0);
+    M.lp = chain_layout(p->normal_model, p->limit_force, p->rolling_model);
+    M.lw = chain_layout(p->wall_normal_model, p->wall_limit_force, p->rolling_model);
+    M.dnum = M.lp.dnum;
+    M.wdnum = M.lw.dnum;
     ctx->P.n = 0;
     return ctx;
 }
@@ -293,10 +322,34 @@ extern "C" int dem_set_materials(dem_ctx *ctx, int ntypes, const double *Yeff, c
         for (int i = 0; i < w; i++)
             for (int j = 0; j < w; j++) h[s * kT1 * kT1 + i * kT1 + j] = src[s][i * w + j];
     }
+    // keep the hysteretic tables if they were set before
+    std::vector<double> old(M_COUNT * kT1 * kT1, 0.0);
+    if (ctx->mat_set)
+        HIPCHK(hipMemcpy(old.data(), ctx->d_mat, old.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (size_t k = (size_t)M_RESTLOG * kT1 * kT1; k < h.size(); k++) h[k] = old[k];
     HIPCHK(hipMemcpy(ctx->d_mat, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
     ctx->ntypes = ntypes;
     ctx->M.krs = krs;
     ctx->mat_set = true;
+    return DEM_OK;
+}
+
+extern "C" int dem_set_hysteretic(dem_ctx *ctx, const double *coeff_rest_log, const double *tables)
+{
+    if (!ctx) return fail(nullptr, DEM_ERR_ARG, "null context");
+    if (!ctx->mat_set || !coeff_rest_log || !tables)
+        return fail(ctx, DEM_ERR_ARG, "dem_set_hysteretic: set materials first, non-null tables");
+    hipSetDevice(ctx->p.device);
+    const int w = ctx->ntypes + 1;
+    std::vector<double> h(M_COUNT * kT1 * kT1, 0.0);
+    HIPCHK(hipMemcpy(h.data(), ctx->d_mat, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int i = 0; i < w; i++)
+        for (int j = 0; j < w; j++) {
+            h[M_RESTLOG * kT1 * kT1 + i * kT1 + j] = coeff_rest_log[i * w + j];
+            for (int t = 0; t < kNHyst; t++)
+                h[(M_HK + t) * kT1 * kT1 + i * kT1 + j] = tables[(size_t)t * w * w + i * w + j];
+        }
+    HIPCHK(hipMemcpy(ctx->d_mat, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
     return DEM_OK;
 }
 
@@ -415,8 +468,7 @@ extern "C" int dem_upload_state(dem_ctx *ctx, int n, const double *x, const doub
         return fail(ctx, DEM_ERR_ARG, "dem_upload_state: bad arguments");
     if (!ctx->mat_set) return fail(ctx, DEM_ERR_ARG, "dem_upload_state: set materials first");
     hipSetDevice(ctx->p.device);
-    const int dnum = ctx->M.dnum;
-    const int wdn = ctx->P.n * dnum;
+    const int wdn = ctx->P.n * ctx->M.wdnum;
     std::vector<double4> hx(n), hv(n), hw(n);
     std::vector<int> ho(n);
     std::vector<int4> hi(n, make_int4(0, 0, 0, 0));
@@ -521,7 +573,7 @@ static int rebuild(dem_ctx *ctx)
     pa.xhold = ctx->xhold;
     pa.wmask = ctx->wmask;
     pa.n = n;
-    pa.wdn = ctx->P.n * ctx->M.dnum;
+    pa.wdn = ctx->P.n * ctx->M.wdnum;
     pa.P = ctx->P;
     pa.skin = ctx->p.skin;
     launch_permute(pa, s);
@@ -729,6 +781,11 @@ extern "C" int dem_history_size(const dem_ctx *ctx)
     return ctx ? ctx->M.dnum : 0;
 }
 
+extern "C" int dem_wall_history_size(const dem_ctx *ctx)
+{
+    return ctx ? ctx->M.wdnum : 0;
+}
+
 extern "C" int dem_get_pairs(dem_ctx *ctx, int64_t *count, int *tag_i, int *tag_j, int *shift,
                              double *hist, int *flag)
 {
@@ -816,7 +873,7 @@ extern "C" int dem_get_wall_history(dem_ctx *ctx, double *hist)
 {
     if (!ctx || !hist) return fail(ctx, DEM_ERR_ARG, "dem_get_wall_history: bad arguments");
     hipSetDevice(ctx->p.device);
-    const int n = ctx->n, c = ctx->cur, wdn = ctx->P.n * ctx->M.dnum;
+    const int n = ctx->n, c = ctx->cur, wdn = ctx->P.n * ctx->M.wdnum;
     if (n == 0 || wdn == 0) return DEM_OK;
     std::vector<double> h((size_t)n * wdn);
     std::vector<int> ho(n);
@@ -832,7 +889,7 @@ extern "C" int dem_set_wall_history(dem_ctx *ctx, const double *hist)
 {
     if (!ctx || !hist) return fail(ctx, DEM_ERR_ARG, "dem_set_wall_history: bad arguments");
     hipSetDevice(ctx->p.device);
-    const int n = ctx->n, c = ctx->cur, wdn = ctx->P.n * ctx->M.dnum;
+    const int n = ctx->n, c = ctx->cur, wdn = ctx->P.n * ctx->M.wdnum;
     if (n == 0 || wdn == 0) return DEM_OK;
     std::vector<double> h((size_t)n * wdn);
     std::vector<int> ho(n);

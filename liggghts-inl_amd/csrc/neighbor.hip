@@ -21,6 +21,8 @@
 // row holds a flagged entry with the same partner tag (neigh_gran.cpp:592-612).
 #include "kernels.h"
 
+#include <algorithm>
+
 namespace dem {
 
 __device__ inline int cell_coord(double x, int d, const Grid &G)
@@ -280,12 +282,19 @@ __global__ void __launch_bounds__(kBlock) k_neigh(NeighArgs a)
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 
-    int ob = 0, oe = 0;
+    // the particle's previous row, one entry per lane: partner tag of every
+    // flagged entry (0 = cannot match; tags are positive)
+    int ob = 0, oe = 0, otag_l = 0;
     if (FILL && a.have_old) {
         const int o = a.perm[p];
         ob = a.onstart[o];
         oe = a.onstart[o + 1];
+        if (lane < oe - ob) {
+            const uint32_t ow = a.onj[ob + lane];
+            if (ow & kFlagBit) otag_l = a.otag[ow & kJMask];
+        }
     }
+    const int on64 = min(oe - ob, 64);
     const int ebase = FILL ? a.nstart[p] : 0;
     int written = 0;
     for (int c0 = 0; c0 < total; c0 += 64) {
@@ -319,29 +328,57 @@ __global__ void __launch_bounds__(kBlock) k_neigh(NeighArgs a)
             }
         }
         const uint64_t m = __ballot(pass);
-        if (FILL && pass) {
-            const int e = ebase + written + __popcll(m & ((1ull << lane) - 1ull));
-            uint32_t w = (uint32_t)q | ((uint32_t)qcode << 26);
-            if (a.have_old && touch) {
-                const int tq = a.tag[q];
-                for (int oe2 = ob; oe2 < oe; oe2++) {
+        if (FILL) {
+            // first flagged entry of the previous row with the partner's tag
+            // (the search of neigh_gran.cpp:596-598), rows held in lanes
+            const bool want = pass && touch && a.have_old;
+            const int tq = want ? a.tag[q] : -1;
+            int found = -1;
+            for (int k = 0; k < on64; k++) {
+                const int t = __shfl(otag_l, k, 64);
+                if (found < 0 && t == tq) found = k;
+            }
+            if (want && found < 0)
+                for (int oe2 = ob + 64; oe2 < oe; oe2++) {
                     const uint32_t ow = a.onj[oe2];
-                    if (!(ow & kFlagBit)) continue;
-                    if (a.otag[ow & kJMask] != tq) continue;
-                    const double *oh = a.ohist + (size_t)oe2 * a.dnum;
+                    if ((ow & kFlagBit) && a.otag[ow & kJMask] == tq) {
+                        found = oe2 - ob;
+                        break;
+                    }
+                }
+            if (pass) {
+                const int e = ebase + written + __popcll(m & ((1ull << lane) - 1ull));
+                uint32_t w = (uint32_t)q | ((uint32_t)qcode << 26);
+                if (found >= 0) {
+                    const double *oh = a.ohist + (size_t)(ob + found) * a.dnum;
                     double *h = a.hist + (size_t)e * a.dnum;
                     for (int k = 0; k < a.dnum; k++) h[k] = oh[k];
                     w |= kFlagBit;
-                    break;
                 }
+                a.nj[e] = w;
             }
-            a.nj[e] = w;
         }
         written += __popcll(m);
     }
-    if (!FILL && lane == 0) {
-        a.count[p] = written;
-        atomicMax(a.maxcount, written);
+    if (!FILL && lane == 0) a.count[p] = written;
+}
+
+// Longest row, for the neigh_modify one check: a grid-stride max over the
+// row lengths with one atomic per block (a per-row atomic on one address
+// serialises a million times per rebuild).
+constexpr int kMaxBlocks = 512;
+
+__global__ void __launch_bounds__(kBlock) k_row_max(const int *count, int n, int *maxcount)
+{
+    __shared__ int s_m[kBlock / 64];
+    int m = 0;
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) m = max(m, count[i]);
+    for (int o = 32; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o, 64));
+    if ((threadIdx.x & 63) == 0) s_m[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kBlock / 64; w++) m = max(m, s_m[w]);
+        atomicMax(maxcount, m);
     }
 }
 
@@ -354,6 +391,8 @@ void launch_neigh_count(const NeighArgs &a, hipStream_t s)
 {
     if (a.n == 0) return;
     k_neigh<false><<<wave_blocks(a.n), kBlock, 0, s>>>(a);
+    const int nb = (int)std::min<long long>(kMaxBlocks, ((long long)a.n + kBlock - 1) / kBlock);
+    k_row_max<<<nb, kBlock, 0, s>>>(a.count, a.n, a.maxcount);
 }
 
 void launch_neigh_fill(const NeighArgs &a, hipStream_t s)

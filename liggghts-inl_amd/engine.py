@@ -41,7 +41,16 @@ class Params(C.Structure):
         ("torsion_torque", C.c_int),
         ("precision", C.c_int),
         ("device", C.c_int),
+        ("normal_model", C.c_int),
+        ("wall_normal_model", C.c_int),
+        ("wall_limit_force", C.c_int),
     ]
+
+
+NORMAL = {"hertz": 0, "hysteretic/nonlinear1": 1, "hysteretic/nonlinear2": 2}
+HYST_KEYS = ("LoadingStiffness", "UnloadingStiffness", "coefficientAdhesionStiffness",
+             "coefficientPlasticityDepth", "pullOffForce", "alphaCustom", "cinCustom",
+             "aoneCustom", "atwoCustom", "athreeCustom", "kcinCustom")
 
 
 def library_path():
@@ -83,6 +92,8 @@ def library():
         "dem_rebuild": (C.c_int, [vp]),
         "dem_get_pairs": (C.c_int, [vp, i64p, ip, ip, ip, dp, ip]),
         "dem_history_size": (C.c_int, [vp]),
+        "dem_wall_history_size": (C.c_int, [vp]),
+        "dem_set_hysteretic": (C.c_int, [vp, dp, dp]),
         "dem_set_history": (C.c_int, [vp, C.c_int64, ip, ip, dp]),
         "dem_get_wall_history": (C.c_int, [vp, dp]),
         "dem_set_wall_history": (C.c_int, [vp, dp]),
@@ -149,7 +160,11 @@ class Simulation:
         p.neigh_one = spec.get("neigh_one", 2000)
         p.rolling_model = ROLL[spec.get("rolling", "off")]
         p.tangential_damping = int(spec.get("tangential_damping", True))
-        p.limit_force = int(spec.get("limit_force", False))
+        p.normal_model = NORMAL[spec.get("normal", "hertz")]
+        p.wall_normal_model = NORMAL[spec.get("wall_normal", "hertz")]
+        # limitForce defaults: off for hertz, on for the hysteretic models
+        p.limit_force = int(spec.get("limit_force", p.normal_model != 0))
+        p.wall_limit_force = int(spec.get("wall_limit_force", p.wall_normal_model != 0))
         p.torsion_torque = int(spec.get("torsion_torque", False))
         p.precision = DEM_MIXED if precision == "mixed" else DEM_FP64
         p.device = device
@@ -173,6 +188,11 @@ class Simulation:
         etar = np.ascontiguousarray(m.get("etar", np.zeros((w, w))), dtype=np.float64).reshape(w, w)
         self._chk(self.L.dem_set_materials(self.h, nt, _dp(Yeff), _dp(Geff), _dp(beta), _dp(mu),
                                            _dp(mur), _dp(etar), float(m.get("krs", 0.0))))
+        if "hysteretic" in m:
+            crl = np.ascontiguousarray(np.log(np.asarray(m["e"], dtype=np.float64).reshape(w, w)))
+            tabs = np.ascontiguousarray(np.stack([np.asarray(m["hysteretic"][k], dtype=np.float64).reshape(w, w)
+                                                  for k in HYST_KEYS]))
+            self._chk(self.L.dem_set_hysteretic(self.h, _dp(crl), _dp(tabs)))
         self.upload(atoms)
 
     def _chk(self, rc):
@@ -244,15 +264,19 @@ class Simulation:
         h = np.ascontiguousarray(hist, dtype=np.float64).reshape(len(ti), self.dnum)
         self._chk(self.L.dem_set_history(self.h, len(ti), _ip(ti), _ip(tj), _dp(h)))
 
+    @property
+    def wall_dnum(self):
+        return self.L.dem_wall_history_size(self.h)
+
     def wall_history(self):
-        h = np.zeros((self.n, max(self.nplanes, 1), self.dnum))
+        h = np.zeros((self.n, max(self.nplanes, 1), self.wall_dnum))
         if self.nplanes:
             self._chk(self.L.dem_get_wall_history(self.h, _dp(h)))
         return h
 
     def set_wall_history(self, hist):
         if self.nplanes:
-            h = np.ascontiguousarray(hist, dtype=np.float64).reshape(self.n, self.nplanes, self.dnum)
+            h = np.ascontiguousarray(hist, dtype=np.float64).reshape(self.n, self.nplanes, self.wall_dnum)
             self._chk(self.L.dem_set_wall_history(self.h, _dp(h)))
 
     def timing(self, on=True):

@@ -25,7 +25,7 @@
 #define MAXDNUM 8
 #define T1 (OR_MAXTYPES + 1)
 
-enum { OP_WALL = 1, OP_GRAVITY = 2, OP_FREEZE = 3, OP_SETFORCE = 4, OP_AVEFORCE = 5 };
+enum { OP_WALL = 1, OP_GRAVITY = 2, OP_FREEZE = 3, OP_SETFORCE = 4, OP_AVEFORCE = 5, OP_MESH = 6 };
 
 /* contact_model_constants.h:64-67 */
 enum { CONTACT_NORMAL_MODEL = 1, CONTACT_COHESION_MODEL = 2, CONTACT_TANGENTIAL_MODEL = 4,
@@ -57,6 +57,29 @@ typedef struct {
     double c0, t0, c1;
     double foriginal[3];
 } op_t;
+
+#define MESH_NEIGH_MAX 5   /* SurfaceMesh<3,5> (tri_mesh.h:67) */
+
+/* one triangle-mesh wall: fix mesh/surface + fix wall/gran ... mesh */
+typedef struct {
+    int ntri;
+    double *node, *center, *rbound, *evec, *elen, *enorm, *snorm;
+    int *obtuse, *nneigh, *neigh;
+    unsigned char *eact, *cact;
+    double curvature, precision;
+    int wtype;
+    int moving;
+    double vel[3];
+    /* triangle -> candidate particles, rebuilt with the neighbour list */
+    int *tstart, *tlist;
+    /* per-particle contact slots (FixContactHistoryMesh) */
+    int *nneighs, *npartner, *sstart;
+    int scap;
+    int *partner;
+    unsigned char *keep, *inter;
+    double *hist;
+    double force[3];
+} mesh_t;
 
 struct or_sim {
     or_params p;
@@ -96,6 +119,9 @@ struct or_sim {
     double ppos[OR_MAXPLANES];
     unsigned char *pin;     /* n * nplanes: atom in the wall's neighbour list */
     double *phist_w;        /* n * nplanes * dnum_w */
+
+    int nmesh;
+    mesh_t mesh[4];
 
     int nops;
     op_t ops[32];
@@ -168,6 +194,14 @@ void or_destroy(or_sim *s)
     if (!s) return;
     free_atoms(s);
     free(s->ent); free(s->hist);
+    for (int k = 0; k < s->nmesh; k++) {
+        mesh_t *m = &s->mesh[k];
+        free(m->node); free(m->center); free(m->rbound); free(m->evec); free(m->elen);
+        free(m->enorm); free(m->snorm); free(m->obtuse); free(m->nneigh); free(m->neigh);
+        free(m->eact); free(m->cact); free(m->tstart); free(m->tlist); free(m->nneighs);
+        free(m->npartner); free(m->sstart); free(m->partner); free(m->keep); free(m->inter);
+        free(m->hist);
+    }
     free(s);
 }
 
@@ -592,9 +626,15 @@ static void build_wall_lists(or_sim *s)
     }
 }
 
+static void mesh_sort_contacts(or_sim *s, mesh_t *m);
+static void mesh_rebuild(or_sim *s, mesh_t *m);
+static void mesh_move(or_sim *s, mesh_t *m);
+static void mesh_post_force(or_sim *s, mesh_t *m, int shearupdate);
+
 void or_rebuild(or_sim *s)
 {
     pre_exchange(s);
+    for (int k = 0; k < s->nmesh; k++) mesh_sort_contacts(s, &s->mesh[k]);
     domain_pbc(s);
     build_list(s);
     memcpy(s->xhold, s->x, (size_t)s->n * 3 * sizeof(double));
@@ -1170,6 +1210,710 @@ static void wall_plane(or_sim *s, int k, int shearupdate)
     }
 }
 
+/* ------------------------------------------------------------------------ */
+/* Triangle meshes.  Topology: SurfaceMesh (surface_mesh_I.h:302-470,       */
+/* 474-580, 874-1240) and MultiNodeMesh (multi_node_mesh_I.h:140-175,       */
+/* 246-350).  Contact: TriMesh::resolveTriSphereContactBary and friends     */
+/* (tri_mesh_I.h:65-320).  History: FixContactHistoryMesh                   */
+/* (fix_contact_history_mesh_I.h:51-200, fix_contact_history_mesh.cpp:      */
+/* 250-520).  Force: FixWallGran::post_force_mesh (fix_wall_gran.cpp:803-980) */
+
+static inline double dot3(const double *a, const double *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+static inline double mag3(const double *a) { return sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]); }
+static inline void cross3(const double *a, const double *b, double *c)
+{
+    c[0] = a[1] * b[2] - a[2] * b[1];
+    c[1] = a[2] * b[0] - a[0] * b[2];
+    c[2] = a[0] * b[1] - a[1] * b[0];
+}
+static inline void sdiv3(double *v, double s)
+{
+    const double sinv = 1. / s;
+    v[0] = sinv * v[0]; v[1] = sinv * v[1]; v[2] = sinv * v[2];
+}
+
+/* MathExtraLiggghts::compDouble (math_extra_liggghts.h:560-571) */
+static int comp_double(double a, double b, double prec)
+{
+    if (a == b) return 1;
+    if (b == 0) return a < prec && a > -prec;
+    const double x = (a - b);
+    return x < prec && x > -prec;
+}
+
+static int nodes_equal(const double *a, const double *b, double prec)
+{
+    for (int i = 0; i < 3; i++)
+        if (!comp_double(a[i], b[i], prec)) return 0;
+    return 1;
+}
+
+#define MN(m, t, k) (&(m)->node[9 * (t) + 3 * (k)])
+#define EV(m, t, k) (&(m)->evec[9 * (t) + 3 * (k)])
+#define EN(m, t, k) (&(m)->enorm[9 * (t) + 3 * (k)])
+
+static int contains_node(const mesh_t *m, int t, const double *p)
+{
+    for (int k = 0; k < 3; k++) {
+        const double *n = MN(m, t, k);
+        if (comp_double(n[0], p[0], m->precision) && comp_double(n[1], p[1], m->precision)
+            && comp_double(n[2], p[2], m->precision))
+            return k;
+    }
+    return -1;
+}
+
+/* share2Nodes (multi_node_mesh_I.h:281-325) */
+static int share2(const mesh_t *m, int a, int b, int *ia1, int *ja1, int *ia2, int *ja2)
+{
+    double d[3];
+    for (int q = 0; q < 3; q++) d[q] = m->center[3 * a + q] - m->center[3 * b + q];
+    const double radsum = m->rbound[a] + m->rbound[b];
+    if (dot3(d, d) > radsum * radsum) return 0;
+    int ns = 0;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+            if (nodes_equal(MN(m, a, i), MN(m, b, j), m->precision)) {
+                if (ns == 0) { *ia1 = i; *ja1 = j; }
+                else { *ia2 = i; *ja2 = j; return 1; }
+                ns++;
+            }
+    return 0;
+}
+
+/* nSharedNodes (multi_node_mesh_I.h:328-346) */
+static int nshared(const mesh_t *m, int a, int b)
+{
+    double d[3];
+    for (int q = 0; q < 3; q++) d[q] = m->center[3 * a + q] - m->center[3 * b + q];
+    const double radsum = m->rbound[a] + m->rbound[b];
+    if (dot3(d, d) > radsum * radsum) return 0;
+    int ns = 0;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+            if (nodes_equal(MN(m, a, i), MN(m, b, j), m->precision)) ns++;
+    return ns;
+}
+
+/* areCoplanarNodeNeighs (surface_mesh_I.h:922-962), ids = indices */
+static int coplanar_node_neighs(const mesh_t *m, int a, int b)
+{
+    int neighs = 0;
+    for (int i = 0; i < m->nneigh[a] && i < MESH_NEIGH_MAX; i++)
+        if (m->neigh[MESH_NEIGH_MAX * a + i] == b) neighs = 1;
+    if (!neighs && nshared(m, a, b) == 0) return 0;
+    const double dot = dot3(&m->snorm[3 * a], &m->snorm[3 * b]);
+    return fabs(dot) > m->curvature;
+}
+
+static void handle_shared_edge(mesh_t *m, int i, int ie, int j, int je, int coplanar)
+{
+    if (m->nneigh[i] < MESH_NEIGH_MAX) m->neigh[MESH_NEIGH_MAX * i + m->nneigh[i]] = j;
+    if (m->nneigh[j] < MESH_NEIGH_MAX) m->neigh[MESH_NEIGH_MAX * j + m->nneigh[j]] = i;
+    m->nneigh[i]++;
+    m->nneigh[j]++;
+    int overlap = 0;
+    if (coplanar) {
+        /* coplanarNeighsOverlap (surface_mesh_I.h:965-1004) */
+        double vi[3], vj[3], pref[3], en[3];
+        for (int q = 0; q < 3; q++) {
+            pref[q] = MN(m, i, ie)[q];
+            en[q] = EN(m, i, ie)[q];
+        }
+        for (int q = 0; q < 3; q++) {
+            vi[q] = MN(m, i, (ie + 2) % 3)[q] - pref[q];
+            vj[q] = MN(m, j, (je + 2) % 3)[q] - pref[q];
+        }
+        overlap = dot3(vi, en) * dot3(vj, en) > 0.;
+    }
+    if (!coplanar || overlap) {
+        if (i < j) { m->eact[3 * i + ie] = 0; m->eact[3 * j + je] = 1; }
+        else { m->eact[3 * i + ie] = 1; m->eact[3 * j + je] = 0; }
+    } else {
+        m->eact[3 * i + ie] = 0;
+        m->eact[3 * j + je] = 0;
+    }
+}
+
+typedef struct {
+    int nvis, nhas;
+    int *vis, *has;
+    double *elist, *eend;
+    int any_active;
+} corner_walk_t;
+
+/* checkNodeRecursive (surface_mesh_I.h:1188-1236) */
+static void check_node(const mesh_t *m, int t, const double *p, corner_walk_t *w)
+{
+    for (int i = 0; i < w->nvis; i++)
+        if (w->vis[i] == t) return;
+    w->vis[w->nvis++] = t;
+    const int k = contains_node(m, t, p);
+    if (k < 0) return;
+    const int ne = 2 * w->nhas;
+    w->has[w->nhas++] = t;
+    const int km = (k - 1 + 3) % 3;
+    for (int q = 0; q < 3; q++) {
+        w->elist[3 * ne + q] = EV(m, t, k)[q];
+        w->elist[3 * (ne + 1) + q] = EV(m, t, km)[q];
+        w->eend[3 * ne + q] = MN(m, t, (k + 1) % 3)[q];
+        w->eend[3 * (ne + 1) + q] = MN(m, t, km)[q];
+    }
+    if (m->eact[3 * t + k]) w->any_active = 1;
+    else if (m->eact[3 * t + km]) w->any_active = 1;
+    for (int in = 0; in < m->nneigh[t] && in < MESH_NEIGH_MAX; in++) {
+        const int nb = m->neigh[MESH_NEIGH_MAX * t + in];
+        if (nb < 0) return;
+        check_node(m, nb, p, w);
+    }
+}
+
+/* handleCorner (surface_mesh_I.h:1139-1185); a node outside the box keeps
+ * its initial active state, as there */
+static void handle_corner(const or_sim *s, mesh_t *m, int t, int k, corner_walk_t *w)
+{
+    double p[3];
+    for (int q = 0; q < 3; q++) p[q] = MN(m, t, k)[q];
+    w->nvis = w->nhas = 0;
+    w->any_active = 0;
+    check_node(m, t, p, w);
+    for (int d = 0; d < 3; d++)
+        if (p[d] < s->p.boxlo[d] || p[d] >= s->p.boxhi[d]) return;
+    const int ne = 2 * w->nhas;
+    int maxid = -1;
+    for (int i = 0; i < w->nhas; i++)
+        if (w->has[i] > maxid) maxid = w->has[i];
+    int colinear = 0;
+    for (int i = 0; i < ne; i++)
+        for (int j = i + 1; j < ne; j++)
+            if (fabs(dot3(&w->elist[3 * i], &w->elist[3 * j])) > m->curvature
+                && !nodes_equal(&w->eend[3 * i], &w->eend[3 * j], m->precision))
+                colinear = 1;
+    if (colinear || !w->any_active) m->cact[3 * t + k] = 0;
+    else if (t == maxid) m->cact[3 * t + k] = 1;
+    else m->cact[3 * t + k] = 0;
+}
+
+static void mesh_element_props(mesh_t *m, int t)
+{
+    double avg[3] = {0, 0, 0};
+    for (int k = 0; k < 3; k++)
+        for (int q = 0; q < 3; q++) avg[q] = MN(m, t, k)[q] + avg[q];
+    sdiv3(avg, 3.0);
+    for (int q = 0; q < 3; q++) m->center[3 * t + q] = avg[q];
+    double rb = 0.;
+    for (int k = 0; k < 3; k++) {
+        double v[3];
+        for (int q = 0; q < 3; q++) v[q] = m->center[3 * t + q] - MN(m, t, k)[q];
+        const double mg = mag3(v);
+        rb = rb > mg ? rb : mg;
+    }
+    m->rbound[t] = rb;
+    /* calcEdgeVecLen */
+    for (int k = 0; k < 3; k++) {
+        double *e = EV(m, t, k);
+        for (int q = 0; q < 3; q++) e[q] = MN(m, t, (k + 1) % 3)[q] - MN(m, t, k)[q];
+        m->elen[3 * t + k] = mag3(e);
+        sdiv3(e, m->elen[3 * t + k]);
+    }
+    /* calcSurfaceNorm */
+    double *sn = &m->snorm[3 * t];
+    cross3(EV(m, t, 0), EV(m, t, 1), sn);
+    if (mag3(sn) < 1e-15) {
+        cross3(EV(m, t, 1), EV(m, t, 2), sn);
+        if (mag3(sn) < 1e-15) {
+            cross3(EV(m, t, 2), EV(m, t, 0), sn);
+            if (mag3(sn) < 1e-15) {
+                const double tmp[3] = {1.1233, 2.123231, -3.3343434};
+                cross3(EV(m, t, 0), tmp, sn);
+            }
+        }
+    }
+    sdiv3(sn, mag3(sn));
+    /* calcEdgeNormals */
+    for (int k = 0; k < 3; k++) {
+        double *en = EN(m, t, k);
+        cross3(EV(m, t, k), sn, en);
+        if (mag3(en) < 1e-15) {
+            for (int q = 0; q < 3; q++) en[q] = EV(m, t, (k + 1) % 3)[q];
+        } else {
+            sdiv3(en, mag3(en));
+        }
+    }
+    /* calcObtuseAngleIndex, called for nodes 0,1,2 in turn: the last call
+     * decides (only an obtuse angle at node 2 is recorded) */
+    m->obtuse[t] = -1;
+    for (int k = 0; k < 3; k++) {
+        const double dd = dot3(EV(m, t, k), EV(m, t, (k - 1 + 3) % 3));
+        m->obtuse[t] = dd > 0. ? k : -1;
+    }
+}
+
+int or_add_mesh(or_sim *s, int ntri, const double *nodes, int wtype, double curvature, const double *vel)
+{
+    if (s->nmesh >= 4 || ntri < 1) return -1;
+    mesh_t *m = &s->mesh[s->nmesh];
+    memset(m, 0, sizeof(*m));
+    m->ntri = ntri;
+    m->wtype = wtype;
+    m->curvature = curvature > 0 ? curvature : 1. - 0.00001;
+    m->precision = 1e-8;
+    if (vel) {
+        m->moving = 1;
+        for (int q = 0; q < 3; q++) m->vel[q] = vel[q];
+    }
+    m->node = malloc(sizeof(double) * 9 * ntri);
+    memcpy(m->node, nodes, sizeof(double) * 9 * ntri);
+    m->center = malloc(sizeof(double) * 3 * ntri);
+    m->rbound = malloc(sizeof(double) * ntri);
+    m->evec = malloc(sizeof(double) * 9 * ntri);
+    m->enorm = malloc(sizeof(double) * 9 * ntri);
+    m->elen = malloc(sizeof(double) * 3 * ntri);
+    m->snorm = malloc(sizeof(double) * 3 * ntri);
+    m->obtuse = malloc(sizeof(int) * ntri);
+    m->nneigh = calloc(ntri, sizeof(int));
+    m->neigh = malloc(sizeof(int) * MESH_NEIGH_MAX * ntri);
+    m->eact = malloc(3 * ntri);
+    m->cact = malloc(3 * ntri);
+    for (int t = 0; t < ntri; t++) {
+        mesh_element_props(m, t);
+        for (int k = 0; k < MESH_NEIGH_MAX; k++) m->neigh[MESH_NEIGH_MAX * t + k] = -1;
+        for (int k = 0; k < 3; k++) m->eact[3 * t + k] = m->cact[3 * t + k] = 1;
+    }
+    /* buildNeighbours: every earlier element that shares an edge */
+    for (int i = 0; i < ntri; i++)
+        for (int j = 0; j < i; j++) {
+            int a1, b1, a2, b2;
+            if (!share2(m, i, j, &a1, &b1, &a2, &b2)) continue;
+            const int ie = (a1 + a2 == 2) ? 2 : (a1 < a2 ? a1 : a2);
+            const int je = (b1 + b2 == 2) ? 2 : (b1 < b2 ? b1 : b2);
+            const int cop = fabs(dot3(&m->snorm[3 * i], &m->snorm[3 * j])) >= m->curvature;
+            handle_shared_edge(m, i, ie, j, je, cop);
+        }
+    corner_walk_t w;
+    w.vis = malloc(sizeof(int) * ntri);
+    w.has = malloc(sizeof(int) * ntri);
+    w.elist = malloc(sizeof(double) * 6 * ntri);
+    w.eend = malloc(sizeof(double) * 6 * ntri);
+    for (int t = 0; t < ntri; t++)
+        for (int k = 0; k < 3; k++) handle_corner(s, m, t, k, &w);
+    free(w.vis); free(w.has); free(w.elist); free(w.eend);
+    const int id = s->nmesh++;
+    op_t *o = new_op(s, OP_MESH);
+    o->arg = id;
+    return o->id;
+}
+
+void or_mesh_topology(const or_sim *s, int im, int *eact, int *cact, int *obtuse, int *nneigh)
+{
+    const mesh_t *m = &s->mesh[im];
+    for (int t = 0; t < m->ntri; t++) {
+        for (int k = 0; k < 3; k++) {
+            eact[3 * t + k] = m->eact[3 * t + k];
+            cact[3 * t + k] = m->cact[3 * t + k];
+        }
+        obtuse[t] = m->obtuse[t];
+        nneigh[t] = m->nneigh[t];
+    }
+}
+
+void or_mesh_force(const or_sim *s, int im, double f[3])
+{
+    for (int q = 0; q < 3; q++) f[q] = s->mesh[im].force[q];
+}
+
+/* TriMesh::calcDist (tri_mesh_I.h:308-312) */
+static double tri_calc_dist(const double *cs, const double *cp, double *delta)
+{
+    for (int q = 0; q < 3; q++) delta[q] = cp[q] - cs[q];
+    double d[3];
+    for (int q = 0; q < 3; q++) d[q] = cs[q] - cp[q];
+    return sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+}
+
+#define SMALL_TRIMESH 1.e-10
+#define LARGE_TRIMESH 1000000
+
+static double tri_edge(const mesh_t *m, int t, int ie, const double *p, double *delta, double *bary)
+{
+    const int ip = (ie + 1) % 3, ipp = (ie + 2) % 3;
+    double ntp[3];
+    for (int q = 0; q < 3; q++) ntp[q] = p[q] - MN(m, t, ie)[q];
+    const double dfn = dot3(ntp, EV(m, t, ie));
+    double d;
+    if (dfn < -SMALL_TRIMESH) {
+        if (!m->cact[3 * t + ie]) return LARGE_TRIMESH;
+        d = tri_calc_dist(p, MN(m, t, ie), delta);
+        bary[ie] = 1.; bary[ip] = 0.; bary[ipp] = 0.;
+    } else if (dfn > m->elen[3 * t + ie] + SMALL_TRIMESH) {
+        if (!m->cact[3 * t + ip]) return LARGE_TRIMESH;
+        d = tri_calc_dist(p, MN(m, t, ip), delta);
+        bary[ie] = 0.; bary[ip] = 1.; bary[ipp] = 0.;
+    } else {
+        if (!m->eact[3 * t + ie]) return LARGE_TRIMESH;
+        double cp[3];
+        for (int q = 0; q < 3; q++) cp[q] = MN(m, t, ie)[q] + dfn * EV(m, t, ie)[q];
+        d = tri_calc_dist(p, cp, delta);
+        bary[ipp] = 0.;
+        bary[ie] = 1. - dfn / m->elen[3 * t + ie];
+        bary[ip] = 1. - bary[ie];
+    }
+    return d;
+}
+
+static double tri_corner(const mesh_t *m, int t, int in, int obtuse, const double *p, double *delta, double *bary)
+{
+    const int ip = (in + 1) % 3, ipp = (in + 2) % 3;
+    const double *n = MN(m, t, in);
+    if (obtuse) {
+        double ntp[3], cp[3];
+        for (int q = 0; q < 3; q++) ntp[q] = p[q] - n[q];
+        double dfn = dot3(ntp, EV(m, t, ipp));
+        if (dfn < SMALL_TRIMESH) {
+            if (dfn > -m->elen[3 * t + ipp]) {
+                if (!m->eact[3 * t + ipp]) return LARGE_TRIMESH;
+                for (int q = 0; q < 3; q++) cp[q] = n[q] + dfn * EV(m, t, ipp)[q];
+                bary[ip] = 0.;
+                bary[in] = 1. + dfn / m->elen[3 * t + ipp];
+                bary[ipp] = 1. - bary[in];
+                return tri_calc_dist(p, cp, delta);
+            } else {
+                if (!m->cact[3 * t + ipp]) return LARGE_TRIMESH;
+                bary[ipp] = 1.; bary[in] = bary[ip] = 0.;
+                return tri_calc_dist(p, MN(m, t, ipp), delta);
+            }
+        }
+        dfn = dot3(ntp, EV(m, t, in));
+        if (dfn > -SMALL_TRIMESH) {
+            if (dfn < m->elen[3 * t + in]) {
+                if (!m->eact[3 * t + in]) return LARGE_TRIMESH;
+                for (int q = 0; q < 3; q++) cp[q] = n[q] + dfn * EV(m, t, in)[q];
+                bary[ipp] = 0.;
+                bary[in] = 1. - dfn / m->elen[3 * t + in];
+                bary[ip] = 1. - bary[in];
+                return tri_calc_dist(p, cp, delta);
+            } else {
+                if (!m->cact[3 * t + ip]) return LARGE_TRIMESH;
+                bary[ip] = 1.; bary[in] = bary[ipp] = 0.;
+                return tri_calc_dist(p, MN(m, t, ip), delta);
+            }
+        }
+    }
+    if (!m->cact[3 * t + in]) return LARGE_TRIMESH;
+    bary[in] = 1.; bary[ip] = bary[ipp] = 0.;
+    return tri_calc_dist(p, MN(m, t, in), delta);
+}
+
+/* resolveTriSphereContactBary: distance - radius, delta from sphere centre
+ * to the closest point, barycentric coordinates, region code */
+static double tri_contact(const mesh_t *m, int t, double r, const double *cs, double *delta,
+                          double *bary, int *barysign)
+{
+    double n0c[3];
+    for (int q = 0; q < 3; q++) n0c[q] = cs[q] - MN(m, t, 0)[q];
+    /* MathExtraLiggghts::calcBaryTriCoords (math_extra_liggghts.h:583-594) */
+    const double a = dot3(n0c, EV(m, t, 0));
+    const double b = dot3(n0c, EV(m, t, 2));
+    const double c = dot3(EV(m, t, 0), EV(m, t, 2));
+    const double oneMinCSqr = 1 - c * c;
+    bary[1] = (a - b * c) / (m->elen[3 * t + 0] * oneMinCSqr);
+    bary[2] = (a * c - b) / (m->elen[3 * t + 2] * oneMinCSqr);
+    bary[0] = 1. - bary[1] - bary[2];
+    const double invlen = 1. / (2. * m->rbound[t]);
+    const int bs = (bary[0] > -m->precision * invlen) + 2 * (bary[1] > -m->precision * invlen)
+                   + 4 * (bary[2] > -m->precision * invlen);
+    *barysign = bs;
+    const int ob = m->obtuse[t];
+    double d = 1.;
+    switch (bs) {
+    case 1: d = tri_corner(m, t, 0, ob == 0, cs, delta, bary); break;
+    case 2: d = tri_corner(m, t, 1, ob == 1, cs, delta, bary); break;
+    case 3: d = tri_edge(m, t, 0, cs, delta, bary); break;
+    case 4: d = tri_corner(m, t, 2, ob == 2, cs, delta, bary); break;
+    case 5: d = tri_edge(m, t, 2, cs, delta, bary); break;
+    case 6: d = tri_edge(m, t, 1, cs, delta, bary); break;
+    case 7: {
+        const double *sn = &m->snorm[3 * t];
+        const double dNorm = dot3(sn, n0c);
+        double csp[3], tmp[3];
+        for (int q = 0; q < 3; q++) tmp[q] = dNorm * sn[q];
+        for (int q = 0; q < 3; q++) csp[q] = cs[q] - tmp[q];
+        d = tri_calc_dist(cs, csp, delta);
+        break;
+    }
+    default: d = LARGE_TRIMESH; break;
+    }
+    return d - r;
+}
+
+/* resolveTriSphereNeighbuild (tri_mesh_I.h:275-307) */
+static int tri_neighbuild(const mesh_t *m, int t, double r, const double *cs, double treshold)
+{
+    const double maxDist = r + treshold;
+    double v[3];
+    for (int q = 0; q < 3; q++) v[q] = cs[q] - m->center[3 * t + q];
+    const double dNorm = fabs(dot3(&m->snorm[3 * t], v));
+    if (dNorm > maxDist) return 0;
+    const double dParaMax = maxDist * maxDist;
+    for (int k = 0; k < 3; k++) {
+        for (int q = 0; q < 3; q++) v[q] = cs[q] - MN(m, t, k)[q];
+        const double d = dot3(EN(m, t, k), v);
+        if (d > 0 && d * d > dParaMax) return 0;
+    }
+    return 1;
+}
+
+/* FixContactHistoryMesh::sort_contacts (fix_contact_history_mesh.cpp:
+ * 375-402): move filled slots to the front, first empty <-> last filled */
+static void mesh_slot_swap(mesh_t *m, int dnum, int i, int a, int b)
+{
+    int *pt = &m->partner[m->sstart[i]];
+    double *h = &m->hist[(size_t)m->sstart[i] * dnum];
+    const int t = pt[a]; pt[a] = pt[b]; pt[b] = t;
+    for (int d = 0; d < dnum; d++) {
+        const double x = h[a * dnum + d]; h[a * dnum + d] = h[b * dnum + d]; h[b * dnum + d] = x;
+    }
+}
+
+static void mesh_sort_contacts(or_sim *s, mesh_t *m)
+{
+    if (!m->partner) return;
+    for (int i = 0; i < s->n; i++) {
+        const int nn = m->nneighs[i];
+        int *pt = &m->partner[m->sstart[i]];
+        int fe, lf;
+        do {
+            fe = lf = -1;
+            for (int j = 0; j < nn; j++) {
+                if (fe == -1 && pt[j] == -1) fe = j;
+                if (pt[j] >= 0) lf = j;
+            }
+            if (fe > -1 && lf > -1 && fe < lf) mesh_slot_swap(m, s->dnum_w, i, fe, lf);
+        } while (fe > -1 && lf > -1 && fe < lf);
+    }
+}
+
+static int tri_has_candidate(const mesh_t *m, int t, int i)
+{
+    for (int q = m->tstart[t]; q < m->tstart[t + 1]; q++)
+        if (m->tlist[q] == i) return 1;
+    return 0;
+}
+
+/* FixNeighlistMesh::pre_force (fix_neighlist_mesh.cpp:230-300) and
+ * FixContactHistoryMesh::pre_force (fix_contact_history_mesh.cpp:314-365)
+ * with cleanUpContactJumps (:466-500) */
+static void mesh_rebuild(or_sim *s, mesh_t *m)
+{
+    const int dnum = s->dnum_w;
+    const double treshold = m->moving ? s->p.skin : 0.5 * s->p.skin;
+    free(m->tstart); free(m->tlist);
+    m->tstart = malloc(sizeof(int) * (m->ntri + 1));
+    int cap = 1024, cnt = 0;
+    m->tlist = malloc(sizeof(int) * cap);
+    int *nn = calloc(s->n, sizeof(int));
+    for (int t = 0; t < m->ntri; t++) {
+        m->tstart[t] = cnt;
+        for (int i = 0; i < s->n; i++)
+            if (tri_neighbuild(m, t, s->radius[i] * s->p.cdf, &s->x[3 * i], treshold)) {
+                if (cnt == cap) { cap *= 2; m->tlist = realloc(m->tlist, sizeof(int) * cap); }
+                m->tlist[cnt++] = i;
+                nn[i]++;
+            }
+    }
+    m->tstart[m->ntri] = cnt;
+    /* carry the first npartner slots into slot arrays of the new size */
+    int *ostart = m->sstart, *opart = m->partner, *onp = m->npartner;
+    double *ohist = m->hist;
+    if (opart) {
+        for (int i = 0; i < s->n; i++) {
+            int ip = 0;
+            while (ip < onp[i]) {
+                const int tri = opart[ostart[i] + ip];
+                if (!tri_has_candidate(m, tri, i)) {
+                    opart[ostart[i] + ip] = -1;
+                    for (int d = 0; d < dnum; d++) ohist[(size_t)(ostart[i] + ip) * dnum + d] = 0.0;
+                    /* swap(i, ip, npartner-1) on the old arrays */
+                    const int last = onp[i] - 1;
+                    const int tt = opart[ostart[i] + ip];
+                    opart[ostart[i] + ip] = opart[ostart[i] + last];
+                    opart[ostart[i] + last] = tt;
+                    for (int d = 0; d < dnum; d++) {
+                        const double x = ohist[(size_t)(ostart[i] + ip) * dnum + d];
+                        ohist[(size_t)(ostart[i] + ip) * dnum + d] = ohist[(size_t)(ostart[i] + last) * dnum + d];
+                        ohist[(size_t)(ostart[i] + last) * dnum + d] = x;
+                    }
+                    onp[i]--;
+                } else {
+                    ip++;
+                }
+            }
+        }
+    }
+    int tot = 0;
+    int *nstart = malloc(sizeof(int) * (s->n + 1));
+    for (int i = 0; i < s->n; i++) { nstart[i] = tot; tot += nn[i]; }
+    nstart[s->n] = tot;
+    int *npart = malloc(sizeof(int) * (tot + 1));
+    double *nhist = calloc((size_t)(tot + 1) * dnum, sizeof(double));
+    for (int q = 0; q < tot; q++) npart[q] = -1;
+    int *newnp = calloc((size_t)(s->n > 0 ? s->n : 1), sizeof(int));
+    for (int i = 0; i < s->n; i++) {
+        if (!opart) continue;
+        for (int ip = 0; ip < onp[i] && ip < nn[i]; ip++) {
+            npart[nstart[i] + ip] = opart[ostart[i] + ip];
+            for (int d = 0; d < dnum; d++)
+                nhist[(size_t)(nstart[i] + ip) * dnum + d] = ohist[(size_t)(ostart[i] + ip) * dnum + d];
+        }
+        newnp[i] = onp[i];
+    }
+    free(ostart); free(opart); free(onp); free(ohist); free(m->keep); free(m->inter); free(m->nneighs);
+    m->sstart = nstart;
+    m->partner = npart;
+    m->hist = nhist;
+    m->npartner = newnp;
+    m->nneighs = nn;
+    m->keep = calloc(tot + 1, 1);
+    m->inter = calloc(tot + 1, 1);
+}
+
+static void mesh_move(or_sim *s, mesh_t *m)
+{
+    if (!m->moving) return;
+    double dx[3];
+    for (int q = 0; q < 3; q++) dx[q] = s->p.dt * m->vel[q];
+    for (int t = 0; t < m->ntri; t++) {
+        for (int k = 0; k < 3; k++)
+            for (int q = 0; q < 3; q++) MN(m, t, k)[q] += dx[q];
+        for (int q = 0; q < 3; q++) m->center[3 * t + q] += dx[q];
+    }
+}
+
+/* FixWallGran::post_force_mesh (fix_wall_gran.cpp:803-980) */
+static void mesh_post_force(or_sim *s, mesh_t *m, int shearupdate)
+{
+    const int dnum = s->dnum_w;
+    const double cdm = s->p.cdf - 1.0;
+    double vwall[3];
+    m->force[0] = m->force[1] = m->force[2] = 0.0;
+    /* markAllContacts */
+    for (int q = 0; q < m->sstart[s->n]; q++) { m->keep[q] = 0; m->inter[q] = 0; }
+    for (int t = 0; t < m->ntri; t++) {
+        for (int qq = m->tstart[t]; qq < m->tstart[t + 1]; qq++) {
+            const int i = m->tlist[qq];
+            double delta[3], bary[3];
+            int bs = -1;
+            const double radi = s->radius[i];
+            const double deltan = tri_contact(m, t, radi, &s->x[3 * i], delta, bary, &bs);
+            if (deltan > s->cutneighmax) continue;
+            const int intersect = deltan <= 0;
+            if (!(deltan <= 0 || deltan < cdm * radi)) continue;
+            /* FixContactHistoryMesh::handleContact */
+            const int base = m->sstart[i], ns = m->nneighs[i];
+            int *pt = &m->partner[base];
+            int slot = -1;
+            for (int k = 0; k < ns; k++)
+                if (pt[k] == t) { slot = k; break; }
+            if (slot >= 0) {
+                m->keep[base + slot] = 1;
+                m->inter[base + slot] = intersect;
+            } else {
+                const int face = (bs == 7);
+                if (face) {
+                    int already = 0;
+                    for (int k = 0; k < ns; k++)
+                        if (pt[k] >= 0 && pt[k] != t && coplanar_node_neighs(m, pt[k], t) && m->keep[base + k])
+                            already = 1;
+                    if (already) continue;
+                }
+                for (int k = 0; k < ns; k++)
+                    if (pt[k] == -1) { slot = k; break; }
+                if (slot < 0) continue;  /* the reference stops with an internal error */
+                pt[slot] = t;
+                m->keep[base + slot] = 1;
+                m->inter[base + slot] = intersect;
+                double *h = &m->hist[(size_t)(base + slot) * dnum];
+                for (int d = 0; d < dnum; d++) h[d] = 0.0;
+                m->npartner[i]++;
+                if (face)
+                    for (int k = 0; k < ns; k++)
+                        if (pt[k] >= 0 && pt[k] != t && coplanar_node_neighs(m, pt[k], t))
+                            for (int d = 0; d < dnum; d++) h[d] = m->hist[(size_t)(base + k) * dnum + d];
+            }
+            double *h = &m->hist[(size_t)(base + slot) * dnum];
+            for (int q = 0; q < 3; q++)
+                vwall[q] = m->moving ? (bary[0] * m->vel[q] + bary[1] * m->vel[q] + bary[2] * m->vel[q]) : 0.0;
+            sidata_t sd;
+            memset(&sd, 0, sizeof(sd));
+            sd.is_wall = 1;
+            sd.radi = radi;
+            sd.deltan = -deltan;
+            sd.delta[0] = -delta[0]; sd.delta[1] = -delta[1]; sd.delta[2] = -delta[2];
+            sd.v_i = &s->v[3 * i]; sd.v_j = vwall;
+            sd.omega_i = &s->omega[3 * i]; sd.omega_j = vwall;
+            sd.r = sd.radi - sd.deltan;
+            sd.rsq = sd.r * sd.r;
+            const double rinv = 1.0 / sd.r;
+            sd.rinv = rinv;
+            sd.itype = s->type[i];
+            sd.jtype = m->wtype;
+            sd.meff = s->rmass[i];
+            sd.mi = s->rmass[i];
+            sd.en[0] = sd.delta[0] * rinv;
+            sd.en[1] = sd.delta[1] * rinv;
+            sd.en[2] = sd.delta[2] * rinv;
+            sd.radsum = sd.radi;
+            sd.update_history = shearupdate;
+            sd.hist = h;
+            sd.model = &s->mw;
+            sd.flags = NULL;
+            double Fi[3] = {0, 0, 0}, Ti[3] = {0, 0, 0}, Fj[3] = {0, 0, 0}, Tj[3] = {0, 0, 0};
+            if (intersect) {
+                chain_intersect(s, &sd, Fi, Ti, Fj, Tj);
+                for (int q = 0; q < 3; q++) {
+                    s->f[3 * i + q] += Fi[q];
+                    s->torque[3 * i + q] += Ti[q];
+                    m->force[q] += Fi[q];
+                }
+                s->fscale[i] += sqrt(Fi[0] * Fi[0] + Fi[1] * Fi[1] + Fi[2] * Fi[2]);
+                s->tscale[i] += sqrt(Ti[0] * Ti[0] + Ti[1] * Ti[1] + Ti[2] * Ti[2]);
+            } else {
+                chain_close(&s->mw, h, NULL);
+            }
+        }
+    }
+    /* cleanUpContacts */
+    for (int i = 0; i < s->n; i++)
+        for (int k = 0; k < m->nneighs[i]; k++) {
+            const int q = m->sstart[i] + k;
+            if (!m->keep[q]) {
+                if (m->partner[q] > -1) m->npartner[i]--;
+                m->partner[q] = -1;
+                m->inter[q] = 0;
+                for (int d = 0; d < dnum; d++) m->hist[(size_t)q * dnum + d] = 0.0;
+            }
+        }
+}
+
+int or_mesh_contacts(const or_sim *s, int im, int *tag, int *tri, double *hist)
+{
+    const mesh_t *m = &s->mesh[im];
+    int c = 0;
+    if (!m->partner) return 0;
+    for (int i = 0; i < s->n; i++)
+        for (int k = 0; k < m->nneighs[i]; k++) {
+            const int q = m->sstart[i] + k;
+            if (m->partner[q] < 0) continue;
+            if (tag) {
+                tag[c] = s->tag[i];
+                tri[c] = m->partner[q];
+                for (int d = 0; d < s->dnum_w; d++) hist[(size_t)c * s->dnum_w + d] = m->hist[(size_t)q * s->dnum_w + d];
+            }
+            c++;
+        }
+    return c;
+}
+
 /* FixGravity::post_force (fix_gravity.cpp:301-352), rmass branch */
 static void gravity(or_sim *s)
 {
@@ -1235,12 +1979,16 @@ void or_compute_forces(or_sim *s, int shearupdate)
     memset(s->fscale, 0, (size_t)s->n * sizeof(double));
     memset(s->tscale, 0, (size_t)s->n * sizeof(double));
     /* FixWallGran::pre_force (fix_wall_gran.cpp:688-706) */
-    if (s->wall_rebuild) build_wall_lists(s);
+    if (s->wall_rebuild) {
+        build_wall_lists(s);
+        for (int k = 0; k < s->nmesh; k++) mesh_rebuild(s, &s->mesh[k]);
+    }
     s->wall_rebuild = 0;
     pair_compute(s, shearupdate);
     for (int q = 0; q < s->nops; q++) {
         op_t *o = &s->ops[q];
         if (o->kind == OP_WALL) wall_plane(s, o->arg, shearupdate);
+        else if (o->kind == OP_MESH) mesh_post_force(s, &s->mesh[o->arg], shearupdate);
         else if (o->kind == OP_GRAVITY) gravity(s);
         else if (o->kind == OP_FREEZE) freeze(s);
         else if (o->kind == OP_SETFORCE) setforce(s, o);
@@ -1256,6 +2004,8 @@ static void initial_integrate(or_sim *s)
     const double dtf = 0.5 * s->p.dt;   /* dtf = 0.5*dt*ftm2v, ftm2v = 1 */
     const double dtv = s->p.dt;
     const double dtfrotate = dtf / 0.4;
+    /* FixMoveMesh::initial_integrate -> MeshMoverLinear (mesh_mover_linear.cpp:94-112) */
+    for (int k = 0; k < s->nmesh; k++) mesh_move(s, &s->mesh[k]);
     for (int i = 0; i < s->n; i++) {
         const double dtfm = dtf / (s->rmass[i] * (1. + 0.0 / 1.0));
         double *v = &s->v[3 * i], *x = &s->x[3 * i], *f = &s->f[3 * i];
@@ -1362,7 +2112,9 @@ int or_get_pairs(const or_sim *s, int *tag_i, int *tag_j, int *shift, double *hi
             if (e->shift == 13) {
                 if (tag_i) {
                     tag_i[c] = s->tag[e->j]; tag_j[c] = s->tag[i]; shift[c] = 13;
-                    if (hist) for (int d = 0; d < dnum; d++) hist[(size_t)c * dnum + d] = -h[d];
+                    if (hist)
+                        for (int d = 0; d < dnum; d++)
+                            hist[(size_t)c * dnum + d] = s->mp.newton[d] ? -h[d] : h[d];
                     if (flag) flag[c] = e->flag;
                 }
                 c++;

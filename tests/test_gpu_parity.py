@@ -149,6 +149,37 @@ def test_mixed_precision_force_error():
     ef, et = check_forces(s, o, 1e-5)
 
 
+HYST = {k: np.full((2, 2), v) for k, v in dict(
+    LoadingStiffness=5e4, UnloadingStiffness=2.5, coefficientAdhesionStiffness=0.0,
+    coefficientPlasticityDepth=0.1, pullOffForce=0.0, alphaCustom=20.0, cinCustom=1e-7,
+    aoneCustom=10e8, atwoCustom=4e4, athreeCustom=8.0, kcinCustom=1e-4).items()}
+
+
+@pytest.mark.parametrize("model,wall", [("hysteretic/nonlinear1", "hertz"),
+                                        ("hysteretic/nonlinear2", "hertz"),
+                                        ("hysteretic/nonlinear1", "hysteretic/nonlinear1")])
+def test_hysteretic_identical_state_and_horizon(model, wall):
+    """Strain-hardening contact (the INL gold examples' parameters) on the
+    pour: forces from an identical state, then a short horizon.  pow/exp/log
+    come from the device math library, hence 1e-11 instead of 1e-12."""
+    spec, atoms = W().pour_box()
+    spec = dict(spec, normal=model, wall_normal=wall)
+    spec["materials"] = dict(spec["materials"], hysteretic=HYST)
+    o = Oracle(spec, atoms)
+    o.setup()
+    o.run(3000)
+    s = sim_from_oracle(spec, atoms, o)
+    check_pairs(s, o, hist_tol=1e-11)
+    check_forces(s, o, 1e-11)
+    fs, _ = o.force_scale()
+    assert (fs > 0).sum() > 100
+    s.step(25)
+    o.run(25)
+    g, r = s.state(), o.state()
+    assert np.abs(g["x"] - r["x"]).max() <= 1e-12 * 0.08
+    check_forces(s, o, 1e-8)
+
+
 def test_deterministic_bitwise():
     spec, atoms = W().packed_column(n=20000, slab_n=5000)
     eng = load_engine()
