@@ -70,6 +70,7 @@ typedef struct {
     int wtype;
     int moving;
     double vel[3];
+    double *node_re;   /* node positions at the last rebuild (moving mesh) */
     /* triangle -> candidate particles, rebuilt with the neighbour list */
     int *tstart, *tlist;
     /* per-particle contact slots (FixContactHistoryMesh) */
@@ -133,6 +134,7 @@ struct or_sim {
     int ago;
     int setupflag;
     int wall_rebuild;
+    long long next_reneighbor;   /* FixMesh::next_reneighbor, -1 none */
     double cutneighmax;
 };
 
@@ -162,7 +164,7 @@ static void model_layout(model_t *m, int normal, int limitf, int rolling)
     m->off_shear = k;
     for (int q = 0; q < 3; q++) m->newton[k++] = 1;
     m->off_roll = -1;
-    if (rolling != OR_ROLL_OFF) {
+    if (rolling == OR_ROLL_EPSD || rolling == OR_ROLL_EPSD2 || rolling == OR_ROLL_EPSD3) {
         m->off_roll = k;
         for (int q = 0; q < 3; q++) m->newton[k++] = 1;
     }
@@ -177,6 +179,7 @@ or_sim *or_create(const or_params *p)
     model_layout(&s->mw, p->wall_normal_model, p->wall_limit_force, p->rolling_model);
     s->dnum = s->mp.dnum;
     s->dnum_w = s->mw.dnum;
+    s->next_reneighbor = -1;
     return s;
 }
 
@@ -196,7 +199,7 @@ void or_destroy(or_sim *s)
     free(s->ent); free(s->hist);
     for (int k = 0; k < s->nmesh; k++) {
         mesh_t *m = &s->mesh[k];
-        free(m->node); free(m->center); free(m->rbound); free(m->evec); free(m->elen);
+        free(m->node); free(m->node_re); free(m->center); free(m->rbound); free(m->evec); free(m->elen);
         free(m->enorm); free(m->snorm); free(m->obtuse); free(m->nneigh); free(m->neigh);
         free(m->eact); free(m->cact); free(m->tstart); free(m->tlist); free(m->nneighs);
         free(m->npartner); free(m->sstart); free(m->partner); free(m->keep); free(m->inter);
@@ -300,10 +303,10 @@ int or_add_plane(or_sim *s, int dim, double pos, int wtype)
     return o->id;
 }
 
-void or_add_gravity(or_sim *s, double ax, double ay, double az)
+int or_add_gravity(or_sim *s, double ax, double ay, double az)
 {
     s->gacc[0] = ax; s->gacc[1] = ay; s->gacc[2] = az;
-    new_op(s, OP_GRAVITY);
+    return new_op(s, OP_GRAVITY)->id;
 }
 
 void or_add_freeze(or_sim *s)
@@ -872,6 +875,45 @@ static void rolling(const or_sim *s, sidata_t *sd, double Ti[3], double Tj[3])
     Tj[0] += rt[0]; Tj[1] += rt[1]; Tj[2] += rt[2];
 }
 
+/* RollingModel<ROLLING_CDTNONLINEAR2>::surfacesIntersect
+ * (rolling_model_cdtnonlinear2.h:96-172): constant torque mu_r |..| Fn reff
+ * along the relative rolling velocity, no history, no contact flag */
+static void rolling_cdt2(const or_sim *s, sidata_t *sd, double Ti[3], double Tj[3])
+{
+    const double rmu = s->mur[sd->itype * T1 + sd->jtype];
+    double rt[3] = {0, 0, 0};
+    const double radi = sd->radi, radj = sd->radj;
+    const double reff = sd->is_wall ? radi : (radi * radj / (radi + radj));
+    const double enx = sd->en[0], eny = sd->en[1], enz = sd->en[2];
+    double mag, w[3];
+    if (sd->is_wall) {
+        w[0] = sd->wr[0]; w[1] = sd->wr[1]; w[2] = sd->wr[2];
+    } else {
+        w[0] = sd->omega_i[0] - sd->omega_j[0];
+        w[1] = sd->omega_i[1] - sd->omega_j[1];
+        w[2] = sd->omega_i[2] - sd->omega_j[2];
+    }
+    mag = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+    if (mag > 0.) {
+        const double Fn = sd->Fn;
+        if (sd->is_wall) {
+            rt[0] = rmu * Fn * w[0] / mag * reff;
+            rt[1] = rmu * Fn * w[1] / mag * reff;
+            rt[2] = rmu * Fn * w[2] / mag * reff;
+        } else {
+            const double sc = rmu * Fn * reff / mag;
+            rt[0] = w[0] * sc; rt[1] = w[1] * sc; rt[2] = w[2] * sc;
+        }
+        if (!s->p.torsion_torque) {
+            const double d = rt[0] * enx + rt[1] * eny + rt[2] * enz;
+            const double n0 = enx * d, n1 = eny * d, n2 = enz * d;
+            rt[0] = rt[0] - n0; rt[1] = rt[1] - n1; rt[2] = rt[2] - n2;
+        }
+    }
+    Ti[0] -= rt[0]; Ti[1] -= rt[1]; Ti[2] -= rt[2];
+    Tj[0] += rt[0]; Tj[1] += rt[1]; Tj[2] += rt[2];
+}
+
 /* NormalModel<HYSTERETIC_NONLINEAR1/2>::surfacesIntersect
  * (normal_model_hysteretic_nonlinear1.h:149-420, ..._nonlinear2.h:149-420).
  * The history is written whatever shearupdate says, and sqrt(5/4) is the
@@ -1051,6 +1093,8 @@ static void chain_intersect(const or_sim *s, sidata_t *sd, double Fi[3], double 
     if (sd->model->off_roll >= 0) {
         if (sd->flags) *sd->flags |= CONTACT_ROLLING_MODEL;
         rolling(s, sd, Ti, Tj);
+    } else if (s->p.rolling_model == OR_ROLL_CDT2) {
+        rolling_cdt2(s, sd, Ti, Tj);
     }
 }
 
@@ -1369,7 +1413,9 @@ static void check_node(const mesh_t *m, int t, const double *p, corner_walk_t *w
 }
 
 /* handleCorner (surface_mesh_I.h:1139-1185); a node outside the box keeps
- * its initial active state, as there */
+ * its initial active state, as there.  Domain::is_in_subdomain pads a
+ * subdomain face lying on the box face by SMALL_DMBRDR = 1e-8 (domain_I.h:
+ * 65-86, domain.h:64), which in one process is every face. */
 static void handle_corner(const or_sim *s, mesh_t *m, int t, int k, corner_walk_t *w)
 {
     double p[3];
@@ -1378,7 +1424,7 @@ static void handle_corner(const or_sim *s, mesh_t *m, int t, int k, corner_walk_
     w->any_active = 0;
     check_node(m, t, p, w);
     for (int d = 0; d < 3; d++)
-        if (p[d] < s->p.boxlo[d] || p[d] >= s->p.boxhi[d]) return;
+        if (!(p[d] >= s->p.boxlo[d] - 1.0e-8 && p[d] < s->p.boxhi[d] + 1.0e-8)) return;
     const int ne = 2 * w->nhas;
     int maxid = -1;
     for (int i = 0; i < w->nhas; i++)
@@ -1464,6 +1510,8 @@ int or_add_mesh(or_sim *s, int ntri, const double *nodes, int wtype, double curv
     }
     m->node = malloc(sizeof(double) * 9 * ntri);
     memcpy(m->node, nodes, sizeof(double) * 9 * ntri);
+    m->node_re = malloc(sizeof(double) * 9 * ntri);
+    memcpy(m->node_re, nodes, sizeof(double) * 9 * ntri);
     m->center = malloc(sizeof(double) * 3 * ntri);
     m->rbound = malloc(sizeof(double) * ntri);
     m->evec = malloc(sizeof(double) * 9 * ntri);
@@ -1707,6 +1755,8 @@ static void mesh_rebuild(or_sim *s, mesh_t *m)
 {
     const int dnum = s->dnum_w;
     const double treshold = m->moving ? s->p.skin : 0.5 * s->p.skin;
+    /* MultiNodeMesh::storeNodePosRebuild (multi_node_mesh_I.h:833) */
+    if (m->moving) memcpy(m->node_re, m->node, sizeof(double) * 9 * m->ntri);
     free(m->tstart); free(m->tlist);
     m->tstart = malloc(sizeof(int) * (m->ntri + 1));
     int cap = 1024, cnt = 0;
@@ -1775,6 +1825,21 @@ static void mesh_rebuild(or_sim *s, mesh_t *m)
     m->nneighs = nn;
     m->keep = calloc(tot + 1, 1);
     m->inter = calloc(tot + 1, 1);
+}
+
+/* MultiNodeMesh::decideRebuild (multi_node_mesh_I.h:792-826): any node
+ * moved more than half the skin since the last rebuild */
+static int mesh_decide_rebuild(const or_sim *s, const mesh_t *m)
+{
+    if (!m->moving) return 0;
+    const double triggersq = 0.25 * s->p.skin * s->p.skin;
+    for (int q = 0; q < 3 * m->ntri; q++) {
+        const double dx = m->node[3 * q] - m->node_re[3 * q];
+        const double dy = m->node[3 * q + 1] - m->node_re[3 * q + 1];
+        const double dz = m->node[3 * q + 2] - m->node_re[3 * q + 2];
+        if (dx * dx + dy * dy + dz * dz > triggersq) return 1;
+    }
+    return 0;
 }
 
 static void mesh_move(or_sim *s, mesh_t *m)
@@ -1895,6 +1960,12 @@ static void mesh_post_force(or_sim *s, mesh_t *m, int shearupdate)
         }
 }
 
+double or_mesh_distance(const or_sim *s, int im, int tri, double r, const double x[3], double delta[3],
+                        double bary[3], int *region)
+{
+    return tri_contact(&s->mesh[im], tri, r, x, delta, bary, region);
+}
+
 int or_mesh_contacts(const or_sim *s, int im, int *tag, int *tri, double *hist)
 {
     const mesh_t *m = &s->mesh[im];
@@ -1982,6 +2053,10 @@ void or_compute_forces(or_sim *s, int shearupdate)
     if (s->wall_rebuild) {
         build_wall_lists(s);
         for (int k = 0; k < s->nmesh; k++) mesh_rebuild(s, &s->mesh[k]);
+    } else {
+        /* FixMesh::pre_force on a regular step (fix_mesh.cpp:553-575) */
+        for (int k = 0; k < s->nmesh; k++)
+            if (mesh_decide_rebuild(s, &s->mesh[k])) s->next_reneighbor = s->ntimestep + 1;
     }
     s->wall_rebuild = 0;
     pair_compute(s, shearupdate);
@@ -2034,6 +2109,7 @@ static void final_integrate(or_sim *s)
 /* Neighbor::decide / check_distance (neighbor.cpp:1362-1465), no box change */
 static int decide(or_sim *s)
 {
+    if (s->ntimestep == s->next_reneighbor) return 1;
     s->ago++;
     if (s->ago >= s->p.neigh_delay && s->ago % s->p.neigh_every == 0) {
         if (!s->p.neigh_check) return 1;

@@ -26,7 +26,7 @@ extern "C" {
 #define OR_MAXPLANES 8
 
 /* rolling_friction keyword of pair_style gran */
-enum { OR_ROLL_OFF = 0, OR_ROLL_EPSD = 1, OR_ROLL_EPSD2 = 2, OR_ROLL_EPSD3 = 3 };
+enum { OR_ROLL_OFF = 0, OR_ROLL_EPSD = 1, OR_ROLL_EPSD2 = 2, OR_ROLL_EPSD3 = 3, OR_ROLL_CDT2 = 4 };
 
 typedef struct {
     double dt;
@@ -77,8 +77,9 @@ int or_set_atoms(or_sim *s, int n, const double *x, const double *v, const doubl
  * (modify.cpp:1673-1750).
  * fix wall/gran ... primitive type <wtype> {x,y,z}plane <pos> */
 int or_add_plane(or_sim *s, int dim, double pos, int wtype);
-/* fix gravity: acceleration vector xacc,yacc,zacc (fix_gravity.cpp:395-397) */
-void or_add_gravity(or_sim *s, double ax, double ay, double az);
+/* fix gravity: acceleration vector xacc,yacc,zacc (fix_gravity.cpp:395-397).
+ * Returns the fix id. */
+int or_add_gravity(or_sim *s, double ax, double ay, double az);
 /* fix freeze on the atoms flagged in or_set_atoms (fix_freeze.cpp:121-144) */
 void or_add_freeze(or_sim *s);
 /* fix mesh/surface (+ fix move/mesh linear when vel != NULL) and
@@ -90,6 +91,12 @@ int or_add_mesh(or_sim *s, int ntri, const double *nodes, int wtype, double curv
  * obtuse angle index (-1 none) and number of edge neighbours per element */
 void or_mesh_topology(const or_sim *s, int mesh, int *edge_active, int *corner_active, int *obtuse,
                       int *nneigh);
+/* TriMesh::resolveTriSphereContactBary of one triangle: returns the signed
+ * gap (distance - r), delta = closest point - centre, barycentric weights and
+ * the region code (7 face, 3/5/6 edge, 1/2/4 corner); LARGE (1e6) - r when
+ * the region's edge or corner is inactive */
+double or_mesh_distance(const or_sim *s, int mesh, int tri, double r, const double x[3], double delta[3],
+                        double bary[3], int *region);
 /* sum of the mesh's contact forces on the particles in the last pass */
 void or_mesh_force(const or_sim *s, int mesh, double f[3]);
 /* current mesh contacts: (atom tag, triangle id, history); NULL to count */

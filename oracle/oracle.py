@@ -13,7 +13,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
 
-ROLL = {"off": 0, "epsd": 1, "epsd2": 2, "epsd3": 3}
+ROLL = {"off": 0, "epsd": 1, "epsd2": 2, "epsd3": 3, "cdtnonlinear2": 4}
 
 
 class _Params(C.Structure):
@@ -67,6 +67,7 @@ def lib():
         L.or_set_atoms.argtypes = [C.c_void_p, C.c_int, dp, dp, dp, dp, dp, ip, ip, ip]
         L.or_add_plane.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int]
         L.or_add_gravity.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double]
+        L.or_add_gravity.restype = C.c_int
         L.or_add_freeze.argtypes = [C.c_void_p]
         L.or_setup.argtypes = [C.c_void_p]
         L.or_run.argtypes = [C.c_void_p, C.c_int]
@@ -90,6 +91,12 @@ def lib():
         L.or_cleanup.argtypes = [C.c_void_p]
         L.or_time.restype = C.c_double
         L.or_time.argtypes = [C.c_void_p]
+        L.or_add_mesh.argtypes = [C.c_void_p, C.c_int, dp, C.c_int, C.c_double, dp]
+        L.or_mesh_topology.argtypes = [C.c_void_p, C.c_int, ip, ip, ip, ip]
+        L.or_mesh_force.argtypes = [C.c_void_p, C.c_int, dp]
+        L.or_mesh_contacts.argtypes = [C.c_void_p, C.c_int, ip, ip, dp]
+        L.or_mesh_distance.restype = C.c_double
+        L.or_mesh_distance.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, dp, dp, dp, ip]
         _LIB = L
     return _LIB
 
@@ -109,7 +116,9 @@ class Oracle:
     box_lo, box_hi, periodic, dt, skin, cdf, neigh_every, neigh_delay,
     neigh_check, rolling, tangential_damping, limit_force, materials
     (ntypes, Y, nu, e, mu, mur, etar, krs), planes [(dim,pos,wtype)],
-    gravity (ax,ay,az) or None, fix_order (list of "wall:k"/"gravity").
+    gravity (ax,ay,az) or None, meshes [dict(nodes (ntri,3,3), type,
+    curvature=None, vel=None)], fix_order (list of "wall:k"/"mesh:k"/
+    "gravity"/"freeze"; default: walls, meshes, gravity).
     """
 
     def __init__(self, spec, atoms):
@@ -171,15 +180,21 @@ class Oracle:
         self.L.or_set_atoms(self.h, self.n, _dp(a["x"]), _dp(a["v"]), _dp(a["omega"]),
                             _dp(a["radius"]), _dp(a["rmass"]), _ip(ty), _ip(tag), _ip(fr))
         planes = spec.get("planes", [])
+        meshes = spec.get("meshes", [])
         order = spec.get("fix_order")
         if order is None:
             order = ["wall:%d" % k for k in range(len(planes))]
+            order += ["mesh:%d" % k for k in range(len(meshes))]
             if spec.get("gravity") is not None:
                 order.append("gravity")
+        self.nmesh = 0
+        self.mesh_ntri = []
         for item in order:
             if item.startswith("wall:"):
                 dim, pos, wt = planes[int(item[5:])]
                 self.L.or_add_plane(self.h, int(dim), float(pos), int(wt))
+            elif item.startswith("mesh:"):
+                self.add_mesh(**meshes[int(item[5:])])
             elif item == "gravity":
                 g = spec["gravity"]
                 self.L.or_add_gravity(self.h, float(g[0]), float(g[1]), float(g[2]))
@@ -272,11 +287,61 @@ class Oracle:
         self.nplanes += 1
         return fid
 
+    def add_gravity(self, ax, ay, az):
+        return self.L.or_add_gravity(self.h, float(ax), float(ay), float(az))
+
     def add_freeze(self):
         self.L.or_add_freeze(self.h)
 
     def unfix(self, fid):
         self.L.or_unfix(self.h, int(fid))
+
+    def add_mesh(self, nodes, type, curvature=None, vel=None):
+        """fix mesh/surface + fix wall/gran mesh (+ fix move/mesh linear vel)."""
+        nd = np.ascontiguousarray(np.asarray(nodes, dtype=np.float64).reshape(-1, 9))
+        vv = None if vel is None else np.ascontiguousarray(vel, dtype=np.float64)
+        fid = self.L.or_add_mesh(self.h, len(nd), _dp(nd), int(type),
+                                 0.0 if curvature is None else float(curvature),
+                                 None if vv is None else _dp(vv))
+        if fid < 0:
+            raise ValueError("oracle: bad mesh")
+        self.mesh_ntri.append(len(nd))
+        self.nmesh += 1
+        return fid
+
+    def mesh_topology(self, k=0):
+        nt = self.mesh_ntri[k]
+        ea = np.zeros((nt, 3), np.int32)
+        ca = np.zeros((nt, 3), np.int32)
+        ob = np.zeros(nt, np.int32)
+        nn = np.zeros(nt, np.int32)
+        self.L.or_mesh_topology(self.h, int(k), _ip(ea), _ip(ca), _ip(ob), _ip(nn))
+        return dict(edge_active=ea, corner_active=ca, obtuse=ob, nneigh=nn)
+
+    def mesh_force(self, k=0):
+        f = np.zeros(3)
+        self.L.or_mesh_force(self.h, int(k), _dp(f))
+        return f
+
+    def mesh_contacts(self, k=0):
+        """(tag, triangle id, history) of the mesh's current contacts."""
+        c = self.L.or_mesh_contacts(self.h, int(k), None, None, None)
+        dw = self.L.or_dnum_wall(self.h)
+        tg = np.zeros(c, np.int32)
+        tr = np.zeros(c, np.int32)
+        hi = np.zeros((c, dw))
+        if c:
+            self.L.or_mesh_contacts(self.h, int(k), _ip(tg), _ip(tr), _dp(hi))
+        return tg, tr, hi
+
+    def mesh_distance(self, k, tri, r, x):
+        """(gap, delta, bary, region) of resolveTriSphereContactBary."""
+        xx = np.ascontiguousarray(x, dtype=np.float64)
+        de = np.zeros(3)
+        ba = np.zeros(3)
+        rg = np.zeros(1, np.int32)
+        g = self.L.or_mesh_distance(self.h, int(k), int(tri), float(r), _dp(xx), _dp(de), _dp(ba), _ip(rg))
+        return g, de, ba, int(rg[0])
 
     def aveforce_vector(self, fid):
         out = np.zeros(3)
