@@ -32,7 +32,7 @@ enum {
 };
 
 enum { DEM_FP64 = 0, DEM_MIXED = 1 };
-enum { DEM_ROLL_OFF = 0, DEM_ROLL_EPSD = 1, DEM_ROLL_EPSD2 = 2, DEM_ROLL_EPSD3 = 3 };
+enum { DEM_ROLL_OFF = 0, DEM_ROLL_EPSD = 1, DEM_ROLL_EPSD2 = 2, DEM_ROLL_EPSD3 = 3, DEM_ROLL_CDTNONLINEAR2 = 4 };
 
 #define DEM_MAXTYPES 8
 #define DEM_MAXPLANES 8
@@ -110,6 +110,35 @@ int dem_download(dem_ctx *ctx, double *x, double *v, double *omega, double *f, d
 /* fix wall/gran ... primitive type <wtype> {x,y,z}plane <pos>
  * (fix_wall_gran.cpp:93-360, primitive_wall_definitions.h:130-148). */
 int dem_add_plane(dem_ctx *ctx, int dim, double pos, int wtype);
+/* Triangle-mesh wall: fix mesh/surface file ... type <wtype> curvature
+ * <cos> (fix_mesh.cpp:86-430, input_mesh_tri.cpp:305-603) + fix wall/gran
+ * ... mesh (fix_wall_gran.cpp:803-980), and with vel != NULL fix
+ * move/mesh linear vx vy vz (mesh_mover_linear.cpp:94-112).  nodes: ntri*9
+ * doubles in file order; triangles with two equal nodes are skipped, the
+ * remaining ones get ids 0.. in order.  curvature <= 0 takes the default
+ * 1-1e-5.  Topology (active edges/corners) is built here on the host.
+ * Returns the mesh index (>= 0) or a negative DEM_ERR_*.  Up to 4 meshes,
+ * box without periodic dimensions. */
+int dem_add_mesh(dem_ctx *ctx, int ntri, const double *nodes, int wtype, double curvature,
+                 const double *vel);
+/* Number of triangles the mesh kept. */
+int dem_mesh_size(const dem_ctx *ctx, int mesh);
+/* Host-only: the topology SurfaceMesh::buildNeighbours + handleCorner
+ * (surface_mesh_I.h:474-560, 1139-1236) computes for a mesh in the given
+ * box: per kept triangle edge_active[3], corner_active[3], obtuse index
+ * (-1 none) and number of edge neighbours.  Output arrays may be NULL. */
+int dem_mesh_topology(int ntri, const double *nodes, double curvature, const double *boxlo,
+                      const double *boxhi, int *ntri_kept, int *edge_active, int *corner_active,
+                      int *obtuse, int *nneigh);
+/* Mesh contact history (FixContactHistoryMesh partner_/contacthistory_,
+ * fix_contact_history_mesh.cpp): every filled slot as (atom tag, triangle
+ * id, slot index, hist[wall dnum]).  NULL arrays: count only. */
+int dem_get_mesh_contacts(dem_ctx *ctx, int mesh, int64_t *count, int *tag, int *tri, int *slot,
+                          double *hist);
+/* Restores mesh contact slots before dem_setup (the restart path of
+ * FixContactHistoryMesh); the next rebuild carries them over. */
+int dem_set_mesh_contacts(dem_ctx *ctx, int mesh, int64_t count, const int *tag, const int *tri,
+                          const int *slot, const double *hist);
 /* fix gravity acceleration (fix_gravity.cpp:301-352, :395-397). */
 int dem_set_gravity(dem_ctx *ctx, double ax, double ay, double az);
 /* fix freeze present (fix_freeze.cpp:121-144; pair_gran.cpp:532-537). */

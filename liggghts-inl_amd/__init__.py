@@ -20,5 +20,7 @@ from .engine import (  # noqa: F401
     library,
     library_path,
     material_tables,
+    mesh_topology,
 )
+from . import meshio  # noqa: F401
 from . import workloads  # noqa: F401

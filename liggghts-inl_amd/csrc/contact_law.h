@@ -346,6 +346,38 @@ __device__ inline void contact_fp64(const Model &M, const Layout &L, const doubl
         T[0] -= rt0;
         T[1] -= rt1;
         T[2] -= rt2;
+    } else if (M.rolling == 4) {
+        // RollingModel<ROLLING_CDTNONLINEAR2> (rolling_model_cdtnonlinear2.h:
+        // 96-172): mu_r Fn reff along the relative rolling velocity, no history
+        const double rmu = tbl(mat, M_MUR, it, jt);
+        double w1, w2, w3;
+        if (c.is_wall) {
+            w1 = wr1; w2 = wr2; w3 = wr3;
+        } else {
+            w1 = c.wi[0] - c.wj[0];
+            w2 = c.wi[1] - c.wj[1];
+            w3 = c.wi[2] - c.wj[2];
+        }
+        const double mag = sqrt(w1 * w1 + w2 * w2 + w3 * w3);
+        if (mag > 0.) {
+            double r0, r1, r2;
+            if (c.is_wall) {
+                r0 = rmu * Fn * w1 / mag * reff;
+                r1 = rmu * Fn * w2 / mag * reff;
+                r2 = rmu * Fn * w3 / mag * reff;
+            } else {
+                const double sc = rmu * Fn * reff / mag;
+                r0 = w1 * sc; r1 = w2 * sc; r2 = w3 * sc;
+            }
+            if (!M.torsion) {
+                const double d = r0 * enx + r1 * eny + r2 * enz;
+                const double n0 = enx * d, n1 = eny * d, n2 = enz * d;
+                r0 = r0 - n0; r1 = r1 - n1; r2 = r2 - n2;
+            }
+            T[0] -= r0;
+            T[1] -= r1;
+            T[2] -= r2;
+        }
     }
 }
 

@@ -10,6 +10,9 @@
 // Neighbour list: full (both directions), CSR rows per particle:
 //   nstart[n+1]; nj[e] = j | shift << 26 | flag << 31; hist[e*dnum + k]
 // Primitive walls: wmask[i] bit k = atom in wall k's list; whist[(i*np+k)*dnum]
+// Mesh walls, per mesh: TriGeo records; per particle a CSR row (mstart) of
+// candidate triangles (mcand, ascending id) and as many contact slots
+// (mpart = triangle id or -1, mhist = wall history), mnp = filled slots
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -56,6 +59,26 @@ struct Planes {
     double pos[kMaxPlanes];
 };
 
+// Triangle of a mesh wall, one 256-byte record per triangle (the contact
+// pass reads one record per candidate).  flags: bit k edge k active, bit
+// 3+k corner k active.  neigh: edge neighbours (first five, mesh ids).
+constexpr int kMaxMeshes = 4;
+constexpr int kMaxMeshSlots = 4096; // candidate triangles (= contact slots) per particle and mesh
+constexpr int kMeshNeigh = 5;
+struct TriGeo {
+    double node[9];
+    double evec[9];
+    double elen[3];
+    double snorm[3];
+    double center[3];
+    double rbound;
+    int obtuse;
+    int flags;
+    int nneigh;
+    int neigh[kMeshNeigh];
+};
+static_assert(sizeof(TriGeo) == 256, "TriGeo is one 256-byte record");
+
 struct Model {
     double dt;
     double cdf;          // contactDistanceFactor
@@ -66,7 +89,7 @@ struct Model {
     double krs;          // coefficientRollingStiffness (epsd3)
     int gravity_on;
     int freeze_on;
-    int rolling;         // 0 off, 1 epsd, 2 epsd2, 3 epsd3
+    int rolling;         // 0 off, 1 epsd, 2 epsd2, 3 epsd3, 4 cdtnonlinear2
     int tdamp;           // tangential_damping
     int torsion;         // rolling torsionTorque
     int dnum;            // history doubles per pair

@@ -17,6 +17,7 @@
 
 #include "../../include/liggghts_dem.h"
 #include "kernels.h"
+#include "mesh_host.h"
 
 using namespace dem;
 
@@ -33,6 +34,30 @@ struct Timer {
     std::vector<EventPair> pool;
     double ms = 0.0;
     int64_t launches = 0;
+};
+
+// One mesh wall (fix mesh/surface + fix wall/gran mesh [+ fix move/mesh
+// linear]).  The host keeps the geometry and advances its copy of the nodes
+// with the same additions as the device, so the re-neighbour trigger of a
+// moving mesh needs no device read-back.
+struct MeshDev {
+    MeshHost h;
+    int wtype = 1;
+    bool moving = false;
+    double vel[3] = {0, 0, 0};
+    TriGeo *tri = nullptr;
+    double *enorm = nullptr;
+    int *gstart = nullptr, *glist = nullptr;
+    MeshGrid grid;
+    double disp[3] = {0, 0, 0};      // displacement since the grid was built
+    std::vector<double> node_re;     // nodes at the last rebuild
+    // per-particle rows (candidates and contact slots), double buffered
+    int *mstart[2] = {}, *mcand[2] = {}, *mpart[2] = {}, *mnp[2] = {};
+    double *mhist[2] = {};
+    int ncap[2] = {0, 0};
+    int64_t scap[2] = {0, 0};
+    int cur = 0;
+    bool have = false;
 };
 
 }  // namespace
@@ -85,6 +110,10 @@ struct dem_ctx {
 
     bool timing = false;
     Timer t_contact, t_integrate, t_rebuild;
+
+    std::vector<MeshDev> meshes;
+    bool mesh_grids_valid = false;    // built for the current radii
+    long long next_reneighbor = -1;   // FixMesh::next_reneighbor
 };
 
 #define HIPCHK(call)                                                              \
@@ -191,7 +220,7 @@ static Layout chain_layout(int normal, int limitf, int rolling)
     L.off_shear = k;
     k += 3;
     L.off_roll = -1;
-    if (rolling) {
+    if (rolling >= 1 && rolling <= 3) {   // epsd*; cdtnonlinear2 keeps no history
         L.off_roll = k;
         k += 3;
     }
@@ -206,7 +235,7 @@ extern "C" dem_ctx *dem_ctx_create(const dem_params *p)
         return nullptr;
     }
     if (!(p->dt > 0.0) || !(p->skin >= 0.0) || !(p->cdf >= 1.0) || p->neigh_every < 1
-        || p->rolling_model < 0 || p->rolling_model > 3 || p->precision < 0 || p->precision > 1
+        || p->rolling_model < 0 || p->rolling_model > 4 || p->precision < 0 || p->precision > 1
         || p->normal_model < 0 || p->normal_model > 2 || p->wall_normal_model < 0 || p->wall_normal_model > 2) {
         g_last_error = "dem_ctx_create: invalid parameters";
         return nullptr;
@@ -269,6 +298,12 @@ static void free_all(dem_ctx *ctx)
     F(ctx->f4); F(ctx->t4); F(ctx->xhold); F(ctx->wmask); F(ctx->ncontact); F(ctx->key);
     F(ctx->sorted); F(ctx->cstart); F(ctx->cursor); F(ctx->scan_tmp); F(ctx->spread);
     F(ctx->d_mat); F(ctx->d_small);
+    for (auto &m : ctx->meshes) {
+        F(m.tri); F(m.enorm); F(m.gstart); F(m.glist);
+        for (int b = 0; b < 2; b++) {
+            F(m.mstart[b]); F(m.mcand[b]); F(m.mpart[b]); F(m.mnp[b]); F(m.mhist[b]);
+        }
+    }
 }
 
 extern "C" void dem_ctx_destroy(dem_ctx *ctx)
@@ -364,6 +399,92 @@ extern "C" int dem_add_plane(dem_ctx *ctx, int dim, double pos, int wtype)
     ctx->P.pos[k] = pos;
     ctx->P.wtype[k] = wtype;
     return DEM_OK;
+}
+
+// TriGeo records of the mesh's current host geometry
+static int upload_mesh(dem_ctx *ctx, MeshDev &m)
+{
+    const MeshHost &h = m.h;
+    std::vector<TriGeo> g(h.ntri);
+    for (int t = 0; t < h.ntri; t++) {
+        TriGeo &r = g[t];
+        for (int q = 0; q < 9; q++) {
+            r.node[q] = h.node[9 * t + q];
+            r.evec[q] = h.evec[9 * t + q];
+        }
+        for (int q = 0; q < 3; q++) {
+            r.elen[q] = h.elen[3 * t + q];
+            r.snorm[q] = h.snorm[3 * t + q];
+            r.center[q] = h.center[3 * t + q];
+        }
+        r.rbound = h.rbound[t];
+        r.obtuse = h.obtuse[t];
+        r.flags = 0;
+        for (int k = 0; k < 3; k++) {
+            if (h.eact[3 * t + k]) r.flags |= 1 << k;
+            if (h.cact[3 * t + k]) r.flags |= 1 << (3 + k);
+        }
+        r.nneigh = h.nneigh[t];
+        for (int k = 0; k < kMeshNeigh; k++) r.neigh[k] = h.neigh[kMeshNeighMax * t + k];
+    }
+    if (dev_alloc(ctx, &m.tri, h.ntri) || dev_alloc(ctx, &m.enorm, 9 * (size_t)h.ntri)) return DEM_ERR_HIP;
+    HIPCHK(hipMemcpy(m.tri, g.data(), sizeof(TriGeo) * h.ntri, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(m.enorm, h.enorm.data(), sizeof(double) * 9 * h.ntri, hipMemcpyHostToDevice));
+    return DEM_OK;
+}
+
+extern "C" int dem_mesh_topology(int ntri, const double *nodes, double curvature, const double *boxlo,
+                                 const double *boxhi, int *ntri_kept, int *edge_active, int *corner_active,
+                                 int *obtuse, int *nneigh)
+{
+    if (ntri < 1 || !nodes || !boxlo || !boxhi || !ntri_kept)
+        return fail(nullptr, DEM_ERR_ARG, "dem_mesh_topology: bad arguments");
+    MeshHost h;
+    mesh_build(h, ntri, nodes, curvature, boxlo, boxhi);
+    *ntri_kept = h.ntri;
+    for (int t = 0; t < h.ntri; t++) {
+        for (int k = 0; k < 3; k++) {
+            if (edge_active) edge_active[3 * t + k] = h.eact[3 * t + k];
+            if (corner_active) corner_active[3 * t + k] = h.cact[3 * t + k];
+        }
+        if (obtuse) obtuse[t] = h.obtuse[t];
+        if (nneigh) nneigh[t] = h.nneigh[t];
+    }
+    return DEM_OK;
+}
+
+extern "C" int dem_add_mesh(dem_ctx *ctx, int ntri, const double *nodes, int wtype, double curvature,
+                            const double *vel)
+{
+    if (!ctx) return fail(nullptr, DEM_ERR_ARG, "null context");
+    if (ntri < 1 || !nodes || wtype < 1 || wtype > DEM_MAXTYPES || (int)ctx->meshes.size() >= kMaxMeshes)
+        return fail(ctx, DEM_ERR_ARG, "dem_add_mesh: bad arguments or too many meshes");
+    for (int d = 0; d < 3; d++)
+        if (ctx->p.periodic[d])
+            return fail(ctx, DEM_ERR_ARG, "dem_add_mesh: mesh walls need a box without periodic dimensions");
+    hipSetDevice(ctx->p.device);
+    ctx->meshes.emplace_back();
+    MeshDev &m = ctx->meshes.back();
+    mesh_build(m.h, ntri, nodes, curvature, ctx->p.boxlo, ctx->p.boxhi);
+    if (m.h.ntri == 0) {
+        ctx->meshes.pop_back();
+        return fail(ctx, DEM_ERR_ARG, "dem_add_mesh: no valid triangle");
+    }
+    m.wtype = wtype;
+    if (vel) {
+        m.moving = true;
+        for (int q = 0; q < 3; q++) m.vel[q] = vel[q];
+    }
+    m.node_re = m.h.node;
+    int rc = upload_mesh(ctx, m);
+    if (rc) return rc;
+    return (int)ctx->meshes.size() - 1;
+}
+
+extern "C" int dem_mesh_size(const dem_ctx *ctx, int mesh)
+{
+    if (!ctx || mesh < 0 || mesh >= (int)ctx->meshes.size()) return 0;
+    return ctx->meshes[mesh].h.ntri;
 }
 
 extern "C" int dem_set_gravity(dem_ctx *ctx, double ax, double ay, double az)
@@ -518,6 +639,8 @@ extern "C" int dem_upload_state(dem_ctx *ctx, int n, const double *x, const doub
     }
     ctx->have_list = false;
     ctx->list_valid = false;
+    ctx->mesh_grids_valid = false;
+    for (auto &m : ctx->meshes) m.have = false;
     int rc = setup_grid(ctx);
     if (rc) return rc;
     return DEM_OK;
@@ -533,6 +656,87 @@ static int ensure_list_cap(dem_ctx *ctx, int b, int64_t entries)
     ctx->ecap[b] = c;
     return DEM_OK;
 }
+
+// Candidate triangles and contact slots of every particle for one mesh, after
+// the particles were re-sorted (perm = new -> old): FixNeighlistMesh::
+// pre_force + FixContactHistoryMesh::sort_contacts / pre_force.
+static int mesh_rebuild(dem_ctx *ctx, MeshDev &m)
+{
+    const int n = ctx->n;
+    hipStream_t s = ctx->stream;
+    const int b = m.have ? 1 - m.cur : m.cur;
+    const int ob = m.cur;
+    if (n + 1 > m.ncap[b]) {
+        if (dev_alloc(ctx, &m.mstart[b], n + 1) || dev_alloc(ctx, &m.mnp[b], n + 1)) return DEM_ERR_HIP;
+        m.ncap[b] = n + 1;
+    }
+    MeshNbArgs na{};
+    na.xr = ctx->xr[ctx->cur];
+    na.tri = m.tri;
+    na.enorm = m.enorm;
+    na.gstart = m.gstart;
+    na.glist = m.glist;
+    for (int d = 0; d < 3; d++) {
+        na.glo[d] = m.grid.lo[d];
+        na.ginv[d] = m.grid.inv[d];
+        na.gn[d] = m.grid.n[d];
+        na.disp[d] = m.disp[d];
+    }
+    na.cdf = ctx->p.cdf;
+    // FixNeighlistMesh::pre_force: skin for a moving mesh, half of it else
+    na.threshold = m.moving ? ctx->p.skin : 0.5 * ctx->p.skin;
+    na.n = n;
+    na.count = m.mstart[b];
+    HIPCHK(hipMemsetAsync(m.mstart[b] + n, 0, sizeof(int), s));
+    HIPCHK(hipMemsetAsync(ctx->d_small + 3, 0, sizeof(int), s));
+    launch_mesh_count(na, s);
+    launch_row_max(m.mstart[b], n, ctx->d_small + 3, s);
+    launch_scan(m.mstart[b], n + 1, ctx->scan_tmp, s);
+    KCHK();
+    int hs[2] = {0, 0};
+    HIPCHK(hipMemcpyAsync(&hs[0], m.mstart[b] + n, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(&hs[1], ctx->d_small + 3, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (hs[1] > kMaxMeshSlots)
+        return fail(ctx, DEM_ERR_OVERFLOW, "mesh contact history overflow: a particle has more than 4096 "
+                                           "candidate triangles");
+    const int64_t tot = hs[0];
+    const int dw = ctx->M.wdnum;
+    if (tot > m.scap[b]) {
+        const int64_t c = tot + tot / 4 + 256;
+        if (dev_alloc(ctx, &m.mcand[b], (size_t)c) || dev_alloc(ctx, &m.mpart[b], (size_t)c)
+            || dev_alloc(ctx, &m.mhist[b], (size_t)c * std::max(dw, 1)))
+            return DEM_ERR_NOMEM;
+        m.scap[b] = c;
+    }
+    na.mstart = m.mstart[b];
+    na.mcand = m.mcand[b];
+    launch_mesh_fill(na, s);
+    KCHK();
+    MeshRemapArgs ra{};
+    ra.perm = ctx->sorted;
+    ra.have_old = m.have ? 1 : 0;
+    ra.ostart = m.mstart[ob];
+    ra.opart = m.mpart[ob];
+    ra.ohist = m.mhist[ob];
+    ra.onp = m.mnp[ob];
+    ra.mstart = m.mstart[b];
+    ra.mcand = m.mcand[b];
+    ra.mpart = m.mpart[b];
+    ra.mhist = m.mhist[b];
+    ra.mnp = m.mnp[b];
+    ra.n = n;
+    ra.dnum = dw;
+    launch_mesh_remap(ra, s);
+    KCHK();
+    m.cur = b;
+    m.have = true;
+    // MultiNodeMesh::storeNodePosRebuild
+    m.node_re = m.h.node;
+    return DEM_OK;
+}
+
+static int setup_mesh_grids(dem_ctx *ctx);
 
 // Neighbour rebuild: pre_exchange + pbc + sort + list + wall lists.
 static int rebuild(dem_ctx *ctx)
@@ -621,6 +825,14 @@ static int rebuild(dem_ctx *ctx)
     ctx->have_list = true;
     ctx->list_valid = true;
     ctx->ago = 0;
+    if (!ctx->meshes.empty() && !ctx->mesh_grids_valid) {
+        const int rc = setup_mesh_grids(ctx);
+        if (rc) return rc;
+    }
+    for (auto &m : ctx->meshes) {
+        const int rc = mesh_rebuild(ctx, m);
+        if (rc) return rc;
+    }
     timer_end(ctx, ctx->t_rebuild);
     return DEM_OK;
 }
@@ -643,11 +855,94 @@ static int force_pass(dem_ctx *ctx, int shearupdate)
     fa.shearupdate = shearupdate;
     fa.M = ctx->M;
     fa.P = ctx->P;
+    fa.defer_freeze = ctx->meshes.empty() ? 0 : 1;
     timer_begin(ctx, ctx->t_contact);
     if (ctx->p.precision == DEM_MIXED) launch_force_mixed(fa, ctx->stream);
     else launch_force_fp64(fa, ctx->stream);
     timer_end(ctx, ctx->t_contact);
     KCHK();
+    for (size_t k = 0; k < ctx->meshes.size(); k++) {
+        MeshDev &m = ctx->meshes[k];
+        MeshForceArgs ma{};
+        ma.xr = fa.xr; ma.vm = fa.vm; ma.wt = fa.wt;
+        ma.f4 = fa.f4; ma.t4 = fa.t4;
+        ma.tri = m.tri;
+        ma.mstart = m.mstart[m.cur];
+        ma.mcand = m.mcand[m.cur];
+        ma.mpart = m.mpart[m.cur];
+        ma.mhist = m.mhist[m.cur];
+        ma.mnp = m.mnp[m.cur];
+        ma.mat = ctx->d_mat;
+        ma.err = ctx->d_small;
+        ma.n = ctx->n;
+        ma.shearupdate = shearupdate;
+        ma.wtype = m.wtype;
+        ma.moving = m.moving ? 1 : 0;
+        for (int q = 0; q < 3; q++) ma.vel[q] = m.vel[q];
+        ma.curvature = m.h.curvature;
+        ma.precision = m.h.precision;
+        ma.apply_freeze = k + 1 == ctx->meshes.size();
+        ma.M = ctx->M;
+        launch_mesh_force(ma, ctx->stream);
+        KCHK();
+    }
+    return DEM_OK;
+}
+
+// fix move/mesh linear, MeshMoverLinear::initial_integrate: the device nodes
+// and the host copy get the same additions
+static int move_meshes(dem_ctx *ctx)
+{
+    for (auto &m : ctx->meshes) {
+        if (!m.moving) continue;
+        double dx[3];
+        for (int q = 0; q < 3; q++) dx[q] = m.vel[q] * ctx->p.dt;
+        launch_mesh_move(m.tri, m.h.ntri, dx[0], dx[1], dx[2], ctx->stream);
+        KCHK();
+        for (int t = 0; t < m.h.ntri; t++) {
+            for (int k = 0; k < 9; k++) m.h.node[9 * t + k] = m.h.node[9 * t + k] + dx[k % 3];
+            for (int q = 0; q < 3; q++) m.h.center[3 * t + q] = m.h.center[3 * t + q] + dx[q];
+        }
+        for (int q = 0; q < 3; q++) m.disp[q] += dx[q];
+    }
+    return DEM_OK;
+}
+
+// MultiNodeMesh::decideRebuild on a step without re-neighbouring: a node
+// moved more than half the skin since the last build -> rebuild next step
+static void mesh_trigger(dem_ctx *ctx)
+{
+    const double trig = 0.25 * ctx->p.skin * ctx->p.skin;
+    for (auto &m : ctx->meshes) {
+        if (!m.moving) continue;
+        for (int q = 0; q < 3 * m.h.ntri; q++) {
+            const double dx = m.h.node[3 * q] - m.node_re[3 * q];
+            const double dy = m.h.node[3 * q + 1] - m.node_re[3 * q + 1];
+            const double dz = m.h.node[3 * q + 2] - m.node_re[3 * q + 2];
+            if (dx * dx + dy * dy + dz * dz > trig) {
+                ctx->next_reneighbor = ctx->ntimestep + 1;
+                return;
+            }
+        }
+    }
+}
+
+// triangle grids for the candidate search, in each mesh's current frame
+static int setup_mesh_grids(dem_ctx *ctx)
+{
+    double ext = 0.0;
+    for (int d = 0; d < 3; d++) ext = std::max(ext, ctx->p.boxhi[d] - ctx->p.boxlo[d]);
+    for (auto &m : ctx->meshes) {
+        const double thr = m.moving ? ctx->p.skin : 0.5 * ctx->p.skin;
+        const double reach = ctx->maxrad * ctx->p.cdf + thr + 1e-9 * ext;
+        mesh_grid_build(m.h, reach, m.grid);
+        if (dev_alloc(ctx, &m.gstart, m.grid.start.size()) || dev_alloc(ctx, &m.glist, m.grid.list.size()))
+            return DEM_ERR_HIP;
+        HIPCHK(hipMemcpy(m.gstart, m.grid.start.data(), sizeof(int) * m.grid.start.size(), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(m.glist, m.grid.list.data(), sizeof(int) * m.grid.list.size(), hipMemcpyHostToDevice));
+        for (int q = 0; q < 3; q++) m.disp[q] = 0.0;
+    }
+    ctx->mesh_grids_valid = true;
     return DEM_OK;
 }
 
@@ -676,6 +971,7 @@ static int check_err_flag(dem_ctx *ctx)
     if (h & 1)
         return fail(ctx, DEM_ERR_STATE,
                     "ilegal situation, particle positions identical. Most probable reason: Simulation diverged");
+    if (h & 4) return fail(ctx, DEM_ERR_OVERFLOW, "mesh contact history: no free contact slot (internal error)");
     return DEM_OK;
 }
 
@@ -701,7 +997,9 @@ extern "C" int dem_setup(dem_ctx *ctx)
     if (!ctx) return fail(nullptr, DEM_ERR_ARG, "null context");
     if (!ctx->mat_set) return fail(ctx, DEM_ERR_ARG, "dem_setup: set materials first");
     hipSetDevice(ctx->p.device);
-    int rc = rebuild(ctx);
+    int rc = setup_mesh_grids(ctx);
+    if (rc) return rc;
+    rc = rebuild(ctx);
     if (rc) return rc;
     rc = force_pass(ctx, 0);
     if (rc) return rc;
@@ -711,8 +1009,12 @@ extern "C" int dem_setup(dem_ctx *ctx)
 // Neighbor::decide + check_distance (neighbor.cpp:1362-1465)
 static int decide(dem_ctx *ctx, int *out)
 {
-    ctx->ago++;
     *out = 0;
+    if (ctx->ntimestep == ctx->next_reneighbor) {
+        *out = 1;
+        return DEM_OK;
+    }
+    ctx->ago++;
     if (!(ctx->ago >= ctx->p.neigh_delay && ctx->ago % ctx->p.neigh_every == 0)) return DEM_OK;
     if (!ctx->p.neigh_check) {
         *out = 1;
@@ -738,12 +1040,15 @@ extern "C" int dem_step(dem_ctx *ctx, int nsteps)
     int rc;
     for (int st = 0; st < nsteps; st++) {
         ctx->ntimestep++;
-        if (st == 0 && (rc = integrate(ctx, 1))) return rc;
+        if (st == 0 && ((rc = integrate(ctx, 1)) || (rc = move_meshes(ctx)))) return rc;
         int nflag = 0;
         if ((rc = decide(ctx, &nflag))) return rc;
         if (nflag && (rc = rebuild(ctx))) return rc;
+        if (!nflag) mesh_trigger(ctx);
         if ((rc = force_pass(ctx, 1))) return rc;
-        if ((rc = integrate(ctx, st == nsteps - 1 ? 2 : 3))) return rc;
+        const bool last = st == nsteps - 1;
+        if ((rc = integrate(ctx, last ? 2 : 3))) return rc;
+        if (!last && (rc = move_meshes(ctx))) return rc;
     }
     return check_err_flag(ctx);
 }
@@ -898,6 +1203,104 @@ extern "C" int dem_set_wall_history(dem_ctx *ctx, const double *hist)
     for (int i = 0; i < n; i++)
         for (int k = 0; k < wdn; k++) h[(size_t)i * wdn + k] = hist[(size_t)ho[i] * wdn + k];
     HIPCHK(hipMemcpy(ctx->whist[c], h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+    return DEM_OK;
+}
+
+extern "C" int dem_get_mesh_contacts(dem_ctx *ctx, int mesh, int64_t *count, int *tag, int *tri, int *slot,
+                                     double *hist)
+{
+    if (!ctx || !count || mesh < 0 || mesh >= (int)ctx->meshes.size())
+        return fail(ctx, DEM_ERR_ARG, "dem_get_mesh_contacts: bad arguments");
+    hipSetDevice(ctx->p.device);
+    MeshDev &m = ctx->meshes[mesh];
+    const int n = ctx->n, dw = ctx->M.wdnum;
+    if (!m.have || n == 0) {
+        *count = 0;
+        return DEM_OK;
+    }
+    std::vector<int> ms(n + 1), tg(n);
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipMemcpy(ms.data(), m.mstart[m.cur], (n + 1) * sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(tg.data(), ctx->tag[ctx->cur], n * sizeof(int), hipMemcpyDeviceToHost));
+    const int64_t S = ms[n];
+    std::vector<int> pt(std::max<int64_t>(S, 1));
+    std::vector<double> hh(std::max<int64_t>(S, 1) * std::max(dw, 1));
+    if (S) {
+        HIPCHK(hipMemcpy(pt.data(), m.mpart[m.cur], S * sizeof(int), hipMemcpyDeviceToHost));
+        if (dw) HIPCHK(hipMemcpy(hh.data(), m.mhist[m.cur], S * dw * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    int64_t c = 0;
+    for (int i = 0; i < n; i++)
+        for (int k = ms[i]; k < ms[i + 1]; k++) {
+            if (pt[k] < 0) continue;
+            if (tag) {
+                if (c >= *count) return fail(ctx, DEM_ERR_ARG, "dem_get_mesh_contacts: output arrays too small");
+                tag[c] = tg[i];
+                if (tri) tri[c] = pt[k];
+                if (slot) slot[c] = k - ms[i];
+                if (hist)
+                    for (int d = 0; d < dw; d++) hist[(size_t)c * dw + d] = hh[(size_t)k * dw + d];
+            }
+            c++;
+        }
+    *count = c;
+    return DEM_OK;
+}
+
+extern "C" int dem_set_mesh_contacts(dem_ctx *ctx, int mesh, int64_t count, const int *tag, const int *tri,
+                                     const int *slot, const double *hist)
+{
+    if (!ctx || mesh < 0 || mesh >= (int)ctx->meshes.size() || count < 0
+        || (count > 0 && (!tag || !tri || !slot || (!hist && ctx->M.wdnum))))
+        return fail(ctx, DEM_ERR_ARG, "dem_set_mesh_contacts: bad arguments");
+    hipSetDevice(ctx->p.device);
+    MeshDev &m = ctx->meshes[mesh];
+    const int n = ctx->n, c = ctx->cur, dw = ctx->M.wdnum;
+    std::vector<int> tg(n);
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (n) HIPCHK(hipMemcpy(tg.data(), ctx->tag[c], n * sizeof(int), hipMemcpyDeviceToHost));
+    std::unordered_map<int, int> idx;
+    idx.reserve(2 * n + 1);
+    for (int i = 0; i < n; i++) idx[tg[i]] = i;
+    std::vector<int> len(n, 0), np(n, 0);
+    for (int64_t k = 0; k < count; k++) {
+        auto it = idx.find(tag[k]);
+        if (it == idx.end() || slot[k] < 0 || slot[k] >= kMaxMeshSlots || tri[k] < 0 || tri[k] >= m.h.ntri)
+            return fail(ctx, DEM_ERR_ARG, "dem_set_mesh_contacts: unknown tag, bad slot or triangle");
+        len[it->second] = std::max(len[it->second], slot[k] + 1);
+        np[it->second]++;
+    }
+    std::vector<int> st(n + 1, 0);
+    for (int i = 0; i < n; i++) st[i + 1] = st[i] + len[i];
+    const int64_t S = st[n];
+    std::vector<int> pt(std::max<int64_t>(S, 1), -1);
+    std::vector<double> hh(std::max<int64_t>(S, 1) * std::max(dw, 1), 0.0);
+    for (int64_t k = 0; k < count; k++) {
+        const int i = idx[tag[k]];
+        const int64_t e = st[i] + slot[k];
+        if (pt[e] != -1) return fail(ctx, DEM_ERR_ARG, "dem_set_mesh_contacts: slot given twice");
+        pt[e] = tri[k];
+        for (int d = 0; d < dw; d++) hh[(size_t)e * dw + d] = hist[(size_t)k * dw + d];
+    }
+    const int b = m.cur;
+    if (n + 1 > m.ncap[b]) {
+        if (dev_alloc(ctx, &m.mstart[b], n + 1) || dev_alloc(ctx, &m.mnp[b], n + 1)) return DEM_ERR_HIP;
+        m.ncap[b] = n + 1;
+    }
+    if (S > m.scap[b]) {
+        if (dev_alloc(ctx, &m.mcand[b], (size_t)S) || dev_alloc(ctx, &m.mpart[b], (size_t)S)
+            || dev_alloc(ctx, &m.mhist[b], (size_t)S * std::max(dw, 1)))
+            return DEM_ERR_NOMEM;
+        m.scap[b] = S;
+    }
+    HIPCHK(hipMemcpy(m.mstart[b], st.data(), (n + 1) * sizeof(int), hipMemcpyHostToDevice));
+    if (n) HIPCHK(hipMemcpy(m.mnp[b], np.data(), n * sizeof(int), hipMemcpyHostToDevice));
+    if (S) {
+        HIPCHK(hipMemcpy(m.mpart[b], pt.data(), S * sizeof(int), hipMemcpyHostToDevice));
+        if (dw) HIPCHK(hipMemcpy(m.mhist[b], hh.data(), S * dw * sizeof(double), hipMemcpyHostToDevice));
+    }
+    m.have = true;
+    ctx->list_valid = false;
     return DEM_OK;
 }
 

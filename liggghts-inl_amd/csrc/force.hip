@@ -221,7 +221,7 @@ __global__ void __launch_bounds__(kBlock) k_force(ForceArgs a)
         F[1] += massone * M.gacc[1];
         F[2] += massone * M.gacc[2];
     }
-    if (M.freeze_on && fzi) {
+    if (M.freeze_on && fzi && !a.defer_freeze) {
         F[0] = F[1] = F[2] = 0.0;
         T[0] = T[1] = T[2] = 0.0;
     }
@@ -278,7 +278,8 @@ static void launch_wn(const ForceArgs &a, hipStream_t s)
     }
 }
 
-// The mixed form applies to Hertz chains; hysteretic chains run in FP64.
+// The mixed form applies to Hertz chains with Hertz-family rolling
+// (epsd*); hysteretic chains and cdtnonlinear2 rolling run in FP64.
 template <bool MIXED>
 static void launch_force(const ForceArgs &a, hipStream_t s)
 {
@@ -286,7 +287,10 @@ static void launch_force(const ForceArgs &a, hipStream_t s)
     switch (a.M.lp.normal) {
     case NM_HYST1: launch_wn<false, NM_HYST1>(a, s); break;
     case NM_HYST2: launch_wn<false, NM_HYST2>(a, s); break;
-    default: launch_wn<MIXED, NM_HERTZ>(a, s); break;
+    default:
+        if (MIXED && a.M.rolling != 4) launch_wn<true, NM_HERTZ>(a, s);
+        else launch_wn<false, NM_HERTZ>(a, s);
+        break;
     }
 }
 

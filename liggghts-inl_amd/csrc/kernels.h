@@ -18,6 +18,7 @@ struct ForceArgs {
     int *err;            // device error flags
     int n;
     int shearupdate;
+    int defer_freeze;    // mesh walls follow: they apply fix freeze
     Model M;
     Planes P;
 };
@@ -80,12 +81,75 @@ struct IntegArgs {
     double dtf, dtv, dtfrotate;
 };
 
+// Mesh wall neighbour build: candidate triangles per particle through a
+// uniform grid of triangles (built on the host in the mesh's initial frame;
+// disp = how far the mesh has moved since).
+struct MeshNbArgs {
+    const double4 *xr;
+    const TriGeo *tri;
+    const double *enorm;     // 9 per triangle
+    const int *gstart;       // per grid cell, CSR into glist
+    const int *glist;        // triangle ids, ascending per cell
+    double glo[3], ginv[3];
+    int gn[3];
+    double disp[3];
+    double cdf, threshold;
+    int n;
+    int *count;              // count pass
+    const int *mstart;       // fill pass
+    int *mcand;
+};
+
+// Carry each particle's contact slots from the previous build (old index
+// space, perm = new -> old) into the new rows: sort_contacts then
+// cleanUpContactJumps, then the first slots are copied.
+struct MeshRemapArgs {
+    const int *perm;
+    const int *ostart;
+    int *opart;
+    double *ohist;
+    int *onp;
+    int have_old;
+    const int *mstart, *mcand;
+    int *mpart;
+    double *mhist;
+    int *mnp;
+    int n, dnum;
+};
+
+struct MeshForceArgs {
+    const double4 *xr, *vm, *wt;
+    double4 *f4, *t4;
+    const TriGeo *tri;
+    const int *mstart, *mcand;
+    int *mpart;
+    double *mhist;
+    int *mnp;
+    const double *mat;
+    int *err;                // bit 2: more contacts than slots
+    int n, shearupdate;
+    int wtype, moving;
+    double vel[3];
+    double curvature, precision;
+    int apply_freeze;        // this pass ends the post_force chain
+    Model M;
+};
+
+void launch_mesh_count(const MeshNbArgs &a, hipStream_t s);
+void launch_mesh_fill(const MeshNbArgs &a, hipStream_t s);
+void launch_mesh_remap(const MeshRemapArgs &a, hipStream_t s);
+void launch_mesh_force(const MeshForceArgs &a, hipStream_t s);
+// MeshMoverLinear: node += dx, center += dx for ntri triangles
+void launch_mesh_move(TriGeo *tri, int ntri, double dx, double dy, double dz, hipStream_t s);
+
 void launch_bin(const BinArgs &a, hipStream_t s);
 void launch_scan(int *data, int n, int *tmp, hipStream_t s);   // exclusive, data[n] = total
 void launch_scatter(const uint32_t *key, int *cursor, int *sorted, int n, hipStream_t s);
 void launch_cellsort(const int *cstart, int *sorted, int nkeys, hipStream_t s);
 void launch_permute(const PermuteArgs &a, hipStream_t s);
 void launch_neigh_count(const NeighArgs &a, hipStream_t s);
+// *maxcount = max(*maxcount, max of count[0..n))
+void launch_row_max(const int *count, int n, int *maxcount, hipStream_t s);
 void launch_neigh_fill(const NeighArgs &a, hipStream_t s);
 void launch_force_fp64(const ForceArgs &a, hipStream_t s);
 void launch_force_mixed(const ForceArgs &a, hipStream_t s);

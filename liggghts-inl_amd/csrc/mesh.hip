@@ -1,0 +1,318 @@
+// mesh.hip -- triangle-mesh walls on the device.
+//
+// Particle-centric restatement of the reference's triangle-centric loops:
+// each particle owns a row of candidate triangles (ascending id, the order in
+// which FixWallGran::post_force_mesh visits a particle's triangles,
+// fix_wall_gran.cpp:803-980) and as many contact slots
+// (FixContactHistoryMesh, fix_contact_history_mesh_I.h:51-215).  One thread
+// handles one particle, so the slot bookkeeping (first free slot, coplanar
+// hand-over, clean-up) runs in the reference's order without atomics.
+//
+// k_mesh_nb<FILL>   FixNeighlistMesh::handleTriangle (fix_neighlist_mesh.cpp:
+//                   355-410) with resolveTriSphereNeighbuild, per particle
+// k_mesh_remap      FixContactHistoryMesh::sort_contacts (:375-402) and
+//                   cleanUpContactJumps (:466-500) at a rebuild
+// k_mesh_force<WN>  post_force_mesh + handleContact + cleanUpContacts
+// k_mesh_move       MeshMoverLinear::initial_integrate (mesh_mover_linear.cpp:
+//                   94-112)
+#include "contact_law.h"
+#include "kernels.h"
+#include "mesh_geom.h"
+
+namespace dem {
+
+template <bool FILL>
+__global__ void __launch_bounds__(kBlock) k_mesh_nb(MeshNbArgs a)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= a.n) return;
+    const double4 p = a.xr[i];
+    const double q[3] = {p.x - a.disp[0], p.y - a.disp[1], p.z - a.disp[2]};
+    int c[3];
+    bool in = true;
+    for (int d = 0; d < 3; d++) {
+        const double f = floor((q[d] - a.glo[d]) * a.ginv[d]);
+        in = in && f >= 0.0 && f < (double)a.gn[d];
+        c[d] = in ? (int)f : 0;
+    }
+    int k = 0;
+    if (in) {
+        const int cell = (c[2] * a.gn[1] + c[1]) * a.gn[0] + c[0];
+        const double cs[3] = {p.x, p.y, p.z};
+        const double r = p.w * a.cdf;
+        const int base = FILL ? a.mstart[i] : 0;
+        for (int e = a.gstart[cell]; e < a.gstart[cell + 1]; e++) {
+            const int t = a.glist[e];
+            if (tri_neighbuild(a.tri[t], a.enorm + 9 * (size_t)t, r, cs, a.threshold)) {
+                if (FILL) a.mcand[base + k] = t;
+                k++;
+            }
+        }
+    }
+    if (!FILL) a.count[i] = k;
+}
+
+__global__ void __launch_bounds__(kBlock) k_mesh_remap(MeshRemapArgs a)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= a.n) return;
+    const int base = a.mstart[i];
+    const int nn = a.mstart[i + 1] - base;
+    const int dnum = a.dnum;
+    int np = 0, onn = 0;
+    int *pt = nullptr;
+    double *oh = nullptr;
+    if (a.have_old) {
+        const int o = a.perm[i];
+        const int ob = a.ostart[o];
+        onn = a.ostart[o + 1] - ob;
+        pt = a.opart + ob;
+        oh = a.ohist + (size_t)ob * dnum;
+        np = a.onp[o];
+        // sort_contacts: first empty <-> last filled until compact
+        for (;;) {
+            int fe = -1, lf = -1;
+            for (int j = 0; j < onn; j++) {
+                if (fe == -1 && pt[j] == -1) fe = j;
+                if (pt[j] >= 0) lf = j;
+            }
+            if (!(fe > -1 && lf > -1 && fe < lf)) break;
+            const int tt = pt[fe];
+            pt[fe] = pt[lf];
+            pt[lf] = tt;
+            for (int d = 0; d < dnum; d++) {
+                const double x = oh[(size_t)fe * dnum + d];
+                oh[(size_t)fe * dnum + d] = oh[(size_t)lf * dnum + d];
+                oh[(size_t)lf * dnum + d] = x;
+            }
+        }
+        // cleanUpContactJumps: a partner that is no longer a candidate is
+        // cleared and swapped with the last filled slot
+        int ip = 0;
+        while (ip < np && ip < onn) {
+            const int tri = pt[ip];
+            bool found = false;
+            for (int k = 0; k < nn; k++)
+                if (a.mcand[base + k] == tri) {
+                    found = true;
+                    break;
+                }
+            if (found) {
+                ip++;
+                continue;
+            }
+            pt[ip] = -1;
+            for (int d = 0; d < dnum; d++) oh[(size_t)ip * dnum + d] = 0.0;
+            const int last = np - 1;
+            const int tt = pt[ip];
+            pt[ip] = pt[last];
+            pt[last] = tt;
+            for (int d = 0; d < dnum; d++) {
+                const double x = oh[(size_t)ip * dnum + d];
+                oh[(size_t)ip * dnum + d] = oh[(size_t)last * dnum + d];
+                oh[(size_t)last * dnum + d] = x;
+            }
+            np--;
+        }
+    }
+    for (int k = 0; k < nn; k++) {
+        const bool carry = k < np && k < onn;
+        a.mpart[base + k] = carry ? pt[k] : -1;
+        for (int d = 0; d < dnum; d++)
+            a.mhist[(size_t)(base + k) * dnum + d] = carry ? oh[(size_t)k * dnum + d] : 0.0;
+    }
+    a.mnp[i] = np;
+}
+
+// During a force pass a slot's keep flag (FixContactHistoryMesh keepflag_)
+// rides in bit 30 of its partner id; -1 marks an empty slot.
+constexpr int kKeep = 1 << 30;
+constexpr int kIdMask = kKeep - 1;
+
+template <int WN>
+__global__ void __launch_bounds__(kBlock) k_mesh_force(MeshForceArgs a)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= a.n) return;
+    const Model &M = a.M;
+    const int base = a.mstart[i];
+    const int nn = a.mstart[i + 1] - base;
+    const double4 wi = a.wt[i];
+    const int meta = (int)wi.w;
+    const bool frozen = a.apply_freeze && M.freeze_on && (meta >> 4) != 0;
+    if (nn > 0) {
+        const double4 pi = a.xr[i];
+        const double4 vmi = a.vm[i];
+        const double cs[3] = {pi.x, pi.y, pi.z};
+        const double radi = pi.w;
+        const double cdm = M.cdf - 1.0;
+        const int dnum = M.wdnum;
+        int *part = a.mpart + base;
+        double *hist = a.mhist + (size_t)base * dnum;
+        int np = a.mnp[i];
+        double F[3] = {0.0, 0.0, 0.0}, T[3] = {0.0, 0.0, 0.0};
+        for (int k = 0; k < nn; k++) {
+            const int t = a.mcand[base + k];
+            const TriGeo &g = a.tri[t];
+            double delta[3], bary[3];
+            int region = -1;
+            const double deltan = tri_contact(g, radi, cs, a.precision, delta, bary, region);
+            if (deltan > M.cutneighmax) continue;
+            const bool intersect = deltan <= 0;
+            if (!(deltan <= 0 || deltan < cdm * radi)) continue;
+            // FixContactHistoryMesh::handleContact
+            int slot = -1;
+            for (int s = 0; s < nn; s++) {
+                const int pv = part[s];
+                if (pv >= 0 && (pv & kIdMask) == t) {
+                    slot = s;
+                    break;
+                }
+            }
+            if (slot >= 0) {
+                part[slot] = t | kKeep;
+            } else {
+                const bool face = region == 7;
+                if (face) {
+                    bool already = false;
+                    for (int s = 0; s < nn && !already; s++) {
+                        const int pv = part[s];
+                        if (pv < 0 || !(pv & kKeep)) continue;
+                        const int pp = pv & kIdMask;
+                        if (pp != t && tri_coplanar_node_neighs(a.tri[pp], t, g, a.curvature, a.precision))
+                            already = true;
+                    }
+                    if (already) continue;
+                }
+                for (int s = 0; s < nn; s++)
+                    if (part[s] == -1) {
+                        slot = s;
+                        break;
+                    }
+                if (slot < 0 || np >= nn) {
+                    atomicOr(a.err, 4);
+                    continue;
+                }
+                part[slot] = t | kKeep;
+                double *h = hist + (size_t)slot * dnum;
+                for (int d = 0; d < dnum; d++) h[d] = 0.0;
+                np++;
+                if (face)
+                    for (int s = 0; s < nn; s++) {
+                        const int pv = part[s];
+                        if (pv < 0) continue;
+                        const int pp = pv & kIdMask;
+                        if (pp != t && tri_coplanar_node_neighs(a.tri[pp], t, g, a.curvature, a.precision))
+                            for (int d = 0; d < dnum; d++) h[d] = hist[(size_t)s * dnum + d];
+                    }
+            }
+            double *hp = hist + (size_t)slot * dnum;
+            if (intersect) {
+                ContactIn cw;
+                cw.is_wall = true;
+                cw.itype = meta & 15;
+                cw.jtype = a.wtype;
+                cw.radi = radi;
+                cw.radj = 0.0;
+                cw.deltan_in = -deltan;
+                cw.delta[0] = -delta[0]; cw.delta[1] = -delta[1]; cw.delta[2] = -delta[2];
+                cw.r = cw.radi - cw.deltan_in;
+                cw.rinv = 1.0 / cw.r;
+                cw.radsum = cw.radi;
+                cw.mi = vmi.w; cw.mj = 0.0; cw.meff = vmi.w;
+                cw.vi[0] = vmi.x; cw.vi[1] = vmi.y; cw.vi[2] = vmi.z;
+                for (int q = 0; q < 3; q++)
+                    cw.vj[q] = a.moving ? (bary[0] * a.vel[q] + bary[1] * a.vel[q] + bary[2] * a.vel[q]) : 0.0;
+                cw.wi[0] = wi.x; cw.wi[1] = wi.y; cw.wi[2] = wi.z;
+                cw.wj[0] = 0.0; cw.wj[1] = 0.0; cw.wj[2] = 0.0;
+                constexpr int WHN = chain_max_dnum(WN);
+                double h[WHN];
+#pragma unroll
+                for (int q = 0; q < WHN; q++) h[q] = q < dnum ? hp[q] : 0.0;
+                double dF[3], dT[3];
+                contact_fp64<WN>(M, M.lw, a.mat, cw, h, a.shearupdate, dF, dT);
+                if (a.shearupdate || WN != NM_HERTZ) {
+#pragma unroll
+                    for (int q = 0; q < WHN; q++)
+                        if (q < dnum) hp[q] = h[q];
+                }
+                F[0] += dF[0]; F[1] += dF[1]; F[2] += dF[2];
+                T[0] += dT[0]; T[1] += dT[1]; T[2] += dT[2];
+            } else {
+                contact_close(M.lw, hp);
+            }
+        }
+        // cleanUpContacts
+        for (int s = 0; s < nn; s++) {
+            const int pv = part[s];
+            if (pv >= 0 && (pv & kKeep)) {
+                part[s] = pv & kIdMask;
+                continue;
+            }
+            if (pv > -1) np--;
+            part[s] = -1;
+            for (int d = 0; d < dnum; d++) hist[(size_t)s * dnum + d] = 0.0;
+        }
+        a.mnp[i] = np;
+        if (!frozen) {
+            double4 f = a.f4[i], tq = a.t4[i];
+            f.x += F[0]; f.y += F[1]; f.z += F[2];
+            tq.x += T[0]; tq.y += T[1]; tq.z += T[2];
+            a.f4[i] = f;
+            a.t4[i] = tq;
+        }
+    }
+    if (frozen) {
+        a.f4[i] = make_double4(0.0, 0.0, 0.0, 0.0);
+        a.t4[i] = make_double4(0.0, 0.0, 0.0, 0.0);
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_mesh_move(TriGeo *tri, int ntri, double dx, double dy, double dz)
+{
+    const int t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= ntri) return;
+    TriGeo &g = tri[t];
+    for (int k = 0; k < 3; k++) {
+        g.node[3 * k] = g.node[3 * k] + dx;
+        g.node[3 * k + 1] = g.node[3 * k + 1] + dy;
+        g.node[3 * k + 2] = g.node[3 * k + 2] + dz;
+    }
+    g.center[0] = g.center[0] + dx;
+    g.center[1] = g.center[1] + dy;
+    g.center[2] = g.center[2] + dz;
+}
+
+static int blocks(long long n) { return (int)((n + kBlock - 1) / kBlock); }
+
+void launch_mesh_count(const MeshNbArgs &a, hipStream_t s)
+{
+    if (a.n) k_mesh_nb<false><<<blocks(a.n), kBlock, 0, s>>>(a);
+}
+
+void launch_mesh_fill(const MeshNbArgs &a, hipStream_t s)
+{
+    if (a.n) k_mesh_nb<true><<<blocks(a.n), kBlock, 0, s>>>(a);
+}
+
+void launch_mesh_remap(const MeshRemapArgs &a, hipStream_t s)
+{
+    if (a.n) k_mesh_remap<<<blocks(a.n), kBlock, 0, s>>>(a);
+}
+
+void launch_mesh_force(const MeshForceArgs &a, hipStream_t s)
+{
+    if (!a.n) return;
+    switch (a.M.lw.normal) {
+    case NM_HYST1: k_mesh_force<NM_HYST1><<<blocks(a.n), kBlock, 0, s>>>(a); break;
+    case NM_HYST2: k_mesh_force<NM_HYST2><<<blocks(a.n), kBlock, 0, s>>>(a); break;
+    default: k_mesh_force<NM_HERTZ><<<blocks(a.n), kBlock, 0, s>>>(a); break;
+    }
+}
+
+void launch_mesh_move(TriGeo *tri, int ntri, double dx, double dy, double dz, hipStream_t s)
+{
+    if (ntri) k_mesh_move<<<blocks(ntri), kBlock, 0, s>>>(tri, ntri, dx, dy, dz);
+}
+
+}  // namespace dem

@@ -391,8 +391,14 @@ void launch_neigh_count(const NeighArgs &a, hipStream_t s)
 {
     if (a.n == 0) return;
     k_neigh<false><<<wave_blocks(a.n), kBlock, 0, s>>>(a);
-    const int nb = (int)std::min<long long>(kMaxBlocks, ((long long)a.n + kBlock - 1) / kBlock);
-    k_row_max<<<nb, kBlock, 0, s>>>(a.count, a.n, a.maxcount);
+    launch_row_max(a.count, a.n, a.maxcount, s);
+}
+
+void launch_row_max(const int *count, int n, int *maxcount, hipStream_t s)
+{
+    if (n == 0) return;
+    const int nb = (int)std::min<long long>(kMaxBlocks, ((long long)n + kBlock - 1) / kBlock);
+    k_row_max<<<nb, kBlock, 0, s>>>(count, n, maxcount);
 }
 
 void launch_neigh_fill(const NeighArgs &a, hipStream_t s)

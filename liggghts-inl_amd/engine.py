@@ -16,7 +16,7 @@ _LIB = None
 
 DEM_FP64 = 0
 DEM_MIXED = 1
-ROLL = {"off": 0, "epsd": 1, "epsd2": 2, "epsd3": 3}
+ROLL = {"off": 0, "epsd": 1, "epsd2": 2, "epsd3": 3, "cdtnonlinear2": 4}
 
 
 class DemError(RuntimeError):
@@ -102,6 +102,11 @@ def library():
         "dem_list_stats": (C.c_int, [vp, dp, dp, i64p]),
         "dem_sync": (C.c_int, [vp]),
         "dem_timestep": (C.c_int64, [vp]),
+        "dem_add_mesh": (C.c_int, [vp, C.c_int, dp, C.c_int, C.c_double, dp]),
+        "dem_mesh_size": (C.c_int, [vp, C.c_int]),
+        "dem_mesh_topology": (C.c_int, [C.c_int, dp, C.c_double, dp, dp, ip, ip, ip, ip, ip]),
+        "dem_get_mesh_contacts": (C.c_int, [vp, C.c_int, i64p, ip, ip, ip, dp]),
+        "dem_set_mesh_contacts": (C.c_int, [vp, C.c_int, C.c_int64, ip, ip, ip, dp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -133,13 +138,35 @@ def material_tables(ntypes, Y, nu, e):
     return out
 
 
+def mesh_topology(nodes, box_lo, box_hi, curvature=None):
+    """Active edges/corners, obtuse index and edge-neighbour counts the
+    engine computes for a triangle mesh (host only, no GPU needed)."""
+    L = library()
+    nd = np.ascontiguousarray(np.asarray(nodes, dtype=np.float64).reshape(-1, 9))
+    nt = len(nd)
+    lo = np.ascontiguousarray(box_lo, dtype=np.float64)
+    hi = np.ascontiguousarray(box_hi, dtype=np.float64)
+    kept = np.zeros(1, np.int32)
+    ea = np.zeros((nt, 3), np.int32)
+    ca = np.zeros((nt, 3), np.int32)
+    ob = np.zeros(nt, np.int32)
+    nn = np.zeros(nt, np.int32)
+    rc = L.dem_mesh_topology(nt, _dp(nd), 0.0 if curvature is None else float(curvature), _dp(lo), _dp(hi),
+                             _ip(kept), _ip(ea), _ip(ca), _ip(ob), _ip(nn))
+    if rc:
+        raise DemError(L.dem_last_error(None).decode())
+    k = int(kept[0])
+    return dict(edge_active=ea[:k], corner_active=ca[:k], obtuse=ob[:k], nneigh=nn[:k])
+
+
 class Simulation:
     """One device simulation.
 
     ``spec``: dict with box_lo, box_hi, periodic, dt, skin, cdf,
     neigh_every, neigh_delay, neigh_check, rolling, tangential_damping,
     limit_force, materials {ntypes, Y, nu, e, mu, mur, etar, krs},
-    planes [(dim, pos, wtype)], gravity (ax, ay, az) or None, freeze (bool).
+    planes [(dim, pos, wtype)], meshes [dict(nodes (ntri,3,3), type,
+    curvature=None, vel=None)], gravity (ax, ay, az) or None, freeze (bool).
     ``atoms``: dict of numpy arrays x, v, omega (n,3), radius, rmass, type,
     tag, optional freeze.
     """
@@ -174,6 +201,9 @@ class Simulation:
         for dim, pos, wt in spec.get("planes", []):
             self._chk(self.L.dem_add_plane(self.h, int(dim), float(pos), int(wt)))
         self.nplanes = len(spec.get("planes", []))
+        self.nmesh = 0
+        for mspec in spec.get("meshes", []):
+            self.add_mesh(**mspec)
         g = spec.get("gravity")
         if g is not None:
             self._chk(self.L.dem_set_gravity(self.h, float(g[0]), float(g[1]), float(g[2])))
@@ -194,6 +224,39 @@ class Simulation:
                                                   for k in HYST_KEYS]))
             self._chk(self.L.dem_set_hysteretic(self.h, _dp(crl), _dp(tabs)))
         self.upload(atoms)
+
+    def add_mesh(self, nodes, type, curvature=None, vel=None):
+        nd = np.ascontiguousarray(np.asarray(nodes, dtype=np.float64).reshape(-1, 9))
+        vv = None if vel is None else np.ascontiguousarray(vel, dtype=np.float64)
+        k = self.L.dem_add_mesh(self.h, len(nd), _dp(nd), int(type),
+                                0.0 if curvature is None else float(curvature), None if vv is None else _dp(vv))
+        if k < 0:
+            raise DemError("dem error %d: %s" % (k, self.L.dem_last_error(self.h).decode()))
+        self.nmesh += 1
+        return k
+
+    def mesh_contacts(self, k=0):
+        """(tag, triangle id, slot, history) of every filled mesh contact slot."""
+        cnt = C.c_int64(0)
+        self._chk(self.L.dem_get_mesh_contacts(self.h, int(k), C.byref(cnt), None, None, None, None))
+        c = cnt.value
+        dw = self.wall_dnum
+        tg = np.zeros(max(c, 1), np.int32)
+        tr = np.zeros(max(c, 1), np.int32)
+        sl = np.zeros(max(c, 1), np.int32)
+        hi = np.zeros((max(c, 1), dw))
+        self._chk(self.L.dem_get_mesh_contacts(self.h, int(k), C.byref(cnt), _ip(tg), _ip(tr), _ip(sl), _dp(hi)))
+        return tg[:c], tr[:c], sl[:c], hi[:c]
+
+    def set_mesh_contacts(self, k, tag, tri, slot, hist):
+        tg = np.ascontiguousarray(tag, dtype=np.int32)
+        tr = np.ascontiguousarray(tri, dtype=np.int32)
+        sl = np.ascontiguousarray(slot, dtype=np.int32)
+        h = np.ascontiguousarray(hist, dtype=np.float64).reshape(len(tg), self.wall_dnum)
+        self._chk(self.L.dem_set_mesh_contacts(self.h, int(k), len(tg), _ip(tg), _ip(tr), _ip(sl), _dp(h)))
+
+    def set_gravity(self, ax, ay, az):
+        self._chk(self.L.dem_set_gravity(self.h, float(ax), float(ay), float(az)))
 
     def _chk(self, rc):
         if rc != 0:

@@ -15,6 +15,9 @@ play for the reference).  Units are SI.
   relaxed by overlap removal at 0.999 of the radii and stacked in z, so at
   full radii neighbouring spheres overlap by ~1e-3 r: a dense, contact-rich
   state, not a gravity-equilibrated one.
+* ``hopper`` -- N spheres r ~ U[0.8, 1.2] mm in a 30-degree conical hopper
+  (100 mm top, 12 mm outlet) with a cylinder on top, ~20,000-30,000
+  triangles, mu = 0.4, e = 0.5, dt = 5e-7, floor plane below the outlet.
 """
 import numpy as np
 
@@ -108,3 +111,74 @@ def packed_column(n=1_000_000, seed=2024, slab_n=12500, phi=0.60, rmin=0.5e-3, r
         gravity=(0.0, 0.0, -9.81),
     )
     return spec, _atoms(xs, rs, rng, 0.0)
+
+
+def hopper_mesh(r_out=6e-3, r_top=50e-3, half_angle_deg=30.0, cyl_height=0.1, ntheta=160,
+                ring_len=2.2e-3, cyl_ring_len=12e-3):
+    """Conical hopper (apex down, outlet open) with a cylinder on top, as a
+    triangle list (ntri, 3, 3).  Rings of ntheta quads, two triangles each;
+    node order gives normals pointing into the hopper."""
+    t = np.tan(np.radians(half_angle_deg))
+    hc = (r_top - r_out) / t
+    ncone = max(1, int(round(hc / np.cos(np.radians(half_angle_deg)) / ring_len)))
+    ncyl = max(1, int(round(cyl_height / cyl_ring_len))) if cyl_height > 0 else 0
+    zs = list(np.linspace(0.0, hc, ncone + 1))
+    rs = [r_out + z * t for z in zs]
+    for k in range(1, ncyl + 1):
+        zs.append(hc + cyl_height * k / ncyl)
+        rs.append(r_top)
+    th = np.linspace(0.0, 2 * np.pi, ntheta + 1)[:-1]
+    ring = [np.stack([r * np.cos(th), r * np.sin(th), np.full(ntheta, z)], 1) for z, r in zip(zs, rs)]
+    tris = []
+    for k in range(len(zs) - 1):
+        a, b = ring[k], ring[k + 1]
+        for j in range(ntheta):
+            j1 = (j + 1) % ntheta
+            tris.append([a[j], b[j], a[j1]])
+            tris.append([a[j1], b[j], b[j1]])
+    return np.asarray(tris), hc
+
+
+def hopper(n=200_000, seed=7, rmin=0.8e-3, rmax=1.2e-3, spacing=2.5e-3, jitter=5e-5, ntheta=160,
+           mu=0.4, e=0.5, Y=5e6, dt=5e-7, skin=2.5e-4, wall_overlap=0.0):
+    """Hopper discharge: n spheres r ~ U[rmin, rmax] on a jittered lattice
+    inside a 30-degree cone (100 mm top, 12 mm outlet) topped by a cylinder
+    tall enough to hold them, ~20,000 triangles, a floor plane 40 mm below
+    the outlet, gravity.  wall_overlap > 0 moves the outermost lattice
+    sites that far into the wall (contacts from the first step)."""
+    rng = np.random.default_rng(seed)
+    r_out, r_top, ang = 6e-3, 50e-3, 30.0
+    t = np.tan(np.radians(ang))
+    hc = (r_top - r_out) / t
+    # lattice sites in the interior, bottom up, until n fit
+    lim = lambda z: np.minimum(r_out + np.maximum(z, 0.0) * t, r_top)  # noqa: E731
+    c = np.cos(np.radians(ang))
+    nxy = int(2 * r_top / spacing) + 2
+    g = (np.arange(nxy) - nxy / 2 + 0.5) * spacing
+    gx, gy = np.meshgrid(g, g, indexing="ij")
+    gx, gy = gx.ravel(), gy.ravel()
+    rho_xy = np.hypot(gx, gy)
+    sites = []
+    z = rmax + 2e-3
+    count = 0
+    while count < n:
+        room = (lim(z) - rho_xy) * c if z < hc else (r_top - rho_xy)
+        ok = room >= rmax + jitter - wall_overlap
+        layer = np.stack([gx[ok], gy[ok], np.full(ok.sum(), z)], 1)
+        sites.append(layer)
+        count += len(layer)
+        z += spacing
+    x = np.concatenate(sites)[:n]
+    x = x + rng.uniform(-jitter, jitter, size=x.shape)
+    r = rng.uniform(rmin, rmax, size=n)
+    top = x[:, 2].max() + rmax
+    cyl = max(top - hc + 5e-3, 5e-3)
+    tris, _ = hopper_mesh(r_out, r_top, ang, cyl, ntheta)
+    mats = _materials(Y=Y, e=e, mu=mu, ntypes=2)
+    ztop = hc + cyl
+    spec = dict(
+        box_lo=[-0.06, -0.06, -0.045], box_hi=[0.06, 0.06, ztop + 0.01], periodic=[0, 0, 0], dt=dt, skin=skin,
+        cdf=1.0, neigh_every=10, neigh_delay=0, neigh_check=False, rolling="off", materials=mats,
+        meshes=[dict(nodes=tris, type=2)], planes=[(2, -0.04, 2)], gravity=(0.0, 0.0, -9.81),
+    )
+    return spec, _atoms(x, r, rng, 0.0)

@@ -1509,9 +1509,21 @@ int or_add_mesh(or_sim *s, int ntri, const double *nodes, int wtype, double curv
         for (int q = 0; q < 3; q++) m->vel[q] = vel[q];
     }
     m->node = malloc(sizeof(double) * 9 * ntri);
-    memcpy(m->node, nodes, sizeof(double) * 9 * ntri);
+    /* MultiNodeMesh::addElement (multi_node_mesh_I.h:138-146) refuses an
+     * element with two equal nodes; ids count the accepted ones */
+    int kept = 0;
+    for (int t = 0; t < ntri; t++) {
+        const double *a = nodes + 9 * t;
+        if (nodes_equal(a, a + 3, 1e-8) || nodes_equal(a + 3, a + 6, 1e-8) || nodes_equal(a, a + 6, 1e-8))
+            continue;
+        memcpy(m->node + 9 * kept, a, sizeof(double) * 9);
+        kept++;
+    }
+    if (kept == 0) { free(m->node); return -1; }
+    ntri = kept;
+    m->ntri = ntri;
     m->node_re = malloc(sizeof(double) * 9 * ntri);
-    memcpy(m->node_re, nodes, sizeof(double) * 9 * ntri);
+    memcpy(m->node_re, m->node, sizeof(double) * 9 * ntri);
     m->center = malloc(sizeof(double) * 3 * ntri);
     m->rbound = malloc(sizeof(double) * ntri);
     m->evec = malloc(sizeof(double) * 9 * ntri);
@@ -1563,6 +1575,11 @@ void or_mesh_topology(const or_sim *s, int im, int *eact, int *cact, int *obtuse
         obtuse[t] = m->obtuse[t];
         nneigh[t] = m->nneigh[t];
     }
+}
+
+int or_mesh_size(const or_sim *s, int im)
+{
+    return (im >= 0 && im < s->nmesh) ? s->mesh[im].ntri : 0;
 }
 
 void or_mesh_force(const or_sim *s, int im, double f[3])
@@ -1966,7 +1983,7 @@ double or_mesh_distance(const or_sim *s, int im, int tri, double r, const double
     return tri_contact(&s->mesh[im], tri, r, x, delta, bary, region);
 }
 
-int or_mesh_contacts(const or_sim *s, int im, int *tag, int *tri, double *hist)
+int or_mesh_contacts(const or_sim *s, int im, int *tag, int *tri, int *slot, double *hist)
 {
     const mesh_t *m = &s->mesh[im];
     int c = 0;
@@ -1978,6 +1995,7 @@ int or_mesh_contacts(const or_sim *s, int im, int *tag, int *tri, double *hist)
             if (tag) {
                 tag[c] = s->tag[i];
                 tri[c] = m->partner[q];
+                if (slot) slot[c] = k;
                 for (int d = 0; d < s->dnum_w; d++) hist[(size_t)c * s->dnum_w + d] = m->hist[(size_t)q * s->dnum_w + d];
             }
             c++;

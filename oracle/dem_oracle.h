@@ -87,6 +87,8 @@ void or_add_freeze(or_sim *s);
  * (element ids 0..ntri-1), curvature = cos(curvature angle), <= 0 for the
  * default 1-1e-5.  Must be added after or_set_atoms.  Returns the fix id. */
 int or_add_mesh(or_sim *s, int ntri, const double *nodes, int wtype, double curvature, const double *vel);
+/* number of triangles kept (elements with two equal nodes are refused) */
+int or_mesh_size(const or_sim *s, int mesh);
 /* topology as SurfaceMesh computes it: edge/corner active flags (ntri*3),
  * obtuse angle index (-1 none) and number of edge neighbours per element */
 void or_mesh_topology(const or_sim *s, int mesh, int *edge_active, int *corner_active, int *obtuse,
@@ -99,8 +101,9 @@ double or_mesh_distance(const or_sim *s, int mesh, int tri, double r, const doub
                         double bary[3], int *region);
 /* sum of the mesh's contact forces on the particles in the last pass */
 void or_mesh_force(const or_sim *s, int mesh, double f[3]);
-/* current mesh contacts: (atom tag, triangle id, history); NULL to count */
-int or_mesh_contacts(const or_sim *s, int mesh, int *tag, int *tri, double *hist);
+/* current mesh contacts: (atom tag, triangle id, slot index, history);
+ * NULL to count */
+int or_mesh_contacts(const or_sim *s, int mesh, int *tag, int *tri, int *slot, double *hist);
 /* hysteretic/nonlinear* parameter tables, see OR_NHYST */
 int or_set_hysteretic(or_sim *s, const double *tables);
 /* group membership: bit 'bit' of atom i's mask (group command) */

@@ -94,7 +94,8 @@ def lib():
         L.or_add_mesh.argtypes = [C.c_void_p, C.c_int, dp, C.c_int, C.c_double, dp]
         L.or_mesh_topology.argtypes = [C.c_void_p, C.c_int, ip, ip, ip, ip]
         L.or_mesh_force.argtypes = [C.c_void_p, C.c_int, dp]
-        L.or_mesh_contacts.argtypes = [C.c_void_p, C.c_int, ip, ip, dp]
+        L.or_mesh_size.argtypes = [C.c_void_p, C.c_int]
+        L.or_mesh_contacts.argtypes = [C.c_void_p, C.c_int, ip, ip, ip, dp]
         L.or_mesh_distance.restype = C.c_double
         L.or_mesh_distance.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, dp, dp, dp, ip]
         _LIB = L
@@ -305,7 +306,7 @@ class Oracle:
                                  None if vv is None else _dp(vv))
         if fid < 0:
             raise ValueError("oracle: bad mesh")
-        self.mesh_ntri.append(len(nd))
+        self.mesh_ntri.append(self.L.or_mesh_size(self.h, self.nmesh))
         self.nmesh += 1
         return fid
 
@@ -323,16 +324,18 @@ class Oracle:
         self.L.or_mesh_force(self.h, int(k), _dp(f))
         return f
 
-    def mesh_contacts(self, k=0):
-        """(tag, triangle id, history) of the mesh's current contacts."""
-        c = self.L.or_mesh_contacts(self.h, int(k), None, None, None)
+    def mesh_contacts(self, k=0, slots=False):
+        """(tag, triangle id, [slot,] history) of the mesh's filled contact
+        slots."""
+        c = self.L.or_mesh_contacts(self.h, int(k), None, None, None, None)
         dw = self.L.or_dnum_wall(self.h)
         tg = np.zeros(c, np.int32)
         tr = np.zeros(c, np.int32)
+        sl = np.zeros(c, np.int32)
         hi = np.zeros((c, dw))
         if c:
-            self.L.or_mesh_contacts(self.h, int(k), _ip(tg), _ip(tr), _dp(hi))
-        return tg, tr, hi
+            self.L.or_mesh_contacts(self.h, int(k), _ip(tg), _ip(tr), _ip(sl), _dp(hi))
+        return (tg, tr, sl, hi) if slots else (tg, tr, hi)
 
     def mesh_distance(self, k, tri, r, x):
         """(gap, delta, bary, region) of resolveTriSphereContactBary."""
