@@ -25,7 +25,7 @@
 #define MAXDNUM 8
 #define T1 (OR_MAXTYPES + 1)
 
-enum { OP_WALL = 1, OP_GRAVITY = 2, OP_FREEZE = 3, OP_SETFORCE = 4, OP_AVEFORCE = 5, OP_MESH = 6 };
+enum { OP_WALL = 1, OP_GRAVITY = 2, OP_FREEZE = 3, OP_SETFORCE = 4, OP_AVEFORCE = 5, OP_MESH = 6, OP_MOVE = 7 };
 
 /* contact_model_constants.h:64-67 */
 enum { CONTACT_NORMAL_MODEL = 1, CONTACT_COHESION_MODEL = 2, CONTACT_TANGENTIAL_MODEL = 4,
@@ -43,6 +43,8 @@ typedef struct {
 typedef struct {
     int normal;      /* OR_NORMAL_* */
     int limitf;
+    int cohesion;    /* OR_COH_* */
+    int off_coh;     /* bond: 14 slots, bond/nonlinear: 28, else -1 */
     int off_norm;    /* hysteretic: 12 slots */
     int off_shear;   /* 3 slots */
     int off_roll;    /* 3 slots or -1 */
@@ -56,6 +58,8 @@ typedef struct {
     double val[3];
     double c0, t0, c1;
     double foriginal[3];
+    double *xorig;          /* fix move: x at creation, n*3 */
+    long long time_origin;
 } op_t;
 
 #define MESH_NEIGH_MAX 5   /* SurfaceMesh<3,5> (tri_mesh.h:67) */
@@ -89,6 +93,9 @@ struct or_sim {
     double mu[T1 * T1], mur[T1 * T1], etar[T1 * T1];
     double restlog[T1 * T1];
     double hyst[OR_NHYST][T1 * T1];
+    double bond[OR_NBOND][T1 * T1];
+    or_bond_settings bs;
+    int integ_bit;    /* nve/sphere group bit, -1 = all */
     double krs;
     model_t mp, mw;   /* pair and wall model chains */
     int *gmask;       /* group bits per atom */
@@ -148,18 +155,33 @@ static const double SQRT_FIVE_OVER_SIX = 0.9128709291752768557616163046680035565
  * normal_model_hysteretic_nonlinear1.h:84-97), shear x/y/z
  * (tangential_model_history.h:79-81), rolling torque x/y/z
  * (rolling_model_epsd.h:71-73), all with newtonflag 1. */
-static void model_layout(model_t *m, int normal, int limitf, int rolling)
+static void model_layout(model_t *m, int normal, int limitf, int rolling, int cohesion)
 {
     int k = 0;
     memset(m, 0, sizeof(*m));
     m->normal = normal;
     m->limitf = limitf;
+    m->cohesion = cohesion;
     m->off_norm = -1;
     if (normal != OR_NORMAL_HERTZ) {
         m->off_norm = k;
         for (int q = 0; q < 10; q++) m->newton[k++] = 0;
         m->newton[k++] = 1;
         m->newton[k++] = 1;
+    }
+    /* cohesion bond (cohesion_model_bond.h:237-250): contflag, initial_dist,
+     * contactPos x3 with newtonflag 0, ft x3, torquen x3, torquet x3 with 1;
+     * bond/nonlinear (cohesion_model_bond_nonlinear.h:204-230): the same 14
+     * with theta_n/theta_t in place of the torques, then deltan_max,
+     * thetan_max x3, thetat_max x3, thetan_min x3, thetat_min x3,
+     * deltan_min, all newtonflag 0 */
+    m->off_coh = -1;
+    if (cohesion != OR_COH_OFF) {
+        m->off_coh = k;
+        for (int q = 0; q < 5; q++) m->newton[k++] = 0;
+        for (int q = 0; q < 9; q++) m->newton[k++] = 1;
+        if (cohesion == OR_COH_BOND_NONLINEAR)
+            for (int q = 0; q < 14; q++) m->newton[k++] = 0;
     }
     m->off_shear = k;
     for (int q = 0; q < 3; q++) m->newton[k++] = 1;
@@ -175,8 +197,10 @@ or_sim *or_create(const or_params *p)
 {
     or_sim *s = (or_sim *)calloc(1, sizeof(or_sim));
     s->p = *p;
-    model_layout(&s->mp, p->normal_model, p->limit_force, p->rolling_model);
-    model_layout(&s->mw, p->wall_normal_model, p->wall_limit_force, p->rolling_model);
+    model_layout(&s->mp, p->normal_model, p->limit_force, p->rolling_model, p->cohesion_model);
+    model_layout(&s->mw, p->wall_normal_model, p->wall_limit_force, p->rolling_model, OR_COH_OFF);
+    s->integ_bit = -1;
+    s->bs.damping = 1;
     s->dnum = s->mp.dnum;
     s->dnum_w = s->mw.dnum;
     s->next_reneighbor = -1;
@@ -205,6 +229,7 @@ void or_destroy(or_sim *s)
         free(m->npartner); free(m->sstart); free(m->partner); free(m->keep); free(m->inter);
         free(m->hist);
     }
+    for (int q = 0; q < s->nops; q++) free(s->ops[q].xorig);
     free(s);
 }
 
@@ -345,8 +370,10 @@ int or_add_aveforce(or_sim *s, int bit, const int style[3], double xv, double yv
 void or_unfix(or_sim *s, int id)
 {
     int k = 0;
-    for (int q = 0; q < s->nops; q++)
+    for (int q = 0; q < s->nops; q++) {
         if (s->ops[q].id != id) s->ops[k++] = s->ops[q];
+        else free(s->ops[q].xorig);
+    }
     s->nops = k;
 }
 
@@ -664,6 +691,7 @@ typedef struct {
     double *hist;       /* dnum doubles */
     const model_t *model;
     int *flags;         /* contact flags, NULL for walls */
+    int i, j;           /* atom indices (cohesion models read x, omega) */
 } sidata_t;
 
 /* surface_model_default.h:146-208 */
@@ -914,6 +942,354 @@ static void rolling_cdt2(const or_sim *s, sidata_t *sd, double Ti[3], double Tj[
     Tj[0] += rt[0]; Tj[1] += rt[1]; Tj[2] += rt[2];
 }
 
+/* ------------------------------------------------------------------------ */
+/* Cohesion models of the pair chain                                         */
+
+enum { BT_LAMBDA = 0, BT_KN, BT_KT, BT_DFN, BT_DFT, BT_DTN, BT_DTT, BT_MAXDIST, BT_MAXSIGMA, BT_MAXTAU,
+       BT_RATIOTC, BT_CREATEDIST, BT_KFN1, BT_KUFN1, BT_KCFN1, BT_KFN2, BT_KUFN2, BT_KCFN2, BT_KFT,
+       BT_KTN, BT_KUTN, BT_KCTN, BT_KTT, BT_KUTT, BT_KCTT };
+
+static inline int isgn(double v) { return (0.0 < v) - (v < 0.0); }
+
+/* vectorProject3D (vector_liggghts.h:437-442) */
+static void project3(const double *v, const double *on, double *out)
+{
+    double nrm[3] = {on[0], on[1], on[2]};
+    const double norm = sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2]);
+    const double invnorm = (norm == 0.) ? 0. : 1. / norm;
+    nrm[0] *= invnorm; nrm[1] *= invnorm; nrm[2] *= invnorm;
+    const double d = v[0] * nrm[0] + v[1] * nrm[1] + v[2] * nrm[2];
+    out[0] = d * nrm[0]; out[1] = d * nrm[1]; out[2] = d * nrm[2];
+}
+
+/* damped component: x - c |x| m with m = sgn(rate) or, smoothed,
+ * clamp(rate / max(minvel, 0.01 x dt), -1, 1) */
+static double bond_damped(double x, double c, double rate, int smooth, double minvel, double dt)
+{
+    double m;
+    if (smooth) {
+        const double dv = 0.01 * x * dt;
+        m = fmin(1.0, fmax(-1.0, rate / fmax(minvel, dv)));
+    } else {
+        m = isgn(rate);
+    }
+    return x - c * fabs(x) * m;
+}
+
+/* one component of the bond/nonlinear hysteretic torque
+ * (cohesion_model_bond_nonlinear.h:660-760): returns the torque, records
+ * the extreme angles */
+static double bond_nl_torque(double th, double *tmax, double *tmin, double k, double ku, double kc, double Jl)
+{
+    const double c1 = (ku - k) / (ku + kc) * (*tmax);
+    const double c2 = (ku - k) / (ku + kc) * (*tmin);
+    double tq;
+    if ((th >= *tmax) || (th <= *tmin)) tq = -k * Jl * th;
+    else if (th > c1) tq = -ku * Jl * th + (ku - k) * Jl * (*tmax);
+    else if (th >= c2) tq = kc * Jl * th;
+    else tq = -ku * Jl * th + (ku - k) * Jl * (*tmin);
+    if (th > *tmax) *tmax = th;
+    if (th < *tmin) *tmin = th;
+    return tq;
+}
+
+/* CohesionModel<COHESION_BOND>::surfacesClose (cohesion_model_bond.h:490-
+ * 935) and CohesionModel<COHESION_BOND_NONLINEAR>::surfacesClose
+ * (cohesion_model_bond_nonlinear.h:396-960) for a particle pair, called
+ * from surfacesIntersect as well.  Dissipation, elastic potential,
+ * strengthening factor, random-id creation and walls are not restated.
+ * bond adds its force to the chain; bond/nonlinear replaces it.  Returns
+ * has_force_update. */
+static int cohesion_bond(const or_sim *s, sidata_t *sd, double Fi[3], double Ti[3], double Fj[3], double Tj[3])
+{
+    const model_t *m = sd->model;
+    const int nl = m->cohesion == OR_COH_BOND_NONLINEAR;
+    const int upd = sd->update_history;
+    const int t = sd->itype * T1 + sd->jtype;
+    const double lambda = s->bond[BT_LAMBDA][t];
+    const double small = nl ? 1.e-15 : 1.e-6;   /* SMALL_COHESION_MODEL_BOND */
+    if (lambda < small) return 0;
+    double *hb = sd->hist + m->off_coh;
+    const double r_create = sqrt(sd->rsq);
+    const double r = r_create;
+    const int i = sd->i;
+    if (upd) {
+        if (hb[0] < small) {
+            const int create = (s->bs.create_always || (s->ntimestep == (long long)(int)s->bs.ts_create))
+                               && (r_create < s->bond[BT_CREATEDIST][t]);
+            if (!create) return 0;
+            /* createBond */
+            if (sd->flags) *sd->flags |= CONTACT_COHESION_MODEL;
+            hb[0] = 1.0;
+            hb[1] = r;
+            for (int d = 0; d < 3; d++) hb[2 + d] = s->x[3 * i + d] - sd->delta[d];
+            for (int d = 5; d < 14; d++) hb[d] = 0.0;
+        }
+    } else if (hb[0] < small) {
+        return 0;
+    }
+    double ft[3], tn[3], tt[3];
+    for (int d = 0; d < 3; d++) {
+        ft[d] = hb[5 + d];
+        tn[d] = hb[8 + d];
+        tt[d] = hb[11 + d];
+    }
+    const double *delta = sd->delta;
+    if (!s->bs.stress_break && r > s->bond[BT_MAXDIST][t] && upd) {
+        /* breakBond */
+        if (sd->flags) *sd->flags &= ~CONTACT_COHESION_MODEL;
+        hb[0] = 0.;
+        hb[1] = 0.;
+        return 0;
+    }
+    if (sd->flags) *sd->flags |= CONTACT_COHESION_MODEL;
+    const double rinv = 1. / r;
+    double en[3] = {rinv * delta[0], rinv * delta[1], rinv * delta[2]};
+    const double *vi = sd->v_i, *vj = sd->v_j;
+    const double *omegai = sd->omega_i, *omegaj = sd->omega_j;
+    const double radi = sd->radi, radj = sd->radj;
+    const double rb = lambda * (radi < radj ? radi : radj);
+    const double A = M_PI * rb * rb;
+    const double J = 0.5 * A * rb * rb;
+    const double I = 0.5 * J;
+    const double dt = s->p.dt;
+    const double radsuminv = 1. / (radi + radj);
+    const double cri = r * radi * radsuminv;
+    const double crj = r * radj * radsuminv;
+    double vr[3], vn[3], vt[3], wr[3], vtr[3], wn[3], wt[3], tmp1[3], tmp2[3];
+    for (int d = 0; d < 3; d++) vr[d] = vi[d] - vj[d];
+    project3(vr, en, vn);
+    for (int d = 0; d < 3; d++) vt[d] = vr[d] - vn[d];
+    const double fi = radi * radsuminv, fj = radj * radsuminv;
+    for (int d = 0; d < 3; d++) {
+        tmp1[d] = fi * omegai[d];
+        tmp2[d] = fj * omegaj[d];
+    }
+    for (int d = 0; d < 3; d++) wr[d] = tmp1[d] + tmp2[d];
+    tmp1[0] = delta[1] * wr[2] - delta[2] * wr[1];
+    tmp1[1] = delta[2] * wr[0] - delta[0] * wr[2];
+    tmp1[2] = delta[0] * wr[1] - delta[1] * wr[0];
+    for (int d = 0; d < 3; d++) vtr[d] = vt[d] + tmp1[d];
+    for (int d = 0; d < 3; d++) wr[d] = omegai[d] - omegaj[d];
+    project3(wr, en, wn);
+    for (int d = 0; d < 3; d++) wt[d] = wr[d] - wn[d];
+
+    double nforce[3] = {0., 0., 0.}, nfd[3] = {0., 0., 0.}, tfd[3] = {0., 0., 0.};
+    double ntd[3] = {0., 0., 0.}, ttd[3] = {0., 0., 0.};
+    const double displacement = hb[1] - r;
+    const double minvel = 1e-5 * fmin(radi, radj) / dt;
+    const int damp = s->bs.damping, smooth = s->bs.damping_smooth;
+    double dmax = 0.0, dmin = 0.0;
+    if (nl) {
+        if (displacement > hb[14]) { hb[14] = displacement; dmax = displacement; }
+        else dmax = hb[14];
+        if (displacement < hb[27]) { hb[27] = displacement; dmin = displacement; }
+        else dmin = hb[27];
+        if (displacement < 0.0) hb[14] = 0.0;
+        if (displacement > 0.0) hb[27] = 0.0;
+    }
+    if (displacement < -1.e-15 || displacement > 1.e-15) {
+        double frcmag;
+        if (!nl) {
+            frcmag = s->bond[BT_KN][t] * A * displacement;
+        } else {
+            const double k1 = s->bond[BT_KFN1][t], ku1 = s->bond[BT_KUFN1][t], kc1 = s->bond[BT_KCFN1][t];
+            const double k2 = s->bond[BT_KFN2][t], ku2 = s->bond[BT_KUFN2][t], kc2 = s->bond[BT_KCFN2][t];
+            const double d = displacement;
+            if (!s->bs.tension_variant) {
+                const double c1 = pow((ku1 - k1) / (ku1 + kc1), 2.0) * dmax;
+                const double c2 = pow((ku2 - k2) / (ku2 + kc2), 1.0) * dmin;
+                if (d < dmin) frcmag = k2 * A * d;
+                else if (d < c2) frcmag = ku2 * A * d + (k2 - ku2) * A * dmin;
+                else if (d < 0.0) frcmag = -kc2 * A * d;
+                else if (d < c1) frcmag = -kc1 * pow(fabs(d), 0.5);
+                else if (d < dmax) frcmag = ku1 * pow(fabs(d), 0.5) + (k1 - ku1) * pow(fabs(dmax), 0.5);
+                else frcmag = k1 * pow(fabs(d), 0.5);
+            } else {
+                /* examples/LIGGGHTS/INL/cohesive_bond_nonlinear_tension/src/
+                 * cohesion_model_bond_nonlinear_tension.h:590-613 */
+                const double c1 = pow((ku1 - k1) / (ku1 + kc1), 1.0) * dmax;
+                const double c2 = pow((ku2 - k2) / (ku2 + kc2), 2.0) * dmin;
+                if (d <= dmin) frcmag = -k2 * pow(fabs(d), 0.5);
+                else if (d < c2) frcmag = -(ku2 * pow(fabs(d), 0.5) + (k2 - ku2) * pow(fabs(dmin), 0.5));
+                else if (d < 0.0) frcmag = kc2 * pow(fabs(d), 0.5);
+                else if (d < c1) frcmag = -kc1 * A * d;
+                else if (d < dmax) frcmag = ku1 * A * d + (k1 - ku1) * A * dmax;
+                else frcmag = k1 * A * d;
+            }
+        }
+        for (int d = 0; d < 3; d++) nforce[d] = frcmag * en[d];
+        for (int d = 0; d < 3; d++)
+            nfd[d] = damp ? bond_damped(nforce[d], s->bond[BT_DFN][t], vn[d], smooth, minvel, dt) : nforce[d];
+    }
+    /* shear: rotate the old force into the tangent plane, add the increment */
+    {
+        const double kft = nl ? s->bond[BT_KFT][t] : s->bond[BT_KT][t];
+        const double c = -kft * A * dt;
+        double dtforce[3] = {c * vtr[0], c * vtr[1], c * vtr[2]};
+        project3(ft, en, tmp1);
+        for (int d = 0; d < 3; d++) ft[d] = ft[d] - tmp1[d];
+        for (int d = 0; d < 3; d++) ft[d] = ft[d] + dtforce[d];
+        for (int d = 0; d < 3; d++)
+            tfd[d] = damp ? bond_damped(ft[d], s->bond[BT_DFT][t], vtr[d], smooth, minvel, dt) : ft[d];
+    }
+    double torque_n[3], torque_t[3];
+    if (!nl) {
+        const double ktp = s->bond[BT_KT][t], knp = s->bond[BT_KN][t];
+        const double cn = -ktp * J * dt;
+        const double dnt[3] = {cn * wn[0], cn * wn[1], cn * wn[2]};
+        project3(tn, en, tn);
+        for (int d = 0; d < 3; d++) tn[d] = tn[d] + dnt[d];
+        for (int d = 0; d < 3; d++)
+            ntd[d] = damp ? tn[d] - s->bond[BT_DTN][t] * fabs(tn[d]) * isgn(wn[d]) : tn[d];
+        const double wtsq = wt[0] * wt[0] + wt[1] * wt[1] + wt[2] * wt[2];
+        if (wtsq > 0) {
+            const double ct = -knp * I * dt;
+            const double dtt[3] = {ct * wt[0], ct * wt[1], ct * wt[2]};
+            project3(tt, wt, tt);
+            for (int d = 0; d < 3; d++) tt[d] = tt[d] + dtt[d];
+            for (int d = 0; d < 3; d++)
+                ttd[d] = damp ? tt[d] - s->bond[BT_DTT][t] * fabs(tt[d]) * isgn(wt[d]) : tt[d];
+        }
+        for (int d = 0; d < 3; d++) { torque_n[d] = tn[d]; torque_t[d] = tt[d]; }
+    } else {
+        /* twisting: tn holds the twist angle theta_n */
+        const double ktn = s->bond[BT_KTN][t], kutn = s->bond[BT_KUTN][t], kctn = s->bond[BT_KCTN][t];
+        const double ktt = s->bond[BT_KTT][t], kutt = s->bond[BT_KUTT][t], kctt = s->bond[BT_KCTT][t];
+        double dth[3] = {dt * wn[0], dt * wn[1], dt * wn[2]};
+        for (int d = 0; d < 3; d++) tn[d] = tn[d] + dth[d];
+        const double cn = -ktn * J;
+        for (int d = 0; d < 3; d++) torque_n[d] = cn * tn[d];
+        project3(torque_n, wn, torque_n);
+        for (int d = 0; d < 3; d++) {
+            if (tn[d] > 0.0) hb[21 + d] = 0.0;
+            else if (tn[d] < 0.0) hb[15 + d] = 0.0;
+        }
+        for (int d = 0; d < 3; d++) torque_n[d] = bond_nl_torque(tn[d], &hb[15 + d], &hb[21 + d], ktn, kutn, kctn, J);
+        for (int d = 0; d < 3; d++)
+            ntd[d] = damp ? torque_n[d] - s->bond[BT_DTN][t] * fabs(torque_n[d]) * isgn(wn[d]) : torque_n[d];
+        /* bending: tt holds the bending angle theta_t */
+        double dtt[3] = {dt * wt[0], dt * wt[1], dt * wt[2]};
+        for (int d = 0; d < 3; d++) tt[d] = tt[d] + dtt[d];
+        const double ct = -ktt * I;
+        for (int d = 0; d < 3; d++) torque_t[d] = ct * tt[d];
+        project3(torque_t, wt, torque_t);
+        for (int d = 0; d < 3; d++) {
+            if (tt[d] > 0.0) hb[24 + d] = 0.0;
+            else if (tt[d] < 0.0) hb[18 + d] = 0.0;
+        }
+        for (int d = 0; d < 3; d++) torque_t[d] = bond_nl_torque(tt[d], &hb[18 + d], &hb[24 + d], ktt, kutt, kctt, I);
+        for (int d = 0; d < 3; d++)
+            ttd[d] = damp ? torque_t[d] - s->bond[BT_DTT][t] * fabs(torque_t[d]) * isgn(wt[d]) : torque_t[d];
+    }
+    if (s->bs.stress_break) {
+        const double nforce_mag = sqrt(nforce[0] * nforce[0] + nforce[1] * nforce[1] + nforce[2] * nforce[2]);
+        const double tforce_mag = sqrt(ft[0] * ft[0] + ft[1] * ft[1] + ft[2] * ft[2]);
+        const double ntorque_mag = sqrt(torque_n[0] * torque_n[0] + torque_n[1] * torque_n[1] + torque_n[2] * torque_n[2]);
+        const double ttorque_mag = sqrt(torque_t[0] * torque_t[0] + torque_t[1] * torque_t[1] + torque_t[2] * torque_t[2]);
+        double maxSigma;
+        int nstress, tstress;
+        if (!nl) {
+            maxSigma = s->bond[BT_MAXSIGMA][t] * ((s->bs.ratio_tc && displacement < 1e-16) ? s->bond[BT_RATIOTC][t] : 1.0);
+            nstress = (1. * maxSigma) < (nforce_mag / A + ttorque_mag * rb / I);
+            tstress = (1. * s->bond[BT_MAXTAU][t]) < (tforce_mag / A + ntorque_mag * rb / J);
+        } else {
+            maxSigma = s->bond[BT_MAXSIGMA][t] * ((s->bs.ratio_tc && displacement < -1.e-15) ? s->bond[BT_RATIOTC][t] : 1.0);
+            nstress = maxSigma < (nforce_mag / A + ttorque_mag * rb / I);
+            tstress = s->bond[BT_MAXTAU][t] < (tforce_mag / A + ntorque_mag * rb / J);
+        }
+        if ((nstress || tstress) && upd) {
+            if (sd->flags) *sd->flags &= ~CONTACT_COHESION_MODEL;
+            hb[0] = 0.;
+            hb[1] = 0.;
+            return 0;
+        }
+    }
+    double tor[3];
+    tor[0] = tfd[1] * en[2] - tfd[2] * en[1];
+    tor[1] = tfd[2] * en[0] - tfd[0] * en[2];
+    tor[2] = tfd[0] * en[1] - tfd[1] * en[0];
+    double F[3], dTi[3], dTj[3];
+    for (int d = 0; d < 3; d++) {
+        F[d] = nfd[d] + tfd[d];
+        dTi[d] = cri * tor[d] + ntd[d] + ttd[d];
+        dTj[d] = crj * tor[d] - ntd[d] - ttd[d];
+    }
+    if (upd)
+        for (int d = 0; d < 3; d++) {
+            hb[5 + d] = ft[d];
+            hb[8 + d] = tn[d];
+            hb[11 + d] = tt[d];
+        }
+    if (nl) {
+        /* bond/nonlinear replaces the chain's force so far */
+        for (int d = 0; d < 3; d++) {
+            Fi[d] = F[d]; Ti[d] = dTi[d];
+            Fj[d] = -F[d]; Tj[d] = dTj[d];
+        }
+    } else {
+        for (int d = 0; d < 3; d++) {
+            Fi[d] += F[d]; Ti[d] += dTi[d];
+            Fj[d] -= F[d]; Tj[d] += dTj[d];
+        }
+    }
+    return 1;
+}
+
+int or_set_bond(or_sim *s, const double *tables, const or_bond_settings *bs)
+{
+    const int w = s->ntypes + 1;
+    for (int q = 0; q < OR_NBOND; q++)
+        for (int i = 0; i < w; i++)
+            for (int j = 0; j < w; j++) s->bond[q][i * T1 + j] = tables[(size_t)q * w * w + i * w + j];
+    s->bs = *bs;
+    /* the model's contact distance factor (cohesion_model_bond.h:415-470,
+     * cohesion_model_bond_nonlinear.h:330-380), registered as a maximum */
+    double minrad = 1e300;
+    for (int i = 0; i < s->n; i++) minrad = s->radius[i] < minrad ? s->radius[i] : minrad;
+    double cdf_all = 1.;
+    const int nl = s->mp.cohesion == OR_COH_BOND_NONLINEAR;
+    for (int i = 1; i < w; i++)
+        for (int j = 1; j < w; j++) {
+            const int t = i * T1 + j;
+            double cdf_one;
+            if (!bs->stress_break) {
+                cdf_one = 1.1 * 0.5 * s->bond[BT_MAXDIST][t] / minrad;
+            } else {
+                double stress = s->bond[BT_MAXSIGMA][t];
+                if (bs->ratio_tc) stress = fmax(stress, stress * s->bond[BT_RATIOTC][t]);
+                if (!nl)
+                    cdf_one = 0.5 * (1.1 * s->bond[BT_CREATEDIST][t] / minrad
+                                     + 1.1 * stress / (s->bond[BT_KN][t] * minrad));
+                else
+                    cdf_one = 0.5 * 1.1 * s->bond[BT_CREATEDIST][t] / minrad
+                              + 0.5 * 1.1 * stress / (s->bond[BT_KFN2][t] * minrad);
+            }
+            cdf_all = cdf_one > cdf_all ? cdf_one : cdf_all;
+        }
+    if (cdf_all > s->p.cdf) s->p.cdf = cdf_all;
+    return 0;
+}
+
+int or_add_move_linear(or_sim *s, int bit, const int set[3], const double v[3])
+{
+    op_t *o = new_op(s, OP_MOVE);
+    o->bit = bit;
+    for (int d = 0; d < 3; d++) {
+        o->style[d] = set[d];
+        o->val[d] = v[d];
+    }
+    o->xorig = malloc(sizeof(double) * 3 * (s->n > 0 ? s->n : 1));
+    memcpy(o->xorig, s->x, sizeof(double) * 3 * s->n);
+    o->time_origin = s->ntimestep;
+    return o->id;
+}
+
+void or_set_integrate_group(or_sim *s, int bit)
+{
+    s->integ_bit = bit;
+}
+
 /* NormalModel<HYSTERETIC_NONLINEAR1/2>::surfacesIntersect
  * (normal_model_hysteretic_nonlinear1.h:149-420, ..._nonlinear2.h:149-420).
  * The history is written whatever shearupdate says, and sqrt(5/4) is the
@@ -1088,6 +1464,7 @@ static void chain_intersect(const or_sim *s, sidata_t *sd, double Fi[3], double 
     } else {
         normal_hysteretic(s, sd, Fi, Fj);
     }
+    if (sd->model->cohesion != OR_COH_OFF) cohesion_bond(s, sd, Fi, Ti, Fj, Tj);
     if (sd->flags) *sd->flags |= CONTACT_TANGENTIAL_MODEL;
     tangential_history(s, sd, Fi, Ti, Fj, Tj);
     if (sd->model->off_roll >= 0) {
@@ -1163,6 +1540,8 @@ static void pair_compute(or_sim *s, int shearupdate)
                 sd.hist = h;
                 sd.model = &s->mp;
                 sd.flags = &e->flag;
+                sd.i = i;
+                sd.j = j;
                 chain_intersect(s, &sd, Fi, Ti, Fj, Tj);
                 for (int d = 0; d < 3; d++) {
                     s->f[3 * i + d] += Fi[d];
@@ -1180,6 +1559,40 @@ static void pair_compute(or_sim *s, int shearupdate)
                 }
             } else if (rsq < cdm * radsum * radsum) {
                 chain_close(&s->mp, h, &e->flag);
+                if (s->mp.cohesion != OR_COH_OFF) {
+                    /* the cohesion model's surfacesClose may apply a force
+                     * (has_force_update, pair_gran_base.h:417-430) */
+                    sidata_t sd;
+                    memset(&sd, 0, sizeof(sd));
+                    sd.itype = s->type[i]; sd.jtype = s->type[j];
+                    sd.radi = radi; sd.radj = radj; sd.radsum = radsum;
+                    sd.rsq = rsq;
+                    sd.delta[0] = delx; sd.delta[1] = dely; sd.delta[2] = delz;
+                    sd.v_i = &s->v[3 * i]; sd.v_j = &s->v[3 * j];
+                    sd.omega_i = &s->omega[3 * i]; sd.omega_j = &s->omega[3 * j];
+                    sd.update_history = shearupdate;
+                    sd.hist = h;
+                    sd.model = &s->mp;
+                    sd.flags = &e->flag;
+                    sd.i = i;
+                    sd.j = j;
+                    if (cohesion_bond(s, &sd, Fi, Ti, Fj, Tj)) {
+                        for (int d = 0; d < 3; d++) {
+                            s->f[3 * i + d] += Fi[d];
+                            s->torque[3 * i + d] += Ti[d];
+                        }
+                        s->fscale[i] += sqrt(Fi[0] * Fi[0] + Fi[1] * Fi[1] + Fi[2] * Fi[2]);
+                        s->tscale[i] += sqrt(Ti[0] * Ti[0] + Ti[1] * Ti[1] + Ti[2] * Ti[2]);
+                        if (e->shift == 13) {
+                            for (int d = 0; d < 3; d++) {
+                                s->f[3 * j + d] += Fj[d];
+                                s->torque[3 * j + d] += Tj[d];
+                            }
+                            s->fscale[j] += sqrt(Fj[0] * Fj[0] + Fj[1] * Fj[1] + Fj[2] * Fj[2]);
+                            s->tscale[j] += sqrt(Tj[0] * Tj[0] + Tj[1] * Tj[1] + Tj[2] * Tj[2]);
+                        }
+                    }
+                }
             }
         }
     }
@@ -2099,7 +2512,27 @@ static void initial_integrate(or_sim *s)
     const double dtfrotate = dtf / 0.4;
     /* FixMoveMesh::initial_integrate -> MeshMoverLinear (mesh_mover_linear.cpp:94-112) */
     for (int k = 0; k < s->nmesh; k++) mesh_move(s, &s->mesh[k]);
+    /* FixMove::initial_integrate, style linear (fix_move.cpp:430-490) */
+    for (int q = 0; q < s->nops; q++) {
+        op_t *o = &s->ops[q];
+        if (o->kind != OP_MOVE) continue;
+        const double delta = (s->ntimestep - o->time_origin) * s->p.dt;
+        for (int i = 0; i < s->n; i++) {
+            if (!(s->gmask[i] & (1 << o->bit))) continue;
+            const double dtfm = dtf / s->rmass[i];
+            for (int d = 0; d < 3; d++) {
+                if (o->style[d]) {
+                    s->v[3 * i + d] = o->val[d];
+                    s->x[3 * i + d] = o->xorig[3 * i + d] + o->val[d] * delta;
+                } else {
+                    s->v[3 * i + d] += dtfm * s->f[3 * i + d];
+                    s->x[3 * i + d] += dtv * s->v[3 * i + d];
+                }
+            }
+        }
+    }
     for (int i = 0; i < s->n; i++) {
+        if (s->integ_bit >= 0 && !(s->gmask[i] & (1 << s->integ_bit))) continue;
         const double dtfm = dtf / (s->rmass[i] * (1. + 0.0 / 1.0));
         double *v = &s->v[3 * i], *x = &s->x[3 * i], *f = &s->f[3 * i];
         v[0] += dtfm * f[0]; v[1] += dtfm * f[1]; v[2] += dtfm * f[2];
@@ -2114,7 +2547,18 @@ static void final_integrate(or_sim *s)
 {
     const double dtf = 0.5 * s->p.dt;
     const double dtfrotate = dtf / 0.4;
+    for (int q = 0; q < s->nops; q++) {
+        op_t *o = &s->ops[q];
+        if (o->kind != OP_MOVE) continue;
+        for (int i = 0; i < s->n; i++) {
+            if (!(s->gmask[i] & (1 << o->bit))) continue;
+            const double dtfm = dtf / s->rmass[i];
+            for (int d = 0; d < 3; d++)
+                if (!o->style[d]) s->v[3 * i + d] += dtfm * s->f[3 * i + d];
+        }
+    }
     for (int i = 0; i < s->n; i++) {
+        if (s->integ_bit >= 0 && !(s->gmask[i] & (1 << s->integ_bit))) continue;
         const double dtfm = dtf / (s->rmass[i] * (1. + 0.0 / 1.0));
         double *v = &s->v[3 * i], *f = &s->f[3 * i];
         v[0] += dtfm * f[0]; v[1] += dtfm * f[1]; v[2] += dtfm * f[2];

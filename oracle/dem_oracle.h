@@ -43,7 +43,34 @@ typedef struct {
     int normal_model;            /* OR_NORMAL_* of pair_style gran          */
     int wall_normal_model;       /* OR_NORMAL_* of the fix wall/gran model  */
     int wall_limit_force;        /* limitForce of the wall model            */
+    int cohesion_model;          /* OR_COH_* of pair_style gran (pairs only) */
 } or_params;
+
+/* cohesion keyword of pair_style gran */
+enum { OR_COH_OFF = 0, OR_COH_BOND = 1, OR_COH_BOND_NONLINEAR = 2 };
+
+/* bond tables, each (ntypes+1)^2, in this order (a table a model does not
+ * use is ignored):
+ *  0 radiusMultiplierBond[nonlinear]   1 normalBondStiffnessPerUnitArea
+ *  2 tangentialBondStiffnessPerUnitArea
+ *  3-6 damping{Normal,Tangential}{Force,Torque}Bond[nonlinear]
+ *  7 maxDistanceBond  8 maxSigmaBond  9 maxTauBond
+ * 10 ratioTensionCompression  11 createDistanceBond
+ * 12-24 stiffnessPerUnitArea K_fn1 Ku_fn1 Kc_fn1 K_fn2 Ku_fn2 Kc_fn2 K_ft
+ *       K_tn Ku_tn Kc_tn K_tt Ku_tt Kc_tt  (bond/nonlinear) */
+#define OR_NBOND 25
+/* bond settings (registerSettings of the cohesion model) */
+typedef struct {
+    double ts_create;        /* tsCreateBond[nonlinear]                      */
+    int stress_break;        /* stressBreak                                  */
+    int create_always;       /* createBondAlways                             */
+    int damping;             /* dampingBond (default on)                     */
+    int damping_smooth;      /* dampingBondSmooth                            */
+    int ratio_tc;            /* ratioTensionCompression[Bond]                */
+    int tension_variant;     /* bond/nonlinear: the normal law of the INL
+                                example header cohesion_model_bond_nonlinear_
+                                tension.h instead of the in-tree one         */
+} or_bond_settings;
 
 /* model keyword of pair_style gran / fix wall/gran */
 enum { OR_NORMAL_HERTZ = 0, OR_NORMAL_HYST1 = 1, OR_NORMAL_HYST2 = 2 };
@@ -104,6 +131,16 @@ void or_mesh_force(const or_sim *s, int mesh, double f[3]);
 /* current mesh contacts: (atom tag, triangle id, slot index, history);
  * NULL to count */
 int or_mesh_contacts(const or_sim *s, int mesh, int *tag, int *tri, int *slot, double *hist);
+/* cohesion bond / bond/nonlinear tables (OR_NBOND) and settings; registers
+ * the model's contact distance factor (neighbor.h:141-142) from the tables
+ * and the smallest atom radius, so call after or_set_atoms.  Returns 0. */
+int or_set_bond(or_sim *s, const double *tables, const or_bond_settings *bs);
+/* fix move linear on the atoms of group bit (fix_move.cpp:430-490):
+ * x = x_original + v * (ntimestep - origin) * dt each step; components with
+ * set[d] == 0 are integrated like nve.  Returns the fix id. */
+int or_add_move_linear(or_sim *s, int bit, const int set[3], const double v[3]);
+/* fix nve/sphere only on the atoms of group bit (default: all atoms) */
+void or_set_integrate_group(or_sim *s, int bit);
 /* hysteretic/nonlinear* parameter tables, see OR_NHYST */
 int or_set_hysteretic(or_sim *s, const double *tables);
 /* group membership: bit 'bit' of atom i's mask (group command) */

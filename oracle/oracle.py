@@ -34,7 +34,31 @@ class _Params(C.Structure):
         ("normal_model", C.c_int),
         ("wall_normal_model", C.c_int),
         ("wall_limit_force", C.c_int),
+        ("cohesion_model", C.c_int),
     ]
+
+
+class _BondSettings(C.Structure):
+    _fields_ = [
+        ("ts_create", C.c_double),
+        ("stress_break", C.c_int),
+        ("create_always", C.c_int),
+        ("damping", C.c_int),
+        ("damping_smooth", C.c_int),
+        ("ratio_tc", C.c_int),
+        ("tension_variant", C.c_int),
+    ]
+
+
+COHESION = {"off": 0, "bond": 1, "bond/nonlinear": 2}
+# bond tables in the order of OR_NBOND (dem_oracle.h); the script property
+# names, with "nonlinear"/"stiffnessPerUnitArea" spellings of bond/nonlinear
+# mapped onto the same slots
+BOND_KEYS = ("radiusMultiplier", "normalBondStiffnessPerUnitArea", "tangentialBondStiffnessPerUnitArea",
+             "dampingNormalForce", "dampingTangentialForce", "dampingNormalTorque", "dampingTangentialTorque",
+             "maxDistance", "maxSigma", "maxTau", "ratioTensionCompression", "createDistance",
+             "K_fn1", "Ku_fn1", "Kc_fn1", "K_fn2", "Ku_fn2", "Kc_fn2", "K_ft", "K_tn", "Ku_tn", "Kc_tn",
+             "K_tt", "Ku_tt", "Kc_tt")
 
 
 NORMAL = {"hertz": 0, "hysteretic/nonlinear1": 1, "hysteretic/nonlinear2": 2}
@@ -96,6 +120,9 @@ def lib():
         L.or_mesh_force.argtypes = [C.c_void_p, C.c_int, dp]
         L.or_mesh_size.argtypes = [C.c_void_p, C.c_int]
         L.or_mesh_contacts.argtypes = [C.c_void_p, C.c_int, ip, ip, ip, dp]
+        L.or_set_bond.argtypes = [C.c_void_p, dp, C.POINTER(_BondSettings)]
+        L.or_add_move_linear.argtypes = [C.c_void_p, C.c_int, ip, dp]
+        L.or_set_integrate_group.argtypes = [C.c_void_p, C.c_int]
         L.or_mesh_distance.restype = C.c_double
         L.or_mesh_distance.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, dp, dp, dp, ip]
         _LIB = L
@@ -143,6 +170,7 @@ class Oracle:
         p.limit_force = int(spec.get("limit_force", p.normal_model != 0))
         p.wall_limit_force = int(spec.get("wall_limit_force", p.wall_normal_model != 0))
         p.torsion_torque = int(spec.get("torsion_torque", False))
+        p.cohesion_model = COHESION[spec.get("cohesion", "off")]
         self.h = self.L.or_create(C.byref(p))
         m = spec["materials"]
         nt = m["ntypes"]
@@ -202,6 +230,8 @@ class Oracle:
             elif item == "freeze":
                 self.L.or_add_freeze(self.h)
         self.nplanes = len(planes)
+        if "bond" in spec:
+            self.set_bond(**spec["bond"])
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -287,6 +317,28 @@ class Oracle:
         fid = self.L.or_add_plane(self.h, int(dim), float(pos), int(wtype))
         self.nplanes += 1
         return fid
+
+    def set_bond(self, tables, ts_create=0.0, stress_break=False, create_always=False, damping=True,
+                 damping_smooth=False, ratio_tc=False, tension_variant=False):
+        """Bond tables {BOND_KEYS name: (ntypes+1)^2} and settings."""
+        w = self.ntypes + 1
+        tabs = np.zeros((len(BOND_KEYS), w, w))
+        for k, v in tables.items():
+            tabs[BOND_KEYS.index(k)] = np.asarray(v, dtype=np.float64).reshape(w, w)
+        tabs = np.ascontiguousarray(tabs)
+        self._keep.append(tabs)
+        b = _BondSettings(float(ts_create), int(stress_break), int(create_always), int(damping),
+                          int(damping_smooth), int(ratio_tc), int(tension_variant))
+        self.L.or_set_bond(self.h, _dp(tabs), C.byref(b))
+
+    def add_move_linear(self, bit, v):
+        """fix move linear on group bit; None components are integrated."""
+        st = np.array([c is not None for c in v], dtype=np.int32)
+        va = np.array([0.0 if c is None else c for c in v])
+        return self.L.or_add_move_linear(self.h, int(bit), _ip(st), _dp(va))
+
+    def set_integrate_group(self, bit):
+        self.L.or_set_integrate_group(self.h, int(bit))
 
     def add_gravity(self, ax, ay, az):
         return self.L.or_add_gravity(self.h, float(ax), float(ay), float(az))
