@@ -54,7 +54,32 @@ typedef struct {
     int normal_model;             /* DEM_NORMAL_* of pair_style gran model   */
     int wall_normal_model;        /* DEM_NORMAL_* of the fix wall/gran model */
     int wall_limit_force;         /* limitForce of the wall model            */
+    int cohesion_model;           /* DEM_COHESION_* of pair_style gran       */
 } dem_params;
+
+/* `cohesion` keyword of the pair chain: off, bond (cohesion_model_bond.h),
+ * bond/nonlinear (cohesion_model_bond_nonlinear.h).  Particle pairs only. */
+enum { DEM_COHESION_OFF = 0, DEM_COHESION_BOND = 1, DEM_COHESION_BOND_NONLINEAR = 2 };
+/* bond tables, each (ntypes+1)^2 row-major, in this order:
+ *  0 radiusMultiplierBond[nonlinear]  1 normalBondStiffnessPerUnitArea
+ *  2 tangentialBondStiffnessPerUnitArea
+ *  3-6 damping{Normal,Tangential}{Force,Torque}Bond[nonlinear]
+ *  7 maxDistanceBond  8 maxSigmaBond  9 maxTauBond  10 ratioTensionCompression
+ * 11 createDistanceBond  12-24 stiffnessPerUnitArea K_fn1 Ku_fn1 Kc_fn1 K_fn2
+ *    Ku_fn2 Kc_fn2 K_ft K_tn Ku_tn Kc_tn K_tt Ku_tt Kc_tt (bond/nonlinear) */
+#define DEM_NBOND 25
+/* settings of the cohesion model (registerSettings) */
+typedef struct {
+    double ts_create;             /* tsCreateBond[nonlinear]                 */
+    int stress_break;             /* stressBreak                             */
+    int create_always;            /* createBondAlways                        */
+    int damping;                  /* dampingBond (default 1)                 */
+    int damping_smooth;           /* dampingBondSmooth                       */
+    int ratio_tc;                 /* ratioTensionCompression[Bond]           */
+    int tension_variant;          /* bond/nonlinear: normal law of the INL
+                                     example header cohesion_model_bond_
+                                     nonlinear_tension.h                     */
+} dem_bond_settings;
 
 /* `model` keyword: hertz (normal_model_hertz.h), hysteretic/nonlinear1
  * (normal_model_hysteretic_nonlinear1.h), hysteretic/nonlinear2 */
@@ -139,6 +164,11 @@ int dem_get_mesh_contacts(dem_ctx *ctx, int mesh, int64_t *count, int *tag, int 
  * FixContactHistoryMesh); the next rebuild carries them over. */
 int dem_set_mesh_contacts(dem_ctx *ctx, int mesh, int64_t count, const int *tag, const int *tri,
                           const int *slot, const double *hist);
+/* Bond tables (DEM_NBOND) and settings of the cohesion model
+ * (cohesion_model_bond.h:237-470, cohesion_model_bond_nonlinear.h:204-386).
+ * Call after dem_upload_state: registers the model's contact distance
+ * factor from the tables and the smallest radius (neighbor.h:141-142). */
+int dem_set_bond(dem_ctx *ctx, const double *tables, const dem_bond_settings *bs);
 /* fix gravity acceleration (fix_gravity.cpp:301-352, :395-397). */
 int dem_set_gravity(dem_ctx *ctx, double ax, double ay, double az);
 /* fix freeze present (fix_freeze.cpp:121-144; pair_gran.cpp:532-537). */

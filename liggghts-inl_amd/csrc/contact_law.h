@@ -182,9 +182,13 @@ __host__ __device__ constexpr int chain_max_dnum(int nm) { return chain_off_shea
 
 // Returns the i-side force and torque increments (ForceData i_forces).
 // h: L.dnum history doubles in the chain's layout, updated in place.
-template <int NM>
+// CH: a cohesion model sits between the normal and tangential models; coh =
+// its force and torque on this side, cmode CM_BOND adds them to the chain,
+// CM_BOND_NONLINEAR replaces the normal force with them, CM_OFF skips.
+template <int NM, bool CH = false>
 __device__ inline void contact_fp64(const Model &M, const Layout &L, const double *mat, const ContactIn &c,
-                                    double *h, bool update, double F[3], double T[3])
+                                    double *h, bool update, double F[3], double T[3],
+                                    const double *coh = nullptr, int cmode = CM_OFF)
 {
     const double SQRT_FIVE_OVER_SIX = 0.91287092917527685576161630466800355658790782499663875;
     const double enx = c.delta[0] * c.rinv;
@@ -247,6 +251,15 @@ __device__ inline void contact_fp64(const Model &M, const Layout &L, const doubl
     F[0] = Fn * enx;
     F[1] = Fn * eny;
     F[2] = Fn * enz;
+    double T0[3] = {0.0, 0.0, 0.0};
+    if (CH && cmode != CM_OFF) {
+        if (cmode == CM_BOND_NONLINEAR) {
+            F[0] = coh[0]; F[1] = coh[1]; F[2] = coh[2];
+        } else {
+            F[0] += coh[0]; F[1] += coh[1]; F[2] += coh[2];
+        }
+        T0[0] = coh[3]; T0[1] = coh[4]; T0[2] = coh[5];
+    }
 
     // ---- tangential history
     double *shear = h + chain_off_shear(NM);
@@ -292,9 +305,15 @@ __device__ inline void contact_fp64(const Model &M, const Layout &L, const doubl
     F[0] += Ft1;
     F[1] += Ft2;
     F[2] += Ft3;
-    T[0] = -cri * tor1;
-    T[1] = -cri * tor2;
-    T[2] = -cri * tor3;
+    if (CH && cmode != CM_OFF) {
+        T[0] = T0[0] + -cri * tor1;
+        T[1] = T0[1] + -cri * tor2;
+        T[2] = T0[2] + -cri * tor3;
+    } else {
+        T[0] = -cri * tor1;
+        T[1] = -cri * tor2;
+        T[2] = -cri * tor3;
+    }
 
     // ---- rolling friction
     if (M.rolling && L.off_roll >= 0) {

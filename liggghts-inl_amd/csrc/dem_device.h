@@ -8,7 +8,8 @@
 //   t4  double4  (tx, ty, tz, -)             32 B
 //   tag int, orig int (upload position), img int4 (periodic image counts)
 // Neighbour list: full (both directions), CSR rows per particle:
-//   nstart[n+1]; nj[e] = j | shift << 26 | flag << 31; hist[e*dnum + k]
+//   nstart[n+1]; nj[e] = j | bond << 25 | shift << 26 | flag << 31; hist[e*dnum + k]
+//   (history slots: hysteretic 12, shear 3, rolling 3, cohesion 14/28)
 // Primitive walls: wmask[i] bit k = atom in wall k's list; whist[(i*np+k)*dnum]
 // Mesh walls, per mesh: TriGeo records; per particle a CSR row (mstart) of
 // candidate triangles (mcand, ascending id) and as many contact slots
@@ -22,9 +23,10 @@ namespace dem {
 constexpr int kMaxTypes = 8;
 constexpr int kT1 = kMaxTypes + 1;
 constexpr int kMaxPlanes = 8;
-constexpr int kMaxDnum = 18;   // hysteretic 12 + shear 3 + rolling 3
-constexpr uint32_t kJMask = (1u << 26) - 1u;
-constexpr uint32_t kFlagBit = 1u << 31;
+constexpr int kMaxDnum = 18;   // hysteretic 12 + shear 3 + rolling 3 (walls)
+constexpr uint32_t kJMask = (1u << 25) - 1u;
+constexpr uint32_t kBondBit = 1u << 25;   // the cohesion model's contact flag
+constexpr uint32_t kFlagBit = 1u << 31;   // the other models' contact flags
 constexpr int kNoShift = 13;
 constexpr int kBlock = 256;
 
@@ -34,9 +36,17 @@ constexpr int kBlock = 256;
 enum {
     M_YEFF = 0, M_GEFF, M_BETA, M_MU, M_MUR, M_ETAR, M_RESTLOG,
     M_HK, M_HKN2K1, M_HKN2KC, M_HPHIF, M_HFADH, M_HALPHA, M_HCIN, M_HA1, M_HA2, M_HA3, M_HKCIN,
+    // bond tables (cohesion bond, bond/nonlinear), in the order of DEM_NBOND
+    M_BLAMBDA, M_BKN, M_BKT, M_BDFN, M_BDFT, M_BDTN, M_BDTT, M_BMAXDIST, M_BMAXSIGMA, M_BMAXTAU,
+    M_BRATIOTC, M_BCREATEDIST, M_BKFN1, M_BKUFN1, M_BKCFN1, M_BKFN2, M_BKUFN2, M_BKCFN2, M_BKFT,
+    M_BKTN, M_BKUTN, M_BKCTN, M_BKTT, M_BKUTT, M_BKCTT,
     M_COUNT
 };
 constexpr int kNHyst = 11;
+constexpr int kNBond = 25;
+
+// cohesion model of the pair chain
+enum { CM_OFF = 0, CM_BOND = 1, CM_BOND_NONLINEAR = 2 };
 
 // normal model of a contact chain
 enum { NM_HERTZ = 0, NM_HYST1 = 1, NM_HYST2 = 2 };
@@ -49,7 +59,14 @@ struct Layout {
     int off_norm;    // hysteretic: 12 slots, else -1
     int off_shear;
     int off_roll;    // -1 without rolling friction
+    int off_coh;     // cohesion slots (after the others on the device), -1 none
     int dnum;
+};
+
+// bond settings (registerSettings of the cohesion models)
+struct BondSet {
+    long long ts_create;   // static_cast<int>(tsCreateBond)
+    int stress_break, create_always, damping, damping_smooth, ratio_tc, tension_variant;
 };
 
 struct Planes {
@@ -95,6 +112,8 @@ struct Model {
     int dnum;            // history doubles per pair
     int wdnum;           // history doubles per (atom, plane)
     Layout lp, lw;       // pair and wall chains
+    int cohesion;        // CM_* of the pair chain
+    BondSet bs;
 };
 
 struct Grid {

@@ -111,6 +111,12 @@ struct dem_ctx {
     bool timing = false;
     Timer t_contact, t_integrate, t_rebuild;
 
+    // cohesion bond pass: per entry force/torque, validity, reverse entry
+    double *bout = nullptr;
+    uint8_t *bvalid = nullptr;
+    int *brev = nullptr;
+    int64_t bcap = 0;
+
     std::vector<MeshDev> meshes;
     bool mesh_grids_valid = false;    // built for the current radii
     long long next_reneighbor = -1;   // FixMesh::next_reneighbor
@@ -206,7 +212,7 @@ extern "C" const char *dem_last_error(const dem_ctx *ctx)
 // 150-157): hysteretic normal state (12, normal_model_hysteretic_nonlinear1.h
 // :84-97), shear (3, tangential_model_history.h:79-81), rolling torque (3,
 // rolling_model_epsd.h:71-73).
-static Layout chain_layout(int normal, int limitf, int rolling)
+static Layout chain_layout(int normal, int limitf, int rolling, int cohesion)
 {
     Layout L;
     int k = 0;
@@ -224,6 +230,13 @@ static Layout chain_layout(int normal, int limitf, int rolling)
         L.off_roll = k;
         k += 3;
     }
+    // cohesion slots last on the device (the reference registers them after
+    // the normal model; dem_get_pairs / dem_set_history convert)
+    L.off_coh = -1;
+    if (cohesion != CM_OFF) {
+        L.off_coh = k;
+        k += cohesion == CM_BOND ? 14 : 28;
+    }
     L.dnum = k;
     return L;
 }
@@ -236,7 +249,8 @@ extern "C" dem_ctx *dem_ctx_create(const dem_params *p)
     }
     if (!(p->dt > 0.0) || !(p->skin >= 0.0) || !(p->cdf >= 1.0) || p->neigh_every < 1
         || p->rolling_model < 0 || p->rolling_model > 4 || p->precision < 0 || p->precision > 1
-        || p->normal_model < 0 || p->normal_model > 2 || p->wall_normal_model < 0 || p->wall_normal_model > 2) {
+        || p->normal_model < 0 || p->normal_model > 2 || p->wall_normal_model < 0 || p->wall_normal_model > 2
+        || p->cohesion_model < 0 || p->cohesion_model > 2) {
         g_last_error = "dem_ctx_create: invalid parameters";
         return nullptr;
     }
@@ -278,8 +292,10 @@ extern "C" dem_ctx *dem_ctx_create(const dem_params *p)
     M.rolling = p->rolling_model;
     M.tdamp = p->tangential_damping;
     M.torsion = p->torsion_torque;
-    M.lp = chain_layout(p->normal_model, p->limit_force, p->rolling_model);
-    M.lw = chain_layout(p->wall_normal_model, p->wall_limit_force, p->rolling_model);
+    M.lp = chain_layout(p->normal_model, p->limit_force, p->rolling_model, p->cohesion_model);
+    M.lw = chain_layout(p->wall_normal_model, p->wall_limit_force, p->rolling_model, CM_OFF);
+    M.cohesion = p->cohesion_model;
+    M.bs = BondSet{0, 0, 0, 1, 0, 0, 0};
     M.dnum = M.lp.dnum;
     M.wdnum = M.lw.dnum;
     ctx->P.n = 0;
@@ -298,6 +314,7 @@ static void free_all(dem_ctx *ctx)
     F(ctx->f4); F(ctx->t4); F(ctx->xhold); F(ctx->wmask); F(ctx->ncontact); F(ctx->key);
     F(ctx->sorted); F(ctx->cstart); F(ctx->cursor); F(ctx->scan_tmp); F(ctx->spread);
     F(ctx->d_mat); F(ctx->d_small);
+    F(ctx->bout); F(ctx->bvalid); F(ctx->brev);
     for (auto &m : ctx->meshes) {
         F(m.tri); F(m.enorm); F(m.gstart); F(m.glist);
         for (int b = 0; b < 2; b++) {
@@ -485,6 +502,62 @@ extern "C" int dem_mesh_size(const dem_ctx *ctx, int mesh)
 {
     if (!ctx || mesh < 0 || mesh >= (int)ctx->meshes.size()) return 0;
     return ctx->meshes[mesh].h.ntri;
+}
+
+static int setup_grid(dem_ctx *ctx);
+
+extern "C" int dem_set_bond(dem_ctx *ctx, const double *tables, const dem_bond_settings *bs)
+{
+    if (!ctx) return fail(nullptr, DEM_ERR_ARG, "null context");
+    if (!tables || !bs || ctx->M.cohesion == CM_OFF || !ctx->mat_set || ctx->n == 0)
+        return fail(ctx, DEM_ERR_ARG,
+                    "dem_set_bond: needs a cohesion model, materials and uploaded atoms, non-null tables");
+    hipSetDevice(ctx->p.device);
+    const int w = ctx->ntypes + 1;
+    std::vector<double> h(M_COUNT * kT1 * kT1, 0.0);
+    HIPCHK(hipMemcpy(h.data(), ctx->d_mat, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int q = 0; q < kNBond; q++)
+        for (int i = 0; i < w; i++)
+            for (int j = 0; j < w; j++)
+                h[(size_t)(M_BLAMBDA + q) * kT1 * kT1 + i * kT1 + j] = tables[(size_t)q * w * w + i * w + j];
+    HIPCHK(hipMemcpy(ctx->d_mat, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+    ctx->M.bs = BondSet{(long long)(int)bs->ts_create, bs->stress_break, bs->create_always, bs->damping,
+                        bs->damping_smooth, bs->ratio_tc, bs->tension_variant};
+    // the model's contact distance factor (cohesion_model_bond.h:415-470,
+    // cohesion_model_bond_nonlinear.h:330-380) with the smallest radius
+    std::vector<double4> hx(ctx->n);
+    HIPCHK(hipMemcpy(hx.data(), ctx->xr[ctx->cur], sizeof(double4) * ctx->n, hipMemcpyDeviceToHost));
+    double minrad = 1e300;
+    for (const auto &q : hx) minrad = std::min(minrad, q.w);
+    const bool nl = ctx->M.cohesion == CM_BOND_NONLINEAR;
+    double cdf_all = 1.;
+    auto T = [&](int slot, int i, int j) { return tables[(size_t)slot * w * w + i * w + j]; };
+    for (int i = 1; i < w; i++)
+        for (int j = 1; j < w; j++) {
+            double cdf_one;
+            if (!bs->stress_break) {
+                cdf_one = 1.1 * 0.5 * T(7, i, j) / minrad;
+            } else {
+                double stress = T(8, i, j);
+                if (bs->ratio_tc) stress = std::fmax(stress, stress * T(10, i, j));
+                const double k = nl ? T(15, i, j) : T(1, i, j);
+                if (k <= 1e-15) return fail(ctx, DEM_ERR_ARG, "Bond settings: In case of stress breakage, the normal bond stiffness can't be <= 0!");
+                if (!nl) cdf_one = 0.5 * (1.1 * T(11, i, j) / minrad + 1.1 * stress / (k * minrad));
+                else cdf_one = 0.5 * 1.1 * T(11, i, j) / minrad + 0.5 * 1.1 * stress / (k * minrad);
+            }
+            cdf_all = std::max(cdf_all, cdf_one);
+        }
+    if (cdf_all > 10.)
+        return fail(ctx, DEM_ERR_ARG, "Maximum bond distance exceeding 10 x particle diameter, please reduce "
+                                      "maxDistanceBond or maxSigmaBond/maxTauBond");
+    if (cdf_all > ctx->p.cdf) {
+        ctx->p.cdf = cdf_all;
+        ctx->M.cdf = cdf_all;
+        ctx->mesh_grids_valid = false;
+        const int rc = setup_grid(ctx);
+        if (rc) return rc;
+    }
+    return DEM_OK;
 }
 
 extern "C" int dem_set_gravity(dem_ctx *ctx, double ax, double ay, double az)
@@ -833,6 +906,18 @@ static int rebuild(dem_ctx *ctx)
         const int rc = mesh_rebuild(ctx, m);
         if (rc) return rc;
     }
+    if (ctx->M.cohesion != CM_OFF) {
+        const int64_t E = hsmall[0];
+        if (E > ctx->bcap) {
+            const int64_t c = E + E / 4 + 1024;
+            if (dev_alloc(ctx, &ctx->bout, (size_t)c * 6) || dev_alloc(ctx, &ctx->bvalid, (size_t)c)
+                || dev_alloc(ctx, &ctx->brev, (size_t)c))
+                return DEM_ERR_NOMEM;
+            ctx->bcap = c;
+        }
+        launch_bond_rev(ctx->nstart[l1], ctx->nj[l1], ctx->brev, n, s);
+        KCHK();
+    }
     timer_end(ctx, ctx->t_rebuild);
     return DEM_OK;
 }
@@ -856,6 +941,26 @@ static int force_pass(dem_ctx *ctx, int shearupdate)
     fa.M = ctx->M;
     fa.P = ctx->P;
     fa.defer_freeze = ctx->meshes.empty() ? 0 : 1;
+    if (ctx->M.cohesion != CM_OFF) {
+        BondArgs ba{};
+        ba.xr = fa.xr; ba.vm = fa.vm; ba.wt = fa.wt;
+        ba.tag = ctx->tag[c];
+        ba.nstart = fa.nstart;
+        ba.nj = fa.nj;
+        ba.hist = fa.hist;
+        ba.rev = ctx->brev;
+        ba.bout = ctx->bout;
+        ba.bvalid = ctx->bvalid;
+        ba.mat = ctx->d_mat;
+        ba.n = ctx->n;
+        ba.shearupdate = shearupdate;
+        ba.ntimestep = ctx->ntimestep;
+        ba.M = ctx->M;
+        launch_bond(ba, ctx->stream);
+        KCHK();
+        fa.bout = ctx->bout;
+        fa.bvalid = ctx->bvalid;
+    }
     timer_begin(ctx, ctx->t_contact);
     if (ctx->p.precision == DEM_MIXED) launch_force_mixed(fa, ctx->stream);
     else launch_force_fp64(fa, ctx->stream);
@@ -1117,16 +1222,49 @@ extern "C" int dem_get_pairs(dem_ctx *ctx, int64_t *count, int *tag_i, int *tag_
         HIPCHK(hipMemcpy(nj.data(), ctx->nj[l], E * sizeof(uint32_t), hipMemcpyDeviceToHost));
         HIPCHK(hipMemcpy(hh.data(), ctx->hist[l], E * dnum * sizeof(double), hipMemcpyDeviceToHost));
     }
+    // reference slot order: [hysteretic][cohesion][shear][rolling]; the
+    // device keeps cohesion last, and only on the evaluating entry of a pair
+    const Layout &L = ctx->M.lp;
+    const int ncoh = L.off_coh >= 0 ? dnum - L.off_coh : 0;
+    const int off_ref_coh = L.off_norm >= 0 ? 12 : 0;
+    std::unordered_map<int64_t, int64_t> entry_of;
+    if (ncoh)
+        for (int i = 0; i < n; i++)
+            for (int e = ns[i]; e < ns[i + 1]; e++)
+                if (((nj[e] >> 26) & 31u) == (uint32_t)kNoShift)
+                    entry_of[(int64_t)i * (int64_t)n + (int64_t)(nj[e] & kJMask)] = e;
     for (int i = 0; i < n; i++)
         for (int e = ns[i]; e < ns[i + 1]; e++) {
+            const int j = (int)(nj[e] & kJMask);
+            const int code = (int)((nj[e] >> 26) & 31u);
             tag_i[e] = tg[i];
-            tag_j[e] = tg[nj[e] & kJMask];
-            if (shift) shift[e] = (int)((nj[e] >> 26) & 31u);
-            const bool known = (nj[e] & kFlagBit) != 0;
+            tag_j[e] = tg[j];
+            if (shift) shift[e] = code;
+            const bool known = (nj[e] & (kFlagBit | kBondBit)) != 0;
             if (flag) flag[e] = known ? 1 : 0;
+            if (!hist) continue;
+            double *out = hist + (size_t)e * dnum;
+            const double *src = &hh[(size_t)e * dnum];
             // an entry that never touched holds no history (zero by definition)
-            if (hist)
-                for (int k = 0; k < dnum; k++) hist[(size_t)e * dnum + k] = known ? hh[(size_t)e * dnum + k] : 0.0;
+            if (!known) {
+                for (int k = 0; k < dnum; k++) out[k] = 0.0;
+                continue;
+            }
+            int k = 0;
+            for (int q = 0; q < off_ref_coh; q++) out[k++] = src[q];
+            if (ncoh) {
+                const double *cs = src + L.off_coh;
+                double sign = 1.0;
+                if (code == kNoShift && tg[j] < tg[i]) {
+                    // the partner's entry evaluates the bond: its copy with the
+                    // newtonflag slots (ft, torque/angle vectors) negated
+                    auto it = entry_of.find((int64_t)j * (int64_t)n + i);
+                    cs = it != entry_of.end() ? &hh[(size_t)it->second * dnum + L.off_coh] : cs;
+                    sign = -1.0;
+                }
+                for (int q = 0; q < ncoh; q++) out[k++] = (q >= 5 && q < 14) ? sign * cs[q] : cs[q];
+            }
+            for (int q = off_ref_coh; q < (L.off_coh >= 0 ? L.off_coh : dnum); q++) out[k++] = src[q];
         }
     *count = E;
     return DEM_OK;
@@ -1160,7 +1298,23 @@ extern "C" int dem_set_history(dem_ctx *ctx, int64_t count, const int *tag_i, co
         const int i = idx[tag_i[k]];
         const int e = fillp[i]++;
         nj[e] = (uint32_t)idx[tag_j[k]] | ((uint32_t)kNoShift << 26) | kFlagBit;
-        for (int d = 0; d < dnum; d++) hh[(size_t)e * dnum + d] = hist[(size_t)k * dnum + d];
+        // reference slot order [hysteretic][cohesion][shear][rolling] to the
+        // device's [hysteretic][shear][rolling][cohesion]
+        const double *src = hist + (size_t)k * dnum;
+        double *dst = &hh[(size_t)e * dnum];
+        const Layout &L = ctx->M.lp;
+        if (L.off_coh < 0) {
+            for (int d = 0; d < dnum; d++) dst[d] = src[d];
+        } else {
+            const int ncoh = dnum - L.off_coh;
+            const int off_ref_coh = L.off_norm >= 0 ? 12 : 0;
+            int q = 0;
+            for (int d = 0; d < off_ref_coh; d++) dst[q++] = src[d];
+            for (int d = off_ref_coh + ncoh; d < dnum; d++) dst[q++] = src[d];
+            for (int d = 0; d < ncoh; d++) dst[q++] = src[off_ref_coh + d];
+            const double small = ctx->M.cohesion == CM_BOND ? 1e-6 : 1e-15;
+            if (src[off_ref_coh] >= small) nj[e] |= kBondBit;
+        }
     }
     const int l = ctx->lcur;
     if (ensure_list_cap(ctx, l, count)) return DEM_ERR_NOMEM;

@@ -51,7 +51,7 @@ __device__ inline int group_sum_int(int v)
     return v;
 }
 
-template <bool MIXED, int G, int NM, int WN>
+template <bool MIXED, int G, int NM, int WN, bool CH = false>
 __global__ void __launch_bounds__(kBlock) k_force(ForceArgs a)
 {
     const int blk = xcd_block(blockIdx.x, gridDim.x);
@@ -99,6 +99,14 @@ __global__ void __launch_bounds__(kBlock) k_force(ForceArgs a)
         const double rsq = delx * delx + dely * dely + delz * delz;
         const double radj = pj.w;
         const double radsum = pi.w + radj;
+        // cohesion force of a bonded entry, computed by the bond pass
+        bool bonded = false;
+        double coh[6];
+        if (CH && (w & kBondBit) && a.bvalid[e]) {
+            bonded = true;
+            const double *bo = a.bout + (size_t)e * 6;
+            for (int q = 0; q < 6; q++) coh[q] = bo[q];
+        }
         if (rsq < radsum * radsum) {
             const double r = sqrt(rsq);
             if (r < 1e-20) atomicOr(a.err, 1);
@@ -120,11 +128,13 @@ __global__ void __launch_bounds__(kBlock) k_force(ForceArgs a)
             constexpr int HN = chain_max_dnum(NM);
             double h[HN];
             double *hp = a.hist + (size_t)e * dnum;
-            const bool known = (w & kFlagBit) != 0;
+            const bool known = (w & (CH ? (kFlagBit | kBondBit) : kFlagBit)) != 0;
 #pragma unroll
             for (int k = 0; k < HN; k++) h[k] = (known && k < dnum) ? hp[k] : 0.0;
             double dF[3], dT[3];
             if (MIXED && NM == NM_HERTZ) contact_mixed(M, a.mat, c, h, a.shearupdate, dF, dT);
+            else if (CH) contact_fp64<NM, true>(M, M.lp, a.mat, c, h, a.shearupdate, dF, dT, coh,
+                                                bonded ? M.cohesion : CM_OFF);
             else contact_fp64<NM>(M, M.lp, a.mat, c, h, a.shearupdate, dF, dT);
             // the hysteretic models write their state even in set-up passes
             if (a.shearupdate || !known || NM != NM_HERTZ) {
@@ -132,7 +142,7 @@ __global__ void __launch_bounds__(kBlock) k_force(ForceArgs a)
                 for (int k = 0; k < HN; k++)
                     if (k < dnum) hp[k] = h[k];
             }
-            if (!known) a.nj[e] = w | kFlagBit;
+            if (!(w & kFlagBit)) a.nj[e] = w | kFlagBit;
             F[0] += dF[0]; F[1] += dF[1]; F[2] += dF[2];
             T[0] += dT[0]; T[1] += dT[1]; T[2] += dT[2];
             nc++;
@@ -141,6 +151,10 @@ __global__ void __launch_bounds__(kBlock) k_force(ForceArgs a)
             // its contact flag, so the entry no longer carries at a rebuild
             double *hp = a.hist + (size_t)e * dnum;
             if (!contact_close(M.lp, hp)) a.nj[e] = w & ~kFlagBit;
+            if (CH && bonded) {
+                F[0] += coh[0]; F[1] += coh[1]; F[2] += coh[2];
+                T[0] += coh[3]; T[1] += coh[4]; T[2] += coh[5];
+            }
         }
     }
     if (G > 1) {
@@ -268,6 +282,24 @@ static void launch_nm(const ForceArgs &a, hipStream_t s)
     k_force<MIXED, 4, NM, WN><<<nb, kBlock, 0, s>>>(a);
 }
 
+// with a cohesion model: FP64, 4 lanes per particle
+template <int NM, int WN>
+static void launch_ch(const ForceArgs &a, hipStream_t s)
+{
+    const int nb = padded_blocks((long long)a.n * 4);
+    k_force<false, 4, NM, WN, true><<<nb, kBlock, 0, s>>>(a);
+}
+
+template <int NM>
+static void launch_ch_wn(const ForceArgs &a, hipStream_t s)
+{
+    switch (a.M.lw.normal) {
+    case NM_HYST1: launch_ch<NM, NM_HYST1>(a, s); break;
+    case NM_HYST2: launch_ch<NM, NM_HYST2>(a, s); break;
+    default: launch_ch<NM, NM_HERTZ>(a, s); break;
+    }
+}
+
 template <bool MIXED, int NM>
 static void launch_wn(const ForceArgs &a, hipStream_t s)
 {
@@ -284,6 +316,14 @@ template <bool MIXED>
 static void launch_force(const ForceArgs &a, hipStream_t s)
 {
     if (a.n == 0) return;
+    if (a.M.cohesion != CM_OFF) {
+        switch (a.M.lp.normal) {
+        case NM_HYST1: launch_ch_wn<NM_HYST1>(a, s); break;
+        case NM_HYST2: launch_ch_wn<NM_HYST2>(a, s); break;
+        default: launch_ch_wn<NM_HERTZ>(a, s); break;
+        }
+        return;
+    }
     switch (a.M.lp.normal) {
     case NM_HYST1: launch_wn<false, NM_HYST1>(a, s); break;
     case NM_HYST2: launch_wn<false, NM_HYST2>(a, s); break;

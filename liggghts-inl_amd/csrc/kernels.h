@@ -19,8 +19,26 @@ struct ForceArgs {
     int n;
     int shearupdate;
     int defer_freeze;    // mesh walls follow: they apply fix freeze
+    const double *bout;  // cohesion force/torque per entry (6), when bonded
+    const uint8_t *bvalid;
     Model M;
     Planes P;
+};
+
+// Cohesion bond pass (one thread per particle, canonical entries).
+struct BondArgs {
+    const double4 *xr, *vm, *wt;
+    const int *tag;
+    const int *nstart;
+    uint32_t *nj;
+    double *hist;
+    const int *rev;
+    double *bout;
+    uint8_t *bvalid;
+    const double *mat;
+    int n, shearupdate;
+    long long ntimestep;
+    Model M;
 };
 
 struct BinArgs {
@@ -141,6 +159,9 @@ void launch_mesh_remap(const MeshRemapArgs &a, hipStream_t s);
 void launch_mesh_force(const MeshForceArgs &a, hipStream_t s);
 // MeshMoverLinear: node += dx, center += dx for ntri triangles
 void launch_mesh_move(TriGeo *tri, int ntri, double dx, double dy, double dz, hipStream_t s);
+
+void launch_bond(const BondArgs &a, hipStream_t s);
+void launch_bond_rev(const int *nstart, const uint32_t *nj, int *rev, int n, hipStream_t s);
 
 void launch_bin(const BinArgs &a, hipStream_t s);
 void launch_scan(int *data, int n, int *tmp, hipStream_t s);   // exclusive, data[n] = total

@@ -291,7 +291,7 @@ __global__ void __launch_bounds__(kBlock) k_neigh(NeighArgs a)
         oe = a.onstart[o + 1];
         if (lane < oe - ob) {
             const uint32_t ow = a.onj[ob + lane];
-            if (ow & kFlagBit) otag_l = a.otag[ow & kJMask];
+            if (ow & (kFlagBit | kBondBit)) otag_l = a.otag[ow & kJMask];
         }
     }
     const int on64 = min(oe - ob, 64);
@@ -341,7 +341,7 @@ __global__ void __launch_bounds__(kBlock) k_neigh(NeighArgs a)
             if (want && found < 0)
                 for (int oe2 = ob + 64; oe2 < oe; oe2++) {
                     const uint32_t ow = a.onj[oe2];
-                    if ((ow & kFlagBit) && a.otag[ow & kJMask] == tq) {
+                    if ((ow & (kFlagBit | kBondBit)) && a.otag[ow & kJMask] == tq) {
                         found = oe2 - ob;
                         break;
                     }
@@ -353,7 +353,7 @@ __global__ void __launch_bounds__(kBlock) k_neigh(NeighArgs a)
                     const double *oh = a.ohist + (size_t)(ob + found) * a.dnum;
                     double *h = a.hist + (size_t)e * a.dnum;
                     for (int k = 0; k < a.dnum; k++) h[k] = oh[k];
-                    w |= kFlagBit;
+                    w |= a.onj[ob + found] & (kFlagBit | kBondBit);
                 }
                 a.nj[e] = w;
             }

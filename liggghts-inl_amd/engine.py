@@ -44,7 +44,31 @@ class Params(C.Structure):
         ("normal_model", C.c_int),
         ("wall_normal_model", C.c_int),
         ("wall_limit_force", C.c_int),
+        ("cohesion_model", C.c_int),
     ]
+
+
+class BondSettings(C.Structure):
+    _fields_ = [
+        ("ts_create", C.c_double),
+        ("stress_break", C.c_int),
+        ("create_always", C.c_int),
+        ("damping", C.c_int),
+        ("damping_smooth", C.c_int),
+        ("ratio_tc", C.c_int),
+        ("tension_variant", C.c_int),
+    ]
+
+
+COHESION = {"off": 0, "bond": 1, "bond/nonlinear": 2}
+# bond tables in the order of DEM_NBOND (include/liggghts_dem.h); names of the
+# script properties, the bond/nonlinear "...Bondnonlinear" / "stiffnessPerUnitArea..."
+# spellings mapped onto the same slots
+BOND_KEYS = ("radiusMultiplier", "normalBondStiffnessPerUnitArea", "tangentialBondStiffnessPerUnitArea",
+             "dampingNormalForce", "dampingTangentialForce", "dampingNormalTorque", "dampingTangentialTorque",
+             "maxDistance", "maxSigma", "maxTau", "ratioTensionCompression", "createDistance",
+             "K_fn1", "Ku_fn1", "Kc_fn1", "K_fn2", "Ku_fn2", "Kc_fn2", "K_ft", "K_tn", "Ku_tn", "Kc_tn",
+             "K_tt", "Ku_tt", "Kc_tt")
 
 
 NORMAL = {"hertz": 0, "hysteretic/nonlinear1": 1, "hysteretic/nonlinear2": 2}
@@ -107,6 +131,7 @@ def library():
         "dem_mesh_topology": (C.c_int, [C.c_int, dp, C.c_double, dp, dp, ip, ip, ip, ip, ip]),
         "dem_get_mesh_contacts": (C.c_int, [vp, C.c_int, i64p, ip, ip, ip, dp]),
         "dem_set_mesh_contacts": (C.c_int, [vp, C.c_int, C.c_int64, ip, ip, ip, dp]),
+        "dem_set_bond": (C.c_int, [vp, dp, C.POINTER(BondSettings)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -193,6 +218,7 @@ class Simulation:
         p.limit_force = int(spec.get("limit_force", p.normal_model != 0))
         p.wall_limit_force = int(spec.get("wall_limit_force", p.wall_normal_model != 0))
         p.torsion_torque = int(spec.get("torsion_torque", False))
+        p.cohesion_model = COHESION[spec.get("cohesion", "off")]
         p.precision = DEM_MIXED if precision == "mixed" else DEM_FP64
         p.device = device
         self.h = self.L.dem_ctx_create(C.byref(p))
@@ -223,7 +249,10 @@ class Simulation:
             tabs = np.ascontiguousarray(np.stack([np.asarray(m["hysteretic"][k], dtype=np.float64).reshape(w, w)
                                                   for k in HYST_KEYS]))
             self._chk(self.L.dem_set_hysteretic(self.h, _dp(crl), _dp(tabs)))
+        self.ntypes = nt
         self.upload(atoms)
+        if "bond" in spec:
+            self.set_bond(**spec["bond"])
 
     def add_mesh(self, nodes, type, curvature=None, vel=None):
         nd = np.ascontiguousarray(np.asarray(nodes, dtype=np.float64).reshape(-1, 9))
@@ -254,6 +283,18 @@ class Simulation:
         sl = np.ascontiguousarray(slot, dtype=np.int32)
         h = np.ascontiguousarray(hist, dtype=np.float64).reshape(len(tg), self.wall_dnum)
         self._chk(self.L.dem_set_mesh_contacts(self.h, int(k), len(tg), _ip(tg), _ip(tr), _ip(sl), _dp(h)))
+
+    def set_bond(self, tables, ts_create=0.0, stress_break=False, create_always=False, damping=True,
+                 damping_smooth=False, ratio_tc=False, tension_variant=False):
+        """Bond tables {BOND_KEYS name: (ntypes+1)^2} and settings."""
+        w = self.ntypes + 1
+        tabs = np.zeros((len(BOND_KEYS), w, w))
+        for k, v in tables.items():
+            tabs[BOND_KEYS.index(k)] = np.asarray(v, dtype=np.float64).reshape(w, w)
+        tabs = np.ascontiguousarray(tabs)
+        b = BondSettings(float(ts_create), int(stress_break), int(create_always), int(damping),
+                         int(damping_smooth), int(ratio_tc), int(tension_variant))
+        self._chk(self.L.dem_set_bond(self.h, _dp(tabs), C.byref(b)))
 
     def set_gravity(self, ax, ay, az):
         self._chk(self.L.dem_set_gravity(self.h, float(ax), float(ay), float(az)))

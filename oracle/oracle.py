@@ -151,6 +151,7 @@ class Oracle:
 
     def __init__(self, spec, atoms):
         self.L = lib()
+        self.spec = spec
         p = _Params()
         p.dt = spec["dt"]
         for d in range(3):

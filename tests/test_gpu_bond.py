@@ -1,0 +1,110 @@
+"""Bonded particles (cohesion bond, bond/nonlinear) on the MI355X against
+the oracle (FP64 engine, MI355X only).
+
+* a 12-sphere random chain bonded at step 1 and advanced 300 steps on the
+  oracle, then handed to the device with its pair history (bond slots
+  included): forces and torques within 1e-12 of each atom's force scale,
+  the same directed pair set with the same history;
+* the same chain from step 0 on both: bonds created on the device at
+  tsCreateBond, 400 steps of trajectory and history;
+* a tight maxDistanceBond so that bonds break during the horizon: the
+  device breaks the same bonds (pair set and history agree after the run).
+"""
+import numpy as np
+import pytest
+
+from conftest import load_engine
+from oracle import Oracle
+from test_gpu_parity import check_forces, check_pairs, sim_from_oracle
+from test_oracle_bond import bond_tables, chain_cluster
+
+pytestmark = pytest.mark.gpu
+
+
+def nonlinear_tables():
+    full = lambda v: np.full((2, 2), v)  # noqa: E731
+    return dict(radiusMultiplier=full(0.8), createDistance=full(2.3e-3), maxDistance=full(2.4e-3),
+                maxSigma=full(1e12), maxTau=full(1e12), K_fn1=full(3e4), Ku_fn1=full(6e4), Kc_fn1=full(1.5e4),
+                K_fn2=full(2e10), Ku_fn2=full(4e10), Kc_fn2=full(1e10), K_ft=full(5e9), K_tn=full(5e9),
+                Ku_tn=full(1e10), Kc_tn=full(2.5e9), K_tt=full(5e9), Ku_tt=full(1e10), Kc_tt=full(2.5e9),
+                dampingNormalForce=full(0.05), dampingTangentialForce=full(0.05),
+                dampingNormalTorque=full(0.05), dampingTangentialTorque=full(0.05))
+
+
+def tables(cohesion, **over):
+    if cohesion == "bond":
+        t = bond_tables(dampingNormalForce=0.1, dampingTangentialForce=0.1, dampingNormalTorque=0.1,
+                        dampingTangentialTorque=0.1)
+    else:
+        t = nonlinear_tables()
+    for k, v in over.items():
+        t[k] = np.full((2, 2), v)
+    return t
+
+
+def nbonded(hist):
+    return int((hist[:, 0] > 0.5).sum()) if len(hist) else 0
+
+
+@pytest.mark.parametrize("cohesion", ["bond", "bond/nonlinear"])
+def test_bond_identical_state(cohesion):
+    o, atoms = chain_cluster(cohesion, tables(cohesion))
+    spec = o.spec
+    o.setup()
+    o.run(300)
+    assert nbonded(o.pairs()[3]) >= 16
+    # the device starts at step 0: move tsCreateBond out of reach so that
+    # no new bond forms on either side (the oracle's window was step 1)
+    dspec = dict(spec, bond=dict(spec["bond"], ts_create=1e8))
+    s = sim_from_oracle(dspec, atoms, o)
+    check_pairs(s, o)
+    check_forces(s, o, 1e-12)
+    s.step(50)
+    o.run(50)
+    g, r = s.state(), o.state()
+    assert np.abs(g["x"] - r["x"]).max() <= 1e-12 * 0.1
+    assert np.abs(g["v"] - r["v"]).max() <= 1e-9 * np.abs(r["v"]).max()
+    check_forces(s, o, 1e-9)
+    check_pairs(s, o, hist_tol=1e-9)
+
+
+@pytest.mark.parametrize("cohesion", ["bond", "bond/nonlinear"])
+def test_bond_creation_and_horizon(cohesion):
+    o, atoms = chain_cluster(cohesion, tables(cohesion))
+    eng = load_engine()
+    s = eng.Simulation(o.spec, atoms)
+    s.setup()
+    o.setup()
+    s.step(400)
+    o.run(400)
+    assert nbonded(o.pairs()[3]) >= 16
+    g, r = s.state(), o.state()
+    assert np.abs(g["x"] - r["x"]).max() <= 1e-12 * 0.1
+    assert np.abs(g["v"] - r["v"]).max() <= 1e-9 * np.abs(r["v"]).max()
+    assert np.abs(g["omega"] - r["omega"]).max() <= 1e-9 * np.abs(r["omega"]).max()
+    check_forces(s, o, 1e-9)
+    check_pairs(s, o, hist_tol=1e-9)
+
+
+@pytest.mark.parametrize("cohesion", ["bond", "bond/nonlinear"])
+def test_bond_break(cohesion):
+    """maxDistanceBond just above the creation distances: the random
+    velocities stretch some bonds past it within the horizon."""
+    o, atoms = chain_cluster(cohesion, tables(cohesion, maxDistance=2.0405e-3), seed=3)
+    atoms = dict(atoms, v=atoms["v"] * 20.0)
+    o = Oracle(o.spec, atoms)
+    eng = load_engine()
+    s = eng.Simulation(o.spec, atoms)
+    s.setup()
+    o.setup()
+    s.step(1)
+    o.run(1)
+    n0 = nbonded(o.pairs()[3])
+    s.step(600)
+    o.run(600)
+    n1 = nbonded(o.pairs()[3])
+    assert n0 >= 16 and n1 < n0, (n0, n1)
+    assert nbonded(s.pairs()[3]) == n1
+    g, r = s.state(), o.state()
+    assert np.abs(g["x"] - r["x"]).max() <= 1e-11 * 0.1
+    check_pairs(s, o, hist_tol=1e-8)
