@@ -224,6 +224,39 @@ int dem_list_stats(dem_ctx *ctx, double *mean_neigh, double *mean_contacts, int6
 int dem_sync(dem_ctx *ctx);
 int64_t dem_timestep(const dem_ctx *ctx);
 
+/* ---- Batched runs.  No counterpart in the reference, which runs one
+ * simulation per process (one LAMMPS instance, lammps.cpp); these let many
+ * independent simulations of the same box, walls and contact model share one
+ * context, each kernel launch covering all of them. ---- */
+/* Assigns every uploaded atom (upload order) to an instance 0..ninst-1.
+ * Call after dem_upload_state and before dem_setup.  Particles of different
+ * instances never interact; each instance may have its own materials. */
+int dem_set_instances(dem_ctx *ctx, int ninst, const int *inst);
+/* Yeff, Geff, betaeff, mu (and mur, etar, may be NULL) of one instance,
+ * shapes as in dem_set_materials; the instance's other tables stay the
+ * common ones. */
+int dem_set_instance_materials(dem_ctx *ctx, int inst, const double *Yeff, const double *Geff,
+                               const double *betaeff, const double *mu, const double *mur,
+                               const double *etar);
+/* Record, in every force pass, each atom's force from primitive plane
+ * `plane` (index in dem_add_plane order) along the plane's axis; -1 off.
+ * Call after dem_upload_state. */
+int dem_observe_plane(dem_ctx *ctx, int plane);
+/* Per instance b, out[5b..5b+4] = atom count, sum of z, sum of the
+ * observed plane's force on the atoms (the wall's normal force; 0 if no
+ * plane is observed), total mass, kinetic energy, at the current state and
+ * last force pass. */
+int dem_instance_observables(dem_ctx *ctx, double *out);
+/* Step with a recorded HIP graph of two rebuild periods when every rebuild
+ * falls on a fixed step (neigh_modify check no, no mesh walls, no bonds,
+ * timing off); other steps run as usual.  The results are bitwise those of
+ * stepping without the graph. */
+int dem_use_graph(dem_ctx *ctx, int on);
+/* Autotuner: times the contact pass on the current state for every lane
+ * group width (1..16 lanes per particle) and block size (64..256) and keeps
+ * the fastest for this context.  State and history are left unchanged. */
+int dem_autotune(dem_ctx *ctx, int reps, int *lanes, int *block, double *ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -283,7 +283,7 @@ __global__ void __launch_bounds__(kBlock) k_bond(BondArgs a)
     const double4 vmi = a.vm[i];
     const double4 wi = a.wt[i];
     const int tagi = a.tag[i];
-    const int it = ((int)wi.w) & 15;
+    const int it = meta_type((int)wi.w);
     const double cdm = M.cdf * M.cdf;
     const int dnum = M.dnum;
     const int off = M.lp.off_coh;
@@ -322,7 +322,7 @@ __global__ void __launch_bounds__(kBlock) k_bond(BondArgs a)
         double *hp = a.hist + (size_t)e * dnum;
         double out[9];
         bool created = false;
-        const int res = bond_eval(M, a.mat, it, ((int)wj.w) & 15, hp + off, delta, rsq, pi.w, pj.w, xi, vi, vj, omi, omj,
+        const int res = bond_eval(M, a.mat, it, meta_type((int)wj.w), hp + off, delta, rsq, pi.w, pj.w, xi, vi, vj, omi, omj,
                                   a.shearupdate != 0, a.ntimestep, created, out);
         if (created) {
             // an entry that carried no flag holds no valid contact history

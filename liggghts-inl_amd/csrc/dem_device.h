@@ -3,7 +3,7 @@
 // HBM layout (all arrays in Morton cell order after each rebuild):
 //   xr  double4  (x, y, z, radius)           32 B / particle
 //   vm  double4  (vx, vy, vz, rmass)         32 B
-//   wt  double4  (wx, wy, wz, meta)          32 B   meta = type + 16*freeze
+//   wt  double4  (wx, wy, wz, meta)          32 B   meta = type | freeze<<4 | instance<<5
 //   f4  double4  (fx, fy, fz, -)             32 B
 //   t4  double4  (tx, ty, tz, -)             32 B
 //   tag int, orig int (upload position), img int4 (periodic image counts)
@@ -44,6 +44,14 @@ enum {
 };
 constexpr int kNHyst = 11;
 constexpr int kNBond = 25;
+// one instance of a batched run owns one block of material tables
+constexpr int kMatStride = M_COUNT * kT1 * kT1;
+
+// wt.w packs the atom type (bits 0-3), fix freeze membership (bit 4) and the
+// instance of a batched run (bits 5 and up; 0 for a single run)
+__host__ __device__ inline int meta_type(int m) { return m & 15; }
+__host__ __device__ inline bool meta_frozen(int m) { return ((m >> 4) & 1) != 0; }
+__host__ __device__ inline int meta_inst(int m) { return m >> 5; }
 
 // cohesion model of the pair chain
 enum { CM_OFF = 0, CM_BOND = 1, CM_BOND_NONLINEAR = 2 };
@@ -121,7 +129,8 @@ struct Grid {
     double inv[3];       // cells per unit length
     int nc[3];
     int periodic[3];
-    int nkeys;           // size of the Morton key space
+    int nkeys;           // size of the key space (all instances)
+    int nkeys_local;     // Morton keys of one instance; key = instance*nkeys_local + morton
     const uint32_t *spread;  // spread[d*maxc + c] = interleaved bits of cell c in dim d
     int maxc;
 };

@@ -70,6 +70,28 @@ struct dem_ctx {
     int ntypes = 0;
     double *d_mat = nullptr;
     bool mat_set = false;
+    std::vector<double> hmat;          // host copy of the material tables (one block)
+    int mat_blocks = 1;                // blocks allocated in d_mat
+
+    // batched run: instances, their material overrides, observed plane
+    int ninst = 1;
+    std::vector<double> inst_mat;      // per instance: slots M_YEFF..M_ETAR
+    std::vector<uint8_t> inst_set;
+    int obs_plane = -1;
+    double *pwf = nullptr;
+    double *d_obs = nullptr;
+    int obs_cap = 0;
+
+    // launch shape of the contact pass (0: defaults), set by dem_autotune
+    int tune_lanes = 0, tune_block = 0;
+
+    // graph-captured stepping: one graph covers two rebuild periods, so the
+    // double buffers are back where they started at its end
+    bool use_graph = false;
+    bool gexec_capturing = false;
+    hipGraphExec_t gexec = nullptr;
+    int g_period = 0;
+    std::vector<const void *> g_key;   // buffers the graph was recorded with
 
     int n = 0, cap = 0;
     int cur = 0;
@@ -284,6 +306,7 @@ extern "C" dem_ctx *dem_ctx_create(const dem_params *p)
         return nullptr;
     }
     hipMemset(ctx->d_small, 0, sizeof(int) * 8);
+    ctx->hmat.assign(kMatStride, 0.0);
     Model &M = ctx->M;
     M.dt = p->dt;
     M.cdf = p->cdf;
@@ -315,6 +338,8 @@ static void free_all(dem_ctx *ctx)
     F(ctx->sorted); F(ctx->cstart); F(ctx->cursor); F(ctx->scan_tmp); F(ctx->spread);
     F(ctx->d_mat); F(ctx->d_small);
     F(ctx->bout); F(ctx->bvalid); F(ctx->brev);
+    F(ctx->pwf); F(ctx->d_obs);
+    if (ctx->gexec) hipGraphExecDestroy(ctx->gexec);
     for (auto &m : ctx->meshes) {
         F(m.tri); F(m.enorm); F(m.gstart); F(m.glist);
         for (int b = 0; b < 2; b++) {
@@ -355,6 +380,27 @@ extern "C" int dem_material_tables(int ntypes, const double *Y, const double *nu
     return DEM_OK;
 }
 
+// Material tables to the device: one block per instance, each the common
+// tables with the instance's own Yeff..etar when it has them.
+static int upload_mat(dem_ctx *ctx)
+{
+    const size_t S = kMatStride, T = (size_t)kT1 * kT1;
+    const int B = ctx->ninst;
+    std::vector<double> all((size_t)B * S);
+    for (int b = 0; b < B; b++) {
+        std::copy(ctx->hmat.begin(), ctx->hmat.end(), all.begin() + (size_t)b * S);
+        if (!ctx->inst_set.empty() && ctx->inst_set[b])
+            std::copy(ctx->inst_mat.begin() + (size_t)b * (M_ETAR + 1) * T,
+                      ctx->inst_mat.begin() + (size_t)(b + 1) * (M_ETAR + 1) * T, all.begin() + (size_t)b * S);
+    }
+    if (B > ctx->mat_blocks) {
+        if (dev_alloc(ctx, &ctx->d_mat, all.size())) return DEM_ERR_HIP;
+        ctx->mat_blocks = B;
+    }
+    HIPCHK(hipMemcpy(ctx->d_mat, all.data(), all.size() * sizeof(double), hipMemcpyHostToDevice));
+    return DEM_OK;
+}
+
 extern "C" int dem_set_materials(dem_ctx *ctx, int ntypes, const double *Yeff, const double *Geff,
                                  const double *betaeff, const double *mu, const double *mur,
                                  const double *etar, double krs)
@@ -366,20 +412,16 @@ extern "C" int dem_set_materials(dem_ctx *ctx, int ntypes, const double *Yeff, c
     if ((ctx->p.rolling_model == 1 || ctx->p.rolling_model == 3) && !etar)
         return fail(ctx, DEM_ERR_ARG, "rolling model needs coeffRollVisc");
     hipSetDevice(ctx->p.device);
-    std::vector<double> h(M_COUNT * kT1 * kT1, 0.0);
+    // slots Yeff..etar; the hysteretic and bond tables set before are kept
+    std::vector<double> &h = ctx->hmat;
     const int w = ntypes + 1;
-    const double *src[M_COUNT] = {Yeff, Geff, betaeff, mu, mur, etar};
-    for (int s = 0; s < M_COUNT; s++) {
-        if (!src[s]) continue;
-        for (int i = 0; i < w; i++)
-            for (int j = 0; j < w; j++) h[s * kT1 * kT1 + i * kT1 + j] = src[s][i * w + j];
-    }
-    // keep the hysteretic tables if they were set before
-    std::vector<double> old(M_COUNT * kT1 * kT1, 0.0);
-    if (ctx->mat_set)
-        HIPCHK(hipMemcpy(old.data(), ctx->d_mat, old.size() * sizeof(double), hipMemcpyDeviceToHost));
-    for (size_t k = (size_t)M_RESTLOG * kT1 * kT1; k < h.size(); k++) h[k] = old[k];
-    HIPCHK(hipMemcpy(ctx->d_mat, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+    const double *src[M_ETAR + 1] = {Yeff, Geff, betaeff, mu, mur, etar};
+    for (int s = 0; s <= M_ETAR; s++)
+        for (int i = 0; i < kT1; i++)
+            for (int j = 0; j < kT1; j++)
+                h[s * kT1 * kT1 + i * kT1 + j] = (src[s] && i < w && j < w) ? src[s][i * w + j] : 0.0;
+    const int rc = upload_mat(ctx);
+    if (rc) return rc;
     ctx->ntypes = ntypes;
     ctx->M.krs = krs;
     ctx->mat_set = true;
@@ -393,16 +435,14 @@ extern "C" int dem_set_hysteretic(dem_ctx *ctx, const double *coeff_rest_log, co
         return fail(ctx, DEM_ERR_ARG, "dem_set_hysteretic: set materials first, non-null tables");
     hipSetDevice(ctx->p.device);
     const int w = ctx->ntypes + 1;
-    std::vector<double> h(M_COUNT * kT1 * kT1, 0.0);
-    HIPCHK(hipMemcpy(h.data(), ctx->d_mat, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+    std::vector<double> &h = ctx->hmat;
     for (int i = 0; i < w; i++)
         for (int j = 0; j < w; j++) {
             h[M_RESTLOG * kT1 * kT1 + i * kT1 + j] = coeff_rest_log[i * w + j];
             for (int t = 0; t < kNHyst; t++)
                 h[(M_HK + t) * kT1 * kT1 + i * kT1 + j] = tables[(size_t)t * w * w + i * w + j];
         }
-    HIPCHK(hipMemcpy(ctx->d_mat, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
-    return DEM_OK;
+    return upload_mat(ctx);
 }
 
 extern "C" int dem_add_plane(dem_ctx *ctx, int dim, double pos, int wtype)
@@ -512,15 +552,18 @@ extern "C" int dem_set_bond(dem_ctx *ctx, const double *tables, const dem_bond_s
     if (!tables || !bs || ctx->M.cohesion == CM_OFF || !ctx->mat_set || ctx->n == 0)
         return fail(ctx, DEM_ERR_ARG,
                     "dem_set_bond: needs a cohesion model, materials and uploaded atoms, non-null tables");
+    if (ctx->ninst > 1) return fail(ctx, DEM_ERR_ARG, "dem_set_bond: bonds are not available in a batched run");
     hipSetDevice(ctx->p.device);
     const int w = ctx->ntypes + 1;
-    std::vector<double> h(M_COUNT * kT1 * kT1, 0.0);
-    HIPCHK(hipMemcpy(h.data(), ctx->d_mat, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+    std::vector<double> &h = ctx->hmat;
     for (int q = 0; q < kNBond; q++)
         for (int i = 0; i < w; i++)
             for (int j = 0; j < w; j++)
                 h[(size_t)(M_BLAMBDA + q) * kT1 * kT1 + i * kT1 + j] = tables[(size_t)q * w * w + i * w + j];
-    HIPCHK(hipMemcpy(ctx->d_mat, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+    {
+        const int rc = upload_mat(ctx);
+        if (rc) return rc;
+    }
     ctx->M.bs = BondSet{(long long)(int)bs->ts_create, bs->stress_break, bs->create_always, bs->damping,
                         bs->damping_smooth, bs->ratio_tc, bs->tension_variant};
     // the model's contact distance factor (cohesion_model_bond.h:415-470,
@@ -639,7 +682,10 @@ static int setup_grid(dem_ctx *ctx)
     HIPCHK(hipMemcpy(ctx->spread, h.data(), h.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
     G.spread = ctx->spread;
     G.maxc = maxc;
-    G.nkeys = 1 << total;
+    G.nkeys_local = 1 << total;
+    if ((int64_t)G.nkeys_local * ctx->ninst >= (int64_t)1 << 30)
+        return fail(ctx, DEM_ERR_ARG, "cell grid too large for this many instances");
+    G.nkeys = G.nkeys_local * ctx->ninst;
     if (G.nkeys + 1 > ctx->nkeys_cap) {
         if (dev_alloc(ctx, &ctx->cstart, G.nkeys + 1) || dev_alloc(ctx, &ctx->cursor, G.nkeys + 1))
             return DEM_ERR_HIP;
@@ -812,16 +858,20 @@ static int mesh_rebuild(dem_ctx *ctx, MeshDev &m)
 static int setup_mesh_grids(dem_ctx *ctx);
 
 // Neighbour rebuild: pre_exchange + pbc + sort + list + wall lists.
-static int rebuild(dem_ctx *ctx)
+// One neighbour rebuild.  With captured = true it is being recorded into a
+// graph: no host round trip, the list buffers were sized beforehand and a
+// longer list raises error bit 3 instead of being written.
+static int rebuild(dem_ctx *ctx, bool captured = false)
 {
     const int n = ctx->n;
     hipStream_t s = ctx->stream;
-    timer_begin(ctx, ctx->t_rebuild);
+    if (!captured) timer_begin(ctx, ctx->t_rebuild);
     const Grid &G = ctx->G;
     const int c0 = ctx->cur, c1 = 1 - ctx->cur;
-    HIPCHK(hipMemsetAsync(ctx->cstart, 0, sizeof(int) * (G.nkeys + 1), s));
+    launch_fill_int(ctx->cstart, 0, G.nkeys + 1, s);
     BinArgs ba{};
     ba.xr = ctx->xr[c0];
+    ba.wt = ctx->wt[c0];
     ba.img = ctx->img[c0];
     ba.key = ctx->key;
     ba.count = ctx->cstart;
@@ -835,7 +885,7 @@ static int rebuild(dem_ctx *ctx)
     KCHK();
     launch_scan(ctx->cstart, G.nkeys + 1, ctx->scan_tmp, s);
     KCHK();
-    HIPCHK(hipMemcpyAsync(ctx->cursor, ctx->cstart, sizeof(int) * (G.nkeys + 1), hipMemcpyDeviceToDevice, s));
+    launch_copy_int(ctx->cursor, ctx->cstart, G.nkeys + 1, s);
     launch_scatter(ctx->key, ctx->cursor, ctx->sorted, n, s);
     launch_cellsort(ctx->cstart, ctx->sorted, G.nkeys, s);
     KCHK();
@@ -860,6 +910,7 @@ static int rebuild(dem_ctx *ctx)
     const int l0 = ctx->lcur, l1 = 1 - ctx->lcur;
     NeighArgs na{};
     na.xr = ctx->xr[c1];
+    na.wt = ctx->wt[c1];
     na.tag = ctx->tag[c1];
     na.cstart = ctx->cstart;
     na.count = ctx->nstart[l1];
@@ -870,19 +921,23 @@ static int rebuild(dem_ctx *ctx)
     na.skin = ctx->p.skin;
     na.G = G;
     for (int d = 0; d < 3; d++) na.prd[d] = ctx->M.prd[d];
-    HIPCHK(hipMemsetAsync(ctx->d_small + 1, 0, sizeof(int), s));
-    HIPCHK(hipMemsetAsync(ctx->nstart[l1] + n, 0, sizeof(int), s));
+    launch_fill_int(ctx->d_small + 1, 0, 1, s);
+    launch_fill_int(ctx->nstart[l1] + n, 0, 1, s);
     launch_neigh_count(na, s);
     KCHK();
     launch_scan(ctx->nstart[l1], n + 1, ctx->scan_tmp, s);
     KCHK();
     int hsmall[2] = {0, 0};
-    HIPCHK(hipMemcpyAsync(&hsmall[0], ctx->nstart[l1] + n, sizeof(int), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipMemcpyAsync(&hsmall[1], ctx->d_small + 1, sizeof(int), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
-    if (hsmall[1] > ctx->p.neigh_one)
-        return fail(ctx, DEM_ERR_OVERFLOW, "Neighbor list overflow, boost neigh_modify one");
-    if (ensure_list_cap(ctx, l1, hsmall[0])) return DEM_ERR_NOMEM;
+    if (!captured) {
+        HIPCHK(hipMemcpyAsync(&hsmall[0], ctx->nstart[l1] + n, sizeof(int), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(&hsmall[1], ctx->d_small + 1, sizeof(int), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        if (hsmall[1] > ctx->p.neigh_one)
+            return fail(ctx, DEM_ERR_OVERFLOW, "Neighbor list overflow, boost neigh_modify one");
+        if (ensure_list_cap(ctx, l1, hsmall[0])) return DEM_ERR_NOMEM;
+    }
+    na.ecap = ctx->ecap[l1];
+    na.err = ctx->d_small;
     na.nstart = ctx->nstart[l1];
     na.nj = ctx->nj[l1];
     na.hist = ctx->hist[l1];
@@ -894,6 +949,7 @@ static int rebuild(dem_ctx *ctx)
     na.otag = ctx->tag[c0];
     launch_neigh_fill(na, s);
     KCHK();
+    if (captured) launch_row_max_check(ctx->d_small + 1, ctx->p.neigh_one, ctx->d_small, s);
     ctx->lcur = l1;
     ctx->have_list = true;
     ctx->list_valid = true;
@@ -918,7 +974,7 @@ static int rebuild(dem_ctx *ctx)
         launch_bond_rev(ctx->nstart[l1], ctx->nj[l1], ctx->brev, n, s);
         KCHK();
     }
-    timer_end(ctx, ctx->t_rebuild);
+    if (!captured) timer_end(ctx, ctx->t_rebuild);
     return DEM_OK;
 }
 
@@ -941,6 +997,10 @@ static int force_pass(dem_ctx *ctx, int shearupdate)
     fa.M = ctx->M;
     fa.P = ctx->P;
     fa.defer_freeze = ctx->meshes.empty() ? 0 : 1;
+    fa.pwf = ctx->obs_plane >= 0 ? ctx->pwf : nullptr;
+    fa.obs_plane = ctx->obs_plane;
+    fa.lanes = ctx->tune_lanes;
+    fa.block = ctx->tune_block;
     if (ctx->M.cohesion != CM_OFF) {
         BondArgs ba{};
         ba.xr = fa.xr; ba.vm = fa.vm; ba.wt = fa.wt;
@@ -961,10 +1021,10 @@ static int force_pass(dem_ctx *ctx, int shearupdate)
         fa.bout = ctx->bout;
         fa.bvalid = ctx->bvalid;
     }
-    timer_begin(ctx, ctx->t_contact);
+    if (!ctx->gexec_capturing) timer_begin(ctx, ctx->t_contact);
     if (ctx->p.precision == DEM_MIXED) launch_force_mixed(fa, ctx->stream);
     else launch_force_fp64(fa, ctx->stream);
-    timer_end(ctx, ctx->t_contact);
+    if (!ctx->gexec_capturing) timer_end(ctx, ctx->t_contact);
     KCHK();
     for (size_t k = 0; k < ctx->meshes.size(); k++) {
         MeshDev &m = ctx->meshes[k];
@@ -1061,9 +1121,9 @@ static int integrate(dem_ctx *ctx, int mode)
     ia.dtf = 0.5 * ctx->p.dt;   // dtf = 0.5*dt*ftm2v (fix_nve.cpp:85), ftm2v = 1 (si)
     ia.dtv = ctx->p.dt;
     ia.dtfrotate = ia.dtf / 0.4;
-    timer_begin(ctx, ctx->t_integrate);
+    if (!ctx->gexec_capturing) timer_begin(ctx, ctx->t_integrate);
     launch_integrate(ia, mode, ctx->stream);
-    timer_end(ctx, ctx->t_integrate);
+    if (!ctx->gexec_capturing) timer_end(ctx, ctx->t_integrate);
     KCHK();
     return DEM_OK;
 }
@@ -1077,6 +1137,10 @@ static int check_err_flag(dem_ctx *ctx)
         return fail(ctx, DEM_ERR_STATE,
                     "ilegal situation, particle positions identical. Most probable reason: Simulation diverged");
     if (h & 4) return fail(ctx, DEM_ERR_OVERFLOW, "mesh contact history: no free contact slot (internal error)");
+    if (h & 8)
+        return fail(ctx, DEM_ERR_OVERFLOW,
+                    "neighbour list longer than the buffers sized for graph stepping (step without dem_use_graph)");
+    if (h & 16) return fail(ctx, DEM_ERR_OVERFLOW, "Neighbor list overflow, boost neigh_modify one");
     return DEM_OK;
 }
 
@@ -1136,6 +1200,97 @@ static int decide(dem_ctx *ctx, int *out)
     return DEM_OK;
 }
 
+// Graph stepping applies when every rebuild falls on a fixed step: no
+// distance check, delay within one period, nothing that needs the host
+// between steps (meshes, bonds, timers).
+static bool graph_ok(const dem_ctx *ctx)
+{
+    return ctx->use_graph && !ctx->p.neigh_check && ctx->p.neigh_delay <= ctx->p.neigh_every
+           && ctx->meshes.empty() && ctx->M.cohesion == CM_OFF && !ctx->timing && ctx->n > 0;
+}
+
+static std::vector<const void *> graph_key(const dem_ctx *ctx)
+{
+    std::vector<const void *> k = {ctx->xr[0], ctx->xr[1], ctx->vm[0], ctx->vm[1], ctx->wt[0], ctx->wt[1],
+                                   ctx->tag[0], ctx->tag[1], ctx->orig[0], ctx->orig[1], ctx->img[0],
+                                   ctx->img[1], ctx->whist[0], ctx->whist[1], ctx->nstart[0], ctx->nstart[1],
+                                   ctx->nj[0], ctx->nj[1], ctx->hist[0], ctx->hist[1], ctx->cstart,
+                                   ctx->cursor, ctx->scan_tmp, ctx->spread, ctx->d_mat, ctx->pwf};
+    k.push_back((const void *)(intptr_t)(ctx->cur + 2 * ctx->lcur));
+    k.push_back((const void *)(intptr_t)ctx->n);
+    k.push_back((const void *)(intptr_t)(ctx->tune_lanes * 4096 + ctx->tune_block));
+    k.push_back((const void *)(intptr_t)ctx->obs_plane);
+    return k;
+}
+
+// Record 2*neigh_every steps, each: [rebuild on the period's last step]
+// force pass, final_integrate(n) + initial_integrate(n+1).  The list
+// buffers are sized first for twice the current list (and at least 32
+// entries per particle); a longer list at replay raises an error.
+static int capture_graph(dem_ctx *ctx)
+{
+    if (ctx->gexec) {
+        HIPCHK(hipGraphExecDestroy(ctx->gexec));
+        ctx->gexec = nullptr;
+    }
+    int64_t total = 0;
+    {
+        int h = 0;
+        HIPCHK(hipMemcpy(&h, ctx->nstart[ctx->lcur] + ctx->n, sizeof(int), hipMemcpyDeviceToHost));
+        total = h;
+    }
+    const int64_t want = std::max<int64_t>(2 * total, (int64_t)ctx->n * 32);
+    for (int b = 0; b < 2; b++) {
+        if (ctx->ecap[b] >= want) continue;
+        if (b == ctx->lcur) {
+            // the current list: grow without losing it
+            uint32_t *nj = nullptr;
+            double *hi = nullptr;
+            const size_t dn = ctx->M.dnum;
+            HIPCHK(hipMalloc((void **)&nj, sizeof(uint32_t) * want));
+            HIPCHK(hipMalloc((void **)&hi, sizeof(double) * want * dn));
+            HIPCHK(hipMemcpy(nj, ctx->nj[b], sizeof(uint32_t) * total, hipMemcpyDeviceToDevice));
+            HIPCHK(hipMemcpy(hi, ctx->hist[b], sizeof(double) * total * dn, hipMemcpyDeviceToDevice));
+            HIPCHK(hipFree(ctx->nj[b]));
+            HIPCHK(hipFree(ctx->hist[b]));
+            ctx->nj[b] = nj;
+            ctx->hist[b] = hi;
+            ctx->ecap[b] = want;
+        } else if (ensure_list_cap(ctx, b, want)) {
+            return DEM_ERR_NOMEM;
+        }
+    }
+    const int every = ctx->p.neigh_every;
+    const int cur0 = ctx->cur, lcur0 = ctx->lcur;
+    hipGraph_t graph = nullptr;
+    HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    ctx->gexec_capturing = true;
+    int rc = DEM_OK;
+    for (int k = 1; k <= 2 * every && rc == DEM_OK; k++) {
+        if (k % every == 0) rc = rebuild(ctx, true);
+        if (rc == DEM_OK) rc = force_pass(ctx, 1);
+        if (rc == DEM_OK) rc = integrate(ctx, 3);
+    }
+    ctx->gexec_capturing = false;
+    const hipError_t ec = hipStreamEndCapture(ctx->stream, &graph);
+    ctx->cur = cur0;
+    ctx->lcur = lcur0;
+    if (rc) {
+        if (graph) hipGraphDestroy(graph);
+        return rc;
+    }
+    if (ec != hipSuccess) return fail(ctx, DEM_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(ec));
+    const hipError_t ei = hipGraphInstantiate(&ctx->gexec, graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    if (ei != hipSuccess) {
+        ctx->gexec = nullptr;
+        return fail(ctx, DEM_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(ei));
+    }
+    ctx->g_period = 2 * every;
+    ctx->g_key = graph_key(ctx);
+    return DEM_OK;
+}
+
 extern "C" int dem_step(dem_ctx *ctx, int nsteps)
 {
     if (!ctx) return fail(nullptr, DEM_ERR_ARG, "null context");
@@ -1143,7 +1298,23 @@ extern "C" int dem_step(dem_ctx *ctx, int nsteps)
     if (nsteps <= 0) return DEM_OK;
     hipSetDevice(ctx->p.device);
     int rc;
+    const bool gok = graph_ok(ctx);
     for (int st = 0; st < nsteps; st++) {
+        // whole graph periods while they fit before the last step, starting
+        // right after a rebuild
+        if (gok && st >= 1 && ctx->ago == 0 && ctx->ntimestep + 1 != ctx->next_reneighbor
+            && st + 2 * ctx->p.neigh_every <= nsteps - 1) {
+            if (!ctx->gexec || ctx->g_key != graph_key(ctx)) {
+                if ((rc = capture_graph(ctx))) return rc;
+            }
+            while (st + ctx->g_period <= nsteps - 1) {
+                HIPCHK(hipGraphLaunch(ctx->gexec, ctx->stream));
+                ctx->ntimestep += ctx->g_period;
+                st += ctx->g_period;
+            }
+            if ((rc = check_err_flag(ctx))) return rc;
+            if (st >= nsteps) break;
+        }
         ctx->ntimestep++;
         if (st == 0 && ((rc = integrate(ctx, 1)) || (rc = move_meshes(ctx)))) return rc;
         int nflag = 0;
@@ -1514,4 +1685,185 @@ extern "C" int dem_sync(dem_ctx *ctx)
 extern "C" int64_t dem_timestep(const dem_ctx *ctx)
 {
     return ctx ? ctx->ntimestep : 0;
+}
+
+// ---------------------------------------------------------------------------
+// Batched runs: independent instances in one context.  Every kernel launch
+// covers all instances; the key space gives each instance its own cells, so
+// no neighbour search crosses instances, and the sort leaves each instance's
+// particles contiguous.
+
+extern "C" int dem_set_instances(dem_ctx *ctx, int ninst, const int *inst)
+{
+    if (!ctx) return fail(nullptr, DEM_ERR_ARG, "null context");
+    if (ninst < 1 || (ninst > 1 && !inst) || ninst > (1 << 20))
+        return fail(ctx, DEM_ERR_ARG, "dem_set_instances: bad arguments");
+    if (ctx->n == 0 || ctx->have_list)
+        return fail(ctx, DEM_ERR_ARG, "dem_set_instances: call after dem_upload_state and before dem_setup");
+    if (ninst > 1 && (!ctx->meshes.empty() || ctx->M.cohesion != CM_OFF))
+        return fail(ctx, DEM_ERR_ARG, "dem_set_instances: mesh walls and bonds are not available in a batched run");
+    hipSetDevice(ctx->p.device);
+    std::vector<double4> hw(ctx->n);
+    HIPCHK(hipMemcpy(hw.data(), ctx->wt[ctx->cur], sizeof(double4) * ctx->n, hipMemcpyDeviceToHost));
+    for (int i = 0; i < ctx->n; i++) {
+        const int b = ninst > 1 ? inst[i] : 0;
+        if (b < 0 || b >= ninst) return fail(ctx, DEM_ERR_ARG, "dem_set_instances: instance out of range");
+        const int m = (int)hw[i].w;
+        hw[i].w = (double)((m & 31) | (b << 5));
+    }
+    HIPCHK(hipMemcpy(ctx->wt[ctx->cur], hw.data(), sizeof(double4) * ctx->n, hipMemcpyHostToDevice));
+    ctx->ninst = ninst;
+    ctx->inst_mat.assign((size_t)ninst * (M_ETAR + 1) * kT1 * kT1, 0.0);
+    ctx->inst_set.assign(ninst, 0);
+    int rc = upload_mat(ctx);
+    if (rc) return rc;
+    return setup_grid(ctx);
+}
+
+extern "C" int dem_set_instance_materials(dem_ctx *ctx, int inst, const double *Yeff, const double *Geff,
+                                          const double *betaeff, const double *mu, const double *mur,
+                                          const double *etar)
+{
+    if (!ctx) return fail(nullptr, DEM_ERR_ARG, "null context");
+    if (inst < 0 || inst >= ctx->ninst || !Yeff || !Geff || !betaeff || !mu || !ctx->mat_set)
+        return fail(ctx, DEM_ERR_ARG, "dem_set_instance_materials: bad arguments");
+    hipSetDevice(ctx->p.device);
+    const int w = ctx->ntypes + 1;
+    const size_t T = (size_t)kT1 * kT1;
+    double *dst = ctx->inst_mat.data() + (size_t)inst * (M_ETAR + 1) * T;
+    const double *src[M_ETAR + 1] = {Yeff, Geff, betaeff, mu, mur, etar};
+    for (int sl = 0; sl <= M_ETAR; sl++)
+        for (int i = 0; i < kT1; i++)
+            for (int j = 0; j < kT1; j++) {
+                const double base = ctx->hmat[sl * T + i * kT1 + j];
+                dst[sl * T + i * kT1 + j] = (src[sl] && i < w && j < w) ? src[sl][i * w + j] : base;
+            }
+    ctx->inst_set[inst] = 1;
+    return upload_mat(ctx);
+}
+
+extern "C" int dem_observe_plane(dem_ctx *ctx, int plane)
+{
+    if (!ctx) return fail(nullptr, DEM_ERR_ARG, "null context");
+    if (plane >= ctx->P.n || plane < -1) return fail(ctx, DEM_ERR_ARG, "dem_observe_plane: no such plane");
+    if (ctx->n == 0) return fail(ctx, DEM_ERR_ARG, "dem_observe_plane: upload the atoms first");
+    hipSetDevice(ctx->p.device);
+    ctx->obs_plane = plane;
+    if (plane >= 0) {
+        if (dev_alloc(ctx, &ctx->pwf, ctx->cap)) return DEM_ERR_HIP;
+        HIPCHK(hipMemset(ctx->pwf, 0, sizeof(double) * ctx->cap));
+    }
+    return DEM_OK;
+}
+
+extern "C" int dem_instance_observables(dem_ctx *ctx, double *out)
+{
+    if (!ctx) return fail(nullptr, DEM_ERR_ARG, "null context");
+    if (!out || !ctx->list_valid) return fail(ctx, DEM_ERR_ARG, "dem_instance_observables: call dem_setup first");
+    hipSetDevice(ctx->p.device);
+    const int B = ctx->ninst;
+    if (B * 5 > ctx->obs_cap) {
+        if (dev_alloc(ctx, &ctx->d_obs, (size_t)B * 5)) return DEM_ERR_HIP;
+        ctx->obs_cap = B * 5;
+    }
+    launch_inst_obs(ctx->cstart, ctx->G.nkeys_local, B, ctx->xr[ctx->cur], ctx->vm[ctx->cur],
+                    ctx->obs_plane >= 0 ? ctx->pwf : nullptr, ctx->d_obs, ctx->stream);
+    KCHK();
+    HIPCHK(hipMemcpyAsync(out, ctx->d_obs, sizeof(double) * B * 5, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return DEM_OK;
+}
+
+extern "C" int dem_use_graph(dem_ctx *ctx, int on)
+{
+    if (!ctx) return fail(nullptr, DEM_ERR_ARG, "null context");
+    ctx->use_graph = on != 0;
+    return DEM_OK;
+}
+
+// Time the contact pass for every lane-group width and block size on the
+// current state (set-up passes, shearupdate = 0) and keep the fastest.  The
+// pair and wall history, list flags and forces are restored afterwards.
+extern "C" int dem_autotune(dem_ctx *ctx, int reps, int *lanes, int *block, double *ms)
+{
+    if (!ctx) return fail(nullptr, DEM_ERR_ARG, "null context");
+    if (!ctx->list_valid || reps < 1) return fail(ctx, DEM_ERR_ARG, "dem_autotune: call dem_setup first, reps >= 1");
+    hipSetDevice(ctx->p.device);
+    hipStream_t s = ctx->stream;
+    const int n = ctx->n, l = ctx->lcur, c = ctx->cur;
+    int64_t E = 0;
+    {
+        int h = 0;
+        HIPCHK(hipMemcpy(&h, ctx->nstart[l] + n, sizeof(int), hipMemcpyDeviceToHost));
+        E = h;
+    }
+    const size_t hb = sizeof(double) * (size_t)E * ctx->M.dnum, jb = sizeof(uint32_t) * (size_t)E;
+    const size_t wb = sizeof(double) * (size_t)n * ctx->P.n * ctx->M.wdnum, fb = sizeof(double4) * (size_t)n;
+    char *save = nullptr;
+    const size_t tot = hb + jb + wb + 2 * fb + 64;
+    HIPCHK(hipMalloc((void **)&save, tot));
+    char *sh = save, *sj = sh + hb, *sw = sj + jb, *sf = sw + wb, *st = sf + fb;
+    auto copy = [&](void *d, const void *q, size_t b) {
+        return b ? hipMemcpyAsync(d, q, b, hipMemcpyDeviceToDevice, s) : hipSuccess;
+    };
+    auto restore = [&]() {
+        return (copy(ctx->hist[l], sh, hb) | copy(ctx->nj[l], sj, jb) | copy(ctx->whist[c], sw, wb)
+                | copy(ctx->f4, sf, fb) | copy(ctx->t4, st, fb));
+    };
+    if (copy(sh, ctx->hist[l], hb) | copy(sj, ctx->nj[l], jb) | copy(sw, ctx->whist[c], wb) | copy(sf, ctx->f4, fb)
+        | copy(st, ctx->t4, fb)) {
+        hipFree(save);
+        return fail(ctx, DEM_ERR_HIP, "dem_autotune: copy failed");
+    }
+    const bool any_width = ctx->p.normal_model == 0 && ctx->p.wall_normal_model == 0 && ctx->M.cohesion == CM_OFF;
+    const int widths[] = {1, 2, 4, 8, 16};
+    const int blocks[] = {64, 128, 256};
+    hipEvent_t e0, e1;
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    const bool was_timing = ctx->timing;
+    ctx->timing = false;
+    double best = 1e300;
+    int bl = 4, bb = kBlock;
+    int rc = DEM_OK;
+    for (int g : widths) {
+        if (!any_width && g != 4) continue;
+        for (int b : blocks) {
+            ctx->tune_lanes = g;
+            ctx->tune_block = b;
+            if ((rc = force_pass(ctx, 0))) break;
+            hipEventRecord(e0, s);
+            for (int r = 0; r < reps && rc == DEM_OK; r++) rc = force_pass(ctx, 0);
+            hipEventRecord(e1, s);
+            if (rc) break;
+            if (hipEventSynchronize(e1) != hipSuccess) {
+                rc = fail(ctx, DEM_ERR_HIP, "dem_autotune: event failed");
+                break;
+            }
+            float t = 0.f;
+            hipEventElapsedTime(&t, e0, e1);
+            if (restore() != hipSuccess) {
+                rc = fail(ctx, DEM_ERR_HIP, "dem_autotune: restore failed");
+                break;
+            }
+            if (t / reps < best) {
+                best = t / reps;
+                bl = g;
+                bb = b;
+            }
+        }
+        if (rc) break;
+    }
+    ctx->timing = was_timing;
+    hipStreamSynchronize(s);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipFree(save);
+    if (rc) return rc;
+    ctx->tune_lanes = bl;
+    ctx->tune_block = bb;
+    if (lanes) *lanes = bl;
+    if (block) *block = bb;
+    if (ms) *ms = best;
+    return check_err_flag(ctx);
 }

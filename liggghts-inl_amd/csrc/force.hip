@@ -55,7 +55,7 @@ template <bool MIXED, int G, int NM, int WN, bool CH = false>
 __global__ void __launch_bounds__(kBlock) k_force(ForceArgs a)
 {
     const int blk = xcd_block(blockIdx.x, gridDim.x);
-    const int tid = blk * kBlock + threadIdx.x;
+    const int tid = blk * blockDim.x + threadIdx.x;
     const int g = threadIdx.x & (G - 1);
     const int i = tid / G;
     const bool live = i < a.n;      // dead groups still join the shuffles
@@ -65,8 +65,10 @@ __global__ void __launch_bounds__(kBlock) k_force(ForceArgs a)
     const double4 vmi = a.vm[ii];
     const double4 wi = a.wt[ii];
     const int metai = (int)wi.w;
-    const int itype = metai & 15;
-    const bool fzi = (metai >> 4) != 0;
+    const int itype = meta_type(metai);
+    const bool fzi = meta_frozen(metai);
+    // material tables of the particle's instance (partners share it)
+    const double *mat = a.mat + (size_t)meta_inst(metai) * kMatStride;
     double F[3] = {0.0, 0.0, 0.0}, T[3] = {0.0, 0.0, 0.0};
     const int dnum = M.dnum;
     const double cdm = M.cdf * M.cdf;
@@ -114,12 +116,12 @@ __global__ void __launch_bounds__(kBlock) k_force(ForceArgs a)
             const double4 vmj = a.vm[j];
             const double4 wj = a.wt[j];
             const int metaj = (int)wj.w;
-            c.jtype = metaj & 15;
+            c.jtype = meta_type(metaj);
             c.radj = radj;
             c.mj = vmj.w;
             double meff = c.mi * c.mj / (c.mi + c.mj);
             if (M.freeze_on && fzi) meff = c.mj;
-            if (M.freeze_on && (metaj >> 4)) meff = c.mi;
+            if (M.freeze_on && meta_frozen(metaj)) meff = c.mi;
             c.meff = meff;
             c.delta[0] = delx; c.delta[1] = dely; c.delta[2] = delz;
             c.r = r; c.rinv = rinv; c.radsum = radsum;
@@ -132,10 +134,10 @@ __global__ void __launch_bounds__(kBlock) k_force(ForceArgs a)
 #pragma unroll
             for (int k = 0; k < HN; k++) h[k] = (known && k < dnum) ? hp[k] : 0.0;
             double dF[3], dT[3];
-            if (MIXED && NM == NM_HERTZ) contact_mixed(M, a.mat, c, h, a.shearupdate, dF, dT);
-            else if (CH) contact_fp64<NM, true>(M, M.lp, a.mat, c, h, a.shearupdate, dF, dT, coh,
+            if (MIXED && NM == NM_HERTZ) contact_mixed(M, mat, c, h, a.shearupdate, dF, dT);
+            else if (CH) contact_fp64<NM, true>(M, M.lp, mat, c, h, a.shearupdate, dF, dT, coh,
                                                 bonded ? M.cohesion : CM_OFF);
-            else contact_fp64<NM>(M, M.lp, a.mat, c, h, a.shearupdate, dF, dT);
+            else contact_fp64<NM>(M, M.lp, mat, c, h, a.shearupdate, dF, dT);
             // the hysteretic models write their state even in set-up passes
             if (a.shearupdate || !known || NM != NM_HERTZ) {
 #pragma unroll
@@ -167,6 +169,7 @@ __global__ void __launch_bounds__(kBlock) k_force(ForceArgs a)
     if (!live || g != 0) return;
 
     // primitive planes, in the order the fixes were defined
+    double pw = 0.0;    // force of the observed plane along its axis
     if (a.P.n) {
         const uint32_t wm = a.wmask[i];
         const double *xc = &pi.x;
@@ -212,8 +215,8 @@ __global__ void __launch_bounds__(kBlock) k_force(ForceArgs a)
 #pragma unroll
                     for (int q = 0; q < WHN; q++) h[q] = q < wdn ? hp[q] : 0.0;
                     double dF[3], dT[3];
-                    if (MIXED && WN == NM_HERTZ) contact_mixed(M, a.mat, cw, h, a.shearupdate, dF, dT);
-                    else contact_fp64<WN>(M, M.lw, a.mat, cw, h, a.shearupdate, dF, dT);
+                    if (MIXED && WN == NM_HERTZ) contact_mixed(M, mat, cw, h, a.shearupdate, dF, dT);
+                    else contact_fp64<WN>(M, M.lw, mat, cw, h, a.shearupdate, dF, dT);
                     if (a.shearupdate || WN != NM_HERTZ) {
 #pragma unroll
                         for (int q = 0; q < WHN; q++)
@@ -221,6 +224,7 @@ __global__ void __launch_bounds__(kBlock) k_force(ForceArgs a)
                     }
                     F[0] += dF[0]; F[1] += dF[1]; F[2] += dF[2];
                     T[0] += dT[0]; T[1] += dT[1]; T[2] += dT[2];
+                    if (k == a.obs_plane) pw = dF[d];
                 } else {
                     contact_close(M.lw, hp);
                 }
@@ -242,24 +246,27 @@ __global__ void __launch_bounds__(kBlock) k_force(ForceArgs a)
     a.f4[i] = make_double4(F[0], F[1], F[2], 0.0);
     a.t4[i] = make_double4(T[0], T[1], T[2], 0.0);
     if (a.ncontact) a.ncontact[i] = nc;
+    if (a.pwf) a.pwf[i] = pw;
 }
 
-static int padded_blocks(long long threads)
+static int padded_blocks(long long threads, int block = kBlock)
 {
-    long long nb = (threads + kBlock - 1) / kBlock;
+    long long nb = (threads + block - 1) / block;
     return (int)((nb + 7) & ~7LL);
 }
 
-// lanes per particle: DEM_CONTACT_LANES (1, 4, 8 or 16), default 4 (the
-// fastest on MI355X for rows of ~8 neighbours: 0.53 ms vs 0.58 (8), 0.64
-// (1), 0.67 (16) per 1M-particle pass)
-static int lanes_per_particle()
+// lanes per particle and block size: the ones the autotuner chose
+// (ForceArgs::lanes/block), else DEM_CONTACT_LANES (1, 2, 4, 8 or 16),
+// default 4 lanes (the fastest on MI355X for rows of ~8 neighbours: 0.53 ms
+// vs 0.58 (8), 0.64 (1), 0.67 (16) per 1M-particle pass) and 256 threads
+static int lanes_per_particle(const ForceArgs &a)
 {
+    if (a.lanes) return a.lanes;
     static int g = -1;
     if (g < 0) {
         const char *s = getenv("DEM_CONTACT_LANES");
         g = s ? atoi(s) : 4;
-        if (g != 1 && g != 4 && g != 8 && g != 16) g = 4;
+        if (g != 1 && g != 2 && g != 4 && g != 8 && g != 16) g = 4;
     }
     return g;
 }
@@ -269,17 +276,19 @@ static int lanes_per_particle()
 template <bool MIXED, int NM, int WN>
 static void launch_nm(const ForceArgs &a, hipStream_t s)
 {
-    const int G = (NM == NM_HERTZ && WN == NM_HERTZ) ? lanes_per_particle() : 4;
-    const int nb = padded_blocks((long long)a.n * G);
+    const int G = (NM == NM_HERTZ && WN == NM_HERTZ) ? lanes_per_particle(a) : 4;
+    const int B = a.block ? a.block : kBlock;
+    const int nb = padded_blocks((long long)a.n * G, B);
     if constexpr (NM == NM_HERTZ && WN == NM_HERTZ) {
         switch (G) {
-        case 1: k_force<MIXED, 1, NM, WN><<<nb, kBlock, 0, s>>>(a); return;
-        case 8: k_force<MIXED, 8, NM, WN><<<nb, kBlock, 0, s>>>(a); return;
-        case 16: k_force<MIXED, 16, NM, WN><<<nb, kBlock, 0, s>>>(a); return;
+        case 1: k_force<MIXED, 1, NM, WN><<<nb, B, 0, s>>>(a); return;
+        case 2: k_force<MIXED, 2, NM, WN><<<nb, B, 0, s>>>(a); return;
+        case 8: k_force<MIXED, 8, NM, WN><<<nb, B, 0, s>>>(a); return;
+        case 16: k_force<MIXED, 16, NM, WN><<<nb, B, 0, s>>>(a); return;
         default: break;
         }
     }
-    k_force<MIXED, 4, NM, WN><<<nb, kBlock, 0, s>>>(a);
+    k_force<MIXED, 4, NM, WN><<<nb, B, 0, s>>>(a);
 }
 
 // with a cohesion model: FP64, 4 lanes per particle

@@ -21,6 +21,9 @@ struct ForceArgs {
     int defer_freeze;    // mesh walls follow: they apply fix freeze
     const double *bout;  // cohesion force/torque per entry (6), when bonded
     const uint8_t *bvalid;
+    double *pwf;         // optional per-atom force of plane obs_plane along its axis
+    int obs_plane;
+    int lanes, block;    // launch shape (0: defaults)
     Model M;
     Planes P;
 };
@@ -43,6 +46,7 @@ struct BondArgs {
 
 struct BinArgs {
     double4 *xr;         // wrapped in place (Domain::pbc)
+    const double4 *wt;   // instance in meta
     int4 *img;
     uint32_t *key;
     int *count;          // per Morton key
@@ -71,6 +75,7 @@ struct PermuteArgs {
 
 struct NeighArgs {
     const double4 *xr;   // new order
+    const double4 *wt;   // new order (instance in meta)
     const int *tag;      // new order
     const int *cstart;   // per key, exclusive prefix + total at nkeys
     int *count;          // per atom (count pass)
@@ -90,6 +95,8 @@ struct NeighArgs {
     double cdf, skin;
     Grid G;
     double prd[3];
+    int64_t ecap;        // fill pass: entries allocated
+    int *err;            // bit 3: list longer than ecap
 };
 
 struct IntegArgs {
@@ -172,6 +179,13 @@ void launch_neigh_count(const NeighArgs &a, hipStream_t s);
 // *maxcount = max(*maxcount, max of count[0..n))
 void launch_row_max(const int *count, int n, int *maxcount, hipStream_t s);
 void launch_neigh_fill(const NeighArgs &a, hipStream_t s);
+void launch_fill_int(int *p, int v, int n, hipStream_t s);
+void launch_row_max_check(const int *maxcount, int limit, int *err, hipStream_t s);
+void launch_copy_int(int *dst, const int *src, int n, hipStream_t s);
+// out[b*5 + {0..4}] = count, sum z, sum of pwf, sum mass, kinetic energy of
+// instance b (particles in key order)
+void launch_inst_obs(const int *cstart, int nkl, int ninst, const double4 *xr, const double4 *vm,
+                     const double *pwf, double *out, hipStream_t s);
 void launch_force_fp64(const ForceArgs &a, hipStream_t s);
 void launch_force_mixed(const ForceArgs &a, hipStream_t s);
 // mode: 1 = initial_integrate, 2 = final_integrate, 3 = final then initial

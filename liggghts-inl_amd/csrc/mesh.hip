@@ -139,7 +139,7 @@ __global__ void __launch_bounds__(kBlock) k_mesh_force(MeshForceArgs a)
     const int nn = a.mstart[i + 1] - base;
     const double4 wi = a.wt[i];
     const int meta = (int)wi.w;
-    const bool frozen = a.apply_freeze && M.freeze_on && (meta >> 4) != 0;
+    const bool frozen = a.apply_freeze && M.freeze_on && meta_frozen(meta);
     if (nn > 0) {
         const double4 pi = a.xr[i];
         const double4 vmi = a.vm[i];
@@ -210,7 +210,7 @@ __global__ void __launch_bounds__(kBlock) k_mesh_force(MeshForceArgs a)
             if (intersect) {
                 ContactIn cw;
                 cw.is_wall = true;
-                cw.itype = meta & 15;
+                cw.itype = meta_type(meta);
                 cw.jtype = a.wtype;
                 cw.radi = radi;
                 cw.radj = 0.0;

@@ -33,9 +33,13 @@ __device__ inline int cell_coord(double x, int d, const Grid &G)
     return c;
 }
 
-__device__ inline uint32_t cell_key(int cx, int cy, int cz, const Grid &G)
+// Morton key of a cell; the cells of instance b of a batched run occupy keys
+// [b*nkeys_local, (b+1)*nkeys_local), so the sort groups each instance's
+// particles into one contiguous range and no stencil crosses instances
+__device__ inline uint32_t cell_key(int cx, int cy, int cz, const Grid &G, int inst)
 {
-    return G.spread[cx] | G.spread[G.maxc + cy] | G.spread[2 * G.maxc + cz];
+    return (uint32_t)inst * (uint32_t)G.nkeys_local
+           + (G.spread[cx] | G.spread[G.maxc + cy] | G.spread[2 * G.maxc + cz]);
 }
 
 __global__ void __launch_bounds__(kBlock) k_bin(BinArgs a)
@@ -63,7 +67,7 @@ __global__ void __launch_bounds__(kBlock) k_bin(BinArgs a)
     a.xr[i] = p;
     a.img[i] = im;
     const uint32_t k = cell_key(cell_coord(p.x, 0, a.G), cell_coord(p.y, 1, a.G),
-                                cell_coord(p.z, 2, a.G), a.G);
+                                cell_coord(p.z, 2, a.G), a.G, meta_inst((int)a.wt[i].w));
     a.key[i] = k;
     atomicAdd(&a.count[k], 1);
 }
@@ -243,6 +247,7 @@ __global__ void __launch_bounds__(kBlock) k_neigh(NeighArgs a)
     const int p = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     if (p >= a.n) return;  // p is uniform across the wavefront
     const double4 xp = a.xr[p];
+    const int inst = meta_inst((int)a.wt[p].w);
     const int cc[3] = {cell_coord(xp.x, 0, a.G), cell_coord(xp.y, 1, a.G), cell_coord(xp.z, 2, a.G)};
     int qb = 0, cnt = 0, code = kNoShift;
     if (lane < 27) {
@@ -261,7 +266,7 @@ __global__ void __launch_bounds__(kBlock) k_neigh(NeighArgs a)
             }
         }
         if (ok) {
-            const uint32_t key = cell_key(c[0], c[1], c[2], a.G);
+            const uint32_t key = cell_key(c[0], c[1], c[2], a.G, inst);
             qb = a.cstart[key];
             cnt = a.cstart[key + 1] - qb;
             code = (sh[0] + 1) + 3 * (sh[1] + 1) + 9 * (sh[2] + 1);
@@ -296,6 +301,12 @@ __global__ void __launch_bounds__(kBlock) k_neigh(NeighArgs a)
     }
     const int on64 = min(oe - ob, 64);
     const int ebase = FILL ? a.nstart[p] : 0;
+    // a list longer than the allocation (possible only when the rebuild runs
+    // inside a captured graph, sized beforehand) is reported, not written
+    if (FILL && (int64_t)a.nstart[a.n] > a.ecap) {
+        if (lane == 0) atomicOr(a.err, 8);
+        return;
+    }
     int written = 0;
     for (int c0 = 0; c0 < total; c0 += 64) {
         const int c = c0 + lane;
@@ -405,6 +416,84 @@ void launch_neigh_fill(const NeighArgs &a, hipStream_t s)
 {
     if (a.n == 0) return;
     k_neigh<true><<<wave_blocks(a.n), kBlock, 0, s>>>(a);
+}
+
+// plain fills and copies: what a rebuild would otherwise do with
+// hipMemsetAsync / hipMemcpyAsync, as kernels so that the rebuild can be
+// recorded into a graph
+__global__ void __launch_bounds__(kBlock) k_fill_int(int *p, int v, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void __launch_bounds__(kBlock) k_copy_int(int *dst, const int *src, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
+// err |= 16 if the longest row exceeds neigh_modify one (the graph form of
+// the host check after the count pass)
+__global__ void k_row_max_check(const int *maxcount, int limit, int *err)
+{
+    if (threadIdx.x == 0 && *maxcount > limit) atomicOr(err, 16);
+}
+
+void launch_row_max_check(const int *maxcount, int limit, int *err, hipStream_t s)
+{
+    k_row_max_check<<<1, 64, 0, s>>>(maxcount, limit, err);
+}
+
+void launch_fill_int(int *p, int v, int n, hipStream_t s)
+{
+    if (n <= 0) return;
+    k_fill_int<<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(p, v, n);
+}
+
+void launch_copy_int(int *dst, const int *src, int n, hipStream_t s)
+{
+    if (n <= 0) return;
+    k_copy_int<<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(dst, src, n);
+}
+
+// Per-instance sums over the instance's contiguous particle range
+// [cstart[b*nkl], cstart[(b+1)*nkl]): count, sum of z, sum of the observed
+// plane's force, sum of mass, kinetic energy.  One block per instance,
+// strided partial sums then a fixed tree: the same order on every call.
+constexpr int kObsThreads = 256;
+constexpr int kObs = 5;
+
+__global__ void __launch_bounds__(kObsThreads) k_inst_obs(const int *cstart, int nkl, const double4 *xr,
+                                                         const double4 *vm, const double *pwf, double *out)
+{
+    __shared__ double s_v[kObs][kObsThreads];
+    const int b = blockIdx.x;
+    const int p0 = cstart[(size_t)b * nkl], p1 = cstart[(size_t)(b + 1) * nkl];
+    double acc[kObs] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int p = p0 + threadIdx.x; p < p1; p += kObsThreads) {
+        const double4 x = xr[p], v = vm[p];
+        acc[0] += 1.0;
+        acc[1] += x.z;
+        acc[2] += pwf ? pwf[p] : 0.0;
+        acc[3] += v.w;
+        acc[4] += 0.5 * v.w * (v.x * v.x + v.y * v.y + v.z * v.z);
+    }
+    for (int k = 0; k < kObs; k++) s_v[k][threadIdx.x] = acc[k];
+    __syncthreads();
+    for (int h = kObsThreads / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h)
+            for (int k = 0; k < kObs; k++) s_v[k][threadIdx.x] += s_v[k][threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x < kObs) out[(size_t)b * kObs + threadIdx.x] = s_v[threadIdx.x][0];
+}
+
+void launch_inst_obs(const int *cstart, int nkl, int ninst, const double4 *xr, const double4 *vm,
+                     const double *pwf, double *out, hipStream_t s)
+{
+    if (ninst <= 0) return;
+    k_inst_obs<<<ninst, kObsThreads, 0, s>>>(cstart, nkl, xr, vm, pwf, out);
 }
 
 }  // namespace dem

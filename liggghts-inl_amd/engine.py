@@ -126,6 +126,12 @@ def library():
         "dem_list_stats": (C.c_int, [vp, dp, dp, i64p]),
         "dem_sync": (C.c_int, [vp]),
         "dem_timestep": (C.c_int64, [vp]),
+        "dem_set_instances": (C.c_int, [vp, C.c_int, ip]),
+        "dem_set_instance_materials": (C.c_int, [vp, C.c_int, dp, dp, dp, dp, dp, dp]),
+        "dem_observe_plane": (C.c_int, [vp, C.c_int]),
+        "dem_instance_observables": (C.c_int, [vp, dp]),
+        "dem_use_graph": (C.c_int, [vp, C.c_int]),
+        "dem_autotune": (C.c_int, [vp, C.c_int, ip, ip, dp]),
         "dem_add_mesh": (C.c_int, [vp, C.c_int, dp, C.c_int, C.c_double, dp]),
         "dem_mesh_size": (C.c_int, [vp, C.c_int]),
         "dem_mesh_topology": (C.c_int, [C.c_int, dp, C.c_double, dp, dp, ip, ip, ip, ip, ip]),
@@ -250,9 +256,17 @@ class Simulation:
                                                   for k in HYST_KEYS]))
             self._chk(self.L.dem_set_hysteretic(self.h, _dp(crl), _dp(tabs)))
         self.ntypes = nt
+        self.nu = np.asarray(m["nu"], dtype=np.float64)
         self.upload(atoms)
         if "bond" in spec:
             self.set_bond(**spec["bond"])
+        self.ninst = 1
+        if "instances" in spec:
+            self.set_instances(**spec["instances"])
+        if spec.get("observe_plane") is not None:
+            self.observe_plane(spec["observe_plane"])
+        if spec.get("graph", False):
+            self.use_graph(True)
 
     def add_mesh(self, nodes, type, curvature=None, vel=None):
         nd = np.ascontiguousarray(np.asarray(nodes, dtype=np.float64).reshape(-1, 9))
@@ -401,3 +415,41 @@ class Simulation:
 
     def timestep(self):
         return self.L.dem_timestep(self.h)
+
+    # ---- batched runs
+    def set_instances(self, inst, Y=None, e=None, mu=None):
+        """Atom -> instance (upload order).  Y, e, mu: per-instance values
+        (every type of instance b gets Y[b], every type pair e[b], mu[b])."""
+        inst = np.ascontiguousarray(inst, dtype=np.int32)
+        ninst = int(inst.max()) + 1 if len(inst) else 1
+        self._chk(self.L.dem_set_instances(self.h, ninst, _ip(inst)))
+        self.ninst = ninst
+        if Y is None:
+            return
+        w = self.ntypes + 1
+        for b in range(ninst):
+            Yb = np.full(w, float(Y[b]))
+            eb = np.full((w, w), float(e[b]))
+            Yeff, Geff, beta = material_tables(self.ntypes, Yb, self.nu, eb)
+            mub = np.ascontiguousarray(np.full((w, w), float(mu[b])))
+            self._chk(self.L.dem_set_instance_materials(self.h, b, _dp(Yeff), _dp(Geff), _dp(beta), _dp(mub),
+                                                        None, None))
+
+    def observe_plane(self, k):
+        self._chk(self.L.dem_observe_plane(self.h, int(k)))
+
+    def observables(self):
+        """(ninst, 5): count, sum z, observed plane force, mass, kinetic energy."""
+        out = np.zeros((self.ninst, 5))
+        self._chk(self.L.dem_instance_observables(self.h, _dp(out)))
+        return out
+
+    def use_graph(self, on=True):
+        self._chk(self.L.dem_use_graph(self.h, int(bool(on))))
+
+    def autotune(self, reps=5):
+        g = C.c_int(0)
+        b = C.c_int(0)
+        ms = C.c_double(0)
+        self._chk(self.L.dem_autotune(self.h, int(reps), C.byref(g), C.byref(b), C.byref(ms)))
+        return g.value, b.value, ms.value

@@ -182,3 +182,65 @@ def hopper(n=200_000, seed=7, rmin=0.8e-3, rmax=1.2e-3, spacing=2.5e-3, jitter=5
         meshes=[dict(nodes=tris, type=2)], planes=[(2, -0.04, 2)], gravity=(0.0, 0.0, -9.81),
     )
     return spec, _atoms(x, r, rng, 0.0)
+
+
+def batch_pours(ninst=256, seed=5, nmin=500, nmax=8000, rmin=0.8e-3, rmax=1.2e-3, spacing=2.5e-3,
+                jitter=5e-5):
+    """Independent small pours for a batched run: instance b holds N_b ~
+    U[nmin, nmax] spheres, r ~ U[rmin, rmax], on a jittered lattice in the
+    same 40x40x100 mm box with 6 planes, v ~ U[-0.1, 0.1] m/s, Hertz +
+    history, dt = 1e-6, skin 0.25 mm, rebuild every 10 steps, gravity; its
+    own Y ~ U[1e6, 1e7] Pa, mu ~ U[0.2, 0.7], e ~ U[0.3, 0.9] (nu = 0.3).
+
+    Returns (spec, atoms, inst, params): atoms of all instances concatenated
+    (tags unique over the batch), inst = instance of each atom, params =
+    dict(Y, e, mu) per instance.  ``instance(spec, atoms, inst, params, b)``
+    gives one instance as a stand-alone (spec, atoms)."""
+    rng = np.random.default_rng(seed)
+    L = np.array([0.04, 0.04, 0.10])
+    counts = rng.integers(nmin, nmax + 1, size=ninst)
+    Y = rng.uniform(1e6, 1e7, size=ninst)
+    mu = rng.uniform(0.2, 0.7, size=ninst)
+    e = rng.uniform(0.3, 0.9, size=ninst)
+    nx = int((L[0] - 2 * rmax - 2 * jitter) // spacing) + 1
+    ny = int((L[1] - 2 * rmax - 2 * jitter) // spacing) + 1
+    xs, rs, vs, ins = [], [], [], []
+    for b in range(ninst):
+        n = int(counts[b])
+        nz = int(np.ceil(n / (nx * ny)))
+        ii, jj, kk = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+        sites = np.stack([ii.ravel(), jj.ravel(), kk.ravel()], 1)
+        sites = sites[np.lexsort((sites[:, 0], sites[:, 1], sites[:, 2]))][:n]
+        xs.append((rmax + jitter) + sites * spacing + rng.uniform(-jitter, jitter, size=(n, 3)))
+        rs.append(rng.uniform(rmin, rmax, size=n))
+        vs.append(rng.uniform(-0.1, 0.1, size=(n, 3)))
+        ins.append(np.full(n, b, np.int32))
+    x = np.concatenate(xs)
+    r = np.concatenate(rs)
+    if x[:, 2].max() + rmax > L[2]:
+        raise ValueError("batch_pours: lattice taller than the box")
+    atoms = _atoms(x, r, rng, 0.0)
+    atoms["v"] = np.concatenate(vs)
+    spec = dict(
+        box_lo=[0.0, 0.0, 0.0], box_hi=list(L), periodic=[0, 0, 0], dt=1e-6, skin=2.5e-4, cdf=1.0,
+        neigh_every=10, neigh_delay=0, neigh_check=False, rolling="off",
+        materials=_materials(Y=5e6, e=0.6, mu=0.45),
+        planes=[(0, 0.0, 1), (0, L[0], 1), (1, 0.0, 1), (1, L[1], 1), (2, 0.0, 1), (2, L[2], 1)],
+        gravity=(0.0, 0.0, -9.81),
+    )
+    return spec, atoms, np.concatenate(ins), dict(Y=Y, e=e, mu=mu)
+
+
+def instance(spec, atoms, inst, params, b):
+    """Instance b of a batch as a stand-alone (spec, atoms): its atoms (tags
+    kept) and its own materials."""
+    k = inst == b
+    a = {key: np.ascontiguousarray(v[k]) for key, v in atoms.items()}
+    m = dict(spec["materials"])
+    w = m["ntypes"] + 1
+    m["Y"] = np.full(w, float(params["Y"][b]))
+    m["e"] = np.full((w, w), float(params["e"][b]))
+    m["mu"] = np.full((w, w), float(params["mu"][b]))
+    s = dict(spec, materials=m)
+    s.pop("instances", None)
+    return s, a

@@ -1,0 +1,147 @@
+"""Batched runs (many independent pours in one context), graph stepping and
+the autotuner, on the MI355X.
+
+* a batch of uneven pours, each with its own Y, e, mu: every instance of
+  the batched run is bitwise the same as that instance run alone on the
+  device (same kernels, same per-instance cell order), with and without
+  graph stepping; and within the short-horizon tolerance of the oracle
+  running the instance by itself;
+* per-instance observables equal numpy sums over the downloaded state, and
+  the observed plane's force equals, for spheres that touch only the floor,
+  f_z - m g_z of each sphere;
+* the autotuner leaves the state and the contact history unchanged.
+"""
+import numpy as np
+import pytest
+
+from conftest import load_engine
+from oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def small_batch(ninst=6, nmax=1500, seed=21):
+    W = load_engine().workloads
+    return W.batch_pours(ninst=ninst, seed=seed, nmin=400, nmax=nmax)
+
+
+def run_batch(spec, atoms, inst, params, steps, graph=False, precision="fp64"):
+    eng = load_engine()
+    sp = dict(spec, instances=dict(inst=inst, **params), observe_plane=4, graph=graph)
+    s = eng.Simulation(sp, atoms, precision=precision)
+    s.setup()
+    s.step(steps)
+    return s
+
+
+def by_tag(st, tags):
+    o = np.argsort(tags)
+    return {k: v[o] for k, v in st.items() if k in ("x", "v", "omega", "f", "torque")}
+
+
+def test_batch_equals_solo_bitwise_and_graph():
+    eng = load_engine()
+    W = eng.workloads
+    spec, atoms, inst, params = small_batch()
+    steps = 95
+    sb = run_batch(spec, atoms, inst, params, steps)
+    gb = by_tag(sb.state(), atoms["tag"])
+    sg = run_batch(spec, atoms, inst, params, steps, graph=True)
+    gg = by_tag(sg.state(), atoms["tag"])
+    for k in gb:
+        assert np.array_equal(gb[k], gg[k]), k
+    order = np.argsort(atoms["tag"])
+
+    def cells(maxrad):   # the cell grid follows the largest radius
+        cut = 2 * maxrad + spec["skin"]
+        return [int(np.floor((h - l) / (cut * 1.0001))) for l, h in zip(spec["box_lo"], spec["box_hi"])]
+
+    for b in range(int(inst.max()) + 1):
+        s1, a1 = W.instance(spec, atoms, inst, params, b)
+        # same grid, hence the same cell order in every row
+        assert cells(a1["radius"].max()) == cells(atoms["radius"].max())
+        s = eng.Simulation(s1, a1)
+        s.setup()
+        s.step(steps)
+        g1 = by_tag(s.state(), a1["tag"])
+        sel = np.isin(atoms["tag"][order], a1["tag"])
+        for k in g1:
+            assert np.array_equal(g1[k], gb[k][sel]), (b, k)
+
+
+def test_batch_against_oracle_short_horizon():
+    eng = load_engine()
+    W = eng.workloads
+    spec, atoms, inst, params = small_batch(ninst=3, nmax=1200, seed=4)
+    steps = 60
+    sb = run_batch(spec, atoms, inst, params, steps)
+    g = by_tag(sb.state(), atoms["tag"])
+    order = np.argsort(atoms["tag"])
+    for b in range(3):
+        s1, a1 = W.instance(spec, atoms, inst, params, b)
+        o = Oracle(s1, a1)
+        o.setup()
+        o.run(steps)
+        r = by_tag(o.state(), a1["tag"])
+        sel = np.isin(atoms["tag"][order], a1["tag"])
+        assert np.abs(g["x"][sel] - r["x"]).max() <= 1e-12 * 0.1
+        assert np.abs(g["v"][sel] - r["v"]).max() <= 1e-9 * np.abs(r["v"]).max()
+        fs = np.abs(r["f"]).max()
+        assert np.abs(g["f"][sel] - r["f"]).max() <= 1e-9 * fs
+
+
+def test_observables():
+    spec, atoms, inst, params = small_batch(ninst=4, nmax=1000, seed=8)
+    s = run_batch(spec, atoms, inst, params, 40, graph=True)
+    obs = s.observables()
+    st = s.state()
+    for b in range(4):
+        k = inst == b
+        assert obs[b, 0] == k.sum()
+        m = atoms["rmass"][k]
+        assert abs(obs[b, 1] - st["x"][k, 2].sum()) <= 1e-12 * abs(obs[b, 1])
+        assert abs(obs[b, 3] - m.sum()) <= 1e-12 * m.sum()
+        ke = 0.5 * (m * (st["v"][k] ** 2).sum(1)).sum()
+        assert abs(obs[b, 4] - ke) <= 1e-12 * ke
+
+
+def test_observed_plane_force_closed_form():
+    """Spheres pressed into the floor, apart from each other: the floor's
+    force on each is its f_z minus its weight."""
+    spec, _, _, _ = small_batch(ninst=1)
+    n = 50
+    rng = np.random.default_rng(3)
+    r = rng.uniform(0.8e-3, 1.2e-3, n)
+    x = np.stack([0.003 + 0.0035 * (np.arange(n) % 10), 0.003 + 0.0035 * (np.arange(n) // 10),
+                  r - rng.uniform(1e-6, 2e-5, n)], 1)
+    atoms = dict(x=x, v=np.zeros((n, 3)), omega=np.zeros((n, 3)), radius=r, rmass=4 / 3 * np.pi * r ** 3 * 2500,
+                 type=np.ones(n, np.int32), tag=np.arange(1, n + 1, dtype=np.int32))
+    inst = (np.arange(n) % 2).astype(np.int32)
+    params = dict(Y=np.array([2e6, 8e6]), e=np.array([0.4, 0.8]), mu=np.array([0.3, 0.6]))
+    s = run_batch(spec, atoms, inst, params, 0)
+    st = s.state()
+    obs = s.observables()
+    wall = st["f"][:, 2] - atoms["rmass"] * -9.81
+    for b in range(2):
+        want = wall[inst == b].sum()
+        assert want > 0
+        assert abs(obs[b, 2] - want) <= 1e-12 * want
+
+
+def test_autotune_keeps_state():
+    spec, atoms, inst, params = small_batch(ninst=4, seed=9)
+    s = run_batch(spec, atoms, inst, params, 30)
+    before = s.state()
+    pb = s.pairs()
+    g, b, ms = s.autotune(reps=2)
+    assert g in (1, 2, 4, 8, 16) and b in (64, 128, 256) and ms > 0
+    after = s.state()
+    pa = s.pairs()
+    for k in ("x", "v", "omega", "f", "torque"):
+        assert np.array_equal(before[k], after[k]), k
+    for u, v in zip(pb, pa):
+        assert np.array_equal(u, v)
+    s.step(20)
+    s2 = run_batch(spec, atoms, inst, params, 50)
+    a, c = s.state(), s2.state()
+    assert np.abs(a["x"] - c["x"]).max() <= 1e-12 * 0.1
