@@ -937,6 +937,7 @@ static int rebuild(dem_ctx *ctx, bool captured = false)
         if (ensure_list_cap(ctx, l1, hsmall[0])) return DEM_ERR_NOMEM;
     }
     na.ecap = ctx->ecap[l1];
+    na.coh_off = ctx->M.cohesion != CM_OFF ? ctx->M.lp.off_coh : -1;
     na.err = ctx->d_small;
     na.nstart = ctx->nstart[l1];
     na.nj = ctx->nj[l1];
@@ -1416,13 +1417,11 @@ extern "C" int dem_get_pairs(dem_ctx *ctx, int64_t *count, int *tag_i, int *tag_
             if (!hist) continue;
             double *out = hist + (size_t)e * dnum;
             const double *src = &hh[(size_t)e * dnum];
-            // an entry that never touched holds no history (zero by definition)
-            if (!known) {
-                for (int k = 0; k < dnum; k++) out[k] = 0.0;
-                continue;
-            }
+            // an entry without contact flags holds no contact history (zero
+            // by definition); its cohesion slots are kept (zero, or what a
+            // broken bond left until the next rebuild)
             int k = 0;
-            for (int q = 0; q < off_ref_coh; q++) out[k++] = src[q];
+            for (int q = 0; q < off_ref_coh; q++) out[k++] = known ? src[q] : 0.0;
             if (ncoh) {
                 const double *cs = src + L.off_coh;
                 double sign = 1.0;
@@ -1435,7 +1434,7 @@ extern "C" int dem_get_pairs(dem_ctx *ctx, int64_t *count, int *tag_i, int *tag_
                 }
                 for (int q = 0; q < ncoh; q++) out[k++] = (q >= 5 && q < 14) ? sign * cs[q] : cs[q];
             }
-            for (int q = off_ref_coh; q < (L.off_coh >= 0 ? L.off_coh : dnum); q++) out[k++] = src[q];
+            for (int q = off_ref_coh; q < (L.off_coh >= 0 ? L.off_coh : dnum); q++) out[k++] = known ? src[q] : 0.0;
         }
     *count = E;
     return DEM_OK;

@@ -71,6 +71,9 @@ __global__ void __launch_bounds__(kBlock) k_force(ForceArgs a)
     const double *mat = a.mat + (size_t)meta_inst(metai) * kMatStride;
     double F[3] = {0.0, 0.0, 0.0}, T[3] = {0.0, 0.0, 0.0};
     const int dnum = M.dnum;
+    // slots of the contact chain; the cohesion slots after them belong to
+    // the bond pass
+    const int hdn = (CH && M.lp.off_coh >= 0) ? M.lp.off_coh : dnum;
     const double cdm = M.cdf * M.cdf;
     int nc = 0;
 
@@ -132,7 +135,7 @@ __global__ void __launch_bounds__(kBlock) k_force(ForceArgs a)
             double *hp = a.hist + (size_t)e * dnum;
             const bool known = (w & (CH ? (kFlagBit | kBondBit) : kFlagBit)) != 0;
 #pragma unroll
-            for (int k = 0; k < HN; k++) h[k] = (known && k < dnum) ? hp[k] : 0.0;
+            for (int k = 0; k < HN; k++) h[k] = (known && k < hdn) ? hp[k] : 0.0;
             double dF[3], dT[3];
             if (MIXED && NM == NM_HERTZ) contact_mixed(M, mat, c, h, a.shearupdate, dF, dT);
             else if (CH) contact_fp64<NM, true>(M, M.lp, mat, c, h, a.shearupdate, dF, dT, coh,
@@ -142,7 +145,7 @@ __global__ void __launch_bounds__(kBlock) k_force(ForceArgs a)
             if (a.shearupdate || !known || NM != NM_HERTZ) {
 #pragma unroll
                 for (int k = 0; k < HN; k++)
-                    if (k < dnum) hp[k] = h[k];
+                    if (k < hdn) hp[k] = h[k];
             }
             if (!(w & kFlagBit)) a.nj[e] = w | kFlagBit;
             F[0] += dF[0]; F[1] += dF[1]; F[2] += dF[2];

@@ -96,6 +96,7 @@ struct NeighArgs {
     Grid G;
     double prd[3];
     int64_t ecap;        // fill pass: entries allocated
+    int coh_off;         // first cohesion history slot, -1 without a cohesion model
     int *err;            // bit 3: list longer than ecap
 };
 

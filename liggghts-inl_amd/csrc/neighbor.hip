@@ -365,6 +365,11 @@ __global__ void __launch_bounds__(kBlock) k_neigh(NeighArgs a)
                     double *h = a.hist + (size_t)e * a.dnum;
                     for (int k = 0; k < a.dnum; k++) h[k] = oh[k];
                     w |= a.onj[ob + found] & (kFlagBit | kBondBit);
+                } else if (a.coh_off >= 0) {
+                    // the cohesion model reads its slots on entries that
+                    // never touched (bond creation), so they start at zero
+                    double *h = a.hist + (size_t)e * a.dnum;
+                    for (int k = a.coh_off; k < a.dnum; k++) h[k] = 0.0;
                 }
                 a.nj[e] = w;
             }
