@@ -257,6 +257,23 @@ int dem_use_graph(dem_ctx *ctx, int on);
  * the fastest for this context.  State and history are left unchanged. */
 int dem_autotune(dem_ctx *ctx, int reps, int *lanes, int *block, double *ms);
 
+/* ---- Adjoint window.  No counterpart in the reference (which has no
+ * derivatives); gradients of per-instance observables with respect to the
+ * contact parameters, for calibration.  FP64 Hertz pairs and planes with
+ * tangential history only. ---- */
+/* Right after dem_setup: runs nsteps with the set-up neighbour list kept
+ * (where a rebuild would fall, the contact history is carried over as the
+ * rebuild does; stepping stops with an error if any particle moves more
+ * than half the skin), keeping a checkpoint every checkpoint_every steps. */
+int dem_adjoint_forward(dem_ctx *ctx, int nsteps, int checkpoint_every);
+/* Reverse sweep over that window, segment by segment from the checkpoints.
+ * J = sum_b w_height[b] * (mean z of instance b) + w_wall[b] * (force of the
+ * observed plane on instance b), at the window's last step (either weight
+ * array may be NULL).  grad[4b..4b+3] = dJ/d(Yeff, Geff, betaeff, mu) of
+ * instance b, summed over its type pairs.  The context is left at the
+ * window's final state. */
+int dem_adjoint_backward(dem_ctx *ctx, const double *w_height, const double *w_wall, double *grad);
+
 #ifdef __cplusplus
 }
 #endif

@@ -10,6 +10,10 @@ The directory name contains a hyphen, so import it by path, e.g.::
     import importlib.util, os
     spec = importlib.util.spec_from_file_location(
         "liggghts_inl_amd", os.path.join(root, "liggghts-inl_amd", "__init__.py"))
+
+The gradient front end (``liggghts_inl_amd.adjoint``, a torch.autograd
+Function) is imported on its own, so importing the package does not pull in
+torch.
 """
 from .engine import (  # noqa: F401
     DEM_FP64,

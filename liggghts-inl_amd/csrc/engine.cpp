@@ -91,6 +91,19 @@ struct dem_ctx {
     bool gexec_capturing = false;
     hipGraphExec_t gexec = nullptr;
     int g_period = 0;
+
+    // adjoint window: fixed list, checkpoints, tape of force-pass inputs
+    long long setup_ts = -1;           // timestep of the last dem_setup
+    bool window = false;               // rebuild steps apply the history mask instead
+    int64_t adj_E = 0;                 // list entries of the window
+    int adj_steps = 0, adj_every = 0;
+    std::vector<char *> ckpts;         // one per segment start, then the final state
+    std::vector<long long> ck_ts;
+    std::vector<int> ck_ago;
+    char *tape = nullptr;
+    int tape_cap = 0, tape_pos = -1;   // tape_pos >= 0: recording
+    std::vector<uint8_t> tape_mask;
+    int *adj_rev = nullptr;
     std::vector<const void *> g_key;   // buffers the graph was recorded with
 
     int n = 0, cap = 0;
@@ -339,6 +352,8 @@ static void free_all(dem_ctx *ctx)
     F(ctx->d_mat); F(ctx->d_small);
     F(ctx->bout); F(ctx->bvalid); F(ctx->brev);
     F(ctx->pwf); F(ctx->d_obs);
+    for (char *c : ctx->ckpts) F(c);
+    F(ctx->tape); F(ctx->adj_rev);
     if (ctx->gexec) hipGraphExecDestroy(ctx->gexec);
     for (auto &m : ctx->meshes) {
         F(m.tri); F(m.enorm); F(m.gstart); F(m.glist);
@@ -1173,6 +1188,7 @@ extern "C" int dem_setup(dem_ctx *ctx)
     if (rc) return rc;
     rc = force_pass(ctx, 0);
     if (rc) return rc;
+    ctx->setup_ts = ctx->ntimestep;
     return check_err_flag(ctx);
 }
 
@@ -1201,12 +1217,118 @@ static int decide(dem_ctx *ctx, int *out)
     return DEM_OK;
 }
 
+// ---- adjoint window helpers (the entry points are at the end of the file)
+
+// layout of a state record: xr vm wt [f4 t4] hist nj whist
+struct StateView {
+    double4 *xr, *vm, *wt, *f4, *t4;
+    double *hist;
+    uint32_t *nj;
+    double *whist;
+};
+
+static StateView state_view(const dem_ctx *ctx, char *base, bool forces)
+{
+    const size_t n = ctx->n;
+    StateView v{};
+    char *q = base;
+    auto take = [&](size_t b) {
+        char *r = q;
+        q += (b + 255) & ~(size_t)255;
+        return r;
+    };
+    v.xr = (double4 *)take(n * sizeof(double4));
+    v.vm = (double4 *)take(n * sizeof(double4));
+    v.wt = (double4 *)take(n * sizeof(double4));
+    if (forces) {
+        v.f4 = (double4 *)take(n * sizeof(double4));
+        v.t4 = (double4 *)take(n * sizeof(double4));
+    }
+    v.hist = (double *)take((size_t)ctx->adj_E * ctx->M.dnum * sizeof(double));
+    v.nj = (uint32_t *)take((size_t)ctx->adj_E * sizeof(uint32_t));
+    v.whist = (double *)take(n * ctx->P.n * ctx->M.wdnum * sizeof(double));
+    return v;
+}
+
+static size_t state_record_bytes(const dem_ctx *ctx, bool forces)
+{
+    const size_t n = ctx->n;
+    auto r = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    return 3 * r(n * sizeof(double4)) + (forces ? 2 * r(n * sizeof(double4)) : 0)
+           + r((size_t)ctx->adj_E * ctx->M.dnum * sizeof(double)) + r((size_t)ctx->adj_E * sizeof(uint32_t))
+           + r(n * ctx->P.n * ctx->M.wdnum * sizeof(double));
+}
+
+static int copy_state(dem_ctx *ctx, char *rec, bool forces, bool save)
+{
+    StateView v = state_view(ctx, rec, forces);
+    const size_t n = ctx->n;
+    const int c = ctx->cur, l = ctx->lcur;
+    hipStream_t s = ctx->stream;
+    auto cp = [&](void *dev, void *r, size_t b) {
+        if (!b) return hipSuccess;
+        return save ? hipMemcpyAsync(r, dev, b, hipMemcpyDeviceToDevice, s)
+                    : hipMemcpyAsync(dev, r, b, hipMemcpyDeviceToDevice, s);
+    };
+    hipError_t e = hipSuccess;
+    auto acc = [&](hipError_t x) {
+        if (x != hipSuccess) e = x;
+    };
+    acc(cp(ctx->xr[c], v.xr, n * sizeof(double4)));
+    acc(cp(ctx->vm[c], v.vm, n * sizeof(double4)));
+    acc(cp(ctx->wt[c], v.wt, n * sizeof(double4)));
+    if (forces) {
+        acc(cp(ctx->f4, v.f4, n * sizeof(double4)));
+        acc(cp(ctx->t4, v.t4, n * sizeof(double4)));
+    }
+    acc(cp(ctx->hist[l], v.hist, (size_t)ctx->adj_E * ctx->M.dnum * sizeof(double)));
+    acc(cp(ctx->nj[l], v.nj, (size_t)ctx->adj_E * sizeof(uint32_t)));
+    acc(cp(ctx->whist[c], v.whist, n * ctx->P.n * ctx->M.wdnum * sizeof(double)));
+    if (e != hipSuccess) return fail(ctx, DEM_ERR_HIP, std::string("adjoint state copy: ") + hipGetErrorString(e));
+    return DEM_OK;
+}
+
+// where a rebuild would fall inside the window: the history carry-over of
+// the rebuild on the unchanged list, and the distance check that keeps the
+// list valid (flag in d_small[2])
+static int window_mask(dem_ctx *ctx)
+{
+    AdjMaskArgs m{};
+    m.xr = ctx->xr[ctx->cur];
+    m.nstart = ctx->nstart[ctx->lcur];
+    m.nj = ctx->nj[ctx->lcur];
+    m.hist = ctx->hist[ctx->lcur];
+    m.n = ctx->n;
+    m.dnum = ctx->M.dnum;
+    m.cdf = ctx->p.cdf;
+    for (int d = 0; d < 3; d++) m.prd[d] = ctx->M.prd[d];
+    launch_adj_mask(m, ctx->stream);
+    KCHK();
+    const double trig = 0.5 * ctx->p.skin;
+    launch_check_distance(ctx->xr[ctx->cur], ctx->xhold, ctx->n, trig * trig, ctx->d_small + 2, ctx->stream);
+    KCHK();
+    ctx->ago = 0;
+    return DEM_OK;
+}
+
+// the force-pass inputs of this step into tape slot tape_pos
+static int tape_store(dem_ctx *ctx, int mask_step)
+{
+    if (ctx->tape_pos >= ctx->tape_cap) return fail(ctx, DEM_ERR_STATE, "adjoint tape overflow (internal error)");
+    char *rec = ctx->tape + (size_t)ctx->tape_pos * state_record_bytes(ctx, false);
+    const int rc = copy_state(ctx, rec, false, true);
+    if (rc) return rc;
+    ctx->tape_mask[ctx->tape_pos] = (uint8_t)(mask_step != 0);
+    ctx->tape_pos++;
+    return DEM_OK;
+}
+
 // Graph stepping applies when every rebuild falls on a fixed step: no
 // distance check, delay within one period, nothing that needs the host
 // between steps (meshes, bonds, timers).
 static bool graph_ok(const dem_ctx *ctx)
 {
-    return ctx->use_graph && !ctx->p.neigh_check && ctx->p.neigh_delay <= ctx->p.neigh_every
+    return ctx->use_graph && !ctx->window && !ctx->p.neigh_check && ctx->p.neigh_delay <= ctx->p.neigh_every
            && ctx->meshes.empty() && ctx->M.cohesion == CM_OFF && !ctx->timing && ctx->n > 0;
 }
 
@@ -1320,8 +1442,9 @@ extern "C" int dem_step(dem_ctx *ctx, int nsteps)
         if (st == 0 && ((rc = integrate(ctx, 1)) || (rc = move_meshes(ctx)))) return rc;
         int nflag = 0;
         if ((rc = decide(ctx, &nflag))) return rc;
-        if (nflag && (rc = rebuild(ctx))) return rc;
+        if (nflag && (rc = ctx->window ? window_mask(ctx) : rebuild(ctx))) return rc;
         if (!nflag) mesh_trigger(ctx);
+        if (ctx->tape_pos >= 0 && (rc = tape_store(ctx, nflag))) return rc;
         if ((rc = force_pass(ctx, 1))) return rc;
         const bool last = st == nsteps - 1;
         if ((rc = integrate(ctx, last ? 2 : 3))) return rc;
@@ -1865,4 +1988,202 @@ extern "C" int dem_autotune(dem_ctx *ctx, int reps, int *lanes, int *block, doub
     if (block) *block = bb;
     if (ms) *ms = best;
     return check_err_flag(ctx);
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint window: gradients of per-instance observables with respect to each
+// instance's Yeff, Geff, betaeff and mu.
+
+static int adjoint_requirements(dem_ctx *ctx)
+{
+    const dem_params &p = ctx->p;
+    if (p.normal_model != 0 || p.wall_normal_model != 0 || p.rolling_model != 0 || p.cohesion_model != 0
+        || p.limit_force || p.wall_limit_force || !p.tangential_damping || p.cdf != 1.0 || p.neigh_check
+        || p.precision != DEM_FP64 || !ctx->meshes.empty() || ctx->M.freeze_on)
+        return fail(ctx, DEM_ERR_ARG,
+                    "adjoint window: needs FP64 Hertz pairs and planes with tangential history (tangential "
+                    "damping on, limitForce off), no rolling, bonds, meshes or freeze, contactDistanceFactor 1 "
+                    "and neigh_modify check no");
+    return DEM_OK;
+}
+
+extern "C" int dem_adjoint_forward(dem_ctx *ctx, int nsteps, int checkpoint_every)
+{
+    if (!ctx) return fail(nullptr, DEM_ERR_ARG, "null context");
+    int rc = adjoint_requirements(ctx);
+    if (rc) return rc;
+    if (nsteps < 1 || checkpoint_every < 1) return fail(ctx, DEM_ERR_ARG, "dem_adjoint_forward: nsteps, checkpoint_every >= 1");
+    if (!ctx->list_valid || ctx->ntimestep != ctx->setup_ts)
+        return fail(ctx, DEM_ERR_ARG, "dem_adjoint_forward: call right after dem_setup");
+    hipSetDevice(ctx->p.device);
+    for (char *c : ctx->ckpts) hipFree(c);
+    ctx->ckpts.clear();
+    ctx->ck_ts.clear();
+    ctx->ck_ago.clear();
+    {
+        int h = 0;
+        HIPCHK(hipMemcpy(&h, ctx->nstart[ctx->lcur] + ctx->n, sizeof(int), hipMemcpyDeviceToHost));
+        ctx->adj_E = h;
+    }
+    if (dev_alloc(ctx, &ctx->adj_rev, (size_t)std::max<int64_t>(ctx->adj_E, 1))) return DEM_ERR_NOMEM;
+    launch_adj_rev(ctx->nstart[ctx->lcur], ctx->nj[ctx->lcur], ctx->adj_rev, ctx->n, ctx->stream);
+    KCHK();
+    launch_fill_int(ctx->d_small + 2, 0, 1, ctx->stream);
+    const size_t rb = state_record_bytes(ctx, true);
+    ctx->window = true;
+    const int nseg = (nsteps + checkpoint_every - 1) / checkpoint_every;
+    for (int sgi = 0; sgi <= nseg && rc == DEM_OK; sgi++) {
+        char *c = nullptr;
+        if (hipMalloc((void **)&c, rb) != hipSuccess) {
+            rc = fail(ctx, DEM_ERR_NOMEM, "dem_adjoint_forward: out of device memory for checkpoints");
+            break;
+        }
+        ctx->ckpts.push_back(c);
+        ctx->ck_ts.push_back(ctx->ntimestep);
+        ctx->ck_ago.push_back(ctx->ago);
+        if ((rc = copy_state(ctx, c, true, true))) break;
+        if (sgi == nseg) break;
+        const int len = std::min(checkpoint_every, nsteps - sgi * checkpoint_every);
+        rc = dem_step(ctx, len);
+    }
+    ctx->window = false;
+    if (rc) return rc;
+    int moved = 0;
+    HIPCHK(hipMemcpy(&moved, ctx->d_small + 2, sizeof(int), hipMemcpyDeviceToHost));
+    if (moved)
+        return fail(ctx, DEM_ERR_STATE,
+                    "adjoint window: a particle moved more than half the skin; the fixed neighbour list would miss "
+                    "pairs (larger skin or a shorter window)");
+    ctx->adj_steps = nsteps;
+    ctx->adj_every = checkpoint_every;
+    return DEM_OK;
+}
+
+static int restore_ckpt(dem_ctx *ctx, int k)
+{
+    const int rc = copy_state(ctx, ctx->ckpts[k], true, false);
+    ctx->ntimestep = ctx->ck_ts[k];
+    ctx->ago = ctx->ck_ago[k];
+    return rc;
+}
+
+extern "C" int dem_adjoint_backward(dem_ctx *ctx, const double *w_height, const double *w_wall, double *grad)
+{
+    if (!ctx) return fail(nullptr, DEM_ERR_ARG, "null context");
+    if (!grad || ctx->adj_steps < 1 || ctx->ckpts.empty())
+        return fail(ctx, DEM_ERR_ARG, "dem_adjoint_backward: run dem_adjoint_forward first");
+    if (w_wall && ctx->obs_plane < 0)
+        return fail(ctx, DEM_ERR_ARG, "dem_adjoint_backward: a wall-force seed needs dem_observe_plane");
+    hipSetDevice(ctx->p.device);
+    hipStream_t s = ctx->stream;
+    const int n = ctx->n, B = ctx->ninst, N = ctx->adj_steps, K = ctx->adj_every;
+    const int nseg = (N + K - 1) / K;
+    const size_t E = (size_t)ctx->adj_E, dn = ctx->M.dnum, pw = (size_t)n * ctx->P.n * 3;
+    // adjoint buffers, double buffered; parameter sums; seeds
+    char *buf = nullptr;
+    auto r256 = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t b4 = r256(n * sizeof(double4)), bh = r256(std::max<size_t>(E * dn, 1) * sizeof(double)),
+                 bw = r256(std::max<size_t>(pw, 1) * sizeof(double)), bg = r256((size_t)n * 4 * sizeof(double)),
+                 bs = r256((size_t)B * sizeof(double));
+    const size_t total = 6 * b4 + 2 * bh + 2 * bw + bg + 2 * bs + r256((size_t)B * 4 * sizeof(double))
+                         + r256((size_t)B * 5 * sizeof(double));
+    HIPCHK(hipMalloc((void **)&buf, total));
+    HIPCHK(hipMemsetAsync(buf, 0, total, s));
+    char *q = buf;
+    auto take = [&](size_t b) {
+        char *r = q;
+        q += b;
+        return r;
+    };
+    double4 *lx[2] = {(double4 *)take(b4), (double4 *)take(b4)};
+    double4 *lv[2] = {(double4 *)take(b4), (double4 *)take(b4)};
+    double4 *lw[2] = {(double4 *)take(b4), (double4 *)take(b4)};
+    double *lh[2] = {(double *)take(bh), (double *)take(bh)};
+    double *lhw[2] = {(double *)take(bw), (double *)take(bw)};
+    double *gp = (double *)take(bg);
+    double *d_wh = (double *)take(bs), *d_ww = (double *)take(bs);
+    double *d_grad = (double *)take(r256((size_t)B * 4 * sizeof(double)));
+    double *d_cnt = (double *)take(r256((size_t)B * 5 * sizeof(double)));
+    int rc = DEM_OK;
+    auto done = [&](int code) {
+        hipStreamSynchronize(s);
+        hipFree(buf);
+        ctx->tape_pos = -1;
+        return code;
+    };
+    if (w_height) HIPCHK(hipMemcpyAsync(d_wh, w_height, sizeof(double) * B, hipMemcpyHostToDevice, s));
+    if (w_wall) HIPCHK(hipMemcpyAsync(d_ww, w_wall, sizeof(double) * B, hipMemcpyHostToDevice, s));
+    // seed at the final state: the mean height's gradient
+    if ((rc = restore_ckpt(ctx, nseg))) return done(rc);
+    launch_inst_obs(ctx->cstart, ctx->G.nkeys_local, B, ctx->xr[ctx->cur], ctx->vm[ctx->cur], nullptr, d_cnt, s);
+    launch_adj_seed(ctx->wt[ctx->cur], w_height ? d_wh : nullptr, d_cnt, lx[0], n, s);
+    KCHK();
+    // tape for the longest segment
+    const size_t tb = state_record_bytes(ctx, false);
+    if (K > ctx->tape_cap) {
+        if (ctx->tape) hipFree(ctx->tape);
+        ctx->tape = nullptr;
+        if (hipMalloc((void **)&ctx->tape, tb * K) != hipSuccess) {
+            ctx->tape_cap = 0;
+            return done(fail(ctx, DEM_ERR_NOMEM, "dem_adjoint_backward: out of device memory for the tape"));
+        }
+        ctx->tape_cap = K;
+    }
+    ctx->tape_mask.assign(K, 0);
+    int cur = 0;
+    AdjArgs a{};
+    a.nstart = ctx->nstart[ctx->lcur];
+    a.rev = ctx->adj_rev;
+    a.wmask = ctx->wmask;
+    a.mat = ctx->d_mat;
+    a.gp = gp;
+    a.obs_plane = ctx->obs_plane;
+    a.n = n;
+    a.dnum = (int)dn;
+    a.wdnum = ctx->M.wdnum;
+    a.dt = ctx->p.dt;
+    a.M = ctx->M;
+    a.P = ctx->P;
+    auto step_back = [&](const StateView &v, int update, int mask_step, double ckick, const double *wseed) {
+        a.xr = v.xr; a.vm = v.vm; a.wt = v.wt;
+        a.nj = v.nj; a.hist = v.hist; a.whist = v.whist;
+        a.lx = lx[cur]; a.lv = lv[cur]; a.lw = lw[cur]; a.lh = lh[cur]; a.lhw = lhw[cur];
+        a.lx1 = lx[1 - cur]; a.lv1 = lv[1 - cur]; a.lw1 = lw[1 - cur]; a.lh1 = lh[1 - cur]; a.lhw1 = lhw[1 - cur];
+        a.update = update;
+        a.mask_step = mask_step;
+        a.ckick = ckick;
+        a.wseed = wseed;
+        launch_adj_step(a, s);
+        cur = 1 - cur;
+    };
+    for (int sgi = nseg - 1; sgi >= 0 && rc == DEM_OK; sgi--) {
+        const int len = std::min(K, N - sgi * K);
+        if ((rc = restore_ckpt(ctx, sgi))) break;
+        ctx->window = true;
+        ctx->tape_pos = 0;
+        rc = dem_step(ctx, len);
+        ctx->window = false;
+        ctx->tape_pos = -1;
+        if (rc) break;
+        for (int t = len - 1; t >= 0; t--) {
+            const int nstep = sgi * K + t + 1;
+            const StateView v = state_view(ctx, ctx->tape + (size_t)t * tb, false);
+            step_back(v, 1, ctx->tape_mask[t], nstep == N ? 0.5 * ctx->p.dt : ctx->p.dt,
+                      (nstep == N && w_wall) ? d_ww : nullptr);
+        }
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) rc = fail(ctx, DEM_ERR_HIP, std::string("adjoint step: ") + hipGetErrorString(e));
+    }
+    if (rc) return done(rc);
+    // the set-up pass (shearupdate = 0): its forces kick the first half step
+    {
+        const StateView v = state_view(ctx, ctx->ckpts[0], true);
+        step_back(v, 0, 0, 0.5 * ctx->p.dt, nullptr);
+    }
+    launch_adj_reduce(ctx->cstart, ctx->G.nkeys_local, B, gp, d_grad, s);
+    KCHK();
+    HIPCHK(hipMemcpyAsync(grad, d_grad, sizeof(double) * B * 4, hipMemcpyDeviceToHost, s));
+    // leave the context at the window's final state
+    if ((rc = restore_ckpt(ctx, nseg))) return done(rc);
+    return done(check_err_flag(ctx));
 }

@@ -168,6 +168,46 @@ void launch_mesh_force(const MeshForceArgs &a, hipStream_t s);
 // MeshMoverLinear: node += dx, center += dx for ntri triangles
 void launch_mesh_move(TriGeo *tri, int ntri, double dx, double dy, double dz, hipStream_t s);
 
+// Adjoint window (adjoint.hip).
+struct AdjMaskArgs {
+    const double4 *xr;
+    const int *nstart;
+    uint32_t *nj;
+    double *hist;
+    int n, dnum;
+    double cdf;
+    double prd[3];
+};
+
+struct AdjArgs {
+    // the step's force-pass inputs, from the tape
+    const double4 *xr, *vm, *wt;
+    const uint32_t *nj;
+    const double *hist, *whist;
+    const int *nstart, *rev;
+    const uint8_t *wmask;
+    const double *mat;
+    // adjoints after the step (in) and before it (out)
+    const double4 *lx, *lv, *lw;
+    double4 *lx1, *lv1, *lw1;
+    const double *lh, *lhw;
+    double *lh1, *lhw1;
+    double *gp;              // per particle: Yeff, Geff, betaeff, mu, summed over steps
+    const double *wseed;     // per instance: adjoint of the observed plane's force, or null
+    int obs_plane;
+    int n, dnum, wdnum, update, mask_step;
+    double ckick;            // lambda_f = ckick / m * lambda_v
+    double dt;
+    Model M;
+    Planes P;
+};
+
+void launch_adj_mask(const AdjMaskArgs &a, hipStream_t s);
+void launch_adj_rev(const int *nstart, const uint32_t *nj, int *rev, int n, hipStream_t s);
+void launch_adj_step(const AdjArgs &a, hipStream_t s);
+void launch_adj_seed(const double4 *wt, const double *wh, const double *count, double4 *lx, int n, hipStream_t s);
+void launch_adj_reduce(const int *cstart, int nkl, int ninst, const double *gp, double *out, hipStream_t s);
+
 void launch_bond(const BondArgs &a, hipStream_t s);
 void launch_bond_rev(const int *nstart, const uint32_t *nj, int *rev, int n, hipStream_t s);
 

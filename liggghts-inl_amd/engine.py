@@ -132,6 +132,8 @@ def library():
         "dem_instance_observables": (C.c_int, [vp, dp]),
         "dem_use_graph": (C.c_int, [vp, C.c_int]),
         "dem_autotune": (C.c_int, [vp, C.c_int, ip, ip, dp]),
+        "dem_adjoint_forward": (C.c_int, [vp, C.c_int, C.c_int]),
+        "dem_adjoint_backward": (C.c_int, [vp, dp, dp, dp]),
         "dem_add_mesh": (C.c_int, [vp, C.c_int, dp, C.c_int, C.c_double, dp]),
         "dem_mesh_size": (C.c_int, [vp, C.c_int]),
         "dem_mesh_topology": (C.c_int, [C.c_int, dp, C.c_double, dp, dp, ip, ip, ip, ip, ip]),
@@ -446,6 +448,19 @@ class Simulation:
 
     def use_graph(self, on=True):
         self._chk(self.L.dem_use_graph(self.h, int(bool(on))))
+
+    def adjoint_forward(self, nsteps, checkpoint_every=100):
+        self._chk(self.L.dem_adjoint_forward(self.h, int(nsteps), int(checkpoint_every)))
+
+    def adjoint_backward(self, w_height=None, w_wall=None):
+        """(ninst, 4): gradient of sum_b w_height[b] mean_z_b + w_wall[b] Fwall_b
+        with respect to each instance's Yeff, Geff, betaeff, mu."""
+        g = np.zeros((self.ninst, 4))
+        wh = None if w_height is None else np.ascontiguousarray(w_height, dtype=np.float64)
+        ww = None if w_wall is None else np.ascontiguousarray(w_wall, dtype=np.float64)
+        self._chk(self.L.dem_adjoint_backward(self.h, None if wh is None else _dp(wh),
+                                              None if ww is None else _dp(ww), _dp(g)))
+        return g
 
     def autotune(self, reps=5):
         g = C.c_int(0)
