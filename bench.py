@@ -139,6 +139,26 @@ def bench_batch(pkg, args):
                          autotune={"lanes": tune[0], "block": tune[1], "contact_ms": tune[2]},
                          mean_bed_height_mm=float((obs[:, 1] / obs[:, 0]).mean() * 1e3))
         print("%s: %.3f s for %d steps" % (prec, dt, args.steps), file=sys.stderr, flush=True)
+    adj = None
+    if args.adjoint:
+        # gradients of every instance's mean height and floor force with
+        # respect to its Y, mu, e (FP64), checkpoints every 100 steps; the
+        # window keeps one neighbour list, so the skin covers the motion
+        sp = dict(spec, skin=8e-4, instances=dict(inst=inst, **params), observe_plane=4)
+        s = pkg.Simulation(sp, atoms, precision="fp64")
+        s.setup()
+        s.sync()
+        t0 = time.perf_counter()
+        s.adjoint_forward(args.steps, 100)
+        s.sync()
+        t1 = time.perf_counter()
+        g = s.adjoint_backward(np.ones(args.instances), np.ones(args.instances))
+        t2 = time.perf_counter()
+        s.close()
+        adj = {"steps": args.steps, "checkpoint_every": 100, "forward_s": t1 - t0, "backward_s": t2 - t1,
+               "backward_particle_steps_per_s": n * args.steps / (t2 - t1),
+               "grad_finite": bool(np.isfinite(g).all())}
+        print("adjoint: forward %.2f s, backward %.2f s" % (t1 - t0, t2 - t1), file=sys.stderr, flush=True)
     main_prec = "mixed" if "mixed" in out else "fp64"
     R = out[main_prec]
     line = {
@@ -153,6 +173,8 @@ def bench_batch(pkg, args):
                    "n_particles": n, "precision": main_prec},
         "forms": out,
     }
+    if adj is not None:
+        line["adjoint"] = adj
     print(json.dumps(line))
 
 
@@ -168,6 +190,7 @@ def main():
     ap.add_argument("--workload", choices=["column", "batch"], default="column",
                     help="column: the headline 1M-sphere metric; batch: 256 batched pours")
     ap.add_argument("--instances", type=int, default=256)
+    ap.add_argument("--adjoint", action="store_true", help="batch: also time the adjoint window (FP64)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MB",
                     help="only run a device-to-device copy of MB MiB, 10 times (for calibrating "
                          "the FETCH_SIZE/WRITE_SIZE counters against known traffic)")

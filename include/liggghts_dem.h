@@ -256,6 +256,10 @@ int dem_use_graph(dem_ctx *ctx, int on);
  * group width (1..16 lanes per particle) and block size (64..256) and keeps
  * the fastest for this context.  State and history are left unchanged. */
 int dem_autotune(dem_ctx *ctx, int reps, int *lanes, int *block, double *ms);
+/* Sets the contact pass's shape by hand (lanes per particle 1, 2, 4, 8 or
+ * 16, block 64, 128 or 256; 0 keeps the default), e.g. to reuse an earlier
+ * autotuner result.  Hertz chains take any width; others run 4 lanes. */
+int dem_set_contact_shape(dem_ctx *ctx, int lanes, int block);
 
 /* ---- Adjoint window.  No counterpart in the reference (which has no
  * derivatives); gradients of per-instance observables with respect to the

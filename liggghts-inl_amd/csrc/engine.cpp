@@ -183,6 +183,15 @@ static int fail(dem_ctx *ctx, int code, const std::string &msg)
     return code;
 }
 
+// Host calls that read or write device data with the null stream first wait
+// for the context's own stream: it is non-blocking, so the null stream does
+// not order against it.
+static void sync_enter(dem_ctx *ctx)
+{
+    hipSetDevice(ctx->p.device);
+    hipStreamSynchronize(ctx->stream);
+}
+
 template <typename T>
 static int dev_alloc(dem_ctx *ctx, T **ptr, size_t count)
 {
@@ -426,7 +435,7 @@ extern "C" int dem_set_materials(dem_ctx *ctx, int ntypes, const double *Yeff, c
     if (ctx->p.rolling_model && !mur) return fail(ctx, DEM_ERR_ARG, "rolling model needs coeffRollFrict");
     if ((ctx->p.rolling_model == 1 || ctx->p.rolling_model == 3) && !etar)
         return fail(ctx, DEM_ERR_ARG, "rolling model needs coeffRollVisc");
-    hipSetDevice(ctx->p.device);
+    sync_enter(ctx);
     // slots Yeff..etar; the hysteretic and bond tables set before are kept
     std::vector<double> &h = ctx->hmat;
     const int w = ntypes + 1;
@@ -448,7 +457,7 @@ extern "C" int dem_set_hysteretic(dem_ctx *ctx, const double *coeff_rest_log, co
     if (!ctx) return fail(nullptr, DEM_ERR_ARG, "null context");
     if (!ctx->mat_set || !coeff_rest_log || !tables)
         return fail(ctx, DEM_ERR_ARG, "dem_set_hysteretic: set materials first, non-null tables");
-    hipSetDevice(ctx->p.device);
+    sync_enter(ctx);
     const int w = ctx->ntypes + 1;
     std::vector<double> &h = ctx->hmat;
     for (int i = 0; i < w; i++)
@@ -534,7 +543,7 @@ extern "C" int dem_add_mesh(dem_ctx *ctx, int ntri, const double *nodes, int wty
     for (int d = 0; d < 3; d++)
         if (ctx->p.periodic[d])
             return fail(ctx, DEM_ERR_ARG, "dem_add_mesh: mesh walls need a box without periodic dimensions");
-    hipSetDevice(ctx->p.device);
+    sync_enter(ctx);
     ctx->meshes.emplace_back();
     MeshDev &m = ctx->meshes.back();
     mesh_build(m.h, ntri, nodes, curvature, ctx->p.boxlo, ctx->p.boxhi);
@@ -568,7 +577,7 @@ extern "C" int dem_set_bond(dem_ctx *ctx, const double *tables, const dem_bond_s
         return fail(ctx, DEM_ERR_ARG,
                     "dem_set_bond: needs a cohesion model, materials and uploaded atoms, non-null tables");
     if (ctx->ninst > 1) return fail(ctx, DEM_ERR_ARG, "dem_set_bond: bonds are not available in a batched run");
-    hipSetDevice(ctx->p.device);
+    sync_enter(ctx);
     const int w = ctx->ntypes + 1;
     std::vector<double> &h = ctx->hmat;
     for (int q = 0; q < kNBond; q++)
@@ -722,7 +731,7 @@ extern "C" int dem_upload_state(dem_ctx *ctx, int n, const double *x, const doub
     if (n < 0 || n > (int)kJMask || (n > 0 && (!x || !v || !omega || !radius || !rmass || !type || !tag)))
         return fail(ctx, DEM_ERR_ARG, "dem_upload_state: bad arguments");
     if (!ctx->mat_set) return fail(ctx, DEM_ERR_ARG, "dem_upload_state: set materials first");
-    hipSetDevice(ctx->p.device);
+    sync_enter(ctx);
     const int wdn = ctx->P.n * ctx->M.wdnum;
     std::vector<double4> hx(n), hv(n), hw(n);
     std::vector<int> ho(n);
@@ -951,6 +960,7 @@ static int rebuild(dem_ctx *ctx, bool captured = false)
             return fail(ctx, DEM_ERR_OVERFLOW, "Neighbor list overflow, boost neigh_modify one");
         if (ensure_list_cap(ctx, l1, hsmall[0])) return DEM_ERR_NOMEM;
     }
+    if (captured) launch_cap_guard(ctx->nstart[l1], n, ctx->ecap[l1], ctx->d_small + 4, ctx->d_small, s);
     na.ecap = ctx->ecap[l1];
     na.coh_off = ctx->M.cohesion != CM_OFF ? ctx->M.lp.off_coh : -1;
     na.err = ctx->d_small;
@@ -1111,6 +1121,7 @@ static void mesh_trigger(dem_ctx *ctx)
 // triangle grids for the candidate search, in each mesh's current frame
 static int setup_mesh_grids(dem_ctx *ctx)
 {
+    HIPCHK(hipStreamSynchronize(ctx->stream));   // the old grids may still be read
     double ext = 0.0;
     for (int d = 0; d < 3; d++) ext = std::max(ext, ctx->p.boxhi[d] - ctx->p.boxlo[d]);
     for (auto &m : ctx->meshes) {
@@ -1149,13 +1160,13 @@ static int check_err_flag(dem_ctx *ctx)
     int h = 0;
     HIPCHK(hipMemcpyAsync(&h, ctx->d_small, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (h & 8)
+        return fail(ctx, DEM_ERR_OVERFLOW,
+                    "neighbour list longer than the buffers sized for graph stepping (step without dem_use_graph)");
     if (h & 1)
         return fail(ctx, DEM_ERR_STATE,
                     "ilegal situation, particle positions identical. Most probable reason: Simulation diverged");
     if (h & 4) return fail(ctx, DEM_ERR_OVERFLOW, "mesh contact history: no free contact slot (internal error)");
-    if (h & 8)
-        return fail(ctx, DEM_ERR_OVERFLOW,
-                    "neighbour list longer than the buffers sized for graph stepping (step without dem_use_graph)");
     if (h & 16) return fail(ctx, DEM_ERR_OVERFLOW, "Neighbor list overflow, boost neigh_modify one");
     return DEM_OK;
 }
@@ -1163,7 +1174,7 @@ static int check_err_flag(dem_ctx *ctx)
 extern "C" int dem_rebuild(dem_ctx *ctx)
 {
     if (!ctx) return fail(nullptr, DEM_ERR_ARG, "null context");
-    hipSetDevice(ctx->p.device);
+    sync_enter(ctx);
     return rebuild(ctx);
 }
 
@@ -1171,7 +1182,7 @@ extern "C" int dem_compute_forces(dem_ctx *ctx, int shearupdate)
 {
     if (!ctx) return fail(nullptr, DEM_ERR_ARG, "null context");
     if (!ctx->list_valid) return fail(ctx, DEM_ERR_ARG, "dem_compute_forces: no neighbour list (call dem_setup)");
-    hipSetDevice(ctx->p.device);
+    sync_enter(ctx);
     int rc = force_pass(ctx, shearupdate);
     if (rc) return rc;
     return check_err_flag(ctx);
@@ -1181,7 +1192,7 @@ extern "C" int dem_setup(dem_ctx *ctx)
 {
     if (!ctx) return fail(nullptr, DEM_ERR_ARG, "null context");
     if (!ctx->mat_set) return fail(ctx, DEM_ERR_ARG, "dem_setup: set materials first");
-    hipSetDevice(ctx->p.device);
+    sync_enter(ctx);
     int rc = setup_mesh_grids(ctx);
     if (rc) return rc;
     rc = rebuild(ctx);
@@ -1357,6 +1368,7 @@ static int capture_graph(dem_ctx *ctx)
         ctx->gexec = nullptr;
     }
     int64_t total = 0;
+    HIPCHK(hipStreamSynchronize(ctx->stream));   // the last rebuild may still be running
     {
         int h = 0;
         HIPCHK(hipMemcpy(&h, ctx->nstart[ctx->lcur] + ctx->n, sizeof(int), hipMemcpyDeviceToHost));
@@ -1419,7 +1431,7 @@ extern "C" int dem_step(dem_ctx *ctx, int nsteps)
     if (!ctx) return fail(nullptr, DEM_ERR_ARG, "null context");
     if (!ctx->list_valid) return fail(ctx, DEM_ERR_ARG, "dem_step: call dem_setup first");
     if (nsteps <= 0) return DEM_OK;
-    hipSetDevice(ctx->p.device);
+    sync_enter(ctx);
     int rc;
     const bool gok = graph_ok(ctx);
     for (int st = 0; st < nsteps; st++) {
@@ -1456,7 +1468,7 @@ extern "C" int dem_step(dem_ctx *ctx, int nsteps)
 extern "C" int dem_download(dem_ctx *ctx, double *x, double *v, double *omega, double *f, double *torque)
 {
     if (!ctx) return fail(nullptr, DEM_ERR_ARG, "null context");
-    hipSetDevice(ctx->p.device);
+    sync_enter(ctx);
     const int n = ctx->n, c = ctx->cur;
     if (n == 0) return DEM_OK;
     std::vector<double4> hx(n), hv(n), hw(n), hf(n), ht(n);
@@ -1495,7 +1507,7 @@ extern "C" int dem_get_pairs(dem_ctx *ctx, int64_t *count, int *tag_i, int *tag_
                              double *hist, int *flag)
 {
     if (!ctx || !count) return fail(ctx, DEM_ERR_ARG, "dem_get_pairs: bad arguments");
-    hipSetDevice(ctx->p.device);
+    sync_enter(ctx);
     const int n = ctx->n, l = ctx->lcur, c = ctx->cur, dnum = ctx->M.dnum;
     if (!ctx->have_list || n == 0) {
         *count = 0;
@@ -1568,7 +1580,7 @@ extern "C" int dem_set_history(dem_ctx *ctx, int64_t count, const int *tag_i, co
 {
     if (!ctx || count < 0 || (count > 0 && (!tag_i || !tag_j || !hist)))
         return fail(ctx, DEM_ERR_ARG, "dem_set_history: bad arguments");
-    hipSetDevice(ctx->p.device);
+    sync_enter(ctx);
     const int n = ctx->n, c = ctx->cur, dnum = ctx->M.dnum;
     std::vector<int> tg(n);
     HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -1624,7 +1636,7 @@ extern "C" int dem_set_history(dem_ctx *ctx, int64_t count, const int *tag_i, co
 extern "C" int dem_get_wall_history(dem_ctx *ctx, double *hist)
 {
     if (!ctx || !hist) return fail(ctx, DEM_ERR_ARG, "dem_get_wall_history: bad arguments");
-    hipSetDevice(ctx->p.device);
+    sync_enter(ctx);
     const int n = ctx->n, c = ctx->cur, wdn = ctx->P.n * ctx->M.wdnum;
     if (n == 0 || wdn == 0) return DEM_OK;
     std::vector<double> h((size_t)n * wdn);
@@ -1640,7 +1652,7 @@ extern "C" int dem_get_wall_history(dem_ctx *ctx, double *hist)
 extern "C" int dem_set_wall_history(dem_ctx *ctx, const double *hist)
 {
     if (!ctx || !hist) return fail(ctx, DEM_ERR_ARG, "dem_set_wall_history: bad arguments");
-    hipSetDevice(ctx->p.device);
+    sync_enter(ctx);
     const int n = ctx->n, c = ctx->cur, wdn = ctx->P.n * ctx->M.wdnum;
     if (n == 0 || wdn == 0) return DEM_OK;
     std::vector<double> h((size_t)n * wdn);
@@ -1658,7 +1670,7 @@ extern "C" int dem_get_mesh_contacts(dem_ctx *ctx, int mesh, int64_t *count, int
 {
     if (!ctx || !count || mesh < 0 || mesh >= (int)ctx->meshes.size())
         return fail(ctx, DEM_ERR_ARG, "dem_get_mesh_contacts: bad arguments");
-    hipSetDevice(ctx->p.device);
+    sync_enter(ctx);
     MeshDev &m = ctx->meshes[mesh];
     const int n = ctx->n, dw = ctx->M.wdnum;
     if (!m.have || n == 0) {
@@ -1700,7 +1712,7 @@ extern "C" int dem_set_mesh_contacts(dem_ctx *ctx, int mesh, int64_t count, cons
     if (!ctx || mesh < 0 || mesh >= (int)ctx->meshes.size() || count < 0
         || (count > 0 && (!tag || !tri || !slot || (!hist && ctx->M.wdnum))))
         return fail(ctx, DEM_ERR_ARG, "dem_set_mesh_contacts: bad arguments");
-    hipSetDevice(ctx->p.device);
+    sync_enter(ctx);
     MeshDev &m = ctx->meshes[mesh];
     const int n = ctx->n, c = ctx->cur, dw = ctx->M.wdnum;
     std::vector<int> tg(n);
@@ -1824,7 +1836,7 @@ extern "C" int dem_set_instances(dem_ctx *ctx, int ninst, const int *inst)
         return fail(ctx, DEM_ERR_ARG, "dem_set_instances: call after dem_upload_state and before dem_setup");
     if (ninst > 1 && (!ctx->meshes.empty() || ctx->M.cohesion != CM_OFF))
         return fail(ctx, DEM_ERR_ARG, "dem_set_instances: mesh walls and bonds are not available in a batched run");
-    hipSetDevice(ctx->p.device);
+    sync_enter(ctx);
     std::vector<double4> hw(ctx->n);
     HIPCHK(hipMemcpy(hw.data(), ctx->wt[ctx->cur], sizeof(double4) * ctx->n, hipMemcpyDeviceToHost));
     for (int i = 0; i < ctx->n; i++) {
@@ -1849,7 +1861,7 @@ extern "C" int dem_set_instance_materials(dem_ctx *ctx, int inst, const double *
     if (!ctx) return fail(nullptr, DEM_ERR_ARG, "null context");
     if (inst < 0 || inst >= ctx->ninst || !Yeff || !Geff || !betaeff || !mu || !ctx->mat_set)
         return fail(ctx, DEM_ERR_ARG, "dem_set_instance_materials: bad arguments");
-    hipSetDevice(ctx->p.device);
+    sync_enter(ctx);
     const int w = ctx->ntypes + 1;
     const size_t T = (size_t)kT1 * kT1;
     double *dst = ctx->inst_mat.data() + (size_t)inst * (M_ETAR + 1) * T;
@@ -1869,7 +1881,7 @@ extern "C" int dem_observe_plane(dem_ctx *ctx, int plane)
     if (!ctx) return fail(nullptr, DEM_ERR_ARG, "null context");
     if (plane >= ctx->P.n || plane < -1) return fail(ctx, DEM_ERR_ARG, "dem_observe_plane: no such plane");
     if (ctx->n == 0) return fail(ctx, DEM_ERR_ARG, "dem_observe_plane: upload the atoms first");
-    hipSetDevice(ctx->p.device);
+    sync_enter(ctx);
     ctx->obs_plane = plane;
     if (plane >= 0) {
         if (dev_alloc(ctx, &ctx->pwf, ctx->cap)) return DEM_ERR_HIP;
@@ -1882,7 +1894,7 @@ extern "C" int dem_instance_observables(dem_ctx *ctx, double *out)
 {
     if (!ctx) return fail(nullptr, DEM_ERR_ARG, "null context");
     if (!out || !ctx->list_valid) return fail(ctx, DEM_ERR_ARG, "dem_instance_observables: call dem_setup first");
-    hipSetDevice(ctx->p.device);
+    sync_enter(ctx);
     const int B = ctx->ninst;
     if (B * 5 > ctx->obs_cap) {
         if (dev_alloc(ctx, &ctx->d_obs, (size_t)B * 5)) return DEM_ERR_HIP;
@@ -1910,7 +1922,7 @@ extern "C" int dem_autotune(dem_ctx *ctx, int reps, int *lanes, int *block, doub
 {
     if (!ctx) return fail(nullptr, DEM_ERR_ARG, "null context");
     if (!ctx->list_valid || reps < 1) return fail(ctx, DEM_ERR_ARG, "dem_autotune: call dem_setup first, reps >= 1");
-    hipSetDevice(ctx->p.device);
+    sync_enter(ctx);
     hipStream_t s = ctx->stream;
     const int n = ctx->n, l = ctx->lcur, c = ctx->cur;
     int64_t E = 0;
@@ -2015,7 +2027,7 @@ extern "C" int dem_adjoint_forward(dem_ctx *ctx, int nsteps, int checkpoint_ever
     if (nsteps < 1 || checkpoint_every < 1) return fail(ctx, DEM_ERR_ARG, "dem_adjoint_forward: nsteps, checkpoint_every >= 1");
     if (!ctx->list_valid || ctx->ntimestep != ctx->setup_ts)
         return fail(ctx, DEM_ERR_ARG, "dem_adjoint_forward: call right after dem_setup");
-    hipSetDevice(ctx->p.device);
+    sync_enter(ctx);
     for (char *c : ctx->ckpts) hipFree(c);
     ctx->ckpts.clear();
     ctx->ck_ts.clear();
@@ -2074,7 +2086,7 @@ extern "C" int dem_adjoint_backward(dem_ctx *ctx, const double *w_height, const 
         return fail(ctx, DEM_ERR_ARG, "dem_adjoint_backward: run dem_adjoint_forward first");
     if (w_wall && ctx->obs_plane < 0)
         return fail(ctx, DEM_ERR_ARG, "dem_adjoint_backward: a wall-force seed needs dem_observe_plane");
-    hipSetDevice(ctx->p.device);
+    sync_enter(ctx);
     hipStream_t s = ctx->stream;
     const int n = ctx->n, B = ctx->ninst, N = ctx->adj_steps, K = ctx->adj_every;
     const int nseg = (N + K - 1) / K;
@@ -2186,4 +2198,15 @@ extern "C" int dem_adjoint_backward(dem_ctx *ctx, const double *w_height, const 
     // leave the context at the window's final state
     if ((rc = restore_ckpt(ctx, nseg))) return done(rc);
     return done(check_err_flag(ctx));
+}
+
+extern "C" int dem_set_contact_shape(dem_ctx *ctx, int lanes, int block)
+{
+    if (!ctx) return fail(nullptr, DEM_ERR_ARG, "null context");
+    if (!(lanes == 0 || lanes == 1 || lanes == 2 || lanes == 4 || lanes == 8 || lanes == 16)
+        || !(block == 0 || block == 64 || block == 128 || block == 256))
+        return fail(ctx, DEM_ERR_ARG, "dem_set_contact_shape: lanes 1, 2, 4, 8, 16 and block 64, 128, 256 (0: default)");
+    ctx->tune_lanes = lanes;
+    ctx->tune_block = block;
+    return DEM_OK;
 }

@@ -221,6 +221,8 @@ void launch_neigh_count(const NeighArgs &a, hipStream_t s);
 void launch_row_max(const int *count, int n, int *maxcount, hipStream_t s);
 void launch_neigh_fill(const NeighArgs &a, hipStream_t s);
 void launch_fill_int(int *p, int v, int n, hipStream_t s);
+// err |= 8 and all rows emptied when nstart[n] > ecap (flag: one int of scratch)
+void launch_cap_guard(int *nstart, int n, int64_t ecap, int *flag, int *err, hipStream_t s);
 void launch_row_max_check(const int *maxcount, int limit, int *err, hipStream_t s);
 void launch_copy_int(int *dst, const int *src, int n, hipStream_t s);
 // out[b*5 + {0..4}] = count, sum z, sum of pwf, sum mass, kinetic energy of

@@ -450,6 +450,30 @@ void launch_row_max_check(const int *maxcount, int limit, int *err, hipStream_t 
     k_row_max_check<<<1, 64, 0, s>>>(maxcount, limit, err);
 }
 
+// A list longer than its buffers (graph stepping sizes them once): flag it
+// (err bit 3) and empty every row, so that no kernel of the step reads
+// entries that were not written.  The host reports the error after replay.
+__global__ void k_cap_flag(const int *nstart, int n, int64_t ecap, int *flag, int *err)
+{
+    if (threadIdx.x == 0) {
+        const int over = (int64_t)nstart[n] > ecap;
+        *flag = over;
+        if (over) atomicOr(err, 8);
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_cap_clear(int *nstart, int n, const int *flag)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i <= n && *flag) nstart[i] = 0;
+}
+
+void launch_cap_guard(int *nstart, int n, int64_t ecap, int *flag, int *err, hipStream_t s)
+{
+    k_cap_flag<<<1, 64, 0, s>>>(nstart, n, ecap, flag, err);
+    k_cap_clear<<<(n + 1 + kBlock - 1) / kBlock, kBlock, 0, s>>>(nstart, n, flag);
+}
+
 void launch_fill_int(int *p, int v, int n, hipStream_t s)
 {
     if (n <= 0) return;

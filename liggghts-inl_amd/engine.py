@@ -133,6 +133,7 @@ def library():
         "dem_use_graph": (C.c_int, [vp, C.c_int]),
         "dem_autotune": (C.c_int, [vp, C.c_int, ip, ip, dp]),
         "dem_adjoint_forward": (C.c_int, [vp, C.c_int, C.c_int]),
+        "dem_set_contact_shape": (C.c_int, [vp, C.c_int, C.c_int]),
         "dem_adjoint_backward": (C.c_int, [vp, dp, dp, dp]),
         "dem_add_mesh": (C.c_int, [vp, C.c_int, dp, C.c_int, C.c_double, dp]),
         "dem_mesh_size": (C.c_int, [vp, C.c_int]),
@@ -461,6 +462,9 @@ class Simulation:
         self._chk(self.L.dem_adjoint_backward(self.h, None if wh is None else _dp(wh),
                                               None if ww is None else _dp(ww), _dp(g)))
         return g
+
+    def set_contact_shape(self, lanes=0, block=0):
+        self._chk(self.L.dem_set_contact_shape(self.h, int(lanes), int(block)))
 
     def autotune(self, reps=5):
         g = C.c_int(0)
