@@ -1950,7 +1950,7 @@ extern "C" int dem_autotune(dem_ctx *ctx, int reps, int *lanes, int *block, doub
         return fail(ctx, DEM_ERR_HIP, "dem_autotune: copy failed");
     }
     const bool any_width = ctx->p.normal_model == 0 && ctx->p.wall_normal_model == 0 && ctx->M.cohesion == CM_OFF;
-    const int widths[] = {1, 2, 4, 8, 16};
+    const int widths[] = {1, 2, 4, 8, 16, -8, -16, -32, -64};   // negative: contact-queue tiles
     const int blocks[] = {64, 128, 256};
     hipEvent_t e0, e1;
     hipEventCreate(&e0);
@@ -2203,9 +2203,12 @@ extern "C" int dem_adjoint_backward(dem_ctx *ctx, const double *w_height, const 
 extern "C" int dem_set_contact_shape(dem_ctx *ctx, int lanes, int block)
 {
     if (!ctx) return fail(nullptr, DEM_ERR_ARG, "null context");
-    if (!(lanes == 0 || lanes == 1 || lanes == 2 || lanes == 4 || lanes == 8 || lanes == 16)
+    if (!(lanes == 0 || lanes == 1 || lanes == 2 || lanes == 4 || lanes == 8 || lanes == 16 || lanes == -8
+          || lanes == -16 || lanes == -32 || lanes == -64)
         || !(block == 0 || block == 64 || block == 128 || block == 256))
-        return fail(ctx, DEM_ERR_ARG, "dem_set_contact_shape: lanes 1, 2, 4, 8, 16 and block 64, 128, 256 (0: default)");
+        return fail(ctx, DEM_ERR_ARG,
+                    "dem_set_contact_shape: lanes 1, 2, 4, 8, 16 (or -8, -16, -32: contact queue with tiles of that "
+                    "many particles) and block 64, 128, 256 (0: default)");
     ctx->tune_lanes = lanes;
     ctx->tune_block = block;
     return DEM_OK;

@@ -201,3 +201,26 @@ def test_neighbor_overflow_error():
     s = eng.Simulation(spec, atoms)
     with pytest.raises(eng.DemError, match="Neighbor list overflow"):
         s.setup()
+
+
+@pytest.mark.parametrize("case", ["pour", "column"])
+def test_contact_shapes_agree(case):
+    """Every contact-pass shape (lane groups 1..16, contact-queue tiles
+    8..64, blocks 64..256) gives the oracle's forces on an identical state;
+    the shapes differ only in the order each particle's contacts are
+    summed."""
+    spec, atoms, o = pour_state() if case == "pour" else column_state(n=6000, steps=40)
+    for lanes, block in ((4, 256), (1, 64), (16, 128), (-8, 256), (-16, 128), (-32, 256), (-64, 256)):
+        eng = load_engine()
+        s = eng.Simulation(spec, snapshot_atoms(atoms, o))
+        s.set_contact_shape(lanes, block)
+        ti, tj, sh, hi, fl = o.pairs()
+        k = fl != 0
+        s.set_history(ti[k], tj[k], hi[k])
+        s.set_wall_history(o.wall_history())
+        s.setup()
+        o2 = o   # the oracle's set-up pass on the same state
+        o2.rebuild()
+        o2.compute_forces(0)
+        check_forces(s, o2, 1e-12)
+        s.close()
