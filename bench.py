@@ -27,10 +27,10 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per particle of one mixed contact pass on this workload, from the
-# FETCH_SIZE and WRITE_SIZE counter passes of rocprofv3 on k_force<true,4,0,0>
+# FETCH_SIZE and WRITE_SIZE counter passes of rocprofv3 on k_force_q<true,32>
 # (FETCH_SIZE scaled x2.0, WRITE_SIZE x1.0 against a 256 MiB device copy):
 # profiles/pmc_traffic_k_force_column1M.md.  Re-measured when the kernel changes.
-MEASURED_TRAFFIC_BPP = 847.7
+MEASURED_TRAFFIC_BPP = 851.1
 
 
 def package():
@@ -268,7 +268,7 @@ def main():
             "traffic": MEASURED_TRAFFIC_BPP * n if main_prec == "mixed" else None,
             "traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE passes, copy-calibrated "
                               "(profiles/pmc_traffic_k_force_column1M.md)",
-            "kernel": "k_force (contact pass)",
+            "kernel": "k_force_q<mixed, 32> (contact pass)",
             "bytes_per_particle": bpp,
             "avg_launch_ms": avg_ms,
         },
