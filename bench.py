@@ -178,6 +178,38 @@ def bench_batch(pkg, args):
     print(json.dumps(line))
 
 
+def bench_fibres(pkg, args):
+    """Bonded fibres (500 chains x 40 spheres, bond/nonlinear along the
+    chains, hysteretic/nonlinear1 contact, floor plane, plate mesh moving
+    down at 0.05 m/s, dt 1e-7), FP64 (bonds and hysteretic chains run in
+    FP64).  Reported beside the headline metric."""
+    spec, atoms = pkg.workloads.fibres()
+    n = len(atoms["radius"])
+    s = pkg.Simulation(spec, atoms, precision="fp64")
+    s.setup()
+    s.step(args.warmup)
+    s.sync()
+    t0 = time.perf_counter()
+    s.step(args.steps)
+    s.sync()
+    dt = time.perf_counter() - t0
+    hist = s.pairs()[3]
+    bonds = int((hist[:, 12] > 0.5).sum()) // 2
+    s.close()
+    print("fibres: %.3f s for %d steps, %d bonds intact" % (dt, args.steps, bonds), file=sys.stderr, flush=True)
+    line = {
+        "metric": "particle-steps/s, bonded fibres under a moving plate", "value": n * args.steps / dt,
+        "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": dt * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "500 chains x 40 spheres r 0.5 mm in a 40 mm cube (fibre mat), bond/nonlinear "
+                               "(K_fn2 1e10 N/m3, break at 5 % strain), hysteretic/nonlinear1, floor plane, plate "
+                               "mesh at -0.05 m/s, dt 1e-7, gravity", "n_particles": n,
+                   "bonds_created": 19500, "bonds_intact": bonds},
+    }
+    print(json.dumps(line))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -187,8 +219,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-check", action="store_true", help="skip the mixed-vs-oracle force check")
     ap.add_argument("--only", choices=["mixed", "fp64"], default=None)
-    ap.add_argument("--workload", choices=["column", "batch"], default="column",
-                    help="column: the headline 1M-sphere metric; batch: 256 batched pours")
+    ap.add_argument("--workload", choices=["column", "batch", "fibres"], default="column",
+                    help="column: the headline 1M-sphere metric; batch: 256 batched pours; fibres: bonded "
+                         "fibres under a moving plate")
     ap.add_argument("--instances", type=int, default=256)
     ap.add_argument("--adjoint", action="store_true", help="batch: also time the adjoint window (FP64)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MB",
@@ -215,6 +248,9 @@ def main():
     pkg = package()
     if args.workload == "batch":
         bench_batch(pkg, args)
+        return
+    if args.workload == "fibres":
+        bench_fibres(pkg, args)
         return
     t0 = time.perf_counter()
     spec, atoms = pkg.workloads.packed_column(n=args.n)

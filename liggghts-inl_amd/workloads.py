@@ -244,3 +244,79 @@ def instance(spec, atoms, inst, params, b):
     s = dict(spec, materials=m)
     s.pop("instances", None)
     return s, a
+
+
+def fibres(nchains=500, nsph=40, seed=4, L=0.04, r=0.5e-3, plate_gap=-2e-6):
+    """Bonded-sphere fibres under a moving plate: nchains straight chains of
+    nsph touching spheres (r = 0.5 mm, rho 2500) in an L-sized cube, laid as
+    a fibre mat: layers 1.4 d apart in z, each layer's chains parallel to x
+    or to y (chosen per layer from the seed) at 1.5 d pitch, each chain with
+    its own small random tilt (up to 3 mrad in the layer, 1 mrad out of it)
+    and shift along its axis.  No two chains come closer than 1.15 d, so bonds
+    form only along chains.  Pair chain hysteretic/nonlinear1 + tangential
+    history + bond/nonlinear, bonds created at step 1 within 1.1 d; tension
+    stiffness K_fn2 = 1e10 N/m^3 (k A d), unloading 2x and reloading 0.5x
+    (the yield hysteresis), break beyond 1.05 d0 (5 % tensile strain); the
+    compression branch is K_fn1 |d|^(1/2) with K_fn1 = 8 N/m^(1/2).  Hertz
+    walls: a fixed floor plane and a plate (two triangles) moving down at
+    0.05 m/s, starting plate_gap above the highest sphere's top (negative:
+    pressing).  dt = 1e-7, gravity."""
+    rng = np.random.default_rng(seed)
+    d = 2 * r
+    span = (nsph - 1) * d
+    pitch = 1.5 * d
+    per_layer = int((L - 2 * r - 0.001) // pitch)
+    pts = []
+    layer = 0
+    while len(pts) < nchains:
+        axis = int(rng.integers(0, 2))
+        z = r + 0.0005 + layer * 1.4 * d
+        if z + r > L:
+            raise RuntimeError("fibres: the mat does not fit the box")
+        for c in range(per_layer):
+            if len(pts) == nchains:
+                break
+            u = np.zeros(3)
+            u[axis] = 1.0
+            u[1 - axis] = rng.uniform(-3e-3, 3e-3)
+            u[2] = rng.uniform(-1e-3, 1e-3)
+            u /= np.linalg.norm(u)
+            centre = np.zeros(3)
+            centre[axis] = L / 2 + rng.uniform(-0.2, 0.2) * (L - span - 2 * r) / 2
+            centre[1 - axis] = r + 0.0005 + c * pitch + 0.5 * pitch
+            centre[2] = z + span / 2 * abs(u[2]) + 1e-5
+            pts.append(centre + np.outer(np.arange(nsph) - (nsph - 1) / 2.0, u) * d)
+        layer += 1
+    x = np.concatenate(pts)
+    n = len(x)
+    rad = np.full(n, r)
+    atoms = _atoms(x, rad, rng, 0.0)
+    w = 3
+    full = lambda v: np.full((w, w), v)  # noqa: E731
+    mats = _materials(Y=5e6, nu=0.3, e=0.5, mu=0.5, ntypes=2)
+    mats["hysteretic"] = {k: full(v) for k, v in dict(
+        LoadingStiffness=5e4, UnloadingStiffness=2.5, coefficientAdhesionStiffness=0.0,
+        coefficientPlasticityDepth=0.1, pullOffForce=0.0, alphaCustom=20.0, cinCustom=1e-7,
+        aoneCustom=10e8, atwoCustom=4e4, athreeCustom=8.0, kcinCustom=1e-4).items()}
+    kt = 1e10
+    bond = dict(radiusMultiplier=full(1.0), createDistance=full(1.1 * d), maxDistance=full(1.05 * d),
+                maxSigma=full(1e12), maxTau=full(1e12), ratioTensionCompression=full(1.0),
+                dampingNormalForce=full(0.0), dampingTangentialForce=full(0.0), dampingNormalTorque=full(0.0),
+                dampingTangentialTorque=full(0.0),
+                K_fn1=full(8.0), Ku_fn1=full(16.0), Kc_fn1=full(4.0),
+                K_fn2=full(kt), Ku_fn2=full(2 * kt), Kc_fn2=full(0.5 * kt),
+                K_ft=full(0.4 * kt), K_tn=full(0.4 * kt), Ku_tn=full(0.8 * kt), Kc_tn=full(0.2 * kt),
+                K_tt=full(kt), Ku_tt=full(2 * kt), Kc_tt=full(0.5 * kt))
+    ztop = x[:, 2].max() + r + plate_gap
+    plate = np.array([[[0.0, 0.0, ztop], [L, 0.0, ztop], [L, L, ztop]],
+                      [[0.0, 0.0, ztop], [L, L, ztop], [0.0, L, ztop]]])
+    spec = dict(
+        box_lo=[0.0, 0.0, 0.0], box_hi=[L, L, L + 0.005], periodic=[0, 0, 0], dt=1e-7, skin=2e-4, cdf=1.0,
+        neigh_every=1, neigh_delay=0, neigh_check=True, rolling="off",
+        normal="hysteretic/nonlinear1", wall_normal="hertz", cohesion="bond/nonlinear",
+        materials=mats, planes=[(2, 0.0, 2)],
+        meshes=[dict(nodes=plate, type=2, vel=(0.0, 0.0, -0.05))],
+        bond=dict(tables=bond, ts_create=1.0),
+        gravity=(0.0, 0.0, -9.81),
+    )
+    return spec, atoms
