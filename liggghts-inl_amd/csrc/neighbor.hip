@@ -22,6 +22,7 @@
 #include "kernels.h"
 
 #include <algorithm>
+#include <stdlib.h>
 
 namespace dem {
 
@@ -226,7 +227,8 @@ void launch_permute(const PermuteArgs &a, hipStream_t s)
 
 // ---------------------------------------------------------------------------
 
-// One wavefront per particle.  Lanes 0..26 each own one cell of the 27-cell
+// One group of W lanes per particle (a wavefront, or half of one).  Lanes
+// 0..26 of the group each own one cell of the 27-cell
 // stencil (dz, dy, dx ascending, as a sequential scan visits them); the
 // cells' candidates are enumerated 64 at a time, tested in parallel, and the
 // passing ones compacted with a ballot + popcount prefix.  The row therefore
@@ -236,22 +238,28 @@ void launch_permute(const PermuteArgs &a, hipStream_t s)
 // carry nothing are written with flag 0 and no history (a flag-0 entry has
 // never been in contact, so its history is zero by definition and the
 // contact pass does not read it).
-constexpr int kNeighWaves = kBlock / 64;
-
-template <bool FILL>
+template <bool FILL, int W>
 __global__ void __launch_bounds__(kBlock) k_neigh(NeighArgs a)
 {
-    __shared__ int s_ex[kNeighWaves][28], s_qb[kNeighWaves][27], s_code[kNeighWaves][27];
-    const int lane = threadIdx.x & 63;
-    const int wv = threadIdx.x >> 6;
-    const int p = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    if (p >= a.n) return;  // p is uniform across the wavefront
+    static_assert(W == 16 || W == 32 || W == 64, "16, 32 or 64 lanes per particle");
+    constexpr int kGroups = kBlock / W;
+    constexpr int kShift = W == 64 ? 6 : (W == 32 ? 5 : 4);
+    __shared__ int s_ex[kGroups][28], s_qb[kGroups][27], s_code[kGroups][27];
+    const int lane = threadIdx.x & (W - 1);            // lane within the particle's group
+    const int wv = threadIdx.x >> kShift;               // group within the block
+    const int gshift = (threadIdx.x & 63) & ~(W - 1);   // the group's first bit in a wave ballot
+    const uint64_t gmask = W == 64 ? ~0ull : ((((1ull << W) - 1ull)) << gshift);
+    const int p = (blockIdx.x * blockDim.x + threadIdx.x) >> kShift;
+    if (p >= a.n) return;  // p is uniform across the group
     const double4 xp = a.xr[p];
     const int inst = meta_inst((int)a.wt[p].w);
     const int cc[3] = {cell_coord(xp.x, 0, a.G), cell_coord(xp.y, 1, a.G), cell_coord(xp.z, 2, a.G)};
-    int qb = 0, cnt = 0, code = kNoShift;
-    if (lane < 27) {
-        const int off[3] = {lane % 3 - 1, (lane / 3) % 3 - 1, lane / 9 - 1};
+    // stencil cell k (dz, dy, dx ascending): first particle, count, image code
+    auto cell_info = [&](int k, int &qb, int &cnt, int &code) {
+        qb = 0;
+        cnt = 0;
+        code = kNoShift;
+        const int off[3] = {k % 3 - 1, (k / 3) % 3 - 1, k / 9 - 1};
         int c[3], sh[3];
         bool ok = true;
         for (int d = 0; d < 3; d++) {
@@ -271,17 +279,37 @@ __global__ void __launch_bounds__(kBlock) k_neigh(NeighArgs a)
             cnt = a.cstart[key + 1] - qb;
             code = (sh[0] + 1) + 3 * (sh[1] + 1) + 9 * (sh[2] + 1);
         }
-    }
-    int incl = cnt;
-    for (int o = 1; o < 64; o <<= 1) {
-        const int t = __shfl_up(incl, o, 64);
-        if (lane >= o) incl += t;
-    }
-    const int total = __shfl(incl, 63, 64);
+    };
+    auto scan = [&](int v) {   // inclusive prefix over the group
+        int incl = v;
+        for (int o = 1; o < W; o <<= 1) {
+            const int t = __shfl_up(incl, o, W);
+            if (lane >= o) incl += t;
+        }
+        return incl;
+    };
+    // cells lane and lane + W (W = 16: two cells per lane)
+    int qb = 0, cnt = 0, code = kNoShift;
+    if (lane < 27) cell_info(lane, qb, cnt, code);
+    const int incl = scan(cnt);
+    const int total_a = __shfl(incl, W - 1, W);
+    int total = total_a;
     if (lane < 27) {
         s_ex[wv][lane] = incl - cnt;
         s_qb[wv][lane] = qb;
         s_code[wv][lane] = code;
+    }
+    if (W < 27) {
+        const int k2 = lane + W;
+        int qb2 = 0, cnt2 = 0, code2 = kNoShift;
+        if (k2 < 27) cell_info(k2, qb2, cnt2, code2);
+        const int incl2 = scan(cnt2);
+        total = total_a + __shfl(incl2, W - 1, W);
+        if (k2 < 27) {
+            s_ex[wv][k2] = total_a + incl2 - cnt2;
+            s_qb[wv][k2] = qb2;
+            s_code[wv][k2] = code2;
+        }
     }
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -299,7 +327,7 @@ __global__ void __launch_bounds__(kBlock) k_neigh(NeighArgs a)
             if (ow & (kFlagBit | kBondBit)) otag_l = a.otag[ow & kJMask];
         }
     }
-    const int on64 = min(oe - ob, 64);
+    const int on64 = min(oe - ob, W);
     const int ebase = FILL ? a.nstart[p] : 0;
     // a list longer than the allocation (possible only when the rebuild runs
     // inside a captured graph, sized beforehand) is reported, not written
@@ -308,7 +336,7 @@ __global__ void __launch_bounds__(kBlock) k_neigh(NeighArgs a)
         return;
     }
     int written = 0;
-    for (int c0 = 0; c0 < total; c0 += 64) {
+    for (int c0 = 0; c0 < total; c0 += W) {
         const int c = c0 + lane;
         bool pass = false, touch = false;
         int q = 0, qcode = kNoShift;
@@ -338,7 +366,7 @@ __global__ void __launch_bounds__(kBlock) k_neigh(NeighArgs a)
                 touch = rsq < radsum * radsum;
             }
         }
-        const uint64_t m = __ballot(pass);
+        const uint64_t m = (__ballot(pass) & gmask) >> gshift;
         if (FILL) {
             // first flagged entry of the previous row with the partner's tag
             // (the search of neigh_gran.cpp:596-598), rows held in lanes
@@ -346,11 +374,11 @@ __global__ void __launch_bounds__(kBlock) k_neigh(NeighArgs a)
             const int tq = want ? a.tag[q] : -1;
             int found = -1;
             for (int k = 0; k < on64; k++) {
-                const int t = __shfl(otag_l, k, 64);
+                const int t = __shfl(otag_l, k, W);
                 if (found < 0 && t == tq) found = k;
             }
             if (want && found < 0)
-                for (int oe2 = ob + 64; oe2 < oe; oe2++) {
+                for (int oe2 = ob + W; oe2 < oe; oe2++) {
                     const uint32_t ow = a.onj[oe2];
                     if ((ow & (kFlagBit | kBondBit)) && a.otag[ow & kJMask] == tq) {
                         found = oe2 - ob;
@@ -398,15 +426,34 @@ __global__ void __launch_bounds__(kBlock) k_row_max(const int *count, int n, int
     }
 }
 
-static int wave_blocks(int n)
+// lanes per particle of the neighbour build: DEM_NEIGH_LANES (16, 32 or
+// 64), default 16 (1M-sphere column: 1.86 ms per rebuild against 1.98 with
+// 32 lanes and 2.34 with 64; the lists are identical)
+static int neigh_lanes()
 {
-    return (int)(((long long)n + kNeighWaves - 1) / kNeighWaves);
+    static int w = 0;
+    if (w == 0) {
+        const char *s = getenv("DEM_NEIGH_LANES");
+        const int v = s ? atoi(s) : 16;
+        w = (v == 32 || v == 64) ? v : 16;
+    }
+    return w;
+}
+
+static int group_blocks(int n, int W)
+{
+    const int per_block = kBlock / W;
+    return (int)(((long long)n + per_block - 1) / per_block);
 }
 
 void launch_neigh_count(const NeighArgs &a, hipStream_t s)
 {
     if (a.n == 0) return;
-    k_neigh<false><<<wave_blocks(a.n), kBlock, 0, s>>>(a);
+    switch (neigh_lanes()) {
+    case 64: k_neigh<false, 64><<<group_blocks(a.n, 64), kBlock, 0, s>>>(a); break;
+    case 32: k_neigh<false, 32><<<group_blocks(a.n, 32), kBlock, 0, s>>>(a); break;
+    default: k_neigh<false, 16><<<group_blocks(a.n, 16), kBlock, 0, s>>>(a); break;
+    }
     launch_row_max(a.count, a.n, a.maxcount, s);
 }
 
@@ -420,7 +467,11 @@ void launch_row_max(const int *count, int n, int *maxcount, hipStream_t s)
 void launch_neigh_fill(const NeighArgs &a, hipStream_t s)
 {
     if (a.n == 0) return;
-    k_neigh<true><<<wave_blocks(a.n), kBlock, 0, s>>>(a);
+    switch (neigh_lanes()) {
+    case 64: k_neigh<true, 64><<<group_blocks(a.n, 64), kBlock, 0, s>>>(a); break;
+    case 32: k_neigh<true, 32><<<group_blocks(a.n, 32), kBlock, 0, s>>>(a); break;
+    default: k_neigh<true, 16><<<group_blocks(a.n, 16), kBlock, 0, s>>>(a); break;
+    }
 }
 
 // plain fills and copies: what a rebuild would otherwise do with
