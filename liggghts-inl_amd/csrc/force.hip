@@ -289,7 +289,7 @@ __global__ void __launch_bounds__(kBlock) k_force_q(ForceArgs a)
     __shared__ double4 s_x[kQWaves][P];
     __shared__ int s_q[kQWaves][128], s_qo[kQWaves][128];
     __shared__ uint32_t s_qw[kQWaves][128];          // list word of the queued entry
-    __shared__ double s_qd[kQWaves][3][128];         // its separation x_i - x_j
+    __shared__ double s_qd[kQWaves][4][128];         // its separation x_i - x_j, partner radius
     // per-contact results; the mixed law's are FP32 values, kept exactly
     using OutT = typename std::conditional<MIXED, float, double>::type;
     __shared__ OutT s_out[kQWaves][6][64];
@@ -332,7 +332,7 @@ __global__ void __launch_bounds__(kBlock) k_force_q(ForceArgs a)
             // independent loads, all in flight together
             const double4 vmj = a.vm[j];
             const double4 wj = a.wt[j];
-            const double radj = a.xr[j].w;
+            const double radj = s_qd[wv][3][lane];
             const double4 pi = s_x[wv][k], vmi = a.vm[p0 + k], wi = a.wt[p0 + k];
             const int metai = (int)wi.w;
             const double *mat = a.mat + (size_t)meta_inst(metai) * kMatStride;
@@ -399,7 +399,7 @@ __global__ void __launch_bounds__(kBlock) k_force_q(ForceArgs a)
         bool touch = false;
         int owner = 0;
         uint32_t w = 0;
-        double dx = 0.0, dy = 0.0, dz = 0.0;
+        double dx = 0.0, dy = 0.0, dz = 0.0, rj = 0.0;
         if (e < E1) {
             int lo = 0, hi = np - 1;   // owner: the last row starting at or before e
             while (lo < hi) {
@@ -421,6 +421,7 @@ __global__ void __launch_bounds__(kBlock) k_force_q(ForceArgs a)
             dy = pi.y - xj1;
             dz = pi.z - xj2;
             const double rsq = dx * dx + dy * dy + dz * dz;
+            rj = pj.w;
             const double radsum = pi.w + pj.w;
             touch = rsq < radsum * radsum;
             if (!touch && rsq < cdm * radsum * radsum) {
@@ -437,6 +438,7 @@ __global__ void __launch_bounds__(kBlock) k_force_q(ForceArgs a)
             s_qd[wv][0][pos] = dx;
             s_qd[wv][1][pos] = dy;
             s_qd[wv][2][pos] = dz;
+            s_qd[wv][3][pos] = rj;
         }
         qlen += __popcll(bm);
         wave_sync();
@@ -445,7 +447,7 @@ __global__ void __launch_bounds__(kBlock) k_force_q(ForceArgs a)
             const int rest = qlen - 64;
             int tq = 0, to = 0;
             uint32_t tw = 0;
-            double t0 = 0.0, t1 = 0.0, t2 = 0.0;
+            double t0 = 0.0, t1 = 0.0, t2 = 0.0, t3 = 0.0;
             if (lane < rest) {
                 tq = s_q[wv][64 + lane];
                 to = s_qo[wv][64 + lane];
@@ -453,6 +455,7 @@ __global__ void __launch_bounds__(kBlock) k_force_q(ForceArgs a)
                 t0 = s_qd[wv][0][64 + lane];
                 t1 = s_qd[wv][1][64 + lane];
                 t2 = s_qd[wv][2][64 + lane];
+                t3 = s_qd[wv][3][64 + lane];
             }
             wave_sync();
             if (lane < rest) {
@@ -462,6 +465,7 @@ __global__ void __launch_bounds__(kBlock) k_force_q(ForceArgs a)
                 s_qd[wv][0][lane] = t0;
                 s_qd[wv][1][lane] = t1;
                 s_qd[wv][2][lane] = t2;
+                s_qd[wv][3][lane] = t3;
             }
             wave_sync();
             qlen = rest;

@@ -245,6 +245,7 @@ __global__ void __launch_bounds__(kBlock) k_neigh(NeighArgs a)
     constexpr int kGroups = kBlock / W;
     constexpr int kShift = W == 64 ? 6 : (W == 32 ? 5 : 4);
     __shared__ int s_ex[kGroups][28], s_qb[kGroups][27], s_code[kGroups][27];
+    __shared__ int s_ot[kGroups][64];                   // partner tags of the previous row (fill)
     const int lane = threadIdx.x & (W - 1);            // lane within the particle's group
     const int wv = threadIdx.x >> kShift;               // group within the block
     const int gshift = (threadIdx.x & 63) & ~(W - 1);   // the group's first bit in a wave ballot
@@ -315,19 +316,22 @@ __global__ void __launch_bounds__(kBlock) k_neigh(NeighArgs a)
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 
-    // the particle's previous row, one entry per lane: partner tag of every
-    // flagged entry (0 = cannot match; tags are positive)
-    int ob = 0, oe = 0, otag_l = 0;
+    // the particle's previous row into LDS: partner tag of every flagged
+    // entry (0 = cannot match; tags are positive), its first 64 entries
+    int ob = 0, oe = 0;
     if (FILL && a.have_old) {
         const int o = a.perm[p];
         ob = a.onstart[o];
         oe = a.onstart[o + 1];
-        if (lane < oe - ob) {
-            const uint32_t ow = a.onj[ob + lane];
-            if (ow & (kFlagBit | kBondBit)) otag_l = a.otag[ow & kJMask];
+        for (int k = lane; k < min(oe - ob, 64); k += W) {
+            const uint32_t ow = a.onj[ob + k];
+            s_ot[wv][k] = (ow & (kFlagBit | kBondBit)) ? a.otag[ow & kJMask] : 0;
         }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
-    const int on64 = min(oe - ob, W);
+    const int on64 = min(oe - ob, 64);
     const int ebase = FILL ? a.nstart[p] : 0;
     // a list longer than the allocation (possible only when the rebuild runs
     // inside a captured graph, sized beforehand) is reported, not written
@@ -369,22 +373,26 @@ __global__ void __launch_bounds__(kBlock) k_neigh(NeighArgs a)
         const uint64_t m = (__ballot(pass) & gmask) >> gshift;
         if (FILL) {
             // first flagged entry of the previous row with the partner's tag
-            // (the search of neigh_gran.cpp:596-598), rows held in lanes
+            // (the search of neigh_gran.cpp:596-598): only the lanes of
+            // touching candidates search, in the LDS copy of the row
             const bool want = pass && touch && a.have_old;
-            const int tq = want ? a.tag[q] : -1;
             int found = -1;
-            for (int k = 0; k < on64; k++) {
-                const int t = __shfl(otag_l, k, W);
-                if (found < 0 && t == tq) found = k;
-            }
-            if (want && found < 0)
-                for (int oe2 = ob + W; oe2 < oe; oe2++) {
-                    const uint32_t ow = a.onj[oe2];
-                    if ((ow & (kFlagBit | kBondBit)) && a.otag[ow & kJMask] == tq) {
-                        found = oe2 - ob;
+            if (want) {
+                const int tq = a.tag[q];
+                for (int k = 0; k < on64; k++)
+                    if (s_ot[wv][k] == tq) {
+                        found = k;
                         break;
                     }
-                }
+                if (found < 0)
+                    for (int oe2 = ob + 64; oe2 < oe; oe2++) {
+                        const uint32_t ow = a.onj[oe2];
+                        if ((ow & (kFlagBit | kBondBit)) && a.otag[ow & kJMask] == tq) {
+                            found = oe2 - ob;
+                            break;
+                        }
+                    }
+            }
             if (pass) {
                 const int e = ebase + written + __popcll(m & ((1ull << lane) - 1ull));
                 uint32_t w = (uint32_t)q | ((uint32_t)qcode << 26);
