@@ -394,11 +394,25 @@ __global__ void __launch_bounds__(kBlock) k_force_q(ForceArgs a)
         wave_sync();
     };
 
+    // software pipeline: the next 64 entries' list words and partner
+    // positions are in flight while this group is tested and queued (and
+    // while a batch of contacts is evaluated)
+    uint32_t w_next = 0;
+    double4 pj_next = make_double4(0.0, 0.0, 0.0, 0.0);
+    if (E0 + lane < E1) {
+        w_next = a.nj[E0 + lane];
+        pj_next = a.xr[w_next & kJMask];
+    }
     for (int base = E0; base < E1; base += 64) {
         const int e = base + lane;
         bool touch = false;
         int owner = 0;
-        uint32_t w = 0;
+        uint32_t w = w_next;
+        const double4 pj = pj_next;
+        if (e + 64 < E1) {
+            w_next = a.nj[e + 64];
+            pj_next = a.xr[w_next & kJMask];
+        }
         double dx = 0.0, dy = 0.0, dz = 0.0, rj = 0.0;
         if (e < E1) {
             int lo = 0, hi = np - 1;   // owner: the last row starting at or before e
@@ -408,10 +422,8 @@ __global__ void __launch_bounds__(kBlock) k_force_q(ForceArgs a)
                 else hi = mid - 1;
             }
             owner = lo;
-            w = a.nj[e];
             const int code = (int)((w >> 26) & 31u);
             const double4 pi = s_x[wv][owner];
-            const double4 pj = a.xr[w & kJMask];
             double xj0 = pj.x, xj1 = pj.y, xj2 = pj.z;
             if (code != kNoShift) {
                 const double3 im = image_of(pj, code, M.prd);
