@@ -2100,7 +2100,6 @@ extern "C" int dem_adjoint_backward(dem_ctx *ctx, const double *w_height, const 
     const size_t total = 6 * b4 + 2 * bh + 2 * bw + bg + 2 * bs + r256((size_t)B * 4 * sizeof(double))
                          + r256((size_t)B * 5 * sizeof(double));
     HIPCHK(hipMalloc((void **)&buf, total));
-    HIPCHK(hipMemsetAsync(buf, 0, total, s));
     char *q = buf;
     auto take = [&](size_t b) {
         char *r = q;
@@ -2123,13 +2122,22 @@ extern "C" int dem_adjoint_backward(dem_ctx *ctx, const double *w_height, const 
         ctx->tape_pos = -1;
         return code;
     };
-    if (w_height) HIPCHK(hipMemcpyAsync(d_wh, w_height, sizeof(double) * B, hipMemcpyHostToDevice, s));
-    if (w_wall) HIPCHK(hipMemcpyAsync(d_ww, w_wall, sizeof(double) * B, hipMemcpyHostToDevice, s));
+    // a failed call from here on frees the buffers before it returns
+    auto hip_ok = [&](hipError_t e, const char *what) {
+        if (e == hipSuccess) return true;
+        fail(ctx, DEM_ERR_HIP, std::string("dem_adjoint_backward: ") + what + ": " + hipGetErrorString(e));
+        return false;
+    };
+    if (!hip_ok(hipMemsetAsync(buf, 0, total, s), "clear")) return done(DEM_ERR_HIP);
+    if (w_height && !hip_ok(hipMemcpyAsync(d_wh, w_height, sizeof(double) * B, hipMemcpyHostToDevice, s), "seed"))
+        return done(DEM_ERR_HIP);
+    if (w_wall && !hip_ok(hipMemcpyAsync(d_ww, w_wall, sizeof(double) * B, hipMemcpyHostToDevice, s), "seed"))
+        return done(DEM_ERR_HIP);
     // seed at the final state: the mean height's gradient
     if ((rc = restore_ckpt(ctx, nseg))) return done(rc);
     launch_inst_obs(ctx->cstart, ctx->G.nkeys_local, B, ctx->xr[ctx->cur], ctx->vm[ctx->cur], nullptr, d_cnt, s);
     launch_adj_seed(ctx->wt[ctx->cur], w_height ? d_wh : nullptr, d_cnt, lx[0], n, s);
-    KCHK();
+    if (!hip_ok(hipGetLastError(), "seed kernel")) return done(DEM_ERR_HIP);
     // tape for the longest segment
     const size_t tb = state_record_bytes(ctx, false);
     if (K > ctx->tape_cap) {
@@ -2193,8 +2201,9 @@ extern "C" int dem_adjoint_backward(dem_ctx *ctx, const double *w_height, const 
         step_back(v, 0, 0, 0.5 * ctx->p.dt, nullptr);
     }
     launch_adj_reduce(ctx->cstart, ctx->G.nkeys_local, B, gp, d_grad, s);
-    KCHK();
-    HIPCHK(hipMemcpyAsync(grad, d_grad, sizeof(double) * B * 4, hipMemcpyDeviceToHost, s));
+    if (!hip_ok(hipGetLastError(), "reduce kernel")
+        || !hip_ok(hipMemcpyAsync(grad, d_grad, sizeof(double) * B * 4, hipMemcpyDeviceToHost, s), "copy"))
+        return done(DEM_ERR_HIP);
     // leave the context at the window's final state
     if ((rc = restore_ckpt(ctx, nseg))) return done(rc);
     return done(check_err_flag(ctx));
