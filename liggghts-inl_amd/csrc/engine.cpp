@@ -124,6 +124,7 @@ struct dem_ctx {
     int nkeys_cap = 0;
     int *scan_tmp = nullptr;
     int scan_tmp_cap = 0;
+    int2 *stage = nullptr;   // n * kStageSlots: staging rows of the single-search rebuild
     uint32_t *spread = nullptr;
     int spread_cap = 0;
 
@@ -357,7 +358,7 @@ static void free_all(dem_ctx *ctx)
         F(ctx->img[b]); F(ctx->whist[b]); F(ctx->nstart[b]); F(ctx->nj[b]); F(ctx->hist[b]);
     }
     F(ctx->f4); F(ctx->t4); F(ctx->xhold); F(ctx->wmask); F(ctx->ncontact); F(ctx->key);
-    F(ctx->sorted); F(ctx->cstart); F(ctx->cursor); F(ctx->scan_tmp); F(ctx->spread);
+    F(ctx->sorted); F(ctx->cstart); F(ctx->cursor); F(ctx->scan_tmp); F(ctx->spread); F(ctx->stage);
     F(ctx->d_mat); F(ctx->d_small);
     F(ctx->bout); F(ctx->bvalid); F(ctx->brev);
     F(ctx->pwf); F(ctx->d_obs);
@@ -761,7 +762,8 @@ extern "C" int dem_upload_state(dem_ctx *ctx, int n, const double *x, const doub
         }
         if (dev_alloc(ctx, &ctx->f4, c) || dev_alloc(ctx, &ctx->t4, c) || dev_alloc(ctx, &ctx->xhold, c)
             || dev_alloc(ctx, &ctx->wmask, c) || dev_alloc(ctx, &ctx->ncontact, c)
-            || dev_alloc(ctx, &ctx->key, c) || dev_alloc(ctx, &ctx->sorted, c))
+            || dev_alloc(ctx, &ctx->key, c) || dev_alloc(ctx, &ctx->sorted, c)
+            || dev_alloc(ctx, &ctx->stage, c * kStageSlots))
             return DEM_ERR_HIP;
         ctx->cap = n;
     }
@@ -885,6 +887,18 @@ static int setup_mesh_grids(dem_ctx *ctx);
 // One neighbour rebuild.  With captured = true it is being recorded into a
 // graph: no host round trip, the list buffers were sized beforehand and a
 // longer list raises error bit 3 instead of being written.
+// DEM_NEIGH_STAGE=0 selects the two-search rebuild (count pass, then a fill
+// pass that repeats the search); the default searches once into staging rows
+static bool neigh_staged()
+{
+    static int v = -1;
+    if (v < 0) {
+        const char *e = getenv("DEM_NEIGH_STAGE");
+        v = (e && atoi(e) == 0) ? 0 : 1;
+    }
+    return v == 1;
+}
+
 static int rebuild(dem_ctx *ctx, bool captured = false)
 {
     const int n = ctx->n;
@@ -943,11 +957,21 @@ static int rebuild(dem_ctx *ctx, bool captured = false)
     na.dnum = ctx->M.dnum;
     na.cdf = ctx->p.cdf;
     na.skin = ctx->p.skin;
+    na.rmax = ctx->maxrad;
     na.G = G;
     for (int d = 0; d < 3; d++) na.prd[d] = ctx->M.prd[d];
+    na.perm = ctx->sorted;
+    na.have_old = ctx->have_list ? 1 : 0;
+    na.onstart = ctx->nstart[l0];
+    na.onj = ctx->nj[l0];
+    na.ohist = ctx->hist[l0];
+    na.otag = ctx->tag[c0];
+    na.stage = ctx->stage;
+    const bool staged = neigh_staged();
     launch_fill_int(ctx->d_small + 1, 0, 1, s);
     launch_fill_int(ctx->nstart[l1] + n, 0, 1, s);
-    launch_neigh_count(na, s);
+    if (staged) launch_neigh_stage(na, s);
+    else launch_neigh_count(na, s);
     KCHK();
     launch_scan(ctx->nstart[l1], n + 1, ctx->scan_tmp, s);
     KCHK();
@@ -967,13 +991,17 @@ static int rebuild(dem_ctx *ctx, bool captured = false)
     na.nstart = ctx->nstart[l1];
     na.nj = ctx->nj[l1];
     na.hist = ctx->hist[l1];
-    na.perm = ctx->sorted;
-    na.have_old = ctx->have_list ? 1 : 0;
-    na.onstart = ctx->nstart[l0];
-    na.onj = ctx->nj[l0];
-    na.ohist = ctx->hist[l0];
-    na.otag = ctx->tag[c0];
-    launch_neigh_fill(na, s);
+    if (!staged) {
+        launch_neigh_fill(na, s);
+    } else if (captured) {   // the device picks: each kernel returns at once unless it is the one
+        launch_neigh_compact(na, s);
+        na.fill_if_over = 1;
+        launch_neigh_fill(na, s);
+    } else if (hsmall[1] <= kStageSlots) {
+        launch_neigh_compact(na, s);
+    } else {
+        launch_neigh_fill(na, s);
+    }
     KCHK();
     if (captured) launch_row_max_check(ctx->d_small + 1, ctx->p.neigh_one, ctx->d_small, s);
     ctx->lcur = l1;

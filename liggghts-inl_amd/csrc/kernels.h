@@ -73,6 +73,10 @@ struct PermuteArgs {
     double skin;
 };
 
+// entries per particle in the staging rows of the single-search rebuild;
+// a longer row sends the rebuild to the count + fill passes
+constexpr int kStageSlots = 32;
+
 struct NeighArgs {
     const double4 *xr;   // new order
     const double4 *wt;   // new order (instance in meta)
@@ -93,11 +97,14 @@ struct NeighArgs {
     int n;
     int dnum;
     double cdf, skin;
+    double rmax;         // largest radius (the cell grid's)
     Grid G;
     double prd[3];
     int64_t ecap;        // fill pass: entries allocated
     int coh_off;         // first cohesion history slot, -1 without a cohesion model
     int *err;            // bit 3: list longer than ecap
+    int2 *stage;         // n * kStageSlots staged entries (list word, carried entry or -1)
+    int fill_if_over;    // fill pass: run only if the longest row exceeds kStageSlots
 };
 
 struct IntegArgs {
@@ -220,6 +227,10 @@ void launch_neigh_count(const NeighArgs &a, hipStream_t s);
 // *maxcount = max(*maxcount, max of count[0..n))
 void launch_row_max(const int *count, int n, int *maxcount, hipStream_t s);
 void launch_neigh_fill(const NeighArgs &a, hipStream_t s);
+// one search: counts (then the row max) and staging rows
+void launch_neigh_stage(const NeighArgs &a, hipStream_t s);
+// staging rows into the list (after the scan); no-op when a row outgrew kStageSlots
+void launch_neigh_compact(const NeighArgs &a, hipStream_t s);
 void launch_fill_int(int *p, int v, int n, hipStream_t s);
 // err |= 8 and all rows emptied when nstart[n] > ecap (flag: one int of scratch)
 void launch_cap_guard(int *nstart, int n, int64_t ecap, int *flag, int *err, hipStream_t s);

@@ -238,9 +238,38 @@ void launch_permute(const PermuteArgs &a, hipStream_t s)
 // carry nothing are written with flag 0 and no history (a flag-0 entry has
 // never been in contact, so its history is zero by definition and the
 // contact pass does not read it).
-template <bool FILL, int W>
+//
+// MODE kCount counts each row; kFill writes the rows into the list (after
+// the count and the scan), copying the carried history; kStage does the
+// search once, counting each row and writing its first kStageSlots entries
+// to a padded staging row as (list word, index of the carried entry in the
+// previous list or -1), which k_neigh_compact then moves into the list.
+constexpr int kCount = 0, kFill = 1, kStage = 2;
+
+// One entry's history, dnum doubles: loads of up to 8 slots are issued
+// together before their stores (a plain element loop waits out one memory
+// round trip per slot, since the compiler cannot move a load above a store
+// to a pointer that might alias it).
+__device__ inline void copy_hist(double *__restrict__ h, const double *__restrict__ oh, int dnum)
+{
+    constexpr int C = 8;
+    for (int d0 = 0; d0 < dnum; d0 += C) {
+        double v[C];
+#pragma unroll
+        for (int d = 0; d < C; d++)
+            if (d0 + d < dnum) v[d] = oh[d0 + d];
+#pragma unroll
+        for (int d = 0; d < C; d++)
+            if (d0 + d < dnum) h[d0 + d] = v[d];
+    }
+}
+
+template <int MODE, int W>
 __global__ void __launch_bounds__(kBlock) k_neigh(NeighArgs a)
 {
+    constexpr bool FILL = MODE == kFill;
+    constexpr bool STAGE = MODE == kStage;
+    constexpr bool OLD = FILL || STAGE;   // looks up the previous row
     static_assert(W == 16 || W == 32 || W == 64, "16, 32 or 64 lanes per particle");
     constexpr int kGroups = kBlock / W;
     constexpr int kShift = W == 64 ? 6 : (W == 32 ? 5 : 4);
@@ -252,9 +281,20 @@ __global__ void __launch_bounds__(kBlock) k_neigh(NeighArgs a)
     const uint64_t gmask = W == 64 ? ~0ull : ((((1ull << W) - 1ull)) << gshift);
     const int p = (blockIdx.x * blockDim.x + threadIdx.x) >> kShift;
     if (p >= a.n) return;  // p is uniform across the group
+    // recorded after a staged pass: fills only when a row outgrew its staging row
+    if (FILL && a.fill_if_over && *a.maxcount <= kStageSlots) return;
     const double4 xp = a.xr[p];
     const int inst = meta_inst((int)a.wt[p].w);
     const int cc[3] = {cell_coord(xp.x, 0, a.G), cell_coord(xp.y, 1, a.G), cell_coord(xp.z, 2, a.G)};
+    // position in cell units, as cell_coord measures it, and the longest
+    // cutoff any partner can have: a stencil cell whose nearest point lies
+    // beyond it holds no partner and is skipped whole (the rows keep their
+    // order).  A partner in the cell below along d has u < cc[d] (clamping
+    // only moves particles outward from edge cells), one in the cell above
+    // u >= cc[d] + 1; the 1e-6-cell margin covers the rounding of u.
+    const double u[3] = {(xp.x - a.G.lo[0]) * a.G.inv[0], (xp.y - a.G.lo[1]) * a.G.inv[1],
+                         (xp.z - a.G.lo[2]) * a.G.inv[2]};
+    const double cutmax = (xp.w + a.rmax) * a.cdf + a.skin;
     // stencil cell k (dz, dy, dx ascending): first particle, count, image code
     auto cell_info = [&](int k, int &qb, int &cnt, int &code) {
         qb = 0;
@@ -274,7 +314,14 @@ __global__ void __launch_bounds__(kBlock) k_neigh(NeighArgs a)
                 else ok = false;
             }
         }
-        if (ok) {
+        double d2 = 0.0;
+        for (int d = 0; d < 3; d++)
+            if (off[d] != 0) {
+                const double g = off[d] < 0 ? u[d] - cc[d] : cc[d] + 1 - u[d];
+                const double t = fmax(g - 1e-6, 0.0) / a.G.inv[d];
+                d2 += t * t;
+            }
+        if (ok && d2 <= cutmax * cutmax) {
             const uint32_t key = cell_key(c[0], c[1], c[2], a.G, inst);
             qb = a.cstart[key];
             cnt = a.cstart[key + 1] - qb;
@@ -319,7 +366,7 @@ __global__ void __launch_bounds__(kBlock) k_neigh(NeighArgs a)
     // the particle's previous row into LDS: partner tag of every flagged
     // entry (0 = cannot match; tags are positive), its first 64 entries
     int ob = 0, oe = 0;
-    if (FILL && a.have_old) {
+    if (OLD && a.have_old) {
         const int o = a.perm[p];
         ob = a.onstart[o];
         oe = a.onstart[o + 1];
@@ -343,7 +390,7 @@ __global__ void __launch_bounds__(kBlock) k_neigh(NeighArgs a)
     for (int c0 = 0; c0 < total; c0 += W) {
         const int c = c0 + lane;
         bool pass = false, touch = false;
-        int q = 0, qcode = kNoShift;
+        int q = 0, qcode = kNoShift, tq = 0;
         if (c < total) {
             int lo = 0, hi = 26;  // largest k with ex[k] <= c
             while (lo < hi) {
@@ -353,6 +400,7 @@ __global__ void __launch_bounds__(kBlock) k_neigh(NeighArgs a)
             }
             q = s_qb[wv][lo] + (c - s_ex[wv][lo]);
             qcode = s_code[wv][lo];
+            if (OLD) tq = a.tag[q];   // issued beside the position load, not after the test
             if (!(q == p && qcode == kNoShift)) {
                 int sx, sy, sz;
                 shift_decode(qcode, sx, sy, sz);
@@ -371,14 +419,13 @@ __global__ void __launch_bounds__(kBlock) k_neigh(NeighArgs a)
             }
         }
         const uint64_t m = (__ballot(pass) & gmask) >> gshift;
-        if (FILL) {
+        if (OLD) {
             // first flagged entry of the previous row with the partner's tag
             // (the search of neigh_gran.cpp:596-598): only the lanes of
             // touching candidates search, in the LDS copy of the row
             const bool want = pass && touch && a.have_old;
             int found = -1;
             if (want) {
-                const int tq = a.tag[q];
                 for (int k = 0; k < on64; k++)
                     if (s_ot[wv][k] == tq) {
                         found = k;
@@ -393,13 +440,17 @@ __global__ void __launch_bounds__(kBlock) k_neigh(NeighArgs a)
                         }
                     }
             }
-            if (pass) {
+            if (STAGE && pass) {
+                const int k = written + __popcll(m & ((1ull << lane) - 1ull));
+                if (k < kStageSlots)
+                    a.stage[(size_t)p * kStageSlots + k]
+                        = make_int2((int)((uint32_t)q | ((uint32_t)qcode << 26)), found >= 0 ? ob + found : -1);
+            }
+            if (FILL && pass) {
                 const int e = ebase + written + __popcll(m & ((1ull << lane) - 1ull));
                 uint32_t w = (uint32_t)q | ((uint32_t)qcode << 26);
                 if (found >= 0) {
-                    const double *oh = a.ohist + (size_t)(ob + found) * a.dnum;
-                    double *h = a.hist + (size_t)e * a.dnum;
-                    for (int k = 0; k < a.dnum; k++) h[k] = oh[k];
+                    copy_hist(a.hist + (size_t)e * a.dnum, a.ohist + (size_t)(ob + found) * a.dnum, a.dnum);
                     w |= a.onj[ob + found] & (kFlagBit | kBondBit);
                 } else if (a.coh_off >= 0) {
                     // the cohesion model reads its slots on entries that
@@ -412,7 +463,72 @@ __global__ void __launch_bounds__(kBlock) k_neigh(NeighArgs a)
         }
         written += __popcll(m);
     }
+    // the staging row's last 64-byte granule written whole (a partial
+    // granule costs a read-modify-write in HBM)
+    if (STAGE) {
+        const int k = written + lane;
+        if (k < kStageSlots && k < ((written + 7) & ~7)) a.stage[(size_t)p * kStageSlots + k] = make_int2(0, -1);
+    }
     if (!FILL && lane == 0) a.count[p] = written;
+}
+
+// The staged rows into the list, once the row lengths are scanned: G lanes
+// per particle.  List words first, entry k on lane k mod G (the first slot
+// of a lane is loaded before the row length is known); a carried
+// entry takes its flag bits from the previous list.  Then the row's history
+// block, cnt * dnum contiguous doubles, one double per lane with consecutive lanes on consecutive doubles (whole-line
+// stores): a carried entry's slots come from the previous list, a new
+// entry's are zero.  Does nothing when a row outgrew its staging row (the
+// fill pass writes the list then).
+template <int G>
+__global__ void __launch_bounds__(kBlock) k_neigh_compact(NeighArgs a)
+{
+    static_assert(2 * G == kStageSlots, "two staging slots per lane");
+    constexpr int U = 4;   // doubles in flight per lane
+    const int lane = threadIdx.x & (G - 1);
+    const int p = (blockIdx.x * blockDim.x + threadIdx.x) / G;
+    if (p >= a.n) return;  // uniform across the group
+    if (*a.maxcount > kStageSlots) return;
+    if ((int64_t)a.nstart[a.n] > a.ecap) {
+        if (lane == 0) atomicOr(a.err, 8);
+        return;
+    }
+    const int2 *row = a.stage + (size_t)p * kStageSlots;
+    const int2 st0 = row[lane];   // inside the staging row, whatever cnt is
+    const int e0 = a.nstart[p];
+    const int cnt = a.nstart[p + 1] - e0;
+    const int2 st1 = cnt > G ? row[lane + G] : make_int2(0, -1);   // rows past G entries are rare
+    if (lane < cnt)
+        a.nj[e0 + lane] = (uint32_t)st0.x | (st0.y >= 0 ? a.onj[st0.y] & (kFlagBit | kBondBit) : 0u);
+    if (lane + G < cnt)
+        a.nj[e0 + lane + G] = (uint32_t)st1.x | (st1.y >= 0 ? a.onj[st1.y] & (kFlagBit | kBondBit) : 0u);
+    const int dnum = a.dnum;
+    if (dnum == 0) return;
+    const int H = cnt * dnum;   // <= kStageSlots * dnum < 2^11
+    // t / dnum as (t * inv) >> 20: exact for t < 2^11, dnum <= 64
+    const uint32_t inv = (1u << 20) / (uint32_t)dnum + 1u;
+    const double *__restrict__ oh = a.ohist;
+    double *__restrict__ hb = a.hist + (size_t)e0 * dnum;
+    for (int t0 = 0; t0 < H; t0 += U * G) {
+        double v[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int t = t0 + u * G + lane;
+            const int tt = min(t, H - 1);   // lanes past the block read inside it
+            const int k = (int)(((uint32_t)tt * inv) >> 20);
+            const int d = tt - k * dnum;
+            // every lane of the group takes part in the shuffles
+            const int s0 = __shfl(st0.y, k & (G - 1), G);
+            const int s1 = __shfl(st1.y, k & (G - 1), G);
+            const int src = k < G ? s0 : s1;
+            v[u] = (t < H && src >= 0) ? oh[(size_t)src * dnum + d] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int t = t0 + u * G + lane;
+            if (t < H) hb[t] = v[u];
+        }
+    }
 }
 
 // Longest row, for the neigh_modify one check: a grid-stride max over the
@@ -458,11 +574,29 @@ void launch_neigh_count(const NeighArgs &a, hipStream_t s)
 {
     if (a.n == 0) return;
     switch (neigh_lanes()) {
-    case 64: k_neigh<false, 64><<<group_blocks(a.n, 64), kBlock, 0, s>>>(a); break;
-    case 32: k_neigh<false, 32><<<group_blocks(a.n, 32), kBlock, 0, s>>>(a); break;
-    default: k_neigh<false, 16><<<group_blocks(a.n, 16), kBlock, 0, s>>>(a); break;
+    case 64: k_neigh<kCount, 64><<<group_blocks(a.n, 64), kBlock, 0, s>>>(a); break;
+    case 32: k_neigh<kCount, 32><<<group_blocks(a.n, 32), kBlock, 0, s>>>(a); break;
+    default: k_neigh<kCount, 16><<<group_blocks(a.n, 16), kBlock, 0, s>>>(a); break;
     }
     launch_row_max(a.count, a.n, a.maxcount, s);
+}
+
+void launch_neigh_stage(const NeighArgs &a, hipStream_t s)
+{
+    if (a.n == 0) return;
+    switch (neigh_lanes()) {
+    case 64: k_neigh<kStage, 64><<<group_blocks(a.n, 64), kBlock, 0, s>>>(a); break;
+    case 32: k_neigh<kStage, 32><<<group_blocks(a.n, 32), kBlock, 0, s>>>(a); break;
+    default: k_neigh<kStage, 16><<<group_blocks(a.n, 16), kBlock, 0, s>>>(a); break;
+    }
+    launch_row_max(a.count, a.n, a.maxcount, s);
+}
+
+void launch_neigh_compact(const NeighArgs &a, hipStream_t s)
+{
+    if (a.n == 0) return;
+    constexpr int G = kStageSlots / 2;
+    k_neigh_compact<G><<<group_blocks(a.n, G), kBlock, 0, s>>>(a);
 }
 
 void launch_row_max(const int *count, int n, int *maxcount, hipStream_t s)
@@ -476,9 +610,9 @@ void launch_neigh_fill(const NeighArgs &a, hipStream_t s)
 {
     if (a.n == 0) return;
     switch (neigh_lanes()) {
-    case 64: k_neigh<true, 64><<<group_blocks(a.n, 64), kBlock, 0, s>>>(a); break;
-    case 32: k_neigh<true, 32><<<group_blocks(a.n, 32), kBlock, 0, s>>>(a); break;
-    default: k_neigh<true, 16><<<group_blocks(a.n, 16), kBlock, 0, s>>>(a); break;
+    case 64: k_neigh<kFill, 64><<<group_blocks(a.n, 64), kBlock, 0, s>>>(a); break;
+    case 32: k_neigh<kFill, 32><<<group_blocks(a.n, 32), kBlock, 0, s>>>(a); break;
+    default: k_neigh<kFill, 16><<<group_blocks(a.n, 16), kBlock, 0, s>>>(a); break;
     }
 }
 
