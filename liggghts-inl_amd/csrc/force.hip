@@ -53,6 +53,46 @@ __device__ inline int group_sum_int(int v)
     return v;
 }
 
+// Slots [0, lim) of one entry's history into h, zeros past lim (all zeros
+// when !known).  vec: the entry starts 16-byte aligned and lim is even, so
+// the slots move as 16-byte pairs and a lane's history reaches the L2 in
+// half the requests (the entries a wave touches are scattered, so every
+// lane's access is a request of its own).
+template <int HN>
+__device__ inline void hist_load(const double *hp, int lim, bool vec, bool known, double (&h)[HN])
+{
+    static_assert(HN % 2 == 0, "pairs");
+    if (vec) {
+        const double2 *h2 = reinterpret_cast<const double2 *>(hp);
+#pragma unroll
+        for (int q = 0; q < HN / 2; q++) {
+            double2 v = make_double2(0.0, 0.0);
+            if (known && 2 * q < lim) v = h2[q];
+            h[2 * q] = v.x;
+            h[2 * q + 1] = v.y;
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < HN; q++) h[q] = (known && q < lim) ? hp[q] : 0.0;
+    }
+}
+
+template <int HN>
+__device__ inline void hist_store(double *hp, int lim, bool vec, const double (&h)[HN])
+{
+    static_assert(HN % 2 == 0, "pairs");
+    if (vec) {
+        double2 *h2 = reinterpret_cast<double2 *>(hp);
+#pragma unroll
+        for (int q = 0; q < HN / 2; q++)
+            if (2 * q < lim) h2[q] = make_double2(h[2 * q], h[2 * q + 1]);
+    } else {
+#pragma unroll
+        for (int q = 0; q < HN; q++)
+            if (q < lim) hp[q] = h[q];
+    }
+}
+
 // What the owner of particle i adds after its pair sum: the primitive
 // planes in the order the fixes were defined, gravity, fix freeze; then the
 // force, torque and optional per-atom outputs are written.
@@ -100,17 +140,13 @@ __device__ inline void particle_epilogue(const ForceArgs &a, int i, const double
                     cw.wi[0] = wi.x; cw.wi[1] = wi.y; cw.wi[2] = wi.z;
                     cw.wj[0] = 0.0; cw.wj[1] = 0.0; cw.wj[2] = 0.0;
                     constexpr int WHN = chain_max_dnum(WN);
+                    const bool vec = (wdn & 1) == 0;
                     double h[WHN];
-#pragma unroll
-                    for (int q = 0; q < WHN; q++) h[q] = q < wdn ? hp[q] : 0.0;
+                    hist_load(hp, wdn, vec, true, h);
                     double dF[3], dT[3];
                     if (MIXED && WN == NM_HERTZ) contact_mixed(M, mat, cw, h, a.shearupdate, dF, dT);
                     else contact_fp64<WN>(M, M.lw, mat, cw, h, a.shearupdate, dF, dT);
-                    if (a.shearupdate || WN != NM_HERTZ) {
-#pragma unroll
-                        for (int q = 0; q < WHN; q++)
-                            if (q < wdn) hp[q] = h[q];
-                    }
+                    if (a.shearupdate || WN != NM_HERTZ) hist_store(hp, wdn, vec, h);
                     F[0] += dF[0]; F[1] += dF[1]; F[2] += dF[2];
                     T[0] += dT[0]; T[1] += dT[1]; T[2] += dT[2];
                     if (k == a.obs_plane) pw = d == 0 ? dF[0] : (d == 1 ? dF[1] : dF[2]);
@@ -221,19 +257,15 @@ __global__ void __launch_bounds__(kBlock) k_force(ForceArgs a)
             double h[HN];
             double *hp = a.hist + (size_t)e * dnum;
             const bool known = (w & (CH ? (kFlagBit | kBondBit) : kFlagBit)) != 0;
-#pragma unroll
-            for (int k = 0; k < HN; k++) h[k] = (known && k < hdn) ? hp[k] : 0.0;
+            const bool vec = ((dnum | hdn) & 1) == 0;
+            hist_load(hp, hdn, vec, known, h);
             double dF[3], dT[3];
             if (MIXED && NM == NM_HERTZ) contact_mixed(M, mat, c, h, a.shearupdate, dF, dT);
             else if (CH) contact_fp64<NM, true>(M, M.lp, mat, c, h, a.shearupdate, dF, dT, coh,
                                                 bonded ? M.cohesion : CM_OFF);
             else contact_fp64<NM>(M, M.lp, mat, c, h, a.shearupdate, dF, dT);
             // the hysteretic models write their state even in set-up passes
-            if (a.shearupdate || !known || NM != NM_HERTZ) {
-#pragma unroll
-                for (int k = 0; k < HN; k++)
-                    if (k < hdn) hp[k] = h[k];
-            }
+            if (a.shearupdate || !known || NM != NM_HERTZ) hist_store(hp, hdn, vec, h);
             if (!(w & kFlagBit)) a.nj[e] = w | kFlagBit;
             F[0] += dF[0]; F[1] += dF[1]; F[2] += dF[2];
             T[0] += dT[0]; T[1] += dT[1]; T[2] += dT[2];
@@ -367,15 +399,11 @@ __global__ void __launch_bounds__(kBlock) k_force_q(ForceArgs a)
             double h[HN];
             double *hp = a.hist + (size_t)e * dnum;
             const bool known = (w & kFlagBit) != 0;
-#pragma unroll
-            for (int q = 0; q < HN; q++) h[q] = (known && q < dnum) ? hp[q] : 0.0;
+            const bool vec = (dnum & 1) == 0;
+            hist_load(hp, dnum, vec, known, h);
             if (MIXED) contact_mixed(M, mat, c, h, a.shearupdate, dF, dT);
             else contact_fp64<NM_HERTZ>(M, M.lp, mat, c, h, a.shearupdate, dF, dT);
-            if (a.shearupdate || !known) {
-#pragma unroll
-                for (int q = 0; q < HN; q++)
-                    if (q < dnum) hp[q] = h[q];
-            }
+            if (a.shearupdate || !known) hist_store(hp, dnum, vec, h);
             if (!known) a.nj[e] = w | kFlagBit;
             if (lane == 0 || s_qo[wv][lane - 1] != k) s_run0[wv][k] = lane;
             if (lane == m - 1 || s_qo[wv][lane + 1] != k) s_run1[wv][k] = lane + 1;
