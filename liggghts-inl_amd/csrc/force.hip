@@ -319,6 +319,7 @@ __global__ void __launch_bounds__(kBlock) k_force_q(ForceArgs a)
     static_assert(P <= 64, "a tile fits one wave");
     __shared__ int s_rs[kQWaves][P + 1];
     __shared__ double4 s_x[kQWaves][P];
+    __shared__ double4 s_v[kQWaves][P], s_w[kQWaves][P];   // the tile's velocity/mass, spin/meta
     __shared__ int s_q[kQWaves][128], s_qo[kQWaves][128];
     __shared__ uint32_t s_qw[kQWaves][128];          // list word of the queued entry
     __shared__ double s_qd[kQWaves][4][128];         // its separation x_i - x_j, partner radius
@@ -334,7 +335,11 @@ __global__ void __launch_bounds__(kBlock) k_force_q(ForceArgs a)
     const int np = min(P, a.n - p0);
     const Model &M = a.M;
     const int dnum = M.dnum;
-    if (lane < np) s_x[wv][lane] = a.xr[p0 + lane];
+    if (lane < np) {
+        s_x[wv][lane] = a.xr[p0 + lane];
+        s_v[wv][lane] = a.vm[p0 + lane];
+        s_w[wv][lane] = a.wt[p0 + lane];
+    }
     // row starts of the tile and, from lane 0, the end of its last row (a
     // tile of 64 has no 65th lane)
     if (lane < np) s_rs[wv][lane] = a.nstart[p0 + lane];
@@ -365,7 +370,7 @@ __global__ void __launch_bounds__(kBlock) k_force_q(ForceArgs a)
             const double4 vmj = a.vm[j];
             const double4 wj = a.wt[j];
             const double radj = s_qd[wv][3][lane];
-            const double4 pi = s_x[wv][k], vmi = a.vm[p0 + k], wi = a.wt[p0 + k];
+            const double4 pi = s_x[wv][k], vmi = s_v[wv][k], wi = s_w[wv][k];
             const int metai = (int)wi.w;
             const double *mat = a.mat + (size_t)meta_inst(metai) * kMatStride;
             ContactIn c;
@@ -514,7 +519,7 @@ __global__ void __launch_bounds__(kBlock) k_force_q(ForceArgs a)
     if (qlen > 0) eval_batch(qlen);
     if (lane < np) {
         const int i = p0 + lane;
-        const double4 pi = s_x[wv][lane], vmi = a.vm[i], wi = a.wt[i];
+        const double4 pi = s_x[wv][lane], vmi = s_v[wv][lane], wi = s_w[wv][lane];
         const int metai = (int)wi.w;
         const double *mat = a.mat + (size_t)meta_inst(metai) * kMatStride;
         particle_epilogue<MIXED, NM_HERTZ>(a, i, pi, vmi, wi, meta_type(metai), meta_frozen(metai), mat, F, T, nc);
