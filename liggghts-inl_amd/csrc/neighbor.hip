@@ -387,6 +387,11 @@ __global__ void __launch_bounds__(kBlock) k_neigh(NeighArgs a)
         return;
     }
     int written = 0;
+    // the fill pass writes a row in two sweeps, touching entries then the
+    // others (each in scan order): the order the staged rows are compacted
+    // in, so both paths give the same list
+    constexpr int kSweeps = FILL ? 2 : 1;
+    for (int sweep = 0; sweep < kSweeps; sweep++)
     for (int c0 = 0; c0 < total; c0 += W) {
         const int c = c0 + lane;
         bool pass = false, touch = false;
@@ -418,12 +423,13 @@ __global__ void __launch_bounds__(kBlock) k_neigh(NeighArgs a)
                 touch = rsq < radsum * radsum;
             }
         }
-        const uint64_t m = (__ballot(pass) & gmask) >> gshift;
+        const bool emit = FILL ? pass && touch == (sweep == 0) : pass;
+        const uint64_t m = (__ballot(emit) & gmask) >> gshift;
         if (OLD) {
             // first flagged entry of the previous row with the partner's tag
             // (the search of neigh_gran.cpp:596-598): only the lanes of
             // touching candidates search, in the LDS copy of the row
-            const bool want = pass && touch && a.have_old;
+            const bool want = emit && touch && a.have_old;
             int found = -1;
             if (want) {
                 for (int k = 0; k < on64; k++)
@@ -440,13 +446,13 @@ __global__ void __launch_bounds__(kBlock) k_neigh(NeighArgs a)
                         }
                     }
             }
-            if (STAGE && pass) {
+            if (STAGE && pass) {   // kFlagBit marks a touching entry here
                 const int k = written + __popcll(m & ((1ull << lane) - 1ull));
+                const uint32_t w = (uint32_t)q | ((uint32_t)qcode << 26) | (touch ? kFlagBit : 0u);
                 if (k < kStageSlots)
-                    a.stage[(size_t)p * kStageSlots + k]
-                        = make_int2((int)((uint32_t)q | ((uint32_t)qcode << 26)), found >= 0 ? ob + found : -1);
+                    a.stage[(size_t)p * kStageSlots + k] = make_int2((int)w, found >= 0 ? ob + found : -1);
             }
-            if (FILL && pass) {
+            if (FILL && emit) {
                 const int e = ebase + written + __popcll(m & ((1ull << lane) - 1ull));
                 uint32_t w = (uint32_t)q | ((uint32_t)qcode << 26);
                 if (found >= 0) {
@@ -473,19 +479,24 @@ __global__ void __launch_bounds__(kBlock) k_neigh(NeighArgs a)
 }
 
 // The staged rows into the list, once the row lengths are scanned: G lanes
-// per particle.  List words first, entry k on lane k mod G (the first slot
-// of a lane is loaded before the row length is known); a carried
-// entry takes its flag bits from the previous list.  Then the row's history
-// block, cnt * dnum contiguous doubles, one double per lane with consecutive lanes on consecutive doubles (whole-line
-// stores): a carried entry's slots come from the previous list, a new
-// entry's are zero.  Does nothing when a row outgrew its staging row (the
-// fill pass writes the list then).
+// per particle, staged entry k on lane k mod G.  A row is written touching
+// entries first, then the others, each in scan order (a stable partition by
+// group ballots): the histories the next steps read and write then sit side
+// by side instead of between entries that do not touch.  A carried entry
+// takes its flag bits from the previous list.  Then the row's history block,
+// cnt * dnum contiguous doubles, one double per lane with consecutive lanes
+// on consecutive doubles (whole-line stores): a carried entry's slots come
+// from the previous list, a new entry's are zero.  Does nothing when a row
+// outgrew its staging row (the fill pass writes the list then).
 template <int G>
 __global__ void __launch_bounds__(kBlock) k_neigh_compact(NeighArgs a)
 {
-    static_assert(2 * G == kStageSlots, "two staging slots per lane");
+    static_assert(2 * G == kStageSlots && G <= 32, "two staging slots per lane");
     constexpr int U = 4;   // doubles in flight per lane
+    __shared__ int s_src[kBlock / G][kStageSlots];   // carried entry behind each slot of the row
     const int lane = threadIdx.x & (G - 1);
+    const int grp = threadIdx.x / G;
+    const int gshift = (threadIdx.x & 63) & ~(G - 1);
     const int p = (blockIdx.x * blockDim.x + threadIdx.x) / G;
     if (p >= a.n) return;  // uniform across the group
     if (*a.maxcount > kStageSlots) return;
@@ -498,10 +509,26 @@ __global__ void __launch_bounds__(kBlock) k_neigh_compact(NeighArgs a)
     const int e0 = a.nstart[p];
     const int cnt = a.nstart[p + 1] - e0;
     const int2 st1 = cnt > G ? row[lane + G] : make_int2(0, -1);   // rows past G entries are rare
-    if (lane < cnt)
-        a.nj[e0 + lane] = (uint32_t)st0.x | (st0.y >= 0 ? a.onj[st0.y] & (kFlagBit | kBondBit) : 0u);
-    if (lane + G < cnt)
-        a.nj[e0 + lane + G] = (uint32_t)st1.x | (st1.y >= 0 ? a.onj[st1.y] & (kFlagBit | kBondBit) : 0u);
+    const bool v0 = lane < cnt, v1 = lane + G < cnt;
+    const bool tc0 = v0 && ((uint32_t)st0.x & kFlagBit), tc1 = v1 && ((uint32_t)st1.x & kFlagBit);
+    auto group_mask = [&](bool b) { return (uint32_t)((__ballot(b) >> gshift) & ((1ull << G) - 1ull)); };
+    const uint32_t mt0 = group_mask(tc0), mt1 = group_mask(tc1);
+    const uint32_t mn0 = group_mask(v0 && !tc0), mn1 = group_mask(v1 && !tc1);
+    const uint32_t below = (1u << lane) - 1u;
+    const int nt = __popc(mt0) + __popc(mt1);
+    const int d0 = tc0 ? __popc(mt0 & below) : nt + __popc(mn0 & below);
+    const int d1 = tc1 ? __popc(mt0) + __popc(mt1 & below) : nt + __popc(mn0) + __popc(mn1 & below);
+    if (v0) {
+        a.nj[e0 + d0] = ((uint32_t)st0.x & ~kFlagBit) | (st0.y >= 0 ? a.onj[st0.y] & (kFlagBit | kBondBit) : 0u);
+        s_src[grp][d0] = st0.y;
+    }
+    if (v1) {
+        a.nj[e0 + d1] = ((uint32_t)st1.x & ~kFlagBit) | (st1.y >= 0 ? a.onj[st1.y] & (kFlagBit | kBondBit) : 0u);
+        s_src[grp][d1] = st1.y;
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     const int dnum = a.dnum;
     if (dnum == 0) return;
     const int H = cnt * dnum;   // <= kStageSlots * dnum < 2^11
@@ -517,10 +544,7 @@ __global__ void __launch_bounds__(kBlock) k_neigh_compact(NeighArgs a)
             const int tt = min(t, H - 1);   // lanes past the block read inside it
             const int k = (int)(((uint32_t)tt * inv) >> 20);
             const int d = tt - k * dnum;
-            // every lane of the group takes part in the shuffles
-            const int s0 = __shfl(st0.y, k & (G - 1), G);
-            const int s1 = __shfl(st1.y, k & (G - 1), G);
-            const int src = k < G ? s0 : s1;
+            const int src = s_src[grp][k];
             v[u] = (t < H && src >= 0) ? oh[(size_t)src * dnum + d] : 0.0;
         }
 #pragma unroll

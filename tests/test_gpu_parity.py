@@ -224,3 +224,44 @@ def test_contact_shapes_agree(case):
         o2.compute_forces(0)
         check_forces(s, o2, 1e-12)
         s.close()
+
+
+@pytest.mark.parametrize("big", [False, True])
+def test_rows_touching_first(big):
+    """Every row lists its touching partners first, then the others, and
+    the pair set is the brute-force one: on random overlapping spheres whose
+    rows stay within the 32 staged slots (single-search rebuild) and, with a
+    3 mm sphere among them whose row is longer, on the two-search rebuild
+    that such a row selects."""
+    eng = load_engine()
+    rng = np.random.default_rng(17)
+    n = 1500
+    lo, hi = [0, 0, 0], [0.02, 0.02, 0.02]
+    x = rng.uniform(0, 0.02, size=(n, 3))
+    r = rng.uniform(0.4e-3, 0.9e-3, size=n)
+    if big:
+        x[0] = [0.01, 0.01, 0.01]
+        r[0] = 3e-3
+    skin = 3e-4
+    spec = dict(box_lo=lo, box_hi=hi, periodic=[1, 1, 1], dt=1e-7, skin=skin, materials=W()._materials())
+    atoms = dict(x=x, v=np.zeros((n, 3)), omega=np.zeros((n, 3)), radius=r,
+                 rmass=4 / 3 * np.pi * r ** 3 * 2500, type=np.ones(n, int), tag=np.arange(1, n + 1))
+    s = eng.Simulation(spec, atoms)
+    s.setup()
+    gi, gj, gs, _, _ = s.pairs()
+    got = set(zip(gi.tolist(), gj.tolist(), gs.tolist()))
+    assert len(got) == len(gi)
+    assert got == brute_pairs(x, r, lo, hi, [1, 1, 1], skin)
+    rowlen = np.bincount(gi, minlength=n + 1)
+    assert (rowlen.max() > 32) == big
+    # touching at the build: rsq < (ri + rj)^2 for the entry's image
+    L = np.asarray(hi) - np.asarray(lo)
+    sh = np.stack([gs % 3 - 1, (gs // 3) % 3 - 1, gs // 9 - 1], 1)
+    d = x[gi - 1] - (x[gj - 1] + sh * L)
+    touch = (d * d).sum(1) < (r[gi - 1] + r[gj - 1]) ** 2
+    assert touch.sum() > n
+    starts = np.flatnonzero(np.r_[True, gi[1:] != gi[:-1]])
+    assert len(starts) == len(np.unique(gi))   # rows are contiguous
+    for a, b in zip(starts, np.r_[starts[1:], len(gi)]):
+        t = touch[a:b]
+        assert not np.any(~t[:-1] & t[1:]), gi[a]   # no touching entry after one that does not touch
