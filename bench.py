@@ -30,7 +30,7 @@ HBM_PEAK_GBS = 8000.0
 # FETCH_SIZE and WRITE_SIZE counter passes of rocprofv3 on k_force_q<true,32>
 # (FETCH_SIZE scaled x2.0, WRITE_SIZE x1.0 against a 256 MiB device copy):
 # profiles/pmc_traffic_k_force_column1M.md.  Re-measured when the kernel changes.
-MEASURED_TRAFFIC_BPP = 851.1
+MEASURED_TRAFFIC_BPP = 783.6
 
 
 def package():
@@ -166,14 +166,14 @@ def bench_batch(pkg, args):
         s.setup()
         s.sync()
         t0 = time.perf_counter()
-        s.adjoint_forward(args.steps, 100)
+        s.adjoint_forward(args.adjoint_steps, 100)
         s.sync()
         t1 = time.perf_counter()
         g = s.adjoint_backward(np.ones(args.instances), np.ones(args.instances))
         t2 = time.perf_counter()
         s.close()
-        adj = {"steps": args.steps, "checkpoint_every": 100, "forward_s": t1 - t0, "backward_s": t2 - t1,
-               "backward_particle_steps_per_s": n * args.steps / (t2 - t1),
+        adj = {"steps": args.adjoint_steps, "checkpoint_every": 100, "forward_s": t1 - t0,
+               "backward_s": t2 - t1, "backward_particle_steps_per_s": n * args.adjoint_steps / (t2 - t1),
                "grad_finite": bool(np.isfinite(g).all())}
         print("adjoint: forward %.2f s, backward %.2f s" % (t1 - t0, t2 - t1), file=sys.stderr, flush=True)
     main_prec = "mixed" if "mixed" in out else "fp64"
@@ -241,6 +241,8 @@ def main():
                          "fibres under a moving plate")
     ap.add_argument("--instances", type=int, default=256)
     ap.add_argument("--adjoint", action="store_true", help="batch: also time the adjoint window (FP64)")
+    ap.add_argument("--adjoint-steps", type=int, default=2000,
+                    help="batch: steps of the adjoint window (one neighbour list: motion stays within half the skin)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MB",
                     help="only run a device-to-device copy of MB MiB, 10 times (for calibrating "
                          "the FETCH_SIZE/WRITE_SIZE counters against known traffic)")
