@@ -270,9 +270,9 @@ __global__ void __launch_bounds__(kBlock) k_neigh(NeighArgs a)
     constexpr bool FILL = MODE == kFill;
     constexpr bool STAGE = MODE == kStage;
     constexpr bool OLD = FILL || STAGE;   // looks up the previous row
-    static_assert(W == 16 || W == 32 || W == 64, "16, 32 or 64 lanes per particle");
+    static_assert(W == 8 || W == 16 || W == 32 || W == 64, "8, 16, 32 or 64 lanes per particle");
     constexpr int kGroups = kBlock / W;
-    constexpr int kShift = W == 64 ? 6 : (W == 32 ? 5 : 4);
+    constexpr int kShift = W == 64 ? 6 : (W == 32 ? 5 : (W == 16 ? 4 : 3));
     __shared__ int s_ex[kGroups][28], s_qb[kGroups][27], s_code[kGroups][27];
     __shared__ int s_ot[kGroups][64];                   // partner tags of the previous row (fill)
     const int lane = threadIdx.x & (W - 1);            // lane within the particle's group
@@ -336,28 +336,21 @@ __global__ void __launch_bounds__(kBlock) k_neigh(NeighArgs a)
         }
         return incl;
     };
-    // cells lane and lane + W (W = 16: two cells per lane)
-    int qb = 0, cnt = 0, code = kNoShift;
-    if (lane < 27) cell_info(lane, qb, cnt, code);
-    const int incl = scan(cnt);
-    const int total_a = __shfl(incl, W - 1, W);
-    int total = total_a;
-    if (lane < 27) {
-        s_ex[wv][lane] = incl - cnt;
-        s_qb[wv][lane] = qb;
-        s_code[wv][lane] = code;
-    }
-    if (W < 27) {
-        const int k2 = lane + W;
-        int qb2 = 0, cnt2 = 0, code2 = kNoShift;
-        if (k2 < 27) cell_info(k2, qb2, cnt2, code2);
-        const int incl2 = scan(cnt2);
-        total = total_a + __shfl(incl2, W - 1, W);
-        if (k2 < 27) {
-            s_ex[wv][k2] = total_a + incl2 - cnt2;
-            s_qb[wv][k2] = qb2;
-            s_code[wv][k2] = code2;
+    // cells lane, lane + W, ... (W < 27: several cells per lane), their
+    // candidate counts scanned in stencil order
+    int total = 0;
+#pragma unroll
+    for (int r = 0; r < (27 + W - 1) / W; r++) {
+        const int k = lane + r * W;
+        int qb = 0, cnt = 0, code = kNoShift;
+        if (k < 27) cell_info(k, qb, cnt, code);
+        const int incl = scan(cnt);
+        if (k < 27) {
+            s_ex[wv][k] = total + incl - cnt;
+            s_qb[wv][k] = qb;
+            s_code[wv][k] = code;
         }
+        total += __shfl(incl, W - 1, W);
     }
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -574,16 +567,18 @@ __global__ void __launch_bounds__(kBlock) k_row_max(const int *count, int n, int
     }
 }
 
-// lanes per particle of the neighbour build: DEM_NEIGH_LANES (16, 32 or
-// 64), default 16 (1M-sphere column: 1.86 ms per rebuild against 1.98 with
-// 32 lanes and 2.34 with 64; the lists are identical)
+// lanes per particle of the neighbour build: DEM_NEIGH_LANES (8, 16, 32 or
+// 64), default 16 (1M-sphere column, single search: 494 us per stage pass
+// against 492 with 8 lanes and 569 with 32; two-search form: 1.86 ms per
+// rebuild against 1.98 with 32 lanes and 2.34 with 64; the lists are
+// identical)
 static int neigh_lanes()
 {
     static int w = 0;
     if (w == 0) {
         const char *s = getenv("DEM_NEIGH_LANES");
         const int v = s ? atoi(s) : 16;
-        w = (v == 32 || v == 64) ? v : 16;
+        w = (v == 8 || v == 32 || v == 64) ? v : 16;
     }
     return w;
 }
@@ -600,6 +595,7 @@ void launch_neigh_count(const NeighArgs &a, hipStream_t s)
     switch (neigh_lanes()) {
     case 64: k_neigh<kCount, 64><<<group_blocks(a.n, 64), kBlock, 0, s>>>(a); break;
     case 32: k_neigh<kCount, 32><<<group_blocks(a.n, 32), kBlock, 0, s>>>(a); break;
+    case 8: k_neigh<kCount, 8><<<group_blocks(a.n, 8), kBlock, 0, s>>>(a); break;
     default: k_neigh<kCount, 16><<<group_blocks(a.n, 16), kBlock, 0, s>>>(a); break;
     }
     launch_row_max(a.count, a.n, a.maxcount, s);
@@ -611,6 +607,7 @@ void launch_neigh_stage(const NeighArgs &a, hipStream_t s)
     switch (neigh_lanes()) {
     case 64: k_neigh<kStage, 64><<<group_blocks(a.n, 64), kBlock, 0, s>>>(a); break;
     case 32: k_neigh<kStage, 32><<<group_blocks(a.n, 32), kBlock, 0, s>>>(a); break;
+    case 8: k_neigh<kStage, 8><<<group_blocks(a.n, 8), kBlock, 0, s>>>(a); break;
     default: k_neigh<kStage, 16><<<group_blocks(a.n, 16), kBlock, 0, s>>>(a); break;
     }
     launch_row_max(a.count, a.n, a.maxcount, s);
@@ -636,6 +633,7 @@ void launch_neigh_fill(const NeighArgs &a, hipStream_t s)
     switch (neigh_lanes()) {
     case 64: k_neigh<kFill, 64><<<group_blocks(a.n, 64), kBlock, 0, s>>>(a); break;
     case 32: k_neigh<kFill, 32><<<group_blocks(a.n, 32), kBlock, 0, s>>>(a); break;
+    case 8: k_neigh<kFill, 8><<<group_blocks(a.n, 8), kBlock, 0, s>>>(a); break;
     default: k_neigh<kFill, 16><<<group_blocks(a.n, 16), kBlock, 0, s>>>(a); break;
     }
 }
