@@ -192,37 +192,90 @@ void launch_bin(const BinArgs &a, hipStream_t s)
 
 // ---------------------------------------------------------------------------
 
+// One entry's history, dnum doubles: loads of up to 8 slots are issued
+// together before their stores (a plain element loop waits out one memory
+// round trip per slot, since the compiler cannot move a load above a store
+// to a pointer that might alias it).
+__device__ inline void copy_hist(double *__restrict__ h, const double *__restrict__ oh, int dnum)
+{
+    constexpr int C = 8;
+    for (int d0 = 0; d0 < dnum; d0 += C) {
+        double v[C];
+#pragma unroll
+        for (int d = 0; d < C; d++)
+            if (d0 + d < dnum) v[d] = oh[d0 + d];
+#pragma unroll
+        for (int d = 0; d < C; d++)
+            if (d0 + d < dnum) h[d0 + d] = v[d];
+    }
+}
+
+constexpr int kWhistInline = 8;
+
+// Every field of the particle is loaded before any store: the stores go
+// through pointers the compiler must assume may alias the sources, so a
+// load placed after a store would wait for it (one memory round trip per
+// field).
 __global__ void __launch_bounds__(kBlock) k_permute(PermuteArgs a)
 {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= a.n) return;
     const int o = a.sorted[p];
     const double4 x = a.xr0[o];
-    a.xr1[p] = x;
-    a.xhold[p] = x;
-    a.vm1[p] = a.vm0[o];
-    a.wt1[p] = a.wt0[o];
-    a.tag1[p] = a.tag0[o];
-    a.orig1[p] = a.orig0[o];
-    a.img1[p] = a.img0[o];
-    for (int k = 0; k < a.wdn; k++) a.whist1[(size_t)p * a.wdn + k] = a.whist0[(size_t)o * a.wdn + k];
+    const double4 vm = a.vm0[o];
+    const double4 wt = a.wt0[o];
+    const int tg = a.tag0[o];
+    const int og = a.orig0[o];
+    const int4 im = a.img0[o];
     // PrimitiveWall::buildNeighList with treshold = skin
     // (fix_wall_gran.cpp:700-703, primitive_wall_definitions.h:142-148)
     uint32_t m = 0;
-    const double *xc = &x.x;
     for (int k = 0; k < a.P.n; k++) {
         const double dMax = x.w + a.skin;
-        const double dist = xc[a.P.dim[k]] - a.P.pos[k];
+        // a blend rather than an index, so x stays in registers
+        const int d = a.P.dim[k];
+        const double xd = x.x * (double)(d == 0) + x.y * (double)(d == 1) + x.z * (double)(d == 2);
+        const double dist = xd - a.P.pos[k];
         const double absdist = (dist > 0.0) ? dist : -dist;
         if (absdist <= dMax) m |= 1u << k;
     }
+    a.xr1[p] = x;
+    a.xhold[p] = x;
+    a.vm1[p] = vm;
+    a.wt1[p] = wt;
+    a.tag1[p] = tg;
+    a.orig1[p] = og;
+    a.img1[p] = im;
     a.wmask[p] = (uint8_t)m;
+    // a short wall-history row (one batch of loads) goes here; longer ones
+    // in k_permute_whist
+    if (a.wdn > 0 && a.wdn <= kWhistInline)
+        copy_hist(a.whist1 + (size_t)p * a.wdn, a.whist0 + (size_t)o * a.wdn, a.wdn);
+}
+
+// A long per-atom wall-history row into the new order, one double per
+// thread, so the stores are whole lines (a thread per particle would store
+// wdn doubles at a stride of wdn * 8 bytes: 6 planes of 3 slots, 1M
+// particles, 210 us in k_permute against 56 + 78 us split).
+__global__ void __launch_bounds__(kBlock) k_permute_whist(const int *sorted, const double *__restrict__ h0,
+                                                         double *__restrict__ h1, int n, int wdn)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)n * wdn) return;
+    const int p = (int)(t / wdn);
+    const int k = (int)(t - (int64_t)p * wdn);
+    h1[t] = h0[(size_t)sorted[p] * wdn + k];
 }
 
 void launch_permute(const PermuteArgs &a, hipStream_t s)
 {
     if (a.n == 0) return;
     k_permute<<<(a.n + kBlock - 1) / kBlock, kBlock, 0, s>>>(a);
+    if (a.wdn > kWhistInline) {
+        const int64_t m = (int64_t)a.n * a.wdn;
+        k_permute_whist<<<(unsigned)((m + kBlock - 1) / kBlock), kBlock, 0, s>>>(a.sorted, a.whist0, a.whist1, a.n,
+                                                                             a.wdn);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -245,24 +298,6 @@ void launch_permute(const PermuteArgs &a, hipStream_t s)
 // to a padded staging row as (list word, index of the carried entry in the
 // previous list or -1), which k_neigh_compact then moves into the list.
 constexpr int kCount = 0, kFill = 1, kStage = 2;
-
-// One entry's history, dnum doubles: loads of up to 8 slots are issued
-// together before their stores (a plain element loop waits out one memory
-// round trip per slot, since the compiler cannot move a load above a store
-// to a pointer that might alias it).
-__device__ inline void copy_hist(double *__restrict__ h, const double *__restrict__ oh, int dnum)
-{
-    constexpr int C = 8;
-    for (int d0 = 0; d0 < dnum; d0 += C) {
-        double v[C];
-#pragma unroll
-        for (int d = 0; d < C; d++)
-            if (d0 + d < dnum) v[d] = oh[d0 + d];
-#pragma unroll
-        for (int d = 0; d < C; d++)
-            if (d0 + d < dnum) h[d0 + d] = v[d];
-    }
-}
 
 template <int MODE, int W>
 __global__ void __launch_bounds__(kBlock) k_neigh(NeighArgs a)
