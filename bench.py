@@ -227,6 +227,45 @@ def bench_fibres(pkg, args):
     print(json.dumps(line))
 
 
+def bench_hopper(pkg, args):
+    """Hopper discharge on a triangle mesh (200,000 spheres r ~ U[0.8, 1.2] mm
+    filled on a jittered lattice inside a 30-degree cone with a cylinder on
+    top, 23,360 triangles, Hertz + history, mu 0.4, e 0.5, dt 5e-7), mixed
+    and FP64.  Reported beside the headline metric."""
+    spec, atoms = pkg.workloads.hopper()
+    n = len(atoms["radius"])
+    ntri = len(spec["meshes"][0]["nodes"])
+    out = {}
+    for prec in (["mixed", "fp64"] if args.only is None else [args.only]):
+        s = pkg.Simulation(spec, atoms, precision=prec)
+        s.setup()
+        s.step(args.warmup)
+        s.sync()
+        t0 = time.perf_counter()
+        s.step(args.steps)
+        s.sync()
+        dt = time.perf_counter() - t0
+        st = s.state()
+        s.close()
+        out[prec] = {"value": n * args.steps / dt, "ms_per_step": dt * 1e3 / args.steps,
+                     "mean_height_mm": float(st["x"][:, 2].mean() * 1e3)}
+        print("hopper %s: %.3f s for %d steps" % (prec, dt, args.steps), file=sys.stderr, flush=True)
+    main_prec = "mixed" if "mixed" in out else "fp64"
+    R = out[main_prec]
+    line = {
+        "metric": "particle-steps/s, hopper discharge on a triangle-mesh wall", "value": R["value"],
+        "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": R["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 law / f64 state" if main_prec == "mixed" else "f64", "data": "synthetic",
+        "config": {"workload": "%d spheres r~U[0.8,1.2] mm in a 30-degree conical hopper (100 mm top, 12 mm "
+                               "outlet) with a cylinder on top, %d triangles, Hertz+history, mu 0.4, e 0.5, "
+                               "floor plane, gravity, dt 5e-7, rebuild every 10" % (n, ntri),
+                   "n_particles": n, "triangles": ntri, "precision": main_prec},
+        "forms": out,
+    }
+    print(json.dumps(line))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -236,9 +275,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-check", action="store_true", help="skip the mixed-vs-oracle force check")
     ap.add_argument("--only", choices=["mixed", "fp64"], default=None)
-    ap.add_argument("--workload", choices=["column", "batch", "fibres"], default="column",
+    ap.add_argument("--workload", choices=["column", "batch", "fibres", "hopper"], default="column",
                     help="column: the headline 1M-sphere metric; batch: 256 batched pours; fibres: bonded "
-                         "fibres under a moving plate")
+                         "fibres under a moving plate; hopper: 200,000 spheres in a triangle-mesh hopper")
     ap.add_argument("--instances", type=int, default=256)
     ap.add_argument("--adjoint", action="store_true", help="batch: also time the adjoint window (FP64)")
     ap.add_argument("--adjoint-steps", type=int, default=2000,
@@ -270,6 +309,9 @@ def main():
         return
     if args.workload == "fibres":
         bench_fibres(pkg, args)
+        return
+    if args.workload == "hopper":
+        bench_hopper(pkg, args)
         return
     t0 = time.perf_counter()
     spec, atoms = pkg.workloads.packed_column(n=args.n)

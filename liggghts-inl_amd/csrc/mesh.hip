@@ -21,11 +21,16 @@
 
 namespace dem {
 
-template <bool FILL>
+// G lanes per particle test the triangles of its grid cell G at a time; the
+// passing ones are compacted by ballot + popcount, so a row keeps the cell
+// list's (ascending id) order.
+template <bool FILL, int G>
 __global__ void __launch_bounds__(kBlock) k_mesh_nb(MeshNbArgs a)
 {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= a.n) return;
+    const int lane = threadIdx.x & (G - 1);
+    const int gshift = (threadIdx.x & 63) & ~(G - 1);
+    const int i = (blockIdx.x * kBlock + threadIdx.x) / G;
+    if (i >= a.n) return;   // uniform across the group
     const double4 p = a.xr[i];
     const double q[3] = {p.x - a.disp[0], p.y - a.disp[1], p.z - a.disp[2]};
     int c[3];
@@ -41,15 +46,21 @@ __global__ void __launch_bounds__(kBlock) k_mesh_nb(MeshNbArgs a)
         const double cs[3] = {p.x, p.y, p.z};
         const double r = p.w * a.cdf;
         const int base = FILL ? a.mstart[i] : 0;
-        for (int e = a.gstart[cell]; e < a.gstart[cell + 1]; e++) {
-            const int t = a.glist[e];
-            if (tri_neighbuild(a.tri[t], a.enorm + 9 * (size_t)t, r, cs, a.threshold)) {
-                if (FILL) a.mcand[base + k] = t;
-                k++;
+        const int e0 = a.gstart[cell], e1 = a.gstart[cell + 1];
+        for (int eb = e0; eb < e1; eb += G) {
+            const int e = eb + lane;
+            int t = 0;
+            bool pass = false;
+            if (e < e1) {
+                t = a.glist[e];
+                pass = tri_neighbuild(a.tri[t], a.enorm + 9 * (size_t)t, r, cs, a.threshold);
             }
+            const uint64_t m = (__ballot(pass) >> gshift) & ((1ull << G) - 1ull);
+            if (FILL && pass) a.mcand[base + k + __popcll(m & ((1ull << lane) - 1ull))] = t;
+            k += __popcll(m);
         }
     }
-    if (!FILL) a.count[i] = k;
+    if (!FILL && lane == 0) a.count[i] = k;
 }
 
 __global__ void __launch_bounds__(kBlock) k_mesh_remap(MeshRemapArgs a)
@@ -129,11 +140,18 @@ __global__ void __launch_bounds__(kBlock) k_mesh_remap(MeshRemapArgs a)
 constexpr int kKeep = 1 << 30;
 constexpr int kIdMask = kKeep - 1;
 
-template <int WN>
+// G lanes per particle evaluate the geometry of its candidate triangles G
+// at a time (no stores, so the gathers of a whole group are in flight
+// together); the candidates within reach then go, in ascending order, to
+// lane 0, which keeps the contact slots and evaluates the contacts exactly
+// as the one-thread form did.
+template <int WN, int G>
 __global__ void __launch_bounds__(kBlock) k_mesh_force(MeshForceArgs a)
 {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= a.n) return;
+    const int lane = threadIdx.x & (G - 1);
+    const int gshift = (threadIdx.x & 63) & ~(G - 1);
+    const int i = (blockIdx.x * kBlock + threadIdx.x) / G;
+    if (i >= a.n) return;   // uniform across the group
     const Model &M = a.M;
     const int base = a.mstart[i];
     const int nn = a.mstart[i + 1] - base;
@@ -151,16 +169,10 @@ __global__ void __launch_bounds__(kBlock) k_mesh_force(MeshForceArgs a)
         double *hist = a.mhist + (size_t)base * dnum;
         int np = a.mnp[i];
         double F[3] = {0.0, 0.0, 0.0}, T[3] = {0.0, 0.0, 0.0};
-        for (int k = 0; k < nn; k++) {
-            const int t = a.mcand[base + k];
+        // FixContactHistoryMesh::handleContact and the contact, lane 0
+        auto handle = [&](int t, double deltan, const double *delta, const double *bary, int region) {
             const TriGeo &g = a.tri[t];
-            double delta[3], bary[3];
-            int region = -1;
-            const double deltan = tri_contact(g, radi, cs, a.precision, delta, bary, region);
-            if (deltan > M.cutneighmax) continue;
             const bool intersect = deltan <= 0;
-            if (!(deltan <= 0 || deltan < cdm * radi)) continue;
-            // FixContactHistoryMesh::handleContact
             int slot = -1;
             for (int s = 0; s < nn; s++) {
                 const int pv = part[s];
@@ -182,7 +194,7 @@ __global__ void __launch_bounds__(kBlock) k_mesh_force(MeshForceArgs a)
                         if (pp != t && tri_coplanar_node_neighs(a.tri[pp], t, g, a.curvature, a.precision))
                             already = true;
                     }
-                    if (already) continue;
+                    if (already) return;
                 }
                 for (int s = 0; s < nn; s++)
                     if (part[s] == -1) {
@@ -191,7 +203,7 @@ __global__ void __launch_bounds__(kBlock) k_mesh_force(MeshForceArgs a)
                     }
                 if (slot < 0 || np >= nn) {
                     atomicOr(a.err, 4);
-                    continue;
+                    return;
                 }
                 part[slot] = t | kKeep;
                 double *h = hist + (size_t)slot * dnum;
@@ -241,28 +253,59 @@ __global__ void __launch_bounds__(kBlock) k_mesh_force(MeshForceArgs a)
             } else {
                 contact_close(M.lw, hp);
             }
-        }
-        // cleanUpContacts
-        for (int s = 0; s < nn; s++) {
-            const int pv = part[s];
-            if (pv >= 0 && (pv & kKeep)) {
-                part[s] = pv & kIdMask;
-                continue;
+        };
+        for (int k0 = 0; k0 < nn; k0 += G) {
+            const int k = k0 + lane;
+            double delta[3] = {0.0, 0.0, 0.0}, bary[3] = {0.0, 0.0, 0.0};
+            double deltan = 0.0;
+            int region = -1, t = 0;
+            bool reach = false;
+            if (k < nn) {
+                t = a.mcand[base + k];
+                deltan = tri_contact(a.tri[t], radi, cs, a.precision, delta, bary, region);
+                reach = !(deltan > M.cutneighmax) && (deltan <= 0 || deltan < cdm * radi);
             }
-            if (pv > -1) np--;
-            part[s] = -1;
-            for (int d = 0; d < dnum; d++) hist[(size_t)s * dnum + d] = 0.0;
+            uint64_t m = (__ballot(reach) >> gshift) & ((1ull << G) - 1ull);
+            while (m) {   // the same mask on every lane of the group
+                const int b = __ffsll((long long)m) - 1;
+                m &= m - 1;
+                // every lane of the group takes part in the shuffles
+                const int tb = __shfl(t, b, G);
+                const double db = __shfl(deltan, b, G);
+                const int rb = __shfl(region, b, G);
+                double dl[3], br[3];
+                for (int q = 0; q < 3; q++) {
+                    dl[q] = __shfl(delta[q], b, G);
+                    br[q] = __shfl(bary[q], b, G);
+                }
+                if (lane == 0) handle(tb, db, dl, br, rb);
+            }
         }
-        a.mnp[i] = np;
-        if (!frozen) {
-            double4 f = a.f4[i], tq = a.t4[i];
-            f.x += F[0]; f.y += F[1]; f.z += F[2];
-            tq.x += T[0]; tq.y += T[1]; tq.z += T[2];
-            a.f4[i] = f;
-            a.t4[i] = tq;
+        if (lane == 0) {
+            // cleanUpContacts; a slot that is already empty holds zero
+            // history (it was zeroed when it emptied), so it is left alone
+            for (int s = 0; s < nn; s++) {
+                const int pv = part[s];
+                if (pv < 0) continue;
+                if (pv & kKeep) {
+                    part[s] = pv & kIdMask;
+                    continue;
+                }
+                np--;
+                part[s] = -1;
+                for (int d = 0; d < dnum; d++) hist[(size_t)s * dnum + d] = 0.0;
+            }
+            a.mnp[i] = np;
+            if (!frozen) {
+                double4 f = a.f4[i], tq = a.t4[i];
+                f.x += F[0]; f.y += F[1]; f.z += F[2];
+                tq.x += T[0]; tq.y += T[1]; tq.z += T[2];
+                a.f4[i] = f;
+                a.t4[i] = tq;
+            }
         }
     }
-    if (frozen) {
+    if (frozen && lane == 0) {
         a.f4[i] = make_double4(0.0, 0.0, 0.0, 0.0);
         a.t4[i] = make_double4(0.0, 0.0, 0.0, 0.0);
     }
@@ -285,14 +328,16 @@ __global__ void __launch_bounds__(kBlock) k_mesh_move(TriGeo *tri, int ntri, dou
 
 static int blocks(long long n) { return (int)((n + kBlock - 1) / kBlock); }
 
+constexpr int kMeshLanes = 8;   // lanes per particle in the mesh kernels
+
 void launch_mesh_count(const MeshNbArgs &a, hipStream_t s)
 {
-    if (a.n) k_mesh_nb<false><<<blocks(a.n), kBlock, 0, s>>>(a);
+    if (a.n) k_mesh_nb<false, kMeshLanes><<<blocks((long long)a.n * kMeshLanes), kBlock, 0, s>>>(a);
 }
 
 void launch_mesh_fill(const MeshNbArgs &a, hipStream_t s)
 {
-    if (a.n) k_mesh_nb<true><<<blocks(a.n), kBlock, 0, s>>>(a);
+    if (a.n) k_mesh_nb<true, kMeshLanes><<<blocks((long long)a.n * kMeshLanes), kBlock, 0, s>>>(a);
 }
 
 void launch_mesh_remap(const MeshRemapArgs &a, hipStream_t s)
@@ -303,10 +348,11 @@ void launch_mesh_remap(const MeshRemapArgs &a, hipStream_t s)
 void launch_mesh_force(const MeshForceArgs &a, hipStream_t s)
 {
     if (!a.n) return;
+    const int nb = blocks((long long)a.n * kMeshLanes);
     switch (a.M.lw.normal) {
-    case NM_HYST1: k_mesh_force<NM_HYST1><<<blocks(a.n), kBlock, 0, s>>>(a); break;
-    case NM_HYST2: k_mesh_force<NM_HYST2><<<blocks(a.n), kBlock, 0, s>>>(a); break;
-    default: k_mesh_force<NM_HERTZ><<<blocks(a.n), kBlock, 0, s>>>(a); break;
+    case NM_HYST1: k_mesh_force<NM_HYST1, kMeshLanes><<<nb, kBlock, 0, s>>>(a); break;
+    case NM_HYST2: k_mesh_force<NM_HYST2, kMeshLanes><<<nb, kBlock, 0, s>>>(a); break;
+    default: k_mesh_force<NM_HERTZ, kMeshLanes><<<nb, kBlock, 0, s>>>(a); break;
     }
 }
 

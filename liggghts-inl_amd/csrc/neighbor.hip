@@ -49,6 +49,7 @@ __global__ void __launch_bounds__(kBlock) k_bin(BinArgs a)
     if (i >= a.n) return;
     double4 p = a.xr[i];
     int4 im = a.img[i];
+    const int inst = meta_inst((int)a.wt[i].w);   // every load before the stores
     double *pc = &p.x;
     int *ic = &im.x;
     // Domain::pbc, orthogonal box
@@ -65,10 +66,10 @@ __global__ void __launch_bounds__(kBlock) k_bin(BinArgs a)
             ic[d]++;
         }
     }
+    const uint32_t k = cell_key(cell_coord(p.x, 0, a.G), cell_coord(p.y, 1, a.G),
+                                cell_coord(p.z, 2, a.G), a.G, inst);
     a.xr[i] = p;
     a.img[i] = im;
-    const uint32_t k = cell_key(cell_coord(p.x, 0, a.G), cell_coord(p.y, 1, a.G),
-                                cell_coord(p.z, 2, a.G), a.G, meta_inst((int)a.wt[i].w));
     a.key[i] = k;
     atomicAdd(&a.count[k], 1);
 }
