@@ -224,6 +224,36 @@ int dem_list_stats(dem_ctx *ctx, double *mean_neigh, double *mean_contacts, int6
 int dem_sync(dem_ctx *ctx);
 int64_t dem_timestep(const dem_ctx *ctx);
 
+/* ---- Groups and per-group force fixes.  They run after every other
+ * post_force contribution (pair, primitive and mesh walls, gravity, freeze),
+ * in the order they were added, as in a script that defines them after its
+ * walls; while any exists, dem_step does not use graph stepping. ---- */
+/* Group bit 0..30: members[i] != 0 puts atom i (upload order) in the group
+ * (Group::assign, group.cpp:119). */
+int dem_set_group(dem_ctx *ctx, int bit, const int *members);
+/* fix setforce with constant values: the group's force components with
+ * style[d] != 0 are set to value[d], NULL components are left alone
+ * (FixSetForce::post_force, fix_setforce.cpp:246-277).  *id names it for
+ * dem_unfix and dem_force_op_vector. */
+int dem_add_setforce(dem_ctx *ctx, int bit, const int *style, const double *value, int *id);
+/* fix aveforce: each styled component becomes the group average plus the
+ * given value (FixAveForce::post_force, fix_aveforce.cpp:222-281); the z
+ * value is the equal-style expression c0 - (time - t0) * c1 evaluated at
+ * the step's time (constant when c1 = 0). */
+int dem_add_aveforce(dem_ctx *ctx, int bit, const int *style, double xvalue, double yvalue, double c0,
+                     double t0, double c1, int *id);
+/* unfix (Modify::delete_fix, modify.cpp:985). */
+int dem_unfix(dem_ctx *ctx, int id);
+/* The fix's vector: the group's total force before the fix acted, at the
+ * last force pass (FixSetForce::compute_vector, fix_setforce.cpp:348;
+ * FixAveForce::compute_vector, fix_aveforce.cpp:346). */
+int dem_force_op_vector(dem_ctx *ctx, int id, double *out3);
+/* End of a run command: Update::update_time (update.cpp:511, called from
+ * Verlet::cleanup, verlet.cpp:398).  dem_time is atime + (ntimestep -
+ * atimestep) * dt, the time equal-style variables see. */
+int dem_cleanup(dem_ctx *ctx);
+double dem_time(const dem_ctx *ctx);
+
 /* ---- Batched runs.  No counterpart in the reference, which runs one
  * simulation per process (one LAMMPS instance, lammps.cpp); these let many
  * independent simulations of the same box, walls and contact model share one

@@ -111,6 +111,7 @@ struct dem_ctx {
     double4 *xr[2] = {}, *vm[2] = {}, *wt[2] = {};
     double4 *f4 = nullptr, *t4 = nullptr, *xhold = nullptr;
     int *tag[2] = {}, *orig[2] = {};
+    int *gmask[2] = {};      // group bits of each particle (fix setforce / aveforce)
     int4 *img[2] = {};
     double *whist[2] = {};
     uint8_t *wmask = nullptr;
@@ -142,6 +143,19 @@ struct dem_ctx {
     Model M{};
     Planes P{};
     long long ntimestep = 0;
+    // Update::atime / atimestep (update.cpp): the time at the end of the last run
+    double atime = 0.0;
+    long long atimestep = 0;
+    // fix setforce / aveforce after the other post_force contributions, in
+    // the order they were added
+    struct ForceOp {
+        int id, kind, bit, slot;   // kind 0 setforce, 1 aveforce
+        int style[3];
+        double val[3], c0, t0, c1;   // aveforce z: c0 - (time - t0) * c1
+    };
+    std::vector<ForceOp> fops;
+    int next_op_id = 1;
+    double *d_gpart = nullptr, *d_gtot = nullptr;   // partial sums; 4 per op slot
     int ago = 0;
 
     bool timing = false;
@@ -356,12 +370,13 @@ static void free_all(dem_ctx *ctx)
     for (int b = 0; b < 2; b++) {
         F(ctx->xr[b]); F(ctx->vm[b]); F(ctx->wt[b]); F(ctx->tag[b]); F(ctx->orig[b]);
         F(ctx->img[b]); F(ctx->whist[b]); F(ctx->nstart[b]); F(ctx->nj[b]); F(ctx->hist[b]);
+        F(ctx->gmask[b]);
     }
     F(ctx->f4); F(ctx->t4); F(ctx->xhold); F(ctx->wmask); F(ctx->ncontact); F(ctx->key);
     F(ctx->sorted); F(ctx->cstart); F(ctx->cursor); F(ctx->scan_tmp); F(ctx->spread); F(ctx->stage);
     F(ctx->d_mat); F(ctx->d_small);
     F(ctx->bout); F(ctx->bvalid); F(ctx->brev);
-    F(ctx->pwf); F(ctx->d_obs);
+    F(ctx->pwf); F(ctx->d_obs); F(ctx->d_gtot); F(ctx->d_gpart);
     for (char *c : ctx->ckpts) F(c);
     F(ctx->tape); F(ctx->adj_rev);
     if (ctx->gexec) hipGraphExecDestroy(ctx->gexec);
@@ -757,7 +772,7 @@ extern "C" int dem_upload_state(dem_ctx *ctx, int n, const double *x, const doub
                 || dev_alloc(ctx, &ctx->wt[b], c) || dev_alloc(ctx, &ctx->tag[b], c)
                 || dev_alloc(ctx, &ctx->orig[b], c) || dev_alloc(ctx, &ctx->img[b], c)
                 || dev_alloc(ctx, &ctx->whist[b], c * std::max(wdn, 1))
-                || dev_alloc(ctx, &ctx->nstart[b], c + 1))
+                || dev_alloc(ctx, &ctx->nstart[b], c + 1) || dev_alloc(ctx, &ctx->gmask[b], c))
                 return DEM_ERR_HIP;
         }
         if (dev_alloc(ctx, &ctx->f4, c) || dev_alloc(ctx, &ctx->t4, c) || dev_alloc(ctx, &ctx->xhold, c)
@@ -781,6 +796,7 @@ extern "C" int dem_upload_state(dem_ctx *ctx, int n, const double *x, const doub
         HIPCHK(hipMemset(ctx->f4, 0, n * sizeof(double4)));
         HIPCHK(hipMemset(ctx->t4, 0, n * sizeof(double4)));
         HIPCHK(hipMemset(ctx->wmask, 0, n));
+        HIPCHK(hipMemset(ctx->gmask[0], 0, n * sizeof(int)));
     }
     ctx->have_list = false;
     ctx->list_valid = false;
@@ -935,6 +951,8 @@ static int rebuild(dem_ctx *ctx, bool captured = false)
     pa.xr1 = ctx->xr[c1]; pa.vm1 = ctx->vm[c1]; pa.wt1 = ctx->wt[c1];
     pa.tag1 = ctx->tag[c1]; pa.orig1 = ctx->orig[c1]; pa.img1 = ctx->img[c1];
     pa.whist1 = ctx->whist[c1];
+    pa.gm0 = ctx->gmask[c0];
+    pa.gm1 = ctx->gmask[c1];
     pa.xhold = ctx->xhold;
     pa.wmask = ctx->wmask;
     pa.n = n;
@@ -1104,6 +1122,28 @@ static int force_pass(dem_ctx *ctx, int shearupdate)
         ma.M = ctx->M;
         launch_mesh_force(ma, ctx->stream);
         KCHK();
+    }
+    if (!ctx->fops.empty()) {
+        // Update::atime + (ntimestep - atimestep) * dt, as the reference's
+        // equal-style variables see it
+        const double time = ctx->atime + (double)(ctx->ntimestep - ctx->atimestep) * ctx->p.dt;
+        for (const auto &o : ctx->fops) {
+            GroupForceArgs ga{};
+            ga.f4 = ctx->f4;
+            ga.gmask = ctx->gmask[c];
+            ga.n = ctx->n;
+            ga.bit = o.bit;
+            ga.aveforce = o.kind;
+            for (int d = 0; d < 3; d++) {
+                ga.style[d] = o.style[d];
+                ga.val[d] = o.val[d];
+            }
+            if (o.kind == 1) ga.val[2] = o.c0 - (time - o.t0) * o.c1;
+            ga.part = ctx->d_gpart;
+            ga.total = ctx->d_gtot + 4 * o.slot;
+            launch_group_force(ga, ctx->stream);
+            KCHK();
+        }
     }
     return DEM_OK;
 }
@@ -1367,7 +1407,7 @@ static int tape_store(dem_ctx *ctx, int mask_step)
 // between steps (meshes, bonds, timers).
 static bool graph_ok(const dem_ctx *ctx)
 {
-    return ctx->use_graph && !ctx->window && !ctx->p.neigh_check && ctx->p.neigh_delay <= ctx->p.neigh_every
+    return ctx->use_graph && !ctx->window && ctx->fops.empty() && !ctx->p.neigh_check && ctx->p.neigh_delay <= ctx->p.neigh_every
            && ctx->meshes.empty() && ctx->M.cohesion == CM_OFF && !ctx->timing && ctx->n > 0;
 }
 
@@ -1855,6 +1895,119 @@ extern "C" int64_t dem_timestep(const dem_ctx *ctx)
 // no neighbour search crosses instances, and the sort leaves each instance's
 // particles contiguous.
 
+// ---- groups, fix setforce / fix aveforce, run boundaries
+
+constexpr int kMaxForceOps = 16;
+
+extern "C" int dem_set_group(dem_ctx *ctx, int bit, const int *members)
+{
+    if (!ctx) return fail(nullptr, DEM_ERR_ARG, "null context");
+    if (bit < 0 || bit > 30 || (ctx->n > 0 && !members)) return fail(ctx, DEM_ERR_ARG, "dem_set_group: bad arguments");
+    sync_enter(ctx);
+    const int n = ctx->n, c = ctx->cur;
+    if (n == 0) return DEM_OK;
+    // members are in upload order; orig maps each device slot to it
+    std::vector<int> og(n), gm(n);
+    HIPCHK(hipMemcpy(og.data(), ctx->orig[c], n * sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(gm.data(), ctx->gmask[c], n * sizeof(int), hipMemcpyDeviceToHost));
+    for (int q = 0; q < n; q++) {
+        if (members[og[q]]) gm[q] |= 1 << bit;
+        else gm[q] &= ~(1 << bit);
+    }
+    HIPCHK(hipMemcpy(ctx->gmask[c], gm.data(), n * sizeof(int), hipMemcpyHostToDevice));
+    return DEM_OK;
+}
+
+static int add_force_op(dem_ctx *ctx, dem_ctx::ForceOp o, int *id)
+{
+    if ((int)ctx->fops.size() >= kMaxForceOps) return fail(ctx, DEM_ERR_ARG, "at most 16 setforce/aveforce fixes");
+    if (o.bit < 0 || o.bit > 30) return fail(ctx, DEM_ERR_ARG, "group bit out of range");
+    sync_enter(ctx);
+    if (!ctx->d_gtot) {
+        if (dev_alloc(ctx, &ctx->d_gtot, 4 * kMaxForceOps) || dev_alloc(ctx, &ctx->d_gpart, 4 * kGroupBlocks))
+            return DEM_ERR_NOMEM;
+        HIPCHK(hipMemset(ctx->d_gtot, 0, 4 * kMaxForceOps * sizeof(double)));
+    }
+    std::vector<bool> used(kMaxForceOps, false);
+    for (const auto &q : ctx->fops) used[q.slot] = true;
+    o.slot = 0;
+    while (used[o.slot]) o.slot++;
+    HIPCHK(hipMemset(ctx->d_gtot + 4 * o.slot, 0, 4 * sizeof(double)));
+    o.id = ctx->next_op_id++;
+    ctx->fops.push_back(o);
+    if (id) *id = o.id;
+    return DEM_OK;
+}
+
+extern "C" int dem_add_setforce(dem_ctx *ctx, int bit, const int *style, const double *value, int *id)
+{
+    if (!ctx) return fail(nullptr, DEM_ERR_ARG, "null context");
+    if (!style || !value) return fail(ctx, DEM_ERR_ARG, "dem_add_setforce: bad arguments");
+    dem_ctx::ForceOp o{};
+    o.kind = 0;
+    o.bit = bit;
+    for (int d = 0; d < 3; d++) {
+        o.style[d] = style[d] ? 1 : 0;
+        o.val[d] = value[d];
+    }
+    return add_force_op(ctx, o, id);
+}
+
+extern "C" int dem_add_aveforce(dem_ctx *ctx, int bit, const int *style, double xvalue, double yvalue, double c0,
+                                double t0, double c1, int *id)
+{
+    if (!ctx) return fail(nullptr, DEM_ERR_ARG, "null context");
+    if (!style) return fail(ctx, DEM_ERR_ARG, "dem_add_aveforce: bad arguments");
+    dem_ctx::ForceOp o{};
+    o.kind = 1;
+    o.bit = bit;
+    for (int d = 0; d < 3; d++) o.style[d] = style[d] ? 1 : 0;
+    o.val[0] = xvalue;
+    o.val[1] = yvalue;
+    o.c0 = c0;
+    o.t0 = t0;
+    o.c1 = c1;
+    return add_force_op(ctx, o, id);
+}
+
+extern "C" int dem_unfix(dem_ctx *ctx, int id)
+{
+    if (!ctx) return fail(nullptr, DEM_ERR_ARG, "null context");
+    for (size_t k = 0; k < ctx->fops.size(); k++)
+        if (ctx->fops[k].id == id) {
+            ctx->fops.erase(ctx->fops.begin() + k);
+            return DEM_OK;
+        }
+    return fail(ctx, DEM_ERR_ARG, "dem_unfix: no such fix");
+}
+
+extern "C" int dem_force_op_vector(dem_ctx *ctx, int id, double *out)
+{
+    if (!ctx || !out) return fail(ctx, DEM_ERR_ARG, "dem_force_op_vector: bad arguments");
+    for (const auto &o : ctx->fops)
+        if (o.id == id) {
+            sync_enter(ctx);
+            double t[4];
+            HIPCHK(hipMemcpy(t, ctx->d_gtot + 4 * o.slot, sizeof(t), hipMemcpyDeviceToHost));
+            for (int d = 0; d < 3; d++) out[d] = t[d];
+            return DEM_OK;
+        }
+    return fail(ctx, DEM_ERR_ARG, "dem_force_op_vector: no such fix");
+}
+
+extern "C" int dem_cleanup(dem_ctx *ctx)
+{
+    if (!ctx) return fail(nullptr, DEM_ERR_ARG, "null context");
+    ctx->atime += (double)(ctx->ntimestep - ctx->atimestep) * ctx->p.dt;
+    ctx->atimestep = ctx->ntimestep;
+    return DEM_OK;
+}
+
+extern "C" double dem_time(const dem_ctx *ctx)
+{
+    return ctx ? ctx->atime + (double)(ctx->ntimestep - ctx->atimestep) * ctx->p.dt : 0.0;
+}
+
 extern "C" int dem_set_instances(dem_ctx *ctx, int ninst, const int *inst)
 {
     if (!ctx) return fail(nullptr, DEM_ERR_ARG, "null context");
@@ -2039,11 +2192,11 @@ static int adjoint_requirements(dem_ctx *ctx)
     const dem_params &p = ctx->p;
     if (p.normal_model != 0 || p.wall_normal_model != 0 || p.rolling_model != 0 || p.cohesion_model != 0
         || p.limit_force || p.wall_limit_force || !p.tangential_damping || p.cdf != 1.0 || p.neigh_check
-        || p.precision != DEM_FP64 || !ctx->meshes.empty() || ctx->M.freeze_on)
+        || p.precision != DEM_FP64 || !ctx->meshes.empty() || ctx->M.freeze_on || !ctx->fops.empty())
         return fail(ctx, DEM_ERR_ARG,
                     "adjoint window: needs FP64 Hertz pairs and planes with tangential history (tangential "
-                    "damping on, limitForce off), no rolling, bonds, meshes or freeze, contactDistanceFactor 1 "
-                    "and neigh_modify check no");
+                    "damping on, limitForce off), no rolling, bonds, meshes, freeze, setforce or aveforce, "
+                    "contactDistanceFactor 1 and neigh_modify check no");
     return DEM_OK;
 }
 

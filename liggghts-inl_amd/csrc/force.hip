@@ -26,6 +26,8 @@
 
 #include <stdlib.h>
 
+#include <algorithm>
+
 #include <type_traits>
 
 namespace dem {
@@ -694,6 +696,77 @@ void launch_check_distance(const double4 *xr, const double4 *xhold, int n, doubl
 {
     if (n == 0) return;
     k_check_distance<<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(xr, xhold, n, deltasq, flag);
+}
+
+
+// ---------------------------------------------------------------------------
+// fix setforce / fix aveforce.  k_group_sum: each thread walks particles i,
+// i + stride, ... in order, a fixed tree sums the block, one partial per
+// block; setforce sets its components in the same pass (each particle is read
+// and written by one thread).  k_group_total: the partials in block order.
+// k_group_apply (aveforce): the average plus the added value.
+
+constexpr int kGroupThreads = 256;
+
+__global__ void __launch_bounds__(kGroupThreads) k_group_sum(GroupForceArgs a)
+{
+    __shared__ double s_p[4][kGroupThreads];
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    const unsigned bit = 1u << a.bit;
+    for (int i = blockIdx.x * kGroupThreads + threadIdx.x; i < a.n; i += gridDim.x * kGroupThreads) {
+        if (!((unsigned)a.gmask[i] & bit)) continue;
+        double4 f = a.f4[i];
+        acc[0] += f.x;
+        acc[1] += f.y;
+        acc[2] += f.z;
+        acc[3] += 1.0;
+        if (!a.aveforce) {
+            if (a.style[0]) f.x = a.val[0];
+            if (a.style[1]) f.y = a.val[1];
+            if (a.style[2]) f.z = a.val[2];
+            a.f4[i] = f;
+        }
+    }
+    for (int c = 0; c < 4; c++) s_p[c][threadIdx.x] = acc[c];
+    __syncthreads();
+    for (int w = kGroupThreads / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int c = 0; c < 4; c++) s_p[c][threadIdx.x] += s_p[c][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x < 4) a.part[blockIdx.x * 4 + threadIdx.x] = s_p[threadIdx.x][0];
+}
+
+__global__ void k_group_total(GroupForceArgs a, int nb)
+{
+    if (threadIdx.x >= 4) return;
+    double t = 0.0;
+    for (int b = 0; b < nb; b++) t += a.part[b * 4 + threadIdx.x];
+    a.total[threadIdx.x] = t;
+}
+
+__global__ void __launch_bounds__(kBlock) k_group_apply(GroupForceArgs a)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    if (!((unsigned)a.gmask[i] & (1u << a.bit))) return;
+    const double cnt = a.total[3];
+    if (cnt == 0.0) return;
+    const int ncount = (int)cnt;
+    double4 f = a.f4[i];
+    if (a.style[0]) f.x = a.total[0] / ncount + a.val[0];
+    if (a.style[1]) f.y = a.total[1] / ncount + a.val[1];
+    if (a.style[2]) f.z = a.total[2] / ncount + a.val[2];
+    a.f4[i] = f;
+}
+
+void launch_group_force(const GroupForceArgs &a, hipStream_t s)
+{
+    if (a.n == 0) return;
+    const int nb = (int)std::min<long long>(kGroupBlocks, ((long long)a.n + kGroupThreads - 1) / kGroupThreads);
+    k_group_sum<<<nb, kGroupThreads, 0, s>>>(a);
+    k_group_total<<<1, 64, 0, s>>>(a, nb);
+    if (a.aveforce) k_group_apply<<<(a.n + kBlock - 1) / kBlock, kBlock, 0, s>>>(a);
 }
 
 }  // namespace dem

@@ -66,6 +66,8 @@ struct PermuteArgs {
     int *tag1, *orig1;
     int4 *img1;
     double *whist1;
+    const int *gm0;      // group bits
+    int *gm1;
     uint8_t *wmask;
     int n;
     int wdn;             // nplanes * dnum
@@ -227,6 +229,25 @@ void launch_neigh_count(const NeighArgs &a, hipStream_t s);
 // *maxcount = max(*maxcount, max of count[0..n))
 void launch_row_max(const int *count, int n, int *maxcount, hipStream_t s);
 void launch_neigh_fill(const NeighArgs &a, hipStream_t s);
+
+// fix setforce / fix aveforce on a group (FixSetForce::post_force,
+// fix_setforce.cpp:246-277; FixAveForce::post_force, fix_aveforce.cpp:
+// 222-281): the group's force before the op and its count are summed in a
+// fixed order (per-block partial sums, then an ordered total into total[4]);
+// setforce sets the styled components to val, aveforce to the group average
+// plus val.
+constexpr int kGroupBlocks = 256;
+struct GroupForceArgs {
+    double4 *f4;
+    const int *gmask;
+    int n, bit;
+    int aveforce;
+    int style[3];        // 1: the component is set, 0: NULL
+    double val[3];
+    double *part;        // kGroupBlocks * 4
+    double *total;       // 4: force sum and count of the group before the op
+};
+void launch_group_force(const GroupForceArgs &a, hipStream_t s);
 // one search: counts (then the row max) and staging rows
 void launch_neigh_stage(const NeighArgs &a, hipStream_t s);
 // staging rows into the list (after the scan); no-op when a row outgrew kStageSlots

@@ -228,6 +228,7 @@ __global__ void __launch_bounds__(kBlock) k_permute(PermuteArgs a)
     const int tg = a.tag0[o];
     const int og = a.orig0[o];
     const int4 im = a.img0[o];
+    const int gm = a.gm0[o];
     // PrimitiveWall::buildNeighList with treshold = skin
     // (fix_wall_gran.cpp:700-703, primitive_wall_definitions.h:142-148)
     uint32_t m = 0;
@@ -247,6 +248,7 @@ __global__ void __launch_bounds__(kBlock) k_permute(PermuteArgs a)
     a.tag1[p] = tg;
     a.orig1[p] = og;
     a.img1[p] = im;
+    a.gm1[p] = gm;
     a.wmask[p] = (uint8_t)m;
     // a short wall-history row (one batch of loads) goes here; longer ones
     // in k_permute_whist

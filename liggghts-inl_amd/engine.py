@@ -141,6 +141,14 @@ def library():
         "dem_get_mesh_contacts": (C.c_int, [vp, C.c_int, i64p, ip, ip, ip, dp]),
         "dem_set_mesh_contacts": (C.c_int, [vp, C.c_int, C.c_int64, ip, ip, ip, dp]),
         "dem_set_bond": (C.c_int, [vp, dp, C.POINTER(BondSettings)]),
+        "dem_set_group": (C.c_int, [vp, C.c_int, ip]),
+        "dem_add_setforce": (C.c_int, [vp, C.c_int, ip, dp, ip]),
+        "dem_add_aveforce": (C.c_int, [vp, C.c_int, ip, C.c_double, C.c_double, C.c_double, C.c_double,
+                                       C.c_double, ip]),
+        "dem_unfix": (C.c_int, [vp, C.c_int]),
+        "dem_force_op_vector": (C.c_int, [vp, C.c_int, dp]),
+        "dem_cleanup": (C.c_int, [vp]),
+        "dem_time": (C.c_double, [vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -418,6 +426,49 @@ class Simulation:
 
     def timestep(self):
         return self.L.dem_timestep(self.h)
+
+    # ---- groups and per-group force fixes (applied after the other
+    # post_force contributions, in the order added)
+    def set_group(self, bit, members):
+        """group assign: members[i] != 0 puts atom i (upload order) in group bit."""
+        m = np.ascontiguousarray(members, dtype=np.int32)
+        self._chk(self.L.dem_set_group(self.h, int(bit), _ip(m)))
+
+    def add_setforce(self, bit, values):
+        """fix setforce with constant values; None = NULL.  Returns the fix id."""
+        st = np.array([v is not None for v in values], dtype=np.int32)
+        va = np.array([0.0 if v is None else float(v) for v in values])
+        fid = C.c_int(0)
+        self._chk(self.L.dem_add_setforce(self.h, int(bit), _ip(st), _dp(va), C.byref(fid)))
+        return fid.value
+
+    def add_aveforce(self, bit, xv, yv, c0, t0=0.0, c1=0.0):
+        """fix aveforce with z = c0 - (time - t0)*c1 (None x/y = NULL).
+        Returns the fix id."""
+        st = np.array([xv is not None, yv is not None, True], dtype=np.int32)
+        fid = C.c_int(0)
+        self._chk(self.L.dem_add_aveforce(self.h, int(bit), _ip(st), 0.0 if xv is None else float(xv),
+                                          0.0 if yv is None else float(yv), float(c0), float(t0), float(c1),
+                                          C.byref(fid)))
+        return fid.value
+
+    def unfix(self, fid):
+        self._chk(self.L.dem_unfix(self.h, int(fid)))
+
+    def force_op_vector(self, fid):
+        """The group's force before the fix acted, at the last force pass."""
+        out = np.zeros(3)
+        self._chk(self.L.dem_force_op_vector(self.h, int(fid), _dp(out)))
+        return out
+
+    aveforce_vector = force_op_vector
+
+    def cleanup(self):
+        """End of a `run` command (time bookkeeping)."""
+        self._chk(self.L.dem_cleanup(self.h))
+
+    def time(self):
+        return self.L.dem_time(self.h)
 
     # ---- batched runs
     def set_instances(self, inst, Y=None, e=None, mu=None):
