@@ -26,5 +26,6 @@ from .engine import (  # noqa: F401
     material_tables,
     mesh_topology,
 )
+from . import datafile  # noqa: F401
 from . import meshio  # noqa: F401
 from . import workloads  # noqa: F401
