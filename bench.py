@@ -30,7 +30,7 @@ HBM_PEAK_GBS = 8000.0
 # FETCH_SIZE and WRITE_SIZE counter passes of rocprofv3 on k_force_q<true,32>
 # (FETCH_SIZE scaled x2.0, WRITE_SIZE x1.0 against a 256 MiB device copy):
 # profiles/pmc_traffic_k_force_column1M.md.  Re-measured when the kernel changes.
-MEASURED_TRAFFIC_BPP = 783.6
+MEASURED_TRAFFIC_BPP = 888.2
 
 
 def package():
@@ -44,12 +44,13 @@ def package():
     return mod
 
 
-def contact_bytes_per_particle(K, C, H, wall_contacts_per_particle):
+def contact_bytes_per_particle(K, C, H, wall_contacts_per_particle, fused=True):
     """Algorithmic HBM bytes of one contact pass per particle, FP64 state:
     own x,v,omega (72) + r,m (16) + type (4) + f,torque out (48) = 140,
     neighbour indices 4K, history read+write 16 H per contact (pairs and
-    walls)."""
-    return 140.0 + 4.0 * K + 16.0 * H * (C + wall_contacts_per_particle)
+    walls); with the integrator fused into the pass, the new x/r, v/m and
+    omega/meta records (96) written to the other state buffers."""
+    return 140.0 + 4.0 * K + 16.0 * H * (C + wall_contacts_per_particle) + (96.0 if fused else 0.0)
 
 
 def run_gpu(pkg, spec, atoms, precision, steps, warmup):
@@ -365,7 +366,7 @@ def main():
             "traffic": MEASURED_TRAFFIC_BPP * n if main_prec == "mixed" else None,
             "traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE passes, copy-calibrated "
                               "(profiles/pmc_traffic_k_force_column1M.md)",
-            "kernel": "k_force_q<mixed, 32> (contact pass)",
+            "kernel": "k_force_q<mixed, 32> (contact pass, velocity-Verlet integrator fused)",
             "bytes_per_particle": bpp,
             "avg_launch_ms": avg_ms,
         },

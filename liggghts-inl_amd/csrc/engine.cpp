@@ -112,6 +112,10 @@ struct dem_ctx {
     double4 *f4 = nullptr, *t4 = nullptr, *xhold = nullptr;
     int *tag[2] = {}, *orig[2] = {};
     int *gmask[2] = {};      // group bits of each particle (fix setforce / aveforce)
+    // x, v, omega of a contact pass that integrates go to the other buffer:
+    // par = 0, the state is in xr/vm/wt[cur]; par = 1, in xn/vn/wn
+    double4 *xn = nullptr, *vn = nullptr, *wn = nullptr;
+    int par = 0;
     int4 *img[2] = {};
     double *whist[2] = {};
     uint8_t *wmask = nullptr;
@@ -218,6 +222,11 @@ static int dev_alloc(dem_ctx *ctx, T **ptr, size_t count)
     HIPCHK(hipMalloc((void **)ptr, count * sizeof(T)));
     return DEM_OK;
 }
+
+// the current x/v/omega buffers
+static double4 *sx(const dem_ctx *ctx) { return ctx->par ? ctx->xn : ctx->xr[ctx->cur]; }
+static double4 *sv(const dem_ctx *ctx) { return ctx->par ? ctx->vn : ctx->vm[ctx->cur]; }
+static double4 *sw(const dem_ctx *ctx) { return ctx->par ? ctx->wn : ctx->wt[ctx->cur]; }
 
 static void timer_begin(dem_ctx *ctx, Timer &t)
 {
@@ -369,6 +378,7 @@ static void free_all(dem_ctx *ctx)
     };
     for (int b = 0; b < 2; b++) {
         F(ctx->xr[b]); F(ctx->vm[b]); F(ctx->wt[b]); F(ctx->tag[b]); F(ctx->orig[b]);
+        if (b == 0) { F(ctx->xn); F(ctx->vn); F(ctx->wn); }
         F(ctx->img[b]); F(ctx->whist[b]); F(ctx->nstart[b]); F(ctx->nj[b]); F(ctx->hist[b]);
         F(ctx->gmask[b]);
     }
@@ -609,7 +619,7 @@ extern "C" int dem_set_bond(dem_ctx *ctx, const double *tables, const dem_bond_s
     // the model's contact distance factor (cohesion_model_bond.h:415-470,
     // cohesion_model_bond_nonlinear.h:330-380) with the smallest radius
     std::vector<double4> hx(ctx->n);
-    HIPCHK(hipMemcpy(hx.data(), ctx->xr[ctx->cur], sizeof(double4) * ctx->n, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(hx.data(), sx(ctx), sizeof(double4) * ctx->n, hipMemcpyDeviceToHost));
     double minrad = 1e300;
     for (const auto &q : hx) minrad = std::min(minrad, q.w);
     const bool nl = ctx->M.cohesion == CM_BOND_NONLINEAR;
@@ -778,12 +788,14 @@ extern "C" int dem_upload_state(dem_ctx *ctx, int n, const double *x, const doub
         if (dev_alloc(ctx, &ctx->f4, c) || dev_alloc(ctx, &ctx->t4, c) || dev_alloc(ctx, &ctx->xhold, c)
             || dev_alloc(ctx, &ctx->wmask, c) || dev_alloc(ctx, &ctx->ncontact, c)
             || dev_alloc(ctx, &ctx->key, c) || dev_alloc(ctx, &ctx->sorted, c)
-            || dev_alloc(ctx, &ctx->stage, c * kStageSlots))
+            || dev_alloc(ctx, &ctx->stage, c * kStageSlots) || dev_alloc(ctx, &ctx->xn, c)
+            || dev_alloc(ctx, &ctx->vn, c) || dev_alloc(ctx, &ctx->wn, c))
             return DEM_ERR_HIP;
         ctx->cap = n;
     }
     ctx->n = n;
     ctx->cur = 0;
+    ctx->par = 0;
     ctx->maxrad = maxrad;
     if (n > 0) {
         HIPCHK(hipMemcpy(ctx->xr[0], hx.data(), n * sizeof(double4), hipMemcpyHostToDevice));
@@ -832,7 +844,7 @@ static int mesh_rebuild(dem_ctx *ctx, MeshDev &m)
         m.ncap[b] = n + 1;
     }
     MeshNbArgs na{};
-    na.xr = ctx->xr[ctx->cur];
+    na.xr = sx(ctx);
     na.tri = m.tri;
     na.enorm = m.enorm;
     na.gstart = m.gstart;
@@ -924,8 +936,8 @@ static int rebuild(dem_ctx *ctx, bool captured = false)
     const int c0 = ctx->cur, c1 = 1 - ctx->cur;
     launch_fill_int(ctx->cstart, 0, G.nkeys + 1, s);
     BinArgs ba{};
-    ba.xr = ctx->xr[c0];
-    ba.wt = ctx->wt[c0];
+    ba.xr = sx(ctx);
+    ba.wt = sw(ctx);
     ba.img = ctx->img[c0];
     ba.key = ctx->key;
     ba.count = ctx->cstart;
@@ -945,7 +957,7 @@ static int rebuild(dem_ctx *ctx, bool captured = false)
     KCHK();
     PermuteArgs pa{};
     pa.sorted = ctx->sorted;
-    pa.xr0 = ctx->xr[c0]; pa.vm0 = ctx->vm[c0]; pa.wt0 = ctx->wt[c0];
+    pa.xr0 = sx(ctx); pa.vm0 = sv(ctx); pa.wt0 = sw(ctx);
     pa.tag0 = ctx->tag[c0]; pa.orig0 = ctx->orig[c0]; pa.img0 = ctx->img[c0];
     pa.whist0 = ctx->whist[c0];
     pa.xr1 = ctx->xr[c1]; pa.vm1 = ctx->vm[c1]; pa.wt1 = ctx->wt[c1];
@@ -962,6 +974,7 @@ static int rebuild(dem_ctx *ctx, bool captured = false)
     launch_permute(pa, s);
     KCHK();
     ctx->cur = c1;
+    ctx->par = 0;
 
     const int l0 = ctx->lcur, l1 = 1 - ctx->lcur;
     NeighArgs na{};
@@ -1050,11 +1063,18 @@ static int rebuild(dem_ctx *ctx, bool captured = false)
     return DEM_OK;
 }
 
-static int force_pass(dem_ctx *ctx, int shearupdate)
+static int integrate(dem_ctx *ctx, int mode);
+
+// The force pass; integ = 2 or 3 also runs final_integrate (and the next
+// step's initial_integrate), inside the contact pass when nothing adds to
+// the forces after it (no mesh walls, setforce/aveforce, bonds, adjoint
+// window or tape) and the queue kernel runs: the new state then goes to the
+// other x/v/omega buffers and par flips.
+static int force_pass(dem_ctx *ctx, int shearupdate, int integ = 0)
 {
     ForceArgs fa{};
     const int c = ctx->cur, l = ctx->lcur;
-    fa.xr = ctx->xr[c]; fa.vm = ctx->vm[c]; fa.wt = ctx->wt[c];
+    fa.xr = sx(ctx); fa.vm = sv(ctx); fa.wt = sw(ctx);
     fa.nstart = ctx->nstart[l];
     fa.nj = ctx->nj[l];
     fa.hist = ctx->hist[l];
@@ -1073,6 +1093,17 @@ static int force_pass(dem_ctx *ctx, int shearupdate)
     fa.obs_plane = ctx->obs_plane;
     fa.lanes = ctx->tune_lanes;
     fa.block = ctx->tune_block;
+    const bool fuse = integ && ctx->meshes.empty() && ctx->fops.empty() && !ctx->window && ctx->tape_pos < 0
+                      && force_can_fuse(fa);
+    if (fuse) {
+        fa.fuse = integ;
+        fa.xn = ctx->par ? ctx->xr[c] : ctx->xn;
+        fa.vn = ctx->par ? ctx->vm[c] : ctx->vn;
+        fa.wn = ctx->par ? ctx->wt[c] : ctx->wn;
+        fa.dtf = 0.5 * ctx->p.dt;   // as in integrate()
+        fa.dtv = ctx->p.dt;
+        fa.dtfrotate = fa.dtf / 0.4;
+    }
     if (ctx->M.cohesion != CM_OFF) {
         BondArgs ba{};
         ba.xr = fa.xr; ba.vm = fa.vm; ba.wt = fa.wt;
@@ -1098,6 +1129,7 @@ static int force_pass(dem_ctx *ctx, int shearupdate)
     else launch_force_fp64(fa, ctx->stream);
     if (!ctx->gexec_capturing) timer_end(ctx, ctx->t_contact);
     KCHK();
+    if (fuse) ctx->par ^= 1;
     for (size_t k = 0; k < ctx->meshes.size(); k++) {
         MeshDev &m = ctx->meshes[k];
         MeshForceArgs ma{};
@@ -1145,6 +1177,7 @@ static int force_pass(dem_ctx *ctx, int shearupdate)
             KCHK();
         }
     }
+    if (integ && !fuse) return integrate(ctx, integ);
     return DEM_OK;
 }
 
@@ -1210,7 +1243,7 @@ static int integrate(dem_ctx *ctx, int mode)
 {
     IntegArgs ia{};
     const int c = ctx->cur;
-    ia.xr = ctx->xr[c]; ia.vm = ctx->vm[c]; ia.wt = ctx->wt[c];
+    ia.xr = sx(ctx); ia.vm = sv(ctx); ia.wt = sw(ctx);
     ia.f4 = ctx->f4; ia.t4 = ctx->t4;
     ia.n = ctx->n;
     ia.dtf = 0.5 * ctx->p.dt;   // dtf = 0.5*dt*ftm2v (fix_nve.cpp:85), ftm2v = 1 (si)
@@ -1287,7 +1320,7 @@ static int decide(dem_ctx *ctx, int *out)
     }
     const double trig = 0.5 * ctx->p.skin;
     HIPCHK(hipMemsetAsync(ctx->d_small + 2, 0, sizeof(int), ctx->stream));
-    launch_check_distance(ctx->xr[ctx->cur], ctx->xhold, ctx->n, trig * trig, ctx->d_small + 2, ctx->stream);
+    launch_check_distance(sx(ctx), ctx->xhold, ctx->n, trig * trig, ctx->d_small + 2, ctx->stream);
     KCHK();
     int h = 0;
     HIPCHK(hipMemcpyAsync(&h, ctx->d_small + 2, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
@@ -1353,9 +1386,9 @@ static int copy_state(dem_ctx *ctx, char *rec, bool forces, bool save)
     auto acc = [&](hipError_t x) {
         if (x != hipSuccess) e = x;
     };
-    acc(cp(ctx->xr[c], v.xr, n * sizeof(double4)));
-    acc(cp(ctx->vm[c], v.vm, n * sizeof(double4)));
-    acc(cp(ctx->wt[c], v.wt, n * sizeof(double4)));
+    acc(cp(sx(ctx), v.xr, n * sizeof(double4)));
+    acc(cp(sv(ctx), v.vm, n * sizeof(double4)));
+    acc(cp(sw(ctx), v.wt, n * sizeof(double4)));
     if (forces) {
         acc(cp(ctx->f4, v.f4, n * sizeof(double4)));
         acc(cp(ctx->t4, v.t4, n * sizeof(double4)));
@@ -1373,7 +1406,7 @@ static int copy_state(dem_ctx *ctx, char *rec, bool forces, bool save)
 static int window_mask(dem_ctx *ctx)
 {
     AdjMaskArgs m{};
-    m.xr = ctx->xr[ctx->cur];
+    m.xr = sx(ctx);
     m.nstart = ctx->nstart[ctx->lcur];
     m.nj = ctx->nj[ctx->lcur];
     m.hist = ctx->hist[ctx->lcur];
@@ -1384,7 +1417,7 @@ static int window_mask(dem_ctx *ctx)
     launch_adj_mask(m, ctx->stream);
     KCHK();
     const double trig = 0.5 * ctx->p.skin;
-    launch_check_distance(ctx->xr[ctx->cur], ctx->xhold, ctx->n, trig * trig, ctx->d_small + 2, ctx->stream);
+    launch_check_distance(sx(ctx), ctx->xhold, ctx->n, trig * trig, ctx->d_small + 2, ctx->stream);
     KCHK();
     ctx->ago = 0;
     return DEM_OK;
@@ -1414,6 +1447,7 @@ static bool graph_ok(const dem_ctx *ctx)
 static std::vector<const void *> graph_key(const dem_ctx *ctx)
 {
     std::vector<const void *> k = {ctx->xr[0], ctx->xr[1], ctx->vm[0], ctx->vm[1], ctx->wt[0], ctx->wt[1],
+                                   ctx->xn, ctx->vn, ctx->wn, (const void *)(intptr_t)ctx->par,
                                    ctx->tag[0], ctx->tag[1], ctx->orig[0], ctx->orig[1], ctx->img[0],
                                    ctx->img[1], ctx->whist[0], ctx->whist[1], ctx->nstart[0], ctx->nstart[1],
                                    ctx->nj[0], ctx->nj[1], ctx->hist[0], ctx->hist[1], ctx->cstart,
@@ -1464,20 +1498,26 @@ static int capture_graph(dem_ctx *ctx)
         }
     }
     const int every = ctx->p.neigh_every;
-    const int cur0 = ctx->cur, lcur0 = ctx->lcur;
+    const int cur0 = ctx->cur, lcur0 = ctx->lcur, par0 = ctx->par;
     hipGraph_t graph = nullptr;
     HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
     ctx->gexec_capturing = true;
     int rc = DEM_OK;
     for (int k = 1; k <= 2 * every && rc == DEM_OK; k++) {
         if (k % every == 0) rc = rebuild(ctx, true);
-        if (rc == DEM_OK) rc = force_pass(ctx, 1);
-        if (rc == DEM_OK) rc = integrate(ctx, 3);
+        if (rc == DEM_OK) rc = force_pass(ctx, 1, 3);
     }
     ctx->gexec_capturing = false;
     const hipError_t ec = hipStreamEndCapture(ctx->stream, &graph);
+    // a replay must leave the state in the buffers it found it in
+    const bool same_par = ctx->par == par0;
     ctx->cur = cur0;
     ctx->lcur = lcur0;
+    ctx->par = par0;
+    if (rc == DEM_OK && ec == hipSuccess && !same_par) {
+        if (graph) hipGraphDestroy(graph);
+        return fail(ctx, DEM_ERR_STATE, "graph stepping: the recorded period changes the state buffers");
+    }
     if (rc) {
         if (graph) hipGraphDestroy(graph);
         return rc;
@@ -1525,9 +1565,8 @@ extern "C" int dem_step(dem_ctx *ctx, int nsteps)
         if (nflag && (rc = ctx->window ? window_mask(ctx) : rebuild(ctx))) return rc;
         if (!nflag) mesh_trigger(ctx);
         if (ctx->tape_pos >= 0 && (rc = tape_store(ctx, nflag))) return rc;
-        if ((rc = force_pass(ctx, 1))) return rc;
         const bool last = st == nsteps - 1;
-        if ((rc = integrate(ctx, last ? 2 : 3))) return rc;
+        if ((rc = force_pass(ctx, 1, last ? 2 : 3))) return rc;
         if (!last && (rc = move_meshes(ctx))) return rc;
     }
     return check_err_flag(ctx);
@@ -1543,9 +1582,9 @@ extern "C" int dem_download(dem_ctx *ctx, double *x, double *v, double *omega, d
     std::vector<int> ho(n);
     HIPCHK(hipStreamSynchronize(ctx->stream));
     HIPCHK(hipMemcpy(ho.data(), ctx->orig[c], n * sizeof(int), hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(hx.data(), ctx->xr[c], n * sizeof(double4), hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(hv.data(), ctx->vm[c], n * sizeof(double4), hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(hw.data(), ctx->wt[c], n * sizeof(double4), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(hx.data(), sx(ctx), n * sizeof(double4), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(hv.data(), sv(ctx), n * sizeof(double4), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(hw.data(), sw(ctx), n * sizeof(double4), hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(hf.data(), ctx->f4, n * sizeof(double4), hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(ht.data(), ctx->t4, n * sizeof(double4), hipMemcpyDeviceToHost));
     auto put = [&](double *dst, const std::vector<double4> &src) {
@@ -2019,14 +2058,14 @@ extern "C" int dem_set_instances(dem_ctx *ctx, int ninst, const int *inst)
         return fail(ctx, DEM_ERR_ARG, "dem_set_instances: mesh walls and bonds are not available in a batched run");
     sync_enter(ctx);
     std::vector<double4> hw(ctx->n);
-    HIPCHK(hipMemcpy(hw.data(), ctx->wt[ctx->cur], sizeof(double4) * ctx->n, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(hw.data(), sw(ctx), sizeof(double4) * ctx->n, hipMemcpyDeviceToHost));
     for (int i = 0; i < ctx->n; i++) {
         const int b = ninst > 1 ? inst[i] : 0;
         if (b < 0 || b >= ninst) return fail(ctx, DEM_ERR_ARG, "dem_set_instances: instance out of range");
         const int m = (int)hw[i].w;
         hw[i].w = (double)((m & 31) | (b << 5));
     }
-    HIPCHK(hipMemcpy(ctx->wt[ctx->cur], hw.data(), sizeof(double4) * ctx->n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(sw(ctx), hw.data(), sizeof(double4) * ctx->n, hipMemcpyHostToDevice));
     ctx->ninst = ninst;
     ctx->inst_mat.assign((size_t)ninst * (M_ETAR + 1) * kT1 * kT1, 0.0);
     ctx->inst_set.assign(ninst, 0);
@@ -2081,7 +2120,7 @@ extern "C" int dem_instance_observables(dem_ctx *ctx, double *out)
         if (dev_alloc(ctx, &ctx->d_obs, (size_t)B * 5)) return DEM_ERR_HIP;
         ctx->obs_cap = B * 5;
     }
-    launch_inst_obs(ctx->cstart, ctx->G.nkeys_local, B, ctx->xr[ctx->cur], ctx->vm[ctx->cur],
+    launch_inst_obs(ctx->cstart, ctx->G.nkeys_local, B, sx(ctx), sv(ctx),
                     ctx->obs_plane >= 0 ? ctx->pwf : nullptr, ctx->d_obs, ctx->stream);
     KCHK();
     HIPCHK(hipMemcpyAsync(out, ctx->d_obs, sizeof(double) * B * 5, hipMemcpyDeviceToHost, ctx->stream));
@@ -2316,8 +2355,8 @@ extern "C" int dem_adjoint_backward(dem_ctx *ctx, const double *w_height, const 
         return done(DEM_ERR_HIP);
     // seed at the final state: the mean height's gradient
     if ((rc = restore_ckpt(ctx, nseg))) return done(rc);
-    launch_inst_obs(ctx->cstart, ctx->G.nkeys_local, B, ctx->xr[ctx->cur], ctx->vm[ctx->cur], nullptr, d_cnt, s);
-    launch_adj_seed(ctx->wt[ctx->cur], w_height ? d_wh : nullptr, d_cnt, lx[0], n, s);
+    launch_inst_obs(ctx->cstart, ctx->G.nkeys_local, B, sx(ctx), sv(ctx), nullptr, d_cnt, s);
+    launch_adj_seed(sw(ctx), w_height ? d_wh : nullptr, d_cnt, lx[0], n, s);
     if (!hip_ok(hipGetLastError(), "seed kernel")) return done(DEM_ERR_HIP);
     // tape for the longest segment
     const size_t tb = state_record_bytes(ctx, false);

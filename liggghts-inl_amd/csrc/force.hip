@@ -296,6 +296,29 @@ __global__ void __launch_bounds__(kBlock) k_force(ForceArgs a)
 }
 
 // ---------------------------------------------------------------------------
+// final_integrate(n) and, with fuse = 3, initial_integrate(n+1) of one
+// particle from its state and its final force and torque: the operations of
+// k_integrate, written to the other state buffers
+__device__ inline void integrate_into(const ForceArgs &a, int i, double4 x, double4 v, double4 w, const double F[3],
+                                      const double T[3])
+{
+    // dtfm = dtf / (rmass*(1.+CAddRhoFluid/density)), CAddRhoFluid = 0
+    const double dtfm = a.dtf / (v.w * (1. + 0.0 / 1.0));
+    const double dtirotate = a.dtfrotate / (x.w * x.w * v.w);
+    if (a.fuse & 2) {
+        v.x += dtfm * F[0]; v.y += dtfm * F[1]; v.z += dtfm * F[2];
+        w.x += dtirotate * T[0]; w.y += dtirotate * T[1]; w.z += dtirotate * T[2];
+    }
+    if (a.fuse & 1) {
+        v.x += dtfm * F[0]; v.y += dtfm * F[1]; v.z += dtfm * F[2];
+        x.x += a.dtv * v.x; x.y += a.dtv * v.y; x.z += a.dtv * v.z;
+        w.x += dtirotate * T[0]; w.y += dtirotate * T[1]; w.z += dtirotate * T[2];
+    }
+    a.xn[i] = x;
+    a.vn[i] = v;
+    a.wn[i] = w;
+}
+
 // k_force_q: the same pass for Hertz pair and wall chains with a wave-level
 // contact queue.  A wave owns a tile of P consecutive particles, whose rows
 // form one contiguous range of the list.  The 64 lanes walk that range 64
@@ -525,6 +548,7 @@ __global__ void __launch_bounds__(kBlock) k_force_q(ForceArgs a)
         const int metai = (int)wi.w;
         const double *mat = a.mat + (size_t)meta_inst(metai) * kMatStride;
         particle_epilogue<MIXED, NM_HERTZ>(a, i, pi, vmi, wi, meta_type(metai), meta_frozen(metai), mat, F, T, nc);
+        if (a.fuse) integrate_into(a, i, pi, vmi, wi, F, T);
     }
 }
 
@@ -551,6 +575,11 @@ static int contact_shape(const ForceArgs &a)
         if (g != 1 && g != 2 && g != 4 && g != 8 && g != 16 && g != -8 && g != -16 && g != -32 && g != -64) g = -32;
     }
     return g;
+}
+
+bool force_can_fuse(const ForceArgs &a)
+{
+    return a.M.lp.normal == NM_HERTZ && a.M.lw.normal == NM_HERTZ && a.M.cohesion == CM_OFF && contact_shape(a) < 0;
 }
 
 // Hertz pairs with Hertz walls (the common case) take any group width;

@@ -24,6 +24,12 @@ struct ForceArgs {
     double *pwf;         // optional per-atom force of plane obs_plane along its axis
     int obs_plane;
     int lanes, block;    // launch shape (0: defaults)
+    // fuse = 2 or 3: the pass also runs final_integrate (and, with 3, the
+    // next step's initial_integrate) of each particle, writing the new x,
+    // v, omega to xn/vn/wn (never the arrays it reads)
+    int fuse;
+    double4 *xn, *vn, *wn;
+    double dtf, dtv, dtfrotate;
     Model M;
     Planes P;
 };
@@ -262,6 +268,9 @@ void launch_copy_int(int *dst, const int *src, int n, hipStream_t s);
 void launch_inst_obs(const int *cstart, int nkl, int ninst, const double4 *xr, const double4 *vm,
                      const double *pwf, double *out, hipStream_t s);
 void launch_force_fp64(const ForceArgs &a, hipStream_t s);
+// whether the contact pass of these arguments can integrate (the queue
+// kernel of Hertz pairs and walls)
+bool force_can_fuse(const ForceArgs &a);
 void launch_force_mixed(const ForceArgs &a, hipStream_t s);
 // mode: 1 = initial_integrate, 2 = final_integrate, 3 = final then initial
 void launch_integrate(const IntegArgs &a, int mode, hipStream_t s);
