@@ -176,6 +176,29 @@ __device__ inline void particle_epilogue(const ForceArgs &a, int i, const double
     if (a.pwf) a.pwf[i] = pw;
 }
 
+// final_integrate(n) and, with fuse = 3, initial_integrate(n+1) of one
+// particle from its state and its final force and torque: the operations of
+// k_integrate, written to the other state buffers
+__device__ inline void integrate_into(const ForceArgs &a, int i, double4 x, double4 v, double4 w, const double F[3],
+                                      const double T[3])
+{
+    // dtfm = dtf / (rmass*(1.+CAddRhoFluid/density)), CAddRhoFluid = 0
+    const double dtfm = a.dtf / (v.w * (1. + 0.0 / 1.0));
+    const double dtirotate = a.dtfrotate / (x.w * x.w * v.w);
+    if (a.fuse & 2) {
+        v.x += dtfm * F[0]; v.y += dtfm * F[1]; v.z += dtfm * F[2];
+        w.x += dtirotate * T[0]; w.y += dtirotate * T[1]; w.z += dtirotate * T[2];
+    }
+    if (a.fuse & 1) {
+        v.x += dtfm * F[0]; v.y += dtfm * F[1]; v.z += dtfm * F[2];
+        x.x += a.dtv * v.x; x.y += a.dtv * v.y; x.z += a.dtv * v.z;
+        w.x += dtirotate * T[0]; w.y += dtirotate * T[1]; w.z += dtirotate * T[2];
+    }
+    a.xn[i] = x;
+    a.vn[i] = v;
+    a.wn[i] = w;
+}
+
 template <bool MIXED, int G, int NM, int WN, bool CH = false>
 __global__ void __launch_bounds__(kBlock) k_force(ForceArgs a)
 {
@@ -293,32 +316,10 @@ __global__ void __launch_bounds__(kBlock) k_force(ForceArgs a)
     if (!live || g != 0) return;
 
     particle_epilogue<MIXED, WN>(a, i, pi, vmi, wi, itype, fzi, mat, F, T, nc);
+    if (a.fuse) integrate_into(a, i, pi, vmi, wi, F, T);
 }
 
 // ---------------------------------------------------------------------------
-// final_integrate(n) and, with fuse = 3, initial_integrate(n+1) of one
-// particle from its state and its final force and torque: the operations of
-// k_integrate, written to the other state buffers
-__device__ inline void integrate_into(const ForceArgs &a, int i, double4 x, double4 v, double4 w, const double F[3],
-                                      const double T[3])
-{
-    // dtfm = dtf / (rmass*(1.+CAddRhoFluid/density)), CAddRhoFluid = 0
-    const double dtfm = a.dtf / (v.w * (1. + 0.0 / 1.0));
-    const double dtirotate = a.dtfrotate / (x.w * x.w * v.w);
-    if (a.fuse & 2) {
-        v.x += dtfm * F[0]; v.y += dtfm * F[1]; v.z += dtfm * F[2];
-        w.x += dtirotate * T[0]; w.y += dtirotate * T[1]; w.z += dtirotate * T[2];
-    }
-    if (a.fuse & 1) {
-        v.x += dtfm * F[0]; v.y += dtfm * F[1]; v.z += dtfm * F[2];
-        x.x += a.dtv * v.x; x.y += a.dtv * v.y; x.z += a.dtv * v.z;
-        w.x += dtirotate * T[0]; w.y += dtirotate * T[1]; w.z += dtirotate * T[2];
-    }
-    a.xn[i] = x;
-    a.vn[i] = v;
-    a.wn[i] = w;
-}
-
 // k_force_q: the same pass for Hertz pair and wall chains with a wave-level
 // contact queue.  A wave owns a tile of P consecutive particles, whose rows
 // form one contiguous range of the list.  The 64 lanes walk that range 64
@@ -577,10 +578,7 @@ static int contact_shape(const ForceArgs &a)
     return g;
 }
 
-bool force_can_fuse(const ForceArgs &a)
-{
-    return a.M.lp.normal == NM_HERTZ && a.M.lw.normal == NM_HERTZ && a.M.cohesion == CM_OFF && contact_shape(a) < 0;
-}
+bool force_can_fuse(const ForceArgs &a) { return a.M.cohesion == CM_OFF; }
 
 // Hertz pairs with Hertz walls (the common case) take any group width;
 // other model combinations run with 4 lanes per particle.

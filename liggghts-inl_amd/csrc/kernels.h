@@ -268,8 +268,8 @@ void launch_copy_int(int *dst, const int *src, int n, hipStream_t s);
 void launch_inst_obs(const int *cstart, int nkl, int ninst, const double4 *xr, const double4 *vm,
                      const double *pwf, double *out, hipStream_t s);
 void launch_force_fp64(const ForceArgs &a, hipStream_t s);
-// whether the contact pass of these arguments can integrate (the queue
-// kernel of Hertz pairs and walls)
+// whether the contact pass of these arguments can integrate (every form
+// without a cohesion model)
 bool force_can_fuse(const ForceArgs &a);
 void launch_force_mixed(const ForceArgs &a, hipStream_t s);
 // mode: 1 = initial_integrate, 2 = final_integrate, 3 = final then initial
