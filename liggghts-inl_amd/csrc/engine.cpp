@@ -989,6 +989,7 @@ static int rebuild(dem_ctx *ctx, bool captured = false)
     na.cdf = ctx->p.cdf;
     na.skin = ctx->p.skin;
     na.rmax = ctx->maxrad;
+    for (int d = 0; d < 3; d++) na.cw[d] = 1.0 / G.inv[d];
     na.G = G;
     for (int d = 0; d < 3; d++) na.prd[d] = ctx->M.prd[d];
     na.perm = ctx->sorted;

@@ -106,6 +106,7 @@ struct NeighArgs {
     int dnum;
     double cdf, skin;
     double rmax;         // largest radius (the cell grid's)
+    double cw[3];        // cell widths, 1 / G.inv
     Grid G;
     double prd[3];
     int64_t ecap;        // fill pass: entries allocated

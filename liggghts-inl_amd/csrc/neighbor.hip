@@ -312,7 +312,11 @@ __global__ void __launch_bounds__(kBlock) k_neigh(NeighArgs a)
     constexpr int kGroups = kBlock / W;
     constexpr int kShift = W == 64 ? 6 : (W == 32 ? 5 : (W == 16 ? 4 : 3));
     __shared__ int s_ex[kGroups][28], s_qb[kGroups][27], s_code[kGroups][27];
-    __shared__ int s_ot[kGroups][64];                   // partner tags of the previous row (fill)
+    __shared__ double s_sh[kGroups][3][27];   // each cell's periodic image shift, shift * prd
+    constexpr int kOld = 32;                            // previous-row entries kept in LDS
+    constexpr int kMap = 256;                           // candidates with a cell map
+    __shared__ int s_ot[kGroups][kOld];                 // partner tags of the previous row (fill)
+    __shared__ uint8_t s_cell[kGroups][kMap];           // stencil cell of each candidate
     const int lane = threadIdx.x & (W - 1);            // lane within the particle's group
     const int wv = threadIdx.x >> kShift;               // group within the block
     const int gshift = (threadIdx.x & 63) & ~(W - 1);   // the group's first bit in a wave ballot
@@ -356,7 +360,7 @@ __global__ void __launch_bounds__(kBlock) k_neigh(NeighArgs a)
         for (int d = 0; d < 3; d++)
             if (off[d] != 0) {
                 const double g = off[d] < 0 ? u[d] - cc[d] : cc[d] + 1 - u[d];
-                const double t = fmax(g - 1e-6, 0.0) / a.G.inv[d];
+                const double t = fmax(g - 1e-6, 0.0) * a.cw[d];
                 d2 += t * t;
             }
         if (ok && d2 <= cutmax * cutmax) {
@@ -387,6 +391,13 @@ __global__ void __launch_bounds__(kBlock) k_neigh(NeighArgs a)
             s_ex[wv][k] = total + incl - cnt;
             s_qb[wv][k] = qb;
             s_code[wv][k] = code;
+            int sx, sy, sz;
+            shift_decode(code, sx, sy, sz);
+            s_sh[wv][0][k] = sx * a.prd[0];
+            s_sh[wv][1][k] = sy * a.prd[1];
+            s_sh[wv][2][k] = sz * a.prd[2];
+            const int ex = total + incl - cnt;
+            for (int j = 0; j < cnt && ex + j < kMap; j++) s_cell[wv][ex + j] = (uint8_t)k;
         }
         total += __shfl(incl, W - 1, W);
     }
@@ -395,13 +406,14 @@ __global__ void __launch_bounds__(kBlock) k_neigh(NeighArgs a)
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 
     // the particle's previous row into LDS: partner tag of every flagged
-    // entry (0 = cannot match; tags are positive), its first 64 entries
+    // entry (0 = cannot match; tags are positive), its first kOld entries
+    // (a longer row is searched in memory past them)
     int ob = 0, oe = 0;
     if (OLD && a.have_old) {
         const int o = a.perm[p];
         ob = a.onstart[o];
         oe = a.onstart[o + 1];
-        for (int k = lane; k < min(oe - ob, 64); k += W) {
+        for (int k = lane; k < min(oe - ob, kOld); k += W) {
             const uint32_t ow = a.onj[ob + k];
             s_ot[wv][k] = (ow & (kFlagBit | kBondBit)) ? a.otag[ow & kJMask] : 0;
         }
@@ -409,7 +421,7 @@ __global__ void __launch_bounds__(kBlock) k_neigh(NeighArgs a)
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
-    const int on64 = min(oe - ob, 64);
+    const int onl = min(oe - ob, kOld);
     const int ebase = FILL ? a.nstart[p] : 0;
     // a list longer than the allocation (possible only when the rebuild runs
     // inside a captured graph, sized beforehand) is reported, not written
@@ -428,22 +440,27 @@ __global__ void __launch_bounds__(kBlock) k_neigh(NeighArgs a)
         bool pass = false, touch = false;
         int q = 0, qcode = kNoShift, tq = 0;
         if (c < total) {
-            int lo = 0, hi = 26;  // largest k with ex[k] <= c
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (s_ex[wv][mid] <= c) lo = mid;
-                else hi = mid - 1;
+            int lo = 0;   // the candidate's cell: the map, or the largest k with ex[k] <= c
+            if (total <= kMap) {
+                lo = s_cell[wv][c];
+            } else {
+                int hi = 26;
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (s_ex[wv][mid] <= c) lo = mid;
+                    else hi = mid - 1;
+                }
             }
             q = s_qb[wv][lo] + (c - s_ex[wv][lo]);
             qcode = s_code[wv][lo];
             if (OLD) tq = a.tag[q];   // issued beside the position load, not after the test
             if (!(q == p && qcode == kNoShift)) {
-                int sx, sy, sz;
-                shift_decode(qcode, sx, sy, sz);
+                // the cell's shift, decoded once per cell: the same values
+                // as sx * prd per candidate
                 const double4 xq = a.xr[q];
-                const double xj0 = xq.x + sx * a.prd[0];
-                const double xj1 = xq.y + sy * a.prd[1];
-                const double xj2 = xq.z + sz * a.prd[2];
+                const double xj0 = xq.x + s_sh[wv][0][lo];
+                const double xj1 = xq.y + s_sh[wv][1][lo];
+                const double xj2 = xq.z + s_sh[wv][2][lo];
                 const double delx = xp.x - xj0;
                 const double dely = xp.y - xj1;
                 const double delz = xp.z - xj2;
@@ -463,13 +480,13 @@ __global__ void __launch_bounds__(kBlock) k_neigh(NeighArgs a)
             const bool want = emit && touch && a.have_old;
             int found = -1;
             if (want) {
-                for (int k = 0; k < on64; k++)
+                for (int k = 0; k < onl; k++)
                     if (s_ot[wv][k] == tq) {
                         found = k;
                         break;
                     }
                 if (found < 0)
-                    for (int oe2 = ob + 64; oe2 < oe; oe2++) {
+                    for (int oe2 = ob + kOld; oe2 < oe; oe2++) {
                         const uint32_t ow = a.onj[oe2];
                         if ((ow & (kFlagBit | kBondBit)) && a.otag[ow & kJMask] == tq) {
                             found = oe2 - ob;
