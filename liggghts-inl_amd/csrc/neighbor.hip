@@ -531,11 +531,12 @@ __global__ void __launch_bounds__(kBlock) k_neigh(NeighArgs a)
 // entries first, then the others, each in scan order (a stable partition by
 // group ballots): the histories the next steps read and write then sit side
 // by side instead of between entries that do not touch.  A carried entry
-// takes its flag bits from the previous list.  Then the row's history block,
-// cnt * dnum contiguous doubles, one double per lane with consecutive lanes
-// on consecutive doubles (whole-line stores): a carried entry's slots come
-// from the previous list, a new entry's are zero.  Does nothing when a row
-// outgrew its staging row (the fill pass writes the list then).
+// takes its flag bits from the previous list.  Then the history of the
+// row's touching block (all of the row with a cohesion model), contiguous
+// doubles, one double per lane with consecutive lanes on consecutive doubles
+// (whole-line stores): a carried entry's slots come from the previous list,
+// a new entry's are zero.  Does nothing when a row outgrew its staging row
+// (the fill pass writes the list then).
 template <int G>
 __global__ void __launch_bounds__(kBlock) k_neigh_compact(NeighArgs a)
 {
@@ -579,7 +580,11 @@ __global__ void __launch_bounds__(kBlock) k_neigh_compact(NeighArgs a)
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     const int dnum = a.dnum;
     if (dnum == 0) return;
-    const int H = cnt * dnum;   // <= kStageSlots * dnum < 2^11
+    // the touching block (the first nt entries) takes its history; the
+    // entries after it have flag 0, and an unflagged entry's history is not
+    // read (a first contact starts from zero), so it is left unwritten,
+    // except that the cohesion models read their slots on every entry
+    const int H = (a.coh_off >= 0 ? cnt : nt) * dnum;   // <= kStageSlots * dnum < 2^11
     // t / dnum as (t * inv) >> 20: exact for t < 2^11, dnum <= 64
     const uint32_t inv = (1u << 20) / (uint32_t)dnum + 1u;
     const double *__restrict__ oh = a.ohist;
