@@ -1,5 +1,6 @@
 """Groups, fix setforce and fix aveforce on the MI355X.
 
+* The contact pass's fused integrator against the separate kernel: bitwise.
 * Semantics from an identical state: the same pour with and without the
   fixes, one set-up force pass each.  A setforce group gets the set values
   in its styled components; an aveforce group gets the group's mean force
@@ -64,6 +65,25 @@ def test_setforce_aveforce_semantics():
     assert not np.array_equal(st["x"], rt["x"])     # the fixes moved the groups
     with pytest.raises(Exception):
         s.force_op_vector(sid)
+
+
+def test_fused_integration_bitwise():
+    """The contact pass integrates when nothing follows it; a setforce on an
+    empty group changes no force but keeps the integrator a kernel of its
+    own.  Both runs, over rebuilds, give the same bits."""
+    eng = load_engine()
+    spec, atoms = eng.workloads.pour_box()
+    a = eng.Simulation(spec, atoms)
+    b = eng.Simulation(spec, atoms)
+    b.set_group(3, np.zeros(len(atoms["radius"]), dtype=np.int32))
+    b.add_setforce(3, [0.0, 0.0, 0.0])
+    for s in (a, b):
+        s.setup()
+        s.step(37)
+        s.step(25)
+    sa, sb = a.state(), b.state()
+    for k in ("x", "v", "omega", "f", "torque"):
+        assert np.array_equal(sa[k], sb[k]), k
 
 
 def gold_sim(model):
