@@ -238,7 +238,7 @@ def bench_hopper(pkg, args):
     ntri = len(spec["meshes"][0]["nodes"])
     out = {}
     for prec in (["mixed", "fp64"] if args.only is None else [args.only]):
-        s = pkg.Simulation(spec, atoms, precision=prec)
+        s = pkg.Simulation(dict(spec, graph=True), atoms, precision=prec)   # a static mesh: graph stepping
         s.setup()
         s.step(args.warmup)
         s.sync()
@@ -260,7 +260,7 @@ def bench_hopper(pkg, args):
         "dtype": "f32 law / f64 state" if main_prec == "mixed" else "f64", "data": "synthetic",
         "config": {"workload": "%d spheres r~U[0.8,1.2] mm in a 30-degree conical hopper (100 mm top, 12 mm "
                                "outlet) with a cylinder on top, %d triangles, Hertz+history, mu 0.4, e 0.5, "
-                               "floor plane, gravity, dt 5e-7, rebuild every 10" % (n, ntri),
+                               "floor plane, gravity, dt 5e-7, rebuild every 10, graph stepping" % (n, ntri),
                    "n_particles": n, "triangles": ntri, "precision": main_prec},
         "forms": out,
     }

@@ -833,7 +833,9 @@ static int ensure_list_cap(dem_ctx *ctx, int b, int64_t entries)
 // Candidate triangles and contact slots of every particle for one mesh, after
 // the particles were re-sorted (perm = new -> old): FixNeighlistMesh::
 // pre_force + FixContactHistoryMesh::sort_contacts / pre_force.
-static int mesh_rebuild(dem_ctx *ctx, MeshDev &m)
+// captured (graph stepping, static meshes): no host reads or allocations;
+// the buffers were sized before the capture and guards flag what outgrows them
+static int mesh_rebuild(dem_ctx *ctx, MeshDev &m, bool captured = false)
 {
     const int n = ctx->n;
     hipStream_t s = ctx->stream;
@@ -860,27 +862,32 @@ static int mesh_rebuild(dem_ctx *ctx, MeshDev &m)
     na.threshold = m.moving ? ctx->p.skin : 0.5 * ctx->p.skin;
     na.n = n;
     na.count = m.mstart[b];
-    HIPCHK(hipMemsetAsync(m.mstart[b] + n, 0, sizeof(int), s));
-    HIPCHK(hipMemsetAsync(ctx->d_small + 3, 0, sizeof(int), s));
+    launch_fill_int(m.mstart[b] + n, 0, 1, s);
+    launch_fill_int(ctx->d_small + 3, 0, 1, s);
     launch_mesh_count(na, s);
     launch_row_max(m.mstart[b], n, ctx->d_small + 3, s);
     launch_scan(m.mstart[b], n + 1, ctx->scan_tmp, s);
     KCHK();
-    int hs[2] = {0, 0};
-    HIPCHK(hipMemcpyAsync(&hs[0], m.mstart[b] + n, sizeof(int), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipMemcpyAsync(&hs[1], ctx->d_small + 3, sizeof(int), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
-    if (hs[1] > kMaxMeshSlots)
-        return fail(ctx, DEM_ERR_OVERFLOW, "mesh contact history overflow: a particle has more than 4096 "
-                                           "candidate triangles");
-    const int64_t tot = hs[0];
     const int dw = ctx->M.wdnum;
-    if (tot > m.scap[b]) {
-        const int64_t c = tot + tot / 4 + 256;
-        if (dev_alloc(ctx, &m.mcand[b], (size_t)c) || dev_alloc(ctx, &m.mpart[b], (size_t)c)
-            || dev_alloc(ctx, &m.mhist[b], (size_t)c * std::max(dw, 1)))
-            return DEM_ERR_NOMEM;
-        m.scap[b] = c;
+    if (captured) {
+        launch_row_max_check(ctx->d_small + 3, kMaxMeshSlots, ctx->d_small, 64, s);
+        launch_cap_guard(m.mstart[b], n, m.scap[b], ctx->d_small + 5, ctx->d_small, 32, s);
+    } else {
+        int hs[2] = {0, 0};
+        HIPCHK(hipMemcpyAsync(&hs[0], m.mstart[b] + n, sizeof(int), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(&hs[1], ctx->d_small + 3, sizeof(int), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        if (hs[1] > kMaxMeshSlots)
+            return fail(ctx, DEM_ERR_OVERFLOW, "mesh contact history overflow: a particle has more than 4096 "
+                                               "candidate triangles");
+        const int64_t tot = hs[0];
+        if (tot > m.scap[b]) {
+            const int64_t c = tot + tot / 4 + 256;
+            if (dev_alloc(ctx, &m.mcand[b], (size_t)c) || dev_alloc(ctx, &m.mpart[b], (size_t)c)
+                || dev_alloc(ctx, &m.mhist[b], (size_t)c * std::max(dw, 1)))
+                return DEM_ERR_NOMEM;
+            m.scap[b] = c;
+        }
     }
     na.mstart = m.mstart[b];
     na.mcand = m.mcand[b];
@@ -1016,7 +1023,7 @@ static int rebuild(dem_ctx *ctx, bool captured = false)
             return fail(ctx, DEM_ERR_OVERFLOW, "Neighbor list overflow, boost neigh_modify one");
         if (ensure_list_cap(ctx, l1, hsmall[0])) return DEM_ERR_NOMEM;
     }
-    if (captured) launch_cap_guard(ctx->nstart[l1], n, ctx->ecap[l1], ctx->d_small + 4, ctx->d_small, s);
+    if (captured) launch_cap_guard(ctx->nstart[l1], n, ctx->ecap[l1], ctx->d_small + 4, ctx->d_small, 8, s);
     na.ecap = ctx->ecap[l1];
     na.coh_off = ctx->M.cohesion != CM_OFF ? ctx->M.lp.off_coh : -1;
     na.err = ctx->d_small;
@@ -1035,7 +1042,7 @@ static int rebuild(dem_ctx *ctx, bool captured = false)
         launch_neigh_fill(na, s);
     }
     KCHK();
-    if (captured) launch_row_max_check(ctx->d_small + 1, ctx->p.neigh_one, ctx->d_small, s);
+    if (captured) launch_row_max_check(ctx->d_small + 1, ctx->p.neigh_one, ctx->d_small, 16, s);
     ctx->lcur = l1;
     ctx->have_list = true;
     ctx->list_valid = true;
@@ -1045,7 +1052,7 @@ static int rebuild(dem_ctx *ctx, bool captured = false)
         if (rc) return rc;
     }
     for (auto &m : ctx->meshes) {
-        const int rc = mesh_rebuild(ctx, m);
+        const int rc = mesh_rebuild(ctx, m, captured);
         if (rc) return rc;
     }
     if (ctx->M.cohesion != CM_OFF) {
@@ -1276,6 +1283,12 @@ static int check_err_flag(dem_ctx *ctx)
                     "ilegal situation, particle positions identical. Most probable reason: Simulation diverged");
     if (h & 4) return fail(ctx, DEM_ERR_OVERFLOW, "mesh contact history: no free contact slot (internal error)");
     if (h & 16) return fail(ctx, DEM_ERR_OVERFLOW, "Neighbor list overflow, boost neigh_modify one");
+    if (h & 32)
+        return fail(ctx, DEM_ERR_OVERFLOW,
+                    "mesh candidate rows longer than the buffers sized for graph stepping (step without dem_use_graph)");
+    if (h & 64)
+        return fail(ctx, DEM_ERR_OVERFLOW, "mesh contact history overflow: a particle has more than 4096 candidate "
+                                           "triangles");
     return DEM_OK;
 }
 
@@ -1447,8 +1460,10 @@ static int tape_store(dem_ctx *ctx, int mask_step)
 // between steps (meshes, bonds, timers).
 static bool graph_ok(const dem_ctx *ctx)
 {
+    bool static_meshes = true;   // a moving mesh's displacement is a per-step host value
+    for (const auto &m : ctx->meshes) static_meshes = static_meshes && !m.moving;
     return ctx->use_graph && !ctx->window && ctx->fops.empty() && !ctx->p.neigh_check && ctx->p.neigh_delay <= ctx->p.neigh_every
-           && ctx->meshes.empty() && ctx->M.cohesion == CM_OFF && !ctx->timing && ctx->n > 0;
+           && static_meshes && ctx->M.cohesion == CM_OFF && !ctx->timing && ctx->n > 0;
 }
 
 static std::vector<const void *> graph_key(const dem_ctx *ctx)
@@ -1463,6 +1478,19 @@ static std::vector<const void *> graph_key(const dem_ctx *ctx)
     k.push_back((const void *)(intptr_t)ctx->n);
     k.push_back((const void *)(intptr_t)(ctx->tune_lanes * 4096 + ctx->tune_block));
     k.push_back((const void *)(intptr_t)ctx->obs_plane);
+    for (const auto &m : ctx->meshes) {
+        for (int b = 0; b < 2; b++) {
+            k.push_back(m.mstart[b]);
+            k.push_back(m.mnp[b]);
+            k.push_back(m.mcand[b]);
+            k.push_back(m.mpart[b]);
+            k.push_back(m.mhist[b]);
+        }
+        k.push_back(m.tri);
+        k.push_back(m.gstart);
+        k.push_back(m.glist);
+        k.push_back((const void *)(intptr_t)(m.cur + 2 * (m.have ? 1 : 0)));
+    }
     return k;
 }
 
@@ -1504,8 +1532,44 @@ static int capture_graph(dem_ctx *ctx)
             return DEM_ERR_NOMEM;
         }
     }
+    // mesh candidate rows: both buffers sized for twice the current rows
+    const size_t dw = std::max(ctx->M.wdnum, 1);
+    for (auto &m : ctx->meshes) {
+        int h = 0;
+        HIPCHK(hipMemcpy(&h, m.mstart[m.cur] + ctx->n, sizeof(int), hipMemcpyDeviceToHost));
+        const int64_t mt = h;
+        const int64_t mwant = std::max<int64_t>(2 * mt, (int64_t)ctx->n + 1024);
+        for (int b = 0; b < 2; b++) {
+            if (ctx->n + 1 > m.ncap[b]) {
+                if (dev_alloc(ctx, &m.mstart[b], ctx->n + 1) || dev_alloc(ctx, &m.mnp[b], ctx->n + 1))
+                    return DEM_ERR_HIP;
+                m.ncap[b] = ctx->n + 1;
+            }
+            if (m.scap[b] >= mwant) continue;
+            // the current rows are kept: grow by copy
+            int *nc = nullptr, *np = nullptr;
+            double *nh = nullptr;
+            HIPCHK(hipMalloc((void **)&nc, sizeof(int) * mwant));
+            HIPCHK(hipMalloc((void **)&np, sizeof(int) * mwant));
+            HIPCHK(hipMalloc((void **)&nh, sizeof(double) * mwant * dw));
+            if (b == m.cur && m.have && mt > 0) {
+                HIPCHK(hipMemcpy(nc, m.mcand[b], sizeof(int) * mt, hipMemcpyDeviceToDevice));
+                HIPCHK(hipMemcpy(np, m.mpart[b], sizeof(int) * mt, hipMemcpyDeviceToDevice));
+                HIPCHK(hipMemcpy(nh, m.mhist[b], sizeof(double) * mt * dw, hipMemcpyDeviceToDevice));
+            }
+            if (m.mcand[b]) HIPCHK(hipFree(m.mcand[b]));
+            if (m.mpart[b]) HIPCHK(hipFree(m.mpart[b]));
+            if (m.mhist[b]) HIPCHK(hipFree(m.mhist[b]));
+            m.mcand[b] = nc;
+            m.mpart[b] = np;
+            m.mhist[b] = nh;
+            m.scap[b] = mwant;
+        }
+    }
     const int every = ctx->p.neigh_every;
     const int cur0 = ctx->cur, lcur0 = ctx->lcur, par0 = ctx->par;
+    std::vector<std::pair<int, bool>> mesh0;
+    for (const auto &m : ctx->meshes) mesh0.emplace_back(m.cur, m.have);
     hipGraph_t graph = nullptr;
     HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
     ctx->gexec_capturing = true;
@@ -1521,6 +1585,10 @@ static int capture_graph(dem_ctx *ctx)
     ctx->cur = cur0;
     ctx->lcur = lcur0;
     ctx->par = par0;
+    for (size_t k = 0; k < ctx->meshes.size(); k++) {
+        ctx->meshes[k].cur = mesh0[k].first;
+        ctx->meshes[k].have = mesh0[k].second;
+    }
     if (rc == DEM_OK && ec == hipSuccess && !same_par) {
         if (graph) hipGraphDestroy(graph);
         return fail(ctx, DEM_ERR_STATE, "graph stepping: the recorded period changes the state buffers");

@@ -260,9 +260,9 @@ void launch_neigh_stage(const NeighArgs &a, hipStream_t s);
 // staging rows into the list (after the scan); no-op when a row outgrew kStageSlots
 void launch_neigh_compact(const NeighArgs &a, hipStream_t s);
 void launch_fill_int(int *p, int v, int n, hipStream_t s);
-// err |= 8 and all rows emptied when nstart[n] > ecap (flag: one int of scratch)
-void launch_cap_guard(int *nstart, int n, int64_t ecap, int *flag, int *err, hipStream_t s);
-void launch_row_max_check(const int *maxcount, int limit, int *err, hipStream_t s);
+// err |= bit and all rows emptied when nstart[n] > ecap (flag: one int of scratch)
+void launch_cap_guard(int *nstart, int n, int64_t ecap, int *flag, int *err, int bit, hipStream_t s);
+void launch_row_max_check(const int *maxcount, int limit, int *err, int bit, hipStream_t s);
 void launch_copy_int(int *dst, const int *src, int n, hipStream_t s);
 // out[b*5 + {0..4}] = count, sum z, sum of pwf, sum mass, kinetic energy of
 // instance b (particles in key order)

@@ -713,27 +713,27 @@ __global__ void __launch_bounds__(kBlock) k_copy_int(int *dst, const int *src, i
     if (i < n) dst[i] = src[i];
 }
 
-// err |= 16 if the longest row exceeds neigh_modify one (the graph form of
-// the host check after the count pass)
-__global__ void k_row_max_check(const int *maxcount, int limit, int *err)
+// err |= bit if the longest row exceeds the limit (the graph form of the
+// host check after a count pass)
+__global__ void k_row_max_check(const int *maxcount, int limit, int *err, int bit)
 {
-    if (threadIdx.x == 0 && *maxcount > limit) atomicOr(err, 16);
+    if (threadIdx.x == 0 && *maxcount > limit) atomicOr(err, bit);
 }
 
-void launch_row_max_check(const int *maxcount, int limit, int *err, hipStream_t s)
+void launch_row_max_check(const int *maxcount, int limit, int *err, int bit, hipStream_t s)
 {
-    k_row_max_check<<<1, 64, 0, s>>>(maxcount, limit, err);
+    k_row_max_check<<<1, 64, 0, s>>>(maxcount, limit, err, bit);
 }
 
 // A list longer than its buffers (graph stepping sizes them once): flag it
-// (err bit 3) and empty every row, so that no kernel of the step reads
+// (err |= bit) and empty every row, so that no kernel of the step reads
 // entries that were not written.  The host reports the error after replay.
-__global__ void k_cap_flag(const int *nstart, int n, int64_t ecap, int *flag, int *err)
+__global__ void k_cap_flag(const int *nstart, int n, int64_t ecap, int *flag, int *err, int bit)
 {
     if (threadIdx.x == 0) {
         const int over = (int64_t)nstart[n] > ecap;
         *flag = over;
-        if (over) atomicOr(err, 8);
+        if (over) atomicOr(err, bit);
     }
 }
 
@@ -743,9 +743,9 @@ __global__ void __launch_bounds__(kBlock) k_cap_clear(int *nstart, int n, const 
     if (i <= n && *flag) nstart[i] = 0;
 }
 
-void launch_cap_guard(int *nstart, int n, int64_t ecap, int *flag, int *err, hipStream_t s)
+void launch_cap_guard(int *nstart, int n, int64_t ecap, int *flag, int *err, int bit, hipStream_t s)
 {
-    k_cap_flag<<<1, 64, 0, s>>>(nstart, n, ecap, flag, err);
+    k_cap_flag<<<1, 64, 0, s>>>(nstart, n, ecap, flag, err, bit);
     k_cap_clear<<<(n + 1 + kBlock - 1) / kBlock, kBlock, 0, s>>>(nstart, n, flag);
 }
 

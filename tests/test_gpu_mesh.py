@@ -225,3 +225,24 @@ def test_moving_plate_device():
         sc = np.abs(q["f"]).max() + 1e-300
         assert np.abs(g["f"] - q["f"]).max() <= 1e-12 * sc
     assert np.abs(q["f"][0, 2]) > 0
+
+
+def test_hopper_graph_stepping_bitwise():
+    """A static mesh steps through recorded graphs (two rebuild periods per
+    graph, the mesh candidate rows rebuilt inside it): state and mesh
+    contact slots equal, bit for bit, to plain stepping over 95 steps."""
+    eng = load_engine()
+    spec, atoms = eng.workloads.hopper(n=2000)
+    atoms = wall_spheres(spec, atoms)
+    out = []
+    for graph in (False, True):
+        s = eng.Simulation(dict(spec, graph=graph), atoms)
+        s.setup()
+        s.step(95)
+        st = s.state()
+        out.append((st, mesh_dict(*s.mesh_contacts(0))))
+    (a, ma), (b, mb) = out
+    for k in ("x", "v", "omega", "f", "torque"):
+        assert np.array_equal(a[k], b[k]), k
+    assert set(ma) == set(mb) and len(ma) > 0
+    assert all(ma[k][0] == mb[k][0] and np.array_equal(ma[k][1], mb[k][1]) for k in ma)
