@@ -18,6 +18,12 @@
 // An entry whose contact flag is still clear has never touched, so its
 // history is zero; it is neither read nor, unless it touches now, written.
 //
+// k_force_q: the same pass with a wave-level contact queue (Hertz chains,
+// the default).  Both kernels end in one per-particle epilogue (planes,
+// gravity, freeze, the force and torque written) that also integrates when
+// nothing adds to the forces after the pass, writing the new state to the
+// other state buffers.
+//
 // k_integrate: FixNVESphere::initial_integrate / final_integrate
 // (fix_nve_sphere.cpp:134-243) with dtf = 0.5*dt, INERTIA = 0.4; mode 3 runs
 // final(n) and initial(n+1) in one pass over the state.

@@ -11,9 +11,15 @@
 //   k_scatter   counting-sort scatter (atomic slot within the cell)
 //   k_cellsort  order each cell by previous index, so the result is unique
 //   k_permute   gather every per-atom array into Morton order; wall lists
-//   k_ncount    full-list candidates per atom over the 27 cells, scan
-//   k_nfill     write the list; carry contact history of pairs that touch,
-//               looked up by partner tag in the atom's previous row
+//               (k_permute_whist: long wall-history rows, one double a thread)
+//   k_neigh<kStage>   one search over the 27 cells: row counts, and each row
+//               staged with the previous-list index of the history it
+//               carries (looked up by partner tag in the atom's previous row)
+//   scan        row starts
+//   k_neigh_compact   staged rows into the list, touching partners first,
+//               history moved with whole-line stores
+//   (k_neigh<kCount> + k_neigh<kFill>: the same search twice, for a rebuild
+//   with a row longer than the staging rows)
 //
 // The membership predicate is the reference's, in FP64 without contraction:
 //   rsq <= ((r_i + r_j) * cdf + skin)^2
