@@ -278,17 +278,21 @@ int dem_observe_plane(dem_ctx *ctx, int plane);
  * last force pass. */
 int dem_instance_observables(dem_ctx *ctx, double *out);
 /* Step with a recorded HIP graph of two rebuild periods when every rebuild
- * falls on a fixed step (neigh_modify check no, no mesh walls, no bonds,
- * timing off); other steps run as usual.  The results are bitwise those of
- * stepping without the graph. */
+ * falls on a fixed step (neigh_modify check no, mesh walls only if none
+ * moves, no bonds, no setforce/aveforce, timing off); other steps run as
+ * usual.  The results are bitwise those of stepping without the graph. */
 int dem_use_graph(dem_ctx *ctx, int on);
-/* Autotuner: times the contact pass on the current state for every lane
- * group width (1..16 lanes per particle) and block size (64..256) and keeps
- * the fastest for this context.  State and history are left unchanged. */
+/* Autotuner: times the contact pass on the current state for every shape
+ * (lane groups of 1..16 lanes per particle, contact-queue tiles of 8..64
+ * particles per wave, given as -8..-64) and block size (64..256), in the
+ * form the steps run (with the integrator when the pass integrates), and
+ * keeps the fastest for this context.  State and history are left
+ * unchanged. */
 int dem_autotune(dem_ctx *ctx, int reps, int *lanes, int *block, double *ms);
 /* Sets the contact pass's shape by hand (lanes per particle 1, 2, 4, 8 or
- * 16, block 64, 128 or 256; 0 keeps the default), e.g. to reuse an earlier
- * autotuner result.  Hertz chains take any width; others run 4 lanes. */
+ * 16, or a contact-queue tile of -8, -16, -32 or -64; block 64, 128 or 256;
+ * 0 keeps the default, the -32 queue), e.g. to reuse an earlier autotuner
+ * result.  Hertz chains take any shape; others run 4 lanes. */
 int dem_set_contact_shape(dem_ctx *ctx, int lanes, int block);
 
 /* ---- Adjoint window.  No counterpart in the reference (which has no
