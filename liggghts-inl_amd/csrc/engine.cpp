@@ -130,6 +130,8 @@ struct dem_ctx {
     int *scan_tmp = nullptr;
     int scan_tmp_cap = 0;
     int2 *stage = nullptr;   // n * kStageSlots: staging rows of the single-search rebuild
+    int *mstage = nullptr;   // n * kMeshStage: staged mesh candidate rows (shared by the meshes)
+    int mstage_cap = 0;
     uint32_t *spread = nullptr;
     int spread_cap = 0;
 
@@ -383,7 +385,7 @@ static void free_all(dem_ctx *ctx)
         F(ctx->gmask[b]);
     }
     F(ctx->f4); F(ctx->t4); F(ctx->xhold); F(ctx->wmask); F(ctx->ncontact); F(ctx->key);
-    F(ctx->sorted); F(ctx->cstart); F(ctx->cursor); F(ctx->scan_tmp); F(ctx->spread); F(ctx->stage);
+    F(ctx->sorted); F(ctx->cstart); F(ctx->cursor); F(ctx->scan_tmp); F(ctx->spread); F(ctx->stage); F(ctx->mstage);
     F(ctx->d_mat); F(ctx->d_small);
     F(ctx->bout); F(ctx->bvalid); F(ctx->brev);
     F(ctx->pwf); F(ctx->d_obs); F(ctx->d_gtot); F(ctx->d_gpart);
@@ -862,13 +864,20 @@ static int mesh_rebuild(dem_ctx *ctx, MeshDev &m, bool captured = false)
     na.threshold = m.moving ? ctx->p.skin : 0.5 * ctx->p.skin;
     na.n = n;
     na.count = m.mstart[b];
+    if (n > ctx->mstage_cap) {   // sized at the first (eager) rebuild, before any capture
+        if (captured) return fail(ctx, DEM_ERR_STATE, "mesh staging rows not sized before graph capture");
+        if (dev_alloc(ctx, &ctx->mstage, (size_t)n * kMeshStage)) return DEM_ERR_NOMEM;
+        ctx->mstage_cap = n;
+    }
+    na.stage = ctx->mstage;
     launch_fill_int(m.mstart[b] + n, 0, 1, s);
     launch_fill_int(ctx->d_small + 3, 0, 1, s);
-    launch_mesh_count(na, s);
+    launch_mesh_stage(na, s);   // one search: counts and staged rows
     launch_row_max(m.mstart[b], n, ctx->d_small + 3, s);
     launch_scan(m.mstart[b], n + 1, ctx->scan_tmp, s);
     KCHK();
     const int dw = ctx->M.wdnum;
+    int hs1 = 0;   // longest row (eager path)
     if (captured) {
         launch_row_max_check(ctx->d_small + 3, kMaxMeshSlots, ctx->d_small, 64, s);
         launch_cap_guard(m.mstart[b], n, m.scap[b], ctx->d_small + 5, ctx->d_small, 32, s);
@@ -880,6 +889,7 @@ static int mesh_rebuild(dem_ctx *ctx, MeshDev &m, bool captured = false)
         if (hs[1] > kMaxMeshSlots)
             return fail(ctx, DEM_ERR_OVERFLOW, "mesh contact history overflow: a particle has more than 4096 "
                                                "candidate triangles");
+        hs1 = hs[1];
         const int64_t tot = hs[0];
         if (tot > m.scap[b]) {
             const int64_t c = tot + tot / 4 + 256;
@@ -891,7 +901,12 @@ static int mesh_rebuild(dem_ctx *ctx, MeshDev &m, bool captured = false)
     }
     na.mstart = m.mstart[b];
     na.mcand = m.mcand[b];
-    launch_mesh_fill(na, s);
+    // rows that fit their stage are copied; the rest are searched again
+    launch_mesh_compact(na, s);
+    if (captured || hs1 > kMeshStage) {
+        na.fill_if_over = 1;
+        launch_mesh_fill(na, s);
+    }
     KCHK();
     MeshRemapArgs ra{};
     ra.perm = ctx->sorted;

@@ -140,7 +140,10 @@ struct MeshNbArgs {
     int *count;              // count pass
     const int *mstart;       // fill pass
     int *mcand;
+    int *stage;              // n * kMeshStage candidates of the single-search pass
+    int fill_if_over;        // fill pass: search only the rows longer than kMeshStage
 };
+constexpr int kMeshStage = 64;
 
 // Carry each particle's contact slots from the previous build (old index
 // space, perm = new -> old) into the new rows: sort_contacts then
@@ -179,6 +182,11 @@ struct MeshForceArgs {
 
 void launch_mesh_count(const MeshNbArgs &a, hipStream_t s);
 void launch_mesh_fill(const MeshNbArgs &a, hipStream_t s);
+// one search: counts and the first kMeshStage candidates of each row staged
+void launch_mesh_stage(const MeshNbArgs &a, hipStream_t s);
+// staged rows into mcand after the scan; rows longer than kMeshStage are left
+// to the fill pass (fill_if_over)
+void launch_mesh_compact(const MeshNbArgs &a, hipStream_t s);
 void launch_mesh_remap(const MeshRemapArgs &a, hipStream_t s);
 void launch_mesh_force(const MeshForceArgs &a, hipStream_t s);
 // MeshMoverLinear: node += dx, center += dx for ntri triangles

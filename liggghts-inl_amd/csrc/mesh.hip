@@ -6,10 +6,13 @@
 // fix_wall_gran.cpp:803-980) and as many contact slots
 // (FixContactHistoryMesh, fix_contact_history_mesh_I.h:51-215).  One thread
 // handles one particle, so the slot bookkeeping (first free slot, coplanar
-// hand-over, clean-up) runs in the reference's order without atomics.
+// hand-over, clean-up) runs in the reference's order without atomics; G
+// lanes of the particle share the geometry and the candidate search.
 //
-// k_mesh_nb<FILL>   FixNeighlistMesh::handleTriangle (fix_neighlist_mesh.cpp:
-//                   355-410) with resolveTriSphereNeighbuild, per particle
+// k_mesh_nb<MODE,G> FixNeighlistMesh::handleTriangle (fix_neighlist_mesh.cpp:
+//                   355-410) with resolveTriSphereNeighbuild, per particle:
+//                   count, fill, or stage (count + first kMeshStage rows)
+// k_mesh_compact    staged rows into the scanned list
 // k_mesh_remap      FixContactHistoryMesh::sort_contacts (:375-402) and
 //                   cleanUpContactJumps (:466-500) at a rebuild
 // k_mesh_force<WN>  post_force_mesh + handleContact + cleanUpContacts
@@ -23,14 +26,22 @@ namespace dem {
 
 // G lanes per particle test the triangles of its grid cell G at a time; the
 // passing ones are compacted by ballot + popcount, so a row keeps the cell
-// list's (ascending id) order.
-template <bool FILL, int G>
+// list's (ascending id) order.  MODE kMCount counts the rows, kMFill writes
+// them (after the scan), kMStage counts them and stages the first
+// kMeshStage candidates of each, for k_mesh_compact.
+constexpr int kMCount = 0, kMFill = 1, kMStage = 2;
+
+template <int MODE, int G>
 __global__ void __launch_bounds__(kBlock) k_mesh_nb(MeshNbArgs a)
 {
+    constexpr bool FILL = MODE == kMFill;
+    constexpr bool STAGE = MODE == kMStage;
     const int lane = threadIdx.x & (G - 1);
     const int gshift = (threadIdx.x & 63) & ~(G - 1);
     const int i = (blockIdx.x * kBlock + threadIdx.x) / G;
     if (i >= a.n) return;   // uniform across the group
+    // recorded after a staged pass: fills only when a row outgrew its stage
+    if (FILL && a.fill_if_over && a.mstart[i + 1] - a.mstart[i] <= kMeshStage) return;
     const double4 p = a.xr[i];
     const double q[3] = {p.x - a.disp[0], p.y - a.disp[1], p.z - a.disp[2]};
     int c[3];
@@ -56,11 +67,27 @@ __global__ void __launch_bounds__(kBlock) k_mesh_nb(MeshNbArgs a)
                 pass = tri_neighbuild(a.tri[t], a.enorm + 9 * (size_t)t, r, cs, a.threshold);
             }
             const uint64_t m = (__ballot(pass) >> gshift) & ((1ull << G) - 1ull);
-            if (FILL && pass) a.mcand[base + k + __popcll(m & ((1ull << lane) - 1ull))] = t;
+            const int pos = k + __popcll(m & ((1ull << lane) - 1ull));
+            if (FILL && pass) a.mcand[base + pos] = t;
+            if (STAGE && pass && pos < kMeshStage) a.stage[(size_t)i * kMeshStage + pos] = t;
             k += __popcll(m);
         }
     }
     if (!FILL && lane == 0) a.count[i] = k;
+}
+
+// The staged rows into mcand once the row lengths are scanned, G lanes per
+// particle.  Nothing when a row outgrew its stage (the fill pass runs then).
+template <int G>
+__global__ void __launch_bounds__(kBlock) k_mesh_compact(MeshNbArgs a)
+{
+    const int lane = threadIdx.x & (G - 1);
+    const int i = (blockIdx.x * kBlock + threadIdx.x) / G;
+    if (i >= a.n) return;
+    const int b0 = a.mstart[i];
+    const int cnt = a.mstart[i + 1] - b0;
+    if (cnt > kMeshStage) return;   // re-searched by the fill pass
+    for (int k = lane; k < cnt; k += G) a.mcand[b0 + k] = a.stage[(size_t)i * kMeshStage + k];
 }
 
 __global__ void __launch_bounds__(kBlock) k_mesh_remap(MeshRemapArgs a)
@@ -332,12 +359,22 @@ constexpr int kMeshLanes = 8;   // lanes per particle in the mesh kernels
 
 void launch_mesh_count(const MeshNbArgs &a, hipStream_t s)
 {
-    if (a.n) k_mesh_nb<false, kMeshLanes><<<blocks((long long)a.n * kMeshLanes), kBlock, 0, s>>>(a);
+    if (a.n) k_mesh_nb<kMCount, kMeshLanes><<<blocks((long long)a.n * kMeshLanes), kBlock, 0, s>>>(a);
 }
 
 void launch_mesh_fill(const MeshNbArgs &a, hipStream_t s)
 {
-    if (a.n) k_mesh_nb<true, kMeshLanes><<<blocks((long long)a.n * kMeshLanes), kBlock, 0, s>>>(a);
+    if (a.n) k_mesh_nb<kMFill, kMeshLanes><<<blocks((long long)a.n * kMeshLanes), kBlock, 0, s>>>(a);
+}
+
+void launch_mesh_stage(const MeshNbArgs &a, hipStream_t s)
+{
+    if (a.n) k_mesh_nb<kMStage, kMeshLanes><<<blocks((long long)a.n * kMeshLanes), kBlock, 0, s>>>(a);
+}
+
+void launch_mesh_compact(const MeshNbArgs &a, hipStream_t s)
+{
+    if (a.n) k_mesh_compact<kMeshLanes><<<blocks((long long)a.n * kMeshLanes), kBlock, 0, s>>>(a);
 }
 
 void launch_mesh_remap(const MeshRemapArgs &a, hipStream_t s)
