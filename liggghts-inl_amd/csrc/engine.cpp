@@ -1088,18 +1088,18 @@ static int rebuild(dem_ctx *ctx, bool captured = false)
 
 static int integrate(dem_ctx *ctx, int mode);
 
-// nothing adds to the forces after the contact pass, so it may integrate
+// nothing adds to the forces after the contact pass (or, with mesh walls,
+// after the last mesh pass), so that pass may integrate
 static bool fuse_possible(const dem_ctx *ctx)
 {
-    return ctx->meshes.empty() && ctx->fops.empty() && !ctx->window && ctx->tape_pos < 0
-           && ctx->M.cohesion == CM_OFF;
+    return ctx->fops.empty() && !ctx->window && ctx->tape_pos < 0 && ctx->M.cohesion == CM_OFF;
 }
 
 // The force pass; integ = 2 or 3 also runs final_integrate (and the next
-// step's initial_integrate), inside the contact pass when nothing adds to
-// the forces after it (no mesh walls, setforce/aveforce, bonds, adjoint
-// window or tape): the new state then goes to the other x/v/omega buffers
-// and par flips.
+// step's initial_integrate), inside the last pass that adds to the forces
+// (the contact pass, or the last mesh pass) when nothing follows it (no
+// setforce/aveforce, bonds, adjoint window or tape): the new state then goes
+// to the other x/v/omega buffers and par flips.
 static int force_pass(dem_ctx *ctx, int shearupdate, int integ = 0)
 {
     ForceArgs fa{};
@@ -1123,7 +1123,8 @@ static int force_pass(dem_ctx *ctx, int shearupdate, int integ = 0)
     fa.obs_plane = ctx->obs_plane;
     fa.lanes = ctx->tune_lanes;
     fa.block = ctx->tune_block;
-    const bool fuse = integ && fuse_possible(ctx) && force_can_fuse(fa);
+    const bool fuse = integ && ctx->meshes.empty() && fuse_possible(ctx) && force_can_fuse(fa);
+    const bool mfuse = integ && !ctx->meshes.empty() && fuse_possible(ctx);
     if (fuse) {
         fa.fuse = integ;
         fa.xn = ctx->par ? ctx->xr[c] : ctx->xn;
@@ -1180,6 +1181,15 @@ static int force_pass(dem_ctx *ctx, int shearupdate, int integ = 0)
         ma.curvature = m.h.curvature;
         ma.precision = m.h.precision;
         ma.apply_freeze = k + 1 == ctx->meshes.size();
+        if (mfuse && ma.apply_freeze) {
+            ma.fuse = integ;
+            ma.xn = ctx->par ? ctx->xr[c] : ctx->xn;
+            ma.vn = ctx->par ? ctx->vm[c] : ctx->vn;
+            ma.wn = ctx->par ? ctx->wt[c] : ctx->wn;
+            ma.dtf = 0.5 * ctx->p.dt;   // as in integrate()
+            ma.dtv = ctx->p.dt;
+            ma.dtfrotate = ma.dtf / 0.4;
+        }
         ma.M = ctx->M;
         launch_mesh_force(ma, ctx->stream);
         KCHK();
@@ -1206,7 +1216,8 @@ static int force_pass(dem_ctx *ctx, int shearupdate, int integ = 0)
             KCHK();
         }
     }
-    if (integ && !fuse) return integrate(ctx, integ);
+    if (mfuse) ctx->par ^= 1;
+    if (integ && !fuse && !mfuse) return integrate(ctx, integ);
     return DEM_OK;
 }
 

@@ -177,6 +177,12 @@ struct MeshForceArgs {
     double vel[3];
     double curvature, precision;
     int apply_freeze;        // this pass ends the post_force chain
+    // fuse = 2: final_integrate(n) of the particle after its last force
+    // term; fuse = 3 also initial_integrate(n+1); the new state goes to
+    // xn/vn/wn (the other state buffers)
+    int fuse;
+    double4 *xn, *vn, *wn;
+    double dtf, dtv, dtfrotate;
     Model M;
 };
 

@@ -336,6 +336,27 @@ __global__ void __launch_bounds__(kBlock) k_mesh_force(MeshForceArgs a)
         a.f4[i] = make_double4(0.0, 0.0, 0.0, 0.0);
         a.t4[i] = make_double4(0.0, 0.0, 0.0, 0.0);
     }
+    // the last force term of the step: the integrator's operations
+    // (k_integrate) on the final force and torque, into the other buffers
+    if (a.fuse && lane == 0) {
+        const double4 f = a.f4[i], t = a.t4[i];
+        double4 x = a.xr[i], v = a.vm[i], w = wi;
+        // dtfm = dtf / (rmass*(1.+CAddRhoFluid/density)), CAddRhoFluid = 0
+        const double dtfm = a.dtf / (v.w * (1. + 0.0 / 1.0));
+        const double dtirotate = a.dtfrotate / (x.w * x.w * v.w);
+        if (a.fuse & 2) {
+            v.x += dtfm * f.x; v.y += dtfm * f.y; v.z += dtfm * f.z;
+            w.x += dtirotate * t.x; w.y += dtirotate * t.y; w.z += dtirotate * t.z;
+        }
+        if (a.fuse & 1) {
+            v.x += dtfm * f.x; v.y += dtfm * f.y; v.z += dtfm * f.z;
+            x.x += a.dtv * v.x; x.y += a.dtv * v.y; x.z += a.dtv * v.z;
+            w.x += dtirotate * t.x; w.y += dtirotate * t.y; w.z += dtirotate * t.z;
+        }
+        a.xn[i] = x;
+        a.vn[i] = v;
+        a.wn[i] = w;
+    }
 }
 
 __global__ void __launch_bounds__(kBlock) k_mesh_move(TriGeo *tri, int ntri, double dx, double dy, double dz)

@@ -246,3 +246,27 @@ def test_hopper_graph_stepping_bitwise():
         assert np.array_equal(a[k], b[k]), k
     assert set(ma) == set(mb) and len(ma) > 0
     assert all(ma[k][0] == mb[k][0] and np.array_equal(ma[k][1], mb[k][1]) for k in ma)
+
+
+def test_hopper_fused_integration_bitwise():
+    """With mesh walls the last mesh pass integrates; a setforce on an empty
+    group changes no force but keeps the integrator a kernel of its own.
+    Both runs, over rebuilds, give the same state and mesh slots bit for bit."""
+    eng = load_engine()
+    spec, atoms = eng.workloads.hopper(n=2000)
+    atoms = wall_spheres(spec, atoms)
+    out = []
+    for split in (False, True):
+        s = eng.Simulation(spec, atoms)
+        if split:
+            s.set_group(3, np.zeros(len(atoms["radius"]), dtype=np.int32))
+            s.add_setforce(3, [0.0, 0.0, 0.0])
+        s.setup()
+        s.step(37)
+        s.step(26)
+        out.append((s.state(), mesh_dict(*s.mesh_contacts(0))))
+    (a, ma), (b, mb) = out
+    for k in ("x", "v", "omega", "f", "torque"):
+        assert np.array_equal(a[k], b[k]), k
+    assert set(ma) == set(mb) and len(ma) > 0
+    assert all(ma[k][0] == mb[k][0] and np.array_equal(ma[k][1], mb[k][1]) for k in ma)
