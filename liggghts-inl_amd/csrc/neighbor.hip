@@ -7,8 +7,9 @@
 //
 // Pipeline (one rebuild):
 //   k_bin       wrap periodic coordinates, Morton key of the cell, histogram
+//               (one atomic per run of equal keys in a wave)
 //   scan        exclusive prefix sum over the key space (cell starts)
-//   k_scatter   counting-sort scatter (atomic slot within the cell)
+//   k_scatter   counting-sort scatter (one atomic per run of a cell in a wave)
 //   k_cellsort  order each cell by previous index, so the result is unique
 //   k_permute   gather every per-atom array into Morton order; wall lists
 //               (k_permute_whist: long wall-history rows, one double a thread)
@@ -49,35 +50,59 @@ __device__ inline uint32_t cell_key(int cx, int cy, int cz, const Grid &G, int i
            + (G.spread[cx] | G.spread[G.maxc + cy] | G.spread[2 * G.maxc + cz]);
 }
 
+// Runs of equal keys in consecutive lanes of a wave (the particles are in
+// the cell order of the last rebuild, so most neighbours in a wave share a
+// cell): the first lane of each run does the run's one atomic.  Valid lanes
+// form a prefix of the wave.  Returns the first lane of this lane's run and,
+// through *len, the run's length (meaningful at the first lane).
+__device__ inline int wave_run(uint32_t k, bool valid, int lane, int *len)
+{
+    const uint32_t kp = __shfl_up(k, 1, 64);
+    const bool head = valid && (lane == 0 || kp != k);
+    const unsigned long long heads = __ballot(head);
+    const unsigned long long vmask = __ballot(valid);
+    const unsigned long long le = lane == 63 ? ~0ull : (2ull << lane) - 1;   // lanes 0..lane
+    const unsigned long long upto = heads & le;
+    const unsigned long long after = heads & ~le;
+    const int end = after ? __builtin_ctzll(after) : 64 - __builtin_clzll(vmask | 1ull);
+    *len = end - lane;
+    return upto ? 63 - __builtin_clzll(upto) : 0;
+}
+
 __global__ void __launch_bounds__(kBlock) k_bin(BinArgs a)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.n) return;
-    double4 p = a.xr[i];
-    int4 im = a.img[i];
-    const int inst = meta_inst((int)a.wt[i].w);   // every load before the stores
-    double *pc = &p.x;
-    int *ic = &im.x;
-    // Domain::pbc, orthogonal box
-    for (int d = 0; d < 3; d++) {
-        if (!a.G.periodic[d]) continue;
-        const double lo = a.G.lo[d], hi = a.hi[d];
-        if (pc[d] < lo) {
-            pc[d] += a.prd[d];
-            ic[d]--;
+    const bool valid = i < a.n;   // every lane joins the run detection
+    uint32_t k = 0;
+    if (valid) {
+        double4 p = a.xr[i];
+        int4 im = a.img[i];
+        const int inst = meta_inst((int)a.wt[i].w);   // every load before the stores
+        double *pc = &p.x;
+        int *ic = &im.x;
+        // Domain::pbc, orthogonal box
+        for (int d = 0; d < 3; d++) {
+            if (!a.G.periodic[d]) continue;
+            const double lo = a.G.lo[d], hi = a.hi[d];
+            if (pc[d] < lo) {
+                pc[d] += a.prd[d];
+                ic[d]--;
+            }
+            if (pc[d] >= hi) {
+                pc[d] -= a.prd[d];
+                pc[d] = pc[d] > lo ? pc[d] : lo;
+                ic[d]++;
+            }
         }
-        if (pc[d] >= hi) {
-            pc[d] -= a.prd[d];
-            pc[d] = pc[d] > lo ? pc[d] : lo;
-            ic[d]++;
-        }
+        k = cell_key(cell_coord(p.x, 0, a.G), cell_coord(p.y, 1, a.G), cell_coord(p.z, 2, a.G), a.G, inst);
+        a.xr[i] = p;
+        a.img[i] = im;
+        a.key[i] = k;
     }
-    const uint32_t k = cell_key(cell_coord(p.x, 0, a.G), cell_coord(p.y, 1, a.G),
-                                cell_coord(p.z, 2, a.G), a.G, inst);
-    a.xr[i] = p;
-    a.img[i] = im;
-    a.key[i] = k;
-    atomicAdd(&a.count[k], 1);
+    const int lane = threadIdx.x & 63;
+    int len;
+    const int first = wave_run(k, valid, lane, &len);
+    if (valid && first == lane) atomicAdd(&a.count[k], len);
 }
 
 // ---------------------------------------------------------------------------
@@ -159,9 +184,15 @@ void launch_scan(int *data, int n, int *tmp, hipStream_t s)
 __global__ void __launch_bounds__(kBlock) k_scatter(const uint32_t *key, int *cursor, int *sorted, int n)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int pos = atomicAdd(&cursor[key[i]], 1);
-    sorted[pos] = i;
+    const bool valid = i < n;
+    const uint32_t k = valid ? key[i] : 0u;
+    const int lane = threadIdx.x & 63;
+    int len;
+    const int first = wave_run(k, valid, lane, &len);
+    int pos = 0;
+    if (valid && first == lane) pos = atomicAdd(&cursor[k], len);   // the run's slots, in lane order
+    pos = __shfl(pos, first, 64);
+    if (valid) sorted[pos + lane - first] = i;
 }
 
 void launch_scatter(const uint32_t *key, int *cursor, int *sorted, int n, hipStream_t s)
