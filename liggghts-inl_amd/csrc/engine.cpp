@@ -1492,6 +1492,22 @@ static bool graph_ok(const dem_ctx *ctx)
            && static_meshes && ctx->M.cohesion == CM_OFF && !ctx->timing && ctx->n > 0;
 }
 
+// Appends the bytes of a by-value kernel argument source to a graph key.
+static void key_bytes(std::vector<const void *> &k, const void *src, size_t n)
+{
+    const unsigned char *b = static_cast<const unsigned char *>(src);
+    for (size_t o = 0; o < n; o += sizeof(intptr_t)) {
+        intptr_t v = 0;
+        memcpy(&v, b + o, std::min(sizeof(intptr_t), n - o));
+        k.push_back((const void *)v);
+    }
+}
+
+// What a recorded graph depends on: the buffers its kernels were given and
+// every value they received by value (model, cell grid, planes, largest
+// radius, list capacities, mesh grids).  A setter that changes any of them
+// (gravity, freeze, materials, a new upload with other radii) makes the key
+// differ, so the next graph period records again.
 static std::vector<const void *> graph_key(const dem_ctx *ctx)
 {
     std::vector<const void *> k = {ctx->xr[0], ctx->xr[1], ctx->vm[0], ctx->vm[1], ctx->wt[0], ctx->wt[1],
@@ -1516,7 +1532,18 @@ static std::vector<const void *> graph_key(const dem_ctx *ctx)
         k.push_back(m.gstart);
         k.push_back(m.glist);
         k.push_back((const void *)(intptr_t)(m.cur + 2 * (m.have ? 1 : 0)));
+        k.push_back((const void *)(intptr_t)m.wtype);
+        key_bytes(k, m.grid.lo, sizeof(m.grid.lo));
+        key_bytes(k, m.grid.inv, sizeof(m.grid.inv));
+        key_bytes(k, m.grid.n, sizeof(m.grid.n));
     }
+    key_bytes(k, &ctx->p, sizeof(ctx->p));
+    key_bytes(k, &ctx->M, sizeof(ctx->M));
+    key_bytes(k, &ctx->G, sizeof(ctx->G));
+    key_bytes(k, &ctx->P, sizeof(ctx->P));
+    key_bytes(k, &ctx->maxrad, sizeof(ctx->maxrad));
+    key_bytes(k, ctx->ecap, sizeof(ctx->ecap));
+    k.push_back((const void *)(intptr_t)(ctx->ntypes + 4096 * ctx->mat_blocks));
     return k;
 }
 
@@ -2254,19 +2281,50 @@ extern "C" int dem_autotune(dem_ctx *ctx, int reps, int *lanes, int *block, doub
     }
     const size_t hb = sizeof(double) * (size_t)E * ctx->M.dnum, jb = sizeof(uint32_t) * (size_t)E;
     const size_t wb = sizeof(double) * (size_t)n * ctx->P.n * ctx->M.wdnum, fb = sizeof(double4) * (size_t)n;
+    // mesh contact slots: the mesh passes create, fill and clean them (and
+    // hysteretic walls write their history even without a shear update)
+    struct MeshSave { int *part, *np; double *hist; size_t pb, nb, hb; };
+    std::vector<MeshSave> msave;
+    size_t mtot = 0;
+    const size_t dw = std::max(ctx->M.wdnum, 1);
+    for (const auto &m : ctx->meshes) {
+        MeshSave ms{nullptr, nullptr, nullptr, 0, 0, 0};
+        if (m.have) {
+            int h = 0;
+            HIPCHK(hipMemcpy(&h, m.mstart[m.cur] + n, sizeof(int), hipMemcpyDeviceToHost));
+            ms = MeshSave{m.mpart[m.cur], m.mnp[m.cur], m.mhist[m.cur], sizeof(int) * (size_t)h,
+                          sizeof(int) * (size_t)n, sizeof(double) * (size_t)h * dw};
+        }
+        msave.push_back(ms);
+        mtot += ms.pb + ms.nb + ms.hb + 48;
+    }
     char *save = nullptr;
-    const size_t tot = hb + jb + wb + 2 * fb + 64;
+    const size_t tot = hb + jb + wb + 2 * fb + 64 + mtot;
     HIPCHK(hipMalloc((void **)&save, tot));
     char *sh = save, *sj = sh + hb, *sw = sj + jb, *sf = sw + wb, *st = sf + fb;
     auto copy = [&](void *d, const void *q, size_t b) {
         return b ? hipMemcpyAsync(d, q, b, hipMemcpyDeviceToDevice, s) : hipSuccess;
     };
+    auto align16 = [](size_t b) { return (b + 15) & ~(size_t)15; };
+    auto mesh_copy = [&](bool back) {
+        hipError_t e = hipSuccess;
+        char *q = st + fb;
+        for (const auto &ms : msave) {
+            char *qp = q, *qn = qp + align16(ms.pb), *qh = qn + align16(ms.nb);
+            q = qh + align16(ms.hb);
+            if (back)
+                e = (hipError_t)(e | copy(ms.part, qp, ms.pb) | copy(ms.np, qn, ms.nb) | copy(ms.hist, qh, ms.hb));
+            else
+                e = (hipError_t)(e | copy(qp, ms.part, ms.pb) | copy(qn, ms.np, ms.nb) | copy(qh, ms.hist, ms.hb));
+        }
+        return e;
+    };
     auto restore = [&]() {
         return (copy(ctx->hist[l], sh, hb) | copy(ctx->nj[l], sj, jb) | copy(ctx->whist[c], sw, wb)
-                | copy(ctx->f4, sf, fb) | copy(ctx->t4, st, fb));
+                | copy(ctx->f4, sf, fb) | copy(ctx->t4, st, fb) | mesh_copy(true));
     };
     if (copy(sh, ctx->hist[l], hb) | copy(sj, ctx->nj[l], jb) | copy(sw, ctx->whist[c], wb) | copy(sf, ctx->f4, fb)
-        | copy(st, ctx->t4, fb)) {
+        | copy(st, ctx->t4, fb) | mesh_copy(false)) {
         hipFree(save);
         return fail(ctx, DEM_ERR_HIP, "dem_autotune: copy failed");
     }

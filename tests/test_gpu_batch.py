@@ -90,6 +90,66 @@ def test_batch_against_oracle_short_horizon():
         assert np.abs(g["f"][sel] - r["f"]).max() <= 1e-9 * fs
 
 
+@pytest.mark.parametrize("lanes,block", [(1, 64), (4, 256), (16, 128), (-32, 256)])
+def test_batch_mixed_against_oracle(lanes, block):
+    """The batch in its stated mixed precision, with the contact shapes the
+    autotuner chooses between: each instance is advanced on the oracle until
+    it has contacts, the batch starts from those states and histories, and
+    its set-up forces and a 40-step horizon match each instance's FP64
+    oracle (forces within 1e-5 of each atom's force scale)."""
+    eng = load_engine()
+    W = eng.workloads
+    spec, atoms, inst, params = small_batch(ninst=3, nmax=1200, seed=4)
+    a = dict(atoms)
+    a["x"], a["v"], a["omega"] = atoms["x"].copy(), atoms["v"].copy(), atoms["omega"].copy()
+    oracles, hist, wh = [], [], None
+    for b in range(3):
+        s1, a1 = W.instance(spec, atoms, inst, params, b)
+        o = Oracle(s1, a1)
+        o.setup()
+        o.run(1500)
+        st = o.state()
+        k = inst == b
+        a["x"][k], a["v"][k], a["omega"][k] = st["x"], st["v"], st["omega"]
+        ti, tj, sh, hi, fl = o.pairs()
+        hist.append((ti[fl != 0], tj[fl != 0], hi[fl != 0]))
+        w = o.wall_history()
+        if wh is None:
+            wh = np.zeros((len(atoms["radius"]),) + w.shape[1:])
+        wh[k] = w
+        oracles.append((o, a1, k))
+    sp = dict(spec, instances=dict(inst=inst, **params), observe_plane=4)
+    s = eng.Simulation(sp, a, precision="mixed")
+    s.set_contact_shape(lanes, block)
+    s.set_history(np.concatenate([h[0] for h in hist]), np.concatenate([h[1] for h in hist]),
+                  np.concatenate([h[2] for h in hist]))
+    s.set_wall_history(wh)
+    s.setup()
+    for o, _, _ in oracles:
+        o.rebuild()
+        o.compute_forces(0)
+
+    def compare(tol_x):
+        g = s.state()
+        ncontact = 0
+        for o, a1, k in oracles:
+            r = o.state()
+            fs, ts = o.force_scale()
+            ncontact += (fs > 0).sum()
+            fs = np.maximum(fs, 1e-300)
+            ts = np.maximum(ts + fs * a1["radius"], 1e-300)
+            assert np.abs(g["x"][k] - r["x"]).max() <= tol_x
+            assert (np.abs(g["f"][k] - r["f"]).max(1) / fs).max() <= 1e-5
+            assert (np.abs(g["torque"][k] - r["torque"]).max(1) / ts).max() <= 1e-5
+        return ncontact
+
+    assert compare(0.0) > 300
+    s.step(40)
+    for o, _, _ in oracles:
+        o.run(40)
+    compare(1e-11)
+
+
 def test_observables():
     spec, atoms, inst, params = small_batch(ninst=4, nmax=1000, seed=8)
     s = run_batch(spec, atoms, inst, params, 40, graph=True)
@@ -145,3 +205,37 @@ def test_autotune_keeps_state():
     s2 = run_batch(spec, atoms, inst, params, 50)
     a, c = s.state(), s2.state()
     assert np.abs(a["x"] - c["x"]).max() <= 1e-12 * 0.1
+
+
+def test_graph_follows_setters():
+    """A recorded graph holds the values its kernels were given: after
+    set_gravity, or a new upload with larger spheres (another cell grid),
+    graph stepping must record again and stay bitwise equal to stepping
+    without the graph."""
+    eng = load_engine()
+    spec, atoms = eng.workloads.packed_column(n=6000, slab_n=3000)
+    big = dict(atoms, radius=atoms["radius"] * 1.15)
+
+    def run(graph):
+        s = eng.Simulation(dict(spec, graph=graph), atoms)
+        s.setup()
+        out = []
+        s.step(47)
+        out.append(s.state())
+        s.set_gravity(0.0, 3.0, -25.0)
+        s.step(46)
+        out.append(s.state())
+        s.upload(big)
+        s.setup()
+        s.step(45)
+        out.append(s.state())
+        s.close()
+        return out
+
+    plain, graph = run(False), run(True)
+    for a, b in zip(plain, graph):
+        for k in ("x", "v", "omega", "f", "torque"):
+            assert np.array_equal(a[k], b[k]), k
+    # the gravity change and the larger spheres do show in the state
+    assert np.abs(plain[1]["v"][:, 1]).max() > 0
+    assert not np.array_equal(plain[2]["f"], plain[1]["f"])

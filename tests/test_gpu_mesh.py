@@ -149,10 +149,10 @@ def hopper_state(steps=200):
     return spec, atoms, o
 
 
-def sim_from_oracle_mesh(spec, atoms, o):
+def sim_from_oracle_mesh(spec, atoms, o, precision="fp64"):
     eng = load_engine()
     a = snapshot_atoms(atoms, o)
-    s = eng.Simulation(spec, a)
+    s = eng.Simulation(spec, a, precision=precision)
     ti, tj, sh, hi, fl = o.pairs()
     k = fl != 0
     s.set_history(ti[k], tj[k], hi[k])
@@ -199,6 +199,22 @@ def test_hopper_short_horizon():
     assert np.abs(g["x"] - r["x"]).max() <= 1e-12 * 0.5
     check_forces(s, o, 1e-9)
     check_mesh_contacts(s, o, 1e-9)
+
+
+def test_hopper_mixed_precision():
+    """The hopper in its stated mixed precision (FP32 pair law in the
+    contact queue, FP64 mesh walls and state): forces within 1e-5 of the
+    FP64 oracle from an identical state, and after 40 steps."""
+    spec, atoms, o = hopper_state()
+    s = sim_from_oracle_mesh(spec, atoms, o, precision="mixed")
+    check_mesh_contacts(s, o, 1e-12)
+    check_forces(s, o, 1e-5)
+    s.step(40)
+    o.run(40)
+    g, r = s.state(), o.state()
+    assert np.abs(g["x"] - r["x"]).max() <= 1e-11
+    check_forces(s, o, 1e-5)
+    check_mesh_contacts(s, o, 1e-5)
 
 
 def test_moving_plate_device():
@@ -265,6 +281,33 @@ def test_hopper_fused_integration_bitwise():
         s.step(37)
         s.step(26)
         out.append((s.state(), mesh_dict(*s.mesh_contacts(0))))
+    (a, ma), (b, mb) = out
+    for k in ("x", "v", "omega", "f", "torque"):
+        assert np.array_equal(a[k], b[k]), k
+    assert set(ma) == set(mb) and len(ma) > 0
+    assert all(ma[k][0] == mb[k][0] and np.array_equal(ma[k][1], mb[k][1]) for k in ma)
+
+
+def test_autotune_leaves_mesh_state():
+    """The autotuner's trial passes create and clean mesh contact slots; it
+    restores them with the pair history and forces, so the steps after it
+    are bitwise those of a run given the same contact shape by hand."""
+    eng = load_engine()
+    spec, atoms = eng.workloads.hopper(n=2000)
+    atoms = wall_spheres(spec, atoms)
+    out = []
+    shape = None
+    for tune in (True, False):
+        s = eng.Simulation(spec, atoms)
+        s.setup()
+        if tune:
+            lanes, block, _ = s.autotune(reps=2)
+            shape = (lanes, block)
+        else:
+            s.set_contact_shape(*shape)
+        s.step(30)
+        out.append((s.state(), mesh_dict(*s.mesh_contacts(0))))
+        s.close()
     (a, ma), (b, mb) = out
     for k in ("x", "v", "omega", "f", "torque"):
         assert np.array_equal(a[k], b[k]), k

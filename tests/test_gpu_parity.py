@@ -203,16 +203,21 @@ def test_neighbor_overflow_error():
         s.setup()
 
 
+SHAPES = ((4, 256), (1, 64), (2, 128), (8, 64), (16, 128), (-8, 256), (-16, 128), (-32, 256), (-64, 256))
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
 @pytest.mark.parametrize("case", ["pour", "column"])
-def test_contact_shapes_agree(case):
+def test_contact_shapes_agree(case, precision):
     """Every contact-pass shape (lane groups 1..16, contact-queue tiles
-    8..64, blocks 64..256) gives the oracle's forces on an identical state;
-    the shapes differ only in the order each particle's contacts are
-    summed."""
+    8..64, blocks 64..256) gives the oracle's forces on an identical state,
+    in FP64 (1e-12: the shapes differ only in the order each particle's
+    contacts are summed) and in mixed precision (1e-5, the FP32 law)."""
     spec, atoms, o = pour_state() if case == "pour" else column_state(n=6000, steps=40)
-    for lanes, block in ((4, 256), (1, 64), (16, 128), (-8, 256), (-16, 128), (-32, 256), (-64, 256)):
+    tol = 1e-12 if precision == "fp64" else 1e-5
+    for lanes, block in SHAPES:
         eng = load_engine()
-        s = eng.Simulation(spec, snapshot_atoms(atoms, o))
+        s = eng.Simulation(spec, snapshot_atoms(atoms, o), precision=precision)
         s.set_contact_shape(lanes, block)
         ti, tj, sh, hi, fl = o.pairs()
         k = fl != 0
@@ -222,8 +227,32 @@ def test_contact_shapes_agree(case):
         o2 = o   # the oracle's set-up pass on the same state
         o2.rebuild()
         o2.compute_forces(0)
-        check_forces(s, o2, 1e-12)
+        check_forces(s, o2, tol)
+        check_pairs(s, o2, hist_tol=1e-12 if precision == "fp64" else 1e-5)
         s.close()
+
+
+@pytest.mark.parametrize("rolling", ["epsd2", "epsd3"])
+def test_rolling_models_identical_state_and_horizon(rolling):
+    """EPSD2 (k_r = k_t R*^2, no viscous term) and EPSD3 (k_r = k_rs k_n
+    mu_r^2 R*^2) on the dense column: forces, torques and rolling history
+    from an identical state, then a short horizon over a rebuild."""
+    spec, atoms = W().packed_column(n=20000, slab_n=5000, rolling=rolling)
+    spec["materials"] = dict(spec["materials"], krs=1.7)
+    o = Oracle(spec, atoms)
+    o.setup()
+    o.run(40)
+    s = sim_from_oracle(spec, atoms, o)
+    check_pairs(s, o)
+    check_forces(s, o, 1e-12)
+    _, ts = o.force_scale()
+    assert (ts > 0).sum() > 1000
+    s.step(25)
+    o.run(25)
+    g, r = s.state(), o.state()
+    assert np.abs(g["x"] - r["x"]).max() <= 1e-12 * 0.06
+    check_forces(s, o, 1e-9)
+    check_pairs(s, o, hist_tol=1e-9)
 
 
 @pytest.mark.parametrize("big", [False, True])
