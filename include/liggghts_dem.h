@@ -242,6 +242,19 @@ int dem_add_setforce(dem_ctx *ctx, int bit, const int *style, const double *valu
  * the step's time (constant when c1 = 0). */
 int dem_add_aveforce(dem_ctx *ctx, int bit, const int *style, double xvalue, double yvalue, double c0,
                      double t0, double c1, int *id);
+/* fix nve/sphere on a group: only atoms with group bit `bit` are integrated
+ * (FixNVESphere::initial_integrate/final_integrate skip atoms outside the
+ * fix's group, fix_nve_sphere.cpp:140-150); -1 integrates every atom. */
+int dem_set_integrate_group(dem_ctx *ctx, int bit);
+/* fix move linear on group bit `bit` (FixMove, style linear,
+ * fix_move.cpp:430-490, :900-940): components with style[d] != 0 follow
+ * x = x_origin + value[d] * (step - time_origin) * dt at velocity value[d];
+ * the others are integrated from the force as fix nve does.  x_origin and
+ * time_origin are the positions and step at this call, made after
+ * dem_upload_state (an upload clears group bits and with them this fix).
+ * It acts before nve/sphere, as in scripts that define it first.  One per
+ * context; *id names it for dem_unfix. */
+int dem_add_move_linear(dem_ctx *ctx, int bit, const int *style, const double *value, int *id);
 /* unfix (Modify::delete_fix, modify.cpp:985). */
 int dem_unfix(dem_ctx *ctx, int id);
 /* The fix's vector: the group's total force before the fix acted, at the

@@ -161,6 +161,15 @@ struct dem_ctx {
     };
     std::vector<ForceOp> fops;
     int next_op_id = 1;
+    // fix nve/sphere on a group (-1: all atoms) and fix move linear
+    int integ_bit = -1;
+    struct MoveOp {
+        int id, bit, style;
+        double val[3];
+        long long t0;        // time_origin (fix_move.cpp: update->ntimestep at creation)
+    };
+    std::vector<MoveOp> moves;
+    double4 *xo[2] = {};     // positions at the fix's creation, per particle
     double *d_gpart = nullptr, *d_gtot = nullptr;   // partial sums; 4 per op slot
     int ago = 0;
 
@@ -383,6 +392,7 @@ static void free_all(dem_ctx *ctx)
         if (b == 0) { F(ctx->xn); F(ctx->vn); F(ctx->wn); }
         F(ctx->img[b]); F(ctx->whist[b]); F(ctx->nstart[b]); F(ctx->nj[b]); F(ctx->hist[b]);
         F(ctx->gmask[b]);
+        F(ctx->xo[b]);
     }
     F(ctx->f4); F(ctx->t4); F(ctx->xhold); F(ctx->wmask); F(ctx->ncontact); F(ctx->key);
     F(ctx->sorted); F(ctx->cstart); F(ctx->cursor); F(ctx->scan_tmp); F(ctx->spread); F(ctx->stage); F(ctx->mstage);
@@ -812,6 +822,9 @@ extern "C" int dem_upload_state(dem_ctx *ctx, int n, const double *x, const doub
         HIPCHK(hipMemset(ctx->wmask, 0, n));
         HIPCHK(hipMemset(ctx->gmask[0], 0, n * sizeof(int)));
     }
+    // the group bits are cleared, and with them the groups fix move acts on
+    ctx->moves.clear();
+    ctx->integ_bit = -1;
     ctx->have_list = false;
     ctx->list_valid = false;
     ctx->mesh_grids_valid = false;
@@ -987,6 +1000,8 @@ static int rebuild(dem_ctx *ctx, bool captured = false)
     pa.whist1 = ctx->whist[c1];
     pa.gm0 = ctx->gmask[c0];
     pa.gm1 = ctx->gmask[c1];
+    pa.xo0 = ctx->moves.empty() ? nullptr : ctx->xo[c0];
+    pa.xo1 = ctx->moves.empty() ? nullptr : ctx->xo[c1];
     pa.xhold = ctx->xhold;
     pa.wmask = ctx->wmask;
     pa.n = n;
@@ -1092,7 +1107,8 @@ static int integrate(dem_ctx *ctx, int mode);
 // after the last mesh pass), so that pass may integrate
 static bool fuse_possible(const dem_ctx *ctx)
 {
-    return ctx->fops.empty() && !ctx->window && ctx->tape_pos < 0 && ctx->M.cohesion == CM_OFF;
+    return ctx->fops.empty() && !ctx->window && ctx->tape_pos < 0 && ctx->M.cohesion == CM_OFF
+           && ctx->moves.empty() && ctx->integ_bit < 0;
 }
 
 // The force pass; integ = 2 or 3 also runs final_integrate (and the next
@@ -1289,6 +1305,20 @@ static int integrate(dem_ctx *ctx, int mode)
     ia.dtf = 0.5 * ctx->p.dt;   // dtf = 0.5*dt*ftm2v (fix_nve.cpp:85), ftm2v = 1 (si)
     ia.dtv = ctx->p.dt;
     ia.dtfrotate = ia.dtf / 0.4;
+    ia.integ_bit = ctx->integ_bit;
+    ia.mv_bit = -1;
+    if (ctx->integ_bit >= 0 || !ctx->moves.empty()) ia.gm = ctx->gmask[c];
+    if (!ctx->moves.empty()) {
+        const auto &mv = ctx->moves[0];
+        ia.mv_bit = mv.bit;
+        ia.mv_style = mv.style;
+        for (int d = 0; d < 3; d++) ia.mv_val[d] = mv.val[d];
+        // the initial half belongs to the step being started: the current
+        // one when a call begins (mode 1), the next one after a force pass
+        const long long step = (mode == 1) ? ctx->ntimestep : ctx->ntimestep + 1;
+        ia.mv_delta = (double)(step - mv.t0) * ctx->p.dt;
+        ia.xo = ctx->xo[c];
+    }
     if (!ctx->gexec_capturing) timer_begin(ctx, ctx->t_integrate);
     launch_integrate(ia, mode, ctx->stream);
     if (!ctx->gexec_capturing) timer_end(ctx, ctx->t_integrate);
@@ -1488,7 +1518,7 @@ static bool graph_ok(const dem_ctx *ctx)
 {
     bool static_meshes = true;   // a moving mesh's displacement is a per-step host value
     for (const auto &m : ctx->meshes) static_meshes = static_meshes && !m.moving;
-    return ctx->use_graph && !ctx->window && ctx->fops.empty() && !ctx->p.neigh_check && ctx->p.neigh_delay <= ctx->p.neigh_every
+    return ctx->use_graph && !ctx->window && ctx->fops.empty() && ctx->moves.empty() && !ctx->p.neigh_check && ctx->p.neigh_delay <= ctx->p.neigh_every
            && static_meshes && ctx->M.cohesion == CM_OFF && !ctx->timing && ctx->n > 0;
 }
 
@@ -2145,7 +2175,46 @@ extern "C" int dem_unfix(dem_ctx *ctx, int id)
             ctx->fops.erase(ctx->fops.begin() + k);
             return DEM_OK;
         }
+    for (size_t k = 0; k < ctx->moves.size(); k++)
+        if (ctx->moves[k].id == id) {
+            ctx->moves.erase(ctx->moves.begin() + k);
+            return DEM_OK;
+        }
     return fail(ctx, DEM_ERR_ARG, "dem_unfix: no such fix");
+}
+
+extern "C" int dem_set_integrate_group(dem_ctx *ctx, int bit)
+{
+    if (!ctx) return fail(nullptr, DEM_ERR_ARG, "null context");
+    if (bit < -1 || bit > 30) return fail(ctx, DEM_ERR_ARG, "dem_set_integrate_group: bit out of range");
+    ctx->integ_bit = bit;
+    return DEM_OK;
+}
+
+extern "C" int dem_add_move_linear(dem_ctx *ctx, int bit, const int *style, const double *value, int *id)
+{
+    if (!ctx) return fail(nullptr, DEM_ERR_ARG, "null context");
+    if (!style || !value || bit < 0 || bit > 30)
+        return fail(ctx, DEM_ERR_ARG, "dem_add_move_linear: bad arguments");
+    if (!ctx->moves.empty()) return fail(ctx, DEM_ERR_ARG, "dem_add_move_linear: one fix move on the device");
+    if (ctx->n <= 0) return fail(ctx, DEM_ERR_ARG, "dem_add_move_linear: upload the atoms first");
+    sync_enter(ctx);
+    const size_t c = (size_t)ctx->cap;
+    for (int b = 0; b < 2; b++)
+        if (dev_alloc(ctx, &ctx->xo[b], c)) return DEM_ERR_HIP;
+    // FixMove keeps each atom's position at its creation (xoriginal)
+    HIPCHK(hipMemcpyAsync(ctx->xo[ctx->cur], sx(ctx), sizeof(double4) * ctx->n, hipMemcpyDeviceToDevice,
+                          ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    dem_ctx::MoveOp m{};
+    m.id = ctx->next_op_id++;
+    m.bit = bit;
+    m.style = (style[0] ? 1 : 0) | (style[1] ? 2 : 0) | (style[2] ? 4 : 0);
+    for (int d = 0; d < 3; d++) m.val[d] = value[d];
+    m.t0 = ctx->ntimestep;
+    ctx->moves.push_back(m);
+    if (id) *id = m.id;
+    return DEM_OK;
 }
 
 extern "C" int dem_force_op_vector(dem_ctx *ctx, int id, double *out)

@@ -677,6 +677,29 @@ void launch_force_mixed(const ForceArgs &a, hipStream_t s) { launch_force<true>(
 
 // ---------------------------------------------------------------------------
 
+// FixMove::initial_integrate / final_integrate, style linear
+// (fix_move.cpp:430-490, :900-940): prescribed components follow
+// x = x_origin + v * (step - time_origin) * dt with v fixed; the others are
+// integrated from the force like fix nve.  It runs before nve/sphere, as
+// the fix that the INL scripts define first.
+__device__ inline void move_final(const IntegArgs &a, double dtfm, const double4 &f, double4 &v)
+{
+    if (!(a.mv_style & 1)) v.x += dtfm * f.x;
+    if (!(a.mv_style & 2)) v.y += dtfm * f.y;
+    if (!(a.mv_style & 4)) v.z += dtfm * f.z;
+}
+
+__device__ inline void move_initial(const IntegArgs &a, int i, double dtfm, const double4 &f, double4 &x, double4 &v)
+{
+    const double4 xo = a.xo[i];
+    if (a.mv_style & 1) { v.x = a.mv_val[0]; x.x = xo.x + a.mv_val[0] * a.mv_delta; }
+    else { v.x += dtfm * f.x; x.x += a.dtv * v.x; }
+    if (a.mv_style & 2) { v.y = a.mv_val[1]; x.y = xo.y + a.mv_val[1] * a.mv_delta; }
+    else { v.y += dtfm * f.y; x.y += a.dtv * v.y; }
+    if (a.mv_style & 4) { v.z = a.mv_val[2]; x.z = xo.z + a.mv_val[2] * a.mv_delta; }
+    else { v.z += dtfm * f.z; x.z += a.dtv * v.z; }
+}
+
 template <int MODE>
 __global__ void __launch_bounds__(kBlock) k_integrate(IntegArgs a)
 {
@@ -690,14 +713,23 @@ __global__ void __launch_bounds__(kBlock) k_integrate(IntegArgs a)
     // dtfm = dtf / (rmass*(1.+CAddRhoFluid/density)), CAddRhoFluid = 0
     const double dtfm = a.dtf / (v.w * (1. + 0.0 / 1.0));
     const double dtirotate = a.dtfrotate / (x.w * x.w * v.w);
+    const int gm = a.gm ? a.gm[i] : 0;
+    const bool moved = a.mv_bit >= 0 && ((gm >> a.mv_bit) & 1);
+    const bool integ = a.integ_bit < 0 || ((gm >> a.integ_bit) & 1);
     if (MODE & 2) {  // final_integrate(n)
-        v.x += dtfm * f.x; v.y += dtfm * f.y; v.z += dtfm * f.z;
-        w.x += dtirotate * t.x; w.y += dtirotate * t.y; w.z += dtirotate * t.z;
+        if (moved) move_final(a, dtfm, f, v);
+        if (integ) {
+            v.x += dtfm * f.x; v.y += dtfm * f.y; v.z += dtfm * f.z;
+            w.x += dtirotate * t.x; w.y += dtirotate * t.y; w.z += dtirotate * t.z;
+        }
     }
     if (MODE & 1) {  // initial_integrate(n+1)
-        v.x += dtfm * f.x; v.y += dtfm * f.y; v.z += dtfm * f.z;
-        x.x += a.dtv * v.x; x.y += a.dtv * v.y; x.z += a.dtv * v.z;
-        w.x += dtirotate * t.x; w.y += dtirotate * t.y; w.z += dtirotate * t.z;
+        if (moved) move_initial(a, i, dtfm, f, x, v);
+        if (integ) {
+            v.x += dtfm * f.x; v.y += dtfm * f.y; v.z += dtfm * f.z;
+            x.x += a.dtv * v.x; x.y += a.dtv * v.y; x.z += a.dtv * v.z;
+            w.x += dtirotate * t.x; w.y += dtirotate * t.y; w.z += dtirotate * t.z;
+        }
         a.xr[i] = x;
     }
     a.vm[i] = v;

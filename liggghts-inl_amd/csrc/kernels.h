@@ -74,6 +74,8 @@ struct PermuteArgs {
     double *whist1;
     const int *gm0;      // group bits
     int *gm1;
+    const double4 *xo0;  // fix move origins (may be null)
+    double4 *xo1;
     uint8_t *wmask;
     int n;
     int wdn;             // nplanes * dnum
@@ -121,6 +123,14 @@ struct IntegArgs {
     const double4 *f4, *t4;
     int n;
     double dtf, dtv, dtfrotate;
+    // fix nve/sphere on a group (integ_bit >= 0) and fix move linear on a
+    // group (mv_bit >= 0); both need the group bits
+    const int *gm;
+    int integ_bit;
+    int mv_bit, mv_style;        // mv_style bit d: component d prescribed
+    double mv_val[3];
+    double mv_delta;             // (step - time_origin) * dt of the initial half
+    const double4 *xo;           // positions at the fix's creation
 };
 
 // Mesh wall neighbour build: candidate triangles per particle through a

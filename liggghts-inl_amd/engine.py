@@ -146,6 +146,8 @@ def library():
         "dem_add_aveforce": (C.c_int, [vp, C.c_int, ip, C.c_double, C.c_double, C.c_double, C.c_double,
                                        C.c_double, ip]),
         "dem_unfix": (C.c_int, [vp, C.c_int]),
+        "dem_set_integrate_group": (C.c_int, [vp, C.c_int]),
+        "dem_add_move_linear": (C.c_int, [vp, C.c_int, ip, dp, ip]),
         "dem_force_op_vector": (C.c_int, [vp, C.c_int, dp]),
         "dem_cleanup": (C.c_int, [vp]),
         "dem_time": (C.c_double, [vp]),
@@ -450,6 +452,18 @@ class Simulation:
         self._chk(self.L.dem_add_aveforce(self.h, int(bit), _ip(st), 0.0 if xv is None else float(xv),
                                           0.0 if yv is None else float(yv), float(c0), float(t0), float(c1),
                                           C.byref(fid)))
+        return fid.value
+
+    def set_integrate_group(self, bit):
+        """fix nve/sphere on group bit `bit` (-1: every atom)."""
+        self._chk(self.L.dem_set_integrate_group(self.h, int(bit)))
+
+    def add_move_linear(self, bit, v):
+        """fix move linear on group bit; None components are integrated."""
+        st = np.array([c is not None for c in v], dtype=np.int32)
+        va = np.array([0.0 if c is None else c for c in v])
+        fid = C.c_int(0)
+        self._chk(self.L.dem_add_move_linear(self.h, int(bit), _ip(st), _dp(va), C.byref(fid)))
         return fid.value
 
     def unfix(self, fid):

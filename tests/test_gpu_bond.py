@@ -108,3 +108,63 @@ def test_bond_break(cohesion):
     g, r = s.state(), o.state()
     assert np.abs(g["x"] - r["x"]).max() <= 1e-11 * 0.1
     check_pairs(s, o, hist_tol=1e-8)
+
+
+def device_tension_chain(nsph):
+    """The INL tension example on the device: sphere 1 frozen, the last one
+    pulled by fix move linear at 0.1 m/s, the ones between integrated by
+    nve/sphere on their group (in.particle_test of cohesive_bond_nonlinear_
+    tension/{2,3}_spheres)."""
+    from test_oracle_bond import tension_chain
+
+    o, m = tension_chain(nsph)
+    eng = load_engine()
+    atoms = dict(x=np.array([[0.0004 * k - 0.0004 * (nsph - 1) / 2.0, 0.0, 0.0] for k in range(nsph)]),
+                 v=np.zeros((nsph, 3)), omega=np.zeros((nsph, 3)), radius=np.full(nsph, 0.0002),
+                 rmass=np.full(nsph, m), type=np.ones(nsph, int), tag=np.arange(1, nsph + 1),
+                 freeze=np.array([1] + [0] * (nsph - 1)))
+    s = eng.Simulation(dict(o.spec, freeze=True), atoms)
+    s.set_group(1, np.array([0] * (nsph - 1) + [1], np.int32))
+    s.add_move_linear(1, (0.1, 0.0, 0.0))
+    s.set_group(2, np.array([0] + [1] * (nsph - 2) + [0], np.int32))
+    s.set_integrate_group(2)
+    return s, m
+
+
+def tension_rows(s, m, nsph, rows, every):
+    out = []
+    for _ in range(rows):
+        s.step(every)
+        st = s.state()
+        ke = 0.5 * m * (st["v"] ** 2).sum()
+        out.append([nsph, ke, 0.0, s.time(), *st["x"][:, 0], *st["v"][:, 0], *st["f"][:, 0], *st["f"][:, 1],
+                    *st["f"][:, 2]])
+    return np.array(out)
+
+
+def test_bond_nonlinear_tension_golds_on_device():
+    """The reference's own 2- and 3-sphere bond/nonlinear tension golds
+    (examples/LIGGGHTS/INL/cohesive_bond_nonlinear_tension) replayed on the
+    device with fix freeze, fix move linear and nve/sphere on a group: the
+    2-sphere run over all 16,200 steps, the 3-sphere run (whose middle
+    sphere rings between two square-root tension laws, reproducible only
+    bit for bit) over its first 200,000 steps."""
+    import os
+
+    from test_oracle_bond import GOLD
+
+    for nsph, every, rows in ((2, 10, None), (3, 1000, 200)):
+        g = np.loadtxt(os.path.join(GOLD, "bond_nonlinear_tension_%d_spheres_out_loading.csv" % nsph),
+                       delimiter=",", skiprows=1)
+        if rows:
+            g = g[:rows]
+        s, m = device_tension_chain(nsph)
+        s.setup()
+        got = tension_rows(s, m, nsph, len(g), every)
+        scale = np.abs(g).max(0)
+        scale[scale == 0] = 1.0
+        err = (np.abs(got - g) / scale).max(0)
+        err[2] = 0.0                                 # cpu time
+        print("tension gold, %d spheres: max column-scaled error %.3e" % (nsph, err.max()))
+        assert err.max() < 1e-12, (nsph, err)
+        s.close()
