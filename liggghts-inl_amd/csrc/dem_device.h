@@ -133,6 +133,7 @@ struct Grid {
     int nkeys_local;     // Morton keys of one instance; key = instance*nkeys_local + morton
     const uint32_t *spread;  // spread[d*maxc + c] = interleaved bits of cell c in dim d
     int maxc;
+    int reach;           // cells per cutneighmax: the stencil is 2*reach+1 cells wide
 };
 
 __host__ __device__ inline void shift_decode(int code, int &sx, int &sy, int &sz)

@@ -683,22 +683,38 @@ extern "C" int dem_set_freeze(dem_ctx *ctx, int on)
 }
 
 // Cell grid + Morton spread tables for the current cut-off.
+// Cells of half cutneighmax, the reference's bin size (neighbor.cpp:1689
+// binsize_optimal = 0.5 * cutneighmax), searched with a 5x5x5 stencil
+// pruned by distance; DEM_NEIGH_REACH=1 selects cells of a whole
+// cutneighmax and the 3x3x3 stencil.  The pair sets are the same; the order
+// of each row (and so each particle's force sum) follows the cells.
+static int neigh_reach()
+{
+    static int r = 0;
+    if (r == 0) {
+        const char *s = getenv("DEM_NEIGH_REACH");
+        r = (s && atoi(s) == 1) ? 1 : 2;
+    }
+    return r;
+}
+
 static int setup_grid(dem_ctx *ctx)
 {
     const dem_params &p = ctx->p;
     const double cut = (ctx->maxrad + ctx->maxrad) * p.cdf + p.skin;
     ctx->M.cutneighmax = cut;
     Grid &G = ctx->G;
+    G.reach = neigh_reach();
     int bits[3];
     int maxc = 1;
     for (int d = 0; d < 3; d++) {
         const double L = p.boxhi[d] - p.boxlo[d];
-        int nc = (int)std::floor(L / (cut * 1.0001));
+        int nc = (int)std::floor(L / (cut / G.reach * 1.0001));
         if (nc < 1) nc = 1;
-        if (p.periodic[d] && nc < 3)
+        if (p.periodic[d] && nc < 2 * G.reach + 1)
             return fail(ctx, DEM_ERR_ARG,
-                        "periodic box dimension must span at least 3 neighbour cells "
-                        "(3 x cutneighmax)");
+                        "periodic box dimension must span at least 3 x cutneighmax (half-size neighbour "
+                        "cells: 2.5 x cutneighmax)");
         if (nc > 4096) nc = 4096;
         G.nc[d] = nc;
         G.lo[d] = p.boxlo[d];

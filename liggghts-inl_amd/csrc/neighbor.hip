@@ -340,17 +340,19 @@ void launch_permute(const PermuteArgs &a, hipStream_t s)
 // previous list or -1), which k_neigh_compact then moves into the list.
 constexpr int kCount = 0, kFill = 1, kStage = 2;
 
-template <int MODE, int W>
+template <int MODE, int W, int S>
 __global__ void __launch_bounds__(kBlock) k_neigh(NeighArgs a)
 {
     constexpr bool FILL = MODE == kFill;
     constexpr bool STAGE = MODE == kStage;
     constexpr bool OLD = FILL || STAGE;   // looks up the previous row
     static_assert(W == 8 || W == 16 || W == 32 || W == 64, "8, 16, 32 or 64 lanes per particle");
+    static_assert(S == 1 || S == 2, "cells of cutneighmax or of half of it");
     constexpr int kGroups = kBlock / W;
     constexpr int kShift = W == 64 ? 6 : (W == 32 ? 5 : (W == 16 ? 4 : 3));
-    __shared__ int s_ex[kGroups][28], s_qb[kGroups][27], s_code[kGroups][27];
-    __shared__ double s_sh[kGroups][3][27];   // each cell's periodic image shift, shift * prd
+    constexpr int kSide = 2 * S + 1;                 // stencil cells per dimension
+    constexpr int NC = kSide * kSide * kSide;        // 27 or 125
+    __shared__ int s_ex[kGroups][NC + 1], s_qb[kGroups][NC], s_code[kGroups][NC];
     constexpr int kOld = 32;                            // previous-row entries kept in LDS
     constexpr int kMap = 256;                           // candidates with a cell map
     __shared__ int s_ot[kGroups][kOld];                 // partner tags of the previous row (fill)
@@ -380,7 +382,7 @@ __global__ void __launch_bounds__(kBlock) k_neigh(NeighArgs a)
         qb = 0;
         cnt = 0;
         code = kNoShift;
-        const int off[3] = {k % 3 - 1, (k / 3) % 3 - 1, k / 9 - 1};
+        const int off[3] = {k % kSide - S, (k / kSide) % kSide - S, k / (kSide * kSide) - S};
         int c[3], sh[3];
         bool ok = true;
         for (int d = 0; d < 3; d++) {
@@ -394,10 +396,12 @@ __global__ void __launch_bounds__(kBlock) k_neigh(NeighArgs a)
                 else ok = false;
             }
         }
+        // distance to the cell's nearest face along each offset dimension:
+        // to the particle's own cell boundary, plus |off| - 1 whole cells
         double d2 = 0.0;
         for (int d = 0; d < 3; d++)
             if (off[d] != 0) {
-                const double g = off[d] < 0 ? u[d] - cc[d] : cc[d] + 1 - u[d];
+                const double g = off[d] < 0 ? (u[d] - cc[d]) + (-off[d] - 1) : (cc[d] + 1 - u[d]) + (off[d] - 1);
                 const double t = fmax(g - 1e-6, 0.0) * a.cw[d];
                 d2 += t * t;
             }
@@ -416,24 +420,19 @@ __global__ void __launch_bounds__(kBlock) k_neigh(NeighArgs a)
         }
         return incl;
     };
-    // cells lane, lane + W, ... (W < 27: several cells per lane), their
+    // cells lane, lane + W, ... (W < NC: several cells per lane), their
     // candidate counts scanned in stencil order
     int total = 0;
 #pragma unroll
-    for (int r = 0; r < (27 + W - 1) / W; r++) {
+    for (int r = 0; r < (NC + W - 1) / W; r++) {
         const int k = lane + r * W;
         int qb = 0, cnt = 0, code = kNoShift;
-        if (k < 27) cell_info(k, qb, cnt, code);
+        if (k < NC) cell_info(k, qb, cnt, code);
         const int incl = scan(cnt);
-        if (k < 27) {
+        if (k < NC) {
             s_ex[wv][k] = total + incl - cnt;
             s_qb[wv][k] = qb;
             s_code[wv][k] = code;
-            int sx, sy, sz;
-            shift_decode(code, sx, sy, sz);
-            s_sh[wv][0][k] = sx * a.prd[0];
-            s_sh[wv][1][k] = sy * a.prd[1];
-            s_sh[wv][2][k] = sz * a.prd[2];
             const int ex = total + incl - cnt;
             for (int j = 0; j < cnt && ex + j < kMap; j++) s_cell[wv][ex + j] = (uint8_t)k;
         }
@@ -482,7 +481,7 @@ __global__ void __launch_bounds__(kBlock) k_neigh(NeighArgs a)
             if (total <= kMap) {
                 lo = s_cell[wv][c];
             } else {
-                int hi = 26;
+                int hi = NC - 1;
                 while (lo < hi) {
                     const int mid = (lo + hi + 1) >> 1;
                     if (s_ex[wv][mid] <= c) lo = mid;
@@ -493,12 +492,13 @@ __global__ void __launch_bounds__(kBlock) k_neigh(NeighArgs a)
             qcode = s_code[wv][lo];
             if (OLD) tq = a.tag[q];   // issued beside the position load, not after the test
             if (!(q == p && qcode == kNoShift)) {
-                // the cell's shift, decoded once per cell: the same values
-                // as sx * prd per candidate
+                // the periodic image of the candidate's cell
                 const double4 xq = a.xr[q];
-                const double xj0 = xq.x + s_sh[wv][0][lo];
-                const double xj1 = xq.y + s_sh[wv][1][lo];
-                const double xj2 = xq.z + s_sh[wv][2][lo];
+                int sx, sy, sz;
+                shift_decode(qcode, sx, sy, sz);
+                const double xj0 = xq.x + sx * a.prd[0];
+                const double xj1 = xq.y + sy * a.prd[1];
+                const double xj2 = xq.z + sz * a.prd[2];
                 const double delx = xp.x - xj0;
                 const double dely = xp.y - xj1;
                 const double delz = xp.z - xj2;
@@ -687,14 +687,22 @@ static int group_blocks(int n, int W)
     return (int)(((long long)n + per_block - 1) / per_block);
 }
 
+// the search kernel for the grid's cell size (reach cells per cutneighmax)
+template <int MODE, int W>
+static void launch_search(const NeighArgs &a, hipStream_t s)
+{
+    if (a.G.reach == 2) k_neigh<MODE, W, 2><<<group_blocks(a.n, W), kBlock, 0, s>>>(a);
+    else k_neigh<MODE, W, 1><<<group_blocks(a.n, W), kBlock, 0, s>>>(a);
+}
+
 void launch_neigh_count(const NeighArgs &a, hipStream_t s)
 {
     if (a.n == 0) return;
     switch (neigh_lanes()) {
-    case 64: k_neigh<kCount, 64><<<group_blocks(a.n, 64), kBlock, 0, s>>>(a); break;
-    case 32: k_neigh<kCount, 32><<<group_blocks(a.n, 32), kBlock, 0, s>>>(a); break;
-    case 8: k_neigh<kCount, 8><<<group_blocks(a.n, 8), kBlock, 0, s>>>(a); break;
-    default: k_neigh<kCount, 16><<<group_blocks(a.n, 16), kBlock, 0, s>>>(a); break;
+    case 64: launch_search<kCount, 64>(a, s); break;
+    case 32: launch_search<kCount, 32>(a, s); break;
+    case 8: launch_search<kCount, 8>(a, s); break;
+    default: launch_search<kCount, 16>(a, s); break;
     }
     launch_row_max(a.count, a.n, a.maxcount, s);
 }
@@ -703,10 +711,10 @@ void launch_neigh_stage(const NeighArgs &a, hipStream_t s)
 {
     if (a.n == 0) return;
     switch (neigh_lanes()) {
-    case 64: k_neigh<kStage, 64><<<group_blocks(a.n, 64), kBlock, 0, s>>>(a); break;
-    case 32: k_neigh<kStage, 32><<<group_blocks(a.n, 32), kBlock, 0, s>>>(a); break;
-    case 8: k_neigh<kStage, 8><<<group_blocks(a.n, 8), kBlock, 0, s>>>(a); break;
-    default: k_neigh<kStage, 16><<<group_blocks(a.n, 16), kBlock, 0, s>>>(a); break;
+    case 64: launch_search<kStage, 64>(a, s); break;
+    case 32: launch_search<kStage, 32>(a, s); break;
+    case 8: launch_search<kStage, 8>(a, s); break;
+    default: launch_search<kStage, 16>(a, s); break;
     }
     launch_row_max(a.count, a.n, a.maxcount, s);
 }
@@ -729,10 +737,10 @@ void launch_neigh_fill(const NeighArgs &a, hipStream_t s)
 {
     if (a.n == 0) return;
     switch (neigh_lanes()) {
-    case 64: k_neigh<kFill, 64><<<group_blocks(a.n, 64), kBlock, 0, s>>>(a); break;
-    case 32: k_neigh<kFill, 32><<<group_blocks(a.n, 32), kBlock, 0, s>>>(a); break;
-    case 8: k_neigh<kFill, 8><<<group_blocks(a.n, 8), kBlock, 0, s>>>(a); break;
-    default: k_neigh<kFill, 16><<<group_blocks(a.n, 16), kBlock, 0, s>>>(a); break;
+    case 64: launch_search<kFill, 64>(a, s); break;
+    case 32: launch_search<kFill, 32>(a, s); break;
+    case 8: launch_search<kFill, 8>(a, s); break;
+    default: launch_search<kFill, 16>(a, s); break;
     }
 }
 
