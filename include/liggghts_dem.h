@@ -164,6 +164,16 @@ int dem_get_mesh_contacts(dem_ctx *ctx, int mesh, int64_t *count, int *tag, int 
  * FixContactHistoryMesh); the next rebuild carries them over. */
 int dem_set_mesh_contacts(dem_ctx *ctx, int mesh, int64_t count, const int *tag, const int *tri,
                           const int *slot, const double *hist);
+/* fix mesh/surface/stress on a mesh (MeshModuleStress, mesh_module_stress.cpp
+ * :288-360): from the next force pass, the force the particles put on the
+ * mesh and its torque about the reference point pref (NULL: the origin;
+ * `reference_point`), which moves with the mesh.  Every particle counts
+ * (the fix's group is all). */
+int dem_mesh_stress(dem_ctx *ctx, int mesh, const double *pref);
+/* FixMeshSurfaceStress's vector (mesh_module_stress.cpp:479-488): out[0..2]
+ * total force, out[3..5] torque about p_ref, out[6..8] p_ref, of the last
+ * force pass. */
+int dem_mesh_stress_vector(dem_ctx *ctx, int mesh, double *out);
 /* Bond tables (DEM_NBOND) and settings of the cohesion model
  * (cohesion_model_bond.h:237-470, cohesion_model_bond_nonlinear.h:204-386).
  * Call after dem_upload_state: registers the model's contact distance

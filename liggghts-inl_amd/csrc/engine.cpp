@@ -58,6 +58,13 @@ struct MeshDev {
     int64_t scap[2] = {0, 0};
     int cur = 0;
     bool have = false;
+    // fix mesh/surface/stress: per-particle contributions, partial sums,
+    // the total force and torque of the last force pass; p_ref moves with
+    // the mesh
+    bool stress = false;
+    double pref[3] = {0, 0, 0};
+    double *d_ms = nullptr, *d_spart = nullptr, *d_sout = nullptr;
+    int ms_cap = 0;
 };
 
 }  // namespace
@@ -116,6 +123,8 @@ struct dem_ctx {
     // par = 0, the state is in xr/vm/wt[cur]; par = 1, in xn/vn/wn
     double4 *xn = nullptr, *vn = nullptr, *wn = nullptr;
     int par = 0;
+    float4 *rec = nullptr;   // FP32 partner records of the mixed law (2 float4 per particle)
+    int rec_cap = 0;
     int4 *img[2] = {};
     double *whist[2] = {};
     uint8_t *wmask = nullptr;
@@ -393,8 +402,9 @@ static void free_all(dem_ctx *ctx)
         F(ctx->img[b]); F(ctx->whist[b]); F(ctx->nstart[b]); F(ctx->nj[b]); F(ctx->hist[b]);
         F(ctx->gmask[b]);
         F(ctx->xo[b]);
+
     }
-    F(ctx->f4); F(ctx->t4); F(ctx->xhold); F(ctx->wmask); F(ctx->ncontact); F(ctx->key);
+    F(ctx->f4); F(ctx->t4); F(ctx->rec); F(ctx->xhold); F(ctx->wmask); F(ctx->ncontact); F(ctx->key);
     F(ctx->sorted); F(ctx->cstart); F(ctx->cursor); F(ctx->scan_tmp); F(ctx->spread); F(ctx->stage); F(ctx->mstage);
     F(ctx->d_mat); F(ctx->d_small);
     F(ctx->bout); F(ctx->bvalid); F(ctx->brev);
@@ -405,7 +415,7 @@ static void free_all(dem_ctx *ctx)
     for (auto &m : ctx->meshes) {
         F(m.tri); F(m.enorm); F(m.gstart); F(m.glist);
         for (int b = 0; b < 2; b++) {
-            F(m.mstart[b]); F(m.mcand[b]); F(m.mpart[b]); F(m.mnp[b]); F(m.mhist[b]);
+            F(m.mstart[b]); F(m.mcand[b]); F(m.mpart[b]); F(m.mnp[b]); F(m.mhist[b]); if (b == 0) { F(m.d_ms); F(m.d_spart); F(m.d_sout); }
         }
     }
 }
@@ -1186,6 +1196,16 @@ static int force_pass(dem_ctx *ctx, int shearupdate, int integ = 0)
         fa.bout = ctx->bout;
         fa.bvalid = ctx->bvalid;
     }
+    static const bool use_rec = getenv("DEM_MIXED_REC") && atoi(getenv("DEM_MIXED_REC")) == 1;
+    if (ctx->p.precision == DEM_MIXED && use_rec) {
+        if (ctx->rec_cap < ctx->n) {
+            if (dev_alloc(ctx, &ctx->rec, 2 * (size_t)ctx->cap)) return DEM_ERR_HIP;
+            ctx->rec_cap = ctx->cap;
+        }
+        launch_pack_rec(fa.vm, fa.wt, ctx->rec, ctx->n, ctx->stream);
+        KCHK();
+        fa.rec = ctx->rec;
+    }
     if (!ctx->gexec_capturing) timer_begin(ctx, ctx->t_contact);
     if (ctx->p.precision == DEM_MIXED) launch_force_mixed(fa, ctx->stream);
     else launch_force_fp64(fa, ctx->stream);
@@ -1222,9 +1242,21 @@ static int force_pass(dem_ctx *ctx, int shearupdate, int integ = 0)
             ma.dtv = ctx->p.dt;
             ma.dtfrotate = ma.dtf / 0.4;
         }
+        if (m.stress) {
+            if (m.ms_cap < ctx->n) {
+                if (dev_alloc(ctx, &m.d_ms, 6 * (size_t)ctx->cap)) return DEM_ERR_HIP;
+                m.ms_cap = ctx->cap;
+            }
+            ma.mstress = m.d_ms;
+            for (int q = 0; q < 3; q++) ma.pref[q] = m.pref[q];
+        }
         ma.M = ctx->M;
         launch_mesh_force(ma, ctx->stream);
         KCHK();
+        if (m.stress) {
+            launch_sum6(m.d_ms, ctx->n, m.d_spart, m.d_sout, ctx->stream);
+            KCHK();
+        }
     }
     if (!ctx->fops.empty()) {
         // Update::atime + (ntimestep - atimestep) * dt, as the reference's
@@ -1268,6 +1300,8 @@ static int move_meshes(dem_ctx *ctx)
             for (int q = 0; q < 3; q++) m.h.center[3 * t + q] = m.h.center[3 * t + q] + dx[q];
         }
         for (int q = 0; q < 3; q++) m.disp[q] += dx[q];
+        // p_ref moves with the mesh (frame_general, mesh_module_stress.cpp:71)
+        for (int q = 0; q < 3; q++) m.pref[q] += dx[q];
     }
     return DEM_OK;
 }
@@ -1579,6 +1613,9 @@ static std::vector<const void *> graph_key(const dem_ctx *ctx)
         k.push_back(m.glist);
         k.push_back((const void *)(intptr_t)(m.cur + 2 * (m.have ? 1 : 0)));
         k.push_back((const void *)(intptr_t)m.wtype);
+        k.push_back(m.d_ms);
+        k.push_back(m.d_sout);
+        key_bytes(k, m.pref, sizeof(m.pref));
         key_bytes(k, m.grid.lo, sizeof(m.grid.lo));
         key_bytes(k, m.grid.inv, sizeof(m.grid.inv));
         key_bytes(k, m.grid.n, sizeof(m.grid.n));
@@ -2197,6 +2234,33 @@ extern "C" int dem_unfix(dem_ctx *ctx, int id)
             return DEM_OK;
         }
     return fail(ctx, DEM_ERR_ARG, "dem_unfix: no such fix");
+}
+
+extern "C" int dem_mesh_stress(dem_ctx *ctx, int mesh, const double *pref)
+{
+    if (!ctx) return fail(nullptr, DEM_ERR_ARG, "null context");
+    if (mesh < 0 || mesh >= (int)ctx->meshes.size()) return fail(ctx, DEM_ERR_ARG, "dem_mesh_stress: no such mesh");
+    MeshDev &m = ctx->meshes[mesh];
+    sync_enter(ctx);
+    if (!m.d_spart && (dev_alloc(ctx, &m.d_spart, 6 * kSumBlocks) || dev_alloc(ctx, &m.d_sout, 6)))
+        return DEM_ERR_HIP;
+    HIPCHK(hipMemset(m.d_sout, 0, 6 * sizeof(double)));
+    m.stress = true;
+    for (int q = 0; q < 3; q++) m.pref[q] = pref ? pref[q] : 0.0;
+    return DEM_OK;
+}
+
+extern "C" int dem_mesh_stress_vector(dem_ctx *ctx, int mesh, double *out)
+{
+    if (!ctx) return fail(nullptr, DEM_ERR_ARG, "null context");
+    if (mesh < 0 || mesh >= (int)ctx->meshes.size() || !out || !ctx->meshes[mesh].stress)
+        return fail(ctx, DEM_ERR_ARG, "dem_mesh_stress_vector: no stress tracking on that mesh");
+    MeshDev &m = ctx->meshes[mesh];
+    sync_enter(ctx);
+    HIPCHK(hipMemcpyAsync(out, m.d_sout, 6 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    for (int q = 0; q < 3; q++) out[6 + q] = m.pref[q];
+    return DEM_OK;
 }
 
 extern "C" int dem_set_integrate_group(dem_ctx *ctx, int bit)

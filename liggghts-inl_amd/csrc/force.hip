@@ -399,8 +399,15 @@ __global__ void __launch_bounds__(kBlock) k_force_q(ForceArgs a)
             const int j = (int)(w & kJMask);
             // the partner's velocity, spin and the entry's history are
             // independent loads, all in flight together
-            const double4 vmj = a.vm[j];
-            const double4 wj = a.wt[j];
+            double4 vmj, wj;
+            if (MIXED && a.rec) {
+                const float4 r0 = a.rec[2 * (size_t)j], r1 = a.rec[2 * (size_t)j + 1];
+                vmj = make_double4(r0.x, r0.y, r0.z, r0.w);
+                wj = make_double4(r1.x, r1.y, r1.z, (double)__float_as_int(r1.w));
+            } else {
+                vmj = a.vm[j];
+                wj = a.wt[j];
+            }
             const double radj = s_qd[wv][3][lane];
             const double4 pi = s_x[wv][k], vmi = s_v[wv][k], wi = s_w[wv][k];
             const int metai = (int)wi.w;
@@ -734,6 +741,21 @@ __global__ void __launch_bounds__(kBlock) k_integrate(IntegArgs a)
     }
     a.vm[i] = v;
     a.wt[i] = w;
+}
+
+__global__ void __launch_bounds__(kBlock) k_pack_rec(const double4 *vm, const double4 *wt, float4 *rec, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double4 v = vm[i], w = wt[i];
+    rec[2 * (size_t)i] = make_float4((float)v.x, (float)v.y, (float)v.z, (float)v.w);
+    rec[2 * (size_t)i + 1] = make_float4((float)w.x, (float)w.y, (float)w.z, __int_as_float((int)w.w));
+}
+
+void launch_pack_rec(const double4 *vm, const double4 *wt, float4 *rec, int n, hipStream_t s)
+{
+    if (n == 0) return;
+    k_pack_rec<<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(vm, wt, rec, n);
 }
 
 void launch_integrate(const IntegArgs &a, int mode, hipStream_t s)

@@ -30,6 +30,9 @@ struct ForceArgs {
     int fuse;
     double4 *xn, *vn, *wn;
     double dtf, dtv, dtfrotate;
+    // mixed law: the partners' velocity/mass and spin/meta as one 32-byte
+    // FP32 record per particle (2 float4), or null to read vm/wt
+    const float4 *rec;
     Model M;
     Planes P;
 };
@@ -193,6 +196,10 @@ struct MeshForceArgs {
     int fuse;
     double4 *xn, *vn, *wn;
     double dtf, dtv, dtfrotate;
+    // fix mesh/surface/stress: per particle, the force its contacts put on
+    // the mesh and that force's torque about pref (6 doubles), or null
+    double *mstress;
+    double pref[3];
     Model M;
 };
 
@@ -299,6 +306,11 @@ bool force_can_fuse(const ForceArgs &a);
 void launch_force_mixed(const ForceArgs &a, hipStream_t s);
 // mode: 1 = initial_integrate, 2 = final_integrate, 3 = final then initial
 void launch_integrate(const IntegArgs &a, int mode, hipStream_t s);
+void launch_pack_rec(const double4 *vm, const double4 *wt, float4 *rec, int n, hipStream_t s);
+// out[0..5] = sum over particles of v[6 i + k], a fixed-order reduction
+// (part: kSumBlocks * 6 doubles of partial sums)
+constexpr int kSumBlocks = 256;
+void launch_sum6(const double *v, int n, double *part, double *out, hipStream_t s);
 int scan_tmp_size(int n);
 // Neighbor::check_distance (neighbor.cpp:1390-1465): flag |= 1 if any atom
 // moved more than sqrt(deltasq) since the last build.

@@ -185,6 +185,8 @@ __global__ void __launch_bounds__(kBlock) k_mesh_force(MeshForceArgs a)
     const double4 wi = a.wt[i];
     const int meta = (int)wi.w;
     const bool frozen = a.apply_freeze && M.freeze_on && meta_frozen(meta);
+    // fix mesh/surface/stress sums of this particle's contacts
+    double SF[3] = {0.0, 0.0, 0.0}, ST[3] = {0.0, 0.0, 0.0};
     if (nn > 0) {
         const double4 pi = a.xr[i];
         const double4 vmi = a.vm[i];
@@ -277,6 +279,18 @@ __global__ void __launch_bounds__(kBlock) k_mesh_force(MeshForceArgs a)
                 }
                 F[0] += dF[0]; F[1] += dF[1]; F[2] += dF[2];
                 T[0] += dT[0]; T[1] += dT[1]; T[2] += dT[2];
+                if (a.mstress) {
+                    // MeshModuleStress::add_particle_contribution
+                    // (mesh_module_stress.cpp:318-344): -F at the contact
+                    // point x + delta, torque about p_ref
+                    const double frc[3] = {-dF[0], -dF[1], -dF[2]};
+                    const double arm[3] = {cs[0] + delta[0] - a.pref[0], cs[1] + delta[1] - a.pref[1],
+                                           cs[2] + delta[2] - a.pref[2]};
+                    SF[0] += frc[0]; SF[1] += frc[1]; SF[2] += frc[2];
+                    ST[0] += arm[1] * frc[2] - arm[2] * frc[1];
+                    ST[1] += arm[2] * frc[0] - arm[0] * frc[2];
+                    ST[2] += arm[0] * frc[1] - arm[1] * frc[0];
+                }
             } else {
                 contact_close(M.lw, hp);
             }
@@ -335,6 +349,11 @@ __global__ void __launch_bounds__(kBlock) k_mesh_force(MeshForceArgs a)
     if (frozen && lane == 0) {
         a.f4[i] = make_double4(0.0, 0.0, 0.0, 0.0);
         a.t4[i] = make_double4(0.0, 0.0, 0.0, 0.0);
+    }
+    if (a.mstress && lane == 0) {
+        double *o = a.mstress + 6 * (size_t)i;
+        o[0] = SF[0]; o[1] = SF[1]; o[2] = SF[2];
+        o[3] = ST[0]; o[4] = ST[1]; o[5] = ST[2];
     }
     // the last force term of the step: the integrator's operations
     // (k_integrate) on the final force and torque, into the other buffers
@@ -417,6 +436,41 @@ void launch_mesh_force(const MeshForceArgs &a, hipStream_t s)
 void launch_mesh_move(TriGeo *tri, int ntri, double dx, double dy, double dz, hipStream_t s)
 {
     if (ntri) k_mesh_move<<<blocks(ntri), kBlock, 0, s>>>(tri, ntri, dx, dy, dz);
+}
+
+// Fixed-order sum of a per-particle 6-vector: each of kSumBlocks blocks
+// sums a contiguous range (a fixed tree inside the block), one block adds
+// the partial sums in order.  Deterministic; no atomics.
+__global__ void __launch_bounds__(kBlock) k_sum6_part(const double *v, int n, double *part)
+{
+    __shared__ double s_v[6][kBlock];
+    const int per = (n + gridDim.x - 1) / gridDim.x;
+    const int p0 = blockIdx.x * per, p1 = min(n, p0 + per);
+    double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int p = p0 + (int)threadIdx.x; p < p1; p += kBlock)
+        for (int k = 0; k < 6; k++) acc[k] += v[6 * (size_t)p + k];
+    for (int k = 0; k < 6; k++) s_v[k][threadIdx.x] = acc[k];
+    __syncthreads();
+    for (int h = kBlock / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h)
+            for (int k = 0; k < 6; k++) s_v[k][threadIdx.x] += s_v[k][threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) part[6 * blockIdx.x + threadIdx.x] = s_v[threadIdx.x][0];
+}
+
+__global__ void k_sum6_total(const double *part, int nb, double *out)
+{
+    if (threadIdx.x >= 6) return;
+    double s = 0.0;
+    for (int b = 0; b < nb; b++) s += part[6 * b + threadIdx.x];
+    out[threadIdx.x] = s;
+}
+
+void launch_sum6(const double *v, int n, double *part, double *out, hipStream_t s)
+{
+    k_sum6_part<<<kSumBlocks, kBlock, 0, s>>>(v, n, part);
+    k_sum6_total<<<1, 64, 0, s>>>(part, kSumBlocks, out);
 }
 
 }  // namespace dem

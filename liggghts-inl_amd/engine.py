@@ -146,6 +146,8 @@ def library():
         "dem_add_aveforce": (C.c_int, [vp, C.c_int, ip, C.c_double, C.c_double, C.c_double, C.c_double,
                                        C.c_double, ip]),
         "dem_unfix": (C.c_int, [vp, C.c_int]),
+        "dem_mesh_stress": (C.c_int, [vp, C.c_int, dp]),
+        "dem_mesh_stress_vector": (C.c_int, [vp, C.c_int, dp]),
         "dem_set_integrate_group": (C.c_int, [vp, C.c_int]),
         "dem_add_move_linear": (C.c_int, [vp, C.c_int, ip, dp, ip]),
         "dem_force_op_vector": (C.c_int, [vp, C.c_int, dp]),
@@ -303,6 +305,17 @@ class Simulation:
         hi = np.zeros((max(c, 1), dw))
         self._chk(self.L.dem_get_mesh_contacts(self.h, int(k), C.byref(cnt), _ip(tg), _ip(tr), _ip(sl), _dp(hi)))
         return tg[:c], tr[:c], sl[:c], hi[:c]
+
+    def mesh_stress(self, k=0, pref=(0.0, 0.0, 0.0)):
+        """fix mesh/surface/stress on mesh k with reference point pref."""
+        p = np.ascontiguousarray(pref, dtype=np.float64)
+        self._chk(self.L.dem_mesh_stress(self.h, int(k), _dp(p)))
+
+    def mesh_stress_vector(self, k=0):
+        """(force[3], torque[3], p_ref[3]) of the last force pass."""
+        out = np.zeros(9)
+        self._chk(self.L.dem_mesh_stress_vector(self.h, int(k), _dp(out)))
+        return out
 
     def set_mesh_contacts(self, k, tag, tri, slot, hist):
         tg = np.ascontiguousarray(tag, dtype=np.int32)

@@ -84,6 +84,10 @@ typedef struct {
     unsigned char *keep, *inter;
     double *hist;
     double force[3];
+    /* fix mesh/surface/stress (MeshModuleStress): total force and torque
+     * on the mesh about p_ref, which moves with the mesh */
+    int stress;
+    double pref[3], stress_f[3], stress_t[3];
 } mesh_t;
 
 struct or_sim {
@@ -1995,6 +1999,25 @@ int or_mesh_size(const or_sim *s, int im)
     return (im >= 0 && im < s->nmesh) ? s->mesh[im].ntri : 0;
 }
 
+void or_mesh_stress(or_sim *s, int im, int on, const double pref[3])
+{
+    mesh_t *m = &s->mesh[im];
+    m->stress = on;
+    for (int q = 0; q < 3; q++) m->pref[q] = pref ? pref[q] : 0.0;
+}
+
+/* FixMeshSurfaceStress::compute_vector (mesh_module_stress.cpp:479-488):
+ * total force, torque about p_ref, p_ref */
+void or_mesh_stress_vector(const or_sim *s, int im, double out[9])
+{
+    const mesh_t *m = &s->mesh[im];
+    for (int q = 0; q < 3; q++) {
+        out[q] = m->stress_f[q];
+        out[3 + q] = m->stress_t[q];
+        out[6 + q] = m->pref[q];
+    }
+}
+
 void or_mesh_force(const or_sim *s, int im, double f[3])
 {
     for (int q = 0; q < 3; q++) f[q] = s->mesh[im].force[q];
@@ -2282,6 +2305,9 @@ static void mesh_move(or_sim *s, mesh_t *m)
             for (int q = 0; q < 3; q++) MN(m, t, k)[q] += dx[q];
         for (int q = 0; q < 3; q++) m->center[3 * t + q] += dx[q];
     }
+    /* p_ref is a mesh property of frame_general: it moves with the nodes
+     * (mesh_module_stress.cpp:71) */
+    for (int q = 0; q < 3; q++) m->pref[q] += dx[q];
 }
 
 /* FixWallGran::post_force_mesh (fix_wall_gran.cpp:803-980) */
@@ -2291,6 +2317,8 @@ static void mesh_post_force(or_sim *s, mesh_t *m, int shearupdate)
     const double cdm = s->p.cdf - 1.0;
     double vwall[3];
     m->force[0] = m->force[1] = m->force[2] = 0.0;
+    /* MeshModuleStress::pre_force (mesh_module_stress.cpp:288-300) */
+    for (int q = 0; q < 3; q++) m->stress_f[q] = m->stress_t[q] = 0.0;
     /* markAllContacts */
     for (int q = 0; q < m->sstart[s->n]; q++) { m->keep[q] = 0; m->inter[q] = 0; }
     for (int t = 0; t < m->ntri; t++) {
@@ -2369,6 +2397,20 @@ static void mesh_post_force(or_sim *s, mesh_t *m, int shearupdate)
                     s->f[3 * i + q] += Fi[q];
                     s->torque[3 * i + q] += Ti[q];
                     m->force[q] += Fi[q];
+                }
+                if (m->stress) {
+                    /* MeshModuleStress::add_particle_contribution
+                     * (mesh_module_stress.cpp:318-344), called with the force
+                     * the contact added to the particle and -sidata.delta
+                     * (fix_wall_gran_base.h:349-362): frc = -F at the contact
+                     * point x + delta, torque (cp - p_ref) x frc */
+                    const double frc[3] = {-Fi[0], -Fi[1], -Fi[2]};
+                    double arm[3];
+                    for (int q = 0; q < 3; q++) arm[q] = s->x[3 * i + q] + delta[q] - m->pref[q];
+                    m->stress_f[0] += frc[0]; m->stress_f[1] += frc[1]; m->stress_f[2] += frc[2];
+                    m->stress_t[0] += arm[1] * frc[2] - arm[2] * frc[1];
+                    m->stress_t[1] += arm[2] * frc[0] - arm[0] * frc[2];
+                    m->stress_t[2] += arm[0] * frc[1] - arm[1] * frc[0];
                 }
                 s->fscale[i] += sqrt(Fi[0] * Fi[0] + Fi[1] * Fi[1] + Fi[2] * Fi[2]);
                 s->tscale[i] += sqrt(Ti[0] * Ti[0] + Ti[1] * Ti[1] + Ti[2] * Ti[2]);

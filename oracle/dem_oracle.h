@@ -128,6 +128,9 @@ double or_mesh_distance(const or_sim *s, int mesh, int tri, double r, const doub
                         double bary[3], int *region);
 /* sum of the mesh's contact forces on the particles in the last pass */
 void or_mesh_force(const or_sim *s, int mesh, double f[3]);
+/* fix mesh/surface/stress on a mesh: force and torque sums about p_ref */
+void or_mesh_stress(or_sim *s, int mesh, int on, const double pref[3]);
+void or_mesh_stress_vector(const or_sim *s, int mesh, double out[9]);
 /* current mesh contacts: (atom tag, triangle id, slot index, history);
  * NULL to count */
 int or_mesh_contacts(const or_sim *s, int mesh, int *tag, int *tri, int *slot, double *hist);

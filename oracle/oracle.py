@@ -119,6 +119,8 @@ def lib():
         L.or_mesh_topology.argtypes = [C.c_void_p, C.c_int, ip, ip, ip, ip]
         L.or_mesh_force.argtypes = [C.c_void_p, C.c_int, dp]
         L.or_mesh_size.argtypes = [C.c_void_p, C.c_int]
+        L.or_mesh_stress.argtypes = [C.c_void_p, C.c_int, C.c_int, dp]
+        L.or_mesh_stress_vector.argtypes = [C.c_void_p, C.c_int, dp]
         L.or_mesh_contacts.argtypes = [C.c_void_p, C.c_int, ip, ip, ip, dp]
         L.or_set_bond.argtypes = [C.c_void_p, dp, C.POINTER(_BondSettings)]
         L.or_add_move_linear.argtypes = [C.c_void_p, C.c_int, ip, dp]
@@ -376,6 +378,18 @@ class Oracle:
         f = np.zeros(3)
         self.L.or_mesh_force(self.h, int(k), _dp(f))
         return f
+
+    def mesh_stress(self, k=0, pref=(0.0, 0.0, 0.0)):
+        """fix mesh/surface/stress on mesh k with reference point pref."""
+        p = np.ascontiguousarray(pref, dtype=np.float64)
+        self._keep.append(p)
+        self.L.or_mesh_stress(self.h, int(k), 1, _dp(p))
+
+    def mesh_stress_vector(self, k=0):
+        """(force[3], torque[3], p_ref[3]) of the last force pass."""
+        out = np.zeros(9)
+        self.L.or_mesh_stress_vector(self.h, int(k), _dp(out))
+        return out
 
     def mesh_contacts(self, k=0, slots=False):
         """(tag, triangle id, [slot,] history) of the mesh's filled contact

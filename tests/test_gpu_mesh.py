@@ -313,3 +313,34 @@ def test_autotune_leaves_mesh_state():
         assert np.array_equal(a[k], b[k]), k
     assert set(ma) == set(mb) and len(ma) > 0
     assert all(ma[k][0] == mb[k][0] and np.array_equal(ma[k][1], mb[k][1]) for k in ma)
+
+
+def test_hopper_mesh_stress():
+    """fix mesh/surface/stress on the hopper: the total force the spheres
+    put on the mesh and its torque about a reference point, from an
+    identical state and after 40 steps, against the oracle (the sums run in
+    another order: 1e-10 of their magnitude)."""
+    eng = load_engine()
+    spec, atoms, o = hopper_state()
+    pref = np.array([0.001, -0.002, 0.03])
+    o.mesh_stress(0, pref)
+    a = snapshot_atoms(atoms, o)
+    s = eng.Simulation(spec, a)
+    ti, tj, sh, hi, fl = o.pairs()
+    k = fl != 0
+    s.set_history(ti[k], tj[k], hi[k])
+    s.set_wall_history(o.wall_history())
+    tg, tr, sl, mh = o.mesh_contacts(0, slots=True)
+    s.set_mesh_contacts(0, tg, tr, sl, mh)
+    s.mesh_stress(0, pref)
+    s.setup()
+    o.rebuild()
+    o.compute_forces(0)
+    for _ in range(2):
+        g, r = s.mesh_stress_vector(0), o.mesh_stress_vector(0)
+        assert np.linalg.norm(r[:3]) > 0
+        assert np.abs(g[:3] - r[:3]).max() <= 1e-10 * np.linalg.norm(r[:3])
+        assert np.abs(g[3:6] - r[3:6]).max() <= 1e-10 * np.linalg.norm(r[3:6])
+        assert np.array_equal(g[6:], r[6:])
+        s.step(40)
+        o.run(40)

@@ -373,3 +373,49 @@ def test_rolling_resistance_gold():
     # the sphere stayed on the mesh: one mesh contact with history
     tg, tr, hi = o.mesh_contacts()
     assert len(tg) == 1 and np.abs(hi).max() > 0
+
+
+def test_mesh_stress_closed_form():
+    """fix mesh/surface/stress: spheres resting at their static Hertz
+    overlap on a flat triangulated floor (one of them on an edge shared by
+    two triangles, one on a node) load the mesh with their weights.  The
+    total force is -sum m g along z and the torque about p_ref is
+    sum (x_c - p_ref) x (0, 0, -m g), x_c the contact point under each
+    sphere (mesh_module_stress.cpp:318-344)."""
+    g = 9.81
+    r = 1e-3
+    L = 0.02
+    tris = flat_grid(4, L, x0=-0.01, y0=-0.01)
+    # Hertz static overlap against the floor: m g = 4/3 E* sqrt(r) d^1.5,
+    # E* of two type pairs (1, 2) with equal Y and nu
+    Y, nu = 5e6, 0.3
+    Es = 1.0 / (2 * (1 - nu * nu) / Y)
+    pos = [(0.0031, -0.0042), (0.0, 0.0025), (-0.005, -0.005), (0.004, 0.0066)]   # face, edge, node, face
+    xs, rs = [], []
+    for (a, b), rr in zip(pos, (r, 0.8 * r, 1.2 * r, 0.9 * r)):
+        m = 4.0 / 3.0 * np.pi * rr ** 3 * 2500.0
+        d = (m * g / (4.0 / 3.0 * Es * np.sqrt(rr))) ** (2.0 / 3.0)
+        xs.append([a, b, rr - d])
+        rs.append(rr)
+    n = len(xs)
+    rs = np.array(rs)
+    ms = 4.0 / 3.0 * np.pi * rs ** 3 * 2500.0
+    atoms = dict(x=np.array(xs), v=np.zeros((n, 3)), omega=np.zeros((n, 3)), radius=rs, rmass=ms,
+                 type=np.ones(n, int), tag=np.arange(1, n + 1))
+    spec = base_spec(gravity=(0.0, 0.0, -g), dt=1e-7, meshes=[dict(nodes=tris, type=2)])
+    o = Oracle(spec, atoms)
+    pref = np.array([0.002, -0.003, 0.01])
+    o.mesh_stress(0, pref)
+    o.setup()
+    o.run(20)
+    sv = o.mesh_stress_vector(0)
+    st = o.state()
+    W = (ms * g).sum()
+    assert sv[2] == pytest.approx(-W, rel=1e-9)
+    assert abs(sv[0]) + abs(sv[1]) < 1e-12 * W
+    cp = np.stack([st["x"][:, 0], st["x"][:, 1], np.zeros(n)], 1) - pref
+    torque = np.cross(cp, np.stack([np.zeros(n), np.zeros(n), -ms * g], 1)).sum(0)
+    assert np.allclose(sv[3:6], torque, rtol=1e-8, atol=1e-12 * W * L)
+    assert np.array_equal(sv[6:9], pref)
+    # the force on the mesh is minus the force the mesh puts on the spheres
+    assert np.allclose(sv[:3], -o.mesh_force(0), rtol=1e-14, atol=0)
