@@ -174,6 +174,19 @@ int dem_mesh_stress(dem_ctx *ctx, int mesh, const double *pref);
  * total force, out[3..5] torque about p_ref, out[6..8] p_ref, of the last
  * force pass. */
 int dem_mesh_stress_vector(dem_ctx *ctx, int mesh, double *out);
+/* fix massflow/mesh ... mesh <face> vec_side vx vy vz count once|multiple
+ * (fix_massflow_mesh.cpp:95-300, post_integrate :366-530): a planar face
+ * mesh (nodes as for dem_add_mesh; it is no wall) through which particles
+ * are counted when they pass to the vec_side side inside a triangle's face
+ * region, once ever (once != 0) or at every crossing; group_bit restricts
+ * the count to a group (-1: all).  One per context; no point_at_outlet or
+ * delete_atoms. */
+int dem_add_massflow(dem_ctx *ctx, int ntri, const double *nodes, const double *vec_side, int once,
+                     int group_bit);
+/* FixMassflowMesh::compute_vector (fix_massflow_mesh.cpp:637-662): out[0]
+ * mass and out[1] number of particles counted so far, out[2] and out[3]
+ * their rates over the time since the previous call. */
+int dem_massflow_vector(dem_ctx *ctx, double *out);
 /* Bond tables (DEM_NBOND) and settings of the cohesion model
  * (cohesion_model_bond.h:237-470, cohesion_model_bond_nonlinear.h:204-386).
  * Call after dem_upload_state: registers the model's contact distance
@@ -200,6 +213,21 @@ int dem_compute_forces(dem_ctx *ctx, int shearupdate);
  * (neigh_gran.cpp:485-640) + primitive-wall lists. */
 int dem_rebuild(dem_ctx *ctx);
 
+/* Binary restart of the device state (write_restart / read_restart,
+ * write_restart.cpp:119-200, read_restart.cpp:106): a header (step, time
+ * bookkeeping, history sizes), the atoms in upload order (tag, type, fix
+ * freeze membership, group bits, x, v, omega, radius, rmass), each atom's
+ * contact-history record in the layout of FixContactHistory::pack_restart
+ * (fix_contact_history.cpp:588-602: record size, partner count, then per
+ * partner its tag and the dnum history values, integers in the bits of a
+ * double as LAMMPS' ubuf stores them), the primitive-wall history per atom
+ * and the mesh contact slots.  dem_read_restart needs a context with the
+ * same contact model, planes and meshes (added in their current positions);
+ * it uploads the atoms, restores their groups and histories and the step,
+ * and is followed by dem_setup.  fix move and massflow counters are not
+ * carried (an upload clears them). */
+int dem_write_restart(dem_ctx *ctx, const char *path);
+int dem_read_restart(dem_ctx *ctx, const char *path);
 /* Directed pair set of the current (full) list and its contact history:
  * entry (tag_i, tag_j, shift, hist[dnum], flag), shift packs periodic
  * image offsets as (sx+1)+3(sy+1)+9(sz+1), 13 = none.  This is the data

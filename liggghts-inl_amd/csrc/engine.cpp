@@ -63,6 +63,8 @@ struct MeshDev {
     // the mesh
     bool stress = false;
     double pref[3] = {0, 0, 0};
+    // a face mesh of fix massflow/mesh: neighbour rows only, no contacts
+    bool passive = false;
     double *d_ms = nullptr, *d_spart = nullptr, *d_sout = nullptr;
     int ms_cap = 0;
 };
@@ -192,9 +194,23 @@ struct dem_ctx {
     int64_t bcap = 0;
 
     std::vector<MeshDev> meshes;
+    // fix massflow/mesh (one per context): its face mesh, side counters
+    // per particle, running totals on the device; the rate bookkeeping of
+    // FixMassflowMesh::compute_vector on the host
+    struct Massflow {
+        int mesh = -1, once = 1, gbit = -1;
+        double pref[3] = {0, 0, 0}, nvec[3] = {0, 0, 0};
+        int *counter[2] = {};
+        double *part = nullptr, *total = nullptr;
+        double t_since = 0.0, delta_t = 0.0, mass_last = 0.0, n_last = 0.0, mass_prev = 0.0, n_prev = 0.0;
+        bool stepped = false;
+    } mf;
     bool mesh_grids_valid = false;    // built for the current radii
     long long next_reneighbor = -1;   // FixMesh::next_reneighbor
 };
+
+static int massflow_step(dem_ctx *ctx);
+static int massflow_counters(dem_ctx *ctx);
 
 #define HIPCHK(call)                                                              \
     do {                                                                          \
@@ -402,11 +418,12 @@ static void free_all(dem_ctx *ctx)
         F(ctx->img[b]); F(ctx->whist[b]); F(ctx->nstart[b]); F(ctx->nj[b]); F(ctx->hist[b]);
         F(ctx->gmask[b]);
         F(ctx->xo[b]);
+        F(ctx->mf.counter[b]);
 
     }
     F(ctx->f4); F(ctx->t4); F(ctx->rec); F(ctx->xhold); F(ctx->wmask); F(ctx->ncontact); F(ctx->key);
     F(ctx->sorted); F(ctx->cstart); F(ctx->cursor); F(ctx->scan_tmp); F(ctx->spread); F(ctx->stage); F(ctx->mstage);
-    F(ctx->d_mat); F(ctx->d_small);
+    F(ctx->d_mat); F(ctx->d_small); F(ctx->mf.part); F(ctx->mf.total);
     F(ctx->bout); F(ctx->bvalid); F(ctx->brev);
     F(ctx->pwf); F(ctx->d_obs); F(ctx->d_gtot); F(ctx->d_gpart);
     for (char *c : ctx->ckpts) F(c);
@@ -850,6 +867,8 @@ extern "C" int dem_upload_state(dem_ctx *ctx, int n, const double *x, const doub
     }
     // the group bits are cleared, and with them the groups fix move acts on
     ctx->moves.clear();
+    // new atoms: massflow side counters not seen yet
+    if (ctx->mf.mesh >= 0 && massflow_counters(ctx)) return DEM_ERR_HIP;
     ctx->integ_bit = -1;
     ctx->have_list = false;
     ctx->list_valid = false;
@@ -1028,6 +1047,8 @@ static int rebuild(dem_ctx *ctx, bool captured = false)
     pa.gm1 = ctx->gmask[c1];
     pa.xo0 = ctx->moves.empty() ? nullptr : ctx->xo[c0];
     pa.xo1 = ctx->moves.empty() ? nullptr : ctx->xo[c1];
+    pa.mf0 = ctx->mf.mesh >= 0 ? ctx->mf.counter[c0] : nullptr;
+    pa.mf1 = ctx->mf.mesh >= 0 ? ctx->mf.counter[c1] : nullptr;
     pa.xhold = ctx->xhold;
     pa.wmask = ctx->wmask;
     pa.n = n;
@@ -1129,6 +1150,15 @@ static int rebuild(dem_ctx *ctx, bool captured = false)
 
 static int integrate(dem_ctx *ctx, int mode);
 
+// index of the last mesh that is a wall (not a massflow face), -1 if none
+static int last_wall_mesh(const dem_ctx *ctx)
+{
+    int k = -1;
+    for (size_t q = 0; q < ctx->meshes.size(); q++)
+        if (!ctx->meshes[q].passive) k = (int)q;
+    return k;
+}
+
 // nothing adds to the forces after the contact pass (or, with mesh walls,
 // after the last mesh pass), so that pass may integrate
 static bool fuse_possible(const dem_ctx *ctx)
@@ -1160,13 +1190,14 @@ static int force_pass(dem_ctx *ctx, int shearupdate, int integ = 0)
     fa.shearupdate = shearupdate;
     fa.M = ctx->M;
     fa.P = ctx->P;
-    fa.defer_freeze = ctx->meshes.empty() ? 0 : 1;
+    const int lastwall = last_wall_mesh(ctx);
+    fa.defer_freeze = lastwall < 0 ? 0 : 1;
     fa.pwf = ctx->obs_plane >= 0 ? ctx->pwf : nullptr;
     fa.obs_plane = ctx->obs_plane;
     fa.lanes = ctx->tune_lanes;
     fa.block = ctx->tune_block;
-    const bool fuse = integ && ctx->meshes.empty() && fuse_possible(ctx) && force_can_fuse(fa);
-    const bool mfuse = integ && !ctx->meshes.empty() && fuse_possible(ctx);
+    const bool fuse = integ && lastwall < 0 && fuse_possible(ctx) && force_can_fuse(fa);
+    const bool mfuse = integ && lastwall >= 0 && fuse_possible(ctx);
     if (fuse) {
         fa.fuse = integ;
         fa.xn = ctx->par ? ctx->xr[c] : ctx->xn;
@@ -1214,6 +1245,7 @@ static int force_pass(dem_ctx *ctx, int shearupdate, int integ = 0)
     if (fuse) ctx->par ^= 1;
     for (size_t k = 0; k < ctx->meshes.size(); k++) {
         MeshDev &m = ctx->meshes[k];
+        if (m.passive) continue;
         MeshForceArgs ma{};
         ma.xr = fa.xr; ma.vm = fa.vm; ma.wt = fa.wt;
         ma.f4 = fa.f4; ma.t4 = fa.t4;
@@ -1232,7 +1264,7 @@ static int force_pass(dem_ctx *ctx, int shearupdate, int integ = 0)
         for (int q = 0; q < 3; q++) ma.vel[q] = m.vel[q];
         ma.curvature = m.h.curvature;
         ma.precision = m.h.precision;
-        ma.apply_freeze = k + 1 == ctx->meshes.size();
+        ma.apply_freeze = (int)k == lastwall;
         if (mfuse && ma.apply_freeze) {
             ma.fuse = integ;
             ma.xn = ctx->par ? ctx->xr[c] : ctx->xn;
@@ -1620,6 +1652,9 @@ static std::vector<const void *> graph_key(const dem_ctx *ctx)
         key_bytes(k, m.grid.inv, sizeof(m.grid.inv));
         key_bytes(k, m.grid.n, sizeof(m.grid.n));
     }
+    k.push_back(ctx->mf.counter[0]);
+    k.push_back(ctx->mf.counter[1]);
+    k.push_back(ctx->mf.total);
     key_bytes(k, &ctx->p, sizeof(ctx->p));
     key_bytes(k, &ctx->M, sizeof(ctx->M));
     key_bytes(k, &ctx->G, sizeof(ctx->G));
@@ -1710,10 +1745,13 @@ static int capture_graph(dem_ctx *ctx)
     HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
     ctx->gexec_capturing = true;
     int rc = DEM_OK;
+    const auto mf_keep = ctx->mf;   // capture records kernels; the host bookkeeping is redone per replay
     for (int k = 1; k <= 2 * every && rc == DEM_OK; k++) {
-        if (k % every == 0) rc = rebuild(ctx, true);
+        if (ctx->mf.mesh >= 0) rc = massflow_step(ctx);
+        if (rc == DEM_OK && k % every == 0) rc = rebuild(ctx, true);
         if (rc == DEM_OK) rc = force_pass(ctx, 1, 3);
     }
+    ctx->mf = mf_keep;
     ctx->gexec_capturing = false;
     const hipError_t ec = hipStreamEndCapture(ctx->stream, &graph);
     // a replay must leave the state in the buffers it found it in
@@ -1765,12 +1803,21 @@ extern "C" int dem_step(dem_ctx *ctx, int nsteps)
                 HIPCHK(hipGraphLaunch(ctx->gexec, ctx->stream));
                 ctx->ntimestep += ctx->g_period;
                 st += ctx->g_period;
+                if (ctx->mf.mesh >= 0) {
+                    if (!ctx->mf.stepped) {
+                        ctx->mf.mass_last = ctx->mf.mass_prev;
+                        ctx->mf.n_last = ctx->mf.n_prev;
+                        ctx->mf.stepped = true;
+                    }
+                    ctx->mf.t_since += ctx->g_period * ctx->p.dt;
+                }
             }
             if ((rc = check_err_flag(ctx))) return rc;
             if (st >= nsteps) break;
         }
         ctx->ntimestep++;
         if (st == 0 && ((rc = integrate(ctx, 1)) || (rc = move_meshes(ctx)))) return rc;
+        if (ctx->mf.mesh >= 0 && (rc = massflow_step(ctx))) return rc;
         int nflag = 0;
         if ((rc = decide(ctx, &nflag))) return rc;
         if (nflag && (rc = ctx->window ? window_mask(ctx) : rebuild(ctx))) return rc;
@@ -2260,6 +2307,354 @@ extern "C" int dem_mesh_stress_vector(dem_ctx *ctx, int mesh, double *out)
     HIPCHK(hipMemcpyAsync(out, m.d_sout, 6 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     for (int q = 0; q < 3; q++) out[6 + q] = m.pref[q];
+    return DEM_OK;
+}
+
+// FixMassflowMesh::post_integrate of one step, on the face mesh's rows of
+// the last rebuild (none before the first)
+static int massflow_step(dem_ctx *ctx)
+{
+    auto &f = ctx->mf;
+    const MeshDev &m = ctx->meshes[f.mesh];
+    // the rate bookkeeping (fix_massflow_mesh.cpp:398-405): the first step
+    // after a read of the vector starts the next interval
+    f.t_since += ctx->p.dt;
+    if (!f.stepped) {
+        f.mass_last = f.mass_prev;
+        f.n_last = f.n_prev;
+        f.stepped = true;
+    }
+    if (!m.have) return DEM_OK;
+    MassflowArgs a{};
+    a.xr = sx(ctx);
+    a.vm = sv(ctx);
+    a.tri = m.tri;
+    a.mstart = m.mstart[m.cur];
+    a.mcand = m.mcand[m.cur];
+    a.counter = f.counter[ctx->cur];
+    a.gmask = f.gbit >= 0 ? ctx->gmask[ctx->cur] : nullptr;
+    a.gbit = f.gbit;
+    a.n = ctx->n;
+    a.once = f.once;
+    for (int q = 0; q < 3; q++) {
+        a.pref[q] = f.pref[q];
+        a.nvec[q] = f.nvec[q];
+    }
+    a.precision = m.h.precision;
+    a.part = f.part;
+    a.total = f.total;
+    launch_massflow(a, ctx->stream);
+    KCHK();
+    return DEM_OK;
+}
+
+static int massflow_counters(dem_ctx *ctx)
+{
+    auto &f = ctx->mf;
+    for (int b = 0; b < 2; b++)
+        if (dev_alloc(ctx, &f.counter[b], std::max(ctx->cap, 1))) return DEM_ERR_HIP;
+    // -1: not seen yet (the counter property's default, fix_massflow_mesh.cpp:240-250)
+    if (ctx->n > 0) HIPCHK(hipMemset(f.counter[ctx->cur], 0xff, sizeof(int) * ctx->n));
+    return DEM_OK;
+}
+
+extern "C" int dem_add_massflow(dem_ctx *ctx, int ntri, const double *nodes, const double *vec_side, int once,
+                                int group_bit)
+{
+    if (!ctx) return fail(nullptr, DEM_ERR_ARG, "null context");
+    if (ctx->mf.mesh >= 0) return fail(ctx, DEM_ERR_ARG, "dem_add_massflow: one fix massflow/mesh per context");
+    if (!vec_side || (vec_side[0] == 0.0 && vec_side[1] == 0.0 && vec_side[2] == 0.0))
+        return fail(ctx, DEM_ERR_ARG, "vec_side > 0 required");
+    if (group_bit < -1 || group_bit > 30) return fail(ctx, DEM_ERR_ARG, "dem_add_massflow: bad group bit");
+    const int k = dem_add_mesh(ctx, ntri, nodes, 1, 0.0, nullptr);
+    if (k < 0) return k;
+    MeshDev &m = ctx->meshes[k];
+    m.passive = true;
+    auto &f = ctx->mf;
+    // FixMassflowMesh::setup: a planar face (TriMesh::isPlanar); setRefPoint
+    // (fix_massflow_mesh.cpp:669-681): node 0 and the normal of element 0,
+    // turned towards vec_side
+    const double *n0 = &m.h.snorm[0];
+    for (int t = 1; t < m.h.ntri; t++) {
+        const double *nt = &m.h.snorm[3 * t];
+        const double d = n0[0] * nt[0] + n0[1] * nt[1] + n0[2] * nt[2];
+        if (std::fabs(std::fabs(d) - 1.0) > 1e-8) {
+            ctx->meshes.pop_back();
+            return fail(ctx, DEM_ERR_ARG, "fix massflow/mesh requires a planar face mass flow measurement");
+        }
+    }
+    for (int q = 0; q < 3; q++) {
+        f.pref[q] = m.h.node[q];
+        f.nvec[q] = n0[q];
+    }
+    const double dot = f.nvec[0] * vec_side[0] + f.nvec[1] * vec_side[1] + f.nvec[2] * vec_side[2];
+    if (std::fabs(dot) < 1e-6) {
+        ctx->meshes.pop_back();
+        return fail(ctx, DEM_ERR_ARG, "need to change 'vec_side', it is currently in or to close to the mesh plane");
+    }
+    if (dot < 0.0)
+        for (int q = 0; q < 3; q++) f.nvec[q] = -f.nvec[q];
+    f.mesh = k;
+    f.once = once ? 1 : 0;
+    f.gbit = group_bit;
+    if (dev_alloc(ctx, &f.part, 2 * kMassflowBlocks) || dev_alloc(ctx, &f.total, 2)) return DEM_ERR_HIP;
+    HIPCHK(hipMemset(f.total, 0, 2 * sizeof(double)));
+    return massflow_counters(ctx);
+}
+
+extern "C" int dem_massflow_vector(dem_ctx *ctx, double *out)
+{
+    if (!ctx) return fail(nullptr, DEM_ERR_ARG, "null context");
+    if (ctx->mf.mesh < 0 || !out) return fail(ctx, DEM_ERR_ARG, "dem_massflow_vector: no fix massflow/mesh");
+    auto &f = ctx->mf;
+    sync_enter(ctx);
+    double tot[2];
+    HIPCHK(hipMemcpyAsync(tot, f.total, sizeof(tot), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    // FixMassflowMesh::compute_vector (fix_massflow_mesh.cpp:637-662)
+    if (f.stepped) {
+        f.delta_t = f.t_since;
+        f.t_since = 0.0;
+        f.stepped = false;
+    }
+    out[0] = tot[0];
+    out[1] = tot[1];
+    out[2] = f.delta_t == 0.0 ? 0.0 : (tot[0] - f.mass_last) / f.delta_t;
+    out[3] = f.delta_t == 0.0 ? 0.0 : (tot[1] - f.n_last) / f.delta_t;
+    f.mass_prev = tot[0];
+    f.n_prev = tot[1];
+    return DEM_OK;
+}
+
+// ---- restart files ----
+// A binary restart of the device state: header, the atoms (upload order),
+// each atom's contact-history record exactly as FixContactHistory::
+// pack_restart writes it (fix_contact_history.cpp:588-602: record size,
+// partner count, then per partner its tag and dnum values, the integers
+// stored in the bits of a double as LAMMPS' ubuf does), the primitive-wall
+// history per atom (property/atom history_<id>) and the mesh contact slots.
+namespace {
+const char kRestartMagic[8] = {'L', 'I', 'G', 'D', 'E', 'M', 'R', '1'};
+struct RestartHeader {
+    char magic[8];
+    int32_t version, n, dnum, wdn, nmesh, dw;
+    int64_t ntimestep, atimestep;
+    double atime;
+};
+double ubuf_int(int v)
+{
+    union { double d; int64_t i; } u;
+    u.i = v;
+    return u.d;
+}
+int ubuf_to_int(double d)
+{
+    union { double d; int64_t i; } u;
+    u.d = d;
+    return (int)u.i;
+}
+}  // namespace
+
+extern "C" int dem_write_restart(dem_ctx *ctx, const char *path)
+{
+    if (!ctx || !path) return fail(ctx, DEM_ERR_ARG, "dem_write_restart: bad arguments");
+    if (!ctx->list_valid) return fail(ctx, DEM_ERR_ARG, "dem_write_restart: call dem_setup first");
+    sync_enter(ctx);
+    const int n = ctx->n, c = ctx->cur, dnum = ctx->M.dnum;
+    const int dw = ctx->M.wdnum, wdn = ctx->P.n * dw;
+    std::vector<double4> hx(n), hv(n), hw(n);
+    std::vector<int> ho(n), htag(n), hgm(n);
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipMemcpy(ho.data(), ctx->orig[c], n * sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(htag.data(), ctx->tag[c], n * sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(hgm.data(), ctx->gmask[c], n * sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(hx.data(), sx(ctx), n * sizeof(double4), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(hv.data(), sv(ctx), n * sizeof(double4), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(hw.data(), sw(ctx), n * sizeof(double4), hipMemcpyDeviceToHost));
+    // atoms in upload order
+    std::vector<int32_t> ai(4 * (size_t)n);   // tag, type, freeze, group bits
+    std::vector<double> ad(11 * (size_t)n);   // x, v, omega, radius, rmass
+    for (int q = 0; q < n; q++) {
+        const int o = ho[q];
+        const int meta = (int)hw[q].w;
+        ai[4 * o] = htag[q];
+        ai[4 * o + 1] = meta_type(meta);
+        ai[4 * o + 2] = meta_frozen(meta) ? 1 : 0;
+        ai[4 * o + 3] = hgm[q];
+        const double v[11] = {hx[q].x, hx[q].y, hx[q].z, hv[q].x, hv[q].y, hv[q].z, hw[q].x, hw[q].y, hw[q].z,
+                              hx[q].w, hv[q].w};
+        for (int k = 0; k < 11; k++) ad[11 * (size_t)o + k] = v[k];
+    }
+    // contact history, per atom in upload order
+    int64_t E = 0;
+    int rc = dem_get_pairs(ctx, &E, nullptr, nullptr, nullptr, nullptr, nullptr);
+    if (rc) return rc;
+    std::vector<int> ti(E), tj(E), sh(E), fl(E);
+    std::vector<double> hh((size_t)E * dnum);
+    if (E && (rc = dem_get_pairs(ctx, &E, ti.data(), tj.data(), sh.data(), hh.data(), fl.data()))) return rc;
+    std::unordered_map<int, int> slot_of_tag;
+    for (int o = 0; o < n; o++) slot_of_tag[ai[4 * o]] = o;
+    std::vector<std::vector<int64_t>> touch(n);
+    for (int64_t e = 0; e < E; e++)
+        if (fl[e]) touch[slot_of_tag[ti[e]]].push_back(e);
+    std::vector<double> rec;
+    for (int o = 0; o < n; o++) {
+        const int np = (int)touch[o].size();
+        rec.push_back((double)((dnum + 1) * np + 2));
+        rec.push_back(ubuf_int(np));
+        for (int64_t e : touch[o]) {
+            rec.push_back(ubuf_int(tj[e]));
+            for (int d = 0; d < dnum; d++) rec.push_back(hh[(size_t)e * dnum + d]);
+        }
+    }
+    std::vector<double> whh((size_t)n * wdn);
+    if (wdn && (rc = dem_get_wall_history(ctx, whh.data()))) return rc;
+    RestartHeader h{};
+    memcpy(h.magic, kRestartMagic, 8);
+    h.version = 1;
+    h.n = n;
+    h.dnum = dnum;
+    h.wdn = wdn;
+    h.nmesh = (int32_t)ctx->meshes.size();
+    h.dw = dw;
+    h.ntimestep = ctx->ntimestep;
+    h.atimestep = ctx->atimestep;
+    h.atime = ctx->atime;
+    FILE *fp = fopen(path, "wb");
+    if (!fp) return fail(ctx, DEM_ERR_ARG, std::string("dem_write_restart: cannot open ") + path);
+    bool ok = fwrite(&h, sizeof(h), 1, fp) == 1;
+    ok = ok && fwrite(ai.data(), sizeof(int32_t), ai.size(), fp) == ai.size();
+    ok = ok && fwrite(ad.data(), sizeof(double), ad.size(), fp) == ad.size();
+    const int64_t nrec = (int64_t)rec.size();
+    ok = ok && fwrite(&nrec, sizeof(nrec), 1, fp) == 1;
+    ok = ok && (nrec == 0 || fwrite(rec.data(), sizeof(double), rec.size(), fp) == rec.size());
+    ok = ok && (whh.empty() || fwrite(whh.data(), sizeof(double), whh.size(), fp) == whh.size());
+    for (int k = 0; k < h.nmesh && ok; k++) {
+        int64_t cnt = 0;
+        if ((rc = dem_get_mesh_contacts(ctx, k, &cnt, nullptr, nullptr, nullptr, nullptr))) {
+            fclose(fp);
+            return rc;
+        }
+        std::vector<int> mt(cnt), mtri(cnt), ms(cnt);
+        std::vector<double> mh((size_t)cnt * std::max(dw, 1));
+        if (cnt && (rc = dem_get_mesh_contacts(ctx, k, &cnt, mt.data(), mtri.data(), ms.data(), mh.data()))) {
+            fclose(fp);
+            return rc;
+        }
+        ok = fwrite(&cnt, sizeof(cnt), 1, fp) == 1;
+        for (int64_t q = 0; q < cnt && ok; q++) {
+            const int32_t tri3[3] = {mt[q], mtri[q], ms[q]};
+            ok = fwrite(tri3, sizeof(int32_t), 3, fp) == 3;
+        }
+        ok = ok && (cnt == 0 || fwrite(mh.data(), sizeof(double), (size_t)cnt * dw, fp) == (size_t)cnt * dw);
+    }
+    ok = (fclose(fp) == 0) && ok;
+    return ok ? DEM_OK : fail(ctx, DEM_ERR_ARG, std::string("dem_write_restart: write failed: ") + path);
+}
+
+extern "C" int dem_read_restart(dem_ctx *ctx, const char *path)
+{
+    if (!ctx || !path) return fail(ctx, DEM_ERR_ARG, "dem_read_restart: bad arguments");
+    FILE *fp = fopen(path, "rb");
+    if (!fp) return fail(ctx, DEM_ERR_ARG, std::string("dem_read_restart: cannot open ") + path);
+    RestartHeader h{};
+    auto bad = [&](const char *what) {
+        fclose(fp);
+        return fail(ctx, DEM_ERR_ARG, std::string("dem_read_restart: ") + what);
+    };
+    if (fread(&h, sizeof(h), 1, fp) != 1 || memcmp(h.magic, kRestartMagic, 8) != 0 || h.version != 1)
+        return bad("not a restart file of this engine");
+    // "saved simulation state used different contact history model - can not
+    // restart" (fix_contact_history.cpp:580-581)
+    if (h.dnum != ctx->M.dnum || h.dw != ctx->M.wdnum || h.wdn != ctx->P.n * ctx->M.wdnum
+        || h.nmesh != (int32_t)ctx->meshes.size())
+        return bad("saved simulation state used different contact history model or walls - can not restart");
+    const int n = h.n, dnum = h.dnum;
+    std::vector<int32_t> ai(4 * (size_t)n);
+    std::vector<double> ad(11 * (size_t)n);
+    if (fread(ai.data(), sizeof(int32_t), ai.size(), fp) != ai.size()
+        || fread(ad.data(), sizeof(double), ad.size(), fp) != ad.size())
+        return bad("truncated atoms");
+    std::vector<double> x(3 * (size_t)n), v(3 * (size_t)n), w(3 * (size_t)n), r(n), m(n);
+    std::vector<int> type(n), tag(n), frz(n), grp(n);
+    for (int o = 0; o < n; o++) {
+        tag[o] = ai[4 * o];
+        type[o] = ai[4 * o + 1];
+        frz[o] = ai[4 * o + 2];
+        grp[o] = ai[4 * o + 3];
+        for (int k = 0; k < 3; k++) {
+            x[3 * o + k] = ad[11 * (size_t)o + k];
+            v[3 * o + k] = ad[11 * (size_t)o + 3 + k];
+            w[3 * o + k] = ad[11 * (size_t)o + 6 + k];
+        }
+        r[o] = ad[11 * (size_t)o + 9];
+        m[o] = ad[11 * (size_t)o + 10];
+    }
+    int64_t nrec = 0;
+    if (fread(&nrec, sizeof(nrec), 1, fp) != 1 || nrec < 0) return bad("truncated history");
+    std::vector<double> rec(nrec);
+    if (nrec && fread(rec.data(), sizeof(double), nrec, fp) != (size_t)nrec) return bad("truncated history");
+    std::vector<double> whh((size_t)n * h.wdn);
+    if (!whh.empty() && fread(whh.data(), sizeof(double), whh.size(), fp) != whh.size())
+        return bad("truncated wall history");
+    struct MeshRec { std::vector<int> tag, tri, slot; std::vector<double> hist; };
+    std::vector<MeshRec> mr(h.nmesh);
+    for (int k = 0; k < h.nmesh; k++) {
+        int64_t cnt = 0;
+        if (fread(&cnt, sizeof(cnt), 1, fp) != 1 || cnt < 0) return bad("truncated mesh contacts");
+        mr[k].tag.resize(cnt);
+        mr[k].tri.resize(cnt);
+        mr[k].slot.resize(cnt);
+        mr[k].hist.resize((size_t)cnt * h.dw);
+        for (int64_t q = 0; q < cnt; q++) {
+            int32_t t3[3];
+            if (fread(t3, sizeof(int32_t), 3, fp) != 3) return bad("truncated mesh contacts");
+            mr[k].tag[q] = t3[0];
+            mr[k].tri[q] = t3[1];
+            mr[k].slot[q] = t3[2];
+        }
+        if (cnt && fread(mr[k].hist.data(), sizeof(double), mr[k].hist.size(), fp) != mr[k].hist.size())
+            return bad("truncated mesh contacts");
+    }
+    fclose(fp);
+    int rc = dem_upload_state(ctx, n, x.data(), v.data(), w.data(), r.data(), m.data(), type.data(), tag.data(),
+                              frz.data());
+    if (rc) return rc;
+    // group bits, in upload order (the device permutes them with the atoms)
+    for (int b = 0; b < 31; b++) {
+        bool any = false;
+        std::vector<int> mem(n);
+        for (int o = 0; o < n; o++) any |= (mem[o] = (grp[o] >> b) & 1) != 0;
+        if (any && (rc = dem_set_group(ctx, b, mem.data()))) return rc;
+    }
+    // FixContactHistory::unpack_restart (fix_contact_history.cpp:608-640)
+    std::vector<int> ti, tj;
+    std::vector<double> hh;
+    size_t q = 0;
+    for (int o = 0; o < n; o++) {
+        if (q + 2 > rec.size()) return fail(ctx, DEM_ERR_ARG, "dem_read_restart: bad history record");
+        const size_t size = (size_t)rec[q];
+        const int np = ubuf_to_int(rec[q + 1]);
+        if (np < 0 || size != (size_t)(dnum + 1) * np + 2 || q + size > rec.size())
+            return fail(ctx, DEM_ERR_ARG, "dem_read_restart: bad history record");
+        size_t p = q + 2;
+        for (int k = 0; k < np; k++) {
+            ti.push_back(tag[o]);
+            tj.push_back(ubuf_to_int(rec[p++]));
+            for (int d = 0; d < dnum; d++) hh.push_back(rec[p++]);
+        }
+        q += size;
+    }
+    if (!ti.empty() && (rc = dem_set_history(ctx, (int64_t)ti.size(), ti.data(), tj.data(), hh.data()))) return rc;
+    if (!whh.empty() && (rc = dem_set_wall_history(ctx, whh.data()))) return rc;
+    for (int k = 0; k < h.nmesh; k++)
+        if (!mr[k].tag.empty()
+            && (rc = dem_set_mesh_contacts(ctx, k, (int64_t)mr[k].tag.size(), mr[k].tag.data(), mr[k].tri.data(),
+                                           mr[k].slot.data(), mr[k].hist.data())))
+            return rc;
+    ctx->ntimestep = h.ntimestep;
+    ctx->atimestep = h.atimestep;
+    ctx->atime = h.atime;
     return DEM_OK;
 }
 

@@ -79,6 +79,8 @@ struct PermuteArgs {
     int *gm1;
     const double4 *xo0;  // fix move origins (may be null)
     double4 *xo1;
+    const int *mf0;      // fix massflow/mesh side counters (may be null)
+    int *mf1;
     uint8_t *wmask;
     int n;
     int wdn;             // nplanes * dnum
@@ -307,6 +309,23 @@ void launch_force_mixed(const ForceArgs &a, hipStream_t s);
 // mode: 1 = initial_integrate, 2 = final_integrate, 3 = final then initial
 void launch_integrate(const IntegArgs &a, int mode, hipStream_t s);
 void launch_pack_rec(const double4 *vm, const double4 *wt, float4 *rec, int n, hipStream_t s);
+// fix massflow/mesh on a planar face mesh (FixMassflowMesh::post_integrate,
+// fix_massflow_mesh.cpp:366-530): per particle in the mesh's candidate rows
+struct MassflowArgs {
+    const double4 *xr, *vm;
+    const TriGeo *tri;
+    const int *mstart, *mcand;
+    int *counter;        // per particle: -1 not yet seen, 0 / 1 side, 2 counted once
+    const int *gmask;    // group bits, null: every particle
+    int gbit;
+    int n, once;
+    double pref[3], nvec[3];
+    double precision;
+    double *part;        // kMassflowBlocks * 2 partial sums (mass, count)
+    double *total;       // running totals (mass, count)
+};
+constexpr int kMassflowBlocks = 256;
+void launch_massflow(const MassflowArgs &a, hipStream_t s);
 // out[0..5] = sum over particles of v[6 i + k], a fixed-order reduction
 // (part: kSumBlocks * 6 doubles of partial sums)
 constexpr int kSumBlocks = 256;

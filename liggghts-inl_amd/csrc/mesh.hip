@@ -473,4 +473,82 @@ void launch_sum6(const double *v, int n, double *part, double *out, hipStream_t 
     k_sum6_total<<<1, 64, 0, s>>>(part, kSumBlocks, out);
 }
 
+// FixMassflowMesh::post_integrate for a planar face (no point_at_outlet):
+// the side of a particle is the sign of (x - p_ref) . n, n the face normal
+// turned towards vec_side.  A particle in some triangle's neighbour list
+// that is seen for the first time only records its side; one that has
+// moved to the n side and lies in the face region (barySign 7) of one of
+// its triangles counts once per crossing (or once ever with `count once`);
+// one on the other side is re-armed.  The triangles of a particle are
+// independent here, so one pass per particle over its candidate row gives
+// the reference's triangle-by-triangle result.  The step's mass and count
+// are summed in a fixed order and added to the running totals.
+__global__ void __launch_bounds__(kBlock) k_massflow(MassflowArgs a)
+{
+    __shared__ double s_m[kBlock], s_c[kBlock];
+    double mass = 0.0, cnt = 0.0;
+    const int per = (a.n + gridDim.x - 1) / gridDim.x;
+    const int p0 = blockIdx.x * per, p1 = min(a.n, p0 + per);
+    for (int i = p0 + (int)threadIdx.x; i < p1; i += kBlock) {
+        if (a.gmask && !((a.gmask[i] >> a.gbit) & 1)) continue;
+        const int c = a.counter[i];
+        if (c == 2) continue;
+        const int b0 = a.mstart[i], b1 = a.mstart[i + 1];
+        if (b1 == b0) continue;   // in no triangle's neighbour list
+        const double4 x = a.xr[i];
+        const double dot = (x.x - a.pref[0]) * a.nvec[0] + (x.y - a.pref[1]) * a.nvec[1] + (x.z - a.pref[2]) * a.nvec[2];
+        if (c == -1) {
+            a.counter[i] = dot <= 0.0 ? 0 : 1;
+            continue;
+        }
+        if (dot > 0.0) {
+            if (c != 0) continue;
+            const double cs[3] = {x.x, x.y, x.z};
+            bool face = false;
+            for (int k = b0; k < b1 && !face; k++) {
+                double delta[3], bary[3];
+                int region = -1;
+                tri_contact(a.tri[a.mcand[k]], x.w, cs, a.precision, delta, bary, region);
+                face = region == 7;
+            }
+            if (face) {
+                mass += a.vm[i].w;
+                cnt += 1.0;
+                a.counter[i] = a.once ? 2 : 1;
+            }
+        } else {
+            a.counter[i] = 0;
+        }
+    }
+    s_m[threadIdx.x] = mass;
+    s_c[threadIdx.x] = cnt;
+    __syncthreads();
+    for (int h = kBlock / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) {
+            s_m[threadIdx.x] += s_m[threadIdx.x + h];
+            s_c[threadIdx.x] += s_c[threadIdx.x + h];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        a.part[2 * blockIdx.x] = s_m[0];
+        a.part[2 * blockIdx.x + 1] = s_c[0];
+    }
+}
+
+__global__ void k_massflow_total(const double *part, int nb, double *total)
+{
+    if (threadIdx.x >= 2) return;
+    double s = 0.0;
+    for (int b = 0; b < nb; b++) s += part[2 * b + threadIdx.x];
+    total[threadIdx.x] += s;
+}
+
+void launch_massflow(const MassflowArgs &a, hipStream_t s)
+{
+    if (a.n == 0) return;
+    k_massflow<<<kMassflowBlocks, kBlock, 0, s>>>(a);
+    k_massflow_total<<<1, 64, 0, s>>>(a.part, kMassflowBlocks, a.total);
+}
+
 }  // namespace dem

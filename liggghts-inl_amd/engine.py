@@ -147,6 +147,10 @@ def library():
                                        C.c_double, ip]),
         "dem_unfix": (C.c_int, [vp, C.c_int]),
         "dem_mesh_stress": (C.c_int, [vp, C.c_int, dp]),
+        "dem_write_restart": (C.c_int, [vp, C.c_char_p]),
+        "dem_read_restart": (C.c_int, [vp, C.c_char_p]),
+        "dem_add_massflow": (C.c_int, [vp, C.c_int, dp, dp, C.c_int, C.c_int]),
+        "dem_massflow_vector": (C.c_int, [vp, dp]),
         "dem_mesh_stress_vector": (C.c_int, [vp, C.c_int, dp]),
         "dem_set_integrate_group": (C.c_int, [vp, C.c_int]),
         "dem_add_move_linear": (C.c_int, [vp, C.c_int, ip, dp, ip]),
@@ -305,6 +309,34 @@ class Simulation:
         hi = np.zeros((max(c, 1), dw))
         self._chk(self.L.dem_get_mesh_contacts(self.h, int(k), C.byref(cnt), _ip(tg), _ip(tr), _ip(sl), _dp(hi)))
         return tg[:c], tr[:c], sl[:c], hi[:c]
+
+    def write_restart(self, path):
+        """Binary restart of the device state (dem_write_restart)."""
+        self._chk(self.L.dem_write_restart(self.h, os.fsencode(path)))
+
+    def read_restart(self, path):
+        """Restores a restart written by write_restart; call setup() next."""
+        self._chk(self.L.dem_read_restart(self.h, os.fsencode(path)))
+        # the atoms are now in the file's order: header (56 bytes: magic,
+        # version, n, ...), then tag, type, freeze, groups per atom
+        with open(path, "rb") as fp:
+            head = fp.read(56)
+            n = int(np.frombuffer(head, dtype=np.int32, count=1, offset=12)[0])
+            ai = np.frombuffer(fp.read(16 * n), dtype=np.int32).reshape(n, 4)
+        self.n = n
+        self.tag = ai[:, 0].copy()
+
+    def add_massflow(self, nodes, vec_side, once=True, group_bit=-1):
+        """fix massflow/mesh on a planar face mesh (ntri, 3, 3)."""
+        nd = np.ascontiguousarray(np.asarray(nodes, dtype=np.float64).reshape(-1, 9))
+        vs = np.ascontiguousarray(vec_side, dtype=np.float64)
+        self._chk(self.L.dem_add_massflow(self.h, len(nd), _dp(nd), _dp(vs), int(bool(once)), int(group_bit)))
+
+    def massflow_vector(self):
+        """(mass, count, mass rate, count rate) as compute_vector gives them."""
+        out = np.zeros(4)
+        self._chk(self.L.dem_massflow_vector(self.h, _dp(out)))
+        return out
 
     def mesh_stress(self, k=0, pref=(0.0, 0.0, 0.0)):
         """fix mesh/surface/stress on mesh k with reference point pref."""

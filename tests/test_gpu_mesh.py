@@ -344,3 +344,61 @@ def test_hopper_mesh_stress():
         assert np.array_equal(g[6:], r[6:])
         s.step(40)
         o.run(40)
+
+
+def freefall_massflow(vec_side, steps, dt=1e-5):
+    """Spheres falling freely (no contacts, no walls) onto a horizontal
+    square face mesh at z0 = 50 mm (two triangles split along x = y), counted
+    by fix massflow/mesh."""
+    eng = load_engine()
+    from test_oracle_mesh import base_spec, flat_grid, mats
+
+    rng = np.random.default_rng(5)
+    g = np.arange(7) * 0.004 + 0.0075
+    xy = np.stack(np.meshgrid(g, g, indexing="ij"), -1).reshape(-1, 2)
+    xy = xy[np.abs(xy[:, 0] - xy[:, 1]) > 1.5e-3]          # off the shared diagonal edge
+    n = len(xy)
+    r = 0.5e-3
+    z = 0.0505 + rng.permutation(n) * (0.035 / n) + rng.uniform(0, 1e-4, n)
+    x = np.column_stack([xy, z])
+    m = 4.0 / 3.0 * np.pi * r ** 3 * 2500.0
+    atoms = dict(x=x, v=np.zeros((n, 3)), omega=np.zeros((n, 3)), radius=np.full(n, r), rmass=np.full(n, m),
+                 type=np.ones(n, int), tag=np.arange(1, n + 1))
+    spec = base_spec(box_lo=[0, 0, 0], box_hi=[0.04, 0.04, 0.1], dt=dt, skin=2.5e-4, neigh_every=1,
+                     neigh_check=False, gravity=(0.0, 0.0, -9.81), materials=mats(ntypes=1))
+    s = eng.Simulation(spec, atoms)
+    s.add_massflow(flat_grid(1, 0.03, z=0.05, x0=0.005, y0=0.005), vec_side, once=True)
+    s.setup()
+    return s, atoms, m
+
+
+def test_massflow_free_fall_closed_form():
+    """fix massflow/mesh counts exactly the spheres whose free-fall height
+    z0 - g t^2 / 2 has passed the face (velocity Verlet is exact for a
+    constant force), and its rates are the count differences over the time
+    between two reads; with vec_side pointing up nothing is counted."""
+    dt = 1e-5
+    s, atoms, m = freefall_massflow((0.0, 0.0, -1.0), 0, dt)
+    z0 = atoms["x"][:, 2]
+    out = []
+    for nsteps in (3000, 7000):
+        s.step(nsteps)
+        out.append(s.massflow_vector())
+    t1, t2 = 3000 * dt, 10000 * dt
+    for t, v in ((t1, out[0]), (t2, out[1])):
+        zt = z0 - 0.5 * 9.81 * t * t
+        assert np.abs(zt - 0.05).min() > 1e-6        # no sphere at the face at the read
+        crossed = (zt < 0.05).sum()
+        assert v[1] == crossed
+        assert v[0] == pytest.approx(crossed * m, rel=1e-14)
+    st = s.state()
+    assert np.allclose(st["x"][:, 2], z0 - 0.5 * 9.81 * t2 * t2, rtol=0, atol=1e-12)
+    assert 0 < out[0][1] < out[1][1] < len(z0)
+    # rates: the second read over the 7000 steps since the first
+    assert out[1][3] == pytest.approx((out[1][1] - out[0][1]) / (7000 * dt), rel=1e-12)
+    assert out[1][2] == pytest.approx((out[1][0] - out[0][0]) / (7000 * dt), rel=1e-12)
+    s.close()
+    s, atoms, m = freefall_massflow((0.0, 0.0, 1.0), 0, dt)
+    s.step(10000)
+    assert s.massflow_vector()[1] == 0
+    s.close()
