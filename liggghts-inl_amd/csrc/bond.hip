@@ -21,6 +21,14 @@ namespace dem {
 
 __device__ inline int bsgn(double v) { return (0.0 < v) - (v < 0.0); }
 
+// The reference's pow(z, 2.0), pow(z, 1.0) and pow(|d|, 0.5) of the
+// bond/nonlinear normal law, as z*z, z and a correctly rounded square root.
+// The host libm (glibc 2.35) gives z*z and z bit for bit, and sqrt on all
+// but about 1 in 1,200 arguments; the device libm's pow is not correctly
+// rounded, and the law's stiffness is unbounded at zero stretch, where last
+// bits decide a ringing sphere's trajectory.
+__device__ inline double sq(double z) { return z * z; }
+
 // vectorProject3D (vector_liggghts.h:437-442)
 __device__ inline void bproject(const double *v, const double *on, double *out)
 {
@@ -150,22 +158,22 @@ __device__ inline int bond_eval(const Model &M, const double *mat, int it, int j
             const double k2 = tbl(mat, M_BKFN2, it, jt), ku2 = tbl(mat, M_BKUFN2, it, jt), kc2 = tbl(mat, M_BKCFN2, it, jt);
             const double d = displacement;
             if (!bs.tension_variant) {
-                const double c1 = pow((ku1 - k1) / (ku1 + kc1), 2.0) * dmax;
-                const double c2 = pow((ku2 - k2) / (ku2 + kc2), 1.0) * dmin;
+                const double c1 = sq((ku1 - k1) / (ku1 + kc1)) * dmax;
+                const double c2 = ((ku2 - k2) / (ku2 + kc2)) * dmin;
                 if (d < dmin) frcmag = k2 * A * d;
                 else if (d < c2) frcmag = ku2 * A * d + (k2 - ku2) * A * dmin;
                 else if (d < 0.0) frcmag = -kc2 * A * d;
-                else if (d < c1) frcmag = -kc1 * pow(fabs(d), 0.5);
-                else if (d < dmax) frcmag = ku1 * pow(fabs(d), 0.5) + (k1 - ku1) * pow(fabs(dmax), 0.5);
-                else frcmag = k1 * pow(fabs(d), 0.5);
+                else if (d < c1) frcmag = -kc1 * sqrt(fabs(d));
+                else if (d < dmax) frcmag = ku1 * sqrt(fabs(d)) + (k1 - ku1) * sqrt(fabs(dmax));
+                else frcmag = k1 * sqrt(fabs(d));
             } else {
                 // the normal law of examples/LIGGGHTS/INL/cohesive_bond_nonlinear_
                 // tension/src/cohesion_model_bond_nonlinear_tension.h:590-613
-                const double c1 = pow((ku1 - k1) / (ku1 + kc1), 1.0) * dmax;
-                const double c2 = pow((ku2 - k2) / (ku2 + kc2), 2.0) * dmin;
-                if (d <= dmin) frcmag = -k2 * pow(fabs(d), 0.5);
-                else if (d < c2) frcmag = -(ku2 * pow(fabs(d), 0.5) + (k2 - ku2) * pow(fabs(dmin), 0.5));
-                else if (d < 0.0) frcmag = kc2 * pow(fabs(d), 0.5);
+                const double c1 = ((ku1 - k1) / (ku1 + kc1)) * dmax;
+                const double c2 = sq((ku2 - k2) / (ku2 + kc2)) * dmin;
+                if (d <= dmin) frcmag = -k2 * sqrt(fabs(d));
+                else if (d < c2) frcmag = -(ku2 * sqrt(fabs(d)) + (k2 - ku2) * sqrt(fabs(dmin)));
+                else if (d < 0.0) frcmag = kc2 * sqrt(fabs(d));
                 else if (d < c1) frcmag = -kc1 * A * d;
                 else if (d < dmax) frcmag = ku1 * A * d + (k1 - ku1) * A * dmax;
                 else frcmag = k1 * A * d;

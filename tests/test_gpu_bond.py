@@ -147,17 +147,16 @@ def test_bond_nonlinear_tension_golds_on_device():
     INL/cohesive_bond_nonlinear_tension) replayed on the device with fix
     freeze, fix move linear and nve/sphere on a group.
 
-    2 spheres, all 16,200 steps: every column to 1e-12 (measured 2.7e-15).
+    2 spheres, all 16,200 steps: every column to 1e-12 (measured 2.5e-15).
     3 spheres, the first 200,000 steps: the middle sphere rings between two
     square-root tension laws whose stiffness is unbounded at zero stretch,
-    so its trajectory is reproducible only bit for bit, and that takes the
-    host libm's pow(|d|, 0.5), which differs from a correctly rounded square
-    root in the last bit for some d (the oracle with sqrt instead of glibc's
-    pow no longer reproduces the gold either).  The device is held to what
-    does not depend on those bits: the frozen and the pulled sphere and the
-    time exactly, the middle sphere within twice the gold's own excursion
-    from the midpoint (7.3e-8 m) and with the gold's mean velocity, half
-    the pulling speed."""
+    so its trajectory follows the gold only while the square roots agree
+    bit for bit.  The device takes a correctly rounded sqrt where the
+    reference calls glibc's pow(|d|, 0.5), which differs from it on about
+    1 argument in 1,200; the device libm's pow differs far more often and
+    left the middle sphere at 18.7 m/s against the gold's 0.92 at the first
+    row.  Measured with sqrt: every column within 2.8e-10 of the gold's
+    column scale; the frozen and the pulled sphere and the time exactly."""
     import os
 
     from test_oracle_bond import GOLD
@@ -182,7 +181,4 @@ def test_bond_nonlinear_tension_golds_on_device():
         # time, x1, x3, v1, v3, f1 (frozen)
         exact = [3, 4, 6, 7, 9, 10]
         assert err[exact].max() < 1e-13, err
-        dev_g = np.abs(g[:, 5] - 0.5 * (g[:, 4] + g[:, 6])).max()
-        dev = np.abs(got[:, 5] - 0.5 * (got[:, 4] + got[:, 6])).max()
-        assert dev <= 2.0 * dev_g, (dev, dev_g)
-        assert abs(got[:, 8].mean() - g[:, 8].mean()) < 0.02, (got[:, 8].mean(), g[:, 8].mean())
+        assert err.max() < 1e-8, err

@@ -381,10 +381,11 @@ def test_massflow_free_fall_closed_form():
     s, atoms, m = freefall_massflow((0.0, 0.0, -1.0), 0, dt)
     z0 = atoms["x"][:, 2]
     out = []
-    for nsteps in (3000, 7000):
+    # reads after 3000 and 7000 steps: 5 and 29 of the 42 spheres crossed
+    for nsteps in (3000, 4000):
         s.step(nsteps)
         out.append(s.massflow_vector())
-    t1, t2 = 3000 * dt, 10000 * dt
+    t1, t2 = 3000 * dt, 7000 * dt
     for t, v in ((t1, out[0]), (t2, out[1])):
         zt = z0 - 0.5 * 9.81 * t * t
         assert np.abs(zt - 0.05).min() > 1e-6        # no sphere at the face at the read
@@ -394,9 +395,9 @@ def test_massflow_free_fall_closed_form():
     st = s.state()
     assert np.allclose(st["x"][:, 2], z0 - 0.5 * 9.81 * t2 * t2, rtol=0, atol=1e-12)
     assert 0 < out[0][1] < out[1][1] < len(z0)
-    # rates: the second read over the 7000 steps since the first
-    assert out[1][3] == pytest.approx((out[1][1] - out[0][1]) / (7000 * dt), rel=1e-12)
-    assert out[1][2] == pytest.approx((out[1][0] - out[0][0]) / (7000 * dt), rel=1e-12)
+    # rates: the second read over the 4000 steps since the first
+    assert out[1][3] == pytest.approx((out[1][1] - out[0][1]) / (4000 * dt), rel=1e-12)
+    assert out[1][2] == pytest.approx((out[1][0] - out[0][0]) / (4000 * dt), rel=1e-12)
     s.close()
     s, atoms, m = freefall_massflow((0.0, 0.0, 1.0), 0, dt)
     s.step(10000)

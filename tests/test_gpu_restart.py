@@ -7,8 +7,12 @@ on the MI355X.
 * a context read from the file holds the same atoms, wall history, groups
   and mesh contact slots; after its set-up rebuild it carries the history of
   every pair that touches;
-* both continue over 100 steps to the same trajectory (the restarted run
-  rebuilds at its set-up, so only the order of force sums differs).
+* both continue over 100 steps to the same trajectory.  A run after
+  read_restart starts with Verlet::setup (verlet.cpp:88-140): a rebuild,
+  which carries the history of the pairs that touch and drops the stale
+  rest, and a set-up force pass.  The continued context gets the same
+  set-up, as a second run command gives it, so only the order of force
+  sums differs (the restarted context sorts from upload order).
 """
 import numpy as np
 import pytest
@@ -88,6 +92,7 @@ def test_restart_pour_round_trip(tmp_path):
     touching = {k for k in F1 if np.sum((x[k[0]] - x[k[1]]) ** 2) < (r[k[0]] + r[k[1]]) ** 2}
     assert touching and touching == set(F2)
     assert all(np.array_equal(F2[k], F1[k]) for k in touching)
+    s1.setup()
     s1.step(100)
     s2.step(100)
     a, b = s1.state(), s2.state()
