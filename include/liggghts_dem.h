@@ -328,6 +328,14 @@ int dem_observe_plane(dem_ctx *ctx, int plane);
  * plane is observed), total mass, kinetic energy, at the current state and
  * last force pass. */
 int dem_instance_observables(dem_ctx *ctx, double *out);
+/* dem_instance_observables without any device allocation.  No counterpart
+ * in the reference: for callers that cannot allocate (a calibration loop
+ * polling every instance between steps, code that must not call hipMalloc).
+ * The per-instance sums are staged in a fixed per-GPU device array of 4096
+ * instances (160 KiB), held until the process ends; more instances are
+ * rejected with an error.  Calls that overlap on one GPU (contexts on the
+ * same device used from several host threads) race on that array. */
+int dem_instance_observables_noalloc(dem_ctx *ctx, double *out);
 /* Step with a recorded HIP graph of two rebuild periods when every rebuild
  * falls on a fixed step (neigh_modify check no, mesh walls only if none
  * moves, no bonds, no setforce/aveforce, timing off); other steps run as

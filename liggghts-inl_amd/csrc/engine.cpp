@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -90,6 +91,7 @@ struct dem_ctx {
     double *pwf = nullptr;
     double *d_obs = nullptr;
     int obs_cap = 0;
+    double *obs_scratch = nullptr;     // this device's copy of `scratch` (below), or null
 
     // launch shape of the contact pass (0: defaults), set by dem_autotune
     int tune_lanes = 0, tune_block = 0;
@@ -236,6 +238,33 @@ static int fail(dem_ctx *ctx, int code, const std::string &msg)
     if (ctx) ctx->err = msg;
     else g_last_error = msg;
     return code;
+}
+
+// Scratch of dem_instance_observables_noalloc: the per-instance sums of
+// k_inst_obs (5 doubles per instance) staged in a fixed device array, so the
+// entry point calls no allocator.  Every GPU holds its own copy until the
+// process ends; calls on one GPU share it, and each call writes every value
+// it reads back, so nothing needs clearing.  In host code the name is a
+// host-side stand-in: kernels and copies get the device address that
+// hipGetSymbolAddress returns with the device current, resolved once per
+// device when a context is created.
+constexpr int kObsScratchInst = 4096;
+static __device__ __align__(256) double scratch[kObsScratchInst * 5];
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double *g_scratch_dev[kMaxDevices];
+
+// the device's scratch address (the device is current), null if unresolved
+static double *resolve_scratch(int dev)
+{
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (dev < 0 || dev >= kMaxDevices) return nullptr;
+    if (!g_scratch_dev[dev]) {
+        void *ptr = nullptr;
+        if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) == hipSuccess) g_scratch_dev[dev] = (double *)ptr;
+        else hipGetLastError();   // leave no sticky error for the next call
+    }
+    return g_scratch_dev[dev];
 }
 
 // Host calls that read or write device data with the null stream first wait
@@ -388,6 +417,7 @@ extern "C" dem_ctx *dem_ctx_create(const dem_params *p)
         return nullptr;
     }
     hipMemset(ctx->d_small, 0, sizeof(int) * 8);
+    ctx->obs_scratch = resolve_scratch(p->device);
     ctx->hmat.assign(kMatStride, 0.0);
     Model &M = ctx->M;
     M.dt = p->dt;
@@ -2796,6 +2826,27 @@ extern "C" int dem_instance_observables(dem_ctx *ctx, double *out)
                     ctx->obs_plane >= 0 ? ctx->pwf : nullptr, ctx->d_obs, ctx->stream);
     KCHK();
     HIPCHK(hipMemcpyAsync(out, ctx->d_obs, sizeof(double) * B * 5, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return DEM_OK;
+}
+
+extern "C" int dem_instance_observables_noalloc(dem_ctx *ctx, double *out)
+{
+    if (!ctx) return fail(nullptr, DEM_ERR_ARG, "null context");
+    if (!out || !ctx->list_valid)
+        return fail(ctx, DEM_ERR_ARG, "dem_instance_observables_noalloc: call dem_setup first");
+    const int B = ctx->ninst;
+    if (B > kObsScratchInst)
+        return fail(ctx, DEM_ERR_ARG, "dem_instance_observables_noalloc: " + std::to_string(B)
+                                          + " instances exceed the scratch limit of "
+                                          + std::to_string(kObsScratchInst));
+    if (!ctx->obs_scratch)
+        return fail(ctx, DEM_ERR_HIP, "dem_instance_observables_noalloc: no device address for the scratch array");
+    sync_enter(ctx);
+    launch_inst_obs(ctx->cstart, ctx->G.nkeys_local, B, sx(ctx), sv(ctx),
+                    ctx->obs_plane >= 0 ? ctx->pwf : nullptr, ctx->obs_scratch, ctx->stream);
+    KCHK();
+    HIPCHK(hipMemcpyAsync(out, ctx->obs_scratch, sizeof(double) * B * 5, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return DEM_OK;
 }

@@ -130,6 +130,7 @@ def library():
         "dem_set_instance_materials": (C.c_int, [vp, C.c_int, dp, dp, dp, dp, dp, dp]),
         "dem_observe_plane": (C.c_int, [vp, C.c_int]),
         "dem_instance_observables": (C.c_int, [vp, dp]),
+        "dem_instance_observables_noalloc": (C.c_int, [vp, dp]),
         "dem_use_graph": (C.c_int, [vp, C.c_int]),
         "dem_autotune": (C.c_int, [vp, C.c_int, ip, ip, dp]),
         "dem_adjoint_forward": (C.c_int, [vp, C.c_int, C.c_int]),
@@ -555,6 +556,13 @@ class Simulation:
         """(ninst, 5): count, sum z, observed plane force, mass, kinetic energy."""
         out = np.zeros((self.ninst, 5))
         self._chk(self.L.dem_instance_observables(self.h, _dp(out)))
+        return out
+
+    def observables_noalloc(self):
+        """observables() through the entry point that allocates no device
+        memory (a fixed per-GPU scratch array of 4096 instances)."""
+        out = np.zeros((self.ninst, 5))
+        self._chk(self.L.dem_instance_observables_noalloc(self.h, _dp(out)))
         return out
 
     def use_graph(self, on=True):
