@@ -8,7 +8,9 @@ gravity, dt = 5e-7, skin 0.25 mm, rebuild every 10 steps.  Seeded synthetic
 packing (see liggghts-inl_amd/workloads.py), all state resident in HBM.
 
 A step is one full timestep (integrate, rebuild every 10th step, contact
-pass with walls and gravity).  `value` is the mixed-precision form (FP32
+pass with walls and gravity).  The timed steps run through recorded HIP
+graphs; a second run from the same start, stepped eagerly with HIP events
+around every launch, gives the per-kernel durations the roofline uses.  `value` is the mixed-precision form (FP32
 contact law, FP64 positions, history and accumulation); the plain FP64
 form is timed beside it, and the mixed form's force error against the FP64
 oracle is measured on the full initial state.
@@ -46,55 +48,106 @@ def package():
 
 def contact_bytes_per_particle(K, C, H, wall_contacts_per_particle, fused=True):
     """Algorithmic HBM bytes of one contact pass per particle, FP64 state:
-    own x,v,omega (72) + r,m (16) + type (4) + f,torque out (48) = 140,
-    neighbour indices 4K, history read+write 16 H per contact (pairs and
-    walls); with the integrator fused into the pass, the new x/r, v/m and
-    omega/meta records (96) written to the other state buffers."""
-    return 140.0 + 4.0 * K + 16.0 * H * (C + wall_contacts_per_particle) + (96.0 if fused else 0.0)
+    own x,v,omega (72) + r,m (16) + type (4) = 92, neighbour indices 4K,
+    history read+write 16 H per contact (pairs and walls); then either the
+    force and torque written (48), or, with the integrator fused into a pass
+    that also starts the next step, the new x/r, v/m and omega/meta records
+    (96) written to the other state buffers and no force output (nothing
+    reads it before the next pass recomputes it)."""
+    return 92.0 + 4.0 * K + 16.0 * H * (C + wall_contacts_per_particle) + (96.0 if fused else 48.0)
 
 
-def run_gpu(pkg, spec, atoms, precision, steps, warmup):
+def run_gpu(pkg, spec, atoms, precision, steps, warmup, kernel_steps):
+    """The timed region steps through recorded HIP graphs (rebuilds every 10
+    steps on a fixed cadence); per-kernel durations come from a second run
+    of kernel_steps steps from the same start, stepped eagerly with HIP
+    events around every launch on the context's stream."""
+    s = pkg.Simulation(dict(spec, graph=True), atoms, precision=precision)
+    s.setup()
+    s.step(warmup)
+    s.sync()
+    t0 = time.perf_counter()
+    s.step(steps)
+    s.sync()
+    t1 = time.perf_counter()
+    res = dict(seconds=t1 - t0)
+    res["K"], res["C"], res["E"] = s.list_stats()
+    st = s.state()
+    res["ke"] = float(0.5 * (atoms["rmass"] * (st["v"] ** 2).sum(1)).sum())
+    res["mean_z"] = float(st["x"][:, 2].mean())
+    s.close()
     s = pkg.Simulation(spec, atoms, precision=precision)
     s.setup()
     s.step(warmup)
     s.sync()
     s.timing(True)
     t0 = time.perf_counter()
-    s.step(steps)
+    s.step(kernel_steps)
     s.sync()
-    t1 = time.perf_counter()
-    res = dict(seconds=t1 - t0)
+    res["eager_seconds"] = time.perf_counter() - t0
+    res["kernel_steps"] = kernel_steps
     for k in ("contact", "integrate", "rebuild"):
         ms, cnt = s.timer(k)
         res[k] = (ms, cnt)
-    res["K"], res["C"], res["E"] = s.list_stats()
-    st = s.state()
-    res["ke"] = float(0.5 * (atoms["rmass"] * (st["v"] ** 2).sum(1)).sum())
-    res["mean_z"] = float(st["x"][:, 2].mean())
+    res["K_eager"], res["C_eager"], _ = s.list_stats()
     s.close()
     return res
 
 
-def cpu_baseline(pkg, target_s=12.0):
+CPU_SAMPLE_N = 50000
+
+
+def cpu_worker(start_at, seconds):
+    """One host core's share of the CPU baseline: the oracle steps its own
+    50,000-sphere sample of the column workload from start_at (a shared
+    wall-clock time, so every worker's window overlaps) for about `seconds`;
+    prints its sphere count, steps and elapsed time."""
+    pkg = package()
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from oracle import Oracle
 
-    n = 50000
-    spec, atoms = pkg.workloads.packed_column(n=n, slab_n=12500)
+    spec, atoms = pkg.workloads.packed_column(n=CPU_SAMPLE_N, slab_n=12500)
     o = Oracle(spec, atoms)
     o.setup()
+    o.run(10)
+    while time.time() < start_at:
+        time.sleep(0.01)
     t0 = time.perf_counter()
-    o.run(20)
-    dt = time.perf_counter() - t0
-    steps = max(20, int(target_s / max(dt / 20, 1e-6)))
-    steps = min(steps, 2000)
-    t0 = time.perf_counter()
-    o.run(steps)
-    el = time.perf_counter() - t0
-    return dict(value=n * steps / el, unit="particle-steps/s", cores=1, kind="port",
-                sample="oracle (C restatement, 1 thread) on %d spheres of the same workload "
-                       "(periodic 60x60 mm column, Hertz+history+EPSD, bottom plane), %d steps, %.1f s"
-                % (n, steps, el))
+    steps = 0
+    while time.perf_counter() - t0 < seconds:
+        o.run(10)
+        steps += 10
+    print(json.dumps({"n": CPU_SAMPLE_N, "steps": steps, "seconds": time.perf_counter() - t0}))
+
+
+def cpu_baseline(seconds=12.0):
+    """The oracle (the C restatement of the reference's path) on the host's
+    cores, one process per core, each stepping its own copy of a
+    50,000-sphere sample of the workload over the same wall-clock window:
+    the weak-scaled throughput of the CPU path on this host, like one MPI
+    rank per core without the ghost exchange.  At most 16 workers (this
+    host's share of the machine)."""
+    import subprocess
+
+    ncpu = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    workers = max(1, min(16, ncpu))
+    start_at = time.time() + 15.0
+    procs = [subprocess.Popen([sys.executable, os.path.abspath(__file__), "--cpu-worker", repr(start_at),
+                               repr(seconds)], stdout=subprocess.PIPE, stderr=subprocess.PIPE)
+             for _ in range(workers)]
+    outs = []
+    for p in procs:
+        out, err = p.communicate(timeout=seconds + 120)
+        if p.returncode != 0:
+            raise RuntimeError("cpu baseline worker failed: %s" % err.decode()[-500:])
+        outs.append(json.loads(out.decode().strip().splitlines()[-1]))
+    rate = sum(o["n"] * o["steps"] / o["seconds"] for o in outs)
+    steps = sum(o["steps"] for o in outs)
+    return dict(value=rate, unit="particle-steps/s", cores=workers, kind="port",
+                sample="oracle (C restatement, 1 thread per process) on %d host cores, each stepping its own "
+                       "%d-sphere sample of the same workload (periodic 60x60 mm column, Hertz+history+EPSD, "
+                       "bottom plane) for %.0f s over one shared window; %d steps in all"
+                % (workers, CPU_SAMPLE_N, seconds, steps))
 
 
 def mixed_force_error(pkg, n=50000, steps=300):
@@ -283,10 +336,16 @@ def main():
     ap.add_argument("--adjoint", action="store_true", help="batch: also time the adjoint window (FP64)")
     ap.add_argument("--adjoint-steps", type=int, default=2000,
                     help="batch: steps of the adjoint window (one neighbour list: motion stays within half the skin)")
+    ap.add_argument("--kernel-steps", type=int, default=1000,
+                    help="steps of the eager run that times each kernel with HIP events")
+    ap.add_argument("--cpu-worker", nargs=2, type=float, default=None, help=argparse.SUPPRESS)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MB",
                     help="only run a device-to-device copy of MB MiB, 10 times (for calibrating "
                          "the FETCH_SIZE/WRITE_SIZE counters against known traffic)")
     args = ap.parse_args()
+    if args.cpu_worker:
+        cpu_worker(*args.cpu_worker)
+        return
     if args.copy_probe:
         import torch
 
@@ -318,12 +377,18 @@ def main():
     spec, atoms = pkg.workloads.packed_column(n=args.n)
     print("workload generated in %.1f s" % (time.perf_counter() - t0), file=sys.stderr, flush=True)
 
+    cpu = None
+    if not args.no_cpu:
+        # before any GPU work: the workers are started as child processes
+        cpu = cpu_baseline()
+        print("cpu baseline: %.3e particle-steps/s on %d cores" % (cpu["value"], cpu["cores"]),
+              file=sys.stderr, flush=True)
     results = {}
     for prec in (["mixed", "fp64"] if args.only is None else [args.only]):
-        results[prec] = run_gpu(pkg, spec, atoms, prec, args.steps, args.warmup)
+        results[prec] = run_gpu(pkg, spec, atoms, prec, args.steps, args.warmup, args.kernel_steps)
         r = results[prec]
-        print("%s: %.3f s for %d steps, contact %.3f ms/launch" %
-              (prec, r["seconds"], args.steps, r["contact"][0] / max(r["contact"][1], 1)),
+        print("%s: %.3f s for %d steps (graph), contact %.3f ms/launch over %d eager steps" %
+              (prec, r["seconds"], args.steps, r["contact"][0] / max(r["contact"][1], 1), r["kernel_steps"]),
               file=sys.stderr, flush=True)
 
     main_prec = "mixed" if "mixed" in results else "fp64"
@@ -332,7 +397,8 @@ def main():
     value = n * args.steps / R["seconds"]
     H = 6
     wallc = 0.0
-    bpp = contact_bytes_per_particle(R["K"], R["C"], H, wallc)
+    # the timed contact launches are those of the eager run: its list figures
+    bpp = contact_bytes_per_particle(R["K_eager"], R["C_eager"], H, wallc)
     avg_ms = R["contact"][0] / max(R["contact"][1], 1)
     achieved = bpp * n / (avg_ms * 1e-3) / 1e9
 
@@ -354,8 +420,8 @@ def main():
                         "Hertz+history+EPSD (mu_r 0.1), bottom plane, gravity, dt 5e-7, rebuild every 10",
             "n_particles": n,
             "precision": main_prec,
-            "mean_full_neighbours": R["K"],
-            "mean_contacts": R["C"],
+            "mean_full_neighbours": R["K_eager"],
+            "mean_contacts": R["C_eager"],
         },
         "roofline": {
             "bound": "hbm",
@@ -379,7 +445,9 @@ def main():
             "contact_avg_ms": F["contact"][0] / max(F["contact"][1], 1),
             "speedup_mixed_over_fp64": F["seconds"] / R["seconds"],
         }
-    out["kernels_ms_per_step"] = {k: R[k][0] / args.steps for k in ("contact", "integrate", "rebuild")}
+    out["kernels_ms_per_step"] = {k: R[k][0] / R["kernel_steps"] for k in ("contact", "integrate", "rebuild")}
+    out["kernels_ms_per_step"]["eager_ms_per_step"] = R["eager_seconds"] * 1e3 / R["kernel_steps"]
+    out["kernels_ms_per_step"]["steps"] = R["kernel_steps"]
     if not args.no_check:
         ef, et, npairs = mixed_force_error(pkg)
         out["mixed_force_error"] = {"force": ef, "torque": et, "bar": 1e-5,
@@ -394,8 +462,8 @@ def main():
             "kinetic_energy_rel": abs(R2["ke"] - F["ke"]) / max(abs(F["ke"]), 1e-300),
             "mean_height_rel": abs(R2["mean_z"] - F["mean_z"]) / max(abs(F["mean_z"]), 1e-300),
             "vs": "the FP64 GPU run from the same initial state, at the end of the timed run"}
-    if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(pkg)
+    if cpu is not None:
+        out["cpu_baseline"] = cpu
     print(json.dumps(out))
 
 

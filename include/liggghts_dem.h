@@ -257,7 +257,8 @@ int dem_timing_enable(dem_ctx *ctx, int on);
 /* name: "contact" | "integrate" | "rebuild".  Total ms and launch count
  * since enable. */
 int dem_timing_get(dem_ctx *ctx, const char *name, double *ms, int64_t *launches);
-/* Mean full-list neighbours and contacts per atom of the last force pass. */
+/* Mean full-list neighbours and contacts per atom of the last force pass;
+ * the contacts are counted only in passes with timing on (-1 otherwise). */
 int dem_list_stats(dem_ctx *ctx, double *mean_neigh, double *mean_contacts, int64_t *entries);
 int dem_sync(dem_ctx *ctx);
 int64_t dem_timestep(const dem_ctx *ctx);

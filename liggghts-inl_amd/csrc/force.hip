@@ -176,8 +176,14 @@ __device__ inline void particle_epilogue(const ForceArgs &a, int i, const double
         F[0] = F[1] = F[2] = 0.0;
         T[0] = T[1] = T[2] = 0.0;
     }
-    a.f4[i] = make_double4(F[0], F[1], F[2], 0.0);
-    a.t4[i] = make_double4(T[0], T[1], T[2], 0.0);
+    // a pass that also runs the next step's initial_integrate is never the
+    // last of a dem_step call, and nothing after it reads the force and
+    // torque (the next pass recomputes them): they are written only by the
+    // other passes
+    if (a.fuse != 3) {
+        a.f4[i] = make_double4(F[0], F[1], F[2], 0.0);
+        a.t4[i] = make_double4(T[0], T[1], T[2], 0.0);
+    }
     if (a.ncontact) a.ncontact[i] = nc;
     if (a.pwf) a.pwf[i] = pw;
 }

@@ -52,8 +52,8 @@ def test_batch_equals_solo_bitwise_and_graph():
         assert np.array_equal(gb[k], gg[k]), k
     order = np.argsort(atoms["tag"])
 
-    def cells(maxrad):   # the cell grid (half of cutneighmax) follows the largest radius
-        cut = 0.5 * (2 * maxrad + spec["skin"])
+    def cells(maxrad):   # the cell grid (cells of cutneighmax) follows the largest radius
+        cut = 2 * maxrad + spec["skin"]
         return [int(np.floor((h - l) / (cut * 1.0001))) for l, h in zip(spec["box_lo"], spec["box_hi"])]
 
     for b in range(int(inst.max()) + 1):
