@@ -101,6 +101,8 @@ int FixWallGranMI355X::register_with(dem_ctx *ctx) const
     }
     const int id = dem_add_mesh(ctx, (int)(nodes.size() / 9), nodes.data(), fm->wall_type(), fm->curvature(), vel);
     if (id < 0) return id;
+    const int rc = fm->attach(ctx, id);
+    if (rc) return rc;
   }
   return 0;
 }

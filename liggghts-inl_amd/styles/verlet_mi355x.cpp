@@ -10,8 +10,10 @@
 #include "fix_contact_history.h"
 #include "fix_force_ops_mi355x.h"
 #include "fix_gravity.h"
+#include "fix_massflow_mesh_mi355x.h"
 #include "fix_mesh_surface_mi355x.h"
 #include "fix_move_mesh_mi355x.h"
+#include "fix_move_mi355x.h"
 #include "fix_nve_sphere_mi355x.h"
 #include "fix_wall_gran_mi355x.h"
 #include "force.h"
@@ -42,7 +44,8 @@ bool device_or_passive(Fix *f)
                                 "check/timestep/gran", "print", "ave/time", NULL};
   if (dynamic_cast<FixWallGranMI355X *>(f) || dynamic_cast<FixMeshSurfaceMI355X *>(f)
       || dynamic_cast<FixMoveMeshMI355X *>(f) || dynamic_cast<FixNVESphereMI355X *>(f)
-      || dynamic_cast<FixSetForceMI355X *>(f) || dynamic_cast<FixAveForceMI355X *>(f))
+      || dynamic_cast<FixSetForceMI355X *>(f) || dynamic_cast<FixAveForceMI355X *>(f)
+      || dynamic_cast<FixMoveMI355X *>(f) || dynamic_cast<FixMassflowMeshMI355X *>(f))
     return true;
   for (int k = 0; names[k]; k++)
     if (strcmp(f->style, names[k]) == 0) return true;
@@ -198,8 +201,37 @@ void VerletMI355X::upload_history()
   if (!ti.empty()) check(dem_set_history(ctx_, (int64_t)ti.size(), ti.data(), tj.data(), h.data()));
 }
 
+// members of a fix's group as dem_set_group takes them; -1 for group all
+int VerletMI355X::device_group(Fix *f)
+{
+  if (f->igroup == 0) return -1;
+  const int n = atom->nlocal, bit = group_bit(f->groupbit);
+  std::vector<int> members(n);
+  for (int k = 0; k < n; k++) members[k] = (atom->mask[k] & f->groupbit) != 0;
+  check(dem_set_group(ctx_, bit, members.data()));
+  return bit;
+}
+
+/* The per-atom fixes on the device, after the upload (groups are per-atom
+   bits): fix move (before the integrator, as in scripts that define it
+   first), nve/sphere on its group, the group force fixes in fix order, and
+   the massflow counters. */
 void VerletMI355X::register_force_ops()
 {
+  int nnve = 0;
+  for (int i = 0; i < modify->nfix; i++) {
+    Fix *f = modify->fix[i];
+    if (FixMoveMI355X *mv = dynamic_cast<FixMoveMI355X *>(f)) {
+      const int bit = device_group(f);
+      int id = 0;
+      check(mv->register_with(ctx_, bit < 0 ? group_bit(f->groupbit) : bit, &id));
+    } else if (dynamic_cast<FixNVESphereMI355X *>(f)) {
+      if (++nnve > 1) error->all(FLERR, "verlet/mi355x: one fix nve/sphere on the device");
+      check(dem_set_integrate_group(ctx_, device_group(f)));
+    } else if (FixMassflowMeshMI355X *mf = dynamic_cast<FixMassflowMeshMI355X *>(f)) {
+      check(mf->register_with(ctx_, device_group(f)));
+    }
+  }
   for (int i = 0; i < modify->nfix; i++) {
     Fix *f = modify->fix[i];
     FixSetForceMI355X *sf = dynamic_cast<FixSetForceMI355X *>(f);
@@ -293,7 +325,8 @@ void VerletMI355X::setup()
     Fix *f = modify->fix[i];
     if (dynamic_cast<FixWallGranMI355X *>(f) || dynamic_cast<FixNVESphereMI355X *>(f)
         || dynamic_cast<FixSetForceMI355X *>(f) || dynamic_cast<FixAveForceMI355X *>(f)
-        || dynamic_cast<FixMoveMeshMI355X *>(f) || strcmp(f->style, "gravity") == 0
+        || dynamic_cast<FixMoveMeshMI355X *>(f) || dynamic_cast<FixMoveMI355X *>(f)
+        || dynamic_cast<FixMassflowMeshMI355X *>(f) || strcmp(f->style, "gravity") == 0
         || strcmp(f->style, "freeze") == 0)
       continue;
     if (modify->fmask[i] & END_OF_STEP) f->setup(vflag);

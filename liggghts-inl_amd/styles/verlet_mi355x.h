@@ -59,6 +59,7 @@ class VerletMI355X : public Verlet {
   void upload();
   void upload_history();
   void register_force_ops();
+  int device_group(class Fix *f);
   void download();
   void save_device_history();
   bigint next_host_step(bigint stop) const;

@@ -47,7 +47,10 @@ def test_style_macro(hdr, tmp_path):
         # the class the macro names is defined in the header's other branch
         assert re.search(r"class %s\b" % cls, text)
         probe = tmp_path / "probe.cpp"
-        probe.write_text('#define %s\n#define %s(key,Class) static const char *k_##Class = #key;\n'
+        # one variable per macro line: a class may serve several styles, as
+        # FixMeshSurface serves mesh/surface and mesh/surface/planar
+        probe.write_text('#define %s\n#define CAT2(a,b) a##b\n#define CAT(a,b) CAT2(a,b)\n'
+                         '#define %s(key,Class) static const char *CAT(k_##Class, __LINE__) = #key;\n'
                          '#include "%s"\n' % (MACROS[macro], macro, hdr))
         r = gxx(str(probe), extra=("-Wno-unused-variable",))
         assert r.returncode == 0, r.stderr[-2000:]
