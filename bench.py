@@ -28,11 +28,12 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per particle of one mixed contact pass on this workload, from the
-# FETCH_SIZE and WRITE_SIZE counter passes of rocprofv3 on k_force_q<true,32>
-# (FETCH_SIZE scaled x2.0, WRITE_SIZE x1.0 against a 256 MiB device copy):
+# HBM bytes per particle of one mixed contact pass (integrator fused, no force
+# output) on this workload, from the FETCH_SIZE and WRITE_SIZE counter passes
+# of rocprofv3 on k_force_q<true,32> (FETCH_SIZE scaled x2.0, WRITE_SIZE x1.0
+# against a 256 MiB device copy), averaged over the stepping dispatches:
 # profiles/pmc_traffic_k_force_column1M.md.  Re-measured when the kernel changes.
-MEASURED_TRAFFIC_BPP = 888.2
+MEASURED_TRAFFIC_BPP = 836.2
 
 
 def package():
