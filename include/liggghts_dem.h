@@ -306,6 +306,88 @@ int dem_force_op_vector(dem_ctx *ctx, int id, double *out3);
 int dem_cleanup(dem_ctx *ctx);
 double dem_time(const dem_ctx *ctx);
 
+/* ---- Particle insertion: fix insert/pack with a particledistribution/
+ * discrete of particletemplate/sphere templates, run inside dem_step at the
+ * fix's insertion steps (FixInsert::pre_exchange, fix_insert.cpp:676-888),
+ * before that step's neighbour build.  The placement loop and its random
+ * streams are the reference's; each batch of candidate positions is tested
+ * against the resident particles on the device, so their state never
+ * leaves HBM.  A context may start empty (dem_upload_state with n = 0). ---- */
+/* fix particletemplate/sphere <seed> atom_type T density constant rho
+ * radius constant r (fix_template_sphere.cpp:86-240; constant radius and
+ * density are the styles this fork parses). */
+typedef struct {
+    int atom_type;
+    double radius;
+    double density;
+} dem_template_sphere;
+
+enum { DEM_INSERT_VOLUMEFRACTION = 0, DEM_INSERT_PARTICLES = 1, DEM_INSERT_MASS = 2 };
+enum { DEM_VEL_CONSTANT = 0, DEM_VEL_UNIFORM = 1, DEM_VEL_GAUSSIAN = 2 };
+
+/* fix <id> <group> insert/pack seed S distributiontemplate D ... with its
+ * region block (fix_insert.cpp:96-330, fix_insert_pack.cpp:72-200,
+ * region_block.cpp) and fix D all particledistribution/discrete S2 n t1 w1 ...
+ * (fix_particledistribution_discrete.cpp:68-233).  dem_insert_pack_defaults
+ * fills the keyword defaults (FixInsert::init_defaults, fix_insert.cpp:446-490;
+ * FixInsertPack::init_defaults, fix_insert_pack.cpp:252-266). */
+typedef struct {
+    int seed;                    /* insert/pack seed (prime > 10000)                  */
+    int dist_seed;               /* particledistribution/discrete seed                */
+    int ntemplates;
+    const dem_template_sphere *templates;
+    const double *weights;       /* per template: mass fraction, or number fraction
+                                    with number_based (style .../numberbased)          */
+    int number_based;
+    double region_lo[3], region_hi[3];   /* region block, units box                  */
+    int target;                  /* DEM_INSERT_*: volumefraction_region,
+                                    particles_in_region, mass_in_region              */
+    double target_value;
+    int maxattempt;              /* default 50                                         */
+    int insert_every;            /* 0: once (the default -1 is an error, as in
+                                    FixInsertPack::calc_insertion_properties)          */
+    long long start;             /* first insertion step; default: next step          */
+    int overlapcheck;            /* default 1                                          */
+    int all_in;                  /* default 0                                          */
+    int check_dist_from_subdomain_border;   /* default 1                               */
+    int ntry_mc;                 /* region volume samples, default 100000             */
+    int exact_number;            /* random_distribute exact (1, default) or
+                                    uncorrelated (0)                                   */
+    int vel_style;               /* DEM_VEL_*                                          */
+    double vel[3], vel_fluct[3];
+    double omega[3];             /* omega constant                                     */
+    int group_mask;              /* engine group bits the inserted atoms carry (the
+                                    insert fix's group; ParticleToInsert::insert sets
+                                    mask = 1 | groupbit, particleToInsert.cpp:113)    */
+} dem_insert_pack_spec;
+
+void dem_insert_pack_defaults(dem_insert_pack_spec *spec);
+/* Registers the fix: validates the spec, builds the distribution
+ * (mass-to-number weights, large-to-small template order) and runs
+ * FixInsert::setup's region volume estimate (FixInsertPack::
+ * calc_insertion_properties, fix_insert_pack.cpp:281-322), which advances
+ * the region's random stream as the reference does.  *id names it for
+ * dem_unfix and dem_insert_vector. */
+int dem_add_insert_pack(dem_ctx *ctx, const dem_insert_pack_spec *spec, int *id);
+/* The fix's totals: out[0] particles inserted, out[1] mass inserted, out[2]
+ * particles inserted by the last insertion, out[3] particles it aimed for,
+ * out[4] attempts it made, out[5] the next insertion step (0: none). */
+int dem_insert_vector(dem_ctx *ctx, int id, double *out6);
+/* Appends atoms to a live context (ParticleToInsert::insert +
+ * Atom::tag_extend, particleToInsert.cpp:97-150, atom.cpp:574-595): state as
+ * for dem_upload_state, engine group bits per atom (NULL: none); the new
+ * atoms follow the existing ones in download order.  The next dem_step step
+ * rebuilds the neighbour list (dem_rebuild rebuilds now); every existing
+ * contact history is carried.  Not in a batched run, an adjoint window or
+ * with an observed plane. */
+int dem_append_atoms(dem_ctx *ctx, int m, const double *x, const double *v, const double *omega,
+                     const double *radius, const double *rmass, const int *type, const int *tag,
+                     const int *gmask);
+/* Atom count (atom->nlocal) and, in download order, each atom's tag, type,
+ * radius and mass (NULL pointers skipped). */
+int dem_natoms(const dem_ctx *ctx);
+int dem_download_props(dem_ctx *ctx, int *tag, int *type, double *radius, double *rmass);
+
 /* ---- Batched runs.  No counterpart in the reference, which runs one
  * simulation per process (one LAMMPS instance, lammps.cpp); these let many
  * independent simulations of the same box, walls and contact model share one

@@ -8,15 +8,18 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <unordered_map>
 #include <vector>
 
 #include "../../include/liggghts_dem.h"
+#include "insert_pack.h"
 #include "kernels.h"
 #include "mesh_host.h"
 
@@ -210,6 +213,25 @@ struct dem_ctx {
     } mf;
     bool mesh_grids_valid = false;    // built for the current radii
     long long next_reneighbor = -1;   // FixMesh::next_reneighbor
+
+    // fix insert/pack (insert_pack.cpp): the fixes in the order they were added,
+    // the largest tag (Atom::tag_extend continues from it), and the buffers
+    // of the device overlap test (grid over the resident particles, built
+    // once per insertion) and of the region sums
+    struct Insert {
+        int id;
+        int gmask;                     // group bits of the atoms it creates
+        dem_ins::InsertPack fix;
+    };
+    std::vector<std::unique_ptr<Insert>> inserts;
+    int tag_max = 0;
+    int *ins_cell = nullptr, *ins_start = nullptr, *ins_idx = nullptr, *ins_tmp = nullptr;
+    int ins_ncap = 0, ins_ccap = 0, ins_tcap = 0;
+    double *ins_cand = nullptr, *ins_rad = nullptr, *ins_part = nullptr;
+    uint32_t *ins_hit = nullptr;
+    int ins_kcap = 0;
+    bool ins_grid_ready = false;
+    InsGrid ins_grid{};
 };
 
 static int massflow_step(dem_ctx *ctx);
@@ -457,6 +479,8 @@ static void free_all(dem_ctx *ctx)
     F(ctx->d_mat); F(ctx->d_small); F(ctx->mf.part); F(ctx->mf.total);
     F(ctx->bout); F(ctx->bvalid); F(ctx->brev);
     F(ctx->pwf); F(ctx->d_obs); F(ctx->d_gtot); F(ctx->d_gpart);
+    F(ctx->ins_cell); F(ctx->ins_start); F(ctx->ins_idx); F(ctx->ins_tmp); F(ctx->ins_cand); F(ctx->ins_rad);
+    F(ctx->ins_part); F(ctx->ins_hit);
     for (char *c : ctx->ckpts) F(c);
     F(ctx->tape); F(ctx->adj_rev);
     if (ctx->gexec) hipGraphExecDestroy(ctx->gexec);
@@ -863,6 +887,9 @@ extern "C" int dem_upload_state(dem_ctx *ctx, int n, const double *x, const doub
         ho[i] = i;
         maxrad = std::max(maxrad, radius[i]);
     }
+    int tag_max = 0;
+    for (int i = 0; i < n; i++) tag_max = std::max(tag_max, tag[i]);
+    ctx->tag_max = tag_max;
     if (n > ctx->cap) {
         const size_t c = (size_t)n;
         for (int b = 0; b < 2; b++) {
@@ -1468,6 +1495,7 @@ extern "C" int dem_rebuild(dem_ctx *ctx)
 {
     if (!ctx) return fail(nullptr, DEM_ERR_ARG, "null context");
     sync_enter(ctx);
+    if (ctx->n == 0) return DEM_OK;
     return rebuild(ctx);
 }
 
@@ -1476,6 +1504,7 @@ extern "C" int dem_compute_forces(dem_ctx *ctx, int shearupdate)
     if (!ctx) return fail(nullptr, DEM_ERR_ARG, "null context");
     if (!ctx->list_valid) return fail(ctx, DEM_ERR_ARG, "dem_compute_forces: no neighbour list (call dem_setup)");
     sync_enter(ctx);
+    if (ctx->n == 0) return DEM_OK;
     int rc = force_pass(ctx, shearupdate);
     if (rc) return rc;
     return check_err_flag(ctx);
@@ -1486,6 +1515,11 @@ extern "C" int dem_setup(dem_ctx *ctx)
     if (!ctx) return fail(nullptr, DEM_ERR_ARG, "null context");
     if (!ctx->mat_set) return fail(ctx, DEM_ERR_ARG, "dem_setup: set materials first");
     sync_enter(ctx);
+    if (ctx->n == 0) {   // an empty box (a pour yet to start): nothing to build
+        ctx->list_valid = true;
+        ctx->setup_ts = ctx->ntimestep;
+        return DEM_OK;
+    }
     int rc = setup_mesh_grids(ctx);
     if (rc) return rc;
     rc = rebuild(ctx);
@@ -1496,11 +1530,375 @@ extern "C" int dem_setup(dem_ctx *ctx)
     return check_err_flag(ctx);
 }
 
-// Neighbor::decide + check_distance (neighbor.cpp:1362-1465)
+// ---- particle insertion (fix insert/pack) ----
+
+// the next step at which an insertion fix runs (LLONG_MAX: none)
+static long long next_insert_step(const dem_ctx *ctx)
+{
+    long long s = LLONG_MAX;
+    for (const auto &in : ctx->inserts)
+        if (in->fix.next_step() > 0) s = std::min(s, in->fix.next_step());
+    return s;
+}
+
+static bool insert_due(const dem_ctx *ctx)
+{
+    for (const auto &in : ctx->inserts)
+        if (in->fix.due(ctx->ntimestep)) return true;
+    return false;
+}
+
+// A buffer of `count` elements keeping its first `keep` (a null buffer is
+// allocated).
+template <typename T>
+static int dev_grow(dem_ctx *ctx, T **ptr, size_t keep, size_t count)
+{
+    T *q = nullptr;
+    HIPCHK(hipMalloc((void **)&q, std::max<size_t>(count, 1) * sizeof(T)));
+    if (*ptr) {
+        if (keep) HIPCHK(hipMemcpyAsync(q, *ptr, keep * sizeof(T), hipMemcpyDeviceToDevice, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        HIPCHK(hipFree(*ptr));
+    }
+    *ptr = q;
+    return DEM_OK;
+}
+
+// Every per-particle buffer to capacity c, contents kept.
+static int grow_particles(dem_ctx *ctx, int c)
+{
+    const size_t oc = ctx->cap, nc = c;
+    const size_t wdn = std::max(ctx->P.n * ctx->M.wdnum, 1);
+    for (int b = 0; b < 2; b++) {
+        if (dev_grow(ctx, &ctx->xr[b], oc, nc) || dev_grow(ctx, &ctx->vm[b], oc, nc)
+            || dev_grow(ctx, &ctx->wt[b], oc, nc) || dev_grow(ctx, &ctx->tag[b], oc, nc)
+            || dev_grow(ctx, &ctx->orig[b], oc, nc) || dev_grow(ctx, &ctx->img[b], oc, nc)
+            || dev_grow(ctx, &ctx->whist[b], oc * wdn, nc * wdn)
+            || dev_grow(ctx, &ctx->nstart[b], oc ? oc + 1 : 0, nc + 1) || dev_grow(ctx, &ctx->gmask[b], oc, nc))
+            return DEM_ERR_HIP;
+        if (ctx->xo[b] && dev_grow(ctx, &ctx->xo[b], oc, nc)) return DEM_ERR_HIP;
+        if (ctx->mf.counter[b] && dev_grow(ctx, &ctx->mf.counter[b], oc, nc)) return DEM_ERR_HIP;
+    }
+    if (dev_grow(ctx, &ctx->f4, oc, nc) || dev_grow(ctx, &ctx->t4, oc, nc) || dev_grow(ctx, &ctx->xhold, oc, nc)
+        || dev_grow(ctx, &ctx->wmask, oc, nc) || dev_grow(ctx, &ctx->ncontact, oc, nc)
+        || dev_grow(ctx, &ctx->key, 0, nc) || dev_grow(ctx, &ctx->sorted, 0, nc)
+        || dev_grow(ctx, &ctx->stage, 0, nc * kStageSlots) || dev_grow(ctx, &ctx->xn, oc, nc)
+        || dev_grow(ctx, &ctx->vn, oc, nc) || dev_grow(ctx, &ctx->wn, oc, nc))
+        return DEM_ERR_HIP;
+    ctx->cap = c;
+    return DEM_OK;
+}
+
+// ParticleToInsert::insert for m atoms: appended after the resident ones,
+// which keep their slots, so the previous list and the mesh rows need only
+// empty rows for the new atoms to carry every history at the next rebuild.
+static int append_atoms(dem_ctx *ctx, int m, const double *x, const double *v, const double *omega,
+                        const double *radius, const double *rmass, const int *type, const int *tag, const int *gm)
+{
+    if (m == 0) return DEM_OK;
+    if (m < 0 || !x || !v || !omega || !radius || !rmass || !type || !tag)
+        return fail(ctx, DEM_ERR_ARG, "dem_append_atoms: bad arguments");
+    if (!ctx->mat_set) return fail(ctx, DEM_ERR_ARG, "dem_append_atoms: set materials first");
+    if (ctx->ninst > 1 || ctx->obs_plane >= 0 || ctx->window || ctx->tape_pos >= 0)
+        return fail(ctx, DEM_ERR_ARG,
+                    "dem_append_atoms: not in a batched run, an adjoint window or with an observed plane");
+    const int n0 = ctx->n;
+    if ((int64_t)n0 + m > (int64_t)kJMask) return fail(ctx, DEM_ERR_ARG, "dem_append_atoms: too many atoms");
+    const int n1 = n0 + m;
+    std::vector<double4> hx(m), hv(m), hw(m);
+    std::vector<int> ho(m), hg(m, 0);
+    std::vector<int4> hi(m, make_int4(0, 0, 0, 0));
+    double maxrad = ctx->maxrad;
+    int tag_max = ctx->tag_max;
+    for (int k = 0; k < m; k++) {
+        if (type[k] < 1 || type[k] > ctx->ntypes) return fail(ctx, DEM_ERR_ARG, "dem_append_atoms: atom type out of range");
+        if (!(radius[k] > 0.0) || !(rmass[k] > 0.0))
+            return fail(ctx, DEM_ERR_ARG, "Fix nve/sphere requires extended particles");
+        if (tag[k] <= 0) return fail(ctx, DEM_ERR_ARG, "dem_append_atoms: tags must be > 0");
+        hx[k] = make_double4(x[3 * k], x[3 * k + 1], x[3 * k + 2], radius[k]);
+        hv[k] = make_double4(v[3 * k], v[3 * k + 1], v[3 * k + 2], rmass[k]);
+        hw[k] = make_double4(omega[3 * k], omega[3 * k + 1], omega[3 * k + 2], (double)type[k]);
+        ho[k] = n0 + k;
+        if (gm) hg[k] = gm[k];
+        for (const auto &mv : ctx->moves)
+            if ((hg[k] >> mv.bit) & 1)
+                return fail(ctx, DEM_ERR_ARG, "dem_append_atoms: a new atom in the group of fix move");
+        maxrad = std::max(maxrad, radius[k]);
+        tag_max = std::max(tag_max, tag[k]);
+    }
+    if (n1 > ctx->cap) {
+        const int c = (int)std::min<int64_t>((int64_t)kJMask, std::max<int64_t>(n1, (int64_t)ctx->cap * 3 / 2 + 1024));
+        if (grow_particles(ctx, c)) return DEM_ERR_HIP;
+    }
+    hipStream_t s = ctx->stream;
+    const int c = ctx->cur;
+    const size_t wdn = ctx->P.n * ctx->M.wdnum;
+    HIPCHK(hipMemcpyAsync(sx(ctx) + n0, hx.data(), m * sizeof(double4), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(sv(ctx) + n0, hv.data(), m * sizeof(double4), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(sw(ctx) + n0, hw.data(), m * sizeof(double4), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(ctx->tag[c] + n0, tag, m * sizeof(int), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(ctx->orig[c] + n0, ho.data(), m * sizeof(int), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(ctx->img[c] + n0, hi.data(), m * sizeof(int4), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(ctx->gmask[c] + n0, hg.data(), m * sizeof(int), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(ctx->xhold + n0, hx.data(), m * sizeof(double4), hipMemcpyHostToDevice, s));
+    if (wdn) HIPCHK(hipMemsetAsync(ctx->whist[c] + n0 * wdn, 0, sizeof(double) * m * wdn, s));
+    HIPCHK(hipMemsetAsync(ctx->f4 + n0, 0, m * sizeof(double4), s));
+    HIPCHK(hipMemsetAsync(ctx->t4 + n0, 0, m * sizeof(double4), s));
+    HIPCHK(hipMemsetAsync(ctx->wmask + n0, 0, m, s));
+    // the counter property's default for a new atom: not seen yet
+    if (ctx->mf.mesh >= 0) HIPCHK(hipMemsetAsync(ctx->mf.counter[c] + n0, 0xff, sizeof(int) * m, s));
+    if (ctx->have_list) launch_extend_rows(ctx->nstart[ctx->lcur], n0, m, s);
+    for (auto &mh : ctx->meshes) {
+        if (!mh.have) continue;
+        const int b = mh.cur;
+        if (n1 + 1 > mh.ncap[b]) {
+            if (dev_grow(ctx, &mh.mstart[b], (size_t)n0 + 1, (size_t)n1 + 1)
+                || dev_grow(ctx, &mh.mnp[b], (size_t)n0 + 1, (size_t)n1 + 1))
+                return DEM_ERR_HIP;
+            mh.ncap[b] = n1 + 1;
+        }
+        launch_extend_rows(mh.mstart[b], n0, m, s);
+        launch_fill_int(mh.mnp[b] + n0, 0, m, s);
+    }
+    KCHK();
+    HIPCHK(hipStreamSynchronize(s));
+    ctx->n = n1;
+    ctx->tag_max = tag_max;
+    if (maxrad > ctx->maxrad) ctx->mesh_grids_valid = false;
+    ctx->maxrad = maxrad;
+    ctx->nc_fresh = false;
+    // cell grid for the new largest radius, scan scratch for n1 + 1 rows
+    return setup_grid(ctx);
+}
+
+// Resident particles into the overlap-test grid: cells at least R wide
+static int ins_build_grid(dem_ctx *ctx, double R)
+{
+    InsGrid &g = ctx->ins_grid;
+    int64_t cells = 1;
+    for (int d = 0; d < 3; d++) {
+        const double L = ctx->p.boxhi[d] - ctx->p.boxlo[d];
+        int nc = (int)std::floor(L / (R * 1.0001));
+        g.n[d] = std::max(1, std::min(nc, 1 << 12));
+    }
+    for (;;) {
+        cells = (int64_t)g.n[0] * g.n[1] * g.n[2];
+        if (cells <= (1 << 22)) break;
+        int dm = 0;
+        for (int d = 1; d < 3; d++)
+            if (g.n[d] > g.n[dm]) dm = d;
+        g.n[dm] = (g.n[dm] + 1) / 2;
+    }
+    for (int d = 0; d < 3; d++) {
+        const double L = ctx->p.boxhi[d] - ctx->p.boxlo[d];
+        g.lo[d] = ctx->p.boxlo[d];
+        g.inv[d] = g.n[d] / L;
+        g.prd[d] = L;
+        g.periodic[d] = ctx->p.periodic[d];
+    }
+    const int n = ctx->n;
+    if (n > ctx->ins_ncap) {
+        if (dev_alloc(ctx, &ctx->ins_cell, n) || dev_alloc(ctx, &ctx->ins_idx, n)) return DEM_ERR_HIP;
+        ctx->ins_ncap = n;
+    }
+    if (cells + 1 > ctx->ins_ccap) {
+        if (dev_alloc(ctx, &ctx->ins_start, 2 * (cells + 1))) return DEM_ERR_HIP;
+        ctx->ins_ccap = (int)(cells + 1);
+    }
+    const int need = scan_tmp_size((int)cells + 1);
+    if (need > ctx->ins_tcap) {
+        if (dev_alloc(ctx, &ctx->ins_tmp, need)) return DEM_ERR_HIP;
+        ctx->ins_tcap = need;
+    }
+    hipStream_t s = ctx->stream;
+    int *start = ctx->ins_start, *cursor = ctx->ins_start + (cells + 1);
+    launch_fill_int(start, 0, (int)cells + 1, s);
+    launch_ins_bin(sx(ctx), n, g, ctx->ins_cell, start, s);
+    launch_scan(start, (int)cells + 1, ctx->ins_tmp, s);
+    launch_copy_int(cursor, start, (int)cells + 1, s);
+    launch_ins_place(ctx->ins_cell, n, cursor, ctx->ins_idx, s);
+    KCHK();
+    ctx->ins_grid_ready = true;
+    return DEM_OK;
+}
+
+static dem_ins::Hooks ins_hooks(dem_ctx *ctx)
+{
+    dem_ins::Hooks h;
+    h.region_sums = [ctx](const double *lo, const double *hi, long long *np, double *vol, double *mass) -> int {
+        *np = 0;
+        *vol = *mass = 0.0;
+        if (ctx->n == 0) return DEM_OK;
+        if (!ctx->ins_part && dev_alloc(ctx, &ctx->ins_part, 4 * kRegionBlocks)) return DEM_ERR_HIP;
+        launch_region_sums(sx(ctx), sv(ctx), ctx->n, lo, hi, ctx->ins_part, ctx->stream);
+        KCHK();
+        std::vector<double> part(4 * kRegionBlocks);
+        HIPCHK(hipMemcpyAsync(part.data(), ctx->ins_part, part.size() * sizeof(double), hipMemcpyDeviceToHost,
+                              ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        double c = 0.0;
+        for (int b = 0; b < kRegionBlocks; b++) {
+            c += part[4 * b];
+            *vol += part[4 * b + 1];
+            *mass += part[4 * b + 2];
+        }
+        *np = (long long)c;
+        return DEM_OK;
+    };
+    h.overlap = [ctx](const std::vector<double> &cand, const std::vector<double> &rad,
+                      std::vector<uint32_t> &hit) -> int {
+        const int K = (int)(cand.size() / 3);
+        hit.assign(K, 0u);
+        if (ctx->n == 0 || K == 0) return DEM_OK;
+        double rmax = 0.0;
+        for (double r : rad) rmax = std::max(rmax, r);
+        if (!ctx->ins_grid_ready && ins_build_grid(ctx, rmax + ctx->maxrad)) return DEM_ERR_HIP;
+        if (K > ctx->ins_kcap) {
+            if (dev_alloc(ctx, &ctx->ins_cand, 3 * (size_t)K) || dev_alloc(ctx, &ctx->ins_hit, K)) return DEM_ERR_HIP;
+            ctx->ins_kcap = K;
+        }
+        if (!ctx->ins_rad && dev_alloc(ctx, &ctx->ins_rad, 32)) return DEM_ERR_HIP;
+        hipStream_t s = ctx->stream;
+        HIPCHK(hipMemcpyAsync(ctx->ins_cand, cand.data(), cand.size() * sizeof(double), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(ctx->ins_rad, rad.data(), rad.size() * sizeof(double), hipMemcpyHostToDevice, s));
+        launch_ins_check(ctx->ins_cand, K, ctx->ins_rad, (int)rad.size(), sx(ctx), ctx->ins_start, ctx->ins_idx,
+                         ctx->ins_grid, ctx->ins_hit, s);
+        KCHK();
+        HIPCHK(hipMemcpyAsync(hit.data(), ctx->ins_hit, K * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        return DEM_OK;
+    };
+    return h;
+}
+
+// FixInsert::pre_exchange of every insertion fix due at this step, in the
+// order they were added (each sees the atoms the earlier ones created)
+static int run_inserts(dem_ctx *ctx)
+{
+    for (auto &in : ctx->inserts) {
+        if (!in->fix.due(ctx->ntimestep)) continue;
+        ctx->ins_grid_ready = false;
+        std::vector<dem_ins::Placed> out;
+        std::string err;
+        const int rc = in->fix.insert(ins_hooks(ctx), ctx->ntimestep, out, err);
+        if (rc) return fail(ctx, rc, "fix insert/pack: " + err);
+        const int m = (int)out.size();
+        if (m == 0) continue;
+        std::vector<double> x(3 * m), v(3 * m), w(3 * m), r(m), ms(m);
+        std::vector<int> ty(m), tg(m), gm(m, in->gmask);
+        for (int k = 0; k < m; k++) {
+            for (int d = 0; d < 3; d++) {
+                x[3 * k + d] = out[k].x[d];
+                v[3 * k + d] = out[k].v[d];
+                w[3 * k + d] = out[k].omega[d];
+            }
+            r[k] = out[k].radius;
+            ms[k] = out[k].mass;
+            ty[k] = out[k].type;
+            // Atom::tag_extend (atom.cpp:574-595): after the largest tag
+            tg[k] = ctx->tag_max + 1 + k;
+        }
+        const int a = append_atoms(ctx, m, x.data(), v.data(), w.data(), r.data(), ms.data(), ty.data(), tg.data(),
+                                   gm.data());
+        if (a) return a;
+    }
+    ctx->ins_grid_ready = false;
+    return DEM_OK;
+}
+
+extern "C" void dem_insert_pack_defaults(dem_insert_pack_spec *s)
+{
+    if (!s) return;
+    *s = dem_insert_pack_spec{};
+    s->target = DEM_INSERT_VOLUMEFRACTION;
+    s->maxattempt = 50;
+    s->insert_every = -1;
+    s->start = 0;
+    s->overlapcheck = 1;
+    s->all_in = 0;
+    s->check_dist_from_subdomain_border = 1;
+    s->ntry_mc = 100000;
+    s->exact_number = 1;
+    s->vel_style = DEM_VEL_CONSTANT;
+}
+
+extern "C" int dem_add_insert_pack(dem_ctx *ctx, const dem_insert_pack_spec *spec, int *id)
+{
+    if (!ctx) return fail(nullptr, DEM_ERR_ARG, "null context");
+    if (!spec || !id) return fail(ctx, DEM_ERR_ARG, "dem_add_insert_pack: bad arguments");
+    if (!ctx->mat_set) return fail(ctx, DEM_ERR_ARG, "dem_add_insert_pack: set materials first");
+    if (ctx->ninst > 1) return fail(ctx, DEM_ERR_ARG, "dem_add_insert_pack: not in a batched run");
+    for (int t = 0; t < spec->ntemplates && spec->templates; t++)
+        if (spec->templates[t].atom_type < 1 || spec->templates[t].atom_type > ctx->ntypes)
+            return fail(ctx, DEM_ERR_ARG, "dem_add_insert_pack: template atom type out of range");
+    auto in = std::make_unique<dem_ctx::Insert>();
+    const std::string err = in->fix.init(*spec, ctx->p.boxlo, ctx->p.boxhi, ctx->ntimestep);
+    if (!err.empty()) return fail(ctx, DEM_ERR_ARG, "fix insert/pack: " + err);
+    in->id = ctx->next_op_id++;
+    in->gmask = spec->group_mask;
+    *id = in->id;
+    ctx->inserts.push_back(std::move(in));
+    return DEM_OK;
+}
+
+extern "C" int dem_insert_vector(dem_ctx *ctx, int id, double *out)
+{
+    if (!ctx) return fail(nullptr, DEM_ERR_ARG, "null context");
+    if (!out) return fail(ctx, DEM_ERR_ARG, "dem_insert_vector: null output");
+    for (const auto &in : ctx->inserts)
+        if (in->id == id) {
+            in->fix.vector(out);
+            return DEM_OK;
+        }
+    return fail(ctx, DEM_ERR_ARG, "dem_insert_vector: no such fix");
+}
+
+extern "C" int dem_append_atoms(dem_ctx *ctx, int m, const double *x, const double *v, const double *omega,
+                                const double *radius, const double *rmass, const int *type, const int *tag,
+                                const int *gmask)
+{
+    if (!ctx) return fail(nullptr, DEM_ERR_ARG, "null context");
+    sync_enter(ctx);
+    const int rc = append_atoms(ctx, m, x, v, omega, radius, rmass, type, tag, gmask);
+    // the next step re-neighbours (Neighbor::decide sees the forced step)
+    if (rc == DEM_OK && m > 0) ctx->next_reneighbor = ctx->ntimestep + 1;
+    return rc;
+}
+
+extern "C" int dem_natoms(const dem_ctx *ctx)
+{
+    return ctx ? ctx->n : 0;
+}
+
+extern "C" int dem_download_props(dem_ctx *ctx, int *tag, int *type, double *radius, double *rmass)
+{
+    if (!ctx) return fail(nullptr, DEM_ERR_ARG, "null context");
+    sync_enter(ctx);
+    const int n = ctx->n, c = ctx->cur;
+    if (n == 0) return DEM_OK;
+    std::vector<int> ho(n), ht(n);
+    std::vector<double4> hx(n), hv(n), hw(n);
+    HIPCHK(hipMemcpy(ho.data(), ctx->orig[c], n * sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(ht.data(), ctx->tag[c], n * sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(hx.data(), sx(ctx), n * sizeof(double4), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(hv.data(), sv(ctx), n * sizeof(double4), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(hw.data(), sw(ctx), n * sizeof(double4), hipMemcpyDeviceToHost));
+    for (int i = 0; i < n; i++) {
+        const int o = ho[i];
+        if (tag) tag[o] = ht[i];
+        if (type) type[o] = (int)hw[i].w & 15;
+        if (radius) radius[o] = hx[i].w;
+        if (rmass) rmass[o] = hv[i].w;
+    }
+    return DEM_OK;
+}
+
+// Neighbor::decide + check_distance (neighbor.cpp:1362-1465); a fix that
+// inserts at this step forces the build (fix_insert.cpp:369-370)
 static int decide(dem_ctx *ctx, int *out)
 {
     *out = 0;
-    if (ctx->ntimestep == ctx->next_reneighbor) {
+    if (ctx->ntimestep == ctx->next_reneighbor || insert_due(ctx)) {
         *out = 1;
         return DEM_OK;
     }
@@ -1510,6 +1908,7 @@ static int decide(dem_ctx *ctx, int *out)
         *out = 1;
         return DEM_OK;
     }
+    if (ctx->n == 0) return DEM_OK;   // no atom moved
     const double trig = 0.5 * ctx->p.skin;
     HIPCHK(hipMemsetAsync(ctx->d_small + 2, 0, sizeof(int), ctx->stream));
     launch_check_distance(sx(ctx), ctx->xhold, ctx->n, trig * trig, ctx->d_small + 2, ctx->stream);
@@ -1828,12 +2227,14 @@ extern "C" int dem_step(dem_ctx *ctx, int nsteps)
     for (int st = 0; st < nsteps; st++) {
         // whole graph periods while they fit before the last step, starting
         // right after a rebuild
+        // an insertion step runs eagerly: graph periods end before it
         if (gok && st >= 1 && ctx->ago == 0 && ctx->ntimestep + 1 != ctx->next_reneighbor
-            && st + 2 * ctx->p.neigh_every <= nsteps - 1) {
+            && st + 2 * ctx->p.neigh_every <= nsteps - 1
+            && ctx->ntimestep + 2 * ctx->p.neigh_every < next_insert_step(ctx)) {
             if (!ctx->gexec || ctx->g_key != graph_key(ctx)) {
                 if ((rc = capture_graph(ctx))) return rc;
             }
-            while (st + ctx->g_period <= nsteps - 1) {
+            while (st + ctx->g_period <= nsteps - 1 && ctx->ntimestep + ctx->g_period < next_insert_step(ctx)) {
                 HIPCHK(hipGraphLaunch(ctx->gexec, ctx->stream));
                 ctx->ntimestep += ctx->g_period;
                 st += ctx->g_period;
@@ -1854,11 +2255,14 @@ extern "C" int dem_step(dem_ctx *ctx, int nsteps)
         if (ctx->mf.mesh >= 0 && (rc = massflow_step(ctx))) return rc;
         int nflag = 0;
         if ((rc = decide(ctx, &nflag))) return rc;
-        if (nflag && (rc = ctx->window ? window_mask(ctx) : rebuild(ctx))) return rc;
+        // pre_exchange (fix insert) before the build, as Verlet::run
+        // (verlet.cpp:304-327)
+        if (nflag && !ctx->inserts.empty() && (rc = run_inserts(ctx))) return rc;
+        if (nflag && ctx->n > 0 && (rc = ctx->window ? window_mask(ctx) : rebuild(ctx))) return rc;
         if (!nflag) mesh_trigger(ctx);
         if (ctx->tape_pos >= 0 && (rc = tape_store(ctx, nflag))) return rc;
         const bool last = st == nsteps - 1;
-        if ((rc = force_pass(ctx, 1, last ? 2 : 3))) return rc;
+        if (ctx->n > 0 && (rc = force_pass(ctx, 1, last ? 2 : 3))) return rc;
         if (!last && (rc = move_meshes(ctx))) return rc;
     }
     return check_err_flag(ctx);
@@ -2312,6 +2716,11 @@ extern "C" int dem_unfix(dem_ctx *ctx, int id)
     for (size_t k = 0; k < ctx->moves.size(); k++)
         if (ctx->moves[k].id == id) {
             ctx->moves.erase(ctx->moves.begin() + k);
+            return DEM_OK;
+        }
+    for (size_t k = 0; k < ctx->inserts.size(); k++)
+        if (ctx->inserts[k]->id == id) {
+            ctx->inserts.erase(ctx->inserts.begin() + k);
             return DEM_OK;
         }
     return fail(ctx, DEM_ERR_ARG, "dem_unfix: no such fix");

@@ -1,0 +1,180 @@
+// insert.hip -- device side of particle insertion (fix insert/pack).
+//
+// The reference tests every candidate position of an insertion against the
+// particles near the insertion region through a host-side bin list
+// (FixInsert::load_xnear, fix_insert.cpp:980-1018; RegionNeighborList::
+// hasOverlap, region_neighbor_list_I.h:72-98).  Here the resident particles
+// stay in HBM: they are binned into a uniform grid over the box (cells at
+// least as wide as the largest overlap distance), and one thread per
+// candidate walks its 27 cells, periodic images included, for every
+// distinct template radius at once.
+//
+// k_ins_bin      cell of each resident particle, cell counts
+// k_ins_place    particle indices into their cells (order within a cell
+//                does not matter: the test is an any-of)
+// k_ins_check    rsq <= (r_t + r_j)^2 per candidate and radius, FP64 without
+//                contraction, the reference's expression
+// k_region_sums  count, volume and mass of the particles inside a block
+//                (FixInsertPack::calc_ninsert_this), fixed-order partials
+// k_extend_rows  empty previous-list rows for appended atoms
+#include "kernels.h"
+
+namespace dem {
+
+__device__ inline int ins_cell_dim(double x, double lo, double inv, int n)
+{
+    int c = (int)floor((x - lo) * inv);
+    return c < 0 ? 0 : (c >= n ? n - 1 : c);
+}
+
+__global__ void __launch_bounds__(kBlock) k_ins_bin(const double4 *__restrict__ xr, int n, InsGrid g, int *cell,
+                                                    int *count)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double4 p = xr[i];
+    const int cx = ins_cell_dim(p.x, g.lo[0], g.inv[0], g.n[0]);
+    const int cy = ins_cell_dim(p.y, g.lo[1], g.inv[1], g.n[1]);
+    const int cz = ins_cell_dim(p.z, g.lo[2], g.inv[2], g.n[2]);
+    const int c = (cz * g.n[1] + cy) * g.n[0] + cx;
+    cell[i] = c;
+    atomicAdd(&count[c], 1);
+}
+
+__global__ void __launch_bounds__(kBlock) k_ins_place(const int *__restrict__ cell, int n, int *cursor, int *idx)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    idx[atomicAdd(&cursor[cell[i]], 1)] = i;
+}
+
+__global__ void __launch_bounds__(kBlock) k_ins_check(const double *__restrict__ cand, int ncand,
+                                                      const double *__restrict__ rad, int nrad,
+                                                      const double4 *__restrict__ xr, const int *__restrict__ start,
+                                                      const int *__restrict__ idx, InsGrid g, uint32_t *hit)
+{
+#pragma clang fp contract(off)
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= ncand) return;
+    const double x0 = cand[3 * k], x1 = cand[3 * k + 1], x2 = cand[3 * k + 2];
+    const int c0 = ins_cell_dim(x0, g.lo[0], g.inv[0], g.n[0]);
+    const int c1 = ins_cell_dim(x1, g.lo[1], g.inv[1], g.n[1]);
+    const int c2 = ins_cell_dim(x2, g.lo[2], g.inv[2], g.n[2]);
+    const uint32_t all = nrad >= 32 ? 0xffffffffu : ((1u << nrad) - 1u);
+    uint32_t h = 0u;
+    for (int oz = -1; oz <= 1; oz++) {
+        int cz = c2 + oz, sz = 0;
+        if (cz < 0 || cz >= g.n[2]) {
+            if (!g.periodic[2]) continue;
+            sz = cz < 0 ? -1 : 1;
+            cz -= sz * g.n[2];
+        }
+        for (int oy = -1; oy <= 1; oy++) {
+            int cy = c1 + oy, sy = 0;
+            if (cy < 0 || cy >= g.n[1]) {
+                if (!g.periodic[1]) continue;
+                sy = cy < 0 ? -1 : 1;
+                cy -= sy * g.n[1];
+            }
+            for (int ox = -1; ox <= 1; ox++) {
+                int cx = c0 + ox, sx = 0;
+                if (cx < 0 || cx >= g.n[0]) {
+                    if (!g.periodic[0]) continue;
+                    sx = cx < 0 ? -1 : 1;
+                    cx -= sx * g.n[0];
+                }
+                const int c = (cz * g.n[1] + cy) * g.n[0] + cx;
+                for (int q = start[c]; q < start[c + 1]; q++) {
+                    const double4 p = xr[idx[q]];
+                    // a periodic image as the reference's ghost atom holds
+                    // it: x + pbc * prd
+                    const double px = sx ? p.x + sx * g.prd[0] : p.x;
+                    const double py = sy ? p.y + sy * g.prd[1] : p.y;
+                    const double pz = sz ? p.z + sz * g.prd[2] : p.z;
+                    const double d0 = x0 - px, d1 = x1 - py, d2 = x2 - pz;
+                    const double rsq = d0 * d0 + d1 * d1 + d2 * d2;
+                    for (int t = 0; t < nrad; t++) {
+                        const double radsum = rad[t] + p.w;
+                        if (rsq <= radsum * radsum) h |= 1u << t;
+                    }
+                    if (h == all) {
+                        hit[k] = h;
+                        return;
+                    }
+                }
+            }
+        }
+    }
+    hit[k] = h;
+}
+
+void launch_ins_bin(const double4 *xr, int n, const InsGrid &g, int *cell, int *count, hipStream_t s)
+{
+    if (n == 0) return;
+    k_ins_bin<<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(xr, n, g, cell, count);
+}
+
+void launch_ins_place(const int *cell, int n, int *cursor, int *idx, hipStream_t s)
+{
+    if (n == 0) return;
+    k_ins_place<<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(cell, n, cursor, idx);
+}
+
+void launch_ins_check(const double *cand, int ncand, const double *rad, int nrad, const double4 *xr,
+                      const int *start, const int *idx, const InsGrid &g, uint32_t *hit, hipStream_t s)
+{
+    if (ncand == 0) return;
+    k_ins_check<<<(ncand + kBlock - 1) / kBlock, kBlock, 0, s>>>(cand, ncand, rad, nrad, xr, start, idx, g, hit);
+}
+
+// per block b of kRegionBlocks: part[4b..4b+2] = count, sum of 4/3 pi r^3,
+// sum of mass of its grid-stride share, summed in a fixed tree order
+__global__ void __launch_bounds__(kBlock) k_region_sums(const double4 *__restrict__ xr,
+                                                        const double4 *__restrict__ vm, int n, double lx, double ly,
+                                                        double lz, double hx, double hy, double hz, double *part)
+{
+#pragma clang fp contract(off)
+    __shared__ double s[3][kBlock];
+    double c = 0.0, vol = 0.0, mass = 0.0;
+    const double f4pi3 = 4. * M_PI / 3.;
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        const double4 p = xr[i];
+        if (p.x >= lx && p.x <= hx && p.y >= ly && p.y <= hy && p.z >= lz && p.z <= hz) {
+            c += 1.0;
+            vol += f4pi3 * p.w * p.w * p.w;
+            mass += vm[i].w;
+        }
+    }
+    s[0][threadIdx.x] = c;
+    s[1][threadIdx.x] = vol;
+    s[2][threadIdx.x] = mass;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int q = 0; q < 3; q++) s[q][threadIdx.x] += s[q][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 3; q++) part[4 * blockIdx.x + q] = s[q][0];
+}
+
+void launch_region_sums(const double4 *xr, const double4 *vm, int n, const double *lo, const double *hi,
+                        double *part, hipStream_t s)
+{
+    k_region_sums<<<kRegionBlocks, kBlock, 0, s>>>(xr, vm, n, lo[0], lo[1], lo[2], hi[0], hi[1], hi[2], part);
+}
+
+__global__ void __launch_bounds__(kBlock) k_extend_rows(int *start, int from, int count)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    start[from + 1 + t] = start[from];
+}
+
+void launch_extend_rows(int *start, int from, int count, hipStream_t s)
+{
+    if (count <= 0) return;
+    k_extend_rows<<<(count + kBlock - 1) / kBlock, kBlock, 0, s>>>(start, from, count);
+}
+
+}  // namespace dem

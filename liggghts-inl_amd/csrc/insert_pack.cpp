@@ -1,0 +1,530 @@
+// fix insert/pack: the reference's insertion step on the host, resident
+// particles tested on the device through Hooks.  The arithmetic that decides
+// what is inserted is the reference's operation for operation (built with
+// -ffp-contract=off), so the inserted particles are the reference's for the
+// same script and the same resident particles.
+#include "insert_pack.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <unordered_map>
+
+namespace dem_ins {
+
+namespace {
+constexpr int IA = 16807;
+constexpr int IM = 2147483647;
+constexpr double AM = 1.0 / IM;
+constexpr int IQ = 127773;
+constexpr int IR = 2836;
+constexpr int kSeedOffset = 12;   // FixInsertPack's SEED_OFFSET (fix_insert_pack.cpp:67)
+}  // namespace
+
+double RanPark::uniform()
+{
+    const int k = seed / IQ;
+    seed = IA * (seed - k * IQ) - IR * k;
+    if (seed < 0) seed += IM;
+    return AM * seed;
+}
+
+double RanPark::gaussian()
+{
+    double first;
+    if (!save) {
+        double v1, v2, rsq;
+        do {
+            v1 = 2.0 * uniform() - 1.0;
+            v2 = 2.0 * uniform() - 1.0;
+            rsq = v1 * v1 + v2 * v2;
+        } while (!(rsq < 1.0 && rsq != 0.0));
+        const double fac = std::sqrt(-2.0 * std::log(rsq) / rsq);
+        second = v1 * fac;
+        first = v2 * fac;
+        save = 1;
+    } else {
+        first = second;
+        save = 0;
+    }
+    return first;
+}
+
+bool Block::inside(const double *x) const
+{
+    return x[0] >= lo[0] && x[0] <= hi[0] && x[1] >= lo[1] && x[1] <= hi[1] && x[2] >= lo[2] && x[2] <= hi[2];
+}
+
+bool Block::near_face(const double *x, double cut) const
+{
+    if (!inside(x)) return false;
+    for (int d = 0; d < 3; d++) {
+        if (x[d] - lo[d] < cut) return true;
+        if (hi[d] - x[d] < cut) return true;
+    }
+    return false;
+}
+
+std::string InsertPack::init(const dem_insert_pack_spec &s, const double *boxlo, const double *boxhi,
+                             long long ntimestep)
+{
+    s_ = s;
+    if (s.seed <= 0 || s.dist_seed <= 0) return "illegal seed";
+    if (s.ntemplates < 1 || s.ntemplates > DEM_MAXTYPES * 4 || !s.templates || !s.weights)
+        return "illegal number of templates";
+    if (s.insert_every < 0) return "must define 'insert_every'";
+    if (s.maxattempt < 1) return "maxattempt must be > 0";
+    if (s.ntry_mc < 1000) return "ntry_mc must be > 1000";
+    if (s.vel_style < DEM_VEL_CONSTANT || s.vel_style > DEM_VEL_GAUSSIAN) return "bad vel style";
+    if (s.target < DEM_INSERT_VOLUMEFRACTION || s.target > DEM_INSERT_MASS) return "bad insertion target";
+    if (s.target == DEM_INSERT_VOLUMEFRACTION && !(s.target_value > 0.0 && s.target_value <= 1.0))
+        return "Invalid volumefraction";
+    if (s.target == DEM_INSERT_PARTICLES && !(s.target_value >= 1.0)) return "'ntotal_region' > 0 required";
+    if (s.target == DEM_INSERT_MASS && !(s.target_value > 0.0)) return "'masstotal_region' > 0 required";
+    for (int d = 0; d < 3; d++) {
+        boxlo_[d] = boxlo[d];
+        boxhi_[d] = boxhi[d];
+        region_.lo[d] = s.region_lo[d];
+        region_.hi[d] = s.region_hi[d];
+        if (!(s.region_lo[d] < s.region_hi[d])) return "region block bounds must be lo < hi";
+    }
+    const int T = s.ntemplates;
+    templ_.assign(s.templates, s.templates + T);
+    distweight_.assign(s.weights, s.weights + T);
+    volexp_.resize(T);
+    massexp_.resize(T);
+    for (int i = 0; i < T; i++) {
+        const auto &t = templ_[i];
+        if (!(t.radius > 0.0) || !(t.density > 0.0)) return "template radius and density must be > 0";
+        if (t.atom_type < 1) return "template atom type must be >= 1";
+        if (distweight_[i] < 0.0) return "invalid weight";
+        // FixTemplateSphere: volume_expect = cubic_expectancy * 4 pi / 3,
+        // mass_expect = expectancy(density) * volume_expect
+        // (fix_template_sphere.cpp:243-245; constant pdfs: mu^3 and mu)
+        volexp_[i] = t.radius * t.radius * t.radius * 4. * M_PI / 3.;
+        massexp_[i] = t.density * volexp_[i];
+    }
+    // FixParticledistributionDiscrete (fix_particledistribution_discrete.cpp:126-173)
+    double wsum = 0.0;
+    for (int i = 0; i < T; i++) wsum += distweight_[i];
+    if (!(wsum > 0.0)) return "distribution weights sum to zero";
+    for (int i = 0; i < T; i++) distweight_[i] /= wsum;
+    if (!s.number_based) {
+        for (int i = 0; i < T; i++) distweight_[i] = distweight_[i] / massexp_[i];
+        wsum = 0.0;
+        for (int i = 0; i < T; i++) wsum += distweight_[i];
+        for (int i = 0; i < T; i++) distweight_[i] /= wsum;
+    }
+    volexpect_ = massexpect_ = 0.0;
+    for (int i = 0; i < T; i++) {
+        volexpect_ += volexp_[i] * distweight_[i];
+        massexpect_ += massexp_[i] * distweight_[i];
+    }
+    // templates by insertion volume, descending, by the reference's bubble
+    // sort (fix_particledistribution_discrete.cpp:197-219)
+    distorder_.resize(T);
+    for (int i = 0; i < T; i++) distorder_[i] = i;
+    {
+        bool swapped;
+        int n = T;
+        do {
+            swapped = false;
+            for (int i = 0; i < T - 1; i++)
+                if (volexp_[distorder_[i]] < volexp_[distorder_[i + 1]]) {
+                    std::swap(distorder_[i], distorder_[i + 1]);
+                    swapped = true;
+                }
+            n--;
+        } while (swapped && n > 0);
+    }
+    // the distinct radii: one overlap bit each
+    radii_.clear();
+    radius_bit_.resize(T);
+    for (int i = 0; i < T; i++) {
+        int b = -1;
+        for (size_t k = 0; k < radii_.size(); k++)
+            if (radii_[k] == templ_[i].radius) b = (int)k;
+        if (b < 0) {
+            b = (int)radii_.size();
+            radii_.push_back(templ_[i].radius);
+        }
+        radius_bit_[i] = b;
+    }
+    if (radii_.size() > 32) return "at most 32 distinct template radii";
+
+    random_.reset(s.seed);
+    dist_random_.reset(s.dist_seed);
+    // FixInsert: first insertion at `start`, else the next step; the next
+    // re-neighbouring step is forced to it (fix_insert.cpp:369-370, :458)
+    next_reneighbor_ = s.start > 0 ? s.start : ntimestep + 1;
+    if (next_reneighbor_ < ntimestep + 1) return "'start' step can not be before current step";
+
+    // FixInsertPack::calc_insertion_properties (fix_insert_pack.cpp:281-290):
+    // Region::reset_random(seed + SEED_OFFSET), then Region::volume_mc
+    // (region.cpp:633-700) on the region's stream
+    reg_random_.reset(s.seed + kSeedOffset);
+    double maxrb = 0.0;
+    for (const auto &t : templ_) maxrb = std::max(maxrb, t.radius);
+    long long n_in = 0, n_in_local = 0;
+    const Block &r = region_;
+    for (int i = 0; i < s.ntry_mc; i++) {
+        double pos[3];
+        pos[0] = r.lo[0] + reg_random_.uniform() * (r.hi[0] - r.lo[0]);
+        pos[1] = r.lo[1] + reg_random_.uniform() * (r.hi[1] - r.lo[1]);
+        pos[2] = r.lo[2] + reg_random_.uniform() * (r.hi[2] - r.lo[2]);
+        // Domain::is_in_domain (domain_I.h:51-63): bounds inclusive
+        bool in_dom = true;
+        for (int d = 0; d < 3; d++) in_dom = in_dom && pos[d] >= boxlo_[d] && pos[d] <= boxhi_[d];
+        if (!in_dom) continue;
+        if (r.inside(pos)) {
+            n_in++;
+            // Domain::is_in_subdomain pads a sub-box face on the box face by
+            // 1e-8, so every point of the box is in the single sub-box
+            if (!s.all_in || !r.near_face(pos, maxrb)) n_in_local++;
+        }
+    }
+    if (n_in == 0) return "Unable to calculate region volume";
+    const double vol_bbox = (r.hi[0] - r.lo[0]) * (r.hi[1] - r.lo[1]) * (r.hi[2] - r.lo[2]);
+    region_volume_ = static_cast<double>(n_in) / static_cast<double>(s.ntry_mc * 1) * vol_bbox;
+    if (!(region_volume_ > 0.0)) return "Fix insert: Region volume calculation with MC failed";
+    double vol_local = static_cast<double>(n_in_local) / static_cast<double>(s.ntry_mc) * vol_bbox;
+    const double vol_local_all = vol_local;
+    if (!(vol_local_all > 0.0)) return "Unable to calculate region volume (particles too large for all_in yes?)";
+    vol_local *= (region_volume_ / vol_local_all);
+    region_volume_local_ = vol_local;
+    // the sub-domain (the box) must hold a point of the region
+    for (int d = 0; d < 3; d++)
+        if (!(std::max(r.lo[d], boxlo_[d]) < std::min(r.hi[d], boxhi_[d])))
+            return "Impossible to generate random points on wrong sub-domain";
+    if (s.all_in)
+        for (int d = 0; d < 3; d++)
+            if (r.hi[d] - r.lo[d] < 2. * maxrb)
+                return "Impossible to generate random points within region - region too small "
+                       "(smaller than twice the particle cutoff)";
+    return "";
+}
+
+int InsertPack::max_type() const
+{
+    int m = 0;
+    for (const auto &t : templ_) m = std::max(m, t.atom_type);
+    return m;
+}
+
+double InsertPack::max_radius() const
+{
+    double m = 0.0;
+    for (const auto &t : templ_) m = std::max(m, t.radius);
+    return m;
+}
+
+void InsertPack::vector(double *out) const
+{
+    out[0] = (double)ninserted_;
+    out[1] = massinserted_;
+    out[2] = last_inserted_;
+    out[3] = last_aimed_;
+    out[4] = last_tries_;
+    out[5] = (double)std::max<long long>(next_reneighbor_, 0);
+}
+
+// FixInsertPack::calc_ninsert_this (fix_insert_pack.cpp:333-432)
+int InsertPack::calc_ninsert_this(const Hooks &h, std::string &err)
+{
+    long long np_region = 0;
+    double vol_region = 0.0, mass_region = 0.0;
+    if (h.region_sums(region_.lo, region_.hi, &np_region, &vol_region, &mass_region)) {
+        err = "region sums failed";
+        return -1;
+    }
+    int ninsert_this = 0;
+    if (s_.target == DEM_INSERT_VOLUMEFRACTION) {
+        const double f = s_.target_value;
+        ninsert_this = static_cast<int>((f * region_volume_ - vol_region) / volexpect_ + random_.uniform());
+        insertion_ratio_ = vol_region / (f * region_volume_);
+    } else if (s_.target == DEM_INSERT_PARTICLES) {
+        const int ntotal = (int)s_.target_value;
+        ninsert_this = ntotal - (int)np_region;
+        insertion_ratio_ = static_cast<double>(np_region) / static_cast<double>(ntotal);
+    } else {
+        const double mt = s_.target_value;
+        ninsert_this = static_cast<int>((mt - mass_region) / massexpect_ + random_.uniform());
+        insertion_ratio_ = mass_region / mt;
+    }
+    if (ninsert_this < -200000) {
+        err = "overflow in particle number calculation: inserting too many particles in one step";
+        return -1;
+    }
+    if (ninsert_this < 0) ninsert_this = 0;
+    if (insertion_ratio_ < 0.) insertion_ratio_ = 0.;
+    if (insertion_ratio_ > 1.) insertion_ratio_ = 1.;
+    return ninsert_this;
+}
+
+// FixParticledistributionDiscrete::randomize_list
+// (fix_particledistribution_discrete.cpp:388-470): how many of each
+// template, then the pti list large to small
+int InsertPack::randomize_list(int ninsert)
+{
+    const int T = s_.ntemplates;
+    std::vector<int> parttogen(T, 0);
+    if (!s_.exact_number) {
+        for (int i = 0; i < T; i++)
+            parttogen[i] = static_cast<int>(static_cast<double>(ninsert) * distweight_[i] + dist_random_.uniform());
+    } else {
+        int trunc = 0;
+        std::vector<double> remainder(T);
+        for (int i = 0; i < T; i++) {
+            parttogen[i] = static_cast<int>(static_cast<double>(ninsert) * distweight_[i]);
+            trunc += parttogen[i];
+            remainder[i] = static_cast<double>(ninsert) * distweight_[i] - static_cast<double>(parttogen[i]);
+        }
+        const int gap = ninsert - trunc;
+        for (int i = 0; i < gap; i++) {
+            const double r = dist_random_.uniform() * static_cast<double>(gap);
+            int j = 0;
+            double rsum = remainder[0];
+            while (rsum < r && j < T - 1) {
+                j++;
+                rsum += remainder[j];
+            }
+            parttogen[j]++;
+        }
+    }
+    pti_.clear();
+    for (int i = 0; i < T; i++) {
+        const int c = distorder_[i];
+        const auto &t = templ_[c];
+        Pti p;
+        p.templ = c;
+        p.type = t.atom_type;
+        p.radius = t.radius;
+        p.density = t.density;
+        // FixTemplateSphere::randomize_ptilist (fix_template_sphere.cpp:352-354)
+        p.mass = p.density * (t.radius * t.radius * t.radius * 4. * M_PI / 3.);
+        for (int k = 0; k < parttogen[c]; k++) pti_.push_back(p);
+    }
+    return (int)pti_.size();
+}
+
+// Region::generate_random / generate_random_shrinkby_cut (region.cpp:530-571)
+// on the sub-domain (the box) of a single process
+void InsertPack::gen_pos(double *pos, double rbound)
+{
+    double lo[3], hi[3], diff[3];
+    for (int d = 0; d < 3; d++) {
+        lo[d] = std::max(region_.lo[d], boxlo_[d]);
+        hi[d] = std::min(region_.hi[d], boxhi_[d]);
+        diff[d] = hi[d] - lo[d];
+    }
+    do {
+        pos[0] = lo[0] + reg_random_.uniform() * diff[0];
+        pos[1] = lo[1] + reg_random_.uniform() * diff[1];
+        pos[2] = lo[2] + reg_random_.uniform() * diff[2];
+    } while (!region_.inside(pos) || (s_.all_in && region_.near_face(pos, rbound)));
+}
+
+// FixInsert::generate_random_velocity (fix_insert.cpp:1025-1040)
+void InsertPack::gen_velocity(double *v)
+{
+    for (int d = 0; d < 3; d++) v[d] = s_.vel[d];
+    if (s_.vel_style == DEM_VEL_UNIFORM) {
+        v[0] = s_.vel[0] + s_.vel_fluct[0] * 2.0 * (random_.uniform() - 0.50);
+        v[1] = s_.vel[1] + s_.vel_fluct[1] * 2.0 * (random_.uniform() - 0.50);
+        v[2] = s_.vel[2] + s_.vel_fluct[2] * 2.0 * (random_.uniform() - 0.50);
+    } else if (s_.vel_style == DEM_VEL_GAUSSIAN) {
+        v[0] = s_.vel[0] + s_.vel_fluct[0] * random_.gaussian();
+        v[1] = s_.vel[1] + s_.vel_fluct[1] * random_.gaussian();
+        v[2] = s_.vel[2] + s_.vel_fluct[2] * random_.gaussian();
+    }
+}
+
+// Domain::check_dist_subbox_borders (domain_I.h:127-149), sub-box = box
+bool InsertPack::border_ok(const double *pos, double rbound) const
+{
+    for (int d = 0; d < 3; d++) {
+        const double dlo = std::fabs(boxlo_[d] - pos[d]);
+        const double dhi = std::fabs(boxhi_[d] - pos[d]);
+        if (dlo < rbound || dhi < rbound) return false;
+    }
+    return true;
+}
+
+namespace {
+// The particles placed in this insertion, binned for the overlap test
+// among themselves (the reference adds each to its RegionNeighborList).
+struct NewGrid {
+    double h = 1.0;
+    std::unordered_map<int64_t, std::vector<int>> cells;
+    const std::vector<Placed> *pts = nullptr;
+    static int64_t key(int64_t a, int64_t b, int64_t c) { return (a * 2097152 + b) * 2097152 + c; }
+    int64_t ci(double v) const { return (int64_t)std::floor(v / h) + 1048576; }
+    void add(int idx)
+    {
+        const double *x = (*pts)[idx].x;
+        cells[key(ci(x[0]), ci(x[1]), ci(x[2]))].push_back(idx);
+    }
+    bool overlaps(const double *x, double radius) const
+    {
+        const int64_t c0 = ci(x[0]), c1 = ci(x[1]), c2 = ci(x[2]);
+        for (int64_t a = c0 - 1; a <= c0 + 1; a++)
+            for (int64_t b = c1 - 1; b <= c1 + 1; b++)
+                for (int64_t c = c2 - 1; c <= c2 + 1; c++) {
+                    auto it = cells.find(key(a, b, c));
+                    if (it == cells.end()) continue;
+                    for (int j : it->second) {
+                        const Placed &p = (*pts)[j];
+                        double del[3];
+                        del[0] = x[0] - p.x[0];
+                        del[1] = x[1] - p.x[1];
+                        del[2] = x[2] - p.x[2];
+                        const double rsq = del[0] * del[0] + del[1] * del[1] + del[2] * del[2];
+                        const double radsum = radius + p.radius;
+                        if (rsq <= radsum * radsum) return true;
+                    }
+                }
+        return false;
+    }
+};
+}  // namespace
+
+// FixInsertPack::x_v_omega (fix_insert_pack.cpp:498-605).  With the overlap
+// check, attempts are drawn ahead in batches from copies of the streams and
+// tested on the device together; the serial walk over them then takes the
+// reference's decisions in order, and the streams are set to the state after
+// the last attempt it used.  A batch is drawn for one bounding radius: after
+// an insertion whose next particle has another radius the rest of the batch
+// is dropped and drawn again.
+int InsertPack::x_v_omega(const Hooks &h, int ninsert, std::vector<Placed> &out, std::string &err)
+{
+    out.clear();
+    last_tries_ = 0;
+    if (!s_.overlapcheck) {
+        for (int k = 0; k < ninsert; k++) {
+            const Pti &p = pti_[k];
+            Placed q;
+            gen_pos(q.x, p.radius);
+            gen_velocity(q.v);
+            for (int d = 0; d < 3; d++) q.omega[d] = s_.omega[d];
+            q.radius = p.radius;
+            q.mass = p.mass;
+            q.type = p.type;
+            out.push_back(q);
+        }
+        return DEM_OK;
+    }
+    NewGrid ng;
+    ng.h = 2.0 * max_radius();
+    ng.pts = &out;
+    int ntry = 0;
+    const int maxtry = maxtry_;
+    std::vector<Attempt> batch;
+    std::vector<double> cand;
+    std::vector<uint32_t> hit;
+    while (ntry < maxtry && (int)out.size() < ninsert) {
+        const double rb = pti_[out.size()].radius;
+        // draw a batch of attempts from copies of the streams
+        const int left = ninsert - (int)out.size();
+        const int want = (int)std::min<long long>(std::max(1024, 4 * left), 1 << 20);
+        batch.clear();
+        cand.clear();
+        RanPark reg = reg_random_, fix = random_;
+        int nt = ntry;
+        while ((int)batch.size() < want && nt < maxtry) {
+            Attempt a;
+            std::swap(reg_random_, reg);
+            std::swap(random_, fix);
+            do {
+                gen_pos(a.x, rb);
+                nt++;
+            } while (s_.check_dist_from_subdomain_border && nt < maxtry && !border_ok(a.x, rb));
+            a.last = nt == maxtry;
+            if (!a.last) gen_velocity(a.v);
+            std::swap(reg_random_, reg);
+            std::swap(random_, fix);
+            a.reg = reg;
+            a.fix = fix;
+            a.ntry = nt;
+            batch.push_back(a);
+            if (a.last) break;
+            cand.insert(cand.end(), a.x, a.x + 3);
+        }
+        hit.assign(cand.size() / 3, 0u);
+        if (!cand.empty() && h.overlap(cand, radii_, hit)) {
+            err = "device overlap test failed";
+            return DEM_ERR_HIP;
+        }
+        // the serial walk
+        size_t k = 0;
+        bool redraw = false;
+        for (; k < batch.size(); k++) {
+            const Attempt &a = batch[k];
+            if (a.last) break;
+            const Pti &p = pti_[out.size()];
+            const bool taken = (hit[k] >> radius_bit_[p.templ]) & 1u;
+            if (taken || ng.overlaps(a.x, p.radius)) continue;
+            Placed q;
+            for (int d = 0; d < 3; d++) {
+                q.x[d] = a.x[d];
+                q.v[d] = a.v[d];
+                q.omega[d] = s_.omega[d];
+            }
+            q.radius = p.radius;
+            q.mass = p.mass;
+            q.type = p.type;
+            out.push_back(q);
+            ng.add((int)out.size() - 1);
+            if ((int)out.size() == ninsert || pti_[out.size()].radius != rb) {
+                redraw = true;
+                break;
+            }
+        }
+        const Attempt &used = batch[std::min(k, batch.size() - 1)];
+        reg_random_ = used.reg;
+        random_ = used.fix;
+        ntry = used.ntry;
+        if (!redraw && k < batch.size() && batch[k].last) break;   // maxtry reached
+    }
+    last_tries_ = ntry;
+    return DEM_OK;
+}
+
+// FixInsert::pre_exchange (fix_insert.cpp:676-888) on one process
+int InsertPack::insert(const Hooks &h, long long step, std::vector<Placed> &out, std::string &err)
+{
+    out.clear();
+    if (next_reneighbor_ != step || most_recent_ == step) return DEM_OK;
+    most_recent_ = step;
+    int ninsert_this = calc_ninsert_this(h, err);
+    if (ninsert_this < 0) return DEM_ERR_ARG;
+    // distribute_ninsert_this (fix_insert.cpp:893-960): a single process
+    // takes all; the uncorrelated form draws one number
+    if (!s_.exact_number) {
+        const double fraction_local = region_volume_local_ / region_volume_;
+        ninsert_this = static_cast<int>(fraction_local * static_cast<double>(ninsert_this) + random_.uniform());
+    }
+    ninsert_this = randomize_list(ninsert_this);
+    last_aimed_ = ninsert_this;
+    last_inserted_ = 0;
+    last_tries_ = 0;
+    if (ninsert_this == 0) {
+        if (s_.insert_every) next_reneighbor_ += s_.insert_every;
+        else next_reneighbor_ = -1;
+        return DEM_OK;
+    }
+    // FixInsertPack::calc_maxtry (fix_insert_pack.cpp:485-490)
+    if (insertion_ratio_ >= 1.) maxtry_ = ninsert_this * s_.maxattempt;
+    else maxtry_ = static_cast<int>(static_cast<double>(ninsert_this * s_.maxattempt) / (1. - insertion_ratio_));
+    const int rc = x_v_omega(h, ninsert_this, out, err);
+    if (rc) return rc;
+    last_inserted_ = (int)out.size();
+    ninserted_ += (long long)out.size();
+    double mass_this = 0.0;
+    for (size_t k = 0; k < out.size(); k++) mass_this += pti_[k].mass;
+    massinserted_ += mass_this;
+    if (s_.insert_every) next_reneighbor_ += s_.insert_every;
+    else next_reneighbor_ = 0;
+    return DEM_OK;
+}
+
+}  // namespace dem_ins

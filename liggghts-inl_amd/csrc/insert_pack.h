@@ -1,0 +1,123 @@
+// fix insert/pack on the device's particles: the reference's placement
+// loop (FixInsertPack::x_v_omega, fix_insert_pack.cpp:498-605) and its
+// random streams on the host, the overlap test of each batch of candidate
+// positions against the resident particles on the device (hooks supplied by
+// the engine).
+#pragma once
+
+#include <cstdint>
+#include <functional>
+#include <string>
+#include <vector>
+
+#include "../../include/liggghts_dem.h"
+
+namespace dem_ins {
+
+// RanPark (random_park.cpp:44-103): Park and Miller's minimal standard
+// generator, seed <- 16807 seed mod (2^31 - 1) by Schrage's factorisation;
+// gaussian() is the polar Box-Muller method keeping the second value.
+struct RanPark {
+    int seed = 1;
+    int save = 0;
+    double second = 0.0;
+    double uniform();
+    double gaussian();
+    void reset(int s)
+    {
+        seed = s;
+        save = 0;
+    }
+};
+
+// A block region's bounds (region_block.cpp), units box.
+struct Block {
+    double lo[3], hi[3];
+    // RegBlock::inside (region_block.cpp:272-277): bounds inclusive
+    bool inside(const double *x) const;
+    // RegBlock::surface_interior (region_block.cpp:286-345) > 0: inside and
+    // closer than cut to a face
+    bool near_face(const double *x, double cut) const;
+};
+
+// One particle of the pti list (ParticleToInsert, single sphere).
+struct Pti {
+    int templ;        // index in the spec's template order
+    int type;
+    double radius;
+    double density;
+    double mass;      // density * r^3 * 4 pi / 3 (fix_template_sphere.cpp:352-354)
+};
+
+struct Placed {
+    double x[3], v[3], omega[3];
+    double radius, mass;
+    int type;
+};
+
+// What the placement needs from the resident particles.
+struct Hooks {
+    // particles with position inside [lo, hi] (inclusive): count, sum of
+    // 4/3 pi r^3, sum of mass (FixInsertPack::calc_ninsert_this)
+    std::function<int(const double *lo, const double *hi, long long *np, double *vol, double *mass)> region_sums;
+    // bit t of hit[k]: a sphere of radius rad[t] at cand[3k..3k+2] overlaps
+    // a resident particle, rsq <= (rad[t] + r_j)^2 (RegionNeighborList::
+    // hasOverlap, region_neighbor_list_I.h:72-98), periodic images included
+    std::function<int(const std::vector<double> &cand, const std::vector<double> &rad,
+                      std::vector<uint32_t> &hit)>
+        overlap;
+};
+
+class InsertPack {
+  public:
+    // Checks the spec, builds the distribution and runs the region volume
+    // estimate; returns an error text, empty on success.
+    std::string init(const dem_insert_pack_spec &s, const double *boxlo, const double *boxhi, long long ntimestep);
+    long long next_step() const { return next_reneighbor_; }
+    bool due(long long step) const { return next_reneighbor_ > 0 && step == next_reneighbor_; }
+    // One FixInsert::pre_exchange: the particles to create, in creation
+    // order.  Returns DEM_OK or an error code with err set.
+    int insert(const Hooks &h, long long step, std::vector<Placed> &out, std::string &err);
+    void vector(double *out6) const;
+    int max_type() const;
+    double max_radius() const;
+
+  private:
+    struct Attempt {
+        double x[3], v[3];
+        RanPark reg, fix;    // the streams after this attempt
+        int ntry;            // tries counted after it
+        bool last;           // reached maxtry: no velocity, no test
+    };
+    int calc_ninsert_this(const Hooks &h, std::string &err);
+    int randomize_list(int ntotal);
+    void gen_pos(double *pos, double rbound);
+    void gen_velocity(double *v);
+    bool border_ok(const double *pos, double rbound) const;
+    int x_v_omega(const Hooks &h, int ninsert, std::vector<Placed> &out, std::string &err);
+
+    dem_insert_pack_spec s_{};
+    std::vector<dem_template_sphere> templ_;
+    std::vector<double> distweight_, volexp_, massexp_;
+    std::vector<int> distorder_;
+    std::vector<double> radii_;        // distinct template radii, the overlap bit order
+    std::vector<int> radius_bit_;      // per template: its bit
+    double volexpect_ = 0.0, massexpect_ = 0.0;
+    double boxlo_[3] = {0, 0, 0}, boxhi_[3] = {0, 0, 0};
+    Block region_{};
+    RanPark random_;                   // the fix's stream (seed)
+    RanPark reg_random_;               // the region's stream (seed + 12)
+    RanPark dist_random_;              // the distribution's stream
+    double region_volume_ = 0.0, region_volume_local_ = 0.0;
+    double insertion_ratio_ = 0.0;
+    int maxtry_ = 0;
+    long long next_reneighbor_ = 0;
+    long long most_recent_ = -1;
+    std::vector<Pti> pti_;
+    // totals
+    long long ninserted_ = 0;
+    double massinserted_ = 0.0;
+    int last_inserted_ = 0, last_aimed_ = 0, last_tries_ = 0;
+};
+
+}  // namespace dem_ins

@@ -336,4 +336,23 @@ int scan_tmp_size(int n);
 void launch_check_distance(const double4 *xr, const double4 *xhold, int n, double deltasq, int *flag,
                            hipStream_t s);
 
+// Particle insertion (insert.hip): a uniform grid over the box, cells at
+// least as wide as the largest overlap distance; periodic dimensions span
+// the box exactly (n cells of prd / n).
+struct InsGrid {
+    double lo[3], inv[3], prd[3];
+    int n[3], periodic[3];
+};
+void launch_ins_bin(const double4 *xr, int n, const InsGrid &g, int *cell, int *count, hipStream_t s);
+void launch_ins_place(const int *cell, int n, int *cursor, int *idx, hipStream_t s);
+// hit[k] bit t: candidate k with radius rad[t] overlaps a binned particle
+void launch_ins_check(const double *cand, int ncand, const double *rad, int nrad, const double4 *xr,
+                      const int *start, const int *idx, const InsGrid &g, uint32_t *hit, hipStream_t s);
+constexpr int kRegionBlocks = 256;
+// part[4b..4b+2]: count, sum 4/3 pi r^3, sum mass of particles in [lo, hi]
+void launch_region_sums(const double4 *xr, const double4 *vm, int n, const double *lo, const double *hi,
+                        double *part, hipStream_t s);
+// start[from + 1 .. from + count] = start[from]
+void launch_extend_rows(int *start, int from, int count, hipStream_t s);
+
 }  // namespace dem

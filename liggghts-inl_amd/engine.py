@@ -60,6 +60,98 @@ class BondSettings(C.Structure):
     ]
 
 
+class TemplateSphere(C.Structure):
+    _fields_ = [("atom_type", C.c_int), ("radius", C.c_double), ("density", C.c_double)]
+
+
+class InsertPackSpec(C.Structure):
+    _fields_ = [
+        ("seed", C.c_int),
+        ("dist_seed", C.c_int),
+        ("ntemplates", C.c_int),
+        ("templates", C.POINTER(TemplateSphere)),
+        ("weights", C.POINTER(C.c_double)),
+        ("number_based", C.c_int),
+        ("region_lo", C.c_double * 3),
+        ("region_hi", C.c_double * 3),
+        ("target", C.c_int),
+        ("target_value", C.c_double),
+        ("maxattempt", C.c_int),
+        ("insert_every", C.c_int),
+        ("start", C.c_longlong),
+        ("overlapcheck", C.c_int),
+        ("all_in", C.c_int),
+        ("check_dist_from_subdomain_border", C.c_int),
+        ("ntry_mc", C.c_int),
+        ("exact_number", C.c_int),
+        ("vel_style", C.c_int),
+        ("vel", C.c_double * 3),
+        ("vel_fluct", C.c_double * 3),
+        ("omega", C.c_double * 3),
+        ("group_mask", C.c_int),
+    ]
+
+
+INSERT_TARGET = {"volumefraction_region": 0, "particles_in_region": 1, "mass_in_region": 2}
+VEL_STYLE = {"constant": 0, "uniform": 1, "gaussian": 2}
+
+
+def insert_pack_spec(d):
+    """dem_insert_pack_spec from a dict of the script's keywords:
+
+    seed, dist_seed, templates [{type, radius, density}], weights,
+    number_based, region (lo, hi), one of volumefraction_region /
+    particles_in_region / mass_in_region, maxattempt, insert_every ("once"
+    or steps), start, overlapcheck, all_in, check_dist_from_subdomain_border,
+    ntry_mc, random_distribute ("exact" / "uncorrelated"), vel ("constant",
+    v) / ("uniform" | "gaussian", v, fluct), omega, group_mask.
+
+    Returns (spec, keep): keep holds the arrays the spec points to."""
+    L = library()
+    s = InsertPackSpec()
+    L.dem_insert_pack_defaults(C.byref(s))
+    tl = d["templates"]
+    tarr = (TemplateSphere * len(tl))()
+    for k, t in enumerate(tl):
+        tarr[k].atom_type = int(t["type"])
+        tarr[k].radius = float(t["radius"])
+        tarr[k].density = float(t["density"])
+    w = np.ascontiguousarray(d.get("weights", [1.0] * len(tl)), dtype=np.float64)
+    s.seed = int(d["seed"])
+    s.dist_seed = int(d["dist_seed"])
+    s.ntemplates = len(tl)
+    s.templates = tarr
+    s.weights = w.ctypes.data_as(C.POINTER(C.c_double))
+    s.number_based = int(d.get("number_based", False))
+    lo, hi = d["region"]
+    for k in range(3):
+        s.region_lo[k] = float(lo[k])
+        s.region_hi[k] = float(hi[k])
+    targets = [key for key in INSERT_TARGET if key in d]
+    if len(targets) != 1:
+        raise DemError("must define exactly one keyword out of 'volumefraction_region', 'particles_in_region', "
+                       "and 'mass_in_region'")
+    s.target = INSERT_TARGET[targets[0]]
+    s.target_value = float(d[targets[0]])
+    s.maxattempt = int(d.get("maxattempt", 50))
+    ev = d.get("insert_every", -1)
+    s.insert_every = 0 if ev == "once" else int(ev)
+    s.start = int(d.get("start", 0))
+    s.overlapcheck = int(d.get("overlapcheck", True))
+    s.all_in = int(d.get("all_in", False))
+    s.check_dist_from_subdomain_border = int(d.get("check_dist_from_subdomain_border", True))
+    s.ntry_mc = int(d.get("ntry_mc", 100000))
+    s.exact_number = int(d.get("random_distribute", "exact") == "exact")
+    vel = d.get("vel", ("constant", (0.0, 0.0, 0.0)))
+    s.vel_style = VEL_STYLE[vel[0]]
+    for k in range(3):
+        s.vel[k] = float(vel[1][k])
+        s.vel_fluct[k] = float(vel[2][k]) if len(vel) > 2 else 0.0
+        s.omega[k] = float(d.get("omega", (0.0, 0.0, 0.0))[k])
+    s.group_mask = int(d.get("group_mask", 0))
+    return s, (tarr, w)
+
+
 COHESION = {"off": 0, "bond": 1, "bond/nonlinear": 2}
 # bond tables in the order of DEM_NBOND (include/liggghts_dem.h); names of the
 # script properties, the bond/nonlinear "...Bondnonlinear" / "stiffnessPerUnitArea..."
@@ -158,6 +250,12 @@ def library():
         "dem_force_op_vector": (C.c_int, [vp, C.c_int, dp]),
         "dem_cleanup": (C.c_int, [vp]),
         "dem_time": (C.c_double, [vp]),
+        "dem_insert_pack_defaults": (None, [C.POINTER(InsertPackSpec)]),
+        "dem_add_insert_pack": (C.c_int, [vp, C.POINTER(InsertPackSpec), ip]),
+        "dem_insert_vector": (C.c_int, [vp, C.c_int, dp]),
+        "dem_append_atoms": (C.c_int, [vp, C.c_int, dp, dp, dp, dp, dp, ip, ip, ip]),
+        "dem_natoms": (C.c_int, [vp]),
+        "dem_download_props": (C.c_int, [vp, ip, ip, dp, dp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -318,14 +416,8 @@ class Simulation:
     def read_restart(self, path):
         """Restores a restart written by write_restart; call setup() next."""
         self._chk(self.L.dem_read_restart(self.h, os.fsencode(path)))
-        # the atoms are now in the file's order: header (56 bytes: magic,
-        # version, n, ...), then tag, type, freeze, groups per atom
-        with open(path, "rb") as fp:
-            head = fp.read(56)
-            n = int(np.frombuffer(head, dtype=np.int32, count=1, offset=12)[0])
-            ai = np.frombuffer(fp.read(16 * n), dtype=np.int32).reshape(n, 4)
-        self.n = n
-        self.tag = ai[:, 0].copy()
+        # the atoms are now in the file's order
+        self._tag = self.props()["tag"]
 
     def add_massflow(self, nodes, vec_side, once=True, group_bit=-1):
         """fix massflow/mesh on a planar face mesh (ntri, 3, 3)."""
@@ -376,14 +468,58 @@ class Simulation:
         if rc != 0:
             raise DemError("dem error %d: %s" % (rc, self.L.dem_last_error(self.h).decode()))
 
+    @property
+    def n(self):
+        """atom count (it grows when an insertion fix places particles)"""
+        return self.L.dem_natoms(self.h)
+
+    @property
+    def tag(self):
+        if len(self._tag) != self.n:
+            self._tag = self.props()["tag"]
+        return self._tag
+
+    def props(self):
+        """tag, type, radius, rmass in download order"""
+        n = self.n
+        out = dict(tag=np.zeros(n, np.int32), type=np.zeros(n, np.int32), radius=np.zeros(n), rmass=np.zeros(n))
+        if n:
+            self._chk(self.L.dem_download_props(self.h, _ip(out["tag"]), _ip(out["type"]), _dp(out["radius"]),
+                                                _dp(out["rmass"])))
+        return out
+
+    def add_insert_pack(self, spec):
+        """fix insert/pack (+ its particledistribution/discrete and
+        particletemplate/sphere templates, region block); see
+        insert_pack_spec.  Returns the fix id."""
+        s, keep = insert_pack_spec(spec)
+        fid = C.c_int(0)
+        self._chk(self.L.dem_add_insert_pack(self.h, C.byref(s), C.byref(fid)))
+        del keep
+        return fid.value
+
+    def insert_vector(self, fid):
+        """ninserted, massinserted, last inserted, last aimed, last tries, next step"""
+        out = np.zeros(6)
+        self._chk(self.L.dem_insert_vector(self.h, int(fid), _dp(out)))
+        return out
+
+    def append_atoms(self, atoms, gmask=None):
+        m = len(atoms["radius"])
+        a = {k: np.ascontiguousarray(atoms[k], dtype=np.float64) for k in ("x", "v", "omega", "radius", "rmass")}
+        ty = np.ascontiguousarray(atoms["type"], dtype=np.int32)
+        tag = np.ascontiguousarray(atoms["tag"], dtype=np.int32)
+        gm = None if gmask is None else np.ascontiguousarray(gmask, dtype=np.int32)
+        self._chk(self.L.dem_append_atoms(self.h, m, _dp(a["x"]), _dp(a["v"]), _dp(a["omega"]), _dp(a["radius"]),
+                                          _dp(a["rmass"]), _ip(ty), _ip(tag), None if gm is None else _ip(gm)))
+
     def upload(self, atoms):
         n = len(atoms["radius"])
-        self.n = n
         a = {k: np.ascontiguousarray(atoms[k], dtype=np.float64) for k in ("x", "v", "omega", "radius", "rmass")}
         ty = np.ascontiguousarray(atoms["type"], dtype=np.int32)
         tag = np.ascontiguousarray(atoms["tag"], dtype=np.int32)
         fr = np.ascontiguousarray(atoms.get("freeze", np.zeros(n)), dtype=np.int32)
-        self.tag = tag.copy()
+        self._tag = tag.copy()
         self._chk(self.L.dem_upload_state(self.h, n, _dp(a["x"]), _dp(a["v"]), _dp(a["omega"]),
                                           _dp(a["radius"]), _dp(a["rmass"]), _ip(ty), _ip(tag), _ip(fr)))
 

@@ -1,0 +1,484 @@
+/*
+ * insert_oracle.c -- CPU restatement of fix insert/pack (test infrastructure
+ * only; see insert_oracle.h).  Reference files are LIGGGHTS-INL src/.
+ */
+#include "insert_oracle.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+static char g_err[256];
+
+const char *oi_error(void) { return g_err; }
+
+/* ---- RanPark (random_park.cpp:44-103) ---- */
+typedef struct {
+    int seed, save;
+    double second;
+} park;
+
+static void park_reset(park *p, int s)
+{
+    p->seed = s;
+    p->save = 0;
+}
+
+static double park_uniform(park *p)
+{
+    /* Schrage: IA 16807, IM 2^31-1, IQ = IM / IA, IR = IM % IA */
+    const int k = p->seed / 127773;
+    p->seed = 16807 * (p->seed - k * 127773) - 2836 * k;
+    if (p->seed < 0) p->seed += 2147483647;
+    return (1.0 / 2147483647) * p->seed;
+}
+
+static double park_gaussian(park *p)
+{
+    double first, v1, v2, rsq, fac;
+    if (!p->save) {
+        int again = 1;
+        while (again) {
+            v1 = 2.0 * park_uniform(p) - 1.0;
+            v2 = 2.0 * park_uniform(p) - 1.0;
+            rsq = v1 * v1 + v2 * v2;
+            if (rsq < 1.0 && rsq != 0.0) again = 0;
+        }
+        fac = sqrt(-2.0 * log(rsq) / rsq);
+        p->second = v1 * fac;
+        first = v2 * fac;
+        p->save = 1;
+    } else {
+        first = p->second;
+        p->save = 0;
+    }
+    return first;
+}
+
+void oi_park_states(int seed, int n, int *states, double *values)
+{
+    park p;
+    park_reset(&p, seed);
+    for (int i = 0; i < n; i++) {
+        const double u = park_uniform(&p);
+        if (states) states[i] = p.seed;
+        if (values) values[i] = u;
+    }
+}
+
+/* ---- the fix ---- */
+struct oi_fix {
+    oi_spec s;
+    park random, region_random, dist_random;
+    double distweight[OI_MAXT], volexp[OI_MAXT], massexp[OI_MAXT];
+    int distorder[OI_MAXT];
+    double volexpect, massexpect, max_r_bound;
+    double region_volume, region_volume_local, insertion_ratio;
+    long long next_reneighbor, most_recent;
+    long long ninserted;
+    double massinserted;
+    int last_inserted, last_aimed, last_tries;
+};
+
+/* RegBlock::inside (region_block.cpp:272-277) */
+static int block_inside(const oi_fix *f, const double *x)
+{
+    const double *lo = f->s.region_lo, *hi = f->s.region_hi;
+    return x[0] >= lo[0] && x[0] <= hi[0] && x[1] >= lo[1] && x[1] <= hi[1] && x[2] >= lo[2] && x[2] <= hi[2];
+}
+
+/* Region::match_cut -> RegBlock::surface_interior (region_block.cpp:286-345):
+ * the number of faces closer than cut to a point inside */
+static int block_surface_interior(const oi_fix *f, const double *x, double cut)
+{
+    const double *lo = f->s.region_lo, *hi = f->s.region_hi;
+    if (x[0] < lo[0] || x[0] > hi[0] || x[1] < lo[1] || x[1] > hi[1] || x[2] < lo[2] || x[2] > hi[2]) return 0;
+    int n = 0;
+    for (int d = 0; d < 3; d++) {
+        if (x[d] - lo[d] < cut) n++;
+        if (hi[d] - x[d] < cut) n++;
+    }
+    return n;
+}
+
+oi_fix *oi_create(const oi_spec *s, long long ntimestep)
+{
+    g_err[0] = 0;
+    if (s->ntemplates < 1 || s->ntemplates > OI_MAXT) {
+        snprintf(g_err, sizeof g_err, "illegal number of templates");
+        return NULL;
+    }
+    if (s->insert_every < 0) {
+        snprintf(g_err, sizeof g_err, "must define 'insert_every'");
+        return NULL;
+    }
+    oi_fix *f = (oi_fix *)calloc(1, sizeof(oi_fix));
+    f->s = *s;
+    const int T = s->ntemplates;
+    /* FixTemplateSphere (fix_template_sphere.cpp:243-245), constant radius
+     * and density: volume_expect = r^3 * 4 pi / 3, mass_expect = rho * that */
+    f->max_r_bound = 0.0;
+    for (int i = 0; i < T; i++) {
+        const double r = s->radius[i];
+        f->volexp[i] = r * r * r * 4. * M_PI / 3.;
+        f->massexp[i] = s->density[i] * f->volexp[i];
+        if (r > f->max_r_bound) f->max_r_bound = r;
+    }
+    /* FixParticledistributionDiscrete constructor
+     * (fix_particledistribution_discrete.cpp:126-233) */
+    double weightsum = 0;
+    for (int i = 0; i < T; i++) {
+        f->distweight[i] = s->weight[i];
+        weightsum += f->distweight[i];
+    }
+    for (int i = 0; i < T; i++) f->distweight[i] /= weightsum;
+    if (!s->number_based) {
+        for (int i = 0; i < T; i++) f->distweight[i] = f->distweight[i] / f->massexp[i];
+        weightsum = 0;
+        for (int i = 0; i < T; i++) weightsum += f->distweight[i];
+        for (int i = 0; i < T; i++) f->distweight[i] /= weightsum;
+    }
+    f->volexpect = f->massexpect = 0.;
+    for (int i = 0; i < T; i++) {
+        f->volexpect += f->volexp[i] * f->distweight[i];
+        f->massexpect += f->massexp[i] * f->distweight[i];
+    }
+    for (int i = 0; i < T; i++) f->distorder[i] = i;
+    int swapped, n = T;
+    do {
+        swapped = 0;
+        for (int i = 0; i < T - 1; i++)
+            if (f->volexp[f->distorder[i]] < f->volexp[f->distorder[i + 1]]) {
+                const int tmp = f->distorder[i];
+                f->distorder[i] = f->distorder[i + 1];
+                f->distorder[i + 1] = tmp;
+                swapped = 1;
+            }
+        n--;
+    } while (swapped && n > 0);
+
+    /* FixInsert constructor (fix_insert.cpp:101, :369-370, :458) */
+    park_reset(&f->random, s->seed);
+    park_reset(&f->dist_random, s->dist_seed);
+    f->next_reneighbor = s->start > 0 ? s->start : ntimestep + 1;
+    f->most_recent = -1;
+
+    /* FixInsert::setup -> FixInsertPack::calc_insertion_properties
+     * (fix_insert_pack.cpp:281-322): reset the region's stream to
+     * seed + SEED_OFFSET (12), then Region::volume_mc (region.cpp:633-700) */
+    park_reset(&f->region_random, s->seed + 12);
+    int n_in_global = 0, n_in_local = 0;
+    const double *lo = s->region_lo, *hi = s->region_hi;
+    for (int i = 0; i < s->ntry_mc; i++) {
+        double pos[3];
+        pos[0] = lo[0] + park_uniform(&f->region_random) * (hi[0] - lo[0]);
+        pos[1] = lo[1] + park_uniform(&f->region_random) * (hi[1] - lo[1]);
+        pos[2] = lo[2] + park_uniform(&f->region_random) * (hi[2] - lo[2]);
+        /* Domain::is_in_domain (domain_I.h:51-63) */
+        if (!(pos[0] >= s->boxlo[0] && pos[0] <= s->boxhi[0] && pos[1] >= s->boxlo[1] && pos[1] <= s->boxhi[1]
+              && pos[2] >= s->boxlo[2] && pos[2] <= s->boxhi[2]))
+            continue;
+        if (!s->all_in) {
+            if (block_inside(f, pos)) {
+                n_in_global++;
+                n_in_local++;   /* one process: the sub-box is the box */
+            }
+        } else {
+            if (block_inside(f, pos)) {
+                n_in_global++;
+                if (!block_surface_interior(f, pos, f->max_r_bound)) n_in_local++;
+            }
+        }
+    }
+    if (n_in_global == 0) {
+        snprintf(g_err, sizeof g_err, "Unable to calculate region volume");
+        free(f);
+        return NULL;
+    }
+    const double vol_bbox = (hi[0] - lo[0]) * (hi[1] - lo[1]) * (hi[2] - lo[2]);
+    f->region_volume = (double)n_in_global / (double)(s->ntry_mc * 1) * vol_bbox;
+    double vol_local = (double)n_in_local / (double)s->ntry_mc * vol_bbox;
+    const double vol_local_all = vol_local;
+    vol_local *= (f->region_volume / vol_local_all);
+    f->region_volume_local = vol_local;
+    return f;
+}
+
+void oi_destroy(oi_fix *f) { free(f); }
+
+long long oi_next_step(const oi_fix *f) { return f->next_reneighbor; }
+
+double oi_region_volume(const oi_fix *f) { return f->region_volume; }
+
+void oi_vector(const oi_fix *f, double out[6])
+{
+    out[0] = (double)f->ninserted;
+    out[1] = f->massinserted;
+    out[2] = f->last_inserted;
+    out[3] = f->last_aimed;
+    out[4] = f->last_tries;
+    out[5] = f->next_reneighbor > 0 ? (double)f->next_reneighbor : 0.0;
+}
+
+/* Region::generate_random / generate_random_shrinkby_cut
+ * (region.cpp:506-571), sub-domain = box */
+static void region_random_point(oi_fix *f, double *pos, double cut)
+{
+    double lo[3], hi[3], diff[3];
+    for (int d = 0; d < 3; d++) {
+        lo[d] = f->s.region_lo[d] > f->s.boxlo[d] ? f->s.region_lo[d] : f->s.boxlo[d];
+        hi[d] = f->s.region_hi[d] < f->s.boxhi[d] ? f->s.region_hi[d] : f->s.boxhi[d];
+        diff[d] = hi[d] - lo[d];
+    }
+    for (;;) {
+        pos[0] = lo[0] + park_uniform(&f->region_random) * diff[0];
+        pos[1] = lo[1] + park_uniform(&f->region_random) * diff[1];
+        pos[2] = lo[2] + park_uniform(&f->region_random) * diff[2];
+        if (!block_inside(f, pos)) continue;
+        if (f->s.all_in && block_surface_interior(f, pos, cut)) continue;
+        return;
+    }
+}
+
+/* FixInsert::generate_random_velocity (fix_insert.cpp:1025-1040) */
+static void random_velocity(oi_fix *f, double *v)
+{
+    if (f->s.vel_style == 1) {
+        v[0] = f->s.vel[0] + f->s.vel_fluct[0] * 2.0 * (park_uniform(&f->random) - 0.50);
+        v[1] = f->s.vel[1] + f->s.vel_fluct[1] * 2.0 * (park_uniform(&f->random) - 0.50);
+        v[2] = f->s.vel[2] + f->s.vel_fluct[2] * 2.0 * (park_uniform(&f->random) - 0.50);
+    } else if (f->s.vel_style == 2) {
+        v[0] = f->s.vel[0] + f->s.vel_fluct[0] * park_gaussian(&f->random);
+        v[1] = f->s.vel[1] + f->s.vel_fluct[1] * park_gaussian(&f->random);
+        v[2] = f->s.vel[2] + f->s.vel_fluct[2] * park_gaussian(&f->random);
+    }
+}
+
+/* rsq <= radsum^2 against every resident particle and its periodic images
+ * (ghost atoms, x + pbc * prd, comm) and every particle placed so far
+ * (RegionNeighborList::hasOverlap, region_neighbor_list_I.h:72-98) */
+static int overlaps(const oi_fix *f, const double *x, double radius, int n, const double *xr, const double *rr,
+                    int nnew, const double *xn, const double *rn)
+{
+    double prd[3];
+    int lo[3], hi[3];
+    for (int d = 0; d < 3; d++) {
+        prd[d] = f->s.boxhi[d] - f->s.boxlo[d];
+        lo[d] = f->s.periodic[d] ? -1 : 0;
+        hi[d] = f->s.periodic[d] ? 1 : 0;
+    }
+    for (int j = 0; j < n; j++)
+        for (int a = lo[0]; a <= hi[0]; a++)
+            for (int b = lo[1]; b <= hi[1]; b++)
+                for (int c = lo[2]; c <= hi[2]; c++) {
+                    double p[3], del[3];
+                    p[0] = a ? xr[3 * j] + a * prd[0] : xr[3 * j];
+                    p[1] = b ? xr[3 * j + 1] + b * prd[1] : xr[3 * j + 1];
+                    p[2] = c ? xr[3 * j + 2] + c * prd[2] : xr[3 * j + 2];
+                    del[0] = x[0] - p[0];
+                    del[1] = x[1] - p[1];
+                    del[2] = x[2] - p[2];
+                    const double rsq = del[0] * del[0] + del[1] * del[1] + del[2] * del[2];
+                    const double radsum = radius + rr[j];
+                    if (rsq <= radsum * radsum) return 1;
+                }
+    for (int j = 0; j < nnew; j++) {
+        double del[3];
+        del[0] = x[0] - xn[3 * j];
+        del[1] = x[1] - xn[3 * j + 1];
+        del[2] = x[2] - xn[3 * j + 2];
+        const double rsq = del[0] * del[0] + del[1] * del[1] + del[2] * del[2];
+        const double radsum = radius + rn[j];
+        if (rsq <= radsum * radsum) return 1;
+    }
+    return 0;
+}
+
+/* Domain::check_dist_subbox_borders (domain_I.h:127-149), sub-box = box */
+static int subbox_dist_ok(const oi_fix *f, const double *pos, double dist)
+{
+    for (int d = 0; d < 3; d++) {
+        const double dlo = fabs(f->s.boxlo[d] - pos[d]);
+        const double dhi = fabs(f->s.boxhi[d] - pos[d]);
+        if (dlo < dist || dhi < dist) return 0;
+    }
+    return 1;
+}
+
+int oi_insert(oi_fix *f, long long step, int n, const double *x, const double *r, const double *m, int cap,
+              double *x_out, double *v_out, double *omega_out, double *r_out, double *m_out, int *type_out)
+{
+    const oi_spec *s = &f->s;
+    /* FixInsert::pre_exchange (fix_insert.cpp:676-888) */
+    if (f->next_reneighbor != step || f->most_recent == step) return 0;
+    f->most_recent = step;
+
+    /* FixInsertPack::calc_ninsert_this (fix_insert_pack.cpp:333-432); the
+     * sums run over the resident particles in the order given */
+    int np_region = 0;
+    double vol_region = 0., mass_region = 0.;
+    const double _4Pi3 = 4. * M_PI / 3.;
+    for (int i = 0; i < n; i++)
+        if (block_inside(f, x + 3 * i)) {
+            np_region++;
+            vol_region += _4Pi3 * r[i] * r[i] * r[i];
+            mass_region += m[i];
+        }
+    int ninsert_this;
+    if (s->target == 0) {
+        ninsert_this = (int)((s->target_value * f->region_volume - vol_region) / f->volexpect
+                             + park_uniform(&f->random));
+        f->insertion_ratio = vol_region / (s->target_value * f->region_volume);
+    } else if (s->target == 1) {
+        const int ntotal_region = (int)s->target_value;
+        ninsert_this = ntotal_region - np_region;
+        f->insertion_ratio = (double)np_region / (double)ntotal_region;
+    } else {
+        ninsert_this = (int)((s->target_value - mass_region) / f->massexpect + park_uniform(&f->random));
+        f->insertion_ratio = mass_region / s->target_value;
+    }
+    if (ninsert_this < 0) ninsert_this = 0;
+    if (f->insertion_ratio < 0.) f->insertion_ratio = 0.;
+    if (f->insertion_ratio > 1.) f->insertion_ratio = 1.;
+
+    /* distribute_ninsert_this (fix_insert.cpp:893-970), one process: the
+     * exact form keeps the number; the uncorrelated one draws */
+    int ninsert_this_local = ninsert_this;
+    if (!s->exact_number)
+        ninsert_this_local = (int)(f->region_volume_local / f->region_volume * (double)ninsert_this
+                                   + park_uniform(&f->random));
+
+    /* FixParticledistributionDiscrete::randomize_list
+     * (fix_particledistribution_discrete.cpp:388-470) */
+    const int T = s->ntemplates;
+    int parttogen[OI_MAXT];
+    if (!s->exact_number) {
+        for (int i = 0; i < T; i++)
+            parttogen[i] = (int)((double)ninsert_this_local * f->distweight[i] + park_uniform(&f->dist_random));
+    } else {
+        int ninsert_truncated = 0;
+        double remainder[OI_MAXT];
+        for (int i = 0; i < T; i++) {
+            parttogen[i] = (int)((double)ninsert_this_local * f->distweight[i]);
+            ninsert_truncated += parttogen[i];
+            remainder[i] = (double)ninsert_this_local * f->distweight[i] - (double)parttogen[i];
+        }
+        const int ninsert_gap = ninsert_this_local - ninsert_truncated;
+        for (int i = 0; i < ninsert_gap; i++) {
+            const double rr = park_uniform(&f->dist_random) * (double)ninsert_gap;
+            int j = 0;
+            double rsum = remainder[0];
+            while (rsum < rr && j < T - 1) {
+                j++;
+                rsum += remainder[j];
+            }
+            parttogen[j]++;
+        }
+    }
+    int npti = 0;
+    for (int i = 0; i < T; i++) npti += parttogen[i];
+    int *pti_t = (int *)malloc(sizeof(int) * (npti > 0 ? npti : 1));
+    {
+        int k = 0;
+        for (int i = 0; i < T; i++) {
+            const int c = f->distorder[i];
+            for (int j = 0; j < parttogen[c]; j++) pti_t[k++] = c;
+        }
+    }
+    ninsert_this_local = npti;
+    f->last_aimed = npti;
+    f->last_inserted = 0;
+    f->last_tries = 0;
+    if (npti == 0) {
+        if (s->insert_every) f->next_reneighbor += s->insert_every;
+        else f->next_reneighbor = -1;
+        free(pti_t);
+        return 0;
+    }
+
+    /* FixInsertPack::calc_maxtry (fix_insert_pack.cpp:485-490) */
+    int maxtry;
+    if (f->insertion_ratio >= 1.) maxtry = ninsert_this_local * s->maxattempt;
+    else maxtry = (int)((double)(ninsert_this_local * s->maxattempt) / (1. - f->insertion_ratio));
+
+    /* FixInsertPack::x_v_omega (fix_insert_pack.cpp:498-605) */
+    int ninserted_this_local = 0, ntry = 0;
+    double mass_inserted_this_local = 0.;
+    double *xn = (double *)malloc(sizeof(double) * 3 * npti);
+    double *rn = (double *)malloc(sizeof(double) * npti);
+    if (!s->overlapcheck) {
+        for (int itotal = 0; itotal < ninsert_this_local; itotal++) {
+            const int t = pti_t[ninserted_this_local];
+            const double rbound = s->radius[t];
+            double pos[3], v[3];
+            region_random_point(f, pos, rbound);
+            v[0] = s->vel[0];
+            v[1] = s->vel[1];
+            v[2] = s->vel[2];
+            random_velocity(f, v);
+            const int k = ninserted_this_local;
+            memcpy(xn + 3 * k, pos, sizeof pos);
+            rn[k] = rbound;
+            if (k < cap) {
+                memcpy(x_out + 3 * k, pos, sizeof pos);
+                memcpy(v_out + 3 * k, v, sizeof v);
+            }
+            mass_inserted_this_local += s->density[t] * (rbound * rbound * rbound * 4. * M_PI / 3.);
+            ninserted_this_local++;
+        }
+    } else {
+        while (ntry < maxtry && ninserted_this_local < ninsert_this_local) {
+            const int t = pti_t[ninserted_this_local];
+            const double rbound = s->radius[t];
+            int nins = 0;
+            double pos[3], v[3];
+            while (nins == 0 && ntry < maxtry) {
+                do {
+                    region_random_point(f, pos, rbound);
+                    ntry++;
+                } while (s->check_border && ntry < maxtry && !subbox_dist_ok(f, pos, rbound));
+                if (ntry == maxtry) break;
+                v[0] = s->vel[0];
+                v[1] = s->vel[1];
+                v[2] = s->vel[2];
+                random_velocity(f, v);
+                /* ParticleToInsert::check_near_set_x_v_omega
+                 * (particleToInsert.cpp:246-262) */
+                if (!overlaps(f, pos, rbound, n, x, r, ninserted_this_local, xn, rn)) nins = 1;
+            }
+            if (nins > 0) {
+                const int k = ninserted_this_local;
+                memcpy(xn + 3 * k, pos, sizeof pos);
+                rn[k] = rbound;
+                if (k < cap) {
+                    memcpy(x_out + 3 * k, pos, sizeof pos);
+                    memcpy(v_out + 3 * k, v, sizeof v);
+                }
+                mass_inserted_this_local += s->density[t] * (rbound * rbound * rbound * 4. * M_PI / 3.);
+                ninserted_this_local++;
+            }
+        }
+    }
+    /* ParticleToInsert::insert (particleToInsert.cpp:97-150): radius,
+     * rmass = mass_ins, type, omega */
+    for (int k = 0; k < ninserted_this_local && k < cap; k++) {
+        const int t = pti_t[k];
+        r_out[k] = s->radius[t];
+        m_out[k] = s->density[t] * (s->radius[t] * s->radius[t] * s->radius[t] * 4. * M_PI / 3.);
+        type_out[k] = s->type[t];
+        omega_out[3 * k] = s->omega[0];
+        omega_out[3 * k + 1] = s->omega[1];
+        omega_out[3 * k + 2] = s->omega[2];
+    }
+    f->ninserted += ninserted_this_local;
+    f->massinserted += mass_inserted_this_local;
+    f->last_inserted = ninserted_this_local;
+    f->last_tries = ntry;
+    if (s->insert_every) f->next_reneighbor += s->insert_every;
+    else f->next_reneighbor = 0;
+    free(pti_t);
+    free(xn);
+    free(rn);
+    return ninserted_this_local;
+}

@@ -1,0 +1,60 @@
+/*
+ * insert_oracle.h -- CPU restatement of LIGGGHTS-INL's fix insert/pack with
+ * a particledistribution/discrete of particletemplate/sphere templates.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker of the engine's insertion.  Only
+ * tests/ load it.  The placement loop is the reference's, serial and plain;
+ * every overlap test is brute force over all resident particles (their
+ * periodic images included) and all particles placed so far, where the
+ * reference bins them (RegionNeighborList) and the engine tests batches on
+ * the device.  Cited per function in insert_oracle.c.
+ */
+#ifndef INSERT_ORACLE_H
+#define INSERT_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OI_MAXT 32
+
+typedef struct {
+    int seed, dist_seed;
+    int ntemplates, number_based;
+    int type[OI_MAXT];
+    double radius[OI_MAXT], density[OI_MAXT], weight[OI_MAXT];
+    double region_lo[3], region_hi[3];
+    int target;                 /* 0 volumefraction_region, 1 particles_in_region, 2 mass_in_region */
+    double target_value;
+    int maxattempt, insert_every;
+    long long start;            /* <= 0: the step after creation */
+    int overlapcheck, all_in, check_border, ntry_mc, exact_number;
+    int vel_style;              /* 0 constant, 1 uniform, 2 gaussian */
+    double vel[3], vel_fluct[3], omega[3];
+    double boxlo[3], boxhi[3];
+    int periodic[3];
+} oi_spec;
+
+typedef struct oi_fix oi_fix;
+
+/* The fix at timestep ntimestep; NULL on a bad spec (oi_error tells). */
+oi_fix *oi_create(const oi_spec *s, long long ntimestep);
+const char *oi_error(void);
+void oi_destroy(oi_fix *f);
+long long oi_next_step(const oi_fix *f);
+/* FixInsert::pre_exchange at `step` against n resident particles; writes
+ * the created particles (at most cap) and returns their number, -1 on an
+ * error, 0 when the step is no insertion step. */
+int oi_insert(oi_fix *f, long long step, int n, const double *x, const double *r, const double *m, int cap,
+              double *x_out, double *v_out, double *omega_out, double *r_out, double *m_out, int *type_out);
+/* ninserted, massinserted, last inserted, last aimed, last tries, next step */
+void oi_vector(const oi_fix *f, double out[6]);
+/* region volume estimate of the fix */
+double oi_region_volume(const oi_fix *f);
+/* RanPark from `seed`: the integer state after each of n draws */
+void oi_park_states(int seed, int n, int *states, double *values);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

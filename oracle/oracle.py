@@ -67,6 +67,38 @@ HYST_KEYS = ("LoadingStiffness", "UnloadingStiffness", "coefficientAdhesionStiff
              "aoneCustom", "atwoCustom", "athreeCustom", "kcinCustom")
 
 
+class _InsertSpec(C.Structure):
+    _fields_ = [
+        ("seed", C.c_int),
+        ("dist_seed", C.c_int),
+        ("ntemplates", C.c_int),
+        ("number_based", C.c_int),
+        ("type", C.c_int * 32),
+        ("radius", C.c_double * 32),
+        ("density", C.c_double * 32),
+        ("weight", C.c_double * 32),
+        ("region_lo", C.c_double * 3),
+        ("region_hi", C.c_double * 3),
+        ("target", C.c_int),
+        ("target_value", C.c_double),
+        ("maxattempt", C.c_int),
+        ("insert_every", C.c_int),
+        ("start", C.c_longlong),
+        ("overlapcheck", C.c_int),
+        ("all_in", C.c_int),
+        ("check_border", C.c_int),
+        ("ntry_mc", C.c_int),
+        ("exact_number", C.c_int),
+        ("vel_style", C.c_int),
+        ("vel", C.c_double * 3),
+        ("vel_fluct", C.c_double * 3),
+        ("omega", C.c_double * 3),
+        ("boxlo", C.c_double * 3),
+        ("boxhi", C.c_double * 3),
+        ("periodic", C.c_int * 3),
+    ]
+
+
 def build():
     """Compile the oracle library with its own Makefile."""
     import subprocess
@@ -127,6 +159,17 @@ def lib():
         L.or_set_integrate_group.argtypes = [C.c_void_p, C.c_int]
         L.or_mesh_distance.restype = C.c_double
         L.or_mesh_distance.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, dp, dp, dp, ip]
+        L.oi_create.restype = C.c_void_p
+        L.oi_create.argtypes = [C.POINTER(_InsertSpec), C.c_longlong]
+        L.oi_error.restype = C.c_char_p
+        L.oi_destroy.argtypes = [C.c_void_p]
+        L.oi_next_step.restype = C.c_longlong
+        L.oi_next_step.argtypes = [C.c_void_p]
+        L.oi_insert.argtypes = [C.c_void_p, C.c_longlong, C.c_int, dp, dp, dp, C.c_int, dp, dp, dp, dp, dp, ip]
+        L.oi_vector.argtypes = [C.c_void_p, dp]
+        L.oi_region_volume.restype = C.c_double
+        L.oi_region_volume.argtypes = [C.c_void_p]
+        L.oi_park_states.argtypes = [C.c_int, C.c_int, ip, dp]
         _LIB = L
     return _LIB
 
@@ -417,3 +460,99 @@ class Oracle:
         out = np.zeros(3)
         self.L.or_aveforce_vector(self.h, int(fid), _dp(out))
         return out
+
+
+_TARGETS = ("volumefraction_region", "particles_in_region", "mass_in_region")
+_VEL = {"constant": 0, "uniform": 1, "gaussian": 2}
+
+
+def park_states(seed, n):
+    """RanPark from seed: integer states and values of n draws."""
+    L = lib()
+    st = np.zeros(n, np.int32)
+    va = np.zeros(n)
+    L.oi_park_states(int(seed), int(n), _ip(st), _dp(va))
+    return st, va
+
+
+class InsertPackOracle:
+    """fix insert/pack restated (insert_oracle.c).  spec: the keyword dict of
+    the engine's Simulation.add_insert_pack; box: (lo, hi, periodic)."""
+
+    def __init__(self, spec, box_lo, box_hi, periodic, ntimestep=0):
+        self.L = lib()
+        s = _InsertSpec()
+        tl = spec["templates"]
+        w = spec.get("weights", [1.0] * len(tl))
+        s.seed = int(spec["seed"])
+        s.dist_seed = int(spec["dist_seed"])
+        s.ntemplates = len(tl)
+        s.number_based = int(spec.get("number_based", False))
+        for k, t in enumerate(tl):
+            s.type[k] = int(t["type"])
+            s.radius[k] = float(t["radius"])
+            s.density[k] = float(t["density"])
+            s.weight[k] = float(w[k])
+        lo, hi = spec["region"]
+        for d in range(3):
+            s.region_lo[d] = float(lo[d])
+            s.region_hi[d] = float(hi[d])
+            s.boxlo[d] = float(box_lo[d])
+            s.boxhi[d] = float(box_hi[d])
+            s.periodic[d] = int(periodic[d])
+        (key,) = [k for k in _TARGETS if k in spec]
+        s.target = _TARGETS.index(key)
+        s.target_value = float(spec[key])
+        s.maxattempt = int(spec.get("maxattempt", 50))
+        ev = spec.get("insert_every", -1)
+        s.insert_every = 0 if ev == "once" else int(ev)
+        s.start = int(spec.get("start", 0))
+        s.overlapcheck = int(spec.get("overlapcheck", True))
+        s.all_in = int(spec.get("all_in", False))
+        s.check_border = int(spec.get("check_dist_from_subdomain_border", True))
+        s.ntry_mc = int(spec.get("ntry_mc", 100000))
+        s.exact_number = int(spec.get("random_distribute", "exact") == "exact")
+        vel = spec.get("vel", ("constant", (0.0, 0.0, 0.0)))
+        s.vel_style = _VEL[vel[0]]
+        for d in range(3):
+            s.vel[d] = float(vel[1][d])
+            s.vel_fluct[d] = float(vel[2][d]) if len(vel) > 2 else 0.0
+            s.omega[d] = float(spec.get("omega", (0.0, 0.0, 0.0))[d])
+        self.h = self.L.oi_create(C.byref(s), int(ntimestep))
+        if not self.h:
+            raise RuntimeError(self.L.oi_error().decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.oi_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def next_step(self):
+        return self.L.oi_next_step(self.h)
+
+    def region_volume(self):
+        return self.L.oi_region_volume(self.h)
+
+    def vector(self):
+        out = np.zeros(6)
+        self.L.oi_vector(self.h, _dp(out))
+        return out
+
+    def insert(self, step, x, radius, rmass, cap=1 << 20):
+        """pre_exchange at step against resident particles x (n, 3): the
+        created particles (x, v, omega, radius, rmass, type)."""
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)
+        r = np.ascontiguousarray(radius, dtype=np.float64)
+        m = np.ascontiguousarray(rmass, dtype=np.float64)
+        n = len(r)
+        o = dict(x=np.zeros((cap, 3)), v=np.zeros((cap, 3)), omega=np.zeros((cap, 3)), radius=np.zeros(cap),
+                 rmass=np.zeros(cap), type=np.zeros(cap, np.int32))
+        k = self.L.oi_insert(self.h, int(step), n, _dp(x) if n else None, _dp(r) if n else None,
+                             _dp(m) if n else None, cap, _dp(o["x"]), _dp(o["v"]), _dp(o["omega"]),
+                             _dp(o["radius"]), _dp(o["rmass"]), _ip(o["type"]))
+        if k < 0:
+            raise RuntimeError("oracle insertion failed")
+        return {key: val[:k].copy() for key, val in o.items()}

@@ -1,0 +1,114 @@
+// CPU driver of the engine's insertion loop (liggghts-inl_amd/csrc/insert_pack.cpp)
+// for tests/test_insert.py: the device hooks replaced by brute force over the
+// resident particles, so the host logic (batched attempts, stream rewinds,
+// the serial walk) can be checked against the oracle without a GPU.
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "../liggghts-inl_amd/csrc/insert_pack.h"
+
+namespace {
+struct Run {
+    dem_ins::InsertPack fix;
+    double lo[3], hi[3];
+    int periodic[3];
+};
+}  // namespace
+
+extern "C" {
+
+void *ih_create(const dem_insert_pack_spec *s, const double *lo, const double *hi, const int *periodic,
+                long long ntimestep, char *err, int errlen)
+{
+    auto *r = new Run();
+    const std::string e = r->fix.init(*s, lo, hi, ntimestep);
+    if (!e.empty()) {
+        std::strncpy(err, e.c_str(), errlen - 1);
+        err[errlen - 1] = 0;
+        delete r;
+        return nullptr;
+    }
+    for (int d = 0; d < 3; d++) {
+        r->lo[d] = lo[d];
+        r->hi[d] = hi[d];
+        r->periodic[d] = periodic[d];
+    }
+    return r;
+}
+
+void ih_destroy(void *h) { delete static_cast<Run *>(h); }
+
+long long ih_next_step(void *h) { return static_cast<Run *>(h)->fix.next_step(); }
+
+void ih_vector(void *h, double *out) { static_cast<Run *>(h)->fix.vector(out); }
+
+// one pre_exchange at `step`; *nbatch counts the overlap calls
+int ih_insert(void *h, long long step, int n, const double *x, const double *rad, const double *mass, int cap,
+              double *xo, double *vo, double *wo, double *ro, double *mo, int *to, int *nbatch)
+{
+    Run *r = static_cast<Run *>(h);
+    dem_ins::Hooks hk;
+    int calls = 0;
+    hk.region_sums = [&](const double *lo, const double *hi, long long *np, double *vol, double *m) -> int {
+        *np = 0;
+        *vol = *m = 0.0;
+        const double f4pi3 = 4. * M_PI / 3.;
+        for (int i = 0; i < n; i++) {
+            const double *p = x + 3 * i;
+            if (p[0] >= lo[0] && p[0] <= hi[0] && p[1] >= lo[1] && p[1] <= hi[1] && p[2] >= lo[2] && p[2] <= hi[2]) {
+                (*np)++;
+                *vol += f4pi3 * rad[i] * rad[i] * rad[i];
+                *m += mass[i];
+            }
+        }
+        return 0;
+    };
+    hk.overlap = [&](const std::vector<double> &cand, const std::vector<double> &rr,
+                     std::vector<uint32_t> &hit) -> int {
+        calls++;
+        const size_t K = cand.size() / 3;
+        hit.assign(K, 0u);
+        double prd[3];
+        for (int d = 0; d < 3; d++) prd[d] = r->hi[d] - r->lo[d];
+        for (size_t k = 0; k < K; k++) {
+            const double *c = &cand[3 * k];
+            uint32_t b = 0u;
+            for (int j = 0; j < n; j++)
+                for (int a0 = r->periodic[0] ? -1 : 0; a0 <= (r->periodic[0] ? 1 : 0); a0++)
+                    for (int a1 = r->periodic[1] ? -1 : 0; a1 <= (r->periodic[1] ? 1 : 0); a1++)
+                        for (int a2 = r->periodic[2] ? -1 : 0; a2 <= (r->periodic[2] ? 1 : 0); a2++) {
+                            const double p0 = a0 ? x[3 * j] + a0 * prd[0] : x[3 * j];
+                            const double p1 = a1 ? x[3 * j + 1] + a1 * prd[1] : x[3 * j + 1];
+                            const double p2 = a2 ? x[3 * j + 2] + a2 * prd[2] : x[3 * j + 2];
+                            const double d0 = c[0] - p0, d1 = c[1] - p1, d2 = c[2] - p2;
+                            const double rsq = d0 * d0 + d1 * d1 + d2 * d2;
+                            for (size_t t = 0; t < rr.size(); t++) {
+                                const double rs = rr[t] + rad[j];
+                                if (rsq <= rs * rs) b |= 1u << t;
+                            }
+                        }
+            hit[k] = b;
+        }
+        return 0;
+    };
+    std::vector<dem_ins::Placed> out;
+    std::string err;
+    const int rc = r->fix.insert(hk, step, out, err);
+    if (nbatch) *nbatch = calls;
+    if (rc) return -1;
+    const int m = (int)out.size();
+    for (int k = 0; k < m && k < cap; k++) {
+        for (int d = 0; d < 3; d++) {
+            xo[3 * k + d] = out[k].x[d];
+            vo[3 * k + d] = out[k].v[d];
+            wo[3 * k + d] = out[k].omega[d];
+        }
+        ro[k] = out[k].radius;
+        mo[k] = out[k].mass;
+        to[k] = out[k].type;
+    }
+    return m;
+}
+}

@@ -1,0 +1,247 @@
+"""fix insert/pack on the MI355X against the oracle (oracle/insert_oracle.c).
+
+* the reference's packing tutorial (examples/LIGGGHTS/Tutorials_public/
+  packing/in.packing: two templates, mass fractions 0.3 / 0.7, volume
+  fraction 0.25, all_in, maxattempt 200, inserted once into an empty box) on
+  the device: the same particles, bit for bit, tags 1..N, at rest after the
+  insertion step (no two particles and no particle and wall touch);
+* insertion into a box holding resident particles, periodic in x and y: the
+  device's overlap test (grid over the residents, periodic images) gives the
+  oracle's brute-force result bit for bit;
+* a pour with gravity and insert_every: at every insertion step the new
+  particles are the ones the oracle places against the device's resident
+  particles at that step, and their velocity after the step is the inserted
+  one plus the final half kick;
+* appending atoms carries every existing contact history: a bed with a far
+  particle appended mid-run steps exactly as the bed without it;
+* a pour of 50,000 spheres into the top of the 1M-sphere packed column,
+  overlapping its surface: no overlaps (checked on the host with a k-d tree),
+  the aimed count placed, times reported.
+"""
+import time
+
+import numpy as np
+import pytest
+
+from conftest import load_engine
+from test_insert import BOX, PACKING, oracle_fix, resident_bed
+
+pytestmark = pytest.mark.gpu
+
+
+def empty_atoms():
+    return dict(x=np.zeros((0, 3)), v=np.zeros((0, 3)), omega=np.zeros((0, 3)), radius=np.zeros(0),
+                rmass=np.zeros(0), type=np.zeros(0, np.int32), tag=np.zeros(0, np.int32))
+
+
+def materials(Y=5e6, nu=0.45, e=0.3, mu=0.5, ntypes=1):
+    w = ntypes + 1
+    return dict(ntypes=ntypes, Y=np.full(w, Y), nu=np.full(w, nu), e=np.full((w, w), e), mu=np.full((w, w), mu))
+
+
+def packing_spec():
+    lo, hi = BOX
+    return dict(box_lo=list(lo), box_hi=list(hi), periodic=[0, 0, 0], dt=1e-5, skin=0.002, cdf=1.0,
+                neigh_every=1, neigh_delay=0, neigh_check=True, rolling="off", materials=materials(),
+                planes=[(0, lo[0], 1), (0, hi[0], 1), (1, lo[1], 1), (1, hi[1], 1), (2, lo[2], 1), (2, hi[2], 1)])
+
+
+def by_tag(st, props):
+    o = np.argsort(props["tag"])
+    return {k: v[o] for k, v in dict(st, **props).items() if k != "tag"}, props["tag"][o]
+
+
+def test_packing_tutorial_insertion_on_device():
+    eng = load_engine()
+    s = eng.Simulation(packing_spec(), empty_atoms())
+    fid = s.add_insert_pack(dict(PACKING, group_mask=0))
+    s.setup()
+    assert s.n == 0
+    s.step(1)
+    o = oracle_fix(PACKING)
+    ref = o.insert(1, np.zeros((0, 3)), np.zeros(0), np.zeros(0))
+    n = len(ref["radius"])
+    print("packing tutorial: %d spheres inserted, device %d" % (n, s.n))
+    assert s.n == n > 300
+    st, props = s.state(), s.props()
+    np.testing.assert_array_equal(props["tag"], np.arange(1, n + 1))
+    np.testing.assert_array_equal(st["x"], ref["x"])
+    np.testing.assert_array_equal(props["radius"], ref["radius"])
+    np.testing.assert_array_equal(props["rmass"], ref["rmass"])
+    np.testing.assert_array_equal(props["type"], ref["type"])
+    assert (st["v"] == 0).all() and (st["f"] == 0).all()
+    np.testing.assert_array_equal(s.insert_vector(fid), o.vector())
+    # once: no more insertions
+    s.step(50)
+    assert s.n == n
+    s.close()
+
+
+def test_insertion_against_residents_periodic():
+    eng = load_engine()
+    # residents clear of each other's periodic images (0.1 periodic, r <= 6 mm)
+    x, r, m = resident_bed(n=300, seed=5, half=0.036)
+    n0 = len(r)
+    atoms = dict(x=x, v=np.zeros((n0, 3)), omega=np.zeros((n0, 3)), radius=r, rmass=m,
+                 type=np.ones(n0, np.int32), tag=np.arange(101, 101 + n0, dtype=np.int32))
+    spec = dict(packing_spec(), periodic=[1, 1, 0], planes=[])
+    ins = dict(PACKING, volumefraction_region=0.35, all_in=False, vel=("uniform", (0.0, 0.0, 0.0), (0.1, 0.1, 0.1)))
+    s = eng.Simulation(spec, atoms)
+    fid = s.add_insert_pack(dict(ins, group_mask=0))
+    s.setup()
+    s.step(1)
+    o = oracle_fix(ins, periodic=(1, 1, 0))
+    ref = o.insert(1, x, r, m)
+    k = len(ref["radius"])
+    print("residents %d, inserted %d (oracle %d)" % (n0, s.n - n0, k))
+    assert s.n == n0 + k and k > 50
+    st, props = s.state(), s.props()
+    np.testing.assert_array_equal(st["x"][:n0], x)          # the residents did not move
+    np.testing.assert_array_equal(st["x"][n0:], ref["x"])
+    np.testing.assert_array_equal(props["tag"][n0:], np.arange(101 + n0, 101 + n0 + k))
+    np.testing.assert_array_equal(props["radius"][n0:], ref["radius"])
+    # the inserted velocities are set (no force acts: nothing touches)
+    np.testing.assert_array_equal(st["v"][n0:], ref["v"])
+    np.testing.assert_array_equal(s.insert_vector(fid), o.vector())
+    s.close()
+
+
+def pour_spec():
+    spec = dict(box_lo=[0.0, 0.0, 0.0], box_hi=[0.04, 0.04, 0.08], periodic=[0, 0, 0], dt=1e-5, skin=2.5e-4,
+                cdf=1.0, neigh_every=10, neigh_delay=0, neigh_check=False, rolling="off",
+                materials=materials(Y=5e6, nu=0.3, e=0.5, mu=0.5),
+                planes=[(0, 0.0, 1), (0, 0.04, 1), (1, 0.0, 1), (1, 0.04, 1), (2, 0.0, 1), (2, 0.08, 1)],
+                gravity=(0.0, 0.0, -9.81))
+    ins = dict(seed=15485863, dist_seed=15485867,
+               templates=[dict(type=1, radius=1.0e-3, density=2500.0), dict(type=1, radius=1.3e-3, density=2500.0)],
+               weights=[0.5, 0.5], region=((0.0, 0.0, 0.05), (0.04, 0.04, 0.08)), volumefraction_region=0.12,
+               maxattempt=50, insert_every=1500, all_in=True,
+               vel=("uniform", (0.0, 0.0, -0.6), (0.05, 0.05, 0.05)))
+    return spec, ins
+
+
+def test_pour_with_insert_every_matches_oracle():
+    eng = load_engine()
+    spec, ins = pour_spec()
+    s = eng.Simulation(spec, empty_atoms())
+    fid = s.add_insert_pack(dict(ins, group_mask=0))
+    o = oracle_fix(ins, box=(spec["box_lo"], spec["box_hi"]))
+    s.setup()
+    dtf = 0.5 * spec["dt"]
+    n_before = 0
+    for event in range(5):
+        t_ins = int(s.insert_vector(fid)[5])
+        assert t_ins == o.next_step()
+        s.step(t_ins - s.timestep())
+        assert s.timestep() == t_ins
+        st, props = s.state(), s.props()
+        ref = o.insert(t_ins, st["x"][:n_before], props["radius"][:n_before], props["rmass"][:n_before])
+        k = len(ref["radius"])
+        print("event %d at step %d: %d resident, %d inserted" % (event, t_ins, n_before, k))
+        assert s.n == n_before + k
+        new = slice(n_before, n_before + k)
+        np.testing.assert_array_equal(st["x"][new], ref["x"])
+        np.testing.assert_array_equal(props["radius"][new], ref["radius"])
+        np.testing.assert_array_equal(props["rmass"][new], ref["rmass"])
+        # after the step's final half kick: v = v_ins + dtf / m * f
+        np.testing.assert_array_equal(st["v"][new], ref["v"] + (dtf / ref["rmass"])[:, None] * st["f"][new])
+        np.testing.assert_array_equal(s.insert_vector(fid)[:5], o.vector()[:5])
+        n_before += k
+    assert n_before > 1000
+    eager = s.state()
+    s.close()
+    # graph stepping ends its periods before each insertion step, which runs
+    # eagerly: the same pour
+    g = eng.Simulation(dict(spec, graph=True), empty_atoms())
+    g.add_insert_pack(dict(ins, group_mask=0))
+    g.setup()
+    g.step(eager_steps := t_ins)
+    assert g.timestep() == eager_steps and g.n == n_before
+    gs = g.state()
+    for key in ("x", "v", "omega"):
+        np.testing.assert_array_equal(gs[key], eager[key], err_msg=key)
+    g.close()
+
+
+def test_append_keeps_contact_history():
+    """A bed with contacts; one far particle appended after 2300 steps, then
+    300 more.  Rebuilding every step in both runs, the bed steps exactly as
+    without it: its rows and their history are carried unchanged (the far
+    particle sits in the last cell of the sort order, so the bed keeps its
+    order too)."""
+    eng = load_engine()
+    spec, atoms = eng.workloads.pour_box(n=2000)
+    spec = dict(spec, neigh_every=1)
+    far = dict(x=np.array([[0.0385, 0.0385, 0.0785]]), v=np.zeros((1, 3)), omega=np.zeros((1, 3)),
+               radius=np.array([1e-3]), rmass=np.array([1e-5]), type=np.ones(1, np.int32),
+               tag=np.array([9999], np.int32))
+    runs = []
+    for append in (False, True):
+        s = eng.Simulation(spec, atoms)
+        s.setup()
+        s.step(2300)
+        if append:
+            s.append_atoms(far)
+        s.step(300)
+        st = s.state()
+        ti, tj, sh, hi, fl = s.pairs()
+        keep = (ti < 9999) & (tj < 9999) & (fl != 0)
+        pairs = {(a, b): h for a, b, h in zip(ti[keep].tolist(), tj[keep].tolist(), hi[keep])}
+        runs.append((st, pairs, s.n))
+        s.close()
+    (a, pa, na), (b, pb, nb) = runs
+    assert nb == na + 1
+    assert len(pa) > 200
+    for k in ("x", "v", "omega", "f", "torque"):
+        np.testing.assert_array_equal(a[k], b[k][:na], err_msg=k)
+    assert set(pa) == set(pb)
+    assert all(np.array_equal(pa[k], pb[k]) for k in pa)
+
+
+def test_pour_into_packed_column_1M():
+    from scipy.spatial import cKDTree
+
+    eng = load_engine()
+    spec, atoms = eng.workloads.packed_column()
+    top = float((atoms["x"][:, 2] + atoms["radius"]).max())
+    n0 = len(atoms["radius"])
+    s = eng.Simulation(spec, atoms, precision="mixed")
+    ins = dict(seed=32452843, dist_seed=32452867, templates=[dict(type=1, radius=1.0e-3, density=2500.0)],
+               weights=[1.0], region=((0.0, 0.0, top - 0.01), (0.06, 0.06, top + 0.3)),
+               particles_in_region=50000, maxattempt=50, insert_every="once", all_in=False,
+               vel=("constant", (0.0, 0.0, -0.5)))
+    # the residents already in the region count toward the target
+    lo, hi = np.array(ins["region"][0]), np.array(ins["region"][1])
+    present = int(((atoms["x"] >= lo) & (atoms["x"] <= hi)).all(1).sum())
+    ins["particles_in_region"] = present + 50000
+    fid = s.add_insert_pack(dict(ins, group_mask=0))
+    s.setup()
+    s.sync()
+    t = time.perf_counter()
+    s.step(1)
+    s.sync()
+    dt = time.perf_counter() - t
+    vec = s.insert_vector(fid)
+    print("1M column: %d residents in the region, %d inserted of %d aimed, %d tries, step with insertion %.1f ms"
+          % (present, vec[2], vec[3], vec[4], 1e3 * dt))
+    assert vec[3] == 50000 and vec[2] == 50000
+    st, props = s.state(), s.props()
+    assert s.n == n0 + 50000
+    x, r = st["x"], props["radius"]
+    # the residents' positions here are the ones the insertion tested
+    # against (a step's final half kick moves no particle); periodic in x, y
+    bs = np.array([0.06, 0.06, 100.0])
+    xp = np.mod(x + np.array([0.0, 0.0, 1.0]), bs)
+    tree = cKDTree(xp[:n0], boxsize=bs)
+    cand = tree.query_ball_point(xp[n0:], r=1.0e-3 + float(atoms["radius"].max()))
+    bad = 0
+    for i, js in enumerate(cand):
+        if js:
+            d = xp[js] - xp[n0 + i]
+            d -= bs * np.round(d / bs)
+            bad += int((np.sqrt((d * d).sum(1)) <= 1.0e-3 + r[js] + 1e-12).any())
+    assert bad == 0
+    # nor each other (they keep their radius from the box faces, so no
+    # periodic image comes closer)
+    assert len(cKDTree(x[n0:]).query_pairs(2.0e-3 - 1e-12)) == 0
+    s.close()
