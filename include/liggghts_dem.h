@@ -383,6 +383,40 @@ int dem_insert_vector(dem_ctx *ctx, int id, double *out6);
 int dem_append_atoms(dem_ctx *ctx, int m, const double *x, const double *v, const double *omega,
                      const double *radius, const double *rmass, const int *type, const int *tag,
                      const int *gmask);
+/* fix <id> all particletemplate/multiplespheres <seed> atom_type T density
+ * constant rho nspheres N ntry M spheres[_different_types] file F scale s
+ * [bonded yes] (fix_template_multiplespheres.cpp:78-300): x (3N) and r (N)
+ * as the file lists them, types per sphere or NULL.  Registering runs the
+ * template's set-up (post_create, :296-330): the bounding sphere and the
+ * Monte Carlo centre of mass on the template's stream, the inside tests of
+ * the ntry sample points on the device. */
+typedef struct {
+    int seed;
+    int atom_type;
+    double density;
+    int nspheres;
+    const double *x;
+    const double *r;
+    const int *types;
+    double scale;
+    int ntry;
+    int bonded;
+} dem_template_multisphere_spec;
+
+int dem_add_template_multisphere(dem_ctx *ctx, const dem_template_multisphere_spec *spec, int *id);
+/* out[0] bounding radius, out[1..3] bounding-sphere centre (centre-of-mass
+ * frame), out[4] volume and out[5] mass expectancy, out[6..8] the centre of
+ * mass in the file's frame. */
+int dem_template_info(dem_ctx *ctx, int id, double *out9);
+/* create_particles <template> single x y z [velocity v] [omega w]
+ * [orientation q] (create_particles.cpp:100-330): one clump, its spheres at
+ * x + rotate(x_j - x_bound, q) with rmass 4.18879020479 r^3 rho
+ * (particleToInsert.cpp:129), tags continuing from the largest; nothing is
+ * created when x lies outside the box.  quat NULL: identity (normalised as
+ * the command does otherwise).  gmask: engine group bits of the new atoms.
+ * Returns the number of spheres created, or a DEM_ERR_* code. */
+int dem_create_particles_single(dem_ctx *ctx, int tmpl, const double *x, const double *v, const double *omega,
+                                const double *quat, int gmask);
 /* Atom count (atom->nlocal) and, in download order, each atom's tag, type,
  * radius and mass (NULL pointers skipped). */
 int dem_natoms(const dem_ctx *ctx);

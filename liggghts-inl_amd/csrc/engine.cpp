@@ -224,6 +224,12 @@ struct dem_ctx {
         dem_ins::InsertPack fix;
     };
     std::vector<std::unique_ptr<Insert>> inserts;
+    // fix particletemplate/multiplespheres templates (create_particles)
+    struct Template {
+        int id;
+        dem_ins::TemplateMultisphere t;
+    };
+    std::vector<std::unique_ptr<Template>> templates;
     int tag_max = 0;
     int *ins_cell = nullptr, *ins_start = nullptr, *ins_idx = nullptr, *ins_tmp = nullptr;
     int ins_ncap = 0, ins_ccap = 0, ins_tcap = 0;
@@ -1863,6 +1869,116 @@ extern "C" int dem_append_atoms(dem_ctx *ctx, int m, const double *x, const doub
     // the next step re-neighbours (Neighbor::decide sees the forced step)
     if (rc == DEM_OK && m > 0) ctx->next_reneighbor = ctx->ntimestep + 1;
     return rc;
+}
+
+extern "C" int dem_add_template_multisphere(dem_ctx *ctx, const dem_template_multisphere_spec *spec, int *id)
+{
+    if (!ctx) return fail(nullptr, DEM_ERR_ARG, "null context");
+    if (!spec || !id) return fail(ctx, DEM_ERR_ARG, "dem_add_template_multisphere: bad arguments");
+    if (ctx->ntypes > 0) {
+        if (spec->atom_type < 1 || spec->atom_type > ctx->ntypes)
+            return fail(ctx, DEM_ERR_ARG, "dem_add_template_multisphere: atom type out of range");
+        for (int j = 0; spec->types && j < spec->nspheres; j++)
+            if (spec->types[j] < 1 || spec->types[j] > ctx->ntypes)
+                return fail(ctx, DEM_ERR_ARG, "dem_add_template_multisphere: sphere type out of range");
+    }
+    sync_enter(ctx);
+    // the inside tests of the Monte Carlo points, on the device
+    auto inside = [ctx](const std::vector<double> &pts, const std::vector<double> &xs, const std::vector<double> &rs,
+                        std::vector<uint8_t> &in) -> int {
+        const int np = (int)(pts.size() / 3), ns = (int)rs.size();
+        in.assign(np, 0);
+        double *dp = nullptr, *dx = nullptr, *dr = nullptr;
+        uint8_t *din = nullptr;
+        hipStream_t s = ctx->stream;
+        int rc = DEM_OK;
+        auto run = [&]() -> int {
+            HIPCHK(hipMalloc((void **)&dp, sizeof(double) * std::max<size_t>(pts.size(), 1)));
+            HIPCHK(hipMalloc((void **)&dx, sizeof(double) * std::max<size_t>(xs.size(), 1)));
+            HIPCHK(hipMalloc((void **)&dr, sizeof(double) * std::max<size_t>(rs.size(), 1)));
+            HIPCHK(hipMalloc((void **)&din, std::max(np, 1)));
+            HIPCHK(hipMemcpyAsync(dp, pts.data(), sizeof(double) * pts.size(), hipMemcpyHostToDevice, s));
+            HIPCHK(hipMemcpyAsync(dx, xs.data(), sizeof(double) * xs.size(), hipMemcpyHostToDevice, s));
+            HIPCHK(hipMemcpyAsync(dr, rs.data(), sizeof(double) * rs.size(), hipMemcpyHostToDevice, s));
+            launch_tmpl_inside(dp, np, dx, dr, ns, din, s);
+            KCHK();
+            HIPCHK(hipMemcpyAsync(in.data(), din, np, hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            return DEM_OK;
+        };
+        rc = run();
+        hipStreamSynchronize(s);
+        if (dp) hipFree(dp);
+        if (dx) hipFree(dx);
+        if (dr) hipFree(dr);
+        if (din) hipFree(din);
+        return rc;
+    };
+    auto tp = std::make_unique<dem_ctx::Template>();
+    const std::string err = tp->t.init(*spec, inside);
+    if (!err.empty()) return fail(ctx, DEM_ERR_ARG, "fix particletemplate/multiplespheres: " + err);
+    tp->id = ctx->next_op_id++;
+    *id = tp->id;
+    ctx->templates.push_back(std::move(tp));
+    return DEM_OK;
+}
+
+extern "C" int dem_template_info(dem_ctx *ctx, int id, double *out)
+{
+    if (!ctx) return fail(nullptr, DEM_ERR_ARG, "null context");
+    if (!out) return fail(ctx, DEM_ERR_ARG, "dem_template_info: null output");
+    for (const auto &t : ctx->templates)
+        if (t->id == id) {
+            t->t.info(out);
+            return DEM_OK;
+        }
+    return fail(ctx, DEM_ERR_ARG, "dem_template_info: no such template");
+}
+
+extern "C" int dem_create_particles_single(dem_ctx *ctx, int tmpl, const double *x, const double *v,
+                                           const double *omega, const double *quat, int gmask)
+{
+    if (!ctx) return fail(nullptr, DEM_ERR_ARG, "null context");
+    if (!x) return fail(ctx, DEM_ERR_ARG, "dem_create_particles_single: null position");
+    dem_ctx::Template *tp = nullptr;
+    for (auto &t : ctx->templates)
+        if (t->id == tmpl) tp = t.get();
+    if (!tp) return fail(ctx, DEM_ERR_ARG, "dem_create_particles_single: no such template");
+    // CreateParticles::add_single (create_particles.cpp:307-331): only a
+    // point of the (single) sub-box creates
+    for (int d = 0; d < 3; d++)
+        if (!(x[d] >= ctx->p.boxlo[d] && x[d] < ctx->p.boxhi[d])) return 0;
+    double q[4] = {1.0, 0.0, 0.0, 0.0};
+    if (quat) {
+        // MathExtraLiggghts::quat_normalize (math_extra_liggghts.h:513-520)
+        const double norm = 1.0 / std::sqrt(quat[0] * quat[0] + quat[1] * quat[1] + quat[2] * quat[2] + quat[3] * quat[3]);
+        for (int k = 0; k < 4; k++) q[k] = quat[k] * norm;
+    }
+    const auto &t = tp->t;
+    const int m = t.nspheres();
+    std::vector<double> xs, vs(3 * (size_t)m), ws(3 * (size_t)m), r(m), ms(m);
+    std::vector<int> ty(m), tg(m), gm(m, gmask);
+    t.place(x, q, xs);
+    for (int j = 0; j < m; j++) {
+        for (int d = 0; d < 3; d++) {
+            vs[3 * j + d] = v ? v[d] : 0.0;
+            ws[3 * j + d] = omega ? omega[d] : 0.0;
+        }
+        r[j] = t.radius(j);
+        // ParticleToInsert::insert (particleToInsert.cpp:129)
+        ms[j] = 4.18879020479 * r[j] * r[j] * r[j] * t.density();
+        ty[j] = t.type(j);
+        tg[j] = ctx->tag_max + 1 + j;
+    }
+    // FixTemplateMultiplespheres::randomize_single draws the clump's
+    // bond_random_id (fix_template_multiplespheres.cpp:600-615)
+    if (t.bonded()) tp->t.bond_id(ctx->ntimestep);
+    sync_enter(ctx);
+    const int rc = append_atoms(ctx, m, xs.data(), vs.data(), ws.data(), r.data(), ms.data(), ty.data(), tg.data(),
+                                gm.data());
+    if (rc) return rc;
+    ctx->next_reneighbor = ctx->ntimestep + 1;
+    return m;
 }
 
 extern "C" int dem_natoms(const dem_ctx *ctx)

@@ -16,6 +16,7 @@
 //                contraction, the reference's expression
 // k_region_sums  count, volume and mass of the particles inside a block
 //                (FixInsertPack::calc_ninsert_this), fixed-order partials
+// k_tmpl_inside  inside tests of a multisphere template's Monte Carlo points
 // k_extend_rows  empty previous-list rows for appended atoms
 #include "kernels.h"
 
@@ -162,6 +163,56 @@ void launch_region_sums(const double4 *xr, const double4 *vm, int n, const doubl
                         double *part, hipStream_t s)
 {
     k_region_sums<<<kRegionBlocks, kBlock, 0, s>>>(xr, vm, n, lo[0], lo[1], lo[2], hi[0], hi[1], hi[2], part);
+}
+
+// FixTemplateMultiplespheres::calc_center_of_mass's inside test
+// (fix_template_multiplespheres.cpp:420-435): point k inside a sphere j when
+// ((0 + dx^2) + dy^2) + dz^2 < r_j^2; the spheres pass through LDS in tiles
+__global__ void __launch_bounds__(kBlock) k_tmpl_inside(const double *__restrict__ pts, int npts,
+                                                        const double *__restrict__ xs, const double *__restrict__ rs,
+                                                        int nsph, uint8_t *inside)
+{
+#pragma clang fp contract(off)
+    __shared__ double sx[kBlock], sy[kBlock], sz[kBlock], sr[kBlock];
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    double p0 = 0.0, p1 = 0.0, p2 = 0.0;
+    if (k < npts) {
+        p0 = pts[3 * (size_t)k];
+        p1 = pts[3 * (size_t)k + 1];
+        p2 = pts[3 * (size_t)k + 2];
+    }
+    bool in = false;
+    for (int base = 0; base < nsph; base += kBlock) {
+        const int j = base + (int)threadIdx.x;
+        if (j < nsph) {
+            sx[threadIdx.x] = xs[3 * (size_t)j];
+            sy[threadIdx.x] = xs[3 * (size_t)j + 1];
+            sz[threadIdx.x] = xs[3 * (size_t)j + 2];
+            sr[threadIdx.x] = rs[j];
+        }
+        __syncthreads();
+        const int m = min(kBlock, nsph - base);
+        if (!in && k < npts)
+            for (int t = 0; t < m; t++) {
+                double d = 0.;
+                d += (p0 - sx[t]) * (p0 - sx[t]);
+                d += (p1 - sy[t]) * (p1 - sy[t]);
+                d += (p2 - sz[t]) * (p2 - sz[t]);
+                if (d < sr[t] * sr[t]) {
+                    in = true;
+                    break;
+                }
+            }
+        __syncthreads();
+    }
+    if (k < npts) inside[k] = in ? 1 : 0;
+}
+
+void launch_tmpl_inside(const double *pts, int npts, const double *xs, const double *rs, int nsph, uint8_t *inside,
+                        hipStream_t s)
+{
+    if (npts == 0) return;
+    k_tmpl_inside<<<(npts + kBlock - 1) / kBlock, kBlock, 0, s>>>(pts, npts, xs, rs, nsph, inside);
 }
 
 __global__ void __launch_bounds__(kBlock) k_extend_rows(int *start, int from, int count)

@@ -65,6 +65,153 @@ bool Block::near_face(const double *x, double cut) const
     return false;
 }
 
+// ---- fix particletemplate/multiplespheres
+
+std::string TemplateMultisphere::init(const dem_template_multisphere_spec &s, const InsideHook &inside)
+{
+    s_ = s;
+    const int n = s.nspheres;
+    if (n < 2) return "illegal number of spheres";
+    if (!s.x || !s.r) return "need to define spheres for the template";
+    if (s.ntry < 1000) return "ntry is too low";
+    if (!(s.scale > 0.)) return "scale factor must be >0";
+    if (!(s.density > 0.)) return "density must be > 0";
+    if (s.seed <= 0) return "illegal seed";
+    x_.assign(s.x, s.x + 3 * (size_t)n);
+    r_.assign(s.r, s.r + n);
+    if (s.types) types_.assign(s.types, s.types + n);
+    else types_.clear();
+    const double LARGE = 1e8;
+    for (int d = 0; d < 3; d++) {
+        x_min_[d] = LARGE;
+        x_max_[d] = -LARGE;
+    }
+    // the file's radii and positions times scale (the coarse-graining factor
+    // of this build is 1), then the bounding box
+    for (int i = 0; i < n; i++) {
+        if (!(r_[i] > 0.)) return "radius must be > 0";
+        r_[i] *= (s.scale * 1.0);
+        for (int d = 0; d < 3; d++) x_[3 * i + d] = (s.scale * 1.0) * x_[3 * i + d];
+    }
+    for (int i = 0; i < n; i++)
+        for (int d = 0; d < 3; d++) {
+            if (x_[3 * i + d] - r_[i] < x_min_[d]) x_min_[d] = x_[3 * i + d] - r_[i];
+            if (x_[3 * i + d] + r_[i] > x_max_[d]) x_max_[d] = x_[3 * i + d] + r_[i];
+        }
+    random_mc_.reset(s.seed);
+    random_insertion_.reset(s.seed);
+
+    // calc_bounding_sphere (fix_template_multiplespheres.cpp:321-375): 200
+    // passes over the spheres in random order, each growing a sphere to
+    // contain them; the smallest is kept
+    r_bound_ = LARGE;
+    std::vector<char> visited(n);
+    for (int shuffle = 0; shuffle < 200; shuffle++) {
+        std::fill(visited.begin(), visited.end(), 0);
+        int isphere = -1, nvisited = 0;
+        double xb[3], rb;
+        while (isphere < 0 || visited[isphere] || isphere >= n)
+            isphere = static_cast<int>(random_mc_.uniform() * n);
+        nvisited++;
+        visited[isphere] = 1;
+        for (int d = 0; d < 3; d++) xb[d] = x_[3 * isphere + d];
+        rb = r_[isphere];
+        while (nvisited < n) {
+            while (isphere < 0 || visited[isphere] || isphere >= n)
+                isphere = static_cast<int>(random_mc_.uniform() * n);
+            nvisited++;
+            visited[isphere] = 1;
+            double dv[3];
+            for (int d = 0; d < 3; d++) dv[d] = x_[3 * isphere + d] - xb[d];
+            const double dist = std::sqrt(dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2]);
+            if (dist + r_[isphere] > rb) {
+                const double fact = (dist + r_[isphere] - rb) / (2. * dist);
+                for (int d = 0; d < 3; d++) dv[d] = fact * dv[d];
+                for (int d = 0; d < 3; d++) xb[d] = xb[d] + dv[d];
+                rb += std::sqrt(dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2]);
+            }
+        }
+        if (rb < r_bound_) {
+            r_bound_ = rb;
+            for (int d = 0; d < 3; d++) x_bound_[d] = xb[d];
+        }
+    }
+    for (int i = 0; i < n; i++) {
+        double t[3];
+        for (int d = 0; d < 3; d++) t[d] = x_bound_[d] - x_[3 * i + d];
+        if (std::sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]) > r_bound_)
+            return "Bounding sphere calculation for template failed";
+    }
+
+    // calc_center_of_mass (fix_template_multiplespheres.cpp:410-450): ntry
+    // points in the bounding box; the inside tests on the device, the
+    // running mean over the inside points in their order here
+    std::vector<double> pts(3 * (size_t)s.ntry);
+    for (int i = 0; i < s.ntry; i++) {
+        pts[3 * (size_t)i] = x_min_[0] + (x_max_[0] - x_min_[0]) * random_mc_.uniform();
+        pts[3 * (size_t)i + 1] = x_min_[1] + (x_max_[1] - x_min_[1]) * random_mc_.uniform();
+        pts[3 * (size_t)i + 2] = x_min_[2] + (x_max_[2] - x_min_[2]) * random_mc_.uniform();
+    }
+    std::vector<uint8_t> in;
+    if (inside(pts, x_, r_, in)) return "device inside test failed";
+    int nsuccess = 0;
+    double xcm[3] = {0., 0., 0.};
+    for (int i = 0; i < s.ntry; i++) {
+        if (!in[i]) continue;
+        for (int d = 0; d < 3; d++)
+            xcm[d] = (xcm[d] * static_cast<double>(nsuccess) + pts[3 * (size_t)i + d]) / static_cast<double>(nsuccess + 1);
+        nsuccess++;
+    }
+    volume_expect_ = static_cast<double>(nsuccess) / static_cast<double>(s.ntry) * (x_max_[0] - x_min_[0])
+                     * (x_max_[1] - x_min_[1]) * (x_max_[2] - x_min_[2]);
+    mass_expect_ = volume_expect_ * s.density;
+    for (int d = 0; d < 3; d++) xcm_[d] = xcm[d];
+    // into the centre-of-mass frame
+    for (int i = 0; i < n; i++)
+        for (int d = 0; d < 3; d++) x_[3 * i + d] = x_[3 * i + d] - xcm[d];
+    for (int d = 0; d < 3; d++) {
+        x_min_[d] = x_min_[d] - xcm[d];
+        x_max_[d] = x_max_[d] - xcm[d];
+        x_bound_[d] = x_bound_[d] - xcm[d];
+    }
+    return "";
+}
+
+void TemplateMultisphere::info(double *out) const
+{
+    out[0] = r_bound_;
+    for (int d = 0; d < 3; d++) out[1 + d] = x_bound_[d];
+    out[4] = volume_expect_;
+    out[5] = mass_expect_;
+    for (int d = 0; d < 3; d++) out[6 + d] = xcm_[d];
+}
+
+// MathExtraLiggghts::vec_quat_rotate (math_extra_liggghts.h:457-473): q (0, v)
+// q*, by MathExtra::quatquat (math_extra.h:596-602)
+static void quatquat(const double *a, const double *b, double *c)
+{
+    c[0] = a[0] * b[0] - a[1] * b[1] - a[2] * b[2] - a[3] * b[3];
+    c[1] = a[0] * b[1] + b[0] * a[1] + a[2] * b[3] - a[3] * b[2];
+    c[2] = a[0] * b[2] + b[0] * a[2] + a[3] * b[1] - a[1] * b[3];
+    c[3] = a[0] * b[3] + b[0] * a[3] + a[1] * b[2] - a[2] * b[1];
+}
+
+void TemplateMultisphere::place(const double *x, const double *quat, std::vector<double> &out) const
+{
+    const int n = nspheres();
+    out.resize(3 * (size_t)n);
+    const double qc[4] = {quat[0], -quat[1], -quat[2], -quat[3]};
+    for (int j = 0; j < n; j++) {
+        double rel[3];
+        for (int d = 0; d < 3; d++) rel[d] = x_[3 * j + d] - x_bound_[d];
+        const double vq[4] = {0., rel[0], rel[1], rel[2]};
+        double t[4], r4[4];
+        quatquat(quat, vq, t);
+        quatquat(t, qc, r4);
+        for (int d = 0; d < 3; d++) out[3 * j + d] = r4[1 + d] + x[d];
+    }
+}
+
 std::string InsertPack::init(const dem_insert_pack_spec &s, const double *boxlo, const double *boxhi,
                              long long ntimestep)
 {

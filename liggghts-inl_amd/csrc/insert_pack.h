@@ -68,6 +68,39 @@ struct Hooks {
         overlap;
 };
 
+// fix particletemplate/multiplespheres (fix_template_multiplespheres.cpp):
+// a clump of spheres with constant density, its bounding sphere (randomised
+// iterative construction, 200 shuffles) and its centre of mass (Monte Carlo
+// over the bounding box), both on the template's `random_mc` stream; the
+// spheres are kept relative to the centre of mass.
+class TemplateMultisphere {
+  public:
+    // inside[k] = 1: point k lies inside a sphere, dist^2 < r^2 with the
+    // squared distance summed x, y, z (FixTemplateMultiplespheres::dist_sqr)
+    using InsideHook = std::function<int(const std::vector<double> &pts, const std::vector<double> &xs,
+                                         const std::vector<double> &rs, std::vector<uint8_t> &inside)>;
+    std::string init(const dem_template_multisphere_spec &s, const InsideHook &inside);
+    // ParticleToInsert::set_x_v_omega (particleToInsert.cpp:330-350): sphere
+    // j at x + rotate(x_j - x_bound, quat)
+    void place(const double *x, const double *quat, std::vector<double> &out) const;
+    int nspheres() const { return (int)r_.size(); }
+    double radius(int j) const { return r_[j]; }
+    int type(int j) const { return types_.empty() ? s_.atom_type : types_[j]; }
+    double density() const { return s_.density; }
+    bool bonded() const { return s_.bonded != 0; }
+    // bond_random_id of a clump created at `step`: step + u (random_insertion)
+    double bond_id(long long step) { return static_cast<double>(step) + random_insertion_.uniform(); }
+    void info(double *out8) const;
+
+  private:
+    dem_template_multisphere_spec s_{};
+    std::vector<double> x_, r_;
+    std::vector<int> types_;
+    double x_min_[3], x_max_[3], x_bound_[3], xcm_[3];
+    double r_bound_ = 0.0, volume_expect_ = 0.0, mass_expect_ = 0.0;
+    RanPark random_mc_, random_insertion_;
+};
+
 class InsertPack {
   public:
     // Checks the spec, builds the distribution and runs the region volume

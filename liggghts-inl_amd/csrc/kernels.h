@@ -352,6 +352,9 @@ constexpr int kRegionBlocks = 256;
 // part[4b..4b+2]: count, sum 4/3 pi r^3, sum mass of particles in [lo, hi]
 void launch_region_sums(const double4 *xr, const double4 *vm, int n, const double *lo, const double *hi,
                         double *part, hipStream_t s);
+// inside[k] = 1 when point k lies inside one of the nsph spheres (xs, rs)
+void launch_tmpl_inside(const double *pts, int npts, const double *xs, const double *rs, int nsph, uint8_t *inside,
+                        hipStream_t s);
 // start[from + 1 .. from + count] = start[from]
 void launch_extend_rows(int *start, int from, int count, hipStream_t s);
 

@@ -92,6 +92,21 @@ class InsertPackSpec(C.Structure):
     ]
 
 
+class TemplateMultisphereSpec(C.Structure):
+    _fields_ = [
+        ("seed", C.c_int),
+        ("atom_type", C.c_int),
+        ("density", C.c_double),
+        ("nspheres", C.c_int),
+        ("x", C.POINTER(C.c_double)),
+        ("r", C.POINTER(C.c_double)),
+        ("types", C.POINTER(C.c_int)),
+        ("scale", C.c_double),
+        ("ntry", C.c_int),
+        ("bonded", C.c_int),
+    ]
+
+
 INSERT_TARGET = {"volumefraction_region": 0, "particles_in_region": 1, "mass_in_region": 2}
 VEL_STYLE = {"constant": 0, "uniform": 1, "gaussian": 2}
 
@@ -255,6 +270,9 @@ def library():
         "dem_insert_vector": (C.c_int, [vp, C.c_int, dp]),
         "dem_append_atoms": (C.c_int, [vp, C.c_int, dp, dp, dp, dp, dp, ip, ip, ip]),
         "dem_natoms": (C.c_int, [vp]),
+        "dem_add_template_multisphere": (C.c_int, [vp, C.POINTER(TemplateMultisphereSpec), ip]),
+        "dem_template_info": (C.c_int, [vp, C.c_int, dp]),
+        "dem_create_particles_single": (C.c_int, [vp, C.c_int, dp, dp, dp, dp, C.c_int]),
         "dem_download_props": (C.c_int, [vp, ip, ip, dp, dp]),
     }
     for name, (res, args) in sig.items():
@@ -503,6 +521,47 @@ class Simulation:
         out = np.zeros(6)
         self._chk(self.L.dem_insert_vector(self.h, int(fid), _dp(out)))
         return out
+
+    def add_template_multisphere(self, seed, atom_type, density, x, r, types=None, scale=1.0, ntry=100000,
+                                 bonded=False):
+        """fix particletemplate/multiplespheres: spheres x (n, 3) and r (n,) as
+        the file lists them.  Runs the template's set-up; returns its id."""
+        xx = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)
+        rr = np.ascontiguousarray(r, dtype=np.float64)
+        tt = None if types is None else np.ascontiguousarray(types, dtype=np.int32)
+        s = TemplateMultisphereSpec()
+        s.seed = int(seed)
+        s.atom_type = int(atom_type)
+        s.density = float(density)
+        s.nspheres = len(rr)
+        s.x = _dp(xx)
+        s.r = _dp(rr)
+        s.types = None if tt is None else _ip(tt)
+        s.scale = float(scale)
+        s.ntry = int(ntry)
+        s.bonded = int(bool(bonded))
+        tid = C.c_int(0)
+        self._chk(self.L.dem_add_template_multisphere(self.h, C.byref(s), C.byref(tid)))
+        return tid.value
+
+    def template_info(self, tid):
+        """r_bound, x_bound (centre-of-mass frame), volume, mass, centre of mass (file frame)"""
+        out = np.zeros(9)
+        self._chk(self.L.dem_template_info(self.h, int(tid), _dp(out)))
+        return out
+
+    def create_particles_single(self, tid, x, v=None, omega=None, quat=None, gmask=0):
+        """create_particles <template> single x y z [velocity] [omega] [orientation]"""
+        xs = np.ascontiguousarray(x, dtype=np.float64)
+        vs = None if v is None else np.ascontiguousarray(v, dtype=np.float64)
+        ws = None if omega is None else np.ascontiguousarray(omega, dtype=np.float64)
+        qs = None if quat is None else np.ascontiguousarray(quat, dtype=np.float64)
+        k = self.L.dem_create_particles_single(self.h, int(tid), _dp(xs), None if vs is None else _dp(vs),
+                                               None if ws is None else _dp(ws), None if qs is None else _dp(qs),
+                                               int(gmask))
+        if k < 0:
+            raise DemError("dem error %d: %s" % (k, self.L.dem_last_error(self.h).decode()))
+        return k
 
     def append_atoms(self, atoms, gmask=None):
         m = len(atoms["radius"])

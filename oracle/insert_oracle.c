@@ -482,3 +482,165 @@ int oi_insert(oi_fix *f, long long step, int n, const double *x, const double *r
     free(rn);
     return ninserted_this_local;
 }
+
+/* ---- fix particletemplate/multiplespheres (fix_template_multiplespheres.cpp) ---- */
+struct oi_tmpl {
+    int n;
+    double *x, *r;
+    double x_min[3], x_max[3], x_bound[3], r_bound, xcm[3], volume;
+};
+
+oi_tmpl *oi_tmpl_create(int seed, int nspheres, const double *x, const double *r, double scale, int ntry)
+{
+    oi_tmpl *t = (oi_tmpl *)calloc(1, sizeof(oi_tmpl));
+    const int n = nspheres;
+    t->n = n;
+    t->x = (double *)malloc(sizeof(double) * 3 * n);
+    t->r = (double *)malloc(sizeof(double) * n);
+    /* constructor, file branch (:118-150): r *= scale*cg, x = (scale*cg) x,
+     * then the bounding box; cg = 1 */
+    for (int d = 0; d < 3; d++) {
+        t->x_min[d] = 1e8;
+        t->x_max[d] = -1e8;
+    }
+    for (int i = 0; i < n; i++) {
+        t->r[i] = r[i] * (scale * 1.0);
+        for (int j = 0; j < 3; j++) t->x[3 * i + j] = (scale * 1.0) * x[3 * i + j];
+    }
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < 3; j++) {
+            if (t->x[3 * i + j] - t->r[i] < t->x_min[j]) t->x_min[j] = t->x[3 * i + j] - t->r[i];
+            if (t->x[3 * i + j] + t->r[i] > t->x_max[j]) t->x_max[j] = t->x[3 * i + j] + t->r[i];
+        }
+    park mc;
+    park_reset(&mc, seed);   /* random_mc: the seed, no process shift */
+
+    /* calc_bounding_sphere (:321-375) */
+    t->r_bound = 1e8;
+    int *visited = (int *)malloc(sizeof(int) * n);
+    for (int shuffle = 0; shuffle < 200; shuffle++) {
+        for (int i = 0; i < n; i++) visited[i] = 0;
+        int isphere = -1, nvisited = 0;
+        double xbt[3], rbt, d[3], dist;
+        while (isphere < 0 || visited[isphere] || isphere >= n) isphere = (int)(park_uniform(&mc) * n);
+        nvisited++;
+        visited[isphere] = 1;
+        xbt[0] = t->x[3 * isphere];
+        xbt[1] = t->x[3 * isphere + 1];
+        xbt[2] = t->x[3 * isphere + 2];
+        rbt = t->r[isphere];
+        while (nvisited < n) {
+            while (isphere < 0 || visited[isphere] || isphere >= n) isphere = (int)(park_uniform(&mc) * n);
+            nvisited++;
+            visited[isphere] = 1;
+            d[0] = t->x[3 * isphere] - xbt[0];
+            d[1] = t->x[3 * isphere + 1] - xbt[1];
+            d[2] = t->x[3 * isphere + 2] - xbt[2];
+            dist = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+            if (dist + t->r[isphere] > rbt) {
+                const double fact = (dist + t->r[isphere] - rbt) / (2. * dist);
+                d[0] = fact * d[0];
+                d[1] = fact * d[1];
+                d[2] = fact * d[2];
+                xbt[0] = xbt[0] + d[0];
+                xbt[1] = xbt[1] + d[1];
+                xbt[2] = xbt[2] + d[2];
+                rbt += sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+            }
+        }
+        if (rbt < t->r_bound) {
+            t->r_bound = rbt;
+            t->x_bound[0] = xbt[0];
+            t->x_bound[1] = xbt[1];
+            t->x_bound[2] = xbt[2];
+        }
+    }
+    free(visited);
+
+    /* calc_center_of_mass (:410-450), brute force over the spheres */
+    int nsuccess = 0;
+    double xcm[3] = {0., 0., 0.};
+    for (int i = 0; i < ntry; i++) {
+        double xt[3];
+        xt[0] = t->x_min[0] + (t->x_max[0] - t->x_min[0]) * park_uniform(&mc);
+        xt[1] = t->x_min[1] + (t->x_max[1] - t->x_min[1]) * park_uniform(&mc);
+        xt[2] = t->x_min[2] + (t->x_max[2] - t->x_min[2]) * park_uniform(&mc);
+        for (int j = 0; j < n; j++) {
+            double dsq = 0.;
+            dsq += (xt[0] - t->x[3 * j]) * (xt[0] - t->x[3 * j]);
+            dsq += (xt[1] - t->x[3 * j + 1]) * (xt[1] - t->x[3 * j + 1]);
+            dsq += (xt[2] - t->x[3 * j + 2]) * (xt[2] - t->x[3 * j + 2]);
+            if (dsq < t->r[j] * t->r[j]) {
+                xcm[0] = (xcm[0] * (double)nsuccess + xt[0]) / (double)(nsuccess + 1);
+                xcm[1] = (xcm[1] * (double)nsuccess + xt[1]) / (double)(nsuccess + 1);
+                xcm[2] = (xcm[2] * (double)nsuccess + xt[2]) / (double)(nsuccess + 1);
+                nsuccess++;
+                break;
+            }
+        }
+    }
+    t->volume = (double)nsuccess / (double)ntry * (t->x_max[0] - t->x_min[0]) * (t->x_max[1] - t->x_min[1])
+                * (t->x_max[2] - t->x_min[2]);
+    for (int d = 0; d < 3; d++) t->xcm[d] = xcm[d];
+    for (int i = 0; i < n; i++)
+        for (int d = 0; d < 3; d++) t->x[3 * i + d] = t->x[3 * i + d] - xcm[d];
+    for (int d = 0; d < 3; d++) {
+        t->x_min[d] = t->x_min[d] - xcm[d];
+        t->x_max[d] = t->x_max[d] - xcm[d];
+        t->x_bound[d] = t->x_bound[d] - xcm[d];
+    }
+    return t;
+}
+
+void oi_tmpl_destroy(oi_tmpl *t)
+{
+    if (!t) return;
+    free(t->x);
+    free(t->r);
+    free(t);
+}
+
+void oi_tmpl_info(const oi_tmpl *t, double out[9])
+{
+    out[0] = t->r_bound;
+    for (int d = 0; d < 3; d++) out[1 + d] = t->x_bound[d];
+    out[4] = t->volume;
+    out[5] = 0.0;
+    for (int d = 0; d < 3; d++) out[6 + d] = t->xcm[d];
+}
+
+/* MathExtra::quatquat (math_extra.h:596-602) */
+static void oi_quatquat(const double *a, const double *b, double *c)
+{
+    c[0] = a[0] * b[0] - a[1] * b[1] - a[2] * b[2] - a[3] * b[3];
+    c[1] = a[0] * b[1] + b[0] * a[1] + a[2] * b[3] - a[3] * b[2];
+    c[2] = a[0] * b[2] + b[0] * a[2] + a[3] * b[1] - a[1] * b[3];
+    c[3] = a[0] * b[3] + b[0] * a[3] + a[1] * b[2] - a[2] * b[1];
+}
+
+void oi_tmpl_place(const oi_tmpl *t, const double x[3], const double *quat, double *out)
+{
+    double q[4] = {1.0, 0.0, 0.0, 0.0};
+    if (quat) {
+        /* MathExtraLiggghts::quat_normalize (math_extra_liggghts.h:513-520) */
+        const double norm = 1.0 / sqrt(quat[0] * quat[0] + quat[1] * quat[1] + quat[2] * quat[2] + quat[3] * quat[3]);
+        for (int k = 0; k < 4; k++) q[k] = quat[k] * norm;
+    }
+    /* ParticleToInsert::set_x_v_omega (particleToInsert.cpp:330-350) with
+     * MathExtraLiggghts::vec_quat_rotate (math_extra_liggghts.h:457-473) */
+    for (int j = 0; j < t->n; j++) {
+        double rel[3], vq[4], qc[4], tmp[4], res[4];
+        for (int d = 0; d < 3; d++) rel[d] = t->x[3 * j + d] - t->x_bound[d];
+        vq[0] = 0.;
+        vq[1] = rel[0];
+        vq[2] = rel[1];
+        vq[3] = rel[2];
+        qc[0] = q[0];
+        qc[1] = -q[1];
+        qc[2] = -q[2];
+        qc[3] = -q[3];
+        oi_quatquat(q, vq, tmp);
+        oi_quatquat(tmp, qc, res);
+        for (int d = 0; d < 3; d++) out[3 * j + d] = res[1 + d] + x[d];
+    }
+}

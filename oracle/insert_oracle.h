@@ -54,6 +54,18 @@ double oi_region_volume(const oi_fix *f);
 /* RanPark from `seed`: the integer state after each of n draws */
 void oi_park_states(int seed, int n, int *states, double *values);
 
+/* fix particletemplate/multiplespheres (no coarse graining): spheres as in
+ * the file, scaled; bounding sphere and Monte Carlo centre of mass on the
+ * template's stream, every inside test brute force. */
+typedef struct oi_tmpl oi_tmpl;
+oi_tmpl *oi_tmpl_create(int seed, int nspheres, const double *x, const double *r, double scale, int ntry);
+void oi_tmpl_destroy(oi_tmpl *t);
+/* r_bound, x_bound[3] (centre-of-mass frame), volume, (unused), xcm[3] */
+void oi_tmpl_info(const oi_tmpl *t, double out[9]);
+/* create_particles single: sphere positions of a clump at x, orientation
+ * quat (normalised as the command does; NULL = identity) */
+void oi_tmpl_place(const oi_tmpl *t, const double x[3], const double *quat, double *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -170,6 +170,11 @@ def lib():
         L.oi_region_volume.restype = C.c_double
         L.oi_region_volume.argtypes = [C.c_void_p]
         L.oi_park_states.argtypes = [C.c_int, C.c_int, ip, dp]
+        L.oi_tmpl_create.restype = C.c_void_p
+        L.oi_tmpl_create.argtypes = [C.c_int, C.c_int, dp, dp, C.c_double, C.c_int]
+        L.oi_tmpl_destroy.argtypes = [C.c_void_p]
+        L.oi_tmpl_info.argtypes = [C.c_void_p, dp]
+        L.oi_tmpl_place.argtypes = [C.c_void_p, dp, dp, dp]
         _LIB = L
     return _LIB
 
@@ -556,3 +561,35 @@ class InsertPackOracle:
         if k < 0:
             raise RuntimeError("oracle insertion failed")
         return {key: val[:k].copy() for key, val in o.items()}
+
+
+class TemplateOracle:
+    """fix particletemplate/multiplespheres + create_particles single,
+    restated (insert_oracle.c)."""
+
+    def __init__(self, seed, x, r, scale=1.0, ntry=100000):
+        self.L = lib()
+        xx = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)
+        rr = np.ascontiguousarray(r, dtype=np.float64)
+        self.n = len(rr)
+        self.h = self.L.oi_tmpl_create(int(seed), self.n, _dp(xx), _dp(rr), float(scale), int(ntry))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.oi_tmpl_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def info(self):
+        out = np.zeros(9)
+        self.L.oi_tmpl_info(self.h, _dp(out))
+        return out
+
+    def place(self, x, quat=None):
+        xs = np.ascontiguousarray(x, dtype=np.float64)
+        q = None if quat is None else np.ascontiguousarray(quat, dtype=np.float64)
+        out = np.zeros((self.n, 3))
+        self.L.oi_tmpl_place(self.h, _dp(xs), None if q is None else _dp(q), _dp(out))
+        return out

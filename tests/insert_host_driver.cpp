@@ -1,7 +1,8 @@
-// CPU driver of the engine's insertion loop (liggghts-inl_amd/csrc/insert_pack.cpp)
-// for tests/test_insert.py: the device hooks replaced by brute force over the
-// resident particles, so the host logic (batched attempts, stream rewinds,
-// the serial walk) can be checked against the oracle without a GPU.
+// CPU driver of the engine's insertion code (liggghts-inl_amd/csrc/insert_pack.cpp)
+// for tests/test_insert.py and tests/test_template.py: the device hooks
+// replaced by brute force, so the host logic (batched attempts, stream
+// rewinds, the serial walk; the template's bounding sphere, centre of mass
+// and placement) can be checked against the oracle without a GPU.
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -110,5 +111,52 @@ int ih_insert(void *h, long long step, int n, const double *x, const double *rad
         to[k] = out[k].type;
     }
     return m;
+}
+
+void *it_create(const dem_template_multisphere_spec *s, char *err, int errlen)
+{
+    auto *t = new dem_ins::TemplateMultisphere();
+    auto inside = [](const std::vector<double> &pts, const std::vector<double> &xs, const std::vector<double> &rs,
+                     std::vector<uint8_t> &in) -> int {
+        const size_t np = pts.size() / 3, ns = rs.size();
+        in.assign(np, 0);
+        for (size_t k = 0; k < np; k++)
+            for (size_t j = 0; j < ns; j++) {
+                double d = 0.;
+                d += (pts[3 * k] - xs[3 * j]) * (pts[3 * k] - xs[3 * j]);
+                d += (pts[3 * k + 1] - xs[3 * j + 1]) * (pts[3 * k + 1] - xs[3 * j + 1]);
+                d += (pts[3 * k + 2] - xs[3 * j + 2]) * (pts[3 * k + 2] - xs[3 * j + 2]);
+                if (d < rs[j] * rs[j]) {
+                    in[k] = 1;
+                    break;
+                }
+            }
+        return 0;
+    };
+    const std::string e = t->init(*s, inside);
+    if (!e.empty()) {
+        std::strncpy(err, e.c_str(), errlen - 1);
+        err[errlen - 1] = 0;
+        delete t;
+        return nullptr;
+    }
+    return t;
+}
+
+void it_destroy(void *h) { delete static_cast<dem_ins::TemplateMultisphere *>(h); }
+
+void it_info(void *h, double *out) { static_cast<dem_ins::TemplateMultisphere *>(h)->info(out); }
+
+void it_place(void *h, const double *x, const double *quat, double *out)
+{
+    auto *t = static_cast<dem_ins::TemplateMultisphere *>(h);
+    double q[4] = {1.0, 0.0, 0.0, 0.0};
+    if (quat) {
+        const double norm = 1.0 / std::sqrt(quat[0] * quat[0] + quat[1] * quat[1] + quat[2] * quat[2] + quat[3] * quat[3]);
+        for (int k = 0; k < 4; k++) q[k] = quat[k] * norm;
+    }
+    std::vector<double> o;
+    t->place(x, q, o);
+    std::memcpy(out, o.data(), o.size() * sizeof(double));
 }
 }

@@ -14,6 +14,11 @@
   one plus the final half kick;
 * appending atoms carries every existing contact history: a bed with a far
   particle appended mid-run steps exactly as the bed without it;
+* particletemplate/multiplespheres and create_particles single on the
+  device (microstructure_compression's 114,704-sphere template, the Monte
+  Carlo inside tests on the GPU): the oracle's template and placement bit
+  for bit, sphere 1 where the reference's gold prints it; an oriented,
+  bonded two-sphere clump;
 * a pour of 50,000 spheres into the top of the 1M-sphere packed column,
   overlapping its surface: no overlaps (checked on the host with a k-d tree),
   the aimed count placed, times reported.
@@ -244,4 +249,50 @@ def test_pour_into_packed_column_1M():
     # nor each other (they keep their radius from the box faces, so no
     # periodic image comes closer)
     assert len(cKDTree(x[n0:]).query_pairs(2.0e-3 - 1e-12)) == 0
+    s.close()
+
+
+def test_template_and_create_particles_on_device():
+    from test_template import GOLD_SPHERE1, microstructure_template, tmpl_oracle
+
+    eng = load_engine()
+    x = microstructure_template()
+    n = len(x)
+    spec = dict(box_lo=[-10.0, -10.0, 0.0], box_hi=[110.0, 110.0, 200.0], periodic=[0, 0, 0], dt=2e-3, skin=2.0,
+                cdf=1.0, neigh_every=1, neigh_delay=0, neigh_check=True, rolling="off",
+                materials=materials(Y=2.5e5, nu=0.3, e=0.1, mu=0.5, ntypes=2))
+    s = eng.Simulation(spec, empty_atoms())
+    t0 = time.perf_counter()
+    tid = s.add_template_multisphere(900019, 1, 1089.0, x, np.ones(n), types=np.full(n, 2), ntry=100000)
+    t_setup = time.perf_counter() - t0
+    o = tmpl_oracle(900019, x, np.ones(n), ntry=100000)
+    info, oi = s.template_info(tid), o.info()
+    print("template of %d spheres set up in %.2f s; r_bound %.6f, volume %.1f" % (n, t_setup, info[0], info[4]))
+    np.testing.assert_array_equal(info[[0, 1, 2, 3, 4, 6, 7, 8]], oi[[0, 1, 2, 3, 4, 6, 7, 8]])
+    assert info[5] == info[4] * 1089.0
+    assert s.create_particles_single(tid, (500.0, 0.0, 0.0)) == 0      # outside the box: none
+    k = s.create_particles_single(tid, (49.0, 49.0, 99.0))
+    assert k == n == s.n
+    st, props = s.state(), s.props()
+    np.testing.assert_array_equal(st["x"], o.place([49.0, 49.0, 99.0]))
+    assert np.abs(st["x"][0] - np.array(GOLD_SPHERE1)).max() < 1e-15
+    np.testing.assert_array_equal(props["tag"], np.arange(1, n + 1))
+    assert (props["type"] == 2).all() and (props["radius"] == 1.0).all()
+    assert (props["rmass"] == 4.18879020479 * 1.0 * 1.0 * 1.0 * 1089.0).all()
+    s.close()
+
+    # an oriented two-sphere clump, velocity and spin, after resident atoms
+    s = eng.Simulation(spec, empty_atoms())
+    x2, r2 = [[0.0, 0.0, 0.0], [0.0004, 0.0, 0.0]], [0.0002, 0.0002]
+    tid = s.add_template_multisphere(900007, 1, 2500.0, x2, r2, ntry=100000, bonded=True)
+    o2 = tmpl_oracle(900007, x2, r2, ntry=100000)
+    q = [0.3, -0.2, 0.9, 0.1]
+    assert s.create_particles_single(tid, (1.0, 2.0, 3.0)) == 2
+    assert s.create_particles_single(tid, (4.0, 5.0, 6.0), v=(0.1, 0.2, 0.3), omega=(1.0, 0.0, 0.0), quat=q) == 2
+    st, props = s.state(), s.props()
+    np.testing.assert_array_equal(st["x"][:2], o2.place([1.0, 2.0, 3.0]))
+    np.testing.assert_array_equal(st["x"][2:], o2.place([4.0, 5.0, 6.0], q))
+    np.testing.assert_array_equal(props["tag"], [1, 2, 3, 4])
+    np.testing.assert_array_equal(st["v"][2:], [[0.1, 0.2, 0.3]] * 2)
+    np.testing.assert_array_equal(st["omega"][2:], [[1.0, 0.0, 0.0]] * 2)
     s.close()
