@@ -189,8 +189,10 @@ int dem_add_massflow(dem_ctx *ctx, int ntri, const double *nodes, const double *
 int dem_massflow_vector(dem_ctx *ctx, double *out);
 /* Bond tables (DEM_NBOND) and settings of the cohesion model
  * (cohesion_model_bond.h:237-470, cohesion_model_bond_nonlinear.h:204-386).
- * Call after dem_upload_state: registers the model's contact distance
- * factor from the tables and the smallest radius (neighbor.h:141-142). */
+ * Call after dem_upload_state and after registering the templates and
+ * insertion fixes: registers the model's contact distance factor from the
+ * tables and the smallest radius of the atoms and templates
+ * (neighbor.h:141-142, modify_liggghts.cpp:515-567). */
 int dem_set_bond(dem_ctx *ctx, const double *tables, const dem_bond_settings *bs);
 /* fix gravity acceleration (fix_gravity.cpp:301-352, :395-397). */
 int dem_set_gravity(dem_ctx *ctx, double ax, double ay, double az);
@@ -359,7 +361,14 @@ typedef struct {
     int group_mask;              /* engine group bits the inserted atoms carry (the
                                     insert fix's group; ParticleToInsert::insert sets
                                     mask = 1 | groupbit, particleToInsert.cpp:113)    */
+    const int *multisphere;      /* per template: the id dem_add_template_multisphere
+                                    gave a particletemplate/multiplespheres, or 0 for
+                                    the sphere of `templates` (NULL: all spheres)      */
+    int orientation;             /* DEM_ORIENT_*, keyword orientation                 */
+    double quat[4];              /* orientation constant q0 q1 q2 q3                  */
 } dem_insert_pack_spec;
+
+enum { DEM_ORIENT_TEMPLATE = 0, DEM_ORIENT_RANDOM = 1, DEM_ORIENT_CONSTANT = 2 };
 
 void dem_insert_pack_defaults(dem_insert_pack_spec *spec);
 /* Registers the fix: validates the spec, builds the distribution

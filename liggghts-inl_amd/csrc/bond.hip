@@ -74,7 +74,8 @@ enum { B_NONE = 0, B_FORCE = 1, B_BROKEN = 2 };
 __device__ inline int bond_eval(const Model &M, const double *mat, int it, int jt, double *hb, const double delta[3],
                                 double rsq, double radi, double radj, const double xi[3], const double vi[3],
                                 const double vj[3], const double omegai[3], const double omegaj[3], bool upd,
-                                long long ntimestep, bool &created, double out[9])
+                                long long ntimestep, const double *brid, int i, int j, bool &created,
+                                double out[9])
 {
     const bool nl = M.cohesion == CM_BOND_NONLINEAR;
     const BondSet &bs = M.bs;
@@ -85,7 +86,14 @@ __device__ inline int bond_eval(const Model &M, const double *mat, int it, int j
     const double r = sqrt(rsq);
     if (upd) {
         if (hb[0] < small) {
-            const bool create = (bs.create_always || ntimestep == bs.ts_create) && (r < tbl(mat, M_BCREATEDIST, it, jt));
+            // create_bond_A, else create_bond_B with a bond_random_id
+            // (cohesion_model_bond.h:522-540; compDouble: |a - b| < 1e-13)
+            const double rc = tbl(mat, M_BCREATEDIST, it, jt);
+            bool create = (bs.create_always || ntimestep == bs.ts_create) && (r < rc);
+            if (!create && brid) {
+                const double bi = brid[i], bj = brid[j], db = bi - bj;
+                create = (double)ntimestep < bi && (bi == bj || (db < 1e-13 && db > -1e-13)) && (r < rc);
+            }
             if (!create) return B_NONE;
             created = true;
             hb[0] = 1.0;
@@ -322,7 +330,9 @@ __global__ void __launch_bounds__(kBlock) k_bond(BondArgs a)
             continue;
         }
         const bool bonded = (w & kBondBit) != 0;
-        if (!bonded && !(a.shearupdate && (M.bs.create_always || a.ntimestep == M.bs.ts_create))) continue;
+        if (!bonded && !(a.shearupdate && (M.bs.create_always || a.ntimestep == M.bs.ts_create
+                                           || (a.brid && (double)a.ntimestep < a.brid[i]))))
+            continue;
         const double4 vmj = a.vm[j];
         const double4 wj = a.wt[j];
         const double vj[3] = {vmj.x, vmj.y, vmj.z};
@@ -331,7 +341,7 @@ __global__ void __launch_bounds__(kBlock) k_bond(BondArgs a)
         double out[9];
         bool created = false;
         const int res = bond_eval(M, a.mat, it, meta_type((int)wj.w), hp + off, delta, rsq, pi.w, pj.w, xi, vi, vj, omi, omj,
-                                  a.shearupdate != 0, a.ntimestep, created, out);
+                                  a.shearupdate != 0, a.ntimestep, a.brid, i, j, created, out);
         if (created) {
             // an entry that carried no flag holds no valid contact history
             if (!(w & (kFlagBit | kBondBit)))

@@ -187,6 +187,7 @@ struct dem_ctx {
     };
     std::vector<MoveOp> moves;
     double4 *xo[2] = {};     // positions at the fix's creation, per particle
+    double *brid[2] = {};    // bond_random_id per particle (null: none set)
     double *d_gpart = nullptr, *d_gtot = nullptr;   // partial sums; 4 per op slot
     int ago = 0;
 
@@ -477,6 +478,7 @@ static void free_all(dem_ctx *ctx)
         F(ctx->img[b]); F(ctx->whist[b]); F(ctx->nstart[b]); F(ctx->nj[b]); F(ctx->hist[b]);
         F(ctx->gmask[b]);
         F(ctx->xo[b]);
+        F(ctx->brid[b]);
         F(ctx->mf.counter[b]);
 
     }
@@ -699,9 +701,10 @@ static int setup_grid(dem_ctx *ctx);
 extern "C" int dem_set_bond(dem_ctx *ctx, const double *tables, const dem_bond_settings *bs)
 {
     if (!ctx) return fail(nullptr, DEM_ERR_ARG, "null context");
-    if (!tables || !bs || ctx->M.cohesion == CM_OFF || !ctx->mat_set || ctx->n == 0)
+    if (!tables || !bs || ctx->M.cohesion == CM_OFF || !ctx->mat_set
+        || (ctx->n == 0 && ctx->templates.empty() && ctx->inserts.empty()))
         return fail(ctx, DEM_ERR_ARG,
-                    "dem_set_bond: needs a cohesion model, materials and uploaded atoms, non-null tables");
+                    "dem_set_bond: needs a cohesion model, materials, atoms or templates, non-null tables");
     if (ctx->ninst > 1) return fail(ctx, DEM_ERR_ARG, "dem_set_bond: bonds are not available in a batched run");
     sync_enter(ctx);
     const int w = ctx->ntypes + 1;
@@ -717,11 +720,16 @@ extern "C" int dem_set_bond(dem_ctx *ctx, const double *tables, const dem_bond_s
     ctx->M.bs = BondSet{(long long)(int)bs->ts_create, bs->stress_break, bs->create_always, bs->damping,
                         bs->damping_smooth, bs->ratio_tc, bs->tension_variant};
     // the model's contact distance factor (cohesion_model_bond.h:415-470,
-    // cohesion_model_bond_nonlinear.h:330-380) with the smallest radius
+    // cohesion_model_bond_nonlinear.h:330-380) with the smallest radius of
+    // the atoms and of the templates that insert or create them
+    // (Modify::get_min_radius, modify_liggghts.cpp:515-567)
     std::vector<double4> hx(ctx->n);
-    HIPCHK(hipMemcpy(hx.data(), sx(ctx), sizeof(double4) * ctx->n, hipMemcpyDeviceToHost));
+    if (ctx->n) HIPCHK(hipMemcpy(hx.data(), sx(ctx), sizeof(double4) * ctx->n, hipMemcpyDeviceToHost));
     double minrad = 1e300;
     for (const auto &q : hx) minrad = std::min(minrad, q.w);
+    for (const auto &t : ctx->templates)
+        for (int j = 0; j < t->t.nspheres(); j++) minrad = std::min(minrad, t->t.radius(j));
+    for (const auto &in : ctx->inserts) minrad = std::min(minrad, in->fix.min_radius());
     const bool nl = ctx->M.cohesion == CM_BOND_NONLINEAR;
     double cdf_all = 1.;
     auto T = [&](int slot, int i, int j) { return tables[(size_t)slot * w * w + i * w + j]; };
@@ -933,6 +941,12 @@ extern "C" int dem_upload_state(dem_ctx *ctx, int n, const double *x, const doub
     }
     // the group bits are cleared, and with them the groups fix move acts on
     ctx->moves.clear();
+    // no atom carries a bond_random_id
+    for (int b = 0; b < 2; b++)
+        if (ctx->brid[b]) {
+            HIPCHK(hipFree(ctx->brid[b]));
+            ctx->brid[b] = nullptr;
+        }
     // new atoms: massflow side counters not seen yet
     if (ctx->mf.mesh >= 0 && massflow_counters(ctx)) return DEM_ERR_HIP;
     ctx->integ_bit = -1;
@@ -1115,6 +1129,8 @@ static int rebuild(dem_ctx *ctx, bool captured = false)
     pa.xo1 = ctx->moves.empty() ? nullptr : ctx->xo[c1];
     pa.mf0 = ctx->mf.mesh >= 0 ? ctx->mf.counter[c0] : nullptr;
     pa.mf1 = ctx->mf.mesh >= 0 ? ctx->mf.counter[c1] : nullptr;
+    pa.br0 = ctx->brid[c0];
+    pa.br1 = ctx->brid[c1];
     pa.xhold = ctx->xhold;
     pa.wmask = ctx->wmask;
     pa.n = n;
@@ -1285,6 +1301,7 @@ static int force_pass(dem_ctx *ctx, int shearupdate, int integ = 0)
         ba.bout = ctx->bout;
         ba.bvalid = ctx->bvalid;
         ba.mat = ctx->d_mat;
+        ba.brid = ctx->brid[c];
         ba.n = ctx->n;
         ba.shearupdate = shearupdate;
         ba.ntimestep = ctx->ntimestep;
@@ -1583,6 +1600,7 @@ static int grow_particles(dem_ctx *ctx, int c)
             || dev_grow(ctx, &ctx->nstart[b], oc ? oc + 1 : 0, nc + 1) || dev_grow(ctx, &ctx->gmask[b], oc, nc))
             return DEM_ERR_HIP;
         if (ctx->xo[b] && dev_grow(ctx, &ctx->xo[b], oc, nc)) return DEM_ERR_HIP;
+        if (ctx->brid[b] && dev_grow(ctx, &ctx->brid[b], oc, nc)) return DEM_ERR_HIP;
         if (ctx->mf.counter[b] && dev_grow(ctx, &ctx->mf.counter[b], oc, nc)) return DEM_ERR_HIP;
     }
     if (dev_grow(ctx, &ctx->f4, oc, nc) || dev_grow(ctx, &ctx->t4, oc, nc) || dev_grow(ctx, &ctx->xhold, oc, nc)
@@ -1599,7 +1617,8 @@ static int grow_particles(dem_ctx *ctx, int c)
 // which keep their slots, so the previous list and the mesh rows need only
 // empty rows for the new atoms to carry every history at the next rebuild.
 static int append_atoms(dem_ctx *ctx, int m, const double *x, const double *v, const double *omega,
-                        const double *radius, const double *rmass, const int *type, const int *tag, const int *gm)
+                        const double *radius, const double *rmass, const int *type, const int *tag, const int *gm,
+                        const double *brid = nullptr)
 {
     if (m == 0) return DEM_OK;
     if (m < 0 || !x || !v || !omega || !radius || !rmass || !type || !tag)
@@ -1653,6 +1672,20 @@ static int append_atoms(dem_ctx *ctx, int m, const double *x, const double *v, c
     HIPCHK(hipMemsetAsync(ctx->wmask + n0, 0, m, s));
     // the counter property's default for a new atom: not seen yet
     if (ctx->mf.mesh >= 0) HIPCHK(hipMemsetAsync(ctx->mf.counter[c] + n0, 0xff, sizeof(int) * m, s));
+    // bond_random_id: the clump's, or the property's default -1
+    if (brid && !ctx->brid[0]) {
+        for (int b = 0; b < 2; b++)
+            if (dev_alloc(ctx, &ctx->brid[b], (size_t)ctx->cap)) return DEM_ERR_HIP;
+        std::vector<double> minus(std::max(n0, 1), -1.0);
+        if (n0) HIPCHK(hipMemcpyAsync(ctx->brid[c], minus.data(), n0 * sizeof(double), hipMemcpyHostToDevice, s));
+        HIPCHK(hipStreamSynchronize(s));
+    }
+    if (ctx->brid[c]) {
+        std::vector<double> hb(m, -1.0);
+        if (brid) hb.assign(brid, brid + m);
+        HIPCHK(hipMemcpyAsync(ctx->brid[c] + n0, hb.data(), m * sizeof(double), hipMemcpyHostToDevice, s));
+        HIPCHK(hipStreamSynchronize(s));
+    }
     if (ctx->have_list) launch_extend_rows(ctx->nstart[ctx->lcur], n0, m, s);
     for (auto &mh : ctx->meshes) {
         if (!mh.have) continue;
@@ -1790,8 +1823,9 @@ static int run_inserts(dem_ctx *ctx)
         if (rc) return fail(ctx, rc, "fix insert/pack: " + err);
         const int m = (int)out.size();
         if (m == 0) continue;
-        std::vector<double> x(3 * m), v(3 * m), w(3 * m), r(m), ms(m);
+        std::vector<double> x(3 * m), v(3 * m), w(3 * m), r(m), ms(m), br(m);
         std::vector<int> ty(m), tg(m), gm(m, in->gmask);
+        bool any_brid = false;
         for (int k = 0; k < m; k++) {
             for (int d = 0; d < 3; d++) {
                 x[3 * k + d] = out[k].x[d];
@@ -1801,11 +1835,13 @@ static int run_inserts(dem_ctx *ctx)
             r[k] = out[k].radius;
             ms[k] = out[k].mass;
             ty[k] = out[k].type;
+            br[k] = out[k].brid;
+            any_brid = any_brid || out[k].brid >= 0.0;
             // Atom::tag_extend (atom.cpp:574-595): after the largest tag
             tg[k] = ctx->tag_max + 1 + k;
         }
         const int a = append_atoms(ctx, m, x.data(), v.data(), w.data(), r.data(), ms.data(), ty.data(), tg.data(),
-                                   gm.data());
+                                   gm.data(), any_brid ? br.data() : nullptr);
         if (a) return a;
     }
     ctx->ins_grid_ready = false;
@@ -1826,6 +1862,8 @@ extern "C" void dem_insert_pack_defaults(dem_insert_pack_spec *s)
     s->ntry_mc = 100000;
     s->exact_number = 1;
     s->vel_style = DEM_VEL_CONSTANT;
+    s->orientation = DEM_ORIENT_TEMPLATE;
+    s->quat[0] = 1.0;
 }
 
 extern "C" int dem_add_insert_pack(dem_ctx *ctx, const dem_insert_pack_spec *spec, int *id)
@@ -1834,11 +1872,22 @@ extern "C" int dem_add_insert_pack(dem_ctx *ctx, const dem_insert_pack_spec *spe
     if (!spec || !id) return fail(ctx, DEM_ERR_ARG, "dem_add_insert_pack: bad arguments");
     if (!ctx->mat_set) return fail(ctx, DEM_ERR_ARG, "dem_add_insert_pack: set materials first");
     if (ctx->ninst > 1) return fail(ctx, DEM_ERR_ARG, "dem_add_insert_pack: not in a batched run");
-    for (int t = 0; t < spec->ntemplates && spec->templates; t++)
-        if (spec->templates[t].atom_type < 1 || spec->templates[t].atom_type > ctx->ntypes)
+    std::vector<dem_ins::TemplateMultisphere *> ms(std::max(spec->ntemplates, 0), nullptr);
+    for (int t = 0; t < spec->ntemplates && spec->templates; t++) {
+        const int mid = spec->multisphere ? spec->multisphere[t] : 0;
+        if (mid) {
+            for (auto &tp : ctx->templates)
+                if (tp->id == mid) ms[t] = &tp->t;
+            if (!ms[t]) return fail(ctx, DEM_ERR_ARG, "dem_add_insert_pack: no such multisphere template");
+            for (int j = 0; j < ms[t]->nspheres(); j++)
+                if (ms[t]->type(j) < 1 || ms[t]->type(j) > ctx->ntypes)
+                    return fail(ctx, DEM_ERR_ARG, "dem_add_insert_pack: template atom type out of range");
+        } else if (spec->templates[t].atom_type < 1 || spec->templates[t].atom_type > ctx->ntypes) {
             return fail(ctx, DEM_ERR_ARG, "dem_add_insert_pack: template atom type out of range");
+        }
+    }
     auto in = std::make_unique<dem_ctx::Insert>();
-    const std::string err = in->fix.init(*spec, ctx->p.boxlo, ctx->p.boxhi, ctx->ntimestep);
+    const std::string err = in->fix.init(*spec, ctx->p.boxlo, ctx->p.boxhi, ctx->ntimestep, ms);
     if (!err.empty()) return fail(ctx, DEM_ERR_ARG, "fix insert/pack: " + err);
     in->id = ctx->next_op_id++;
     in->gmask = spec->group_mask;
@@ -1972,10 +2021,11 @@ extern "C" int dem_create_particles_single(dem_ctx *ctx, int tmpl, const double 
     }
     // FixTemplateMultiplespheres::randomize_single draws the clump's
     // bond_random_id (fix_template_multiplespheres.cpp:600-615)
-    if (t.bonded()) tp->t.bond_id(ctx->ntimestep);
+    std::vector<double> bid;
+    if (t.bonded()) bid.assign(m, tp->t.bond_id(ctx->ntimestep));
     sync_enter(ctx);
     const int rc = append_atoms(ctx, m, xs.data(), vs.data(), ws.data(), r.data(), ms.data(), ty.data(), tg.data(),
-                                gm.data());
+                                gm.data(), bid.empty() ? nullptr : bid.data());
     if (rc) return rc;
     ctx->next_reneighbor = ctx->ntimestep + 1;
     return m;

@@ -213,7 +213,7 @@ void TemplateMultisphere::place(const double *x, const double *quat, std::vector
 }
 
 std::string InsertPack::init(const dem_insert_pack_spec &s, const double *boxlo, const double *boxhi,
-                             long long ntimestep)
+                             long long ntimestep, const std::vector<TemplateMultisphere *> &ms)
 {
     s_ = s;
     if (s.seed <= 0 || s.dist_seed <= 0) return "illegal seed";
@@ -236,20 +236,33 @@ std::string InsertPack::init(const dem_insert_pack_spec &s, const double *boxlo,
         if (!(s.region_lo[d] < s.region_hi[d])) return "region block bounds must be lo < hi";
     }
     const int T = s.ntemplates;
+    if (s.orientation < DEM_ORIENT_TEMPLATE || s.orientation > DEM_ORIENT_CONSTANT) return "bad orientation";
     templ_.assign(s.templates, s.templates + T);
     distweight_.assign(s.weights, s.weights + T);
+    ms_ = ms;
+    ms_.resize(T, nullptr);
     volexp_.resize(T);
     massexp_.resize(T);
+    rbound_.resize(T);
     for (int i = 0; i < T; i++) {
+        if (distweight_[i] < 0.0) return "invalid weight";
+        if (ms_[i]) {
+            // FixTemplateMultiplespheres: the Monte Carlo volume and mass,
+            // the bounding sphere's radius
+            volexp_[i] = ms_[i]->volume();
+            massexp_[i] = ms_[i]->mass();
+            rbound_[i] = ms_[i]->r_bound();
+            continue;
+        }
         const auto &t = templ_[i];
         if (!(t.radius > 0.0) || !(t.density > 0.0)) return "template radius and density must be > 0";
         if (t.atom_type < 1) return "template atom type must be >= 1";
-        if (distweight_[i] < 0.0) return "invalid weight";
         // FixTemplateSphere: volume_expect = cubic_expectancy * 4 pi / 3,
         // mass_expect = expectancy(density) * volume_expect
         // (fix_template_sphere.cpp:243-245; constant pdfs: mu^3 and mu)
         volexp_[i] = t.radius * t.radius * t.radius * 4. * M_PI / 3.;
         massexp_[i] = t.density * volexp_[i];
+        rbound_[i] = t.radius;
     }
     // FixParticledistributionDiscrete (fix_particledistribution_discrete.cpp:126-173)
     double wsum = 0.0;
@@ -284,20 +297,24 @@ std::string InsertPack::init(const dem_insert_pack_spec &s, const double *boxlo,
             n--;
         } while (swapped && n > 0);
     }
-    // the distinct radii: one overlap bit each
+    // the distinct sphere radii: one overlap bit each
     radii_.clear();
-    radius_bit_.resize(T);
+    radius_bit_.assign(T, {});
     for (int i = 0; i < T; i++) {
-        int b = -1;
-        for (size_t k = 0; k < radii_.size(); k++)
-            if (radii_[k] == templ_[i].radius) b = (int)k;
-        if (b < 0) {
-            b = (int)radii_.size();
-            radii_.push_back(templ_[i].radius);
+        const int ns = ms_[i] ? ms_[i]->nspheres() : 1;
+        for (int j = 0; j < ns; j++) {
+            const double rj = ms_[i] ? ms_[i]->radius(j) : templ_[i].radius;
+            int b = -1;
+            for (size_t k = 0; k < radii_.size(); k++)
+                if (radii_[k] == rj) b = (int)k;
+            if (b < 0) {
+                b = (int)radii_.size();
+                radii_.push_back(rj);
+            }
+            radius_bit_[i].push_back(b);
         }
-        radius_bit_[i] = b;
     }
-    if (radii_.size() > 32) return "at most 32 distinct template radii";
+    if (radii_.size() > 32) return "at most 32 distinct sphere radii";
 
     random_.reset(s.seed);
     dist_random_.reset(s.dist_seed);
@@ -310,8 +327,9 @@ std::string InsertPack::init(const dem_insert_pack_spec &s, const double *boxlo,
     // Region::reset_random(seed + SEED_OFFSET), then Region::volume_mc
     // (region.cpp:633-700) on the region's stream
     reg_random_.reset(s.seed + kSeedOffset);
+    // FixParticledistributionDiscrete::max_r_bound
     double maxrb = 0.0;
-    for (const auto &t : templ_) maxrb = std::max(maxrb, t.radius);
+    for (double rb : rbound_) maxrb = std::max(maxrb, rb);
     long long n_in = 0, n_in_local = 0;
     const Block &r = region_;
     for (int i = 0; i < s.ntry_mc; i++) {
@@ -354,14 +372,26 @@ std::string InsertPack::init(const dem_insert_pack_spec &s, const double *boxlo,
 int InsertPack::max_type() const
 {
     int m = 0;
-    for (const auto &t : templ_) m = std::max(m, t.atom_type);
+    for (size_t i = 0; i < templ_.size(); i++)
+        if (ms_[i])
+            for (int j = 0; j < ms_[i]->nspheres(); j++) m = std::max(m, ms_[i]->type(j));
+        else
+            m = std::max(m, templ_[i].atom_type);
     return m;
 }
 
+// the largest and smallest sphere radius of any template
 double InsertPack::max_radius() const
 {
     double m = 0.0;
-    for (const auto &t : templ_) m = std::max(m, t.radius);
+    for (double r : radii_) m = std::max(m, r);
+    return m;
+}
+
+double InsertPack::min_radius() const
+{
+    double m = 1e300;
+    for (double r : radii_) m = std::min(m, r);
     return m;
 }
 
@@ -411,7 +441,7 @@ int InsertPack::calc_ninsert_this(const Hooks &h, std::string &err)
 // FixParticledistributionDiscrete::randomize_list
 // (fix_particledistribution_discrete.cpp:388-470): how many of each
 // template, then the pti list large to small
-int InsertPack::randomize_list(int ninsert)
+int InsertPack::randomize_list(int ninsert, long long step)
 {
     const int T = s_.ntemplates;
     std::vector<int> parttogen(T, 0);
@@ -438,18 +468,37 @@ int InsertPack::randomize_list(int ninsert)
             parttogen[j]++;
         }
     }
+    // each template's randomize_ptilist, in template order: a bonded
+    // clump template draws one bond_random_id per body
+    // (fix_template_multiplespheres.cpp:650-672)
+    std::vector<std::vector<double>> ids(T);
+    for (int i = 0; i < T; i++)
+        if (ms_[i] && ms_[i]->bonded())
+            for (int k = 0; k < parttogen[i]; k++) ids[i].push_back(ms_[i]->bond_id(step));
+    // the lists wired large to small
     pti_.clear();
     for (int i = 0; i < T; i++) {
         const int c = distorder_[i];
-        const auto &t = templ_[c];
         Pti p;
         p.templ = c;
-        p.type = t.atom_type;
-        p.radius = t.radius;
-        p.density = t.density;
-        // FixTemplateSphere::randomize_ptilist (fix_template_sphere.cpp:352-354)
-        p.mass = p.density * (t.radius * t.radius * t.radius * 4. * M_PI / 3.);
-        for (int k = 0; k < parttogen[c]; k++) pti_.push_back(p);
+        p.brid = -1.0;
+        if (ms_[c]) {
+            p.type = 0;
+            p.radius = ms_[c]->r_bound();
+            p.density = ms_[c]->density();
+            p.mass = ms_[c]->mass();
+        } else {
+            const auto &t = templ_[c];
+            p.type = t.atom_type;
+            p.radius = t.radius;
+            p.density = t.density;
+            // FixTemplateSphere::randomize_ptilist (fix_template_sphere.cpp:352-354)
+            p.mass = p.density * (t.radius * t.radius * t.radius * 4. * M_PI / 3.);
+        }
+        for (int k = 0; k < parttogen[c]; k++) {
+            if (!ids[c].empty()) p.brid = ids[c][k];
+            pti_.push_back(p);
+        }
     }
     return (int)pti_.size();
 }
@@ -498,17 +547,19 @@ bool InsertPack::border_ok(const double *pos, double rbound) const
 }
 
 namespace {
-// The particles placed in this insertion, binned for the overlap test
-// among themselves (the reference adds each to its RegionNeighborList).
+// The spheres placed in this insertion, binned for the overlap test among
+// themselves (the reference adds each to its RegionNeighborList).
 struct NewGrid {
     double h = 1.0;
     std::unordered_map<int64_t, std::vector<int>> cells;
-    const std::vector<Placed> *pts = nullptr;
+    std::vector<double> xs, rs;
     static int64_t key(int64_t a, int64_t b, int64_t c) { return (a * 2097152 + b) * 2097152 + c; }
     int64_t ci(double v) const { return (int64_t)std::floor(v / h) + 1048576; }
-    void add(int idx)
+    void add(const double *x, double r)
     {
-        const double *x = (*pts)[idx].x;
+        const int idx = (int)rs.size();
+        xs.insert(xs.end(), x, x + 3);
+        rs.push_back(r);
         cells[key(ci(x[0]), ci(x[1]), ci(x[2]))].push_back(idx);
     }
     bool overlaps(const double *x, double radius) const
@@ -520,63 +571,96 @@ struct NewGrid {
                     auto it = cells.find(key(a, b, c));
                     if (it == cells.end()) continue;
                     for (int j : it->second) {
-                        const Placed &p = (*pts)[j];
                         double del[3];
-                        del[0] = x[0] - p.x[0];
-                        del[1] = x[1] - p.x[1];
-                        del[2] = x[2] - p.x[2];
+                        del[0] = x[0] - xs[3 * j];
+                        del[1] = x[1] - xs[3 * j + 1];
+                        del[2] = x[2] - xs[3 * j + 2];
                         const double rsq = del[0] * del[0] + del[1] * del[1] + del[2] * del[2];
-                        const double radsum = radius + p.radius;
+                        const double radsum = radius + rs[j];
                         if (rsq <= radsum * radsum) return true;
                     }
                 }
         return false;
     }
 };
+
+// MathExtraLiggghts::random_unit_quat (math_extra_liggghts.h:616-629)
+void random_unit_quat(RanPark &r, double *q)
+{
+    const double u1 = r.uniform();
+    const double u2 = r.uniform();
+    const double u3 = r.uniform();
+    const double h1 = std::sqrt(1. - u1);
+    const double h2 = std::sqrt(u1);
+    q[0] = h1 * std::sin(2. * M_PI * u2);
+    q[1] = h1 * std::cos(2. * M_PI * u2);
+    q[2] = h2 * std::sin(2. * M_PI * u3);
+    q[3] = h2 * std::cos(2. * M_PI * u3);
+}
 }  // namespace
+
+// The body's spheres at x with orientation q: a sphere at x itself
+// (particleToInsert.cpp:252), a clump's at x + rotate(x_j - x_bound, q)
+// (particleToInsert.cpp:286-296)
+void InsertPack::spheres_of(const Pti &p, const double *x, const double *q, std::vector<double> &xs) const
+{
+    if (ms_[p.templ]) {
+        ms_[p.templ]->place(x, q, xs);
+        return;
+    }
+    xs.assign(x, x + 3);
+}
 
 // FixInsertPack::x_v_omega (fix_insert_pack.cpp:498-605).  With the overlap
 // check, attempts are drawn ahead in batches from copies of the streams and
 // tested on the device together; the serial walk over them then takes the
 // reference's decisions in order, and the streams are set to the state after
-// the last attempt it used.  A batch is drawn for one bounding radius: after
-// an insertion whose next particle has another radius the rest of the batch
-// is dropped and drawn again.
-int InsertPack::x_v_omega(const Hooks &h, int ninsert, std::vector<Placed> &out, std::string &err)
+// the last attempt it used.  A batch is drawn for one template: after an
+// insertion whose next body is of another template the rest of the batch is
+// dropped and drawn again (positions depend on the bounding radius, the
+// spheres tested on the template).
+int InsertPack::x_v_omega(const Hooks &h, int ninsert, std::vector<Body> &out, std::string &err)
 {
     out.clear();
     last_tries_ = 0;
+    // quat_insert: identity, or the constant orientation (fix_insert.cpp:468, :316-330)
+    double qins[4] = {1.0, 0.0, 0.0, 0.0};
+    if (s_.orientation == DEM_ORIENT_CONSTANT)
+        for (int k = 0; k < 4; k++) qins[k] = s_.quat[k];
+    const bool qrand = s_.orientation == DEM_ORIENT_RANDOM;
     if (!s_.overlapcheck) {
         for (int k = 0; k < ninsert; k++) {
-            const Pti &p = pti_[k];
-            Placed q;
-            gen_pos(q.x, p.radius);
-            gen_velocity(q.v);
-            for (int d = 0; d < 3; d++) q.omega[d] = s_.omega[d];
-            q.radius = p.radius;
-            q.mass = p.mass;
-            q.type = p.type;
-            out.push_back(q);
+            const Pti &p = pti_[out.size()];
+            Body b;
+            b.pti = k;
+            gen_pos(b.x, rbound_[p.templ]);
+            gen_velocity(b.v);
+            if (qrand) random_unit_quat(random_, qins);
+            for (int d = 0; d < 4; d++) b.q[d] = qins[d];
+            out.push_back(b);
         }
         return DEM_OK;
     }
     NewGrid ng;
     ng.h = 2.0 * max_radius();
-    ng.pts = &out;
     int ntry = 0;
     const int maxtry = maxtry_;
     std::vector<Attempt> batch;
-    std::vector<double> cand;
+    std::vector<double> cand, xs;
     std::vector<uint32_t> hit;
     while (ntry < maxtry && (int)out.size() < ninsert) {
-        const double rb = pti_[out.size()].radius;
+        const Pti &p0 = pti_[out.size()];
+        const int templ = p0.templ;
+        const double rb = rbound_[templ];
+        const int ns = nsph(p0);
         // draw a batch of attempts from copies of the streams
         const int left = ninsert - (int)out.size();
-        const int want = (int)std::min<long long>(std::max(1024, 4 * left), 1 << 20);
+        const int want = (int)std::min<long long>(std::max(1024, 4 * left), std::max(1, (1 << 20) / ns));
         batch.clear();
         cand.clear();
         RanPark reg = reg_random_, fix = random_;
         int nt = ntry;
+        double q[4] = {qins[0], qins[1], qins[2], qins[3]};
         while ((int)batch.size() < want && nt < maxtry) {
             Attempt a;
             std::swap(reg_random_, reg);
@@ -586,15 +670,21 @@ int InsertPack::x_v_omega(const Hooks &h, int ninsert, std::vector<Placed> &out,
                 nt++;
             } while (s_.check_dist_from_subdomain_border && nt < maxtry && !border_ok(a.x, rb));
             a.last = nt == maxtry;
-            if (!a.last) gen_velocity(a.v);
+            if (!a.last) {
+                gen_velocity(a.v);
+                if (qrand) random_unit_quat(random_, q);
+            }
             std::swap(reg_random_, reg);
             std::swap(random_, fix);
+            for (int d = 0; d < 4; d++) a.q[d] = q[d];
             a.reg = reg;
             a.fix = fix;
             a.ntry = nt;
+            a.cand = (int)(cand.size() / 3);
             batch.push_back(a);
             if (a.last) break;
-            cand.insert(cand.end(), a.x, a.x + 3);
+            spheres_of(p0, a.x, a.q, xs);
+            cand.insert(cand.end(), xs.begin(), xs.end());
         }
         hit.assign(cand.size() / 3, 0u);
         if (!cand.empty() && h.overlap(cand, radii_, hit)) {
@@ -608,20 +698,20 @@ int InsertPack::x_v_omega(const Hooks &h, int ninsert, std::vector<Placed> &out,
             const Attempt &a = batch[k];
             if (a.last) break;
             const Pti &p = pti_[out.size()];
-            const bool taken = (hit[k] >> radius_bit_[p.templ]) & 1u;
-            if (taken || ng.overlaps(a.x, p.radius)) continue;
-            Placed q;
+            bool taken = false;
+            for (int j = 0; j < ns && !taken; j++) taken = (hit[a.cand + j] >> radius_bit_[templ][j]) & 1u;
+            for (int j = 0; j < ns && !taken; j++) taken = ng.overlaps(&cand[3 * (size_t)(a.cand + j)], sph_radius(p, j));
+            if (taken) continue;
+            Body b;
+            b.pti = (int)out.size();
             for (int d = 0; d < 3; d++) {
-                q.x[d] = a.x[d];
-                q.v[d] = a.v[d];
-                q.omega[d] = s_.omega[d];
+                b.x[d] = a.x[d];
+                b.v[d] = a.v[d];
             }
-            q.radius = p.radius;
-            q.mass = p.mass;
-            q.type = p.type;
-            out.push_back(q);
-            ng.add((int)out.size() - 1);
-            if ((int)out.size() == ninsert || pti_[out.size()].radius != rb) {
+            for (int d = 0; d < 4; d++) b.q[d] = a.q[d];
+            out.push_back(b);
+            for (int j = 0; j < ns; j++) ng.add(&cand[3 * (size_t)(a.cand + j)], sph_radius(p, j));
+            if ((int)out.size() == ninsert || pti_[out.size()].templ != templ) {
                 redraw = true;
                 break;
             }
@@ -629,6 +719,8 @@ int InsertPack::x_v_omega(const Hooks &h, int ninsert, std::vector<Placed> &out,
         const Attempt &used = batch[std::min(k, batch.size() - 1)];
         reg_random_ = used.reg;
         random_ = used.fix;
+        if (qrand)
+            for (int d = 0; d < 4; d++) qins[d] = used.q[d];
         ntry = used.ntry;
         if (!redraw && k < batch.size() && batch[k].last) break;   // maxtry reached
     }
@@ -650,7 +742,7 @@ int InsertPack::insert(const Hooks &h, long long step, std::vector<Placed> &out,
         const double fraction_local = region_volume_local_ / region_volume_;
         ninsert_this = static_cast<int>(fraction_local * static_cast<double>(ninsert_this) + random_.uniform());
     }
-    ninsert_this = randomize_list(ninsert_this);
+    ninsert_this = randomize_list(ninsert_this, step);
     last_aimed_ = ninsert_this;
     last_inserted_ = 0;
     last_tries_ = 0;
@@ -662,12 +754,40 @@ int InsertPack::insert(const Hooks &h, long long step, std::vector<Placed> &out,
     // FixInsertPack::calc_maxtry (fix_insert_pack.cpp:485-490)
     if (insertion_ratio_ >= 1.) maxtry_ = ninsert_this * s_.maxattempt;
     else maxtry_ = static_cast<int>(static_cast<double>(ninsert_this * s_.maxattempt) / (1. - insertion_ratio_));
-    const int rc = x_v_omega(h, ninsert_this, out, err);
+    std::vector<Body> bodies;
+    const int rc = x_v_omega(h, ninsert_this, bodies, err);
     if (rc) return rc;
-    last_inserted_ = (int)out.size();
-    ninserted_ += (long long)out.size();
+    // FixParticledistributionDiscrete::insert -> ParticleToInsert::insert
+    // (particleToInsert.cpp:97-150): the bodies' spheres in order; a single
+    // sphere has the body's mass, a clump's sphere 4.18879020479 r^3 rho
+    std::vector<double> xs;
     double mass_this = 0.0;
-    for (size_t k = 0; k < out.size(); k++) mass_this += pti_[k].mass;
+    for (const Body &b : bodies) {
+        const Pti &p = pti_[b.pti];
+        mass_this += p.mass;
+        spheres_of(p, b.x, b.q, xs);
+        const int ns = nsph(p);
+        for (int j = 0; j < ns; j++) {
+            Placed a;
+            for (int d = 0; d < 3; d++) {
+                a.x[d] = xs[3 * j + d];
+                a.v[d] = b.v[d];
+                a.omega[d] = s_.omega[d];
+            }
+            a.radius = sph_radius(p, j);
+            if (ms_[p.templ]) {
+                a.mass = 4.18879020479 * a.radius * a.radius * a.radius * p.density;
+                a.type = ms_[p.templ]->type(j);
+            } else {
+                a.mass = p.mass;
+                a.type = p.type;
+            }
+            a.brid = p.brid;
+            out.push_back(a);
+        }
+    }
+    last_inserted_ = (int)bodies.size();
+    ninserted_ += (long long)bodies.size();
     massinserted_ += mass_this;
     if (s_.insert_every) next_reneighbor_ += s_.insert_every;
     else next_reneighbor_ = 0;

@@ -40,19 +40,24 @@ struct Block {
     bool near_face(const double *x, double cut) const;
 };
 
-// One particle of the pti list (ParticleToInsert, single sphere).
+// One body of the pti list (ParticleToInsert): a sphere of a
+// particletemplate/sphere, or a clump of a particletemplate/multiplespheres.
 struct Pti {
     int templ;        // index in the spec's template order
     int type;
-    double radius;
+    double radius;    // a sphere's radius; a clump's bounding radius
     double density;
-    double mass;      // density * r^3 * 4 pi / 3 (fix_template_sphere.cpp:352-354)
+    double mass;      // mass_ins: density * r^3 * 4 pi / 3 (fix_template_sphere.cpp:352-354),
+                      // or the clump's Monte Carlo mass
+    double brid;      // bond_random_id of a bonded clump, -1 else
 };
 
+// One created atom.
 struct Placed {
     double x[3], v[3], omega[3];
     double radius, mass;
     int type;
+    double brid;
 };
 
 // What the placement needs from the resident particles.
@@ -91,6 +96,9 @@ class TemplateMultisphere {
     // bond_random_id of a clump created at `step`: step + u (random_insertion)
     double bond_id(long long step) { return static_cast<double>(step) + random_insertion_.uniform(); }
     void info(double *out8) const;
+    double r_bound() const { return r_bound_; }
+    double volume() const { return volume_expect_; }
+    double mass() const { return mass_expect_; }
 
   private:
     dem_template_multisphere_spec s_{};
@@ -105,7 +113,9 @@ class InsertPack {
   public:
     // Checks the spec, builds the distribution and runs the region volume
     // estimate; returns an error text, empty on success.
-    std::string init(const dem_insert_pack_spec &s, const double *boxlo, const double *boxhi, long long ntimestep);
+    // ms[i]: the multisphere template of template i, or null for a sphere
+    std::string init(const dem_insert_pack_spec &s, const double *boxlo, const double *boxhi, long long ntimestep,
+                     const std::vector<TemplateMultisphere *> &ms);
     long long next_step() const { return next_reneighbor_; }
     bool due(long long step) const { return next_reneighbor_ > 0 && step == next_reneighbor_; }
     // One FixInsert::pre_exchange: the particles to create, in creation
@@ -114,27 +124,38 @@ class InsertPack {
     void vector(double *out6) const;
     int max_type() const;
     double max_radius() const;
+    double min_radius() const;
 
   private:
     struct Attempt {
-        double x[3], v[3];
+        double x[3], v[3], q[4];
         RanPark reg, fix;    // the streams after this attempt
         int ntry;            // tries counted after it
         bool last;           // reached maxtry: no velocity, no test
+        int cand;            // its first sphere's candidate index
+    };
+    struct Body {
+        int pti;
+        double x[3], v[3], q[4];
     };
     int calc_ninsert_this(const Hooks &h, std::string &err);
-    int randomize_list(int ntotal);
+    int randomize_list(int ntotal, long long step);
     void gen_pos(double *pos, double rbound);
     void gen_velocity(double *v);
     bool border_ok(const double *pos, double rbound) const;
-    int x_v_omega(const Hooks &h, int ninsert, std::vector<Placed> &out, std::string &err);
+    int x_v_omega(const Hooks &h, int ninsert, std::vector<Body> &out, std::string &err);
+    void spheres_of(const Pti &p, const double *x, const double *q, std::vector<double> &xs) const;
+    int nsph(const Pti &p) const { return ms_[p.templ] ? ms_[p.templ]->nspheres() : 1; }
+    double sph_radius(const Pti &p, int j) const { return ms_[p.templ] ? ms_[p.templ]->radius(j) : p.radius; }
 
     dem_insert_pack_spec s_{};
     std::vector<dem_template_sphere> templ_;
     std::vector<double> distweight_, volexp_, massexp_;
     std::vector<int> distorder_;
-    std::vector<double> radii_;        // distinct template radii, the overlap bit order
-    std::vector<int> radius_bit_;      // per template: its bit
+    std::vector<TemplateMultisphere *> ms_;
+    std::vector<double> rbound_;       // per template: bounding radius
+    std::vector<double> radii_;        // distinct sphere radii, the overlap bit order
+    std::vector<std::vector<int>> radius_bit_;   // per template and sphere: its bit
     double volexpect_ = 0.0, massexpect_ = 0.0;
     double boxlo_[3] = {0, 0, 0}, boxhi_[3] = {0, 0, 0};
     Block region_{};

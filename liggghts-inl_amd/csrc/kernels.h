@@ -48,6 +48,10 @@ struct BondArgs {
     double *bout;
     uint8_t *bvalid;
     const double *mat;
+    // bond_random_id per atom, or null (FixPropertyAtom bond_random_id,
+    // default -1: bonds of rule B form only between spheres of one clump at
+    // the step it was inserted)
+    const double *brid;
     int n, shearupdate;
     long long ntimestep;
     Model M;
@@ -81,6 +85,8 @@ struct PermuteArgs {
     double4 *xo1;
     const int *mf0;      // fix massflow/mesh side counters (may be null)
     int *mf1;
+    const double *br0;   // bond_random_id per atom (may be null)
+    double *br1;
     uint8_t *wmask;
     int n;
     int wdn;             // nplanes * dnum

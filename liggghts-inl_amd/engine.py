@@ -89,6 +89,9 @@ class InsertPackSpec(C.Structure):
         ("vel_fluct", C.c_double * 3),
         ("omega", C.c_double * 3),
         ("group_mask", C.c_int),
+        ("multisphere", C.POINTER(C.c_int)),
+        ("orientation", C.c_int),
+        ("quat", C.c_double * 4),
     ]
 
 
@@ -119,7 +122,9 @@ def insert_pack_spec(d):
     particles_in_region / mass_in_region, maxattempt, insert_every ("once"
     or steps), start, overlapcheck, all_in, check_dist_from_subdomain_border,
     ntry_mc, random_distribute ("exact" / "uncorrelated"), vel ("constant",
-    v) / ("uniform" | "gaussian", v, fluct), omega, group_mask.
+    v) / ("uniform" | "gaussian", v, fluct), omega, group_mask, multisphere
+    (per template an add_template_multisphere id, 0 for a sphere),
+    orientation ("template", "random" or ("constant", q)).
 
     Returns (spec, keep): keep holds the arrays the spec points to."""
     L = library()
@@ -164,7 +169,15 @@ def insert_pack_spec(d):
         s.vel_fluct[k] = float(vel[2][k]) if len(vel) > 2 else 0.0
         s.omega[k] = float(d.get("omega", (0.0, 0.0, 0.0))[k])
     s.group_mask = int(d.get("group_mask", 0))
-    return s, (tarr, w)
+    # multisphere: per template an add_template_multisphere id, 0 for a sphere
+    ms = np.ascontiguousarray(d.get("multisphere", [0] * len(tl)), dtype=np.int32)
+    s.multisphere = ms.ctypes.data_as(C.POINTER(C.c_int))
+    o = d.get("orientation", "template")
+    s.orientation = {"template": 0, "random": 1}.get(o, 2) if isinstance(o, str) else 2
+    q = o[1] if not isinstance(o, str) else (1.0, 0.0, 0.0, 0.0)
+    for k in range(4):
+        s.quat[k] = float(q[k])
+    return s, (tarr, w, ms)
 
 
 COHESION = {"off": 0, "bond": 1, "bond/nonlinear": 2}

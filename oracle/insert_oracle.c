@@ -67,11 +67,193 @@ void oi_park_states(int seed, int n, int *states, double *values)
     }
 }
 
+/* ---- fix particletemplate/multiplespheres (fix_template_multiplespheres.cpp) ---- */
+struct oi_tmpl {
+    int n;
+    double *x, *r;
+    int *types, atom_type, bonded;
+    double density;
+    double x_min[3], x_max[3], x_bound[3], r_bound, xcm[3], volume;
+    park rins;   /* random_insertion: the seed plus the process shift 0 */
+};
+
+oi_tmpl *oi_tmpl_create(int seed, int nspheres, const double *x, const double *r, const int *types, int atom_type,
+                        double density, double scale, int ntry, int bonded)
+{
+    oi_tmpl *t = (oi_tmpl *)calloc(1, sizeof(oi_tmpl));
+    const int n = nspheres;
+    t->n = n;
+    t->atom_type = atom_type;
+    t->density = density;
+    t->bonded = bonded;
+    t->types = NULL;
+    if (types) {
+        t->types = (int *)malloc(sizeof(int) * n);
+        memcpy(t->types, types, sizeof(int) * n);
+    }
+    park_reset(&t->rins, seed);
+    t->x = (double *)malloc(sizeof(double) * 3 * n);
+    t->r = (double *)malloc(sizeof(double) * n);
+    /* constructor, file branch (:118-150): r *= scale*cg, x = (scale*cg) x,
+     * then the bounding box; cg = 1 */
+    for (int d = 0; d < 3; d++) {
+        t->x_min[d] = 1e8;
+        t->x_max[d] = -1e8;
+    }
+    for (int i = 0; i < n; i++) {
+        t->r[i] = r[i] * (scale * 1.0);
+        for (int j = 0; j < 3; j++) t->x[3 * i + j] = (scale * 1.0) * x[3 * i + j];
+    }
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < 3; j++) {
+            if (t->x[3 * i + j] - t->r[i] < t->x_min[j]) t->x_min[j] = t->x[3 * i + j] - t->r[i];
+            if (t->x[3 * i + j] + t->r[i] > t->x_max[j]) t->x_max[j] = t->x[3 * i + j] + t->r[i];
+        }
+    park mc;
+    park_reset(&mc, seed);   /* random_mc: the seed, no process shift */
+
+    /* calc_bounding_sphere (:321-375) */
+    t->r_bound = 1e8;
+    int *visited = (int *)malloc(sizeof(int) * n);
+    for (int shuffle = 0; shuffle < 200; shuffle++) {
+        for (int i = 0; i < n; i++) visited[i] = 0;
+        int isphere = -1, nvisited = 0;
+        double xbt[3], rbt, d[3], dist;
+        while (isphere < 0 || visited[isphere] || isphere >= n) isphere = (int)(park_uniform(&mc) * n);
+        nvisited++;
+        visited[isphere] = 1;
+        xbt[0] = t->x[3 * isphere];
+        xbt[1] = t->x[3 * isphere + 1];
+        xbt[2] = t->x[3 * isphere + 2];
+        rbt = t->r[isphere];
+        while (nvisited < n) {
+            while (isphere < 0 || visited[isphere] || isphere >= n) isphere = (int)(park_uniform(&mc) * n);
+            nvisited++;
+            visited[isphere] = 1;
+            d[0] = t->x[3 * isphere] - xbt[0];
+            d[1] = t->x[3 * isphere + 1] - xbt[1];
+            d[2] = t->x[3 * isphere + 2] - xbt[2];
+            dist = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+            if (dist + t->r[isphere] > rbt) {
+                const double fact = (dist + t->r[isphere] - rbt) / (2. * dist);
+                d[0] = fact * d[0];
+                d[1] = fact * d[1];
+                d[2] = fact * d[2];
+                xbt[0] = xbt[0] + d[0];
+                xbt[1] = xbt[1] + d[1];
+                xbt[2] = xbt[2] + d[2];
+                rbt += sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+            }
+        }
+        if (rbt < t->r_bound) {
+            t->r_bound = rbt;
+            t->x_bound[0] = xbt[0];
+            t->x_bound[1] = xbt[1];
+            t->x_bound[2] = xbt[2];
+        }
+    }
+    free(visited);
+
+    /* calc_center_of_mass (:410-450), brute force over the spheres */
+    int nsuccess = 0;
+    double xcm[3] = {0., 0., 0.};
+    for (int i = 0; i < ntry; i++) {
+        double xt[3];
+        xt[0] = t->x_min[0] + (t->x_max[0] - t->x_min[0]) * park_uniform(&mc);
+        xt[1] = t->x_min[1] + (t->x_max[1] - t->x_min[1]) * park_uniform(&mc);
+        xt[2] = t->x_min[2] + (t->x_max[2] - t->x_min[2]) * park_uniform(&mc);
+        for (int j = 0; j < n; j++) {
+            double dsq = 0.;
+            dsq += (xt[0] - t->x[3 * j]) * (xt[0] - t->x[3 * j]);
+            dsq += (xt[1] - t->x[3 * j + 1]) * (xt[1] - t->x[3 * j + 1]);
+            dsq += (xt[2] - t->x[3 * j + 2]) * (xt[2] - t->x[3 * j + 2]);
+            if (dsq < t->r[j] * t->r[j]) {
+                xcm[0] = (xcm[0] * (double)nsuccess + xt[0]) / (double)(nsuccess + 1);
+                xcm[1] = (xcm[1] * (double)nsuccess + xt[1]) / (double)(nsuccess + 1);
+                xcm[2] = (xcm[2] * (double)nsuccess + xt[2]) / (double)(nsuccess + 1);
+                nsuccess++;
+                break;
+            }
+        }
+    }
+    t->volume = (double)nsuccess / (double)ntry * (t->x_max[0] - t->x_min[0]) * (t->x_max[1] - t->x_min[1])
+                * (t->x_max[2] - t->x_min[2]);
+    for (int d = 0; d < 3; d++) t->xcm[d] = xcm[d];
+    for (int i = 0; i < n; i++)
+        for (int d = 0; d < 3; d++) t->x[3 * i + d] = t->x[3 * i + d] - xcm[d];
+    for (int d = 0; d < 3; d++) {
+        t->x_min[d] = t->x_min[d] - xcm[d];
+        t->x_max[d] = t->x_max[d] - xcm[d];
+        t->x_bound[d] = t->x_bound[d] - xcm[d];
+    }
+    return t;
+}
+
+void oi_tmpl_destroy(oi_tmpl *t)
+{
+    if (!t) return;
+    free(t->x);
+    free(t->r);
+    free(t->types);
+    free(t);
+}
+
+void oi_tmpl_info(const oi_tmpl *t, double out[9])
+{
+    out[0] = t->r_bound;
+    for (int d = 0; d < 3; d++) out[1 + d] = t->x_bound[d];
+    out[4] = t->volume;
+    out[5] = t->volume * t->density;
+    for (int d = 0; d < 3; d++) out[6 + d] = t->xcm[d];
+}
+
+/* MathExtra::quatquat (math_extra.h:596-602) */
+static void oi_quatquat(const double *a, const double *b, double *c)
+{
+    c[0] = a[0] * b[0] - a[1] * b[1] - a[2] * b[2] - a[3] * b[3];
+    c[1] = a[0] * b[1] + b[0] * a[1] + a[2] * b[3] - a[3] * b[2];
+    c[2] = a[0] * b[2] + b[0] * a[2] + a[3] * b[1] - a[1] * b[3];
+    c[3] = a[0] * b[3] + b[0] * a[3] + a[1] * b[2] - a[2] * b[1];
+}
+
+/* ParticleToInsert::set_x_v_omega (particleToInsert.cpp:330-350) with
+ * MathExtraLiggghts::vec_quat_rotate (math_extra_liggghts.h:457-473), the
+ * orientation as given */
+static void tmpl_place_q(const oi_tmpl *t, const double x[3], const double q[4], double *out)
+{
+    for (int j = 0; j < t->n; j++) {
+        double rel[3], vq[4], qc[4], tmp[4], res[4];
+        for (int d = 0; d < 3; d++) rel[d] = t->x[3 * j + d] - t->x_bound[d];
+        vq[0] = 0.;
+        vq[1] = rel[0];
+        vq[2] = rel[1];
+        vq[3] = rel[2];
+        qc[0] = q[0];
+        qc[1] = -q[1];
+        qc[2] = -q[2];
+        qc[3] = -q[3];
+        oi_quatquat(q, vq, tmp);
+        oi_quatquat(tmp, qc, res);
+        for (int d = 0; d < 3; d++) out[3 * j + d] = res[1 + d] + x[d];
+    }
+}
+
+void oi_tmpl_place(const oi_tmpl *t, const double x[3], const double *quat, double *out)
+{
+    double q[4] = {1.0, 0.0, 0.0, 0.0};
+    if (quat) {
+        /* MathExtraLiggghts::quat_normalize (math_extra_liggghts.h:513-520) */
+        const double norm = 1.0 / sqrt(quat[0] * quat[0] + quat[1] * quat[1] + quat[2] * quat[2] + quat[3] * quat[3]);
+        for (int k = 0; k < 4; k++) q[k] = quat[k] * norm;
+    }
+    tmpl_place_q(t, x, q, out);
+}
+
 /* ---- the fix ---- */
 struct oi_fix {
     oi_spec s;
     park random, region_random, dist_random;
-    double distweight[OI_MAXT], volexp[OI_MAXT], massexp[OI_MAXT];
+    double distweight[OI_MAXT], volexp[OI_MAXT], massexp[OI_MAXT], rbound[OI_MAXT];
     int distorder[OI_MAXT];
     double volexpect, massexpect, max_r_bound;
     double region_volume, region_volume_local, insertion_ratio;
@@ -120,10 +302,19 @@ oi_fix *oi_create(const oi_spec *s, long long ntimestep)
      * and density: volume_expect = r^3 * 4 pi / 3, mass_expect = rho * that */
     f->max_r_bound = 0.0;
     for (int i = 0; i < T; i++) {
-        const double r = s->radius[i];
-        f->volexp[i] = r * r * r * 4. * M_PI / 3.;
-        f->massexp[i] = s->density[i] * f->volexp[i];
-        if (r > f->max_r_bound) f->max_r_bound = r;
+        const oi_tmpl *ms = (const oi_tmpl *)s->ms[i];
+        if (ms) {
+            /* FixTemplateMultiplespheres: Monte Carlo volume and mass, bounding radius */
+            f->volexp[i] = ms->volume;
+            f->massexp[i] = ms->volume * ms->density;
+            f->rbound[i] = ms->r_bound;
+        } else {
+            const double r = s->radius[i];
+            f->volexp[i] = r * r * r * 4. * M_PI / 3.;
+            f->massexp[i] = s->density[i] * f->volexp[i];
+            f->rbound[i] = r;
+        }
+        if (f->rbound[i] > f->max_r_bound) f->max_r_bound = f->rbound[i];
     }
     /* FixParticledistributionDiscrete constructor
      * (fix_particledistribution_discrete.cpp:126-233) */
@@ -306,8 +497,38 @@ static int subbox_dist_ok(const oi_fix *f, const double *pos, double dist)
     return 1;
 }
 
+/* MathExtraLiggghts::random_unit_quat (math_extra_liggghts.h:616-629) */
+static void unit_quat(park *p, double *quat)
+{
+    const double u1 = park_uniform(p);
+    const double u2 = park_uniform(p);
+    const double u3 = park_uniform(p);
+    const double h1 = sqrt(1. - u1);
+    const double h2 = sqrt(u1);
+    quat[0] = h1 * sin(2. * M_PI * u2);
+    quat[1] = h1 * cos(2. * M_PI * u2);
+    quat[2] = h2 * sin(2. * M_PI * u3);
+    quat[3] = h2 * cos(2. * M_PI * u3);
+}
+
+/* the body's spheres: a sphere at pos, a clump through set_x_v_omega's
+ * rotation; returns their number */
+static int body_spheres(const oi_fix *f, int t, const double *pos, const double *q, double *xs, double *rs)
+{
+    const oi_tmpl *ms = (const oi_tmpl *)f->s.ms[t];
+    if (!ms) {
+        memcpy(xs, pos, 3 * sizeof(double));
+        rs[0] = f->s.radius[t];
+        return 1;
+    }
+    tmpl_place_q(ms, pos, q, xs);
+    memcpy(rs, ms->r, ms->n * sizeof(double));
+    return ms->n;
+}
+
 int oi_insert(oi_fix *f, long long step, int n, const double *x, const double *r, const double *m, int cap,
-              double *x_out, double *v_out, double *omega_out, double *r_out, double *m_out, int *type_out)
+              double *x_out, double *v_out, double *omega_out, double *r_out, double *m_out, int *type_out,
+              double *brid_out)
 {
     const oi_spec *s = &f->s;
     /* FixInsert::pre_exchange (fix_insert.cpp:676-888) */
@@ -378,14 +599,30 @@ int oi_insert(oi_fix *f, long long step, int n, const double *x, const double *r
     }
     int npti = 0;
     for (int i = 0; i < T; i++) npti += parttogen[i];
+    /* each template's randomize_ptilist in template order: a bonded clump
+     * template draws a bond_random_id per body (fix_template_multiplespheres.cpp:650-672) */
+    double *ids[OI_MAXT];
+    for (int i = 0; i < T; i++) {
+        ids[i] = NULL;
+        oi_tmpl *ms = (oi_tmpl *)s->ms[i];
+        if (ms && ms->bonded) {
+            ids[i] = (double *)malloc(sizeof(double) * (parttogen[i] > 0 ? parttogen[i] : 1));
+            for (int k = 0; k < parttogen[i]; k++) ids[i][k] = (double)step + park_uniform(&ms->rins);
+        }
+    }
     int *pti_t = (int *)malloc(sizeof(int) * (npti > 0 ? npti : 1));
+    double *pti_b = (double *)malloc(sizeof(double) * (npti > 0 ? npti : 1));
     {
         int k = 0;
         for (int i = 0; i < T; i++) {
             const int c = f->distorder[i];
-            for (int j = 0; j < parttogen[c]; j++) pti_t[k++] = c;
+            for (int j = 0; j < parttogen[c]; j++) {
+                pti_b[k] = ids[c] ? ids[c][j] : -1.0;
+                pti_t[k++] = c;
+            }
         }
     }
+    for (int i = 0; i < T; i++) free(ids[i]);
     ninsert_this_local = npti;
     f->last_aimed = npti;
     f->last_inserted = 0;
@@ -394,6 +631,7 @@ int oi_insert(oi_fix *f, long long step, int n, const double *x, const double *r
         if (s->insert_every) f->next_reneighbor += s->insert_every;
         else f->next_reneighbor = -1;
         free(pti_t);
+        free(pti_b);
         return 0;
     }
 
@@ -402,35 +640,49 @@ int oi_insert(oi_fix *f, long long step, int n, const double *x, const double *r
     if (f->insertion_ratio >= 1.) maxtry = ninsert_this_local * s->maxattempt;
     else maxtry = (int)((double)(ninsert_this_local * s->maxattempt) / (1. - f->insertion_ratio));
 
+    int maxsph = 1;
+    for (int i = 0; i < T; i++)
+        if (s->ms[i] && ((oi_tmpl *)s->ms[i])->n > maxsph) maxsph = ((oi_tmpl *)s->ms[i])->n;
+    /* the accepted bodies: position, velocity, orientation */
+    double *bx = (double *)malloc(sizeof(double) * 3 * npti);
+    double *bv = (double *)malloc(sizeof(double) * 3 * npti);
+    double *bq = (double *)malloc(sizeof(double) * 4 * npti);
+    /* the spheres placed so far (neighList) */
+    double *xn = (double *)malloc(sizeof(double) * 3 * (size_t)npti * maxsph);
+    double *rn = (double *)malloc(sizeof(double) * (size_t)npti * maxsph);
+    double *xs = (double *)malloc(sizeof(double) * 3 * maxsph);
+    double *rs = (double *)malloc(sizeof(double) * maxsph);
+    int nnew = 0;
+    /* quat_insert: identity, or the constant orientation (fix_insert.cpp:468, :316-330) */
+    double quat_insert[4] = {1.0, 0.0, 0.0, 0.0};
+    if (s->orientation == 2)
+        for (int k = 0; k < 4; k++) quat_insert[k] = s->quat[k];
+
     /* FixInsertPack::x_v_omega (fix_insert_pack.cpp:498-605) */
     int ninserted_this_local = 0, ntry = 0;
     double mass_inserted_this_local = 0.;
-    double *xn = (double *)malloc(sizeof(double) * 3 * npti);
-    double *rn = (double *)malloc(sizeof(double) * npti);
     if (!s->overlapcheck) {
         for (int itotal = 0; itotal < ninsert_this_local; itotal++) {
             const int t = pti_t[ninserted_this_local];
-            const double rbound = s->radius[t];
             double pos[3], v[3];
-            region_random_point(f, pos, rbound);
+            region_random_point(f, pos, f->rbound[t]);
             v[0] = s->vel[0];
             v[1] = s->vel[1];
             v[2] = s->vel[2];
             random_velocity(f, v);
+            if (s->orientation == 1) unit_quat(&f->random, quat_insert);
             const int k = ninserted_this_local;
-            memcpy(xn + 3 * k, pos, sizeof pos);
-            rn[k] = rbound;
-            if (k < cap) {
-                memcpy(x_out + 3 * k, pos, sizeof pos);
-                memcpy(v_out + 3 * k, v, sizeof v);
-            }
-            mass_inserted_this_local += s->density[t] * (rbound * rbound * rbound * 4. * M_PI / 3.);
+            memcpy(bx + 3 * k, pos, sizeof pos);
+            memcpy(bv + 3 * k, v, sizeof v);
+            memcpy(bq + 4 * k, quat_insert, sizeof quat_insert);
+            mass_inserted_this_local += s->ms[t] ? ((oi_tmpl *)s->ms[t])->volume * ((oi_tmpl *)s->ms[t])->density
+                                                 : s->density[t] * (s->radius[t] * s->radius[t] * s->radius[t] * 4. * M_PI / 3.);
             ninserted_this_local++;
         }
     } else {
         while (ntry < maxtry && ninserted_this_local < ninsert_this_local) {
             const int t = pti_t[ninserted_this_local];
-            const double rbound = s->radius[t];
+            const double rbound = f->rbound[t];
             int nins = 0;
             double pos[3], v[3];
             while (nins == 0 && ntry < maxtry) {
@@ -443,33 +695,57 @@ int oi_insert(oi_fix *f, long long step, int n, const double *x, const double *r
                 v[1] = s->vel[1];
                 v[2] = s->vel[2];
                 random_velocity(f, v);
-                /* ParticleToInsert::check_near_set_x_v_omega
-                 * (particleToInsert.cpp:246-262) */
-                if (!overlaps(f, pos, rbound, n, x, r, ninserted_this_local, xn, rn)) nins = 1;
+                if (s->orientation == 1) unit_quat(&f->random, quat_insert);
+                /* ParticleToInsert::check_near_set_x_v_omega[_ms]
+                 * (particleToInsert.cpp:246-323): every sphere free */
+                const int ns = body_spheres(f, t, pos, quat_insert, xs, rs);
+                int free_ = 1;
+                for (int j = 0; j < ns && free_; j++)
+                    if (overlaps(f, xs + 3 * j, rs[j], n, x, r, nnew, xn, rn)) free_ = 0;
+                if (free_) {
+                    for (int j = 0; j < ns; j++) {
+                        memcpy(xn + 3 * nnew, xs + 3 * j, 3 * sizeof(double));
+                        rn[nnew++] = rs[j];
+                    }
+                    nins = ns;
+                }
             }
             if (nins > 0) {
                 const int k = ninserted_this_local;
-                memcpy(xn + 3 * k, pos, sizeof pos);
-                rn[k] = rbound;
-                if (k < cap) {
-                    memcpy(x_out + 3 * k, pos, sizeof pos);
-                    memcpy(v_out + 3 * k, v, sizeof v);
-                }
-                mass_inserted_this_local += s->density[t] * (rbound * rbound * rbound * 4. * M_PI / 3.);
+                memcpy(bx + 3 * k, pos, sizeof pos);
+                memcpy(bv + 3 * k, v, sizeof v);
+                memcpy(bq + 4 * k, quat_insert, sizeof quat_insert);
+                mass_inserted_this_local += s->ms[t] ? ((oi_tmpl *)s->ms[t])->volume * ((oi_tmpl *)s->ms[t])->density
+                                                     : s->density[t] * (rbound * rbound * rbound * 4. * M_PI / 3.);
                 ninserted_this_local++;
             }
         }
     }
-    /* ParticleToInsert::insert (particleToInsert.cpp:97-150): radius,
-     * rmass = mass_ins, type, omega */
-    for (int k = 0; k < ninserted_this_local && k < cap; k++) {
+    /* ParticleToInsert::insert (particleToInsert.cpp:97-150): each body's
+     * spheres; a single sphere has mass_ins, a clump's sphere
+     * 4.18879020479 r^3 rho */
+    int natoms = 0;
+    for (int k = 0; k < ninserted_this_local; k++) {
         const int t = pti_t[k];
-        r_out[k] = s->radius[t];
-        m_out[k] = s->density[t] * (s->radius[t] * s->radius[t] * s->radius[t] * 4. * M_PI / 3.);
-        type_out[k] = s->type[t];
-        omega_out[3 * k] = s->omega[0];
-        omega_out[3 * k + 1] = s->omega[1];
-        omega_out[3 * k + 2] = s->omega[2];
+        const oi_tmpl *ms = (const oi_tmpl *)s->ms[t];
+        const int ns = body_spheres(f, t, bx + 3 * k, bq + 4 * k, xs, rs);
+        for (int j = 0; j < ns; j++, natoms++) {
+            if (natoms >= cap) continue;
+            memcpy(x_out + 3 * natoms, xs + 3 * j, 3 * sizeof(double));
+            memcpy(v_out + 3 * natoms, bv + 3 * k, 3 * sizeof(double));
+            omega_out[3 * natoms] = s->omega[0];
+            omega_out[3 * natoms + 1] = s->omega[1];
+            omega_out[3 * natoms + 2] = s->omega[2];
+            r_out[natoms] = rs[j];
+            if (ms) {
+                m_out[natoms] = 4.18879020479 * rs[j] * rs[j] * rs[j] * ms->density;
+                type_out[natoms] = ms->types ? ms->types[j] : ms->atom_type;
+            } else {
+                m_out[natoms] = s->density[t] * (rs[j] * rs[j] * rs[j] * 4. * M_PI / 3.);
+                type_out[natoms] = s->type[t];
+            }
+            if (brid_out) brid_out[natoms] = pti_b[k];
+        }
     }
     f->ninserted += ninserted_this_local;
     f->massinserted += mass_inserted_this_local;
@@ -478,169 +754,13 @@ int oi_insert(oi_fix *f, long long step, int n, const double *x, const double *r
     if (s->insert_every) f->next_reneighbor += s->insert_every;
     else f->next_reneighbor = 0;
     free(pti_t);
+    free(pti_b);
+    free(bx);
+    free(bv);
+    free(bq);
     free(xn);
     free(rn);
-    return ninserted_this_local;
-}
-
-/* ---- fix particletemplate/multiplespheres (fix_template_multiplespheres.cpp) ---- */
-struct oi_tmpl {
-    int n;
-    double *x, *r;
-    double x_min[3], x_max[3], x_bound[3], r_bound, xcm[3], volume;
-};
-
-oi_tmpl *oi_tmpl_create(int seed, int nspheres, const double *x, const double *r, double scale, int ntry)
-{
-    oi_tmpl *t = (oi_tmpl *)calloc(1, sizeof(oi_tmpl));
-    const int n = nspheres;
-    t->n = n;
-    t->x = (double *)malloc(sizeof(double) * 3 * n);
-    t->r = (double *)malloc(sizeof(double) * n);
-    /* constructor, file branch (:118-150): r *= scale*cg, x = (scale*cg) x,
-     * then the bounding box; cg = 1 */
-    for (int d = 0; d < 3; d++) {
-        t->x_min[d] = 1e8;
-        t->x_max[d] = -1e8;
-    }
-    for (int i = 0; i < n; i++) {
-        t->r[i] = r[i] * (scale * 1.0);
-        for (int j = 0; j < 3; j++) t->x[3 * i + j] = (scale * 1.0) * x[3 * i + j];
-    }
-    for (int i = 0; i < n; i++)
-        for (int j = 0; j < 3; j++) {
-            if (t->x[3 * i + j] - t->r[i] < t->x_min[j]) t->x_min[j] = t->x[3 * i + j] - t->r[i];
-            if (t->x[3 * i + j] + t->r[i] > t->x_max[j]) t->x_max[j] = t->x[3 * i + j] + t->r[i];
-        }
-    park mc;
-    park_reset(&mc, seed);   /* random_mc: the seed, no process shift */
-
-    /* calc_bounding_sphere (:321-375) */
-    t->r_bound = 1e8;
-    int *visited = (int *)malloc(sizeof(int) * n);
-    for (int shuffle = 0; shuffle < 200; shuffle++) {
-        for (int i = 0; i < n; i++) visited[i] = 0;
-        int isphere = -1, nvisited = 0;
-        double xbt[3], rbt, d[3], dist;
-        while (isphere < 0 || visited[isphere] || isphere >= n) isphere = (int)(park_uniform(&mc) * n);
-        nvisited++;
-        visited[isphere] = 1;
-        xbt[0] = t->x[3 * isphere];
-        xbt[1] = t->x[3 * isphere + 1];
-        xbt[2] = t->x[3 * isphere + 2];
-        rbt = t->r[isphere];
-        while (nvisited < n) {
-            while (isphere < 0 || visited[isphere] || isphere >= n) isphere = (int)(park_uniform(&mc) * n);
-            nvisited++;
-            visited[isphere] = 1;
-            d[0] = t->x[3 * isphere] - xbt[0];
-            d[1] = t->x[3 * isphere + 1] - xbt[1];
-            d[2] = t->x[3 * isphere + 2] - xbt[2];
-            dist = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-            if (dist + t->r[isphere] > rbt) {
-                const double fact = (dist + t->r[isphere] - rbt) / (2. * dist);
-                d[0] = fact * d[0];
-                d[1] = fact * d[1];
-                d[2] = fact * d[2];
-                xbt[0] = xbt[0] + d[0];
-                xbt[1] = xbt[1] + d[1];
-                xbt[2] = xbt[2] + d[2];
-                rbt += sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-            }
-        }
-        if (rbt < t->r_bound) {
-            t->r_bound = rbt;
-            t->x_bound[0] = xbt[0];
-            t->x_bound[1] = xbt[1];
-            t->x_bound[2] = xbt[2];
-        }
-    }
-    free(visited);
-
-    /* calc_center_of_mass (:410-450), brute force over the spheres */
-    int nsuccess = 0;
-    double xcm[3] = {0., 0., 0.};
-    for (int i = 0; i < ntry; i++) {
-        double xt[3];
-        xt[0] = t->x_min[0] + (t->x_max[0] - t->x_min[0]) * park_uniform(&mc);
-        xt[1] = t->x_min[1] + (t->x_max[1] - t->x_min[1]) * park_uniform(&mc);
-        xt[2] = t->x_min[2] + (t->x_max[2] - t->x_min[2]) * park_uniform(&mc);
-        for (int j = 0; j < n; j++) {
-            double dsq = 0.;
-            dsq += (xt[0] - t->x[3 * j]) * (xt[0] - t->x[3 * j]);
-            dsq += (xt[1] - t->x[3 * j + 1]) * (xt[1] - t->x[3 * j + 1]);
-            dsq += (xt[2] - t->x[3 * j + 2]) * (xt[2] - t->x[3 * j + 2]);
-            if (dsq < t->r[j] * t->r[j]) {
-                xcm[0] = (xcm[0] * (double)nsuccess + xt[0]) / (double)(nsuccess + 1);
-                xcm[1] = (xcm[1] * (double)nsuccess + xt[1]) / (double)(nsuccess + 1);
-                xcm[2] = (xcm[2] * (double)nsuccess + xt[2]) / (double)(nsuccess + 1);
-                nsuccess++;
-                break;
-            }
-        }
-    }
-    t->volume = (double)nsuccess / (double)ntry * (t->x_max[0] - t->x_min[0]) * (t->x_max[1] - t->x_min[1])
-                * (t->x_max[2] - t->x_min[2]);
-    for (int d = 0; d < 3; d++) t->xcm[d] = xcm[d];
-    for (int i = 0; i < n; i++)
-        for (int d = 0; d < 3; d++) t->x[3 * i + d] = t->x[3 * i + d] - xcm[d];
-    for (int d = 0; d < 3; d++) {
-        t->x_min[d] = t->x_min[d] - xcm[d];
-        t->x_max[d] = t->x_max[d] - xcm[d];
-        t->x_bound[d] = t->x_bound[d] - xcm[d];
-    }
-    return t;
-}
-
-void oi_tmpl_destroy(oi_tmpl *t)
-{
-    if (!t) return;
-    free(t->x);
-    free(t->r);
-    free(t);
-}
-
-void oi_tmpl_info(const oi_tmpl *t, double out[9])
-{
-    out[0] = t->r_bound;
-    for (int d = 0; d < 3; d++) out[1 + d] = t->x_bound[d];
-    out[4] = t->volume;
-    out[5] = 0.0;
-    for (int d = 0; d < 3; d++) out[6 + d] = t->xcm[d];
-}
-
-/* MathExtra::quatquat (math_extra.h:596-602) */
-static void oi_quatquat(const double *a, const double *b, double *c)
-{
-    c[0] = a[0] * b[0] - a[1] * b[1] - a[2] * b[2] - a[3] * b[3];
-    c[1] = a[0] * b[1] + b[0] * a[1] + a[2] * b[3] - a[3] * b[2];
-    c[2] = a[0] * b[2] + b[0] * a[2] + a[3] * b[1] - a[1] * b[3];
-    c[3] = a[0] * b[3] + b[0] * a[3] + a[1] * b[2] - a[2] * b[1];
-}
-
-void oi_tmpl_place(const oi_tmpl *t, const double x[3], const double *quat, double *out)
-{
-    double q[4] = {1.0, 0.0, 0.0, 0.0};
-    if (quat) {
-        /* MathExtraLiggghts::quat_normalize (math_extra_liggghts.h:513-520) */
-        const double norm = 1.0 / sqrt(quat[0] * quat[0] + quat[1] * quat[1] + quat[2] * quat[2] + quat[3] * quat[3]);
-        for (int k = 0; k < 4; k++) q[k] = quat[k] * norm;
-    }
-    /* ParticleToInsert::set_x_v_omega (particleToInsert.cpp:330-350) with
-     * MathExtraLiggghts::vec_quat_rotate (math_extra_liggghts.h:457-473) */
-    for (int j = 0; j < t->n; j++) {
-        double rel[3], vq[4], qc[4], tmp[4], res[4];
-        for (int d = 0; d < 3; d++) rel[d] = t->x[3 * j + d] - t->x_bound[d];
-        vq[0] = 0.;
-        vq[1] = rel[0];
-        vq[2] = rel[1];
-        vq[3] = rel[2];
-        qc[0] = q[0];
-        qc[1] = -q[1];
-        qc[2] = -q[2];
-        qc[3] = -q[3];
-        oi_quatquat(q, vq, tmp);
-        oi_quatquat(tmp, qc, res);
-        for (int d = 0; d < 3; d++) out[3 * j + d] = res[1 + d] + x[d];
-    }
+    free(xs);
+    free(rs);
+    return natoms;
 }

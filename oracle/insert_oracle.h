@@ -33,6 +33,9 @@ typedef struct {
     double vel[3], vel_fluct[3], omega[3];
     double boxlo[3], boxhi[3];
     int periodic[3];
+    void *ms[OI_MAXT];          /* per template: an oi_tmpl (particletemplate/multiplespheres) or NULL */
+    int orientation;            /* 0 template, 1 random, 2 constant */
+    double quat[4];
 } oi_spec;
 
 typedef struct oi_fix oi_fix;
@@ -46,8 +49,11 @@ long long oi_next_step(const oi_fix *f);
  * the created particles (at most cap) and returns their number, -1 on an
  * error, 0 when the step is no insertion step. */
 int oi_insert(oi_fix *f, long long step, int n, const double *x, const double *r, const double *m, int cap,
-              double *x_out, double *v_out, double *omega_out, double *r_out, double *m_out, int *type_out);
-/* ninserted, massinserted, last inserted, last aimed, last tries, next step */
+              double *x_out, double *v_out, double *omega_out, double *r_out, double *m_out, int *type_out,
+              double *brid_out);
+/* the created atoms (a clump's spheres in order) and, per atom, the clump's
+ * bond_random_id (-1 for spheres and unbonded clumps).
+ * ninserted, massinserted, last inserted, last aimed, last tries, next step */
 void oi_vector(const oi_fix *f, double out[6]);
 /* region volume estimate of the fix */
 double oi_region_volume(const oi_fix *f);
@@ -58,7 +64,8 @@ void oi_park_states(int seed, int n, int *states, double *values);
  * the file, scaled; bounding sphere and Monte Carlo centre of mass on the
  * template's stream, every inside test brute force. */
 typedef struct oi_tmpl oi_tmpl;
-oi_tmpl *oi_tmpl_create(int seed, int nspheres, const double *x, const double *r, double scale, int ntry);
+oi_tmpl *oi_tmpl_create(int seed, int nspheres, const double *x, const double *r, const int *types, int atom_type,
+                        double density, double scale, int ntry, int bonded);
 void oi_tmpl_destroy(oi_tmpl *t);
 /* r_bound, x_bound[3] (centre-of-mass frame), volume, (unused), xcm[3] */
 void oi_tmpl_info(const oi_tmpl *t, double out[9]);

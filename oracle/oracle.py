@@ -96,6 +96,9 @@ class _InsertSpec(C.Structure):
         ("boxlo", C.c_double * 3),
         ("boxhi", C.c_double * 3),
         ("periodic", C.c_int * 3),
+        ("ms", C.c_void_p * 32),
+        ("orientation", C.c_int),
+        ("quat", C.c_double * 4),
     ]
 
 
@@ -165,13 +168,14 @@ def lib():
         L.oi_destroy.argtypes = [C.c_void_p]
         L.oi_next_step.restype = C.c_longlong
         L.oi_next_step.argtypes = [C.c_void_p]
-        L.oi_insert.argtypes = [C.c_void_p, C.c_longlong, C.c_int, dp, dp, dp, C.c_int, dp, dp, dp, dp, dp, ip]
+        L.oi_insert.argtypes = [C.c_void_p, C.c_longlong, C.c_int, dp, dp, dp, C.c_int, dp, dp, dp, dp, dp, ip, dp]
         L.oi_vector.argtypes = [C.c_void_p, dp]
         L.oi_region_volume.restype = C.c_double
         L.oi_region_volume.argtypes = [C.c_void_p]
         L.oi_park_states.argtypes = [C.c_int, C.c_int, ip, dp]
         L.oi_tmpl_create.restype = C.c_void_p
-        L.oi_tmpl_create.argtypes = [C.c_int, C.c_int, dp, dp, C.c_double, C.c_int]
+        L.oi_tmpl_create.argtypes = [C.c_int, C.c_int, dp, dp, ip, C.c_int, C.c_double, C.c_double, C.c_int,
+                                     C.c_int]
         L.oi_tmpl_destroy.argtypes = [C.c_void_p]
         L.oi_tmpl_info.argtypes = [C.c_void_p, dp]
         L.oi_tmpl_place.argtypes = [C.c_void_p, dp, dp, dp]
@@ -523,6 +527,15 @@ class InsertPackOracle:
             s.vel[d] = float(vel[1][d])
             s.vel_fluct[d] = float(vel[2][d]) if len(vel) > 2 else 0.0
             s.omega[d] = float(spec.get("omega", (0.0, 0.0, 0.0))[d])
+        # multisphere templates: TemplateOracle objects (None for spheres)
+        self.templates = list(spec.get("multisphere_oracle", [None] * len(tl)))
+        for k, t in enumerate(self.templates):
+            s.ms[k] = t.h if t is not None else None
+        o = spec.get("orientation", "template")
+        s.orientation = {"template": 0, "random": 1}.get(o, 2) if isinstance(o, str) else 2
+        q = o[1] if not isinstance(o, str) else (1.0, 0.0, 0.0, 0.0)
+        for d in range(4):
+            s.quat[d] = float(q[d])
         self.h = self.L.oi_create(C.byref(s), int(ntimestep))
         if not self.h:
             raise RuntimeError(self.L.oi_error().decode())
@@ -554,10 +567,10 @@ class InsertPackOracle:
         m = np.ascontiguousarray(rmass, dtype=np.float64)
         n = len(r)
         o = dict(x=np.zeros((cap, 3)), v=np.zeros((cap, 3)), omega=np.zeros((cap, 3)), radius=np.zeros(cap),
-                 rmass=np.zeros(cap), type=np.zeros(cap, np.int32))
+                 rmass=np.zeros(cap), type=np.zeros(cap, np.int32), brid=np.zeros(cap))
         k = self.L.oi_insert(self.h, int(step), n, _dp(x) if n else None, _dp(r) if n else None,
                              _dp(m) if n else None, cap, _dp(o["x"]), _dp(o["v"]), _dp(o["omega"]),
-                             _dp(o["radius"]), _dp(o["rmass"]), _ip(o["type"]))
+                             _dp(o["radius"]), _dp(o["rmass"]), _ip(o["type"]), _dp(o["brid"]))
         if k < 0:
             raise RuntimeError("oracle insertion failed")
         return {key: val[:k].copy() for key, val in o.items()}
@@ -567,12 +580,14 @@ class TemplateOracle:
     """fix particletemplate/multiplespheres + create_particles single,
     restated (insert_oracle.c)."""
 
-    def __init__(self, seed, x, r, scale=1.0, ntry=100000):
+    def __init__(self, seed, x, r, scale=1.0, ntry=100000, types=None, atom_type=1, density=2500.0, bonded=False):
         self.L = lib()
         xx = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)
         rr = np.ascontiguousarray(r, dtype=np.float64)
+        tt = None if types is None else np.ascontiguousarray(types, dtype=np.int32)
         self.n = len(rr)
-        self.h = self.L.oi_tmpl_create(int(seed), self.n, _dp(xx), _dp(rr), float(scale), int(ntry))
+        self.h = self.L.oi_tmpl_create(int(seed), self.n, _dp(xx), _dp(rr), None if tt is None else _ip(tt),
+                                       int(atom_type), float(density), float(scale), int(ntry), int(bool(bonded)))
 
     def close(self):
         if getattr(self, "h", None):

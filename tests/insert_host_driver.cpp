@@ -20,11 +20,14 @@ struct Run {
 
 extern "C" {
 
+// ms: per template a handle of it_create, or null (NULL: all spheres)
 void *ih_create(const dem_insert_pack_spec *s, const double *lo, const double *hi, const int *periodic,
-                long long ntimestep, char *err, int errlen)
+                long long ntimestep, void **ms, char *err, int errlen)
 {
     auto *r = new Run();
-    const std::string e = r->fix.init(*s, lo, hi, ntimestep);
+    std::vector<dem_ins::TemplateMultisphere *> tm(s->ntemplates, nullptr);
+    for (int i = 0; ms && i < s->ntemplates; i++) tm[i] = static_cast<dem_ins::TemplateMultisphere *>(ms[i]);
+    const std::string e = r->fix.init(*s, lo, hi, ntimestep, tm);
     if (!e.empty()) {
         std::strncpy(err, e.c_str(), errlen - 1);
         err[errlen - 1] = 0;
@@ -47,7 +50,7 @@ void ih_vector(void *h, double *out) { static_cast<Run *>(h)->fix.vector(out); }
 
 // one pre_exchange at `step`; *nbatch counts the overlap calls
 int ih_insert(void *h, long long step, int n, const double *x, const double *rad, const double *mass, int cap,
-              double *xo, double *vo, double *wo, double *ro, double *mo, int *to, int *nbatch)
+              double *xo, double *vo, double *wo, double *ro, double *mo, int *to, double *bo, int *nbatch)
 {
     Run *r = static_cast<Run *>(h);
     dem_ins::Hooks hk;
@@ -109,6 +112,7 @@ int ih_insert(void *h, long long step, int n, const double *x, const double *rad
         ro[k] = out[k].radius;
         mo[k] = out[k].mass;
         to[k] = out[k].type;
+        if (bo) bo[k] = out[k].brid;
     }
     return m;
 }

@@ -296,3 +296,73 @@ def test_template_and_create_particles_on_device():
     np.testing.assert_array_equal(st["v"][2:], [[0.1, 0.2, 0.3]] * 2)
     np.testing.assert_array_equal(st["omega"][2:], [[1.0, 0.0, 0.0]] * 2)
     s.close()
+
+
+def bond_tables(create=0.007):
+    w = 2
+    full = lambda v: np.full((w, w), v)  # noqa: E731
+    return dict(radiusMultiplier=full(1.0), normalBondStiffnessPerUnitArea=full(1e7),
+                tangentialBondStiffnessPerUnitArea=full(1e6), maxDistance=full(0.009), maxSigma=full(1e12),
+                maxTau=full(1e12), createDistance=full(create), dampingNormalForce=full(0.0),
+                dampingTangentialForce=full(0.0), dampingNormalTorque=full(0.0), dampingTangentialTorque=full(0.0))
+
+
+def test_bonded_clumps_poured_and_bonded():
+    """insert/pack pouring bonded 3-sphere clumps (particletemplate/
+    multiplespheres ... bonded yes) and single spheres with random
+    orientations under gravity: every insertion equals the oracle's bit for
+    bit, and at each insertion step the bond model's rule B bonds exactly
+    the spheres of one clump closer than createDistance (the neighbours in
+    the chain, not its ends) and nothing else, ever"""
+    from oracle import TemplateOracle
+    from test_insert import CLUMP3, clump_spec
+
+    eng = load_engine()
+    spec = dict(packing_spec(), gravity=(0.0, 0.0, -9.81), cohesion="bond")
+    s = eng.Simulation(spec, empty_atoms())
+    x3, r3 = CLUMP3
+    tid = s.add_template_multisphere(999671, 1, 2500.0, x3, r3, ntry=100000, bonded=True)
+    to = TemplateOracle(999671, x3, r3, ntry=100000, atom_type=1, density=2500.0, bonded=True)
+    ins = dict(clump_spec(), volumefraction_region=0.06, insert_every=400,
+               region=((-0.05, -0.05, 0.06), (0.05, 0.05, 0.15)))
+    fid = s.add_insert_pack(dict(ins, multisphere=[tid, 0], group_mask=0))
+    # the bond model sees the templates' smallest radius (an empty box)
+    s.set_bond(bond_tables(), ts_create=-1.0)
+    o = oracle_fix(dict(ins, multisphere_oracle=[to, None]))
+    s.setup()
+    n_before = 0
+    clumps = []
+    for event in range(3):
+        t_ins = int(s.insert_vector(fid)[5])
+        assert t_ins == o.next_step()
+        s.step(t_ins - s.timestep())
+        st, props = s.state(), s.props()
+        ref = o.insert(t_ins, st["x"][:n_before], props["radius"][:n_before], props["rmass"][:n_before])
+        k = len(ref["radius"])
+        new = slice(n_before, n_before + k)
+        assert s.n == n_before + k
+        np.testing.assert_array_equal(st["x"][new], ref["x"])
+        np.testing.assert_array_equal(props["radius"][new], ref["radius"])
+        np.testing.assert_array_equal(props["rmass"][new], ref["rmass"])
+        nc = int(np.sum(ref["brid"] >= 0)) // 3
+        tags = props["tag"][new]
+        clumps += [tuple(tags[3 * c:3 * c + 3]) for c in range(nc)]
+        print("event %d at step %d: %d clumps and %d spheres inserted" % (event, t_ins, nc, k - 3 * nc))
+        assert nc > 50 or event > 0
+        # bonds: within each clump, neighbours only
+        ti, tj, sh, hi, fl = s.pairs()
+        # the reference's slot order puts the bond's flag first (Hertz keeps
+        # no normal history)
+        bonded = {(a, b) for a, b, h in zip(ti.tolist(), tj.tolist(), hi[:, 0]) if h == 1.0}
+        want = set()
+        for c in clumps:
+            for a, b in ((c[0], c[1]), (c[1], c[2])):
+                want |= {(a, b), (b, a)}
+        assert bonded == want, (len(bonded), len(want))
+        n_before += k
+    np.testing.assert_array_equal(s.insert_vector(fid)[:5], o.vector()[:5])
+    s.step(300)
+    ti, tj, sh, hi, fl = s.pairs()
+    bonded = {(a, b) for a, b, h in zip(ti.tolist(), tj.tolist(), hi[:, 0]) if h == 1.0}
+    assert bonded == want
+    s.close()

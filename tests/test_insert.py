@@ -225,12 +225,14 @@ def driver():
     ip = C.POINTER(C.c_int)
     eng = load_engine()
     L.ih_create.restype = C.c_void_p
-    L.ih_create.argtypes = [C.POINTER(eng.engine.InsertPackSpec), dp, dp, ip, C.c_longlong, C.c_char_p, C.c_int]
+    L.ih_create.argtypes = [C.POINTER(eng.engine.InsertPackSpec), dp, dp, ip, C.c_longlong,
+                            C.POINTER(C.c_void_p), C.c_char_p, C.c_int]
     L.ih_destroy.argtypes = [C.c_void_p]
     L.ih_next_step.restype = C.c_longlong
     L.ih_next_step.argtypes = [C.c_void_p]
     L.ih_vector.argtypes = [C.c_void_p, dp]
-    L.ih_insert.argtypes = [C.c_void_p, C.c_longlong, C.c_int, dp, dp, dp, C.c_int, dp, dp, dp, dp, dp, ip, ip]
+    L.ih_insert.argtypes = [C.c_void_p, C.c_longlong, C.c_int, dp, dp, dp, C.c_int, dp, dp, dp, dp, dp, ip, dp,
+                            ip]
     _DRV = L
     return L
 
@@ -240,16 +242,20 @@ def _dp(a):
 
 
 class HostLoop:
+    """spec["multisphere_host"]: per template an it_create handle or None"""
+
     def __init__(self, spec, box=BOX, periodic=(0, 0, 0), ntimestep=0):
         eng = load_engine()
         self.L = driver()
         self.s, self.keep = eng.engine.insert_pack_spec(spec)
+        hs = spec.get("multisphere_host")
+        self.ms = None if hs is None else (C.c_void_p * len(hs))(*hs)
         lo = np.ascontiguousarray(box[0], dtype=np.float64)
         hi = np.ascontiguousarray(box[1], dtype=np.float64)
         pe = np.ascontiguousarray(periodic, dtype=np.int32)
         err = C.create_string_buffer(256)
         self.h = self.L.ih_create(C.byref(self.s), _dp(lo), _dp(hi), pe.ctypes.data_as(C.POINTER(C.c_int)),
-                                  int(ntimestep), err, 256)
+                                  int(ntimestep), self.ms, err, 256)
         if not self.h:
             raise RuntimeError(err.value.decode())
 
@@ -258,12 +264,12 @@ class HostLoop:
         r = np.ascontiguousarray(r, dtype=np.float64)
         m = np.ascontiguousarray(m, dtype=np.float64)
         o = dict(x=np.zeros((cap, 3)), v=np.zeros((cap, 3)), omega=np.zeros((cap, 3)), radius=np.zeros(cap),
-                 rmass=np.zeros(cap), type=np.zeros(cap, np.int32))
+                 rmass=np.zeros(cap), type=np.zeros(cap, np.int32), brid=np.zeros(cap))
         nb = C.c_int(0)
         k = self.L.ih_insert(self.h, int(step), len(r), _dp(x) if len(r) else None, _dp(r) if len(r) else None,
                              _dp(m) if len(r) else None, cap, _dp(o["x"]), _dp(o["v"]), _dp(o["omega"]),
                              _dp(o["radius"]), _dp(o["rmass"]), o["type"].ctypes.data_as(C.POINTER(C.c_int)),
-                             C.byref(nb))
+                             _dp(o["brid"]), C.byref(nb))
         assert k >= 0
         self.batches = nb.value
         return {key: val[:k].copy() for key, val in o.items()}
@@ -338,4 +344,55 @@ def test_host_loop_repeated_insertions():
         m = np.concatenate([m, a["rmass"]])
         step += 100
     np.testing.assert_array_equal(o.vector(), h.vector())
+    h.close()
+
+
+# ---- clumps: particletemplate/multiplespheres in the distribution
+
+CLUMP3 = (np.array([[-0.006, 0.0, 0.0], [0.0, 0.0, 0.0], [0.006, 0.0, 0.0]]), np.full(3, 0.003))
+
+
+def clump_spec(periodic_res=False):
+    spec = dict(PACKING, volumefraction_region=0.2, all_in=False, orientation="random",
+                templates=[dict(type=1, radius=0.0, density=0.0), dict(type=1, radius=0.004, density=2500.0)],
+                weights=[0.6, 0.4], vel=("uniform", (0.0, 0.0, -0.5), (0.1, 0.1, 0.1)))
+    return spec
+
+
+def test_clump_insertion_host_loop_equals_oracle():
+    """a bonded 3-sphere clump template (the particle3.txt of the reference's
+    cohesive_bond examples) with a sphere template, random orientations,
+    residents: the host loop equals the oracle bit for bit, bond ids
+    included; every clump keeps the template's shape; the first clump's
+    bond id is the insertion step plus the template stream's first draw"""
+    from oracle import TemplateOracle
+    from test_template import host_template
+
+    x3, r3 = CLUMP3
+    seed = 999671
+    to = TemplateOracle(seed, x3, r3, ntry=100000, atom_type=1, density=2500.0, bonded=True)
+    L, th, _ = host_template(seed, x3, r3, ntry=100000, bonded=True)
+    xr, rr, mr = resident_bed(n=150, seed=9)
+    spec = clump_spec()
+    o = oracle_fix(dict(spec, multisphere_oracle=[to, None]))
+    h = HostLoop(dict(spec, multisphere=[1, 0], multisphere_host=[th, None]))
+    a = o.insert(1, xr, rr, mr)
+    b = h.insert(1, xr, rr, mr)
+    same(a, b)
+    np.testing.assert_array_equal(o.vector(), h.vector())
+    nclump = int(np.sum(a["brid"] >= 0)) // 3
+    assert nclump > 20 and np.sum(a["brid"] < 0) > 20
+    # clumps come first (larger volume), three spheres each, one id per clump
+    ids = a["brid"][:3 * nclump].reshape(-1, 3)
+    assert (ids == ids[:, :1]).all() and len(np.unique(ids[:, 0])) == nclump
+    from oracle import park_states
+    assert ids[0, 0] == 1.0 + park_states(seed, 1)[1][0]
+    xs = a["x"][:3 * nclump].reshape(-1, 3, 3)
+    d01 = np.linalg.norm(xs[:, 0] - xs[:, 1], axis=1)
+    d02 = np.linalg.norm(xs[:, 0] - xs[:, 2], axis=1)
+    np.testing.assert_allclose(d01, 0.006, rtol=1e-12)
+    np.testing.assert_allclose(d02, 0.012, rtol=1e-12)
+    np.testing.assert_array_equal(a["rmass"][:3], 4.18879020479 * 0.003 ** 3 * 2500.0)
+    assert no_overlap(a["x"][3 * nclump:], a["radius"][3 * nclump:], xr, rr)
+    L.it_destroy(th)
     h.close()
