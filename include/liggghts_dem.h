@@ -431,6 +431,65 @@ int dem_create_particles_single(dem_ctx *ctx, int tmpl, const double *x, const d
 int dem_natoms(const dem_ctx *ctx);
 int dem_download_props(dem_ctx *ctx, int *tag, int *type, double *radius, double *rmass);
 
+/* fix <id> <group> insert/stream seed S distributiontemplate D ...
+ * insertion_face F extrude_length L | insert_every N [duration K]
+ * (fix_insert.cpp:96-330, fix_insert_stream.cpp:76-151).  Bodies are placed
+ * in the face extruded against the inflow direction: a point of the face
+ * drawn on the face mesh's own stream (area-weighted triangle, barycentric
+ * coordinates; MultiNodeMesh seeds it 179424799), pushed back along the
+ * face normal by a random length.  Until its release step an inserted atom
+ * moves with the velocity's normal component and no spin, forces and
+ * torques zeroed (FixInsertStream::end_of_step, fix_insert_stream.cpp:
+ * 828-921); at the release step it takes the insertion velocity and omega.
+ * While any atom awaits release the integrator runs un-fused and steps run
+ * without graphs.  dem_insert_stream_defaults fills the keyword defaults
+ * (FixInsert::init_defaults, fix_insert.cpp:446-490; FixInsertStream::
+ * init_defaults, fix_insert_stream.cpp:247-262). */
+typedef struct {
+    int seed, dist_seed;         /* as dem_insert_pack_spec                            */
+    int ntemplates;
+    const dem_template_sphere *templates;
+    const double *weights;
+    int number_based;
+    const int *multisphere;
+    int face_ntri;               /* the insertion face: a planar fix mesh/surface,   */
+    const double *face_nodes;    /* 9 doubles per triangle, scaled and moved         */
+    double extrude_length;       /* 0: not given                                       */
+    int insert_every;            /* -1 (default): from extrude_length                 */
+    int duration;                /* 0: insert_every                                    */
+    long long start;             /* first insertion step; default: next step          */
+    int nparticles;              /* bodies to insert; -1: INF; 0: from mass           */
+    double mass;                 /* mass to insert when nparticles is 0; -1: INF      */
+    double particlerate, massrate;   /* exactly one > 0                               */
+    int maxattempt;              /* default 50                                         */
+    int overlapcheck;            /* default 1                                          */
+    int all_in;                  /* 1 needs fix mesh/surface/planar: refused here      */
+    int parallel;                /* parallel yes: extrusion clipped to the sub-box    */
+    int ntry_mc;                 /* insertion-fraction samples, default 100000        */
+    int exact_number;            /* random_distribute exact (1) or uncorrelated (0)  */
+    int vel_style;               /* DEM_VEL_*                                          */
+    double vel[3], vel_fluct[3];
+    double omega[3];
+    int group_mask;              /* group bits of the inserted atoms; the release
+                                    applies to atoms holding the first stream fix's  */
+    int orientation;             /* DEM_ORIENT_*                                       */
+    double quat[4];
+} dem_insert_stream_spec;
+
+void dem_insert_stream_defaults(dem_insert_stream_spec *spec);
+/* Registers the fix and runs FixInsertStream::calc_insertion_properties
+ * (fix_insert_stream.cpp:287-423): face normal flipped toward the velocity,
+ * insert_every = (int)((L + 1e-14) / (dt |v_n|)) or L = duration dt |v_n|,
+ * bodies per insertion = flow rate * insert_every * dt.  The timestep must
+ * be set.  dem_insert_vector and dem_unfix take its id. */
+int dem_add_insert_stream(dem_ctx *ctx, const dem_insert_stream_spec *spec, int *id);
+/* The release data of atoms tag 1..ntags (fix property/atom
+ * release_fix_insert_stream, fix_insert_stream.cpp:763-826): 14 doubles per
+ * tag, x at insertion (3), insertion step, release step, integration
+ * velocity (3), velocity and omega at release (3 + 3); zero for atoms no
+ * stream created. */
+int dem_stream_release_data(dem_ctx *ctx, int ntags, double *out14);
+
 /* ---- Batched runs.  No counterpart in the reference, which runs one
  * simulation per process (one LAMMPS instance, lammps.cpp); these let many
  * independent simulations of the same box, walls and contact model share one

@@ -188,6 +188,12 @@ struct dem_ctx {
     std::vector<MoveOp> moves;
     double4 *xo[2] = {};     // positions at the fix's creation, per particle
     double *brid[2] = {};    // bond_random_id per particle (null: none set)
+    // fix insert/stream release data, 4 per particle (null: no stream fix);
+    // rel_until: the last release step of any atom, rel_mask: the group bits
+    // of the first stream fix (FixInsertStream::i_am_integrator)
+    double4 *rel[2] = {};
+    long long rel_until = -1;
+    int rel_mask = 0;
     double *d_gpart = nullptr, *d_gtot = nullptr;   // partial sums; 4 per op slot
     int ago = 0;
 
@@ -222,7 +228,12 @@ struct dem_ctx {
     struct Insert {
         int id;
         int gmask;                     // group bits of the atoms it creates
-        dem_ins::InsertPack fix;
+        std::unique_ptr<dem_ins::InsertPack> fix;   // insert/pack or insert/stream
+        double *face_dev = nullptr;    // insert/stream: the face triangles, 9 each
+        ~Insert()
+        {
+            if (face_dev) (void)hipFree(face_dev);
+        }
     };
     std::vector<std::unique_ptr<Insert>> inserts;
     // fix particletemplate/multiplespheres templates (create_particles)
@@ -479,6 +490,7 @@ static void free_all(dem_ctx *ctx)
         F(ctx->gmask[b]);
         F(ctx->xo[b]);
         F(ctx->brid[b]);
+        F(ctx->rel[b]);
         F(ctx->mf.counter[b]);
 
     }
@@ -729,7 +741,7 @@ extern "C" int dem_set_bond(dem_ctx *ctx, const double *tables, const dem_bond_s
     for (const auto &q : hx) minrad = std::min(minrad, q.w);
     for (const auto &t : ctx->templates)
         for (int j = 0; j < t->t.nspheres(); j++) minrad = std::min(minrad, t->t.radius(j));
-    for (const auto &in : ctx->inserts) minrad = std::min(minrad, in->fix.min_radius());
+    for (const auto &in : ctx->inserts) minrad = std::min(minrad, in->fix->min_radius());
     const bool nl = ctx->M.cohesion == CM_BOND_NONLINEAR;
     double cdf_all = 1.;
     auto T = [&](int slot, int i, int j) { return tables[(size_t)slot * w * w + i * w + j]; };
@@ -941,12 +953,18 @@ extern "C" int dem_upload_state(dem_ctx *ctx, int n, const double *x, const doub
     }
     // the group bits are cleared, and with them the groups fix move acts on
     ctx->moves.clear();
-    // no atom carries a bond_random_id
-    for (int b = 0; b < 2; b++)
+    // no atom carries a bond_random_id or awaits a stream's release
+    for (int b = 0; b < 2; b++) {
         if (ctx->brid[b]) {
             HIPCHK(hipFree(ctx->brid[b]));
             ctx->brid[b] = nullptr;
         }
+        if (ctx->rel[b]) {
+            HIPCHK(hipFree(ctx->rel[b]));
+            ctx->rel[b] = nullptr;
+        }
+    }
+    ctx->rel_until = -1;
     // new atoms: massflow side counters not seen yet
     if (ctx->mf.mesh >= 0 && massflow_counters(ctx)) return DEM_ERR_HIP;
     ctx->integ_bit = -1;
@@ -1131,6 +1149,8 @@ static int rebuild(dem_ctx *ctx, bool captured = false)
     pa.mf1 = ctx->mf.mesh >= 0 ? ctx->mf.counter[c1] : nullptr;
     pa.br0 = ctx->brid[c0];
     pa.br1 = ctx->brid[c1];
+    pa.rl0 = ctx->rel[c0];
+    pa.rl1 = ctx->rel[c1];
     pa.xhold = ctx->xhold;
     pa.wmask = ctx->wmask;
     pa.n = n;
@@ -1560,14 +1580,14 @@ static long long next_insert_step(const dem_ctx *ctx)
 {
     long long s = LLONG_MAX;
     for (const auto &in : ctx->inserts)
-        if (in->fix.next_step() > 0) s = std::min(s, in->fix.next_step());
+        if (in->fix->next_step() > 0) s = std::min(s, in->fix->next_step());
     return s;
 }
 
 static bool insert_due(const dem_ctx *ctx)
 {
     for (const auto &in : ctx->inserts)
-        if (in->fix.due(ctx->ntimestep)) return true;
+        if (in->fix->due(ctx->ntimestep)) return true;
     return false;
 }
 
@@ -1601,6 +1621,7 @@ static int grow_particles(dem_ctx *ctx, int c)
             return DEM_ERR_HIP;
         if (ctx->xo[b] && dev_grow(ctx, &ctx->xo[b], oc, nc)) return DEM_ERR_HIP;
         if (ctx->brid[b] && dev_grow(ctx, &ctx->brid[b], oc, nc)) return DEM_ERR_HIP;
+        if (ctx->rel[b] && dev_grow(ctx, &ctx->rel[b], 4 * oc, 4 * nc)) return DEM_ERR_HIP;
         if (ctx->mf.counter[b] && dev_grow(ctx, &ctx->mf.counter[b], oc, nc)) return DEM_ERR_HIP;
     }
     if (dev_grow(ctx, &ctx->f4, oc, nc) || dev_grow(ctx, &ctx->t4, oc, nc) || dev_grow(ctx, &ctx->xhold, oc, nc)
@@ -1618,7 +1639,7 @@ static int grow_particles(dem_ctx *ctx, int c)
 // empty rows for the new atoms to carry every history at the next rebuild.
 static int append_atoms(dem_ctx *ctx, int m, const double *x, const double *v, const double *omega,
                         const double *radius, const double *rmass, const int *type, const int *tag, const int *gm,
-                        const double *brid = nullptr)
+                        const double *brid = nullptr, const double *rel = nullptr)
 {
     if (m == 0) return DEM_OK;
     if (m < 0 || !x || !v || !omega || !radius || !rmass || !type || !tag)
@@ -1686,6 +1707,18 @@ static int append_atoms(dem_ctx *ctx, int m, const double *x, const double *v, c
         HIPCHK(hipMemcpyAsync(ctx->brid[c] + n0, hb.data(), m * sizeof(double), hipMemcpyHostToDevice, s));
         HIPCHK(hipStreamSynchronize(s));
     }
+    // release data: the stream's, or zero (the property's default)
+    if (rel && !ctx->rel[0]) {
+        for (int b = 0; b < 2; b++)
+            if (dev_alloc(ctx, &ctx->rel[b], 4 * (size_t)ctx->cap)) return DEM_ERR_HIP;
+        if (n0) HIPCHK(hipMemsetAsync(ctx->rel[c], 0, 4 * (size_t)n0 * sizeof(double4), s));
+    }
+    if (ctx->rel[c]) {
+        if (rel) HIPCHK(hipMemcpyAsync(ctx->rel[c] + 4 * (size_t)n0, rel, 4 * (size_t)m * sizeof(double4),
+                                       hipMemcpyHostToDevice, s));
+        else HIPCHK(hipMemsetAsync(ctx->rel[c] + 4 * (size_t)n0, 0, 4 * (size_t)m * sizeof(double4), s));
+        HIPCHK(hipStreamSynchronize(s));
+    }
     if (ctx->have_list) launch_extend_rows(ctx->nstart[ctx->lcur], n0, m, s);
     for (auto &mh : ctx->meshes) {
         if (!mh.have) continue;
@@ -1711,7 +1744,7 @@ static int append_atoms(dem_ctx *ctx, int m, const double *x, const double *v, c
 }
 
 // Resident particles into the overlap-test grid: cells at least R wide
-static int ins_build_grid(dem_ctx *ctx, double R)
+static int ins_build_grid(dem_ctx *ctx, double R, const StreamNear &f)
 {
     InsGrid &g = ctx->ins_grid;
     int64_t cells = 1;
@@ -1752,7 +1785,7 @@ static int ins_build_grid(dem_ctx *ctx, double R)
     hipStream_t s = ctx->stream;
     int *start = ctx->ins_start, *cursor = ctx->ins_start + (cells + 1);
     launch_fill_int(start, 0, (int)cells + 1, s);
-    launch_ins_bin(sx(ctx), n, g, ctx->ins_cell, start, s);
+    launch_ins_bin(sx(ctx), n, g, f, ctx->ins_cell, start, s);
     launch_scan(start, (int)cells + 1, ctx->ins_tmp, s);
     launch_copy_int(cursor, start, (int)cells + 1, s);
     launch_ins_place(ctx->ins_cell, n, cursor, ctx->ins_idx, s);
@@ -1761,9 +1794,21 @@ static int ins_build_grid(dem_ctx *ctx, double R)
     return DEM_OK;
 }
 
-static dem_ins::Hooks ins_hooks(dem_ctx *ctx)
+// Modify::get_max_radius (modify_liggghts.cpp:575-620): atoms, templates
+// and insertion fixes
+static double max_radius_sim(const dem_ctx *ctx)
+{
+    double m = ctx->maxrad;
+    for (const auto &t : ctx->templates)
+        for (int j = 0; j < t->t.nspheres(); j++) m = std::max(m, t->t.radius(j));
+    for (const auto &in : ctx->inserts) m = std::max(m, in->fix->max_radius());
+    return m;
+}
+
+static dem_ins::Hooks ins_hooks(dem_ctx *ctx, const dem_ctx::Insert *in)
 {
     dem_ins::Hooks h;
+    h.maxrad_sim = max_radius_sim(ctx);
     h.region_sums = [ctx](const double *lo, const double *hi, long long *np, double *vol, double *mass) -> int {
         *np = 0;
         *vol = *mass = 0.0;
@@ -1784,14 +1829,28 @@ static dem_ins::Hooks ins_hooks(dem_ctx *ctx)
         *np = (long long)c;
         return DEM_OK;
     };
-    h.overlap = [ctx](const std::vector<double> &cand, const std::vector<double> &rad,
-                      std::vector<uint32_t> &hit) -> int {
+    h.overlap = [ctx, in](const std::vector<double> &cand, const std::vector<double> &rad,
+                          const dem_ins::FaceFilter *near, std::vector<uint32_t> &hit) -> int {
         const int K = (int)(cand.size() / 3);
         hit.assign(K, 0u);
         if (ctx->n == 0 || K == 0) return DEM_OK;
         double rmax = 0.0;
         for (double r : rad) rmax = std::max(rmax, r);
-        if (!ctx->ins_grid_ready && ins_build_grid(ctx, rmax + ctx->maxrad)) return DEM_ERR_HIP;
+        StreamNear f{};
+        if (near) {
+            f.on = 1;
+            for (int d = 0; d < 3; d++) {
+                f.pref[d] = near->pref[d];
+                f.nrm[d] = near->nrm[d];
+                f.bblo[d] = near->bblo[d];
+                f.bbhi[d] = near->bbhi[d];
+            }
+            f.dmin = near->dmin;
+            f.dmax = near->dmax;
+            f.face = in->face_dev;
+            f.ntri = (int)(near->face->size() / 9);
+        }
+        if (!ctx->ins_grid_ready && ins_build_grid(ctx, rmax + ctx->maxrad, f)) return DEM_ERR_HIP;
         if (K > ctx->ins_kcap) {
             if (dev_alloc(ctx, &ctx->ins_cand, 3 * (size_t)K) || dev_alloc(ctx, &ctx->ins_hit, K)) return DEM_ERR_HIP;
             ctx->ins_kcap = K;
@@ -1801,7 +1860,7 @@ static dem_ins::Hooks ins_hooks(dem_ctx *ctx)
         HIPCHK(hipMemcpyAsync(ctx->ins_cand, cand.data(), cand.size() * sizeof(double), hipMemcpyHostToDevice, s));
         HIPCHK(hipMemcpyAsync(ctx->ins_rad, rad.data(), rad.size() * sizeof(double), hipMemcpyHostToDevice, s));
         launch_ins_check(ctx->ins_cand, K, ctx->ins_rad, (int)rad.size(), sx(ctx), ctx->ins_start, ctx->ins_idx,
-                         ctx->ins_grid, ctx->ins_hit, s);
+                         ctx->ins_grid, f, ctx->ins_hit, s);
         KCHK();
         HIPCHK(hipMemcpyAsync(hit.data(), ctx->ins_hit, K * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
@@ -1815,18 +1874,23 @@ static dem_ins::Hooks ins_hooks(dem_ctx *ctx)
 static int run_inserts(dem_ctx *ctx)
 {
     for (auto &in : ctx->inserts) {
-        if (!in->fix.due(ctx->ntimestep)) continue;
+        if (!in->fix->due(ctx->ntimestep)) continue;
         ctx->ins_grid_ready = false;
         std::vector<dem_ins::Placed> out;
         std::string err;
-        const int rc = in->fix.insert(ins_hooks(ctx), ctx->ntimestep, out, err);
-        if (rc) return fail(ctx, rc, "fix insert/pack: " + err);
+        const int rc = in->fix->insert(ins_hooks(ctx, in.get()), ctx->ntimestep, out, err);
+        if (rc) return fail(ctx, rc, std::string(in->fix->is_stream() ? "fix insert/stream: " : "fix insert/pack: ") + err);
         const int m = (int)out.size();
         if (m == 0) continue;
-        std::vector<double> x(3 * m), v(3 * m), w(3 * m), r(m), ms(m), br(m);
+        std::vector<double> x(3 * m), v(3 * m), w(3 * m), r(m), ms(m), br(m), rel;
         std::vector<int> ty(m), tg(m), gm(m, in->gmask);
         bool any_brid = false;
+        if (out[0].has_release) rel.resize(16 * (size_t)m);
         for (int k = 0; k < m; k++) {
+            if (!rel.empty()) {
+                std::copy(out[k].release, out[k].release + 16, rel.begin() + 16 * (size_t)k);
+                ctx->rel_until = std::max(ctx->rel_until, (long long)(int)(out[k].release[7] + 1e-14));
+            }
             for (int d = 0; d < 3; d++) {
                 x[3 * k + d] = out[k].x[d];
                 v[3 * k + d] = out[k].v[d];
@@ -1841,7 +1905,7 @@ static int run_inserts(dem_ctx *ctx)
             tg[k] = ctx->tag_max + 1 + k;
         }
         const int a = append_atoms(ctx, m, x.data(), v.data(), w.data(), r.data(), ms.data(), ty.data(), tg.data(),
-                                   gm.data(), any_brid ? br.data() : nullptr);
+                                   gm.data(), any_brid ? br.data() : nullptr, rel.empty() ? nullptr : rel.data());
         if (a) return a;
     }
     ctx->ins_grid_ready = false;
@@ -1887,12 +1951,103 @@ extern "C" int dem_add_insert_pack(dem_ctx *ctx, const dem_insert_pack_spec *spe
         }
     }
     auto in = std::make_unique<dem_ctx::Insert>();
-    const std::string err = in->fix.init(*spec, ctx->p.boxlo, ctx->p.boxhi, ctx->ntimestep, ms);
+    auto fix = std::make_unique<dem_ins::InsertPack>();
+    const std::string err = fix->init(*spec, ctx->p.boxlo, ctx->p.boxhi, ctx->ntimestep, ms);
     if (!err.empty()) return fail(ctx, DEM_ERR_ARG, "fix insert/pack: " + err);
+    in->fix = std::move(fix);
     in->id = ctx->next_op_id++;
     in->gmask = spec->group_mask;
     *id = in->id;
     ctx->inserts.push_back(std::move(in));
+    return DEM_OK;
+}
+
+// the group whose atoms the releases move: the first stream fix's
+// (FixInsertStream::init, i_am_integrator = first of its style); no stream
+// fix left: nothing is released (the property goes with the last one,
+// FixInsertStream::pre_delete)
+static void stream_integrator(dem_ctx *ctx)
+{
+    for (const auto &in : ctx->inserts)
+        if (in->fix->is_stream()) {
+            ctx->rel_mask = in->gmask;
+            return;
+        }
+    ctx->rel_until = -1;
+}
+
+extern "C" void dem_insert_stream_defaults(dem_insert_stream_spec *s)
+{
+    if (!s) return;
+    *s = dem_insert_stream_spec{};
+    s->insert_every = -1;
+    s->maxattempt = 50;
+    s->overlapcheck = 1;
+    s->ntry_mc = 100000;
+    s->exact_number = 1;
+    s->vel_style = DEM_VEL_CONSTANT;
+    s->orientation = DEM_ORIENT_TEMPLATE;
+    s->quat[0] = 1.0;
+}
+
+extern "C" int dem_add_insert_stream(dem_ctx *ctx, const dem_insert_stream_spec *spec, int *id)
+{
+    if (!ctx) return fail(nullptr, DEM_ERR_ARG, "null context");
+    if (!spec || !id) return fail(ctx, DEM_ERR_ARG, "dem_add_insert_stream: bad arguments");
+    if (!ctx->mat_set) return fail(ctx, DEM_ERR_ARG, "dem_add_insert_stream: set materials first");
+    if (ctx->ninst > 1) return fail(ctx, DEM_ERR_ARG, "dem_add_insert_stream: not in a batched run");
+    std::vector<dem_ins::TemplateMultisphere *> ms(std::max(spec->ntemplates, 0), nullptr);
+    for (int t = 0; t < spec->ntemplates && spec->templates; t++) {
+        const int mid = spec->multisphere ? spec->multisphere[t] : 0;
+        if (mid) {
+            for (auto &tp : ctx->templates)
+                if (tp->id == mid) ms[t] = &tp->t;
+            if (!ms[t]) return fail(ctx, DEM_ERR_ARG, "dem_add_insert_stream: no such multisphere template");
+            for (int j = 0; j < ms[t]->nspheres(); j++)
+                if (ms[t]->type(j) < 1 || ms[t]->type(j) > ctx->ntypes)
+                    return fail(ctx, DEM_ERR_ARG, "dem_add_insert_stream: template atom type out of range");
+        } else if (spec->templates[t].atom_type < 1 || spec->templates[t].atom_type > ctx->ntypes) {
+            return fail(ctx, DEM_ERR_ARG, "dem_add_insert_stream: template atom type out of range");
+        }
+    }
+    sync_enter(ctx);
+    auto in = std::make_unique<dem_ctx::Insert>();
+    auto fix = std::make_unique<dem_ins::InsertStream>();
+    const std::string err = fix->init(*spec, ctx->p.boxlo, ctx->p.boxhi, ctx->ntimestep, ctx->p.dt, ms);
+    if (!err.empty()) return fail(ctx, DEM_ERR_ARG, "fix insert/stream: " + err);
+    const std::vector<double> &face = fix->face();
+    if (dev_alloc(ctx, &in->face_dev, face.size())) return DEM_ERR_HIP;
+    HIPCHK(hipMemcpy(in->face_dev, face.data(), face.size() * sizeof(double), hipMemcpyHostToDevice));
+    in->fix = std::move(fix);
+    in->id = ctx->next_op_id++;
+    in->gmask = spec->group_mask;
+    *id = in->id;
+    ctx->inserts.push_back(std::move(in));
+    stream_integrator(ctx);
+    return DEM_OK;
+}
+
+extern "C" int dem_stream_release_data(dem_ctx *ctx, int ntags, double *out)
+{
+    if (!ctx) return fail(nullptr, DEM_ERR_ARG, "null context");
+    if (ntags < 0 || (ntags > 0 && !out)) return fail(ctx, DEM_ERR_ARG, "dem_stream_release_data: bad arguments");
+    sync_enter(ctx);
+    std::fill(out, out + 14 * (size_t)ntags, 0.0);
+    const int n = ctx->n, c = ctx->cur;
+    if (n == 0 || !ctx->rel[c]) return DEM_OK;
+    std::vector<double4> hr(4 * (size_t)n);
+    std::vector<int> tg(n);
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipMemcpy(hr.data(), ctx->rel[c], hr.size() * sizeof(double4), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(tg.data(), ctx->tag[c], n * sizeof(int), hipMemcpyDeviceToHost));
+    for (int i = 0; i < n; i++) {
+        if (tg[i] < 1 || tg[i] > ntags) continue;
+        double *o = out + 14 * (size_t)(tg[i] - 1);
+        const double4 *r = &hr[4 * (size_t)i];
+        const double v[14] = {r[0].x, r[0].y, r[0].z, r[0].w, r[1].w, r[1].x, r[1].y,
+                              r[1].z, r[2].x, r[2].y, r[2].z, r[3].x, r[3].y, r[3].z};
+        std::copy(v, v + 14, o);
+    }
     return DEM_OK;
 }
 
@@ -1902,7 +2057,7 @@ extern "C" int dem_insert_vector(dem_ctx *ctx, int id, double *out)
     if (!out) return fail(ctx, DEM_ERR_ARG, "dem_insert_vector: null output");
     for (const auto &in : ctx->inserts)
         if (in->id == id) {
-            in->fix.vector(out);
+            in->fix->vector(out);
             return DEM_OK;
         }
     return fail(ctx, DEM_ERR_ARG, "dem_insert_vector: no such fix");
@@ -2226,7 +2381,8 @@ static std::vector<const void *> graph_key(const dem_ctx *ctx)
                                    ctx->tag[0], ctx->tag[1], ctx->orig[0], ctx->orig[1], ctx->img[0],
                                    ctx->img[1], ctx->whist[0], ctx->whist[1], ctx->nstart[0], ctx->nstart[1],
                                    ctx->nj[0], ctx->nj[1], ctx->hist[0], ctx->hist[1], ctx->cstart,
-                                   ctx->cursor, ctx->scan_tmp, ctx->spread, ctx->d_mat, ctx->pwf};
+                                   ctx->cursor, ctx->scan_tmp, ctx->spread, ctx->d_mat, ctx->pwf,
+                                   ctx->rel[0], ctx->rel[1]};
     k.push_back((const void *)(intptr_t)(ctx->cur + 2 * ctx->lcur));
     k.push_back((const void *)(intptr_t)ctx->n);
     k.push_back((const void *)(intptr_t)(ctx->tune_lanes * 4096 + ctx->tune_block));
@@ -2390,13 +2546,15 @@ extern "C" int dem_step(dem_ctx *ctx, int nsteps)
     sync_enter(ctx);
     int rc;
     const bool gok = graph_ok(ctx);
+    bool split_prev = false;   // the last step ran a stream release: no fused initial half
     for (int st = 0; st < nsteps; st++) {
         // whole graph periods while they fit before the last step, starting
         // right after a rebuild
-        // an insertion step runs eagerly: graph periods end before it
-        if (gok && st >= 1 && ctx->ago == 0 && ctx->ntimestep + 1 != ctx->next_reneighbor
+        // an insertion step runs eagerly: graph periods end before it, and
+        // none starts while an atom awaits a stream's release
+        if (gok && st >= 1 && !split_prev && ctx->ago == 0 && ctx->ntimestep + 1 != ctx->next_reneighbor
             && st + 2 * ctx->p.neigh_every <= nsteps - 1
-            && ctx->ntimestep + 2 * ctx->p.neigh_every < next_insert_step(ctx)) {
+            && ctx->ntimestep + 2 * ctx->p.neigh_every < next_insert_step(ctx) && ctx->ntimestep > ctx->rel_until) {
             if (!ctx->gexec || ctx->g_key != graph_key(ctx)) {
                 if ((rc = capture_graph(ctx))) return rc;
             }
@@ -2418,6 +2576,7 @@ extern "C" int dem_step(dem_ctx *ctx, int nsteps)
         }
         ctx->ntimestep++;
         if (st == 0 && ((rc = integrate(ctx, 1)) || (rc = move_meshes(ctx)))) return rc;
+        if (st > 0 && split_prev && ctx->n > 0 && (rc = integrate(ctx, 1))) return rc;
         if (ctx->mf.mesh >= 0 && (rc = massflow_step(ctx))) return rc;
         int nflag = 0;
         if ((rc = decide(ctx, &nflag))) return rc;
@@ -2428,7 +2587,25 @@ extern "C" int dem_step(dem_ctx *ctx, int nsteps)
         if (!nflag) mesh_trigger(ctx);
         if (ctx->tape_pos >= 0 && (rc = tape_store(ctx, nflag))) return rc;
         const bool last = st == nsteps - 1;
-        if (ctx->n > 0 && (rc = force_pass(ctx, 1, last ? 2 : 3))) return rc;
+        // end_of_step of fix insert/stream after the final half: the
+        // integrator runs un-fused on such steps
+        const bool release = ctx->ntimestep <= ctx->rel_until && ctx->rel[ctx->cur];
+        if (ctx->n > 0 && (rc = force_pass(ctx, 1, (last || release) ? 2 : 3))) return rc;
+        if (release && ctx->n > 0) {
+            ReleaseArgs ra{};
+            const int c = ctx->cur;
+            ra.xr = sx(ctx); ra.vm = sv(ctx); ra.wt = sw(ctx);
+            ra.f4 = ctx->f4; ra.t4 = ctx->t4;
+            ra.gm = ctx->gmask[c];
+            ra.rel = ctx->rel[c];
+            ra.n = ctx->n;
+            ra.mask = ctx->rel_mask;
+            ra.step = ctx->ntimestep;
+            ra.dt = ctx->p.dt;
+            launch_stream_release(ra, ctx->stream);
+            KCHK();
+        }
+        split_prev = release;
         if (!last && (rc = move_meshes(ctx))) return rc;
     }
     return check_err_flag(ctx);
@@ -2887,6 +3064,7 @@ extern "C" int dem_unfix(dem_ctx *ctx, int id)
     for (size_t k = 0; k < ctx->inserts.size(); k++)
         if (ctx->inserts[k]->id == id) {
             ctx->inserts.erase(ctx->inserts.begin() + k);
+            stream_integrator(ctx);
             return DEM_OK;
         }
     return fail(ctx, DEM_ERR_ARG, "dem_unfix: no such fix");

@@ -18,6 +18,8 @@
 //                (FixInsertPack::calc_ninsert_this), fixed-order partials
 // k_tmpl_inside  inside tests of a multisphere template's Monte Carlo points
 // k_extend_rows  empty previous-list rows for appended atoms
+// k_stream_release  fix insert/stream's end_of_step: atoms awaiting release
+//                ride the extrusion at the normal velocity
 #include "kernels.h"
 
 namespace dem {
@@ -28,12 +30,53 @@ __device__ inline int ins_cell_dim(double x, double lo, double inv, int n)
     return c < 0 ? 0 : (c >= n ? n - 1 : c);
 }
 
-__global__ void __launch_bounds__(kBlock) k_ins_bin(const double4 *__restrict__ xr, int n, InsGrid g, int *cell,
-                                                    int *count)
+// TriMesh::isInElement (tri_mesh_I.h:345-368): barycentric coordinates of
+// pos in triangle t, 1e-10 slack
+__device__ inline bool in_element(const double *nd, double p0, double p1, double p2)
+{
+#pragma clang fp contract(off)
+    const double v00 = nd[6] - nd[0], v01 = nd[7] - nd[1], v02 = nd[8] - nd[2];
+    const double v10 = nd[3] - nd[0], v11 = nd[4] - nd[1], v12 = nd[5] - nd[2];
+    const double v20 = p0 - nd[0], v21 = p1 - nd[1], v22 = p2 - nd[2];
+    const double dot00 = v00 * v00 + v01 * v01 + v02 * v02;
+    const double dot01 = v00 * v10 + v01 * v11 + v02 * v12;
+    const double dot02 = v00 * v20 + v01 * v21 + v02 * v22;
+    const double dot11 = v10 * v10 + v11 * v11 + v12 * v12;
+    const double dot12 = v10 * v20 + v11 * v21 + v12 * v22;
+    const double inv = 1. / (dot00 * dot11 - dot01 * dot01);
+    const double u = (dot11 * dot02 - dot01 * dot12) * inv;
+    const double v = (dot00 * dot12 - dot01 * dot02) * inv;
+    return u > -1e-10 && v > -1e-10 && u + v < 1. + 1e-10;
+}
+
+// FixInsertStream::is_nearby (fix_insert_stream.cpp:568-593) and
+// RegionNeighborList::isInBoundingBox (region_neighbor_list_I.h:696-709):
+// the point x + d n, d the signed normal distance, tested on the face
+__device__ inline bool stream_near(const StreamNear &f, double x0, double x1, double x2)
+{
+#pragma clang fp contract(off)
+    const double r0 = x0 - f.pref[0], r1 = x1 - f.pref[1], r2 = x2 - f.pref[2];
+    const double d = r0 * f.nrm[0] + r1 * f.nrm[1] + r2 * f.nrm[2];
+    if (d > f.dmax) return false;
+    if (d < f.dmin) return false;
+    const double q0 = x0 + f.nrm[0] * d, q1 = x1 + f.nrm[1] * d, q2 = x2 + f.nrm[2] * d;
+    bool on = false;
+    for (int t = 0; t < f.ntri && !on; t++) on = in_element(f.face + 9 * t, q0, q1, q2);
+    if (!on) return false;
+    return x0 >= f.bblo[0] && x0 <= f.bbhi[0] && x1 >= f.bblo[1] && x1 <= f.bbhi[1] && x2 >= f.bblo[2]
+           && x2 <= f.bbhi[2];
+}
+
+__global__ void __launch_bounds__(kBlock) k_ins_bin(const double4 *__restrict__ xr, int n, InsGrid g, StreamNear f,
+                                                    int *cell, int *count)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const double4 p = xr[i];
+    if (f.on && !(g.periodic[0] || g.periodic[1] || g.periodic[2]) && !stream_near(f, p.x, p.y, p.z)) {
+        cell[i] = -1;
+        return;
+    }
     const int cx = ins_cell_dim(p.x, g.lo[0], g.inv[0], g.n[0]);
     const int cy = ins_cell_dim(p.y, g.lo[1], g.inv[1], g.n[1]);
     const int cz = ins_cell_dim(p.z, g.lo[2], g.inv[2], g.n[2]);
@@ -45,18 +88,22 @@ __global__ void __launch_bounds__(kBlock) k_ins_bin(const double4 *__restrict__ 
 __global__ void __launch_bounds__(kBlock) k_ins_place(const int *__restrict__ cell, int n, int *cursor, int *idx)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= n || cell[i] < 0) return;
     idx[atomicAdd(&cursor[cell[i]], 1)] = i;
 }
 
 __global__ void __launch_bounds__(kBlock) k_ins_check(const double *__restrict__ cand, int ncand,
                                                       const double *__restrict__ rad, int nrad,
                                                       const double4 *__restrict__ xr, const int *__restrict__ start,
-                                                      const int *__restrict__ idx, InsGrid g, uint32_t *hit)
+                                                      const int *__restrict__ idx, InsGrid g, StreamNear f,
+                                                      uint32_t *hit)
 {
 #pragma clang fp contract(off)
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= ncand) return;
+    // with a periodic dimension the stream filter applies to the position
+    // tested, the image as the reference's ghost atom holds it
+    const bool fimg = f.on && (g.periodic[0] || g.periodic[1] || g.periodic[2]);
     const double x0 = cand[3 * k], x1 = cand[3 * k + 1], x2 = cand[3 * k + 2];
     const int c0 = ins_cell_dim(x0, g.lo[0], g.inv[0], g.n[0]);
     const int c1 = ins_cell_dim(x1, g.lo[1], g.inv[1], g.n[1]);
@@ -92,6 +139,7 @@ __global__ void __launch_bounds__(kBlock) k_ins_check(const double *__restrict__
                     const double px = sx ? p.x + sx * g.prd[0] : p.x;
                     const double py = sy ? p.y + sy * g.prd[1] : p.y;
                     const double pz = sz ? p.z + sz * g.prd[2] : p.z;
+                    if (fimg && !stream_near(f, px, py, pz)) continue;
                     const double d0 = x0 - px, d1 = x1 - py, d2 = x2 - pz;
                     const double rsq = d0 * d0 + d1 * d1 + d2 * d2;
                     for (int t = 0; t < nrad; t++) {
@@ -109,10 +157,11 @@ __global__ void __launch_bounds__(kBlock) k_ins_check(const double *__restrict__
     hit[k] = h;
 }
 
-void launch_ins_bin(const double4 *xr, int n, const InsGrid &g, int *cell, int *count, hipStream_t s)
+void launch_ins_bin(const double4 *xr, int n, const InsGrid &g, const StreamNear &f, int *cell, int *count,
+                    hipStream_t s)
 {
     if (n == 0) return;
-    k_ins_bin<<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(xr, n, g, cell, count);
+    k_ins_bin<<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(xr, n, g, f, cell, count);
 }
 
 void launch_ins_place(const int *cell, int n, int *cursor, int *idx, hipStream_t s)
@@ -122,10 +171,56 @@ void launch_ins_place(const int *cell, int n, int *cursor, int *idx, hipStream_t
 }
 
 void launch_ins_check(const double *cand, int ncand, const double *rad, int nrad, const double4 *xr,
-                      const int *start, const int *idx, const InsGrid &g, uint32_t *hit, hipStream_t s)
+                      const int *start, const int *idx, const InsGrid &g, const StreamNear &f, uint32_t *hit,
+                      hipStream_t s)
 {
     if (ncand == 0) return;
-    k_ins_check<<<(ncand + kBlock - 1) / kBlock, kBlock, 0, s>>>(cand, ncand, rad, nrad, xr, start, idx, g, hit);
+    k_ins_check<<<(ncand + kBlock - 1) / kBlock, kBlock, 0, s>>>(cand, ncand, rad, nrad, xr, start, idx, g, f, hit);
+}
+
+// FixInsertStream::end_of_step (fix_insert_stream.cpp:828-921) per atom:
+// x = x_ins + v_integrate * ((step - i_step) dt), force and torque zeroed;
+// before the release step v = v_integrate and no spin, at it the release
+// velocity and omega
+__global__ void __launch_bounds__(kBlock) k_stream_release(ReleaseArgs a)
+{
+#pragma clang fp contract(off)
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    if (a.mask && !(a.gm[i] & a.mask)) return;
+    const double4 r0 = a.rel[4 * (size_t)i];
+    // MathExtraLiggghts::compDouble(i_step, 0., 1e-13)
+    if (r0.w < 1e-13 && r0.w > -1e-13) return;
+    const double4 r1 = a.rel[4 * (size_t)i + 1];
+    const int i_step = (int)(r0.w + 1e-14);
+    const int r_step = (int)(r1.w + 1e-14);
+    const int step = (int)a.step;
+    if (step > r_step) return;
+    const double te = (double)(step - i_step) * a.dt;
+    double4 x = a.xr[i];
+    x.x = r0.x + r1.x * te;
+    x.y = r0.y + r1.y * te;
+    x.z = r0.z + r1.z * te;
+    a.xr[i] = x;
+    a.f4[i] = make_double4(0.0, 0.0, 0.0, a.f4[i].w);
+    a.t4[i] = make_double4(0.0, 0.0, 0.0, a.t4[i].w);
+    double4 v = a.vm[i], w = a.wt[i];
+    if (r_step == step) {
+        const double4 vr = a.rel[4 * (size_t)i + 2], wr = a.rel[4 * (size_t)i + 3];
+        v.x = vr.x; v.y = vr.y; v.z = vr.z;
+        w.x = wr.x; w.y = wr.y; w.z = wr.z;
+    } else {
+        v.x = r1.x; v.y = r1.y; v.z = r1.z;
+        w.x = 0.0; w.y = 0.0; w.z = 0.0;
+    }
+    a.vm[i] = v;
+    a.wt[i] = w;
+}
+
+void launch_stream_release(const ReleaseArgs &a, hipStream_t s)
+{
+    if (a.n == 0) return;
+    k_stream_release<<<(a.n + kBlock - 1) / kBlock, kBlock, 0, s>>>(a);
 }
 
 // per block b of kRegionBlocks: part[4b..4b+2] = count, sum of 4/3 pi r^3,

@@ -58,17 +58,37 @@ struct Placed {
     double radius, mass;
     int type;
     double brid;
+    // fix insert/stream's release data: (x_ins, i_step), (v_integrate,
+    // r_step), (v_release, 0), (omega_release, 0)
+    bool has_release = false;
+    double release[16];
+};
+
+// The resident particles a fix insert/stream tests against: normal distance
+// d = (x - pref).nrm within [dmin, dmax], x + d nrm on a face triangle (9
+// doubles each in face), x inside [bblo, bbhi] (FixInsertStream::is_nearby,
+// fix_insert_stream.cpp:568-593; FixInsert::load_xnear, fix_insert.cpp:
+// 980-1018).
+struct FaceFilter {
+    double pref[3], nrm[3];
+    double dmin, dmax;
+    double bblo[3], bbhi[3];
+    const std::vector<double> *face;
 };
 
 // What the placement needs from the resident particles.
 struct Hooks {
+    // Modify::get_max_radius (modify_liggghts.cpp:575-620): the largest
+    // radius of any atom or insertion template
+    double maxrad_sim = 0.0;
     // particles with position inside [lo, hi] (inclusive): count, sum of
     // 4/3 pi r^3, sum of mass (FixInsertPack::calc_ninsert_this)
     std::function<int(const double *lo, const double *hi, long long *np, double *vol, double *mass)> region_sums;
     // bit t of hit[k]: a sphere of radius rad[t] at cand[3k..3k+2] overlaps
     // a resident particle, rsq <= (rad[t] + r_j)^2 (RegionNeighborList::
-    // hasOverlap, region_neighbor_list_I.h:72-98), periodic images included
-    std::function<int(const std::vector<double> &cand, const std::vector<double> &rad,
+    // hasOverlap, region_neighbor_list_I.h:72-98), periodic images
+    // included; near != null: only the residents it keeps
+    std::function<int(const std::vector<double> &cand, const std::vector<double> &rad, const FaceFilter *near,
                       std::vector<uint32_t> &hit)>
         overlap;
 };
@@ -109,8 +129,11 @@ class TemplateMultisphere {
     RanPark random_mc_, random_insertion_;
 };
 
+// fix insert/pack, and the FixInsert machinery fix insert/stream shares
+// (the distribution, the batched placement walk, pre_exchange's bookkeeping).
 class InsertPack {
   public:
+    virtual ~InsertPack() = default;
     // Checks the spec, builds the distribution and runs the region volume
     // estimate; returns an error text, empty on success.
     // ms[i]: the multisphere template of template i, or null for a sphere
@@ -125,18 +148,44 @@ class InsertPack {
     int max_type() const;
     double max_radius() const;
     double min_radius() const;
+    virtual bool is_stream() const { return false; }
 
-  private:
+  protected:
+    struct Body {
+        int pti;
+        double x[3], v[3], q[4];
+    };
+    // the spec's template, distribution, seed and keyword fields
+    std::string init_common(const dem_insert_pack_spec &s, const double *boxlo, const double *boxhi,
+                            long long ntimestep, const std::vector<TemplateMultisphere *> &ms);
+    // bodies to insert at this step, after distribute_ninsert_this; < 0:
+    // error (err set)
+    virtual int count_this(const Hooks &h, std::string &err);
+    virtual int calc_maxtry(int ninsert) const;
+    // FixInsert's `!ninsert_exists || ninserted < ninsert`
+    virtual bool more_to_insert() const { return true; }
+    // one attempt's position: draws until it passes the fix's tests or
+    // ntry reaches maxtry
+    virtual void draw_position(double *pos, double rbound, int &ntry, int maxtry);
+    // an attempt's velocity
+    virtual void attempt_velocity(double *v) { gen_velocity(v); }
+    // the placement loop without overlap check
+    virtual void place_unchecked(int ninsert, std::vector<Body> &out);
+    virtual const FaceFilter *near_filter(const Hooks &) { return nullptr; }
+    // after the atoms are created (FixInsert*::finalize_insertion)
+    virtual void finalize(long long step, std::vector<Placed> &out)
+    {
+        (void)step;
+        (void)out;
+    }
+    void initial_quat(double *q) const;
+
     struct Attempt {
         double x[3], v[3], q[4];
         RanPark reg, fix;    // the streams after this attempt
         int ntry;            // tries counted after it
         bool last;           // reached maxtry: no velocity, no test
         int cand;            // its first sphere's candidate index
-    };
-    struct Body {
-        int pti;
-        double x[3], v[3], q[4];
     };
     int calc_ninsert_this(const Hooks &h, std::string &err);
     int randomize_list(int ntotal, long long step);
@@ -172,6 +221,50 @@ class InsertPack {
     long long ninserted_ = 0;
     double massinserted_ = 0.0;
     int last_inserted_ = 0, last_aimed_ = 0, last_tries_ = 0;
+};
+
+// fix insert/stream (fix_insert_stream.cpp): bodies placed in the insertion
+// face extruded against the inflow, the face's points drawn on the mesh's
+// stream (held in reg_random_), released after the time they need to cross
+// the face at the normal velocity.
+class InsertStream : public InsertPack {
+  public:
+    std::string init(const dem_insert_stream_spec &s, const double *boxlo, const double *boxhi, long long ntimestep,
+                     double dt, const std::vector<TemplateMultisphere *> &ms);
+    bool is_stream() const override { return true; }
+    int insert_every() const { return s_.insert_every; }
+    double ninsert_per() const { return ninsert_per_; }
+    const double *normal() const { return nrm_; }
+    const double *v_normal() const { return vn_; }
+    double extrude_length() const { return extrude_; }
+    // the face triangles, 9 doubles each
+    const std::vector<double> &face() const { return face_; }
+
+  protected:
+    int count_this(const Hooks &h, std::string &err) override;
+    int calc_maxtry(int ninsert) const override { return ninsert * s_.maxattempt; }
+    bool more_to_insert() const override { return !ninsert_exists_ || ninserted_ < ninsert_; }
+    void draw_position(double *pos, double rbound, int &ntry, int maxtry) override;
+    void attempt_velocity(double *v) override;
+    void place_unchecked(int ninsert, std::vector<Body> &out) override;
+    const FaceFilter *near_filter(const Hooks &h) override;
+    void finalize(long long step, std::vector<Placed> &out) override;
+
+  private:
+    void face_point(double *pos);
+    void gen_random(double *pos, double rad);
+    bool in_subdomain(const double *pos) const;
+    std::string calc_ins_fraction();
+    std::vector<double> face_, area_acc_;
+    double area_ = 0.0;
+    double nrm_[3] = {0, 0, 0}, vn_[3] = {0, 0, 0}, pref_[3] = {0, 0, 0};
+    double extrude_ = 0.0, ext_min_ = 0.0, ext_max_ = 0.0;
+    double dt_ = 0.0, ninsert_per_ = 0.0, nflowrate_ = 0.0, massflowrate_ = 0.0, massinsert_ = 0.0;
+    int ninsert_ = 0, duration_ = 0;
+    bool ninsert_exists_ = true, parallel_ = false;
+    bool do_fraction_ = true;
+    double ins_fraction_ = 0.0;
+    FaceFilter near_{};
 };
 
 }  // namespace dem_ins

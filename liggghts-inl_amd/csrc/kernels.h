@@ -87,6 +87,8 @@ struct PermuteArgs {
     int *mf1;
     const double *br0;   // bond_random_id per atom (may be null)
     double *br1;
+    const double4 *rl0;  // fix insert/stream release data, 4 per atom (may be null)
+    double4 *rl1;
     uint8_t *wmask;
     int n;
     int wdn;             // nplanes * dnum
@@ -349,11 +351,41 @@ struct InsGrid {
     double lo[3], inv[3], prd[3];
     int n[3], periodic[3];
 };
-void launch_ins_bin(const double4 *xr, int n, const InsGrid &g, int *cell, int *count, hipStream_t s);
+// Which resident particles a fix insert/stream tests its candidates against
+// (FixInsertStream::is_nearby, fix_insert_stream.cpp:568-593, and the
+// bounding box of FixInsert::load_xnear, fix_insert.cpp:980-1018): normal
+// distance d = (x - p_ref).n within [dmin, dmax], x + d n inside a face
+// triangle, x inside [bblo, bbhi].  on = 0: every particle.
+struct StreamNear {
+    int on;
+    double pref[3], nrm[3];
+    double dmin, dmax;
+    double bblo[3], bbhi[3];
+    const double *face;     // 9 per triangle (device)
+    int ntri;
+};
+// cell[i] = -1 (not binned) for a particle the filter rejects; with a
+// periodic dimension the filter runs in the check instead, on the image
+void launch_ins_bin(const double4 *xr, int n, const InsGrid &g, const StreamNear &f, int *cell, int *count,
+                    hipStream_t s);
 void launch_ins_place(const int *cell, int n, int *cursor, int *idx, hipStream_t s);
 // hit[k] bit t: candidate k with radius rad[t] overlaps a binned particle
 void launch_ins_check(const double *cand, int ncand, const double *rad, int nrad, const double4 *xr,
-                      const int *start, const int *idx, const InsGrid &g, uint32_t *hit, hipStream_t s);
+                      const int *start, const int *idx, const InsGrid &g, const StreamNear &f, uint32_t *hit,
+                      hipStream_t s);
+// fix insert/stream's end_of_step on every atom holding a bit of `mask`
+// (every atom when mask = 0).  rel: 4 per atom, (x_ins, i_step),
+// (v_integrate, r_step), (v_release, 0), (omega_release, 0); i_step 0 marks
+// an atom no stream created, atoms past their release step are left alone.
+struct ReleaseArgs {
+    double4 *xr, *vm, *wt, *f4, *t4;
+    const int *gm;
+    const double4 *rel;
+    int n, mask;
+    long long step;
+    double dt;
+};
+void launch_stream_release(const ReleaseArgs &a, hipStream_t s);
 constexpr int kRegionBlocks = 256;
 // part[4b..4b+2]: count, sum 4/3 pi r^3, sum mass of particles in [lo, hi]
 void launch_region_sums(const double4 *xr, const double4 *vm, int n, const double *lo, const double *hi,

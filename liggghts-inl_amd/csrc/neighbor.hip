@@ -289,6 +289,8 @@ __global__ void __launch_bounds__(kBlock) k_permute(PermuteArgs a)
     if (a.xo0) a.xo1[p] = a.xo0[o];
     if (a.mf0) a.mf1[p] = a.mf0[o];
     if (a.br0) a.br1[p] = a.br0[o];
+    if (a.rl0)
+        for (int q = 0; q < 4; q++) a.rl1[4 * (size_t)p + q] = a.rl0[4 * (size_t)o + q];
     a.wmask[p] = (uint8_t)m;
     // a short wall-history row (one batch of loads) goes here; longer ones
     // in k_permute_whist

@@ -95,6 +95,41 @@ class InsertPackSpec(C.Structure):
     ]
 
 
+class InsertStreamSpec(C.Structure):
+    _fields_ = [
+        ("seed", C.c_int),
+        ("dist_seed", C.c_int),
+        ("ntemplates", C.c_int),
+        ("templates", C.POINTER(TemplateSphere)),
+        ("weights", C.POINTER(C.c_double)),
+        ("number_based", C.c_int),
+        ("multisphere", C.POINTER(C.c_int)),
+        ("face_ntri", C.c_int),
+        ("face_nodes", C.POINTER(C.c_double)),
+        ("extrude_length", C.c_double),
+        ("insert_every", C.c_int),
+        ("duration", C.c_int),
+        ("start", C.c_longlong),
+        ("nparticles", C.c_int),
+        ("mass", C.c_double),
+        ("particlerate", C.c_double),
+        ("massrate", C.c_double),
+        ("maxattempt", C.c_int),
+        ("overlapcheck", C.c_int),
+        ("all_in", C.c_int),
+        ("parallel", C.c_int),
+        ("ntry_mc", C.c_int),
+        ("exact_number", C.c_int),
+        ("vel_style", C.c_int),
+        ("vel", C.c_double * 3),
+        ("vel_fluct", C.c_double * 3),
+        ("omega", C.c_double * 3),
+        ("group_mask", C.c_int),
+        ("orientation", C.c_int),
+        ("quat", C.c_double * 4),
+    ]
+
+
 class TemplateMultisphereSpec(C.Structure):
     _fields_ = [
         ("seed", C.c_int),
@@ -178,6 +213,70 @@ def insert_pack_spec(d):
     for k in range(4):
         s.quat[k] = float(q[k])
     return s, (tarr, w, ms)
+
+
+def insert_stream_spec(d):
+    """dem_insert_stream_spec from a dict of the script's keywords:
+
+    seed, dist_seed, templates, weights, number_based, multisphere,
+    orientation, vel, omega, group_mask, maxattempt, overlapcheck, all_in,
+    ntry_mc, random_distribute, start (as insert_pack_spec);
+    insertion_face (triangles, (ntri, 9) or (ntri, 3, 3), scaled and moved
+    as fix mesh/surface leaves them), extrude_length or insert_every
+    [duration], nparticles (or "INF") or mass (or "INF"), particlerate or
+    massrate, parallel.
+
+    Returns (spec, keep): keep holds the arrays the spec points to."""
+    L = library()
+    s = InsertStreamSpec()
+    L.dem_insert_stream_defaults(C.byref(s))
+    tl = d["templates"]
+    tarr = (TemplateSphere * len(tl))()
+    for k, t in enumerate(tl):
+        tarr[k].atom_type = int(t["type"])
+        tarr[k].radius = float(t["radius"])
+        tarr[k].density = float(t["density"])
+    w = np.ascontiguousarray(d.get("weights", [1.0] * len(tl)), dtype=np.float64)
+    s.seed = int(d["seed"])
+    s.dist_seed = int(d["dist_seed"])
+    s.ntemplates = len(tl)
+    s.templates = tarr
+    s.weights = w.ctypes.data_as(C.POINTER(C.c_double))
+    s.number_based = int(d.get("number_based", False))
+    ms = np.ascontiguousarray(d.get("multisphere", [0] * len(tl)), dtype=np.int32)
+    s.multisphere = ms.ctypes.data_as(C.POINTER(C.c_int))
+    face = np.ascontiguousarray(d["insertion_face"], dtype=np.float64).reshape(-1, 9)
+    s.face_ntri = len(face)
+    s.face_nodes = face.ctypes.data_as(C.POINTER(C.c_double))
+    s.extrude_length = float(d.get("extrude_length", 0.0))
+    s.insert_every = int(d.get("insert_every", -1))
+    s.duration = int(d.get("duration", 0))
+    s.start = int(d.get("start", 0))
+    npart = d.get("nparticles", 0)
+    s.nparticles = -1 if npart == "INF" else int(npart)
+    mass = d.get("mass", 0.0)
+    s.mass = -1.0 if mass == "INF" else float(mass)
+    s.particlerate = float(d.get("particlerate", 0.0))
+    s.massrate = float(d.get("massrate", 0.0))
+    s.maxattempt = int(d.get("maxattempt", 50))
+    s.overlapcheck = int(d.get("overlapcheck", True))
+    s.all_in = int(d.get("all_in", False))
+    s.parallel = int(d.get("parallel", False))
+    s.ntry_mc = int(d.get("ntry_mc", 100000))
+    s.exact_number = int(d.get("random_distribute", "exact") == "exact")
+    vel = d.get("vel", ("constant", (0.0, 0.0, 0.0)))
+    s.vel_style = VEL_STYLE[vel[0]]
+    for k in range(3):
+        s.vel[k] = float(vel[1][k])
+        s.vel_fluct[k] = float(vel[2][k]) if len(vel) > 2 else 0.0
+        s.omega[k] = float(d.get("omega", (0.0, 0.0, 0.0))[k])
+    s.group_mask = int(d.get("group_mask", 0))
+    o = d.get("orientation", "template")
+    s.orientation = {"template": 0, "random": 1}.get(o, 2) if isinstance(o, str) else 2
+    q = o[1] if not isinstance(o, str) else (1.0, 0.0, 0.0, 0.0)
+    for k in range(4):
+        s.quat[k] = float(q[k])
+    return s, (tarr, w, ms, face)
 
 
 COHESION = {"off": 0, "bond": 1, "bond/nonlinear": 2}
@@ -280,6 +379,9 @@ def library():
         "dem_time": (C.c_double, [vp]),
         "dem_insert_pack_defaults": (None, [C.POINTER(InsertPackSpec)]),
         "dem_add_insert_pack": (C.c_int, [vp, C.POINTER(InsertPackSpec), ip]),
+        "dem_insert_stream_defaults": (None, [C.POINTER(InsertStreamSpec)]),
+        "dem_add_insert_stream": (C.c_int, [vp, C.POINTER(InsertStreamSpec), ip]),
+        "dem_stream_release_data": (C.c_int, [vp, C.c_int, dp]),
         "dem_insert_vector": (C.c_int, [vp, C.c_int, dp]),
         "dem_append_atoms": (C.c_int, [vp, C.c_int, dp, dp, dp, dp, dp, ip, ip, ip]),
         "dem_natoms": (C.c_int, [vp]),
@@ -528,6 +630,23 @@ class Simulation:
         self._chk(self.L.dem_add_insert_pack(self.h, C.byref(s), C.byref(fid)))
         del keep
         return fid.value
+
+    def add_insert_stream(self, spec):
+        """fix insert/stream (+ its distribution and templates, insertion
+        face); see insert_stream_spec.  Returns the fix id."""
+        s, keep = insert_stream_spec(spec)
+        fid = C.c_int(0)
+        self._chk(self.L.dem_add_insert_stream(self.h, C.byref(s), C.byref(fid)))
+        del keep
+        return fid.value
+
+    def stream_release_data(self, ntags=None):
+        """(ntags, 14) release data by tag - 1: x_ins, i_step, r_step,
+        v_integrate, v_release, omega_release (zero for atoms no stream made)"""
+        nt = (int(self.tag.max()) if self.n else 0) if ntags is None else int(ntags)
+        out = np.zeros((max(nt, 0), 14))
+        self._chk(self.L.dem_stream_release_data(self.h, nt, _dp(out)))
+        return out
 
     def insert_vector(self, fid):
         """ninserted, massinserted, last inserted, last aimed, last tries, next step"""

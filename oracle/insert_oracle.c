@@ -261,7 +261,251 @@ struct oi_fix {
     long long ninserted;
     double massinserted;
     int last_inserted, last_aimed, last_tries;
+    /* fix insert/stream */
+    park mesh_random;               /* the face mesh's stream */
+    double *area_acc, area;
+    double nrm[3], vn[3], pref[3];
+    double extrude, ext_min, ext_max;
+    double ninsert_per;
+    int ninsert, ninsert_exists, do_frac;
+    double ins_fraction;
+    double *rel;                    /* release data of the last insertion */
+    int nrel;
 };
+
+static double o_dot(const double *a, const double *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+static double o_mag(const double *a) { return sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]); }
+
+/* fix insert/stream's set-up: the face's element properties (SurfaceMesh::
+ * calcEdgeVecLen, calcSurfaceNorm, TriMesh::calcArea; surface_mesh_I.h:
+ * 187-217, tri_mesh_I.h:327-338; no degenerate triangles) and
+ * FixInsertStream::calc_insertion_properties (fix_insert_stream.cpp:287-423) */
+static int stream_setup(oi_fix *f)
+{
+    const oi_spec *s = &f->s;
+    const int nt = s->face_ntri;
+    if (nt < 1 || !s->face) {
+        snprintf(g_err, sizeof g_err, "must define an insertion face");
+        return -1;
+    }
+    f->area_acc = (double *)malloc(sizeof(double) * nt);
+    f->area = 0.;
+    double n0[3] = {0, 0, 0};
+    for (int t = 0; t < nt; t++) {
+        const double *nd = s->face + 9 * t;
+        double ev[2][3], len[2];
+        for (int k = 0; k < 2; k++) {
+            for (int q = 0; q < 3; q++) ev[k][q] = nd[3 * (k + 1) + q] - nd[3 * k + q];
+            len[k] = o_mag(ev[k]);
+            const double inv = 1. / len[k];
+            for (int q = 0; q < 3; q++) ev[k][q] = inv * ev[k][q];
+        }
+        double c[3];
+        c[0] = ev[0][1] * ev[1][2] - ev[0][2] * ev[1][1];
+        c[1] = ev[0][2] * ev[1][0] - ev[0][0] * ev[1][2];
+        c[2] = ev[0][0] * ev[1][1] - ev[0][1] * ev[1][0];
+        if (t == 0) {
+            const double inv = 1. / o_mag(c);
+            for (int q = 0; q < 3; q++) n0[q] = inv * c[q];
+        }
+        const double a = 0.5 * o_mag(c) * len[0] * len[1];
+        f->area_acc[t] = t > 0 ? a + f->area_acc[t - 1] : a;
+        f->area += a;
+    }
+    park_reset(&f->mesh_random, 179424799);   /* MultiNodeMesh (multi_node_mesh_I.h:61) */
+    /* the normal of face 0 turned toward the insertion velocity */
+    memcpy(f->nrm, n0, sizeof n0);
+    if (o_dot(s->vel, f->nrm) < 0)
+        for (int q = 0; q < 3; q++) f->nrm[q] = f->nrm[q] * -1.;
+    const double dot = o_dot(s->vel, f->nrm);
+    for (int q = 0; q < 3; q++) f->vn[q] = f->nrm[q] * dot;
+    memcpy(f->pref, s->face, 3 * sizeof(double));
+    const double vnm = o_mag(f->vn);
+    if (vnm < 1.e-3 || o_mag(s->vel) < 1e-5) {
+        snprintf(g_err, sizeof g_err, "insertion velocity too low");
+        return -1;
+    }
+    f->extrude = s->extrude_length;
+    if (f->s.insert_every == -1) {
+        f->s.insert_every = (int)((f->extrude + 1e-14) / (s->dt * vnm));
+        if (f->s.insert_every == 0) {
+            snprintf(g_err, sizeof g_err, "insertion velocity too high or extrude_length too low");
+            return -1;
+        }
+    } else {
+        const int duration = s->duration > 0 ? s->duration : f->s.insert_every;
+        f->extrude = (double)duration * s->dt * vnm;
+    }
+    f->ninsert_exists = !(s->nparticles < 0 || s->mass < 0.);
+    f->ninsert = s->nparticles > 0 ? s->nparticles : 0;
+    if (f->ninsert == 0 && f->ninsert_exists) {
+        if (!(s->mass > 0.)) {
+            snprintf(g_err, sizeof g_err, "must define either 'nparticles' or 'mass'");
+            return -1;
+        }
+        f->ninsert = (int)((s->mass + 1e-14) / f->massexpect);
+    }
+    double nflowrate = s->particlerate;
+    if (nflowrate == 0.) nflowrate = s->massrate / f->massexpect;
+    f->ninsert_per = nflowrate * ((double)f->s.insert_every * s->dt);
+    f->ext_min = 0.;
+    f->ext_max = f->extrude;
+    f->do_frac = 1;
+    return 0;
+}
+
+/* TriMesh::generateRandomOwnedGhost (tri_mesh_I.h:395-428) with
+ * SurfaceMesh::randomOwnedGhostElement and searchElementByAreaAcc
+ * (surface_mesh_I.h:264-295) */
+static int area_search(const double *acc, double a, int lo, int hi)
+{
+    if ((lo < 1 || a > acc[lo - 1]) && (a <= acc[lo])) return lo;
+    if ((hi < 1 || a > acc[hi - 1]) && (a <= acc[hi])) return hi;
+    const int mid = (lo + hi) / 2;
+    if (a > acc[mid]) return area_search(acc, a, mid, hi);
+    return area_search(acc, a, lo, mid);
+}
+
+static void face_point(oi_fix *f, double *pos)
+{
+    const double a = park_uniform(&f->mesh_random) * (f->area + 0.);
+    const int t = area_search(f->area_acc, a, 0, f->s.face_ntri - 1);
+    double u, v;
+    do {
+        u = park_uniform(&f->mesh_random);
+        v = park_uniform(&f->mesh_random);
+    } while (u + v > 1);
+    const double b0 = 1. - u - v, b1 = v, b2 = u;
+    const double *nd = f->s.face + 9 * t;
+    for (int q = 0; q < 3; q++) pos[q] = b0 * nd[q] + b1 * nd[3 + q] + b2 * nd[6 + q];
+}
+
+/* Domain::is_in_subdomain (domain_I.h:65-88), one sub-box padded by
+ * SMALL_DMBRDR (1e-8) on every face */
+static int in_subdomain(const oi_fix *f, const double *p)
+{
+    for (int d = 0; d < 3; d++)
+        if (!(p[d] >= f->s.boxlo[d] - 1.0e-8 && p[d] < f->s.boxhi[d] + 1.0e-8)) return 0;
+    return 1;
+}
+
+/* FixInsertStream::generate_random (fix_insert_stream.cpp:617-641) */
+static void stream_random(oi_fix *f, double *pos, double rad)
+{
+    face_point(f, pos);
+    double r;
+    if (f->s.overlapcheck) r = -1. * (park_uniform(&f->random) * (f->ext_max) + rad + f->ext_min);
+    else r = -1. * (park_uniform(&f->random) * (f->ext_max - 2. * rad) + rad + f->ext_min);
+    for (int q = 0; q < 3; q++) pos[q] = pos[q] + f->nrm[q] * r;
+}
+
+/* FixInsertStream::calc_ins_fraction (fix_insert_stream.cpp:493-556) */
+static int stream_fraction(oi_fix *f)
+{
+    f->do_frac = 0;
+    int n_in = 0;
+    for (int i = 0; i < f->s.ntry_mc; i++) {
+        double pos[3];
+        face_point(f, pos);
+        const double r = -1. * (park_uniform(&f->random) * f->extrude);
+        for (int q = 0; q < 3; q++) pos[q] = pos[q] + f->nrm[q] * r;
+        if (in_subdomain(f, pos)) n_in++;
+    }
+    f->ins_fraction = (double)n_in / (double)f->s.ntry_mc;
+    if (f->s.parallel) {
+        f->ext_min = f->extrude;
+        f->ext_max = 0.;
+        for (int c = 0; c < 8; c++) {
+            double be[3];
+            for (int d = 0; d < 3; d++) be[d] = (((c >> d) & 1) ? f->s.boxhi[d] : f->s.boxlo[d]) - f->pref[d];
+            const double dot = -o_dot(be, f->nrm);
+            if (dot > 0. && dot < f->extrude) {
+                if (dot > f->ext_max) f->ext_max = dot;
+                if (dot < f->ext_min) f->ext_min = dot;
+            } else if (dot < 0.) {
+                f->ext_min = 0.;
+            } else if (dot >= f->extrude) {
+                f->ext_max = f->extrude;
+            }
+        }
+        if (f->ext_min == f->extrude) f->ext_min = 0.;
+        if (f->ext_max == 0.) f->ext_max = f->extrude;
+    }
+    if (f->ins_fraction < 0.9 || f->ins_fraction > 1.1) {
+        snprintf(g_err, sizeof g_err, "insertion volume could not be distributed properly");
+        return -1;
+    }
+    return 0;
+}
+
+/* TriMesh::isInElement (tri_mesh_I.h:345-368) */
+static int in_element(const double *nd, const double *p)
+{
+    double v0[3], v1[3], v2[3];
+    for (int q = 0; q < 3; q++) {
+        v0[q] = nd[6 + q] - nd[q];
+        v1[q] = nd[3 + q] - nd[q];
+        v2[q] = p[q] - nd[q];
+    }
+    const double d00 = o_dot(v0, v0), d01 = o_dot(v0, v1), d02 = o_dot(v0, v2);
+    const double d11 = o_dot(v1, v1), d12 = o_dot(v1, v2);
+    const double inv = 1. / (d00 * d11 - d01 * d01);
+    const double u = (d11 * d02 - d01 * d12) * inv;
+    const double v = (d00 * d12 - d01 * d02) * inv;
+    return u > -1e-10 && v > -1e-10 && u + v < 1. + 1e-10;
+}
+
+/* FixInsertStream::is_nearby (fix_insert_stream.cpp:568-593) and the
+ * bounding box of FixInsert::load_xnear / FixInsertStream::getBoundingBox
+ * (fix_insert.cpp:980-1018, fix_insert_stream.cpp:597-610) */
+static int stream_nearby(const oi_fix *f, const double *x, double maxrad)
+{
+    double rel[3];
+    for (int q = 0; q < 3; q++) rel[q] = x[q] - f->pref[q];
+    const double d = o_dot(rel, f->nrm);
+    if (d > maxrad) return 0;
+    if (d < -(f->extrude + 3. * maxrad)) return 0;
+    double pr[3];
+    for (int q = 0; q < 3; q++) pr[q] = x[q] + f->nrm[q] * d;
+    int on = 0;
+    for (int t = 0; t < f->s.face_ntri && !on; t++) on = in_element(f->s.face + 9 * t, pr);
+    if (!on) return 0;
+    double lo[3], hi[3];
+    for (int q = 0; q < 3; q++) lo[q] = hi[q] = f->s.face[q];
+    for (int k = 0; k < 9 * f->s.face_ntri; k++) {
+        if (f->s.face[k] < lo[k % 3]) lo[k % 3] = f->s.face[k];
+        if (f->s.face[k] > hi[k % 3]) hi[k % 3] = f->s.face[k];
+    }
+    const double cut = 3 * maxrad, delta = -(f->extrude + 2 * cut);
+    const double extend = 3 * maxrad + 2. * f->max_r_bound;
+    for (int q = 0; q < 3; q++) {
+        const double a = lo[q] + delta * f->nrm[q], b = hi[q] + delta * f->nrm[q];
+        if (a < lo[q]) lo[q] = a;
+        if (b > hi[q]) hi[q] = b;
+        if (lo[q] < f->s.boxlo[q]) lo[q] = f->s.boxlo[q];
+        if (hi[q] > f->s.boxhi[q]) hi[q] = f->s.boxhi[q];
+        if (!(x[q] >= lo[q] - extend && x[q] <= hi[q] + extend)) return 0;
+    }
+    return 1;
+}
+
+int oi_release(const oi_fix *f, int cap, double *out)
+{
+    const int n = f->nrel < cap ? f->nrel : cap;
+    if (n > 0) memcpy(out, f->rel, sizeof(double) * 14 * n);
+    return f->nrel;
+}
+
+void oi_stream_info(const oi_fix *f, double out[9])
+{
+    out[0] = f->s.insert_every;
+    out[1] = f->ninsert_per;
+    out[2] = f->ins_fraction;
+    for (int q = 0; q < 3; q++) {
+        out[3 + q] = f->nrm[q];
+        out[6 + q] = f->vn[q];
+    }
+}
 
 /* RegBlock::inside (region_block.cpp:272-277) */
 static int block_inside(const oi_fix *f, const double *x)
@@ -291,13 +535,14 @@ oi_fix *oi_create(const oi_spec *s, long long ntimestep)
         snprintf(g_err, sizeof g_err, "illegal number of templates");
         return NULL;
     }
-    if (s->insert_every < 0) {
+    if (s->insert_every < 0 && !(s->stream && s->insert_every == -1)) {
         snprintf(g_err, sizeof g_err, "must define 'insert_every'");
         return NULL;
     }
     oi_fix *f = (oi_fix *)calloc(1, sizeof(oi_fix));
     f->s = *s;
     const int T = s->ntemplates;
+    if (s->stream) f->s.check_border = 1;
     /* FixTemplateSphere (fix_template_sphere.cpp:243-245), constant radius
      * and density: volume_expect = r^3 * 4 pi / 3, mass_expect = rho * that */
     f->max_r_bound = 0.0;
@@ -354,6 +599,13 @@ oi_fix *oi_create(const oi_spec *s, long long ntimestep)
     park_reset(&f->dist_random, s->dist_seed);
     f->next_reneighbor = s->start > 0 ? s->start : ntimestep + 1;
     f->most_recent = -1;
+    if (s->stream) {
+        if (stream_setup(f)) {
+            oi_destroy(f);
+            return NULL;
+        }
+        return f;
+    }
 
     /* FixInsert::setup -> FixInsertPack::calc_insertion_properties
      * (fix_insert_pack.cpp:281-322): reset the region's stream to
@@ -396,7 +648,13 @@ oi_fix *oi_create(const oi_spec *s, long long ntimestep)
     return f;
 }
 
-void oi_destroy(oi_fix *f) { free(f); }
+void oi_destroy(oi_fix *f)
+{
+    if (!f) return;
+    free(f->area_acc);
+    free(f->rel);
+    free(f);
+}
 
 long long oi_next_step(const oi_fix *f) { return f->next_reneighbor; }
 
@@ -526,6 +784,10 @@ static int body_spheres(const oi_fix *f, int t, const double *pos, const double 
     return ms->n;
 }
 
+static double maxrad_extra = 0.0;
+
+void oi_set_maxrad(double r) { maxrad_extra = r; }
+
 int oi_insert(oi_fix *f, long long step, int n, const double *x, const double *r, const double *m, int cap,
               double *x_out, double *v_out, double *omega_out, double *r_out, double *m_out, int *type_out,
               double *brid_out)
@@ -535,6 +797,24 @@ int oi_insert(oi_fix *f, long long step, int n, const double *x, const double *r
     if (f->next_reneighbor != step || f->most_recent == step) return 0;
     f->most_recent = step;
 
+    int ninsert_this, ninsert_this_local;
+    if (s->stream) {
+        /* FixInsert::calc_ninsert_this (fix_insert.cpp:658-668), the cap in
+         * pre_exchange (:693-700), distribute_ninsert_this (:893-960) with
+         * FixInsertStream::insertion_fraction (fix_insert_stream.cpp:475-486) */
+        ninsert_this = (int)(f->ninsert_per + park_uniform(&f->random));
+        if (f->ninsert_exists && f->ninserted + ninsert_this > f->ninsert)
+            ninsert_this = (int)(f->ninsert - f->ninserted);
+        if (f->ninsert_exists && f->ninserted + ninsert_this >= f->ninsert) {
+            ninsert_this = (int)(f->ninsert - f->ninserted);
+            if (ninsert_this < 0) ninsert_this = 0;
+            f->s.exact_number = 1;
+        }
+        if (f->do_frac && stream_fraction(f)) return -1;
+        ninsert_this_local = ninsert_this;
+        if (!s->exact_number) ninsert_this_local = (int)(f->ins_fraction * (double)ninsert_this + park_uniform(&f->random));
+        goto distribute;
+    }
     /* FixInsertPack::calc_ninsert_this (fix_insert_pack.cpp:333-432); the
      * sums run over the resident particles in the order given */
     int np_region = 0;
@@ -546,7 +826,6 @@ int oi_insert(oi_fix *f, long long step, int n, const double *x, const double *r
             vol_region += _4Pi3 * r[i] * r[i] * r[i];
             mass_region += m[i];
         }
-    int ninsert_this;
     if (s->target == 0) {
         ninsert_this = (int)((s->target_value * f->region_volume - vol_region) / f->volexpect
                              + park_uniform(&f->random));
@@ -565,10 +844,11 @@ int oi_insert(oi_fix *f, long long step, int n, const double *x, const double *r
 
     /* distribute_ninsert_this (fix_insert.cpp:893-970), one process: the
      * exact form keeps the number; the uncorrelated one draws */
-    int ninsert_this_local = ninsert_this;
+    ninsert_this_local = ninsert_this;
     if (!s->exact_number)
         ninsert_this_local = (int)(f->region_volume_local / f->region_volume * (double)ninsert_this
                                    + park_uniform(&f->random));
+distribute:;
 
     /* FixParticledistributionDiscrete::randomize_list
      * (fix_particledistribution_discrete.cpp:388-470) */
@@ -627,17 +907,21 @@ int oi_insert(oi_fix *f, long long step, int n, const double *x, const double *r
     f->last_aimed = npti;
     f->last_inserted = 0;
     f->last_tries = 0;
+    /* FixInsert's `!ninsert_exists || ninserted < ninsert` */
+    const int more = !s->stream || !f->ninsert_exists || f->ninserted < f->ninsert;
+    f->nrel = 0;
     if (npti == 0) {
-        if (s->insert_every) f->next_reneighbor += s->insert_every;
-        else f->next_reneighbor = -1;
+        if (s->insert_every && more) f->next_reneighbor += s->insert_every;
+        else if (s->insert_every == 0) f->next_reneighbor = -1;
         free(pti_t);
         free(pti_b);
         return 0;
     }
 
-    /* FixInsertPack::calc_maxtry (fix_insert_pack.cpp:485-490) */
+    /* FixInsertPack::calc_maxtry (fix_insert_pack.cpp:485-490); the stream
+     * tries ninsert * maxattempt (fix_insert_stream.cpp:680) */
     int maxtry;
-    if (f->insertion_ratio >= 1.) maxtry = ninsert_this_local * s->maxattempt;
+    if (s->stream || f->insertion_ratio >= 1.) maxtry = ninsert_this_local * s->maxattempt;
     else maxtry = (int)((double)(ninsert_this_local * s->maxattempt) / (1. - f->insertion_ratio));
 
     int maxsph = 1;
@@ -661,7 +945,109 @@ int oi_insert(oi_fix *f, long long step, int n, const double *x, const double *r
     /* FixInsertPack::x_v_omega (fix_insert_pack.cpp:498-605) */
     int ninserted_this_local = 0, ntry = 0;
     double mass_inserted_this_local = 0.;
-    if (!s->overlapcheck) {
+    /* the largest radius of any atom or template (Modify::get_max_radius,
+     * modify_liggghts.cpp:575-620) for FixInsertStream::is_nearby */
+    double maxrad = 0.0;
+    if (s->stream) {
+        /* a template's max_rad is its largest sphere
+         * (fix_template_multiplespheres.cpp:556-564, fix_template_sphere.cpp:420-424) */
+        for (int i = 0; i < T; i++) {
+            const oi_tmpl *ms = (const oi_tmpl *)s->ms[i];
+            if (ms)
+                for (int j = 0; j < ms->n; j++) maxrad = ms->r[j] > maxrad ? ms->r[j] : maxrad;
+            else
+                maxrad = s->radius[i] > maxrad ? s->radius[i] : maxrad;
+        }
+        for (int i = 0; i < n; i++) maxrad = r[i] > maxrad ? r[i] : maxrad;
+        if (maxrad_extra > maxrad) maxrad = maxrad_extra;
+    }
+    if (s->stream && !s->overlapcheck) {
+        /* FixInsertStream::x_v_omega without overlap check
+         * (fix_insert_stream.cpp:684-716) */
+        for (int itotal = 0; itotal < ninsert_this_local; itotal++) {
+            const int t = pti_t[ninserted_this_local];
+            double pos[3];
+            do {
+                stream_random(f, pos, f->rbound[t]);
+                ntry++;
+            } while (ntry < maxtry && !in_subdomain(f, pos));
+            if (ntry < maxtry) {
+                if (s->orientation == 1) unit_quat(&f->random, quat_insert);
+                const int k = ninserted_this_local;
+                memcpy(bx + 3 * k, pos, sizeof pos);
+                memcpy(bv + 3 * k, f->vn, 3 * sizeof(double));
+                memcpy(bq + 4 * k, quat_insert, sizeof quat_insert);
+                mass_inserted_this_local += s->ms[t] ? ((oi_tmpl *)s->ms[t])->volume * ((oi_tmpl *)s->ms[t])->density
+                                                     : s->density[t] * (s->radius[t] * s->radius[t] * s->radius[t] * 4. * M_PI / 3.);
+                ninserted_this_local++;
+            }
+        }
+    } else if (s->stream) {
+        /* with overlap check (fix_insert_stream.cpp:717-758): residents that
+         * is_nearby keeps, periodic images as ghosts hold them */
+        double prd[3];
+        int lo[3], hi[3];
+        for (int d = 0; d < 3; d++) {
+            prd[d] = s->boxhi[d] - s->boxlo[d];
+            lo[d] = s->periodic[d] ? -1 : 0;
+            hi[d] = s->periodic[d] ? 1 : 0;
+        }
+        int nnear = 0;
+        double *xnear = (double *)malloc(sizeof(double) * 3 * (size_t)(27 * n + 1));
+        double *rnear = (double *)malloc(sizeof(double) * (size_t)(27 * n + 1));
+        for (int j = 0; j < n; j++)
+            for (int a = lo[0]; a <= hi[0]; a++)
+                for (int b = lo[1]; b <= hi[1]; b++)
+                    for (int c = lo[2]; c <= hi[2]; c++) {
+                        double p[3];
+                        p[0] = a ? x[3 * j] + a * prd[0] : x[3 * j];
+                        p[1] = b ? x[3 * j + 1] + b * prd[1] : x[3 * j + 1];
+                        p[2] = c ? x[3 * j + 2] + c * prd[2] : x[3 * j + 2];
+                        if (!stream_nearby(f, p, maxrad)) continue;
+                        memcpy(xnear + 3 * nnear, p, sizeof p);
+                        rnear[nnear++] = r[j];
+                    }
+        oi_spec flat = *s;
+        for (int d = 0; d < 3; d++) flat.periodic[d] = 0;
+        oi_fix g = *f;
+        g.s = flat;
+        while (ntry < maxtry && ninserted_this_local < ninsert_this_local) {
+            const int t = pti_t[ninserted_this_local];
+            const double rbound = f->rbound[t];
+            int nins = 0;
+            double pos[3];
+            while (nins == 0 && ntry < maxtry) {
+                do {
+                    stream_random(f, pos, rbound);
+                    ntry++;
+                } while (ntry < maxtry && (!in_subdomain(f, pos) || !subbox_dist_ok(f, pos, rbound)));
+                if (ntry >= maxtry) break;
+                if (s->orientation == 1) unit_quat(&f->random, quat_insert);
+                const int ns = body_spheres(f, t, pos, quat_insert, xs, rs);
+                int free_ = 1;
+                for (int j = 0; j < ns && free_; j++)
+                    if (overlaps(&g, xs + 3 * j, rs[j], nnear, xnear, rnear, nnew, xn, rn)) free_ = 0;
+                if (free_) {
+                    for (int j = 0; j < ns; j++) {
+                        memcpy(xn + 3 * nnew, xs + 3 * j, 3 * sizeof(double));
+                        rn[nnew++] = rs[j];
+                    }
+                    nins = ns;
+                }
+            }
+            if (nins > 0) {
+                const int k = ninserted_this_local;
+                memcpy(bx + 3 * k, pos, sizeof pos);
+                memcpy(bv + 3 * k, f->vn, 3 * sizeof(double));
+                memcpy(bq + 4 * k, quat_insert, sizeof quat_insert);
+                mass_inserted_this_local += s->ms[t] ? ((oi_tmpl *)s->ms[t])->volume * ((oi_tmpl *)s->ms[t])->density
+                                                     : s->density[t] * (rbound * rbound * rbound * 4. * M_PI / 3.);
+                ninserted_this_local++;
+            }
+        }
+        free(xnear);
+        free(rnear);
+    } else if (!s->overlapcheck) {
         for (int itotal = 0; itotal < ninsert_this_local; itotal++) {
             const int t = pti_t[ninserted_this_local];
             double pos[3], v[3];
@@ -733,9 +1119,10 @@ int oi_insert(oi_fix *f, long long step, int n, const double *x, const double *r
             if (natoms >= cap) continue;
             memcpy(x_out + 3 * natoms, xs + 3 * j, 3 * sizeof(double));
             memcpy(v_out + 3 * natoms, bv + 3 * k, 3 * sizeof(double));
-            omega_out[3 * natoms] = s->omega[0];
-            omega_out[3 * natoms + 1] = s->omega[1];
-            omega_out[3 * natoms + 2] = s->omega[2];
+            /* insert/stream: set_x_v_omega gets no spin (fix_insert_stream.cpp:677) */
+            omega_out[3 * natoms] = s->stream ? 0. : s->omega[0];
+            omega_out[3 * natoms + 1] = s->stream ? 0. : s->omega[1];
+            omega_out[3 * natoms + 2] = s->stream ? 0. : s->omega[2];
             r_out[natoms] = rs[j];
             if (ms) {
                 m_out[natoms] = 4.18879020479 * rs[j] * rs[j] * rs[j] * ms->density;
@@ -747,11 +1134,35 @@ int oi_insert(oi_fix *f, long long step, int n, const double *x, const double *r
             if (brid_out) brid_out[natoms] = pti_b[k];
         }
     }
+    if (s->stream) {
+        /* FixInsertStream::finalize_insertion (fix_insert_stream.cpp:763-826):
+         * release data per atom, its velocity drawn in atom order */
+        f->rel = (double *)realloc(f->rel, sizeof(double) * 14 * (natoms > 0 ? natoms : 1));
+        f->nrel = natoms;
+        const double vnm = o_mag(f->vn);
+        const int istep = (int)step;
+        for (int i = 0; i < natoms && i < cap; i++) {
+            double pr[3];
+            for (int q = 0; q < 3; q++) pr[q] = f->pref[q] - x_out[3 * i + q];
+            const double dist = o_dot(pr, f->nrm);
+            const int n_steps = (int)((dist + 1e-14) / (vnm * s->dt));
+            double vi[3] = {s->vel[0], s->vel[1], s->vel[2]};
+            random_velocity(f, vi);
+            double *o = f->rel + 14 * i;
+            memcpy(o, x_out + 3 * i, 3 * sizeof(double));
+            o[3] = (double)istep;
+            o[4] = (double)(istep + n_steps);
+            memcpy(o + 5, f->vn, 3 * sizeof(double));
+            memcpy(o + 8, vi, 3 * sizeof(double));
+            memcpy(o + 11, s->omega, 3 * sizeof(double));
+        }
+    }
     f->ninserted += ninserted_this_local;
     f->massinserted += mass_inserted_this_local;
     f->last_inserted = ninserted_this_local;
     f->last_tries = ntry;
-    if (s->insert_every) f->next_reneighbor += s->insert_every;
+    const int more_after = !s->stream || !f->ninsert_exists || f->ninserted < f->ninsert;
+    if (s->insert_every && more_after) f->next_reneighbor += s->insert_every;
     else f->next_reneighbor = 0;
     free(pti_t);
     free(pti_b);

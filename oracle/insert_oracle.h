@@ -1,6 +1,7 @@
 /*
- * insert_oracle.h -- CPU restatement of LIGGGHTS-INL's fix insert/pack with
- * a particledistribution/discrete of particletemplate/sphere templates.
+ * insert_oracle.h -- CPU restatement of LIGGGHTS-INL's fix insert/pack and
+ * fix insert/stream with a particledistribution/discrete of
+ * particletemplate/sphere and particletemplate/multiplespheres templates.
  *
  * TEST INFRASTRUCTURE ONLY: the checker of the engine's insertion.  Only
  * tests/ load it.  The placement loop is the reference's, serial and plain;
@@ -36,6 +37,19 @@ typedef struct {
     void *ms[OI_MAXT];          /* per template: an oi_tmpl (particletemplate/multiplespheres) or NULL */
     int orientation;            /* 0 template, 1 random, 2 constant */
     double quat[4];
+    /* fix insert/stream (stream = 1; the region and target fields are then
+     * unused): the insertion face, 9 doubles per triangle, the extrusion,
+     * the totals and rates, the timestep.  insert_every -1: from
+     * extrude_length; nparticles -1: INF, 0: from mass; mass -1: INF. */
+    int stream;
+    int face_ntri;
+    const double *face;
+    double extrude_length;
+    int duration;
+    int nparticles;
+    double mass, particlerate, massrate;
+    int parallel;
+    double dt;
 } oi_spec;
 
 typedef struct oi_fix oi_fix;
@@ -57,6 +71,16 @@ int oi_insert(oi_fix *f, long long step, int n, const double *x, const double *r
 void oi_vector(const oi_fix *f, double out[6]);
 /* region volume estimate of the fix */
 double oi_region_volume(const oi_fix *f);
+/* fix insert/stream: radius of atoms or templates outside this fix that
+ * Modify::get_max_radius would also see (0: none) */
+void oi_set_maxrad(double r);
+/* fix insert/stream: the release data of the atoms the last insertion
+ * created, 14 doubles each (x_ins, i_step, r_step, v_integrate, v_release,
+ * omega_release); returns their number */
+int oi_release(const oi_fix *f, int cap, double *out14);
+/* fix insert/stream: insert_every, ninsert_per, insertion fraction, face
+ * normal (3), normal velocity (3) */
+void oi_stream_info(const oi_fix *f, double out[9]);
 /* RanPark from `seed`: the integer state after each of n draws */
 void oi_park_states(int seed, int n, int *states, double *values);
 

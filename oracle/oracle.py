@@ -99,6 +99,17 @@ class _InsertSpec(C.Structure):
         ("ms", C.c_void_p * 32),
         ("orientation", C.c_int),
         ("quat", C.c_double * 4),
+        ("stream", C.c_int),
+        ("face_ntri", C.c_int),
+        ("face", C.POINTER(C.c_double)),
+        ("extrude_length", C.c_double),
+        ("duration", C.c_int),
+        ("nparticles", C.c_int),
+        ("mass", C.c_double),
+        ("particlerate", C.c_double),
+        ("massrate", C.c_double),
+        ("parallel", C.c_int),
+        ("dt", C.c_double),
     ]
 
 
@@ -170,6 +181,10 @@ def lib():
         L.oi_next_step.argtypes = [C.c_void_p]
         L.oi_insert.argtypes = [C.c_void_p, C.c_longlong, C.c_int, dp, dp, dp, C.c_int, dp, dp, dp, dp, dp, ip, dp]
         L.oi_vector.argtypes = [C.c_void_p, dp]
+        L.oi_release.restype = C.c_int
+        L.oi_release.argtypes = [C.c_void_p, C.c_int, dp]
+        L.oi_stream_info.argtypes = [C.c_void_p, dp]
+        L.oi_set_maxrad.argtypes = [C.c_double]
         L.oi_region_volume.restype = C.c_double
         L.oi_region_volume.argtypes = [C.c_void_p]
         L.oi_park_states.argtypes = [C.c_int, C.c_int, ip, dp]
@@ -574,6 +589,87 @@ class InsertPackOracle:
         if k < 0:
             raise RuntimeError("oracle insertion failed")
         return {key: val[:k].copy() for key, val in o.items()}
+
+
+class InsertStreamOracle(InsertPackOracle):
+    """fix insert/stream restated (insert_oracle.c).  spec: the keyword dict
+    of the engine's Simulation.add_insert_stream (with multisphere_oracle for
+    clump templates); box: (lo, hi, periodic); dt: the timestep."""
+
+    def __init__(self, spec, box_lo, box_hi, periodic, dt, ntimestep=0):
+        self.L = lib()
+        s = _InsertSpec()
+        tl = spec["templates"]
+        w = spec.get("weights", [1.0] * len(tl))
+        s.seed = int(spec["seed"])
+        s.dist_seed = int(spec["dist_seed"])
+        s.ntemplates = len(tl)
+        s.number_based = int(spec.get("number_based", False))
+        for k, t in enumerate(tl):
+            s.type[k] = int(t["type"])
+            s.radius[k] = float(t["radius"])
+            s.density[k] = float(t["density"])
+            s.weight[k] = float(w[k])
+        for d in range(3):
+            s.boxlo[d] = float(box_lo[d])
+            s.boxhi[d] = float(box_hi[d])
+            s.periodic[d] = int(periodic[d])
+        s.maxattempt = int(spec.get("maxattempt", 50))
+        s.insert_every = int(spec.get("insert_every", -1))
+        s.start = int(spec.get("start", 0))
+        s.overlapcheck = int(spec.get("overlapcheck", True))
+        s.check_border = 1
+        s.ntry_mc = int(spec.get("ntry_mc", 100000))
+        s.exact_number = int(spec.get("random_distribute", "exact") == "exact")
+        vel = spec.get("vel", ("constant", (0.0, 0.0, 0.0)))
+        s.vel_style = _VEL[vel[0]]
+        for d in range(3):
+            s.vel[d] = float(vel[1][d])
+            s.vel_fluct[d] = float(vel[2][d]) if len(vel) > 2 else 0.0
+            s.omega[d] = float(spec.get("omega", (0.0, 0.0, 0.0))[d])
+        self.templates = list(spec.get("multisphere_oracle", [None] * len(tl)))
+        for k, t in enumerate(self.templates):
+            s.ms[k] = t.h if t is not None else None
+        o = spec.get("orientation", "template")
+        s.orientation = {"template": 0, "random": 1}.get(o, 2) if isinstance(o, str) else 2
+        q = o[1] if not isinstance(o, str) else (1.0, 0.0, 0.0, 0.0)
+        for d in range(4):
+            s.quat[d] = float(q[d])
+        self.face = np.ascontiguousarray(spec["insertion_face"], dtype=np.float64).reshape(-1, 9)
+        s.stream = 1
+        s.face_ntri = len(self.face)
+        s.face = _dp(self.face)
+        s.extrude_length = float(spec.get("extrude_length", 0.0))
+        s.duration = int(spec.get("duration", 0))
+        npart = spec.get("nparticles", 0)
+        s.nparticles = -1 if npart == "INF" else int(npart)
+        mass = spec.get("mass", 0.0)
+        s.mass = -1.0 if mass == "INF" else float(mass)
+        s.particlerate = float(spec.get("particlerate", 0.0))
+        s.massrate = float(spec.get("massrate", 0.0))
+        s.parallel = int(spec.get("parallel", False))
+        s.dt = float(dt)
+        self.h = self.L.oi_create(C.byref(s), int(ntimestep))
+        if not self.h:
+            raise RuntimeError(self.L.oi_error().decode())
+
+    def insert(self, step, x, radius, rmass, cap=1 << 20, maxrad=0.0):
+        """as InsertPackOracle.insert, plus the release data (k, 14) of the
+        created atoms; maxrad: a radius outside this fix that the nearby
+        test sees (other templates)"""
+        self.L.oi_set_maxrad(float(maxrad))
+        o = InsertPackOracle.insert(self, step, x, radius, rmass, cap)
+        k = len(o["radius"])
+        rel = np.zeros((max(k, 1), 14))
+        self.L.oi_release(self.h, k, _dp(rel))
+        o["release"] = rel[:k].copy()
+        return o
+
+    def info(self):
+        """insert_every, ninsert_per, insertion fraction, normal (3), normal velocity (3)"""
+        out = np.zeros(9)
+        self.L.oi_stream_info(self.h, _dp(out))
+        return out
 
 
 class TemplateOracle:

@@ -219,7 +219,8 @@ def driver():
     subprocess.run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
                     "-I", os.path.join(ROOT, "include"), "-o", out,
                     os.path.join(ROOT, "tests", "insert_host_driver.cpp"),
-                    os.path.join(ROOT, "liggghts-inl_amd", "csrc", "insert_pack.cpp")], check=True)
+                    os.path.join(ROOT, "liggghts-inl_amd", "csrc", "insert_pack.cpp"),
+                    os.path.join(ROOT, "liggghts-inl_amd", "csrc", "mesh_host.cpp")], check=True)
     L = C.CDLL(out)
     dp = C.POINTER(C.c_double)
     ip = C.POINTER(C.c_int)
@@ -232,7 +233,10 @@ def driver():
     L.ih_next_step.argtypes = [C.c_void_p]
     L.ih_vector.argtypes = [C.c_void_p, dp]
     L.ih_insert.argtypes = [C.c_void_p, C.c_longlong, C.c_int, dp, dp, dp, C.c_int, dp, dp, dp, dp, dp, ip, dp,
-                            ip]
+                            ip, dp, C.c_double]
+    L.ih_stream_create.restype = C.c_void_p
+    L.ih_stream_create.argtypes = [C.POINTER(eng.engine.InsertStreamSpec), dp, dp, ip, C.c_longlong, C.c_double,
+                                   C.POINTER(C.c_void_p), C.c_char_p, C.c_int]
     _DRV = L
     return L
 
@@ -244,32 +248,42 @@ def _dp(a):
 class HostLoop:
     """spec["multisphere_host"]: per template an it_create handle or None"""
 
-    def __init__(self, spec, box=BOX, periodic=(0, 0, 0), ntimestep=0):
+    def __init__(self, spec, box=BOX, periodic=(0, 0, 0), ntimestep=0, stream_dt=None):
         eng = load_engine()
         self.L = driver()
-        self.s, self.keep = eng.engine.insert_pack_spec(spec)
+        self.stream = stream_dt is not None
+        if self.stream:
+            self.s, self.keep = eng.engine.insert_stream_spec(spec)
+        else:
+            self.s, self.keep = eng.engine.insert_pack_spec(spec)
         hs = spec.get("multisphere_host")
         self.ms = None if hs is None else (C.c_void_p * len(hs))(*hs)
         lo = np.ascontiguousarray(box[0], dtype=np.float64)
         hi = np.ascontiguousarray(box[1], dtype=np.float64)
         pe = np.ascontiguousarray(periodic, dtype=np.int32)
         err = C.create_string_buffer(256)
-        self.h = self.L.ih_create(C.byref(self.s), _dp(lo), _dp(hi), pe.ctypes.data_as(C.POINTER(C.c_int)),
-                                  int(ntimestep), self.ms, err, 256)
+        if self.stream:
+            self.h = self.L.ih_stream_create(C.byref(self.s), _dp(lo), _dp(hi), pe.ctypes.data_as(C.POINTER(C.c_int)),
+                                             int(ntimestep), float(stream_dt), self.ms, err, 256)
+        else:
+            self.h = self.L.ih_create(C.byref(self.s), _dp(lo), _dp(hi), pe.ctypes.data_as(C.POINTER(C.c_int)),
+                                      int(ntimestep), self.ms, err, 256)
         if not self.h:
             raise RuntimeError(err.value.decode())
 
-    def insert(self, step, x, r, m, cap=1 << 18):
+    def insert(self, step, x, r, m, cap=1 << 18, maxrad=0.0):
         x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)
         r = np.ascontiguousarray(r, dtype=np.float64)
         m = np.ascontiguousarray(m, dtype=np.float64)
         o = dict(x=np.zeros((cap, 3)), v=np.zeros((cap, 3)), omega=np.zeros((cap, 3)), radius=np.zeros(cap),
                  rmass=np.zeros(cap), type=np.zeros(cap, np.int32), brid=np.zeros(cap))
+        if self.stream:
+            o["release"] = np.zeros((cap, 14))
         nb = C.c_int(0)
         k = self.L.ih_insert(self.h, int(step), len(r), _dp(x) if len(r) else None, _dp(r) if len(r) else None,
                              _dp(m) if len(r) else None, cap, _dp(o["x"]), _dp(o["v"]), _dp(o["omega"]),
                              _dp(o["radius"]), _dp(o["rmass"]), o["type"].ctypes.data_as(C.POINTER(C.c_int)),
-                             _dp(o["brid"]), C.byref(nb))
+                             _dp(o["brid"]), C.byref(nb), _dp(o["release"]) if self.stream else None, float(maxrad))
         assert k >= 0
         self.batches = nb.value
         return {key: val[:k].copy() for key, val in o.items()}
