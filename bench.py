@@ -257,7 +257,9 @@ def bench_fibres(pkg, args):
     FP64).  Reported beside the headline metric."""
     spec, atoms = pkg.workloads.fibres()
     n = len(atoms["radius"])
-    s = pkg.Simulation(spec, atoms, precision="fp64")
+    # graph stepping with the distance check on the device (neigh_modify
+    # check yes): periods halt on the device when a rebuild is due
+    s = pkg.Simulation(dict(spec, graph=True), atoms, precision="fp64")
     s.setup()
     s.step(args.warmup)
     s.sync()

@@ -294,6 +294,8 @@ __global__ void __launch_bounds__(kBlock) k_bond(BondArgs a)
 {
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= a.n) return;
+    if (a.halt && *a.halt) return;
+    const long long ntimestep = a.step_dev ? *a.step_dev : a.ntimestep;
     const Model &M = a.M;
     const double4 pi = a.xr[i];
     const double4 vmi = a.vm[i];
@@ -330,8 +332,8 @@ __global__ void __launch_bounds__(kBlock) k_bond(BondArgs a)
             continue;
         }
         const bool bonded = (w & kBondBit) != 0;
-        if (!bonded && !(a.shearupdate && (M.bs.create_always || a.ntimestep == M.bs.ts_create
-                                           || (a.brid && (double)a.ntimestep < a.brid[i]))))
+        if (!bonded && !(a.shearupdate && (M.bs.create_always || ntimestep == M.bs.ts_create
+                                           || (a.brid && (double)ntimestep < a.brid[i]))))
             continue;
         const double4 vmj = a.vm[j];
         const double4 wj = a.wt[j];
@@ -341,7 +343,7 @@ __global__ void __launch_bounds__(kBlock) k_bond(BondArgs a)
         double out[9];
         bool created = false;
         const int res = bond_eval(M, a.mat, it, meta_type((int)wj.w), hp + off, delta, rsq, pi.w, pj.w, xi, vi, vj, omi, omj,
-                                  a.shearupdate != 0, a.ntimestep, a.brid, i, j, created, out);
+                                  a.shearupdate != 0, ntimestep, a.brid, i, j, created, out);
         if (created) {
             // an entry that carried no flag holds no valid contact history
             if (!(w & (kFlagBit | kBondBit)))

@@ -194,6 +194,18 @@ struct dem_ctx {
     double4 *rel[2] = {};
     long long rel_until = -1;
     int rel_mask = 0;
+    // checked graph periods (neigh_modify check yes): ctl[0], ctl[1] the
+    // current step, alternately; ctl[2] the step whose distance check asked
+    // for a rebuild; while such a period is being recorded halt_dev = ctl + 2
+    // and step_dev = the recorded step's counter
+    long long *ctl = nullptr;
+    long long ctl_h[3] = {0, 0, 0};
+    const long long *halt_dev = nullptr;
+    const long long *step_dev = nullptr;
+    hipGraphExec_t cgexec = nullptr;
+    std::vector<const void *> cg_key;
+    int cg_period = 0;
+    int cg_flip = 0;    // 1: each recorded step swaps the state buffers (fused integrator)
     double *d_gpart = nullptr, *d_gtot = nullptr;   // partial sums; 4 per op slot
     int ago = 0;
 
@@ -496,7 +508,7 @@ static void free_all(dem_ctx *ctx)
     }
     F(ctx->f4); F(ctx->t4); F(ctx->rec); F(ctx->xhold); F(ctx->wmask); F(ctx->ncontact); F(ctx->key);
     F(ctx->sorted); F(ctx->cstart); F(ctx->cursor); F(ctx->scan_tmp); F(ctx->spread); F(ctx->stage); F(ctx->mstage);
-    F(ctx->d_mat); F(ctx->d_small); F(ctx->mf.part); F(ctx->mf.total);
+    F(ctx->d_mat); F(ctx->d_small); F(ctx->mf.part); F(ctx->mf.total); F(ctx->ctl);
     F(ctx->bout); F(ctx->bvalid); F(ctx->brev);
     F(ctx->pwf); F(ctx->d_obs); F(ctx->d_gtot); F(ctx->d_gpart);
     F(ctx->ins_cell); F(ctx->ins_start); F(ctx->ins_idx); F(ctx->ins_tmp); F(ctx->ins_cand); F(ctx->ins_rad);
@@ -504,6 +516,7 @@ static void free_all(dem_ctx *ctx)
     for (char *c : ctx->ckpts) F(c);
     F(ctx->tape); F(ctx->adj_rev);
     if (ctx->gexec) hipGraphExecDestroy(ctx->gexec);
+    if (ctx->cgexec) hipGraphExecDestroy(ctx->cgexec);
     for (auto &m : ctx->meshes) {
         F(m.tri); F(m.enorm); F(m.gstart); F(m.glist);
         for (int b = 0; b < 2; b++) {
@@ -1299,6 +1312,7 @@ static int force_pass(dem_ctx *ctx, int shearupdate, int integ = 0)
     fa.obs_plane = ctx->obs_plane;
     fa.lanes = ctx->tune_lanes;
     fa.block = ctx->tune_block;
+    fa.halt = ctx->halt_dev;
     const bool fuse = integ && lastwall < 0 && fuse_possible(ctx) && force_can_fuse(fa);
     const bool mfuse = integ && lastwall >= 0 && fuse_possible(ctx);
     if (fuse) {
@@ -1325,6 +1339,8 @@ static int force_pass(dem_ctx *ctx, int shearupdate, int integ = 0)
         ba.n = ctx->n;
         ba.shearupdate = shearupdate;
         ba.ntimestep = ctx->ntimestep;
+        ba.halt = ctx->halt_dev;
+        ba.step_dev = ctx->step_dev;
         ba.M = ctx->M;
         launch_bond(ba, ctx->stream);
         KCHK();
@@ -1351,6 +1367,7 @@ static int force_pass(dem_ctx *ctx, int shearupdate, int integ = 0)
         MeshDev &m = ctx->meshes[k];
         if (m.passive) continue;
         MeshForceArgs ma{};
+        ma.halt = ctx->halt_dev;
         ma.xr = fa.xr; ma.vm = fa.vm; ma.wt = fa.wt;
         ma.f4 = fa.f4; ma.t4 = fa.t4;
         ma.tri = m.tri;
@@ -1421,16 +1438,14 @@ static int force_pass(dem_ctx *ctx, int shearupdate, int integ = 0)
     return DEM_OK;
 }
 
-// fix move/mesh linear, MeshMoverLinear::initial_integrate: the device nodes
-// and the host copy get the same additions
-static int move_meshes(dem_ctx *ctx)
+// the host side of move_meshes: the host copy of the nodes, the
+// displacement since the grid build, p_ref
+static void move_meshes_host(dem_ctx *ctx)
 {
     for (auto &m : ctx->meshes) {
         if (!m.moving) continue;
         double dx[3];
         for (int q = 0; q < 3; q++) dx[q] = m.vel[q] * ctx->p.dt;
-        launch_mesh_move(m.tri, m.h.ntri, dx[0], dx[1], dx[2], ctx->stream);
-        KCHK();
         for (int t = 0; t < m.h.ntri; t++) {
             for (int k = 0; k < 9; k++) m.h.node[9 * t + k] = m.h.node[9 * t + k] + dx[k % 3];
             for (int q = 0; q < 3; q++) m.h.center[3 * t + q] = m.h.center[3 * t + q] + dx[q];
@@ -1439,6 +1454,21 @@ static int move_meshes(dem_ctx *ctx)
         // p_ref moves with the mesh (frame_general, mesh_module_stress.cpp:71)
         for (int q = 0; q < 3; q++) m.pref[q] += dx[q];
     }
+}
+
+// fix move/mesh linear, MeshMoverLinear::initial_integrate: the device nodes
+// and the host copy get the same additions (a captured graph records the
+// device part; the host part is applied per replayed step)
+static int move_meshes(dem_ctx *ctx)
+{
+    for (auto &m : ctx->meshes) {
+        if (!m.moving) continue;
+        double dx[3];
+        for (int q = 0; q < 3; q++) dx[q] = m.vel[q] * ctx->p.dt;
+        launch_mesh_move(m.tri, m.h.ntri, dx[0], dx[1], dx[2], ctx->stream, ctx->halt_dev);
+        KCHK();
+    }
+    if (!ctx->gexec_capturing) move_meshes_host(ctx);
     return DEM_OK;
 }
 
@@ -1492,6 +1522,7 @@ static int integrate(dem_ctx *ctx, int mode)
     ia.dtv = ctx->p.dt;
     ia.dtfrotate = ia.dtf / 0.4;
     ia.integ_bit = ctx->integ_bit;
+    ia.halt = ctx->halt_dev;
     ia.mv_bit = -1;
     if (ctx->integ_bit >= 0 || !ctx->moves.empty()) ia.gm = ctx->gmask[c];
     if (!ctx->moves.empty()) {
@@ -2538,6 +2569,176 @@ static int capture_graph(dem_ctx *ctx)
     return DEM_OK;
 }
 
+// ---- checked graph periods (neigh_modify check yes, every 1, delay 0)
+//
+// Neighbor::decide's distance check runs on the device inside the recorded
+// steps: the first step whose check asks for a rebuild halts the period (the
+// step's own kernels and every later one return at once), and the host,
+// reading the halt word once per batch of periods, goes on eagerly from that
+// step, whose check then asks again.  Rebuilds therefore never run inside
+// such a graph, and a moving mesh's trigger is foreseen on the host (its
+// motion is known), so no host round trip is needed per step.
+
+static bool checked_graph_ok(const dem_ctx *ctx)
+{
+    if (!ctx->use_graph || ctx->window || !ctx->fops.empty() || !ctx->moves.empty() || !ctx->p.neigh_check
+        || ctx->p.neigh_every != 1 || ctx->p.neigh_delay != 0 || ctx->mf.mesh >= 0 || ctx->timing || ctx->n == 0
+        || ctx->tape_pos >= 0)
+        return false;
+    // a moving stress mesh's p_ref is a per-step kernel argument
+    for (const auto &m : ctx->meshes)
+        if (m.passive || (m.moving && m.stress)) return false;
+    return true;
+}
+
+static std::vector<const void *> checked_key(const dem_ctx *ctx)
+{
+    std::vector<const void *> k = graph_key(ctx);
+    k.push_back(ctx->ctl);
+    k.push_back(ctx->xhold);
+    k.push_back(ctx->f4);
+    k.push_back(ctx->t4);
+    k.push_back(ctx->brev);
+    k.push_back(ctx->bout);
+    k.push_back(ctx->bvalid);
+    k.push_back(ctx->brid[0]);
+    k.push_back(ctx->brid[1]);
+    k.push_back((const void *)(intptr_t)ctx->par);
+    for (const auto &m : ctx->meshes) {
+        k.push_back((const void *)(intptr_t)m.moving);
+        key_bytes(k, m.vel, sizeof(m.vel));
+    }
+    return k;
+}
+
+// Record P steps (P even), each: the distance check (which also advances
+// the step counter), the force pass with final_integrate(n) +
+// initial_integrate(n+1), the mesh move.
+static int capture_checked_graph(dem_ctx *ctx, int P)
+{
+    if (ctx->cgexec) {
+        HIPCHK(hipGraphExecDestroy(ctx->cgexec));
+        ctx->cgexec = nullptr;
+    }
+    if (!ctx->ctl && dev_alloc(ctx, &ctx->ctl, 3)) return DEM_ERR_HIP;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    const int cur0 = ctx->cur, lcur0 = ctx->lcur, par0 = ctx->par;
+    const double trig = 0.5 * ctx->p.skin;
+    hipGraph_t graph = nullptr;
+    HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    ctx->gexec_capturing = true;
+    ctx->halt_dev = ctx->ctl + 2;
+    int rc = DEM_OK, flip = 0;
+    for (int k = 0; k < P && rc == DEM_OK; k++) {
+        ctx->step_dev = ctx->ctl + (k & 1);
+        launch_check_halt(sx(ctx), ctx->xhold, ctx->n, trig * trig, ctx->ctl, k & 1, ctx->stream);
+        const int pb = ctx->par;
+        rc = force_pass(ctx, 1, 3);
+        if (k == 0) flip = ctx->par != pb;
+        if (rc == DEM_OK) rc = move_meshes(ctx);
+    }
+    ctx->halt_dev = nullptr;
+    ctx->step_dev = nullptr;
+    ctx->gexec_capturing = false;
+    const hipError_t ec = hipStreamEndCapture(ctx->stream, &graph);
+    const bool same_par = ctx->par == par0;
+    ctx->cur = cur0;
+    ctx->lcur = lcur0;
+    ctx->par = par0;
+    if (rc == DEM_OK && ec == hipSuccess && !same_par) {
+        if (graph) hipGraphDestroy(graph);
+        return fail(ctx, DEM_ERR_STATE, "checked graph: the recorded period changes the state buffers");
+    }
+    if (rc) {
+        if (graph) hipGraphDestroy(graph);
+        return rc;
+    }
+    if (ec != hipSuccess) return fail(ctx, DEM_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(ec));
+    const hipError_t ei = hipGraphInstantiate(&ctx->cgexec, graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    if (ei != hipSuccess) {
+        ctx->cgexec = nullptr;
+        return fail(ctx, DEM_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(ei));
+    }
+    ctx->cg_period = P;
+    ctx->cg_flip = flip;
+    ctx->cg_key = checked_key(ctx);
+    return DEM_OK;
+}
+
+// Of the next `room` steps, how many may run before a moving mesh's trigger
+// forces a rebuild: the steps through the first one at which a node has
+// moved more than half the skin since the last rebuild (its trigger makes
+// the step after it rebuild).  The nodes advance as the host's copy will.
+static long long mesh_horizon(const dem_ctx *ctx, long long room)
+{
+    const double trig = 0.25 * ctx->p.skin * ctx->p.skin;
+    for (const auto &m : ctx->meshes) {
+        if (!m.moving) continue;
+        double dx[3];
+        for (int q = 0; q < 3; q++) dx[q] = m.vel[q] * ctx->p.dt;
+        std::vector<double> nd = m.h.node;   // the nodes of the next step
+        for (long long k = 1; k <= room; k++) {
+            if (k > 1)
+                for (size_t q = 0; q < nd.size(); q++) nd[q] = nd[q] + dx[q % 3];
+            bool hit = false;
+            for (size_t q = 0; q < nd.size() / 3 && !hit; q++) {
+                const double ex = nd[3 * q] - m.node_re[3 * q];
+                const double ey = nd[3 * q + 1] - m.node_re[3 * q + 1];
+                const double ez = nd[3 * q + 2] - m.node_re[3 * q + 2];
+                hit = ex * ex + ey * ey + ez * ez > trig;
+            }
+            if (hit) {
+                room = k;
+                break;
+            }
+        }
+    }
+    return room;
+}
+
+// Whole checked periods from iteration st (step ntimestep + 1) on, at most
+// up to the step before the last, the next insertion or forced rebuild.
+// Advances st and ntimestep by the steps that completed.
+static int checked_periods(dem_ctx *ctx, int nsteps, int &st)
+{
+    constexpr int P = 16;
+    constexpr long long kMaxRun = 64 * P;   // steps per host read-back
+    long long room = (long long)(nsteps - 1) - st;
+    room = std::min(room, next_insert_step(ctx) - 1 - ctx->ntimestep);
+    if (ctx->next_reneighbor > ctx->ntimestep) room = std::min(room, ctx->next_reneighbor - 1 - ctx->ntimestep);
+    room = std::min(room, kMaxRun);
+    if (room < P) return DEM_OK;
+    room = mesh_horizon(ctx, room);
+    const long long R = room / P;
+    if (R < 1) return DEM_OK;
+    if (!ctx->cgexec || ctx->cg_key != checked_key(ctx)) {
+        const int rc = capture_checked_graph(ctx, P);
+        if (rc) return rc;
+    }
+    const long long t0 = ctx->ntimestep;
+    ctx->ctl_h[0] = t0 + 1;   // the first recorded step (P is even: every period starts at ctl[0])
+    ctx->ctl_h[1] = 0;
+    ctx->ctl_h[2] = 0;
+    HIPCHK(hipMemcpyAsync(ctx->ctl, ctx->ctl_h, sizeof(ctx->ctl_h), hipMemcpyHostToDevice, ctx->stream));
+    for (long long r = 0; r < R; r++) HIPCHK(hipGraphLaunch(ctx->cgexec, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->ctl_h, ctx->ctl, sizeof(ctx->ctl_h), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    const long long halt = ctx->ctl_h[2];
+    const long long done = halt ? halt - t0 - 1 : R * P;
+    // the host side of each completed step: Neighbor::decide's count, the
+    // mesh trigger (foreseen: it can fire on the last one only), the mesh move
+    for (long long k = 0; k < done; k++) {
+        ctx->ntimestep++;
+        ctx->ago++;
+        mesh_trigger(ctx);
+        move_meshes_host(ctx);
+    }
+    if (ctx->cg_flip && (done & 1)) ctx->par ^= 1;
+    st += (int)done;
+    return DEM_OK;
+}
+
 extern "C" int dem_step(dem_ctx *ctx, int nsteps)
 {
     if (!ctx) return fail(nullptr, DEM_ERR_ARG, "null context");
@@ -2546,8 +2747,13 @@ extern "C" int dem_step(dem_ctx *ctx, int nsteps)
     sync_enter(ctx);
     int rc;
     const bool gok = graph_ok(ctx);
+    const bool cgok = !gok && checked_graph_ok(ctx);
     bool split_prev = false;   // the last step ran a stream release: no fused initial half
     for (int st = 0; st < nsteps; st++) {
+        if (cgok && st >= 1 && !split_prev && ctx->ntimestep + 1 != ctx->next_reneighbor
+            && ctx->ntimestep > ctx->rel_until) {
+            if ((rc = checked_periods(ctx, nsteps, st))) return rc;
+        }
         // whole graph periods while they fit before the last step, starting
         // right after a rebuild
         // an insertion step runs eagerly: graph periods end before it, and

@@ -214,6 +214,7 @@ __device__ inline void integrate_into(const ForceArgs &a, int i, double4 x, doub
 template <bool MIXED, int G, int NM, int WN, bool CH = false>
 __global__ void __launch_bounds__(kBlock) k_force(ForceArgs a)
 {
+    if (a.halt && *a.halt) return;
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const int tid = blk * blockDim.x + threadIdx.x;
     const int g = threadIdx.x & (G - 1);
@@ -370,6 +371,7 @@ __global__ void __launch_bounds__(kBlock) k_force_q(ForceArgs a)
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const int p0 = (blk * (int)(blockDim.x >> 6) + wv) * P;
     if (p0 >= a.n) return;   // uniform over the wave
+    if (a.halt && *a.halt) return;
     const int np = min(P, a.n - p0);
     const Model &M = a.M;
     const int dnum = M.dnum;
@@ -718,6 +720,7 @@ __global__ void __launch_bounds__(kBlock) k_integrate(IntegArgs a)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.n) return;
+    if (a.halt && *a.halt) return;
     double4 x = a.xr[i];
     double4 v = a.vm[i];
     double4 w = a.wt[i];
@@ -782,6 +785,28 @@ __global__ void __launch_bounds__(kBlock) k_check_distance(const double4 *xr, co
     const double dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
     const double rsq = dx * dx + dy * dy + dz * dz;
     if (rsq > deltasq) atomicOr(flag, 1);
+}
+
+__global__ void __launch_bounds__(kBlock) k_check_halt(const double4 *xr, const double4 *xhold, int n,
+                                                      double deltasq, long long *ctl, int par)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const long long *cc = ctl;
+    if (cc[2]) return;
+    const long long step = cc[par];
+    if (i == 0) ctl[par ^ 1] = step + 1;
+    const double4 a = xr[i], b = xhold[i];
+    const double dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
+    const double rsq = dx * dx + dy * dy + dz * dz;
+    if (rsq > deltasq) atomicCAS((unsigned long long *)&ctl[2], 0ull, (unsigned long long)step);
+}
+
+void launch_check_halt(const double4 *xr, const double4 *xhold, int n, double deltasq, long long *ctl, int par,
+                       hipStream_t s)
+{
+    if (n == 0) return;
+    k_check_halt<<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(xr, xhold, n, deltasq, ctl, par);
 }
 
 void launch_check_distance(const double4 *xr, const double4 *xhold, int n, double deltasq, int *flag,

@@ -33,6 +33,9 @@ struct ForceArgs {
     // mixed law: the partners' velocity/mass and spin/meta as one 32-byte
     // FP32 record per particle (2 float4), or null to read vm/wt
     const float4 *rec;
+    // a checked graph period: nonzero *halt (the step whose distance check
+    // asked for a rebuild) makes the pass return at once; null: always run
+    const long long *halt;
     Model M;
     Planes P;
 };
@@ -54,6 +57,10 @@ struct BondArgs {
     const double *brid;
     int n, shearupdate;
     long long ntimestep;
+    // in a checked graph period: the step counter on the device (read
+    // instead of ntimestep) and the halt word, as ForceArgs::halt
+    const long long *step_dev;
+    const long long *halt;
     Model M;
 };
 
@@ -144,6 +151,7 @@ struct IntegArgs {
     double mv_val[3];
     double mv_delta;             // (step - time_origin) * dt of the initial half
     const double4 *xo;           // positions at the fix's creation
+    const long long *halt;       // as ForceArgs::halt
 };
 
 // Mesh wall neighbour build: candidate triangles per particle through a
@@ -210,6 +218,7 @@ struct MeshForceArgs {
     // the mesh and that force's torque about pref (6 doubles), or null
     double *mstress;
     double pref[3];
+    const long long *halt;   // as ForceArgs::halt
     Model M;
 };
 
@@ -223,7 +232,8 @@ void launch_mesh_compact(const MeshNbArgs &a, hipStream_t s);
 void launch_mesh_remap(const MeshRemapArgs &a, hipStream_t s);
 void launch_mesh_force(const MeshForceArgs &a, hipStream_t s);
 // MeshMoverLinear: node += dx, center += dx for ntri triangles
-void launch_mesh_move(TriGeo *tri, int ntri, double dx, double dy, double dz, hipStream_t s);
+void launch_mesh_move(TriGeo *tri, int ntri, double dx, double dy, double dz, hipStream_t s,
+                      const long long *halt = nullptr);
 
 // Adjoint window (adjoint.hip).
 struct AdjMaskArgs {
@@ -341,6 +351,13 @@ void launch_sum6(const double *v, int n, double *part, double *out, hipStream_t 
 int scan_tmp_size(int n);
 // Neighbor::check_distance (neighbor.cpp:1390-1465): flag |= 1 if any atom
 // moved more than sqrt(deltasq) since the last build.
+// A checked graph period keeps the current step in ctl[par] (par alternates
+// over the recorded steps) and in ctl[2] the step whose distance check asked
+// for a rebuild (0: none yet).  k_check_halt, unless halted, writes the next
+// step to ctl[par ^ 1] and records the step when an atom moved more than
+// sqrt(deltasq).
+void launch_check_halt(const double4 *xr, const double4 *xhold, int n, double deltasq, long long *ctl, int par,
+                       hipStream_t s);
 void launch_check_distance(const double4 *xr, const double4 *xhold, int n, double deltasq, int *flag,
                            hipStream_t s);
 

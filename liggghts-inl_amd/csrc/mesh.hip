@@ -179,6 +179,7 @@ __global__ void __launch_bounds__(kBlock) k_mesh_force(MeshForceArgs a)
     const int gshift = (threadIdx.x & 63) & ~(G - 1);
     const int i = (blockIdx.x * kBlock + threadIdx.x) / G;
     if (i >= a.n) return;   // uniform across the group
+    if (a.halt && *a.halt) return;
     const Model &M = a.M;
     const int base = a.mstart[i];
     const int nn = a.mstart[i + 1] - base;
@@ -378,10 +379,12 @@ __global__ void __launch_bounds__(kBlock) k_mesh_force(MeshForceArgs a)
     }
 }
 
-__global__ void __launch_bounds__(kBlock) k_mesh_move(TriGeo *tri, int ntri, double dx, double dy, double dz)
+__global__ void __launch_bounds__(kBlock) k_mesh_move(TriGeo *tri, int ntri, double dx, double dy, double dz,
+                                                    const long long *halt)
 {
     const int t = blockIdx.x * kBlock + threadIdx.x;
     if (t >= ntri) return;
+    if (halt && *halt) return;
     TriGeo &g = tri[t];
     for (int k = 0; k < 3; k++) {
         g.node[3 * k] = g.node[3 * k] + dx;
@@ -433,9 +436,9 @@ void launch_mesh_force(const MeshForceArgs &a, hipStream_t s)
     }
 }
 
-void launch_mesh_move(TriGeo *tri, int ntri, double dx, double dy, double dz, hipStream_t s)
+void launch_mesh_move(TriGeo *tri, int ntri, double dx, double dy, double dz, hipStream_t s, const long long *halt)
 {
-    if (ntri) k_mesh_move<<<blocks(ntri), kBlock, 0, s>>>(tri, ntri, dx, dy, dz);
+    if (ntri) k_mesh_move<<<blocks(ntri), kBlock, 0, s>>>(tri, ntri, dx, dy, dz, halt);
 }
 
 // Fixed-order sum of a per-particle 6-vector: each of kSumBlocks blocks
