@@ -15,7 +15,11 @@ insertion), gravity, Hertz with tangential history, neigh_modify delay 0:
   exactly, with v = v_n, no spin, zero force and torque; at its release
   step it takes the insertion velocity;
 * graph stepping gives the same run (graphs stay off while atoms await
-  release, the integrator un-fused).
+  release, the integrator un-fused);
+* the same in a box periodic in x and y whose x span is the face's (the
+  stream's nearby test then runs in the overlap check on each periodic
+  image), and with a three-sphere clump template in the distribution,
+  placed with random orientations.
 """
 import numpy as np
 import pytest
@@ -29,17 +33,33 @@ pytestmark = pytest.mark.gpu
 DT = 1e-5
 
 
-def stream_run(graph=False):
+CLUMP_X = np.array([[0.0, 0.0, 0.0], [0.012, 0.0, 0.0], [0.0, 0.01, 0.004]])
+CLUMP_R = np.array([0.01, 0.008, 0.006])
+
+
+def stream_run(case="closed", graph=False):
     eng = load_engine()
-    spec = dict(box_lo=[-1.0, -1.0, -0.05], box_hi=[1.0, 1.0, 2.85], periodic=[0, 0, 0], dt=DT, skin=0.01,
+    lo, hi, per = [-1.0, -1.0, -0.05], [1.0, 1.0, 2.85], [0, 0, 0]
+    if case == "periodic":
+        lo, hi, per = [-0.15, 0.1, -0.05], [0.15, 0.55, 2.85], [1, 1, 0]
+    spec = dict(box_lo=lo, box_hi=hi, periodic=per, dt=DT, skin=0.01,
                 cdf=1.0, neigh_every=1, neigh_delay=0, neigh_check=True, rolling="off",
                 materials=materials(Y=5e6, nu=0.45, e=0.9, mu=0.05), planes=[(2, -0.05, 1)],
                 gravity=(0.0, 0.0, -9.81), graph=graph)
     ins = tutorial(extrude_length=0.12, nparticles=240)
     s = eng.Simulation(spec, empty_atoms())
+    ins_o = ins
+    if case == "clumps":
+        from oracle import TemplateOracle
+
+        tid = s.add_template_multisphere(15485863, 1, 2500.0, CLUMP_X, CLUMP_R, ntry=20000)
+        ins = dict(ins, templates=[dict(type=1, radius=0.015, density=2500.0),
+                                   dict(type=1, radius=0.02, density=2500.0)],
+                   multisphere=[0, tid], orientation="random")
+        ins_o = dict(ins, multisphere_oracle=[None, TemplateOracle(15485863, CLUMP_X, CLUMP_R, ntry=20000)])
     fid = s.add_insert_stream(dict(ins, group_mask=0))
     s.setup()
-    return s, fid, ins
+    return s, fid, ins_o, (lo, hi, per)
 
 
 def check_pending(s, t):
@@ -57,11 +77,12 @@ def check_pending(s, t):
     return int(pend.sum())
 
 
-def test_stream_insertion_and_release_match_oracle():
+@pytest.mark.parametrize("case", ["closed", "periodic", "clumps"])
+def test_stream_insertion_and_release_match_oracle(case):
     from oracle import InsertStreamOracle
 
-    s, fid, ins = stream_run()
-    o = InsertStreamOracle(ins, [-1.0, -1.0, -0.05], [1.0, 1.0, 2.85], [0, 0, 0], DT)
+    s, fid, ins, (lo, hi, per) = stream_run(case)
+    o = InsertStreamOracle(ins, lo, hi, per, DT)
     dtf = 0.5 * DT
     n_before = 0
     for event in range(4):
@@ -95,14 +116,14 @@ def test_stream_insertion_and_release_match_oracle():
         # mid-way to the next insertion
         s.step(2500)
         check_pending(s, s.timestep())
-    assert n_before == 240 and o.next_step() == 0
+    assert o.next_step() == 0 and (n_before == 240 or case == "clumps")
     s.step(7000)   # every atom released
     assert check_pending(s, s.timestep()) == 0
     eager = s.state()
     steps = s.timestep()
     s.close()
 
-    g, _, _ = stream_run(graph=True)
+    g = stream_run(case, graph=True)[0]
     g.step(steps)
     gs = g.state()
     for key in ("x", "v", "omega"):

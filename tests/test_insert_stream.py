@@ -78,14 +78,16 @@ def on_footprint(p, tris):
 
 
 def advance(atoms, step):
-    """residents for the next insertion: each atom rides its release data
-    (x_ins + v_int (t - i_step) dt) up to its release step, then moves on
-    with the release velocity; no forces"""
+    """residents for the next insertion: each atom the stream made rides
+    its release data (x_ins + v_int (t - i_step) dt) up to its release step,
+    then moves on with the release velocity; no forces"""
     rel = atoms["release"]
     x = np.empty_like(atoms["x"])
     for k in range(len(x)):
         i, r = int(rel[k, 3]), int(rel[k, 4])
-        if step <= r:
+        if i == 0:                       # not made by the stream: stays put
+            x[k] = atoms["x"][k]
+        elif step <= r:
             x[k] = rel[k, :3] + rel[k, 5:8] * ((step - i) * DT)
         else:
             x[k] = rel[k, :3] + rel[k, 5:8] * ((r - i) * DT) + rel[k, 8:11] * ((step - r) * DT)
@@ -294,3 +296,30 @@ def test_host_stream_equals_oracle(case):
     hl.close()
     if host_ms:
         L.it_destroy(h)
+
+
+def test_host_stream_equals_oracle_periodic():
+    """x and y periodic, the box's x span that of the face: residents'
+    images are tested through is_nearby on the image position"""
+    box = ((-0.15, 0.1, -0.05), (0.15, 0.55, 2.85))
+    spec = tutorial(nparticles=900)
+    o = oracle(spec, box=box, periodic=(1, 1, 0))
+    hl = HostLoop(spec, box=box, periodic=(1, 1, 0), stream_dt=DT)
+    # residents near the periodic faces, inside the extrusion
+    extra = dict(x=np.array([[0.148, 0.3, FACE_Z + 0.2], [-0.149, 0.25, FACE_Z + 0.3], [0.0, 0.545, FACE_Z + 0.1]]),
+                 radius=np.array([0.02, 0.03, 0.025]), rmass=np.array([0.08, 0.3, 0.16]), release=np.zeros((3, 14)))
+    atoms = dict(extra)
+    for event in range(3):
+        t = o.next_step()
+        assert hl.next_step() == t
+        x = advance(atoms, t)
+        ro = o.insert(t, x, atoms["radius"], atoms["rmass"])
+        rh = hl.insert(t, x, atoms["radius"], atoms["rmass"])
+        ro.pop("brid")
+        rh.pop("brid")
+        same(rh, ro)
+        assert len(ro["radius"]) > 100
+        for k in atoms:
+            atoms[k] = np.concatenate([atoms[k], ro[k]])
+    hl.close()
+
