@@ -104,6 +104,20 @@ struct TriGeo {
 };
 static_assert(sizeof(TriGeo) == 256, "TriGeo is one 256-byte record");
 
+// MeshMoverLinear's step of one triangle: every node and the centre moved by
+// (dx, dy, dz), one addition per component
+__device__ inline void move_tri(TriGeo &g, double dx, double dy, double dz)
+{
+    for (int k = 0; k < 3; k++) {
+        g.node[3 * k] = g.node[3 * k] + dx;
+        g.node[3 * k + 1] = g.node[3 * k + 1] + dy;
+        g.node[3 * k + 2] = g.node[3 * k + 2] + dz;
+    }
+    g.center[0] = g.center[0] + dx;
+    g.center[1] = g.center[1] + dy;
+    g.center[2] = g.center[2] + dz;
+}
+
 struct Model {
     double dt;
     double cdf;          // contactDistanceFactor

@@ -206,6 +206,12 @@ struct dem_ctx {
     std::vector<const void *> cg_key;
     int cg_period = 0;
     int cg_flip = 0;    // 1: each recorded step swaps the state buffers (fused integrator)
+    // the integrator's own pass also checks the next step and moves the
+    // meshes (fold_par: the recorded step's counter); cg_fold: the recorded
+    // period does so, and its first check runs before the first replay
+    bool fold = false;
+    int fold_par = 0;
+    bool cg_fold = false;
     double *d_gpart = nullptr, *d_gtot = nullptr;   // partial sums; 4 per op slot
     int ago = 0;
 
@@ -1523,6 +1529,20 @@ static int integrate(dem_ctx *ctx, int mode)
     ia.dtfrotate = ia.dtf / 0.4;
     ia.integ_bit = ctx->integ_bit;
     ia.halt = ctx->halt_dev;
+    if (ctx->fold && mode == 3) {
+        ia.ctl = ctx->ctl;
+        ia.par = ctx->fold_par;
+        ia.xhold = ctx->xhold;
+        const double trig = 0.5 * ctx->p.skin;   // as decide() squares it
+        ia.deltasq = trig * trig;
+        for (auto &m : ctx->meshes) {
+            if (!m.moving) continue;
+            ia.mtri[ia.nmove] = m.tri;
+            ia.mntri[ia.nmove] = m.h.ntri;
+            for (int q = 0; q < 3; q++) ia.mdx[ia.nmove][q] = m.vel[q] * ctx->p.dt;
+            ia.nmove++;
+        }
+    }
     ia.mv_bit = -1;
     if (ctx->integ_bit >= 0 || !ctx->moves.empty()) ia.gm = ctx->gmask[c];
     if (!ctx->moves.empty()) {
@@ -2613,9 +2633,14 @@ static std::vector<const void *> checked_key(const dem_ctx *ctx)
 
 // Record P steps (P even), each: the distance check (which also advances
 // the step counter), the force pass with final_integrate(n) +
-// initial_integrate(n+1), the mesh move.
+// initial_integrate(n+1), the mesh move.  When the integrator has its own
+// pass (no fused integration) that pass does the next step's check and the
+// mesh moves instead (two launches fewer per step).
 static int capture_checked_graph(dem_ctx *ctx, int P)
 {
+    int nmoving = 0;
+    for (const auto &m : ctx->meshes) nmoving += m.moving ? 1 : 0;
+    const bool fold = !fuse_possible(ctx) && nmoving <= kFoldMeshes;
     if (ctx->cgexec) {
         HIPCHK(hipGraphExecDestroy(ctx->cgexec));
         ctx->cgexec = nullptr;
@@ -2631,14 +2656,18 @@ static int capture_checked_graph(dem_ctx *ctx, int P)
     int rc = DEM_OK, flip = 0;
     for (int k = 0; k < P && rc == DEM_OK; k++) {
         ctx->step_dev = ctx->ctl + (k & 1);
-        launch_check_halt(sx(ctx), ctx->xhold, ctx->n, trig * trig, ctx->ctl, k & 1, ctx->stream);
+        if (!fold) launch_check_halt(sx(ctx), ctx->xhold, ctx->n, trig * trig, ctx->ctl, k & 1, ctx->stream);
         const int pb = ctx->par;
+        ctx->fold = fold;
+        ctx->fold_par = k & 1;
         rc = force_pass(ctx, 1, 3);
+        ctx->fold = false;
         if (k == 0) flip = ctx->par != pb;
-        if (rc == DEM_OK) rc = move_meshes(ctx);
+        if (rc == DEM_OK && !fold) rc = move_meshes(ctx);
     }
     ctx->halt_dev = nullptr;
     ctx->step_dev = nullptr;
+    ctx->fold = false;
     ctx->gexec_capturing = false;
     const hipError_t ec = hipStreamEndCapture(ctx->stream, &graph);
     const bool same_par = ctx->par == par0;
@@ -2662,6 +2691,7 @@ static int capture_checked_graph(dem_ctx *ctx, int P)
     }
     ctx->cg_period = P;
     ctx->cg_flip = flip;
+    ctx->cg_fold = fold;
     ctx->cg_key = checked_key(ctx);
     return DEM_OK;
 }
@@ -2721,6 +2751,13 @@ static int checked_periods(dem_ctx *ctx, int nsteps, int &st)
     ctx->ctl_h[1] = 0;
     ctx->ctl_h[2] = 0;
     HIPCHK(hipMemcpyAsync(ctx->ctl, ctx->ctl_h, sizeof(ctx->ctl_h), hipMemcpyHostToDevice, ctx->stream));
+    // a folded period checks each next step at the end of the step before:
+    // the first step's check runs here
+    if (ctx->cg_fold) {
+        const double trig = 0.5 * ctx->p.skin;
+        launch_check_halt(sx(ctx), ctx->xhold, ctx->n, trig * trig, ctx->ctl, 0, ctx->stream);
+        KCHK();
+    }
     for (long long r = 0; r < R; r++) HIPCHK(hipGraphLaunch(ctx->cgexec, ctx->stream));
     HIPCHK(hipMemcpyAsync(ctx->ctl_h, ctx->ctl, sizeof(ctx->ctl_h), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));

@@ -719,8 +719,25 @@ template <int MODE>
 __global__ void __launch_bounds__(kBlock) k_integrate(IntegArgs a)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    long long step = 0;
+    if (a.ctl) {
+        // a checked period: nothing runs once it halted at or before this
+        // step (a later halt is the one this pass may record itself)
+        step = a.ctl[a.par];
+        const long long h = a.ctl[2];
+        if (h != 0 && h <= step) {
+            // the counter still advances, so that a later recorded step
+            // reads a number past the halt
+            if (i == 0) a.ctl[a.par ^ 1] = step + 1;
+            return;
+        }
+        for (int m = 0; m < a.nmove; m++)
+            if (i < a.mntri[m]) move_tri(a.mtri[m][i], a.mdx[m][0], a.mdx[m][1], a.mdx[m][2]);
+        if (i == 0) a.ctl[a.par ^ 1] = step + 1;
+    } else if (a.halt && *a.halt) {
+        return;
+    }
     if (i >= a.n) return;
-    if (a.halt && *a.halt) return;
     double4 x = a.xr[i];
     double4 v = a.vm[i];
     double4 w = a.wt[i];
@@ -747,6 +764,12 @@ __global__ void __launch_bounds__(kBlock) k_integrate(IntegArgs a)
             w.x += dtirotate * t.x; w.y += dtirotate * t.y; w.z += dtirotate * t.z;
         }
         a.xr[i] = x;
+        if (a.ctl) {   // Neighbor::check_distance of the next step
+            const double4 b = a.xhold[i];
+            const double dx = x.x - b.x, dy = x.y - b.y, dz = x.z - b.z;
+            if (dx * dx + dy * dy + dz * dz > a.deltasq)
+                atomicCAS((unsigned long long *)&a.ctl[2], 0ull, (unsigned long long)(step + 1));
+        }
     }
     a.vm[i] = v;
     a.wt[i] = w;
@@ -770,7 +793,9 @@ void launch_pack_rec(const double4 *vm, const double4 *wt, float4 *rec, int n, h
 void launch_integrate(const IntegArgs &a, int mode, hipStream_t s)
 {
     if (a.n == 0) return;
-    const int nb = (a.n + kBlock - 1) / kBlock;
+    int nt = a.n;
+    for (int m = 0; m < a.nmove; m++) nt = nt > a.mntri[m] ? nt : a.mntri[m];
+    const int nb = (nt + kBlock - 1) / kBlock;
     if (mode == 1) k_integrate<1><<<nb, kBlock, 0, s>>>(a);
     else if (mode == 2) k_integrate<2><<<nb, kBlock, 0, s>>>(a);
     else k_integrate<3><<<nb, kBlock, 0, s>>>(a);

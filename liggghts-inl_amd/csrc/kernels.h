@@ -152,7 +152,20 @@ struct IntegArgs {
     double mv_delta;             // (step - time_origin) * dt of the initial half
     const double4 *xo;           // positions at the fix's creation
     const long long *halt;       // as ForceArgs::halt
+    // a checked graph period with the integrator in its own pass (mode 3):
+    // ctl as in launch_check_halt; the pass runs unless the period halted
+    // before this step, then checks the next step's positions (the next
+    // step's decision) and moves up to kFoldMeshes meshes by their step
+    long long *ctl;
+    int par;
+    const double4 *xhold;
+    double deltasq;
+    int nmove;
+    TriGeo *mtri[2];
+    int mntri[2];
+    double mdx[2][3];
 };
+constexpr int kFoldMeshes = 2;
 
 // Mesh wall neighbour build: candidate triangles per particle through a
 // uniform grid of triangles (built on the host in the mesh's initial frame;

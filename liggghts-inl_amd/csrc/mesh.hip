@@ -385,15 +385,7 @@ __global__ void __launch_bounds__(kBlock) k_mesh_move(TriGeo *tri, int ntri, dou
     const int t = blockIdx.x * kBlock + threadIdx.x;
     if (t >= ntri) return;
     if (halt && *halt) return;
-    TriGeo &g = tri[t];
-    for (int k = 0; k < 3; k++) {
-        g.node[3 * k] = g.node[3 * k] + dx;
-        g.node[3 * k + 1] = g.node[3 * k + 1] + dy;
-        g.node[3 * k + 2] = g.node[3 * k + 2] + dz;
-    }
-    g.center[0] = g.center[0] + dx;
-    g.center[1] = g.center[1] + dy;
-    g.center[2] = g.center[2] + dz;
+    move_tri(tri[t], dx, dy, dz);
 }
 
 static int blocks(long long n) { return (int)((n + kBlock - 1) / kBlock); }
