@@ -37,6 +37,8 @@ typedef struct {
     int flag;    /* contact flags (CONTACT_* bits) */
 } entry_t;
 
+#define OR_MAX_SLOTS 64
+
 /* history layout of one model chain; slots are registered in constructor
  * order surface, normal, cohesion, tangential, rolling (contact_models.h:
  * 150-157) */
@@ -49,7 +51,9 @@ typedef struct {
     int off_shear;   /* 3 slots */
     int off_roll;    /* 3 slots or -1 */
     int dnum;
-    int newton[32];  /* newtonflag per slot (fix_contact_history.cpp:404-408) */
+    /* newtonflag per slot (fix_contact_history.cpp:404-408); the longest
+     * chain, hysteretic + bond/nonlinear + shear + rolling, has 46 slots */
+    int newton[OR_MAX_SLOTS];
 } model_t;
 
 typedef struct {

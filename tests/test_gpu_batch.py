@@ -9,7 +9,14 @@ the autotuner, on the MI355X.
 * per-instance observables equal numpy sums over the downloaded state, and
   the observed plane's force equals, for spheres that touch only the floor,
   f_z - m g_z of each sphere;
-* the autotuner leaves the state and the contact history unchanged.
+* the autotuner leaves the state and the contact history unchanged;
+* stream ordering: graph-stepped chunks interleaved with downloads of the
+  state and the per-instance observables, and with a gravity change, each
+  read right after the step call returns (no synchronisation by the
+  caller), equal bitwise to the same chunks stepped eagerly.  The context's
+  stream does not order against the null stream the copies use; a download
+  that did not wait for the graph would read a state the graph is still
+  writing.
 """
 import numpy as np
 import pytest
@@ -239,3 +246,26 @@ def test_graph_follows_setters():
     # the gravity change and the larger spheres do show in the state
     assert np.abs(plain[1]["v"][:, 1]).max() > 0
     assert not np.array_equal(plain[2]["f"], plain[1]["f"])
+
+
+def test_graph_chunks_interleaved_with_host_copies():
+    spec, atoms, inst, params = small_batch(ninst=8, nmax=3000, seed=11)
+    eng = load_engine()
+    runs = []
+    for graph in (False, True):
+        sp = dict(spec, instances=dict(inst=inst, **params), observe_plane=4, graph=graph)
+        s = eng.Simulation(sp, atoms, precision="fp64")
+        s.setup()
+        seen = []
+        for k, chunk in enumerate([20, 7, 33, 20, 1, 40, 19]):
+            if k == 3:
+                s.set_gravity(0.0, 0.0, -4.0)
+            s.step(chunk)
+            st = s.state()
+            seen.append((by_tag(st, st["tag"]), s.observables()))
+        runs.append(seen)
+        s.close()
+    for (ge, oe), (gg, og) in zip(*runs):
+        for key in ge:
+            assert np.array_equal(ge[key], gg[key]), key
+        assert np.array_equal(oe, og)

@@ -17,6 +17,13 @@ little (kinetic energy 6e-7 relative, 2e-6 for a 1e-12 change; mean height
 3e-8; every bond intact; 1,000 plate contact slots).  Bars: the same bond
 count and plate contact count, kinetic energy within 1e-4 relative, mean
 height within 1e-6, positions within 1e-5 m (2 % of the radius).
+
+The configuration's full size (500 chains, 20,000 spheres, 19,500 bonds)
+with bonds breaking: maxDistance cut to 1.0005 d and Gaussian velocities of
+0.2 m/s per component, so about a third of the bonds break within 300
+steps.  The device breaks the same bonds as the oracle (the bonded pair
+sets by tag are equal), and positions, forces and pair histories agree to
+the bars of the two-layer test.
 """
 import numpy as np
 import pytest
@@ -85,3 +92,39 @@ def test_fibres_full_run_bulk():
     assert dke <= 1e-4
     assert dmz <= 1e-6
     assert dx <= 1e-5
+
+
+def test_fibres_full_size_bonds_break():
+    eng = load_engine()
+    spec, atoms = eng.workloads.fibres(nchains=500, nsph=40)
+    d = 2 * atoms["radius"][0]
+    bond = dict(spec["bond"])
+    tables = dict(bond["tables"])
+    tables["maxDistance"] = np.full((3, 3), 1.0005 * d)
+    bond["tables"] = tables
+    spec = dict(spec, bond=bond)
+    atoms = dict(atoms, v=np.random.default_rng(7).normal(0.0, 0.2, atoms["x"].shape))
+    o = Oracle(spec, atoms)
+    s = eng.Simulation(spec, atoms)
+    o.setup()
+    s.setup()
+    o.run(1)
+    s.step(1)
+    assert nbonded(o.pairs()[3]) == nbonded(s.pairs()[3]) == 2 * 500 * 39
+    o.run(299)
+    s.step(299)
+
+    def bonded_set(sim):
+        ti, tj, _, h = sim.pairs()[:4]
+        on = h[:, 12] > 0.5 if len(h) else np.zeros(0, bool)
+        return {(min(a, b), max(a, b)) for a, b in zip(np.asarray(ti)[on].tolist(), np.asarray(tj)[on].tolist())}
+
+    bg, bo = bonded_set(s), bonded_set(o)
+    print("intact bonds", len(bg), len(bo), "of", 500 * 39, flush=True)
+    assert bg == bo
+    assert 0.3 * 500 * 39 < len(bo) < 0.9 * 500 * 39   # a sizeable share broke
+    g, r = s.state(), o.state()
+    assert np.abs(g["x"] - r["x"]).max() <= 1e-12 * 0.04
+    assert np.abs(g["v"] - r["v"]).max() <= 1e-9 * np.abs(r["v"]).max()
+    check_forces(s, o, 1e-9)
+    check_pairs(s, o, hist_tol=1e-8)

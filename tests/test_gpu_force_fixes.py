@@ -9,7 +9,7 @@
   atom order in the reference: 1e-12 relative).
 * The reference's own gold for the INL hysteretic examples
   (examples/LIGGGHTS/INL/normal_contact_hysteretic_nonlinear{1,2}): the
-  first load stages replayed on the device through the same script
+  20 load stages replayed on the device through the same script
   semantics the oracle test follows (a set-up pass per run, `unfix
   loading` + a new aveforce per run, the time bookkeeping of update.cpp),
   every printed row compared with the gold CSV.  The engine applies the
@@ -133,8 +133,8 @@ def replay_gpu(model, nstages):
 
 
 @pytest.mark.parametrize("model,fname,nstages", [
-    ("hysteretic/nonlinear1", "hysteretic_nonlinear1_out_loading.csv", 3),
-    ("hysteretic/nonlinear2", "hysteretic_nonlinear2_out_loading.csv", 2),
+    ("hysteretic/nonlinear1", "hysteretic_nonlinear1_out_loading.csv", 20),
+    ("hysteretic/nonlinear2", "hysteretic_nonlinear2_out_loading.csv", 20),
 ])
 def test_hysteretic_gold_on_device(model, fname, nstages):
     g = gold(fname)[: 100 * nstages]
@@ -143,10 +143,10 @@ def test_hysteretic_gold_on_device(model, fname, nstages):
     scale = np.abs(g).max(0)
     err = (np.abs(got - g) / scale).max(0)
     print(model, "column-scaled errors", err, flush=True)
-    # measured: time 1.9e-16, displacement 3.3e-15, velocity 6.3e-15, net
-    # force 3.5e-13, group force 7.7e-16 (nonlinear1, three stages)
+    # all 20 load stages (2,000,000 steps), the bars of the oracle's own
+    # replay of these golds: the device evaluates the same FP64 operations
     assert err[0] < 1e-15, err
     assert err[1] < 1e-13, err
-    assert err[2] < 1e-12, err
-    assert err[3] < 1e-11, err
-    assert err[4] < 1e-13, err
+    assert err[2] < 1e-10, err
+    assert err[3] < 1e-9, err
+    assert err[4] < 1e-12, err

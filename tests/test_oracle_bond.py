@@ -14,7 +14,13 @@
   come in +-F pairs and the two bond torques cancel the moment of F, which
   fails for a wrong lever arm or sign;
 * bonds form only at tsCreateBond within createDistanceBond, break beyond
-  maxDistanceBond, and survive rebuilds (history carried by partner tag).
+  maxDistanceBond, and survive rebuilds (history carried by partner tag);
+* the partner's copy of a pair's history follows the newtonflags the
+  reference registers (normal_model_hysteretic_nonlinear1.h:84-97,
+  cohesion_model_bond_nonlinear.h:204-230, tangential_model_history.h:
+  79-81): on the 43-slot hysteretic + bond/nonlinear + shear chain exactly
+  kc, fo, ft, theta_n, theta_t and the shear change sign, and none of the
+  14 extreme-value slots does.
 """
 import os
 
@@ -316,3 +322,30 @@ def test_bond_creation_window_and_break():
     ti, tj, sh, hi, fl = o2.pairs()
     assert len(hi) == 0 or hi[:, 0].max() == 0.0
     assert np.abs(o2.state()["f"]).max() == 0.0
+
+
+def test_partner_copy_follows_newtonflags():
+    from conftest import load_engine
+    W = load_engine().workloads
+    spec, atoms = W.fibres(nchains=6, nsph=12)
+    atoms = dict(atoms, v=np.random.default_rng(3).normal(0.0, 0.2, atoms["x"].shape))
+    o = Oracle(spec, atoms)
+    o.setup()
+    o.run(200)
+    ti, tj, _, h, _ = o.pairs()
+    assert h.shape[1] == 43
+    # registration order: 12 hysteretic (newtonflag 1: kc, fo), 28
+    # bond/nonlinear (1: ft x3, theta_n x3, theta_t x3), 3 shear (1)
+    flip = np.zeros(43, bool)
+    flip[[10, 11]] = True
+    flip[12 + 5:12 + 14] = True
+    flip[40:43] = True
+    at = {(a, b): k for k, (a, b) in enumerate(zip(ti.tolist(), tj.tolist()))}
+    checked = nonzero = 0
+    for (a, b), k in at.items():
+        if a < b and (b, a) in at:
+            f, r = h[k], h[at[(b, a)]]
+            assert np.array_equal(r, np.where(flip, -f, f)), (a, b)
+            checked += 1
+            nonzero += int(np.count_nonzero(f[26:40]) > 0)
+    assert checked > 50 and nonzero > 10
