@@ -19,6 +19,7 @@
 #include "dem_oracle.h"
 
 #include <math.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -319,6 +320,10 @@ int or_set_atoms(or_sim *s, int n, const double *x, const double *v, const doubl
 
 static op_t *new_op(or_sim *s, int kind)
 {
+    if (s->nops >= (int)(sizeof(s->ops) / sizeof(s->ops[0]))) {
+        fprintf(stderr, "dem_oracle: more than %d fixes\n", (int)(sizeof(s->ops) / sizeof(s->ops[0])));
+        abort();
+    }
     op_t *o = &s->ops[s->nops++];
     memset(o, 0, sizeof(*o));
     o->id = s->next_op_id++;
